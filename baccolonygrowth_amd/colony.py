@@ -1,0 +1,234 @@
+"""Synthetic colonies and field inputs for the mechanics path.
+
+* `planar_restart_fields` - height / normal / wall / growth-rate arrays for
+  a planar (x-z) state, built the way the reference derives them: the
+  column maximum of cell tops (Compute.cpp:256-340), the central-difference
+  surface normal (Compute.cpp:466-491) and a rough wall of uniform draws
+  (Simulation.cpp:161-168). They are inputs for parity runs, so only their
+  shape and plausibility matter.
+* `dome_colony` - the 3-D dome-shaped colony of the 2^20-cell kernel bench:
+  spherocylinders of radius 0.5 um and cylinder length U[2,4] um in a
+  spherical cap (radius 600 um, height 20 um at the centre), orientations
+  near the plane with tilt ~ N(0, 15 deg), small overlaps.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .cells import CELL_DTYPE, Params
+
+
+def grid_shape(p: Params):
+    return p.BoxX, p.BoxY, p.BoxZ
+
+
+def height_normal_from_cells(p: Params, cells: np.ndarray):
+    """Column height (max vertex z, floor cellRadius) and surface normal."""
+    ref = p.refinementGridHeight
+    nx, ny = p.BoxX * ref, p.BoxY * ref
+    h = np.full((nx, ny), p.cellRadius)
+    if len(cells):
+        hx = p.BoxLength / ref
+        cm = 0.5 * (cells["p"] + cells["q"])
+        top = np.maximum(cells["p"][:, 2], cells["q"][:, 2])
+        ix = np.clip(((cm[:, 0] + p.BoxX // 2 * p.BoxLength) / hx).astype(np.int64), 0, nx - 1)
+        if ny > 1:
+            iy = np.clip(((cm[:, 1] + p.BoxY // 2 * p.BoxLength) / hx).astype(np.int64), 0, ny - 1)
+        else:
+            iy = np.zeros_like(ix)
+        np.maximum.at(h, (ix, iy), top)
+    hx = p.BoxLength / ref
+    dxh = np.zeros_like(h)
+    dxh[1:-1, :] = (h[2:, :] - h[:-2, :]) / (2.0 * hx)
+    dyh = np.zeros_like(h)
+    if ny > 2:
+        dyh[:, 1:-1] = (h[:, 2:] - h[:, :-2]) / (2.0 * hx)
+    ln = np.sqrt(dxh * dxh + dyh * dyh + 1.0)
+    normal = np.stack([dxh / ln, dyh / ln, -1.0 / ln], axis=-1)
+    return h, normal
+
+
+def synthetic_fields(p: Params, cells: np.ndarray, seed: int = 0, gr_max: float = 0.9):
+    """(height, normal, wall, gr1, gr2) arrays in the reference layouts."""
+    rng = np.random.default_rng(seed)
+    height, normal = height_normal_from_cells(p, cells)
+    wall = (rng.random((p.BoxX, p.BoxY)) - 0.5) * p.wall_rough
+    x = np.arange(p.BoxX)[:, None, None]
+    z = np.arange(p.BoxZ)[None, None, :]
+    prof = 0.5 + 0.5 * np.cos(2 * np.pi * x / max(p.BoxX, 1)) * np.exp(-z / max(p.BoxZ, 1))
+    gr1 = np.broadcast_to(gr_max * prof, (p.BoxX, p.BoxY, p.BoxZ)).copy()
+    gr2 = 0.8 * gr1
+    return height, normal, wall, gr1, gr2
+
+
+def _rot_axis(theta, tilt):
+    """Unit axis from in-plane azimuth theta and elevation tilt."""
+    ct = np.cos(tilt)
+    return np.stack([ct * np.cos(theta), ct * np.sin(theta), np.sin(tilt)], axis=-1)
+
+
+def _seg_dist(p1, q1, p2, q2):
+    """Vectorised segment-segment distance (same clamping rules as the
+    reference's min_distance)."""
+    v1 = q1 - p1
+    v2 = q2 - p2
+    r = p1 - p2
+    l1 = np.einsum("ij,ij->i", v1, v1)
+    l2 = np.einsum("ij,ij->i", v2, v2)
+    f = np.einsum("ij,ij->i", v2, r)
+    c = np.einsum("ij,ij->i", v1, r)
+    b = np.einsum("ij,ij->i", v1, v2)
+    den = l1 * l2 - b * b
+    with np.errstate(divide="ignore", invalid="ignore"):
+        s = np.where(den != 0, np.clip((b * f - c * l2) / den, 0, 1), 0.0)
+        tn = b * s + f
+        t = np.where(tn < 0, 0.0, np.where(tn > l2, 1.0, tn / l2))
+        s = np.where(tn < 0, np.clip(-c / l1, 0, 1), np.where(tn > l2, np.clip((b - c) / l1, 0, 1), s))
+    c1 = p1 + v1 * s[:, None]
+    c2 = p2 + v2 * t[:, None]
+    return np.sqrt(np.einsum("ij,ij->i", c1 - c2, c1 - c2))
+
+
+def dome_colony(
+    n: int = 1 << 20,
+    radius: float = 600.0,
+    height: float = 20.0,
+    seed: int = 2,
+    cell_radius: float = 0.5,
+    max_overlap: float = 0.05,
+    tilt_deg: float = 15.0,
+):
+    """Synthetic dome: returns a CELL_DTYPE array of exactly n cells.
+
+    Centres sit on a jittered lattice inside the cap (volume per cell about
+    10.9 um^3); cells in one 20 um patch share an in-plane direction, so
+    rods pack along their axis at a 5 um pitch, and tilts ~N(0, tilt_deg).
+    After placement every pair closer than 2R - max_overlap is resolved by
+    shortening the higher-ID cell (centre kept), repeated until no pair
+    overlaps by more than max_overlap.
+    """
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(seed)
+    R = cell_radius
+    # spherical cap: z_top(r) = sqrt(Rs^2 - r^2) - (Rs - h), Rs = (a^2 + h^2) / 2h
+    Rs = (radius * radius + height * height) / (2 * height)
+    a_par, a_perp, a_z = 5.0, 1.5, 1.45
+    pts = []
+    patch = 20.0
+    npx = int(math.ceil(2 * radius / patch))
+    for ix in range(npx):
+        for iy in range(npx):
+            x0 = -radius + ix * patch
+            y0 = -radius + iy * patch
+            xc, yc = x0 + patch / 2, y0 + patch / 2
+            if math.hypot(xc, yc) > radius + patch:
+                continue
+            th = rng.uniform(0, 2 * np.pi)
+            u = np.array([np.cos(th), np.sin(th)])
+            w = np.array([-u[1], u[0]])
+            m = int(patch * 1.5 / a_par) + 1
+            k = int(patch * 1.5 / a_perp) + 1
+            ii, jj = np.meshgrid(np.arange(-m, m + 1), np.arange(-k, k + 1), indexing="ij")
+            off = ii[..., None] * a_par * u + jj[..., None] * a_perp * w
+            off = off.reshape(-1, 2) + np.array([xc, yc])
+            keep = (off[:, 0] >= x0) & (off[:, 0] < x0 + patch) & (off[:, 1] >= y0) & (off[:, 1] < y0 + patch)
+            off = off[keep]
+            rr = np.hypot(off[:, 0], off[:, 1])
+            off = off[rr < radius]
+            if not len(off):
+                continue
+            rr = np.hypot(off[:, 0], off[:, 1])
+            ztop = np.sqrt(np.maximum(Rs * Rs - rr * rr, 0)) - (Rs - height)
+            nz = np.maximum(1, np.floor((ztop - 0.0) / a_z).astype(int))
+            for lz in range(int(nz.max())):
+                sel = nz > lz
+                z = 0.5 + lz * a_z
+                o = off[sel]
+                pts.append(np.column_stack([o, np.full(len(o), z), np.full(len(o), th)]))
+    pts = np.concatenate(pts)
+    if len(pts) < n:
+        raise ValueError(f"lattice holds {len(pts)} < {n} cells; enlarge the dome")
+    # choose n sites: keep the inner ones (by a random tie-break on radius)
+    rr = np.hypot(pts[:, 0], pts[:, 1]) + rng.random(len(pts)) * 1e-3
+    pts = pts[np.argsort(rr, kind="stable")[:n]]
+    # order IDs by patch-major lattice order (spatially coherent like a grown colony)
+    key = np.floor((pts[:, 0] + radius) / patch) * 1e6 + np.floor((pts[:, 1] + radius) / patch) * 1e3 + pts[:, 2]
+    pts = pts[np.argsort(key, kind="stable")]
+    cm = pts[:, :3].copy()
+    cm[:, :2] += rng.uniform(-0.15, 0.15, (n, 2))
+    cm[:, 2] += rng.uniform(0.0, 0.08, n)
+    theta = pts[:, 3] + rng.normal(0, 0.05, n)
+    tilt = np.deg2rad(rng.normal(0, tilt_deg, n))
+    # keep the lowest layer's vertices at or above the agar (z >= R - max_overlap)
+    L = rng.uniform(2.0, 4.0, n)
+    axis = _rot_axis(theta, tilt)
+    for _ in range(12):
+        half = 0.5 * L[:, None] * axis
+        p = cm + half
+        q = cm - half
+        # vertices below the agar: reduce the tilt
+        low = np.minimum(p[:, 2], q[:, 2]) < R - max_overlap
+        if low.any():
+            tilt[low] *= 0.5
+            axis = _rot_axis(theta, tilt)
+            continue
+        tree = cKDTree(cm)
+        pairs = tree.query_pairs(4.0 + 2 * R, output_type="ndarray")
+        if len(pairs) == 0:
+            break
+        d = _seg_dist(p[pairs[:, 0]], q[pairs[:, 0]], p[pairs[:, 1]], q[pairs[:, 1]])
+        bad = d < 2 * R - max_overlap
+        if not bad.any():
+            break
+        j = np.maximum(pairs[bad, 0], pairs[bad, 1])
+        excess = (2 * R - max_overlap) - d[bad] + 1e-3
+        shrink = np.zeros(n)
+        np.maximum.at(shrink, j, 2.0 * excess + 0.05)
+        L = np.maximum(L - shrink, 0.3)
+    half = 0.5 * L[:, None] * axis
+    cells = np.zeros(n, CELL_DTYPE)
+    cells["p"] = cm + half
+    cells["q"] = cm - half
+    cells["Length"] = L
+    cells["Radius"] = R
+    cells["Ldiv"] = 2.0 * np.maximum(L, 2.0)
+    cells["Type"] = 1
+    cells["Ancestor"] = np.arange(1, n + 1)
+    return cells
+
+
+def dome_params(cells: np.ndarray, bin_size: float = 5.0, max_neighbours: int = 64, extra_cells: int = 0) -> Params:
+    """Bin grid covering a dome colony, in the reference's bin convention
+    (x index = int(x/BL + BoxX/2), y index = int(y/BL + BoxY/2 + 0.5))."""
+    from .cells import reference_defaults
+
+    p = reference_defaults()
+    p.cellRadius = 0.5
+    p.L_divide = 3.0
+    p.k_cc = 200000.0
+    p.k_wc = 200000.0
+    p.varL = 0.25
+    p.varAngle = 0.001
+    p.viscosity = 0.1
+    p.wall_rough = 0.1
+    p.gamma_t = 5000.0
+    p.wall_mu = 0.4
+    p.cell_mu = 0.1
+    p.density_threshold = 0.55
+    p.tension = 500.0
+    p.DH = -0.01
+    p.crossFeeding = 0
+    p.planar = 0
+    p.refinementGridHeight = 1
+    p.BoxLength = bin_size
+    ext = np.abs(np.concatenate([cells["p"][:, :2], cells["q"][:, :2]])).max() + 4 * bin_size
+    nb = 2 * int(math.ceil(ext / bin_size)) + 2
+    p.BoxX = p.BoxY = nb
+    top = max(cells["p"][:, 2].max(), cells["q"][:, 2].max())
+    p.BoxZ = int(math.ceil((top + 4 * bin_size) / bin_size))
+    p.maxCells = len(cells) + extra_cells
+    p.maxNeighbours = max_neighbours
+    return p

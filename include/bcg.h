@@ -1,0 +1,166 @@
+/*
+ * bcg.h - C ABI of the MI355X (gfx950) implementation of the BacColonyGrowth
+ * per-timestep hot path: spherocylinder mechanics on a neighbour grid.
+ *
+ * Every entry point returns an int status (BCG_OK == 0); no C++ exception
+ * crosses this boundary. Host buffers belong to the caller, device buffers to
+ * the context. One context per host thread; all work is ordered on the
+ * context's HIP stream and every call returns after that work has finished.
+ *
+ * The reference has no plugin or FFI layer: its seam is a set of free C++
+ * functions that take extern globals as parameters. Each entry point below
+ * names the reference function (file:line, relative to
+ * AgentBasedModel-BacColonyGrowth/) whose behaviour it replaces.
+ */
+#ifndef BCG_H_
+#define BCG_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define BCG_OK                      0
+#define BCG_ERR_ARG                -1  /* bad argument or size                          */
+#define BCG_ERR_CAPACITY           -2  /* more cells than bcg_params.maxCells            */
+#define BCG_ERR_OUT_OF_DOMAIN      -3  /* a centre of mass lies outside the bin grid     */
+#define BCG_ERR_NEIGHBOUR_OVERFLOW -4  /* a reduced neighbour list exceeds maxNeighbours */
+#define BCG_ERR_HIP                -5  /* HIP runtime error (text in bcg_last_error)     */
+#define BCG_ERR_UNSUPPORTED        -6  /* parameter combination not implemented          */
+#define BCG_ERR_STATE              -7  /* call out of order (e.g. no cells uploaded)     */
+
+/* ---- cell record -------------------------------------------------------
+ * Byte-identical to the reference's `struct Cell` (ABM/Cell.h:15-28 with
+ * `Segment` from ABM/tools.h:71-87): 208 bytes, little-endian, no padding.
+ * This is the record of the reference's Restart files
+ * (ABM/InputOutput.cpp:434-448): 16-byte header (t, dt) then Cell[N].        */
+typedef struct bcg_cell {
+    double p[3];              /* Position.p                        */
+    double q[3];              /* Position.q                        */
+    double time_p, time_q;    /* Position.time_p / time_q          */
+    int32_t age_p, age_q;     /* Position.age_p / age_q            */
+    double Length;
+    double Ldiv;
+    double Radius;
+    double GrowthRate;
+    double Velocity[3];
+    double AngularVelocity[3];
+    double DynFric[3];
+    double StaFric[3];
+    int32_t Type;
+    int32_t Ancestor;
+} bcg_cell;
+
+/* ---- parameters --------------------------------------------------------
+ * Names follow the reference's globals (ABM/Constants.cpp:1-110) and the
+ * input-file keys parsed at ABM/InputOutput.cpp:481-661.                    */
+typedef struct bcg_params {
+    /* physical constants (ABM/Constants.cpp:1-15, :55) */
+    double cellRadius;         /* "Radius"            */
+    double L_divide;           /* "L_divide"          */
+    double k_cc;               /* "k_cc"              */
+    double k_wc;               /* "k_wc"              */
+    double varL;               /* "var_L"             */
+    double varAngle;           /* "var_angle"         */
+    double var_pos;            /* "var_pos"  (only 0 is supported, see below) */
+    double viscosity;          /* "Viscosity"         */
+    double wall_rough;         /* "Wall_Rough"        */
+    double gamma_t;            /* "Gamma"             */
+    double wall_mu;            /* "Wall_Mu"           */
+    double cell_mu;            /* "Cell_Mu"           */
+    double density_threshold;  /* "Density_Threshold" */
+    double tension;            /* "Surface_Tension"   */
+    double DH;                 /* "Delta_H"           */
+    double Rc;                 /* "Rc" (default 3.0)  */
+    /* neighbour bins = UniformGrid (ABM/CellMD3D.cpp:52-55) */
+    double BoxLength;          /* bin edge; the reference uses L_divide + 2*Radius */
+    int32_t BoxX, BoxY, BoxZ;  /* "Box_x", "Box_y", "Box_z" */
+    int32_t refinementGridHeight;
+    int32_t maxCells;          /* "MaxCells": capacity of the cell buffers          */
+    int32_t maxNeighbours;     /* capacity of one reduced neighbour list            */
+    int32_t crossFeeding;      /* "CrossFeeding": Type 2 grows with GrowthRate2     */
+    int32_t planar;            /* 1: reference x-z model (y forced to 0),
+                                  0: full 3-D generalisation                        */
+} bcg_params;
+
+/* Opaque context: device-resident cell state, bins, lists, fields, RNG. */
+typedef struct bcg_ctx bcg_ctx;
+
+/* Fill `p` with the reference defaults (ABM/Constants.cpp:1-110) overlaid
+ * with the demo input (ABM/run_sim/inputFile_20mM_SmallSimulation.txt),
+ * i.e. what the reference's ReadParameters (ABM/InputOutput.cpp:450) yields
+ * for that file, and BoxLength = L_divide + 2*Radius (ABM/CellMD3D.cpp:52). */
+int bcg_default_params(bcg_params* p);
+
+/* Replaces the allocation block of main (ABM/CellMD3D.cpp:46-55,82-104,
+ * 138-142): cell double buffers, UniformGrid, neighbour lists, fields.     */
+int bcg_create(const bcg_params* p, bcg_ctx** out);
+int bcg_destroy(bcg_ctx* ctx);
+
+/* Text of the last error on this context (NUL-terminated, truncated). */
+int bcg_last_error(const bcg_ctx* ctx, char* buf, int len);
+
+/* Upload N cells in ID order. Replaces LoadCells (ABM/InputOutput.cpp:394-431)
+ * minus the file read: it also places every cell in its bin and clears the
+ * neighbour lists (call bcg_rebuild_neighbours before stepping).            */
+int bcg_set_cells(bcg_ctx* ctx, const bcg_cell* cells, int32_t n);
+/* Download the current cells (after the last step's buffer swap). */
+int bcg_get_cells(bcg_ctx* ctx, bcg_cell* cells, int32_t cap, int32_t* n);
+int bcg_num_cells(bcg_ctx* ctx, int32_t* n);
+
+/* Fields read by the mechanics, in the reference layouts (x-major):
+ *   height  [BoxX*ref][BoxY*ref]     DoubleArray2D Height (ABM/CellMD3D.cpp:82)
+ *   normal  [BoxX*ref][BoxY*ref][3]  CoordArray2D Normal  (ABM/CellMD3D.cpp:138)
+ *   wall    [BoxX][BoxY]             DoubleArray2D Wall   (ABM/Simulation.cpp:161)
+ *   gr1,gr2 [BoxX][BoxY][BoxZ]       Environment(x,y,z).GrowthRate1/2
+ *                                    (EnvArray3D, read by grow at ABM/grow.cpp:26-36)
+ * Any pointer may be NULL to leave that field unchanged.                    */
+int bcg_set_fields(bcg_ctx* ctx, const double* height, const double* normal,
+                   const double* wall, const double* gr1, const double* gr2);
+
+/* glibc-compatible rand() stream owned by the context (the reference draws
+ * from the process-wide rand(): ABM/CellMD3D.cpp:23, ABM/Forces.cpp:102,
+ * ABM/grow.cpp:109-152, ABM/Simulation.cpp:166). */
+int bcg_srand(bcg_ctx* ctx, uint32_t seed);
+int bcg_rand(bcg_ctx* ctx, int32_t* out);   /* one draw, as rand() */
+
+/* Neighbour lists. Replaces getNeighbours (ABM/Neighbours.cpp:24-55) with the
+ * UniformGrid 27-box gather (ABM/UniformGrid.h:141-170) and the
+ * reduceNeighbours filter (ABM/Neighbours.cpp:57-93). ids == NULL (or n < 0)
+ * rebuilds every cell; otherwise only the listed cells, sequentially as the
+ * reference does.                                                          */
+int bcg_rebuild_neighbours(bcg_ctx* ctx, const int32_t* ids, int32_t n);
+/* counts[N]; lists[N*stride], each list sorted ascending by neighbour ID.   */
+int bcg_get_neighbours(bcg_ctx* ctx, int32_t* counts, int32_t* lists, int32_t stride);
+
+/* sum_forces (ABM/Forces.cpp:332-381) on the current cells against their
+ * current lists: F[3N], T[3N]. Diagnostic: consumes no rand() draws.       */
+int bcg_sum_forces(bcg_ctx* ctx, double* F, double* T);
+
+/* One mechanics step = the body of the reference time loop after the field
+ * update (ABM/Simulation.cpp:330-413):
+ *   MoveCell + GrowCell for every cell (ABM/Cells.cpp:14-57), the division
+ *   loop with DivideCell and the neighbour-list patch (:380-404), then the
+ *   old/new buffer swap (:409-413).
+ * rebuild_lists != 0 first rebuilds every neighbour list from the current
+ * cells, as the reference does at each field-update step (:292).
+ * lineage (may be NULL) receives (parent, daughter) pairs, lineage_cap pairs
+ * at most; *n_div receives the number of divisions performed.              */
+int bcg_step(bcg_ctx* ctx, double dt, double t, int32_t rebuild_lists,
+             int32_t* n_div, int32_t* lineage, int32_t lineage_cap);
+
+/* Counters of the last bcg_step: wall-contact draws (calls of F_cw), cells
+ * that reached Ldiv, and maximum reduced list length seen.                 */
+int bcg_last_step_stats(bcg_ctx* ctx, int64_t* wall_draws, int32_t* n_flagged,
+                        int32_t* max_list);
+
+/* Per-phase device time of the last bcg_step in milliseconds, from HIP
+ * events: [0] bins, [1] lists, [2] move+grow, [3] divisions.               */
+int bcg_last_step_times(bcg_ctx* ctx, float* ms4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BCG_H_ */

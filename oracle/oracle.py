@@ -1,0 +1,252 @@
+"""ctypes loaders for the test oracles (TEST INFRASTRUCTURE ONLY).
+
+`Oracle` wraps oracle/liboracle.so, the C restatement of the reference
+mechanics (bcg_oracle.c). `RefLib` wraps oracle/_ref/libbcgref.so, the
+reference's own sources compiled with a small harness (ref_shim.cpp); it is
+only present where the reference checkout was available at build time.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use
+this module, and only as the checker or the timed CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from baccolonygrowth_amd.cells import CELL_DTYPE, Params
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libbcgref.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_vp = ctypes.c_void_p
+
+
+def build(ref: bool = True) -> None:
+    """Compile the oracle (and the reference harness when its sources exist)."""
+    targets = ["oracle"] + (["ref"] if ref else [])
+    subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
+
+
+def _cells_ptr(cells: np.ndarray):
+    assert cells.dtype == CELL_DTYPE and cells.flags.c_contiguous
+    return cells.ctypes.data_as(_vp)
+
+
+def _dptr(a):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a.ctypes.data_as(_dp), a
+
+
+class _Base:
+    prefix = ""
+
+    def _f(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    def min_distance(self, a, b):
+        out = np.zeros(7)
+        self._f("min_distance")(*self._pre(), _cells_ptr(a), _cells_ptr(b), out.ctypes.data_as(_dp))
+        return out
+
+    def F_cc(self, p: Params, a, b):
+        out = np.zeros(7)
+        self._f("F_cc")(ctypes.byref(p), _cells_ptr(a), _cells_ptr(b), out.ctypes.data_as(_dp))
+        return out
+
+    def F_cw(self, p: Params, c, wall_z: float):
+        out = np.zeros(18)
+        self._f("F_cw")(ctypes.byref(p), _cells_ptr(c), ctypes.c_double(wall_z), out.ctypes.data_as(_dp))
+        return out
+
+
+class Sim:
+    """A stateful mechanics simulator from either library (same surface)."""
+
+    def __init__(self, owner, p: Params):
+        self.o = owner
+        self.p = p.copy()
+        self.h = owner._f("sim_create")(ctypes.byref(self.p))
+
+    def close(self):
+        if self.h:
+            self.o._f("sim_destroy")(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_cells(self, cells):
+        cells = np.ascontiguousarray(cells, dtype=CELL_DTYPE)
+        rc = self.o._f("sim_set_cells")(self.h, _cells_ptr(cells), len(cells))
+        if rc not in (None, 0):
+            raise RuntimeError(f"set_cells rc={rc}")
+
+    def set_fields(self, height=None, normal=None, wall=None, gr1=None, gr2=None):
+        keep = []
+        args = []
+        for a in (height, normal, wall, gr1, gr2):
+            if a is None:
+                args.append(None)
+            else:
+                ptr, arr = _dptr(a)
+                keep.append(arr)
+                args.append(ptr)
+        self.o._f("sim_set_fields")(self.h, *args)
+
+    @property
+    def n(self) -> int:
+        return self.o._f("sim_num_cells")(self.h)
+
+    def cells(self):
+        out = np.zeros(self.n, CELL_DTYPE)
+        self.o._f("sim_get_cells")(self.h, _cells_ptr(out))
+        return out
+
+    def rebuild(self, ids=None):
+        if ids is None:
+            rc = self.o._f("sim_rebuild")(self.h)
+        else:
+            ids = np.ascontiguousarray(ids, np.int32)
+            rc = self.o._f("sim_rebuild_ids")(self.h, ids.ctypes.data_as(_ip), len(ids))
+        if rc not in (None, 0):
+            raise RuntimeError(f"rebuild rc={rc}")
+
+    def neighbours(self, stride=256):
+        n = self.n
+        counts = np.zeros(n, np.int32)
+        lists = np.full((n, stride), -1, np.int32)
+        self.o._f("sim_get_neighbours")(self.h, counts.ctypes.data_as(_ip), lists.ctypes.data_as(_ip), stride)
+        return counts, lists
+
+    def boxes(self, stride=256):
+        p = self.p
+        nbox = p.BoxX * p.BoxY * p.BoxZ
+        counts = np.zeros(nbox, np.int32)
+        ids = np.full((nbox, stride), -1, np.int32)
+        self.o._f("sim_get_boxes")(self.h, counts.ctypes.data_as(_ip), ids.ctypes.data_as(_ip), stride)
+        return counts, ids
+
+    def sum_forces(self):
+        n = self.n
+        F = np.zeros((n, 3))
+        T = np.zeros((n, 3))
+        self.o._f("sim_sum_forces")(self.h, F.ctypes.data_as(_dp), T.ctypes.data_as(_dp))
+        return F, T
+
+    def step(self, dt: float, t: float, rebuild: bool, lineage_cap: int = 100000):
+        lin = np.zeros((lineage_cap, 2), np.int32)
+        nd = self.o._f("sim_step")(
+            self.h, ctypes.c_double(dt), ctypes.c_double(t), int(bool(rebuild)), lin.ctypes.data_as(_ip), lineage_cap
+        )
+        if nd < 0:
+            raise RuntimeError(f"step rc={nd}")
+        return lin[: min(nd, lineage_cap)].copy()
+
+
+def _declare(lib, prefix):
+    f = lambda n: getattr(lib, prefix + n)  # noqa: E731
+    f("sim_create").restype = _vp
+    f("sim_create").argtypes = [ctypes.POINTER(Params)]
+    f("sim_destroy").argtypes = [_vp]
+    f("sim_set_cells").argtypes = [_vp, _vp, ctypes.c_int]
+    f("sim_set_fields").argtypes = [_vp, _dp, _dp, _dp, _dp, _dp]
+    f("sim_num_cells").argtypes = [_vp]
+    f("sim_num_cells").restype = ctypes.c_int
+    f("sim_get_cells").argtypes = [_vp, _vp]
+    f("sim_rebuild").argtypes = [_vp]
+    f("sim_rebuild_ids").argtypes = [_vp, _ip, ctypes.c_int]
+    f("sim_get_neighbours").argtypes = [_vp, _ip, _ip, ctypes.c_int]
+    f("sim_get_boxes").argtypes = [_vp, _ip, _ip, ctypes.c_int]
+    f("sim_sum_forces").argtypes = [_vp, _dp, _dp]
+    f("sim_step").argtypes = [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _ip, ctypes.c_int]
+    f("sim_step").restype = ctypes.c_int
+    f("F_cc").argtypes = [ctypes.POINTER(Params), _vp, _vp, _dp]
+    f("F_cw").argtypes = [ctypes.POINTER(Params), _vp, ctypes.c_double, _dp]
+
+
+class Oracle(_Base):
+    """The C restatement (liboracle.so)."""
+
+    prefix = "or_"
+
+    def __init__(self):
+        if not os.path.exists(ORACLE_SO):
+            build(ref=False)
+        self.lib = ctypes.CDLL(ORACLE_SO)
+        _declare(self.lib, self.prefix)
+        self.lib.or_sim_set_cells.restype = ctypes.c_int
+        self.lib.or_sim_rebuild.restype = ctypes.c_int
+        self.lib.or_sim_rebuild_ids.restype = ctypes.c_int
+        self.lib.or_sim_wall_draws.argtypes = [_vp]
+        self.lib.or_sim_wall_draws.restype = ctypes.c_int64
+        self.lib.or_min_distance.argtypes = [_vp, _vp, _dp]
+        self.lib.srand.argtypes = [ctypes.c_uint]
+        self.lib.rand.restype = ctypes.c_int
+
+    def _pre(self):
+        return ()
+
+    def sim(self, p: Params) -> Sim:
+        return Sim(self, p)
+
+    def srand(self, seed: int):
+        """Seed the process C library rand() (the oracle draws from it)."""
+        self.lib.srand(seed)
+
+    def rand(self) -> int:
+        return self.lib.rand()
+
+    def glibc_stream(self, seed: int, n: int) -> np.ndarray:
+        class G(ctypes.Structure):
+            _fields_ = [("r", ctypes.c_int32 * 34), ("i", ctypes.c_int32)]
+
+        g = G()
+        self.lib.or_glibc_srand(ctypes.byref(g), ctypes.c_uint32(seed))
+        self.lib.or_glibc_rand.restype = ctypes.c_int32
+        return np.array([self.lib.or_glibc_rand(ctypes.byref(g)) for _ in range(n)], np.int64)
+
+
+class RefLib(_Base):
+    """The reference's own code (oracle/_ref/libbcgref.so)."""
+
+    prefix = "ref_"
+
+    def __init__(self):
+        if not os.path.exists(REF_SO):
+            raise FileNotFoundError(REF_SO)
+        self.lib = ctypes.CDLL(REF_SO)
+        _declare(self.lib, self.prefix)
+        self.lib.ref_min_distance.argtypes = [ctypes.POINTER(Params), _vp, _vp, _dp]
+        self.lib.ref_set_params.argtypes = [ctypes.POINTER(Params)]
+        self.lib.ref_srand.argtypes = [ctypes.c_uint]
+        for name in ("sim_set_cells", "sim_rebuild", "sim_rebuild_ids", "sim_set_fields"):
+            getattr(self.lib, "ref_" + name).restype = None
+        self.lib.ref_rand.restype = ctypes.c_int
+        self._p = Params()
+
+    def _pre(self):
+        return (ctypes.byref(self._p),)
+
+    def sim(self, p: Params) -> Sim:
+        return Sim(self, p)
+
+    def srand(self, seed: int):
+        self.lib.ref_srand(seed)
+
+    def rand(self) -> int:
+        return self.lib.ref_rand()
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
