@@ -91,33 +91,9 @@ def _seg_dist(p1, q1, p2, q2):
     return np.sqrt(np.einsum("ij,ij->i", c1 - c2, c1 - c2))
 
 
-def dome_colony(
-    n: int = 1 << 20,
-    radius: float = 600.0,
-    height: float = 20.0,
-    seed: int = 2,
-    cell_radius: float = 0.5,
-    max_overlap: float = 0.05,
-    tilt_deg: float = 15.0,
-):
-    """Synthetic dome: returns a CELL_DTYPE array of exactly n cells.
-
-    Centres sit on a jittered lattice inside the cap (volume per cell about
-    10.9 um^3); cells in one 20 um patch share an in-plane direction, so
-    rods pack along their axis at a 5 um pitch, and tilts ~N(0, tilt_deg).
-    After placement every pair closer than 2R - max_overlap is resolved by
-    shortening the higher-ID cell (centre kept), repeated until no pair
-    overlaps by more than max_overlap.
-    """
-    from scipy.spatial import cKDTree
-
-    rng = np.random.default_rng(seed)
-    R = cell_radius
-    # spherical cap: z_top(r) = sqrt(Rs^2 - r^2) - (Rs - h), Rs = (a^2 + h^2) / 2h
-    Rs = (radius * radius + height * height) / (2 * height)
-    a_par, a_perp, a_z = 5.0, 1.5, 1.45
+def _lattice_sites(rng, radius, height, Rs, a_par, a_perp, a_z, patch=20.0):
+    """Sites (x, y, z, azimuth) of patch-aligned rod lattices inside the cap."""
     pts = []
-    patch = 20.0
     npx = int(math.ceil(2 * radius / patch))
     for ix in range(npx):
         for iy in range(npx):
@@ -142,51 +118,97 @@ def dome_colony(
                 continue
             rr = np.hypot(off[:, 0], off[:, 1])
             ztop = np.sqrt(np.maximum(Rs * Rs - rr * rr, 0)) - (Rs - height)
-            nz = np.maximum(1, np.floor((ztop - 0.0) / a_z).astype(int))
+            nz = np.maximum(1, np.floor(ztop / a_z).astype(int))
             for lz in range(int(nz.max())):
                 sel = nz > lz
-                z = 0.5 + lz * a_z
                 o = off[sel]
-                pts.append(np.column_stack([o, np.full(len(o), z), np.full(len(o), th)]))
-    pts = np.concatenate(pts)
-    if len(pts) < n:
-        raise ValueError(f"lattice holds {len(pts)} < {n} cells; enlarge the dome")
+                pts.append(np.column_stack([o, np.full(len(o), 0.5 + lz * a_z), np.full(len(o), th)]))
+    return np.concatenate(pts)
+
+
+def dome_colony(
+    n: int = 1 << 20,
+    radius: float = 600.0,
+    height: float = 20.0,
+    seed: int = 2,
+    cell_radius: float = 0.5,
+    max_overlap: float = 0.05,
+    tilt_deg: float = 15.0,
+):
+    """Synthetic dome: returns a CELL_DTYPE array of exactly n cells.
+
+    Centres sit on a jittered lattice inside the cap (pitch 5 um along the
+    rod axis, 1.45 um across, 1.5 um vertically, about 10.9 um^3 per cell,
+    shrunk by a few percent if the cap would not hold n cells); cells in one
+    20 um patch share an in-plane direction and tilt ~N(0, tilt_deg). Every
+    pair overlapping by more than max_overlap is then resolved by shortening
+    the higher-ID cell (centre kept), repeated until none is left.
+    """
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(seed)
+    R = cell_radius
+    # spherical cap: z_top(r) = sqrt(Rs^2 - r^2) - (Rs - h), Rs = (a^2 + h^2) / 2h
+    Rs = (radius * radius + height * height) / (2 * height)
+    scale = 1.0
+    while True:
+        pts = _lattice_sites(rng, radius, height, Rs, 5.0 * scale, 1.45 * scale, 1.5 * scale)
+        # sites of neighbouring patches (different azimuth) closer than 1.4 um
+        # cannot be separated by shortening: drop one of each such pair
+        close = cKDTree(pts[:, :3]).query_pairs(1.4, output_type="ndarray")
+        cross = close[pts[close[:, 0], 3] != pts[close[:, 1], 3]]
+        keep = np.ones(len(pts), bool)
+        keep[np.maximum(cross[:, 0], cross[:, 1])] = False
+        pts = pts[keep]
+        if len(pts) >= n:
+            break
+        scale *= 0.98
+        if scale < 0.9:
+            raise ValueError(f"dome of radius {radius} cannot hold {n} cells")
     # choose n sites: keep the inner ones (by a random tie-break on radius)
     rr = np.hypot(pts[:, 0], pts[:, 1]) + rng.random(len(pts)) * 1e-3
     pts = pts[np.argsort(rr, kind="stable")[:n]]
     # order IDs by patch-major lattice order (spatially coherent like a grown colony)
+    patch = 20.0
     key = np.floor((pts[:, 0] + radius) / patch) * 1e6 + np.floor((pts[:, 1] + radius) / patch) * 1e3 + pts[:, 2]
     pts = pts[np.argsort(key, kind="stable")]
     cm = pts[:, :3].copy()
-    cm[:, :2] += rng.uniform(-0.15, 0.15, (n, 2))
-    cm[:, 2] += rng.uniform(0.0, 0.08, n)
+    cm[:, :2] += rng.uniform(-0.1, 0.1, (n, 2))
+    cm[:, 2] += rng.uniform(0.0, 0.05, n)
     theta = pts[:, 3] + rng.normal(0, 0.05, n)
     tilt = np.deg2rad(rng.normal(0, tilt_deg, n))
     # keep the lowest layer's vertices at or above the agar (z >= R - max_overlap)
     L = rng.uniform(2.0, 4.0, n)
-    axis = _rot_axis(theta, tilt)
-    for _ in range(12):
-        half = 0.5 * L[:, None] * axis
-        p = cm + half
-        q = cm - half
-        # vertices below the agar: reduce the tilt
-        low = np.minimum(p[:, 2], q[:, 2]) < R - max_overlap
-        if low.any():
-            tilt[low] *= 0.5
-            axis = _rot_axis(theta, tilt)
-            continue
-        tree = cKDTree(cm)
-        pairs = tree.query_pairs(4.0 + 2 * R, output_type="ndarray")
-        if len(pairs) == 0:
+    # vertices below the agar (z < R - max_overlap): halve the tilt until none
+    for _ in range(30):
+        axis = _rot_axis(theta, tilt)
+        zlo = cm[:, 2] - 0.5 * L * np.abs(axis[:, 2])
+        low = zlo < R - max_overlap
+        if not low.any():
             break
-        d = _seg_dist(p[pairs[:, 0]], q[pairs[:, 0]], p[pairs[:, 1]], q[pairs[:, 1]])
-        bad = d < 2 * R - max_overlap
-        if not bad.any():
+        tilt[low] *= 0.5
+    # Shortening a rod (centre kept) never brings it closer to another rod,
+    # so only pairs that overlap after placement can ever need work.
+    thr = 2 * R - max_overlap
+    pairs = cKDTree(cm).query_pairs(L.max() + 2 * R, output_type="ndarray")
+    for _ in range(60):
+        if not len(pairs):
             break
-        j = np.maximum(pairs[bad, 0], pairs[bad, 1])
-        excess = (2 * R - max_overlap) - d[bad] + 1e-3
+        a, b = pairs[:, 0], pairs[:, 1]
+        ha = 0.5 * L[a, None] * axis[a]
+        hb = 0.5 * L[b, None] * axis[b]
+        d = _seg_dist(cm[a] + ha, cm[a] - ha, cm[b] + hb, cm[b] - hb)
+        bad = d < thr
+        pairs, d = pairs[bad], d[bad]
+        if not len(pairs):
+            break
+        hi = np.maximum(pairs[:, 0], pairs[:, 1])
+        lo = np.minimum(pairs[:, 0], pairs[:, 1])
+        excess = thr - d + 1e-3
         shrink = np.zeros(n)
-        np.maximum.at(shrink, j, 2.0 * excess + 0.05)
+        np.maximum.at(shrink, hi, 2.0 * excess + 0.05)
+        stuck = L[hi] <= 0.3  # the higher ID is already minimal: shorten the other
+        np.maximum.at(shrink, lo[stuck], 2.0 * excess[stuck] + 0.05)
         L = np.maximum(L - shrink, 0.3)
     half = 0.5 * L[:, None] * axis
     cells = np.zeros(n, CELL_DTYPE)

@@ -1,0 +1,60 @@
+// mech_kernels.h - kernel declarations shared by mech_kernels.hip and the
+// host side (bcg_api.hip).
+#pragma once
+
+#include "mech_device.h"
+
+namespace bcg {
+
+constexpr int ERRBIT_DOMAIN = 1;  // a centre of mass outside the bin grid
+constexpr int ERRBIT_NLIST = 2;   // a reduced list longer than its capacity
+
+constexpr int MOVE_BLOCK = 128;
+constexpr int PATCH_BLOCK = 256;
+constexpr int SCAN_BLOCK_ITEMS = 2048;  // items per block of the scan kernels
+
+__global__ void k_scan_reduce(const int* in, int n, int* tot);
+__global__ void k_scan_totals(int* tot, int nb, int* grand);
+__global__ void k_scan_apply(const int* in, int n, const int* tot, int* out);
+
+__global__ void k_bin_count(const double* hot, int n, PhysParams P, int* key, int* cnt, int* err);
+__global__ void k_bin_scatter(const int* key, int n, const int* start, int* fill, int* sorted);
+__global__ void k_bin_sort(const int* start, int nbin, int* sorted);
+
+__global__ void k_gather_all(const double* hot, int n, PhysParams P, const int* start, const int* sorted, int* nl,
+                             int* nl_count, int stride, int cap, int* order, int* slot_of, int* maxlist, int* err);
+__global__ void k_gather_ids(const double* hot, const int* ids, int nid, PhysParams P, const int* start,
+                             const int* sorted, int* nl, int* nl_count, int stride, int cap, const int* slot_of,
+                             int* maxlist, int* err);
+
+__global__ void k_move_grow(const double* hot_old, double* hot_new, int n, PhysParams P, double dt, const int* order,
+                            const int* nl, const int* nl_count, int stride, const double* H, const double* Nrm,
+                            const double* wall, const double* gr1, const double* gr2, const int* type,
+                            double* dynfric, double* stafric, int* divflag, unsigned long long* wall_draws, int* err);
+__global__ void k_sum_forces(const double* hot, int n, PhysParams P, const int* order, const int* nl,
+                             const int* nl_count, int stride, const double* H, const double* Nrm, const double* wall,
+                             double* Fo, double* To, int* err);
+
+__global__ void k_flag_count(const int* flag, int n, int* tot);
+__global__ void k_flag_write(const int* flag, int n, const int* off, int* ids);
+
+__global__ void k_divide(double* hot, int n0, int D, const int* div_ids, const double* u, PhysParams P, double t,
+                         double* mother_pre, int* div_index_of, double* time_p, double* time_q, int* age_p,
+                         int* age_q, int* type, int* ancestor, double* dynfric, double* stafric, int* order,
+                         int* slot_of, int* nl_count);
+
+// Position of cell j as the reference's grid and cell array hold it just
+// after division k: a mother of a later division is still undivided, a
+// daughter of a later division does not exist yet.
+struct Versioned {
+    const double* hot;         // final states (all divisions applied)
+    const double* mother_pre;  // pre-division mother records, by division
+    const int* div_index_of;   // division index of a mother, -1 otherwise
+    int n0;                    // cells before the divisions
+};
+
+__global__ void k_patch(PhysParams P, Versioned V, int D, const int* div_ids, const int* start, const int* sorted,
+                        int* nl, int* nl_count, int stride, int cap, const int* slot_of, int* maxlist, int* err);
+__global__ void k_reset_div_index(int* div_index_of, const int* div_ids, int D);
+
+}  // namespace bcg

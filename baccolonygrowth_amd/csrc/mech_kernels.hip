@@ -1,0 +1,596 @@
+// mech_kernels.hip - gfx950 kernels of the mechanics step.
+//
+//   bins      counting sort of cells by UniformGrid box (atomic counts,
+//             wave-level scan, scatter, per-bin sort to ascending ID)
+//   lists     27-box gather + reduceNeighbours filter, one thread per cell,
+//             ELL layout [k][slot] so a wave's list reads coalesce
+//   move_grow both sum_forces passes, the two half kicks and the drift,
+//             then grow and the division flag; one thread per cell, no
+//             force atomics, neighbour records read as 128-byte lines
+//   compact   ascending list of dividing cells by wave ballot + scan
+//   divide    one thread per division
+//   patch     the reference's post-division list patch, replayed in
+//             division order by one workgroup
+#include "mech_kernels.h"
+
+namespace bcg {
+
+// ------------------------------------------------------------- scans ---
+// Inclusive wave scan (64 lanes) of one int per lane.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+// Exclusive block scan of one int per thread (blockDim multiple of 64,
+// at most 1024). Returns the exclusive prefix; *total gets the block sum.
+__device__ __forceinline__ int block_excl_scan(int v, int* lds, int* total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int inc = wave_incl_scan(v);
+    if (lane == 63) lds[wid] = inc;
+    __syncthreads();
+    if (wid == 0) {
+        int w = lane < nw ? lds[lane] : 0;
+        int wi = wave_incl_scan(w);
+        if (lane < nw) lds[32 + lane] = wi - w;
+        if (lane == nw - 1) lds[31] = wi;
+    }
+    __syncthreads();
+    int r = inc - v + lds[32 + wid];
+    if (total) *total = lds[31];
+    __syncthreads();
+    return r;
+}
+
+constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_B = SCAN_T * SCAN_I;
+
+__global__ void __launch_bounds__(SCAN_T) k_scan_reduce(const int* __restrict__ in, int n, int* __restrict__ tot) {
+    __shared__ int lds[96];
+    const int base = blockIdx.x * SCAN_B + threadIdx.x * SCAN_I;
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_I; i++) s += (base + i < n) ? in[base + i] : 0;
+    int t;
+    block_excl_scan(s, lds, &t);
+    if (threadIdx.x == 0) tot[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(1024) k_scan_totals(int* __restrict__ tot, int nb, int* __restrict__ grand) {
+    __shared__ int lds[96];
+    const int per = (nb + 1023) / 1024;
+    const int base = threadIdx.x * per;
+    int s = 0;
+    for (int i = 0; i < per; i++) s += (base + i < nb) ? tot[base + i] : 0;
+    int t;
+    int ex = block_excl_scan(s, lds, &t);
+    for (int i = 0; i < per; i++)
+        if (base + i < nb) {
+            int v = tot[base + i];
+            tot[base + i] = ex;
+            ex += v;
+        }
+    if (threadIdx.x == 0 && grand) *grand = t;
+}
+
+__global__ void __launch_bounds__(SCAN_T) k_scan_apply(const int* __restrict__ in, int n, const int* __restrict__ tot,
+                                                       int* __restrict__ out) {
+    __shared__ int lds[96];
+    const int base = blockIdx.x * SCAN_B + threadIdx.x * SCAN_I;
+    int v[SCAN_I];
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_I; i++) {
+        v[i] = (base + i < n) ? in[base + i] : 0;
+        s += v[i];
+    }
+    int ex = block_excl_scan(s, lds, nullptr) + tot[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < SCAN_I; i++)
+        if (base + i < n) {
+            out[base + i] = ex;
+            ex += v[i];
+        }
+}
+
+// ------------------------------------------------------------- bins ---
+__global__ void k_bin_count(const double* __restrict__ hot, int n, PhysParams P, int* __restrict__ key,
+                            int* __restrict__ cnt, int* __restrict__ err) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Seg s;
+    load_seg(hot, i, s);
+    int x, y, z;
+    int a = bin_of(P, cm_of(s), x, y, z);
+    if (a < 0) {
+        // reported as an error; the cell is filed in the nearest bin so that
+        // every later kernel still sees a complete permutation
+        atomicOr(err, ERRBIT_DOMAIN);
+        x = x < 0 ? 0 : (x >= P.BoxX ? P.BoxX - 1 : x);
+        y = y < 0 ? 0 : (y >= P.BoxY ? P.BoxY - 1 : y);
+        z = z < 0 ? 0 : (z >= P.BoxZ ? P.BoxZ - 1 : z);
+        a = x * (P.BoxY * P.BoxZ) + y * P.BoxZ + z;
+    }
+    key[i] = a;
+    atomicAdd(&cnt[a], 1);
+}
+
+__global__ void k_bin_scatter(const int* __restrict__ key, int n, const int* __restrict__ start, int* __restrict__ fill,
+                              int* __restrict__ sorted) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int a = key[i];
+    int pos = start[a] + atomicAdd(&fill[a], 1);
+    sorted[pos] = i;
+}
+
+// Ascending IDs inside each bin (insertion sort; bins hold tens of cells).
+__global__ void k_bin_sort(const int* __restrict__ start, int nbin, int* __restrict__ sorted) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbin) return;
+    int s = start[b], e = start[b + 1];
+    for (int i = s + 1; i < e; i++) {
+        int v = sorted[i];
+        int j = i - 1;
+        while (j >= s && sorted[j] > v) {
+            sorted[j + 1] = sorted[j];
+            j--;
+        }
+        sorted[j + 1] = v;
+    }
+}
+
+// ------------------------------------------------------------ lists ---
+// UniformGrid::GetNeighbours (ABM/UniformGrid.h:141-170) + reduceNeighbours
+// (ABM/Neighbours.cpp:57-93) for the cell in `slot`, against the current bins.
+__device__ __forceinline__ int gather_one(const PhysParams& P, const double* __restrict__ hot, int id, int slot,
+                                          const int* __restrict__ start, const int* __restrict__ sorted,
+                                          int* __restrict__ nl, int stride, int cap, int* __restrict__ err) {
+    Seg A;
+    load_seg(hot, id, A);
+    d3 cmA = cm_of(A);
+    int x0, y0, z0;
+    if (bin_of(P, cmA, x0, y0, z0) < 0) {
+        atomicOr(err, ERRBIT_DOMAIN);
+        return 0;
+    }
+    int m = 0;
+    const int zlo = max(z0 - 1, 0), zhi = min(z0 + 2, P.BoxZ);
+    const int ylo = max(y0 - 1, 0), yhi = min(y0 + 2, P.BoxY);
+    const int xlo = max(x0 - 1, 0), xhi = min(x0 + 2, P.BoxX);
+    for (int za = zlo; za < zhi; za++)
+        for (int ya = ylo; ya < yhi; ya++)
+            for (int xa = xlo; xa < xhi; xa++) {
+                const int a = xa * (P.BoxY * P.BoxZ) + ya * P.BoxZ + za;
+                const int e = start[a + 1];
+                for (int k = start[a]; k < e; k++) {
+                    const int j = sorted[k];
+                    if (j == id) continue;
+                    Seg B;
+                    load_seg(hot, j, B);
+                    if (keep_neighbour(A, cmA, B, P.Rc)) {
+                        if (m < cap) nl[(size_t)m * stride + slot] = j;
+                        m++;
+                    }
+                }
+            }
+    if (m > cap) {
+        atomicOr(err, ERRBIT_NLIST);
+        m = cap;
+    }
+    return m;
+}
+
+__global__ void __launch_bounds__(256) k_gather_all(const double* __restrict__ hot, int n, PhysParams P,
+                                                    const int* __restrict__ start, const int* __restrict__ sorted,
+                                                    int* __restrict__ nl, int* __restrict__ nl_count, int stride,
+                                                    int cap, int* __restrict__ order, int* __restrict__ slot_of,
+                                                    int* __restrict__ maxlist, int* __restrict__ err) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int id = sorted[t];
+    order[t] = id;
+    slot_of[id] = t;
+    int m = gather_one(P, hot, id, t, start, sorted, nl, stride, cap, err);
+    nl_count[t] = m;
+    atomicMax(maxlist, m);
+}
+
+__global__ void __launch_bounds__(256) k_gather_ids(const double* __restrict__ hot, const int* __restrict__ ids, int nid,
+                                                    PhysParams P, const int* __restrict__ start,
+                                                    const int* __restrict__ sorted, int* __restrict__ nl,
+                                                    int* __restrict__ nl_count, int stride, int cap,
+                                                    const int* __restrict__ slot_of, int* __restrict__ maxlist,
+                                                    int* __restrict__ err) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nid) return;
+    const int id = ids[i];
+    const int t = slot_of[id];
+    int m = gather_one(P, hot, id, t, start, sorted, nl, stride, cap, err);
+    nl_count[t] = m;
+    atomicMax(maxlist, m);
+}
+
+// -------------------------------------------------------- move + grow ---
+// sum_forces (ABM/Forces.cpp:332-381) of cell S against its list.
+__device__ __forceinline__ void sum_forces(const PhysParams& P, const Seg& S, const Kin& K, const double* __restrict__ hot,
+                                           const int* __restrict__ nl, int stride, int slot, int cnt, double wall_z,
+                                           const double* __restrict__ H, const double* __restrict__ N, d3& F, d3& T,
+                                           d3& sf, d3& df, int& draws) {
+    F = mk(0, 0, 0);
+    T = mk(0, 0, 0);
+    const d3 cm = cm_of(S);
+    for (int k = 0; k < cnt; k++) {
+        const int j = nl[(size_t)k * stride + slot];
+        Seg B;
+        load_seg(hot, j, B);
+        F_cc_acc(P, S, K, cm, B, hot, j, F, T);
+    }
+    sf = mk(0, 0, 0);
+    df = mk(0, 0, 0);
+    if (dmin(S.q.z, S.p.z) < 1.2 * S.R) {
+        // F_cw: ABM/Forces.cpp:88-204 (one rand() draw per call, counted)
+        double L = S.L + 4.0 / 3 * S.R;
+        double M = L * S.R * S.R * kPI;
+        d3 r1, r2;
+        d3 F1 = F_cw_locus(P, S, K, cm, S.p, wall_z, M, r1, sf, df);
+        d3 F2 = F_cw_locus(P, S, K, cm, S.q, wall_z, M, r2, sf, df);
+        draws++;
+        F = add(F, F1);
+        T = add(T, cross(sub(r1, cm), F1));
+        F = add(F, F2);
+        T = add(T, cross(sub(r2, cm), F2));
+    }
+    // F_v: ABM/Forces.cpp:320-329
+    F = add(F, scl(K.v, -P.viscosity * 6.0 * kPI * S.R));
+    T = add(T, scl(K.w, -P.viscosity * 6.0 * kPI * S.R));
+    d3 Fs, Ts;
+    F_surf_tension(P, S, cm, H, N, Fs, Ts);
+    F = add(F, Fs);
+    T = add(T, Ts);
+    if (P.planar) {
+        T.z = 0;
+        F.y = 0;
+    }
+}
+
+__global__ void __launch_bounds__(MOVE_BLOCK) k_move_grow(
+    const double* __restrict__ hot_old, double* __restrict__ hot_new, int n, PhysParams P, double dt,
+    const int* __restrict__ order, const int* __restrict__ nl, const int* __restrict__ nl_count, int stride,
+    const double* __restrict__ H, const double* __restrict__ Nrm, const double* __restrict__ wall,
+    const double* __restrict__ gr1, const double* __restrict__ gr2, const int* __restrict__ type,
+    double* __restrict__ dynfric, double* __restrict__ stafric, int* __restrict__ divflag,
+    unsigned long long* __restrict__ wall_draws, int* __restrict__ err) {
+    __shared__ int lds_draws[MOVE_BLOCK / 64];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int draws = 0;
+    if (t < n) {
+        const int id = order[t];
+        Seg S;
+        Kin K;
+        load_seg(hot_old, id, S);
+        load_kin(hot_old, id, K);
+        const double2 ld = reinterpret_cast<const double2*>(hot_old + (size_t)id * HOT)[7];
+        const double Ldiv = ld.x;
+        d3 cm = cm_of(S);
+        int x, y, z;
+        int a = bin_of(P, cm, x, y, z);  // MoveCell oldAddress: ABM/Cells.cpp:25-26
+        if (a < 0) {
+            // reported; the cell is carried over unchanged and does not divide
+            atomicOr(err, ERRBIT_DOMAIN);
+            const double2* src = reinterpret_cast<const double2*>(hot_old + (size_t)id * HOT);
+            double2* dst = reinterpret_cast<double2*>(hot_new + (size_t)id * HOT);
+            for (int c = 0; c < HOT / 2; c++) dst[c] = src[c];
+            divflag[id] = 0;
+        } else {
+            const double wall_z = wall[x * P.BoxY + y];
+            const int cnt = nl_count[t];
+            d3 F, T, sf, df;
+            // integrate: ABM/Integrate.cpp:11-31
+            sum_forces(P, S, K, hot_old, nl, stride, t, cnt, wall_z, H, Nrm, F, T, sf, df, draws);
+            // UpdatePositions: ABM/Integrate.cpp:34-90
+            Kin K1;
+            half_kick(dt, F, T, S, K, K1);
+            Seg S1;
+            S1.p = add(S.p, scl(add(K1.v, cross(K1.w, sub(S.p, cm))), dt));
+            S1.q = add(S.q, scl(add(K1.v, cross(K1.w, sub(S.q, cm))), dt));
+            S1.L = S.L;
+            S1.R = S.R;
+            if (P.planar) {
+                S1.p.y = 0;
+                S1.q.y = 0;
+            }
+            sum_forces(P, S1, K1, hot_old, nl, stride, t, cnt, wall_z, H, Nrm, F, T, sf, df, draws);
+            // UpdateVelocities: ABM/Integrate.cpp:93-127
+            Kin K2;
+            half_kick(dt, F, T, S1, K1, K2);
+            if (P.planar) K2.v.y = 0;
+            // grow: ABM/grow.cpp:17-102 at the MoveCell newAddress (ABM/Cells.cpp:32)
+            int a2 = bin_of(P, cm_of(S1), x, y, z);
+            double GR = 0;
+            if (a2 < 0) {
+                atomicOr(err, ERRBIT_DOMAIN);
+            } else {
+                GR = (P.crossFeeding && type[id] != 1) ? gr2[a2] : gr1[a2];
+                GR = GR * 1.5850;
+                d3 v = sub(S1.q, S1.p);
+                double dL = S1.L * dt * GR;
+                d3 dv = scl(v, dL * 0.5 / S1.L);
+                S1.L += dL;
+                S1.p = sub(S1.p, dv);
+                S1.q = add(S1.q, dv);
+            }
+            double2* o = reinterpret_cast<double2*>(hot_new + (size_t)id * HOT);
+            o[0] = make_double2(S1.p.x, S1.p.y);
+            o[1] = make_double2(S1.p.z, S1.q.x);
+            o[2] = make_double2(S1.q.y, S1.q.z);
+            o[3] = make_double2(K2.v.x, K2.v.y);
+            o[4] = make_double2(K2.v.z, K2.w.x);
+            o[5] = make_double2(K2.w.y, K2.w.z);
+            o[6] = make_double2(S1.L, S1.R);
+            o[7] = make_double2(Ldiv, GR);
+            dynfric[3 * (size_t)id] = df.x;
+            dynfric[3 * (size_t)id + 1] = df.y;
+            dynfric[3 * (size_t)id + 2] = df.z;
+            stafric[3 * (size_t)id] = sf.x;
+            stafric[3 * (size_t)id + 1] = sf.y;
+            stafric[3 * (size_t)id + 2] = sf.z;
+            divflag[id] = (S1.L >= Ldiv) ? 1 : 0;  // GrowCell: ABM/Cells.cpp:48
+        }
+    }
+    // block sum of F_cw draws -> one 64-bit atomic (integer: order-free)
+    int w = draws;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o, 64);
+    if ((threadIdx.x & 63) == 0) lds_draws[threadIdx.x >> 6] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int i = 0; i < MOVE_BLOCK / 64; i++) s += lds_draws[i];
+        if (s) atomicAdd(wall_draws, (unsigned long long)s);
+    }
+}
+
+// Pass-1 forces only, in ID order (bcg_sum_forces).
+__global__ void __launch_bounds__(MOVE_BLOCK) k_sum_forces(const double* __restrict__ hot, int n, PhysParams P,
+                                                           const int* __restrict__ order, const int* __restrict__ nl,
+                                                           const int* __restrict__ nl_count, int stride,
+                                                           const double* __restrict__ H, const double* __restrict__ Nrm,
+                                                           const double* __restrict__ wall, double* __restrict__ Fo,
+                                                           double* __restrict__ To, int* __restrict__ err) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int id = order[t];
+    Seg S;
+    Kin K;
+    load_seg(hot, id, S);
+    load_kin(hot, id, K);
+    int x, y, z;
+    if (bin_of(P, cm_of(S), x, y, z) < 0) {
+        atomicOr(err, ERRBIT_DOMAIN);
+        return;
+    }
+    d3 F, T, sf, df;
+    int draws = 0;
+    sum_forces(P, S, K, hot, nl, stride, t, nl_count[t], wall[x * P.BoxY + y], H, Nrm, F, T, sf, df, draws);
+    Fo[3 * (size_t)id] = F.x;
+    Fo[3 * (size_t)id + 1] = F.y;
+    Fo[3 * (size_t)id + 2] = F.z;
+    To[3 * (size_t)id] = T.x;
+    To[3 * (size_t)id + 1] = T.y;
+    To[3 * (size_t)id + 2] = T.z;
+}
+
+// ----------------------------------------------------------- compact ---
+// Per-block count of flagged cells (wave ballot + popcount).
+__global__ void __launch_bounds__(256) k_flag_count(const int* __restrict__ flag, int n, int* __restrict__ tot) {
+    __shared__ int lds[4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    unsigned long long b = __ballot(i < n && flag[i] != 0);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = __popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) tot[blockIdx.x] = lds[0] + lds[1] + lds[2] + lds[3];
+}
+
+// Write flagged IDs in ascending order: block offset + wave offset + lane rank.
+__global__ void __launch_bounds__(256) k_flag_write(const int* __restrict__ flag, int n, const int* __restrict__ off,
+                                                    int* __restrict__ ids) {
+    __shared__ int lds[4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const bool f = i < n && flag[i] != 0;
+    unsigned long long b = __ballot(f);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) lds[w] = __popcll(b);
+    __syncthreads();
+    int woff = 0;
+    for (int k = 0; k < w; k++) woff += lds[k];
+    if (f) {
+        unsigned long long below = lane ? (b & ((~0ull) >> (64 - lane))) : 0ull;
+        ids[off[blockIdx.x] + woff + __popcll(below)] = i;
+    }
+}
+
+// ------------------------------------------------------------ divide ---
+// divide (ABM/grow.cpp:107-186) + DivideCell bookkeeping, one thread per
+// division; u holds the 9 draws of each division, already mapped to
+// ((float)rand()/RAND_MAX - 0.5) on the host.
+__global__ void k_divide(double* __restrict__ hot, int n0, int D, const int* __restrict__ div_ids,
+                         const double* __restrict__ u, PhysParams P, double t, double* __restrict__ mother_pre,
+                         int* __restrict__ div_index_of, double* __restrict__ time_p, double* __restrict__ time_q,
+                         int* __restrict__ age_p, int* __restrict__ age_q, int* __restrict__ type,
+                         int* __restrict__ ancestor, double* __restrict__ dynfric, double* __restrict__ stafric,
+                         int* __restrict__ order, int* __restrict__ slot_of, int* __restrict__ nl_count) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= D) return;
+    const int m = div_ids[k];
+    const int d = n0 + k;
+    double* hm = hot + (size_t)m * HOT;
+    double* hd = hot + (size_t)d * HOT;
+    for (int i = 0; i < HOT; i++) mother_pre[(size_t)k * HOT + i] = hm[i];
+    div_index_of[m] = k;
+    const double* r = u + 9 * (size_t)k;
+    d3 mp = mk(hm[0], hm[1], hm[2]), mq = mk(hm[3], hm[4], hm[5]);
+    const d3 mv = mk(hm[6], hm[7], hm[8]);
+    double mL = hm[12], mR = hm[13];
+    const double GR = hm[15];
+
+    double dl = r[0] * P.varL;
+    double dangle = r[1] * P.varAngle;
+    d3 pq = sub(mp, mq);
+    d3 ddir1 = mk(r[2], r[3], r[4]);
+    ddir1 = sub(ddir1, scl(pq, dot(ddir1, pq) / dot(pq, pq)));
+    ddir1 = scl(ddir1, 1 / sqrt(dot(ddir1, ddir1)));
+    d3 dOrien1 = scl(ddir1, dangle);
+    d3 ddir2 = mk(r[5], r[6], r[7]);
+    ddir2 = sub(ddir2, scl(pq, dot(ddir2, pq) / dot(pq, pq)));
+    ddir2 = scl(ddir2, 1 / sqrt(dot(ddir2, ddir2)));
+    d3 dOrien2 = scl(ddir2, dangle);
+
+    d3 uv = sub(mp, mq);
+    double L = mL;
+    uv = scl(uv, 1 / L);
+    d3 dq = mq;
+    double dR = P.cellRadius;
+    mL = L / 2 - mR + dl;
+    double dLen = L / 2 + mR - 2 * dR - dl;
+    double dwz = r[8] * 0.001;
+    mq = sub(mp, scl(uv, mL));
+    time_q[d] = time_q[m];
+    time_q[m] = t;
+    time_p[d] = t;
+    age_p[m] = age_p[m] + 1;
+    age_q[d] = age_q[m] + 1;
+    age_p[d] = 0;  // the reference leaves it as found in fresh (zeroed) storage
+    d3 dp = sub(mq, scl(uv, dR + mR));
+    dq = sub(dp, scl(uv, dLen));
+    mp = add(mp, scl(dOrien1, mL));
+    mq = sub(mq, scl(dOrien1, mL));
+    dp = add(dp, scl(dOrien2, dLen));
+    dq = sub(dq, scl(dOrien2, dLen));
+    type[d] = type[m];
+    ancestor[d] = ancestor[m];
+    const double mLdiv = P.L_divide / 2 + P.cellRadius + mL;
+    const double dLdiv = P.L_divide / 2 + P.cellRadius + dLen;
+    for (int c = 0; c < 3; c++) {
+        dynfric[3 * (size_t)d + c] = 0;
+        stafric[3 * (size_t)d + c] = 0;
+    }
+    hm[0] = mp.x; hm[1] = mp.y; hm[2] = mp.z;
+    hm[3] = mq.x; hm[4] = mq.y; hm[5] = mq.z;
+    hm[9] = 0; hm[10] = 0; hm[11] = 0;
+    hm[12] = mL;
+    hm[14] = mLdiv;
+    hd[0] = dp.x; hd[1] = dp.y; hd[2] = dp.z;
+    hd[3] = dq.x; hd[4] = dq.y; hd[5] = dq.z;
+    hd[6] = mv.x; hd[7] = mv.y; hd[8] = mv.z;
+    hd[9] = 0; hd[10] = 0; hd[11] = dwz;
+    hd[12] = dLen;
+    hd[13] = dR;
+    hd[14] = dLdiv;
+    hd[15] = GR;
+    order[d] = d;
+    slot_of[d] = d;
+    nl_count[d] = 0;
+}
+
+// ------------------------------------------------------------- patch ---
+__device__ __forceinline__ void vload(const Versioned& V, int j, int k, Seg& s) {
+    const int kj = V.div_index_of[j];
+    if (kj > k) load_seg(V.mother_pre, kj, s);
+    else load_seg(V.hot, j, s);
+}
+
+__device__ int gather_versioned(const PhysParams& P, const Versioned& V, int c, int k, int D,
+                                const int* __restrict__ div_ids, const int* __restrict__ start,
+                                const int* __restrict__ sorted, int* __restrict__ nl, int stride, int cap, int slot,
+                                int* __restrict__ err) {
+    Seg A;
+    vload(V, c, k, A);
+    const d3 cmA = cm_of(A);
+    int x0, y0, z0;
+    if (bin_of(P, cmA, x0, y0, z0) < 0) {
+        atomicOr(err, ERRBIT_DOMAIN);
+        return 0;
+    }
+    const int zlo = max(z0 - 1, 0), zhi = min(z0 + 2, P.BoxZ);
+    const int ylo = max(y0 - 1, 0), yhi = min(y0 + 2, P.BoxY);
+    const int xlo = max(x0 - 1, 0), xhi = min(x0 + 2, P.BoxX);
+    int m = 0;
+    for (int za = zlo; za < zhi; za++)
+        for (int ya = ylo; ya < yhi; ya++)
+            for (int xa = xlo; xa < xhi; xa++) {
+                const int a = xa * (P.BoxY * P.BoxZ) + ya * P.BoxZ + za;
+                for (int q = start[a]; q < start[a + 1]; q++) {
+                    const int j = sorted[q];
+                    if (j == c || j > V.n0 + k || V.div_index_of[j] > k) continue;
+                    Seg B;
+                    load_seg(V.hot, j, B);
+                    if (keep_neighbour(A, cmA, B, P.Rc)) {
+                        if (m < cap) nl[(size_t)m * stride + slot] = j;
+                        m++;
+                    }
+                }
+            }
+    // mothers of later divisions, in the boxes they occupied before dividing
+    for (int kk = k + 1; kk < D; kk++) {
+        const int j = div_ids[kk];
+        if (j == c) continue;
+        Seg B;
+        load_seg(V.mother_pre, kk, B);
+        int bx, by, bz;
+        if (bin_of(P, cm_of(B), bx, by, bz) < 0) continue;
+        if (bx < xlo || bx >= xhi || by < ylo || by >= yhi || bz < zlo || bz >= zhi) continue;
+        if (keep_neighbour(A, cmA, B, P.Rc)) {
+            if (m < cap) nl[(size_t)m * stride + slot] = j;
+            m++;
+        }
+    }
+    if (m > cap) {
+        atomicOr(err, ERRBIT_NLIST);
+        m = cap;
+    }
+    return m;
+}
+
+// ABM/Simulation.cpp:395-403 for every division in order: the mother, the
+// mother's current neighbours and the daughter are re-gathered against the
+// grid as it stands after that division.
+__global__ void __launch_bounds__(PATCH_BLOCK) k_patch(PhysParams P, Versioned V, int D, const int* __restrict__ div_ids,
+                                                       const int* __restrict__ start, const int* __restrict__ sorted,
+                                                       int* __restrict__ nl, int* __restrict__ nl_count, int stride,
+                                                       int cap, const int* __restrict__ slot_of,
+                                                       int* __restrict__ maxlist, int* __restrict__ err) {
+    extern __shared__ int members[];  // cap + 2
+    __shared__ int nmem;
+    for (int k = 0; k < D; k++) {
+        if (threadIdx.x == 0) {
+            const int m = div_ids[k];
+            const int sm = slot_of[m];
+            const int c = nl_count[sm];
+            members[0] = m;
+            for (int i = 0; i < c; i++) members[1 + i] = nl[(size_t)i * stride + sm];
+            members[c + 1] = V.n0 + k;
+            nmem = c + 2;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < nmem; i += blockDim.x) {
+            const int c = members[i];
+            const int s = slot_of[c];
+            const int cnt = gather_versioned(P, V, c, k, D, div_ids, start, sorted, nl, stride, cap, s, err);
+            nl_count[s] = cnt;
+            atomicMax(maxlist, cnt);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_reset_div_index(int* __restrict__ div_index_of, const int* __restrict__ div_ids, int D) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < D) div_index_of[div_ids[k]] = -1;
+}
+
+}  // namespace bcg

@@ -125,6 +125,9 @@ int bcg_set_fields(bcg_ctx* ctx, const double* height, const double* normal,
  * ABM/grow.cpp:109-152, ABM/Simulation.cpp:166). */
 int bcg_srand(bcg_ctx* ctx, uint32_t seed);
 int bcg_rand(bcg_ctx* ctx, int32_t* out);   /* one draw, as rand() */
+/* Host-only (no device needed): out[0..n) = the draws number skip .. skip+n-1
+ * of the stream after srand(seed), skipping by jump-ahead (O(log skip)).   */
+int bcg_rand_stream(uint32_t seed, uint64_t skip, int32_t n, int32_t* out);
 
 /* Neighbour lists. Replaces getNeighbours (ABM/Neighbours.cpp:24-55) with the
  * UniformGrid 27-box gather (ABM/UniformGrid.h:141-170) and the
@@ -144,11 +147,18 @@ int bcg_sum_forces(bcg_ctx* ctx, double* F, double* T);
  *   MoveCell + GrowCell for every cell (ABM/Cells.cpp:14-57), the division
  *   loop with DivideCell and the neighbour-list patch (:380-404), then the
  *   old/new buffer swap (:409-413).
- * rebuild_lists != 0 first rebuilds every neighbour list from the current
- * cells, as the reference does at each field-update step (:292).
+ * flags bit 0 (BCG_STEP_REBUILD): first rebuild every neighbour list from
+ *   the current cells, as the reference does at each field-update step (:292).
+ * flags bit 1 (BCG_STEP_NO_PATCH): skip the post-division list patch; the
+ *   caller promises BCG_STEP_REBUILD on the next step (the lists are then
+ *   rebuilt before anything reads them, so the result is unchanged).
  * lineage (may be NULL) receives (parent, daughter) pairs, lineage_cap pairs
- * at most; *n_div receives the number of divisions performed.              */
-int bcg_step(bcg_ctx* ctx, double dt, double t, int32_t rebuild_lists,
+ * at most; *n_div receives the number of divisions performed. The draws of
+ * the step (one per F_cw call, nine per division) come from the context's
+ * rand() stream in the reference's order.                                 */
+#define BCG_STEP_REBUILD  1
+#define BCG_STEP_NO_PATCH 2
+int bcg_step(bcg_ctx* ctx, double dt, double t, int32_t flags,
              int32_t* n_div, int32_t* lineage, int32_t lineage_cap);
 
 /* Counters of the last bcg_step: wall-contact draws (calls of F_cw), cells

@@ -1,0 +1,311 @@
+"""Parity of the HIP mechanics path (through the C ABI) with the reference.
+
+Checked against the reference's golden vectors (tests/golden, written by the
+reference's own code) and against the oracle restatement on the same inputs:
+  * neighbour sets: bit-exact;
+  * division events (lineage) and the rand() stream position: exact;
+  * forces: |dF| <= 1e-10 * (|F| + sum_j |F_ij|) per component;
+  * positions: within 1e-10 relative, velocities within 1e-8 of the
+    velocity scale (FP64 throughout; summation order differs).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from baccolonygrowth_amd.cells import CELL_DTYPE
+from helpers import GOLDEN, demo_params, planar_case, sorted_lists
+
+pytestmark = pytest.mark.gpu
+
+POS_RTOL = 1e-10
+FORCE_RTOL = 1e-10
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def Colony():
+    from baccolonygrowth_amd.simulation import Colony
+
+    return Colony
+
+
+def _cells(a):
+    return np.ascontiguousarray(a).view(CELL_DTYPE).reshape(-1).copy()
+
+
+def _abs_pair_sums(oracle, p, cells, counts, lists, idx=None):
+    """sum_j |F_ij| per cell (scale of the cancellation in sum_forces)."""
+    S = np.zeros(len(cells))
+    for i in range(len(cells)) if idx is None else idx:
+        for j in lists[i, : counts[i]]:
+            f = oracle.F_cc(p, cells[i : i + 1], cells[j : j + 1])
+            S[i] += np.abs(f[:3]).max()
+    return S
+
+
+def _check_cells(a, b, label):
+    assert len(a) == len(b), label
+    for f in ("age_p", "age_q", "Type", "Ancestor"):
+        assert np.array_equal(a[f], b[f]), (label, f)
+    for f in ("Length", "Ldiv", "Radius", "GrowthRate", "time_p", "time_q"):
+        d = np.abs(a[f] - b[f]).max()
+        assert d <= 1e-12 * max(np.abs(b[f]).max(), 1.0), (label, f, d)
+    for f in ("p", "q"):
+        d = np.abs(a[f] - b[f]).max()
+        assert d <= POS_RTOL * max(np.abs(b[f]).max(), 1.0), (label, f, d)
+    for f in ("Velocity", "AngularVelocity"):
+        d = np.abs(a[f] - b[f]).max()
+        assert d <= 1e-8 * max(np.abs(b[f]).max(), 1.0), (label, f, d)
+
+
+@pytest.mark.parametrize("k", [7, 8])
+def test_neighbours_and_forces_vs_reference_golden(Colony, oracle, k):
+    z = np.load(os.path.join(GOLDEN, f"golden_step_r{k}.npz"))
+    p = demo_params()
+    cells = _cells(z["cells_in"])
+    say(f"[gpu] golden r{k}: {len(cells)} cells")
+    with Colony(p) as col:
+        col.LoadCells(cells)
+        col.set_fields(z["height"], z["normal"], z["wall"], z["gr1"], z["gr2"])
+        col.getNeighbours()
+        counts, lists = col.neighbours()
+        assert np.array_equal(counts, z["nl_counts0"])
+        assert sorted_lists(counts, lists) == sorted_lists(z["nl_counts0"], z["nl_lists0"])
+        F, T = col.sum_forces()
+    S = _abs_pair_sums(oracle, p, cells, counts, lists)
+    for got, ref in ((F, z["F0"]), (T, z["T0"])):
+        scale = np.abs(ref) + S[:, None] * (1.0 if got is F else 10.0) + 1e-300
+        err = np.abs(got - ref) / scale
+        say(f"    max rel force/torque error {err.max():.3e}")
+        assert err.max() <= FORCE_RTOL
+
+
+@pytest.mark.parametrize("k", [7, 8])
+def test_steps_vs_reference_golden(Colony, k):
+    z = np.load(os.path.join(GOLDEN, f"golden_step_r{k}.npz"))
+    p = demo_params()
+    cells = _cells(z["cells_in"])
+    say(f"[gpu] golden r{k}: {int(z['steps'])} steps")
+    with Colony(p) as col:
+        col.LoadCells(cells)
+        col.set_fields(z["height"], z["normal"], z["wall"], z["gr1"], z["gr2"])
+        col.getNeighbours()
+        col.srand(int(z["seed"]))
+        t = float(z["t"])
+        lin = []
+        for s in range(int(z["steps"])):
+            L = col.step(float(z["dt"]), t, update=(s == int(z["rebuild_step"])))
+            lin.append(np.column_stack([np.full(len(L), s), L]))
+            t += float(z["dt"])
+        lin = np.concatenate(lin)
+        assert np.array_equal(lin, z["lineage"])
+        out = col.cells()
+        _check_cells(out, _cells(z["cells_out"]), f"r{k}")
+        counts, lists = col.neighbours()
+        assert sorted_lists(counts, lists) == sorted_lists(z["nl_counts1"], z["nl_lists1"])
+        assert [col.rand() for _ in range(4)] == z["next_draws"].tolist()
+
+
+def test_many_steps_vs_oracle_planar(Colony, oracle):
+    """30 steps with divisions every few steps and list rebuilds on the
+    reference cadence: lineage, lists and rand() position exact."""
+    p, t, dt, cells, fields = planar_case(7, seed=21, divide_frac=0.15)
+    say(f"[gpu] planar 30 steps, {len(cells)} cells")
+    o = oracle.sim(p)
+    o.set_cells(cells)
+    o.set_fields(*fields)
+    o.rebuild()
+    oracle.srand(5)
+    with Colony(p) as col:
+        col.LoadCells(cells)
+        col.set_fields(*fields)
+        col.getNeighbours()
+        col.srand(5)
+        tt = t
+        for s in range(30):
+            upd = s % 10 == 0
+            la = o.step(dt, tt, rebuild=upd)
+            lb = col.step(dt, tt, update=upd)
+            assert np.array_equal(la, lb), s
+            tt += dt
+        _check_cells(col.cells(), o.cells(), "30 steps")
+        ca, la_ = o.neighbours(stride=p.maxNeighbours)
+        cb, lb_ = col.neighbours()
+        assert sorted_lists(ca, la_) == sorted_lists(cb, lb_)
+        assert col.rand() == oracle.rand()
+    o.close()
+
+
+def _dome_case(n=20000, seed=2):
+    from baccolonygrowth_amd.colony import dome_colony, dome_params, synthetic_fields
+
+    cells = dome_colony(n, radius=110.0 * (n / 20000) ** 0.5, height=12.0, seed=seed)
+    p = dome_params(cells, bin_size=5.0, max_neighbours=96, extra_cells=n)
+    rng = np.random.default_rng(seed)
+    cells["Ldiv"] = cells["Length"] * (1.0 + rng.uniform(0, 3e-3, n))
+    cells["Velocity"] = rng.normal(0, 0.5, (n, 3))
+    cells["AngularVelocity"] = rng.normal(0, 0.5, (n, 3))
+    fields = synthetic_fields(p, cells, seed=seed)
+    return p, cells, fields
+
+
+def test_3d_dome_vs_oracle(Colony, oracle):
+    """3-D model: lists exact, forces at 1e-10, 3 steps with divisions."""
+    p, cells, fields = _dome_case()
+    say(f"[gpu] 3-D dome {len(cells)} cells, bins {p.BoxX}x{p.BoxY}x{p.BoxZ}")
+    o = oracle.sim(p)
+    o.set_cells(cells)
+    o.set_fields(*fields)
+    o.rebuild()
+    with Colony(p) as col:
+        col.LoadCells(cells)
+        col.set_fields(*fields)
+        col.getNeighbours()
+        ca, la = o.neighbours(stride=p.maxNeighbours)
+        cb, lb = col.neighbours()
+        assert np.array_equal(ca, cb)
+        assert sorted_lists(ca, la) == sorted_lists(cb, lb)
+        Fa, Ta = o.sum_forces()
+        Fb, Tb = col.sum_forces()
+        sel = np.arange(0, len(cells), 37)
+        S = _abs_pair_sums(oracle, p, cells, ca, la, sel)
+        err = np.abs(Fb - Fa)[sel] / (np.abs(Fa)[sel] + S[sel, None] + 1e-300)
+        say(f"    3-D max rel force error {err.max():.3e}")
+        assert err.max() <= FORCE_RTOL
+        oracle.srand(11)
+        col.srand(11)
+        tt = 0.0
+        dt = 1.0 / 3600
+        nd = 0
+        for s in range(3):
+            la_ = o.step(dt, tt, rebuild=(s == 1))
+            lb_ = col.step(dt, tt, update=(s == 1))
+            assert np.array_equal(la_, lb_)
+            nd += len(la_)
+            tt += dt
+        assert nd > 0
+        _check_cells(col.cells(), o.cells(), "3-D")
+        assert col.rand() == oracle.rand()
+    o.close()
+
+
+def test_rebuild_every_step_no_patch_equals_patch(Colony):
+    """Skipping the post-division patch when the next step rebuilds gives
+    bit-identical cells (the patched lists are never read)."""
+    p, cells, fields = _dome_case(8000, seed=5)
+    res = []
+    for no_patch in (False, True):
+        with Colony(p) as col:
+            col.LoadCells(cells)
+            col.set_fields(*fields)
+            col.srand(3)
+            for s in range(4):
+                col.step(1.0 / 3600, s / 3600.0, update=True, no_patch=no_patch)
+            res.append(col.cells())
+    assert res[0].tobytes() == res[1].tobytes()
+
+
+def test_deterministic_bitwise(Colony):
+    p, cells, fields = _dome_case(8000, seed=6)
+    outs = []
+    for _ in range(2):
+        with Colony(p) as col:
+            col.LoadCells(cells)
+            col.set_fields(*fields)
+            col.srand(9)
+            for s in range(3):
+                col.step(1.0 / 3600, s / 3600.0, update=True)
+            outs.append((col.cells().tobytes(), col.neighbours()[0].tobytes()))
+    assert outs[0] == outs[1]
+
+
+# ----------------------------------------------------------- edge cases ---
+def test_empty_and_single(Colony):
+    p = demo_params()
+    with Colony(p) as col:
+        col.LoadCells(np.zeros(0, CELL_DTYPE))
+        assert col.step(1e-4, 0.0, update=True).shape == (0, 2)
+        assert col.n == 0
+        c = np.zeros(1, CELL_DTYPE)
+        c["p"] = (1.0, 0, 0.5)
+        c["q"] = (-1.0, 0, 0.5)
+        c["Length"] = 2.0
+        c["Radius"] = 0.5
+        c["Ldiv"] = 10.0
+        c["Type"] = 1
+        c["Ancestor"] = 1
+        col.LoadCells(c)
+        col.step(1e-4, 0.0, update=True)
+        assert col.n == 1
+        assert col.neighbours()[0].tolist() == [0]
+
+
+def test_out_of_domain_is_an_error(Colony):
+    from baccolonygrowth_amd._lib import BcgError
+
+    p = demo_params()
+    c = np.zeros(1, CELL_DTYPE)
+    c["p"] = (1e4, 0, 0.5)
+    c["q"] = (1e4 - 2, 0, 0.5)
+    c["Length"] = 2.0
+    c["Radius"] = 0.5
+    with Colony(p) as col:
+        with pytest.raises(BcgError) as e:
+            col.LoadCells(c)
+        assert e.value.code == -3
+
+
+def test_neighbour_overflow_is_an_error(Colony):
+    from baccolonygrowth_amd._lib import BcgError
+
+    p, t, dt, cells, fields = planar_case(8, seed=1)
+    p = p.replace(maxNeighbours=2)
+    with Colony(p) as col:
+        col.LoadCells(cells)
+        with pytest.raises(BcgError) as e:
+            col.getNeighbours()
+        assert e.value.code == -4
+
+
+def test_max_cells_stops_division(Colony, oracle):
+    """Reaching MaxCells stops the division loop (Simulation.cpp:382-386)."""
+    p, t, dt, cells, fields = planar_case(5, seed=8, divide_frac=0.5)
+    p = p.replace(maxCells=len(cells) + 3)
+    o = oracle.sim(p)
+    o.set_cells(cells)
+    o.set_fields(*fields)
+    o.rebuild()
+    oracle.srand(4)
+    with Colony(p) as col:
+        col.LoadCells(cells)
+        col.set_fields(*fields)
+        col.getNeighbours()
+        col.srand(4)
+        for s in range(3):
+            la = o.step(dt, t + s * dt, rebuild=False)
+            lb = col.step(dt, t + s * dt, update=False)
+            assert np.array_equal(la, lb)
+        assert col.n == len(cells) + 3 == o.n
+        _check_cells(col.cells(), o.cells(), "maxCells")
+    o.close()
+
+
+def test_partial_rebuild_matches_oracle(Colony, oracle):
+    p, t, dt, cells, fields = planar_case(8, seed=2)
+    ids = np.arange(0, len(cells), 13)
+    o = oracle.sim(p)
+    o.set_cells(cells)
+    o.rebuild(ids)
+    with Colony(p) as col:
+        col.LoadCells(cells)
+        col.getNeighbours(ids)
+        ca, la = o.neighbours(stride=p.maxNeighbours)
+        cb, lb = col.neighbours()
+        assert sorted_lists(ca, la) == sorted_lists(cb, lb)
+    o.close()
