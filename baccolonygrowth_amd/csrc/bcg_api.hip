@@ -59,8 +59,8 @@ struct bcg_ctx {
     Stats* stats = nullptr;  // device
     Stats* hstats = nullptr; // pinned host
     GlibcRand rng;
-    hipEvent_t ev[5];
-    float last_ms[4] = {0, 0, 0, 0};
+    hipEvent_t ev[7];
+    float last_ms[6] = {0, 0, 0, 0, 0, 0};
     Stats last{};
     int last_flagged = 0;
     std::string err;
@@ -289,7 +289,7 @@ int bcg_create(const bcg_params* p, bcg_ctx** out) {
     A(&c->stats, 1);
     if (!rc && hipHostMalloc((void**)&c->hstats, sizeof(Stats), hipHostMallocDefault) != hipSuccess)
         rc = BCG_ERR_HIP;
-    for (int i = 0; i < 5 && !rc; i++)
+    for (int i = 0; i < 7 && !rc; i++)
         if (hipEventCreate(&c->ev[i]) != hipSuccess) rc = BCG_ERR_HIP;
     if (rc) {
         bcg_destroy(c);
@@ -328,7 +328,7 @@ int bcg_destroy(bcg_ctx* c) {
     for (void* q : ptrs)
         if (q) hipFree(q);
     if (c->hstats) hipHostFree(c->hstats);
-    for (int i = 0; i < 5; i++)
+    for (int i = 0; i < 7; i++)
         if (c->ev[i]) hipEventDestroy(c->ev[i]);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -563,18 +563,20 @@ int bcg_step(bcg_ctx* c, double dt, double t, int32_t flags, int32_t* n_div, int
     }
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
     // MoveCell + GrowCell for every cell (ABM/Simulation.cpp:330-335)
-    if (N0 > 0) {
+    if (N0 > 0)
         hipLaunchKernelGGL(k_move_grow, dim3(nblk(N0, MOVE_BLOCK)), dim3(MOVE_BLOCK), 0, c->stream, c->hot[in],
                            c->hot[out], N0, c->P, dt, c->order, c->nl, c->nl_count, c->stride, c->height, c->normal,
                            c->wall, c->gr1, c->gr2, c->type, c->dynfric, c->stafric, c->divflag,
                            &c->stats->wall_draws, &c->stats->err);
+    HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+    if (N0 > 0) {
         const int nb = nblk(N0, 256);
         hipLaunchKernelGGL(k_flag_count, dim3(nb), dim3(256), 0, c->stream, c->divflag, N0, c->ids_tmp);
         if ((rc = scan(c, c->ids_tmp, nb, c->ids_tmp, &c->stats->ndiv))) return rc;
         hipLaunchKernelGGL(k_flag_write, dim3(nb), dim3(256), 0, c->stream, c->divflag, N0, c->ids_tmp, c->div_ids);
         HIPCHK(c, hipGetLastError());
     }
-    HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+    HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
     if ((rc = read_stats(c))) return rc;
     if ((rc = check_err(c))) return rc;
     // draws: every F_cw call took one (values unused: var_pos == 0), then
@@ -606,10 +608,10 @@ int bcg_step(bcg_ctx* c, double dt, double t, int32_t flags, int32_t* n_div, int
                            c->div_ids, D);
         HIPCHK(c, hipGetLastError());
     }
-    HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+    HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
     if ((rc = read_stats(c))) return rc;
     if ((rc = check_err(c))) return rc;
-    for (int i = 0; i < 4; i++) hipEventElapsedTime(&c->last_ms[i], c->ev[i], c->ev[i + 1]);
+    for (int i = 0; i < 5; i++) hipEventElapsedTime(&c->last_ms[i], c->ev[i], c->ev[i + 1]);
     if (lineage && D > 0) {
         std::vector<int> ids(D);
         HIPCHK(c, hipMemcpy(ids.data(), c->div_ids, D * sizeof(int), hipMemcpyDeviceToHost));
@@ -632,9 +634,9 @@ int bcg_last_step_stats(bcg_ctx* c, int64_t* wall_draws, int32_t* n_flagged, int
     return BCG_OK;
 }
 
-int bcg_last_step_times(bcg_ctx* c, float* ms4) {
-    if (!c || !ms4) return BCG_ERR_ARG;
-    for (int i = 0; i < 4; i++) ms4[i] = c->last_ms[i];
+int bcg_last_step_times(bcg_ctx* c, float* ms5) {
+    if (!c || !ms5) return BCG_ERR_ARG;
+    for (int i = 0; i < 5; i++) ms5[i] = c->last_ms[i];
     return BCG_OK;
 }
 

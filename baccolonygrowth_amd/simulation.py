@@ -19,7 +19,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import BcgError, _dp, _fp, _i64p, _ip, _vp, lib
+from ._lib import BcgError, _dp, _ip, _vp, lib
 from .cells import CELL_DTYPE, Params
 
 STEP_REBUILD = 1
@@ -150,9 +150,9 @@ class Colony:
         return {"wall_draws": w.value, "flagged": f.value, "max_list": m.value}
 
     def last_step_times(self):
-        ms = (ctypes.c_float * 4)()
+        ms = (ctypes.c_float * 5)()
         self._c(lib().bcg_last_step_times(self._h, ms))
-        return {"bins": ms[0], "lists": ms[1], "move_grow": ms[2], "divide": ms[3]}
+        return {"bins": ms[0], "lists": ms[1], "move_grow": ms[2], "compact": ms[3], "divide": ms[4]}
 
 
 __all__ = ["Colony", "BcgError", "default_params", "STEP_REBUILD", "STEP_NO_PATCH"]

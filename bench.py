@@ -1,0 +1,201 @@
+"""Benchmark of the mechanics hot path on one MI355X.
+
+Workload: the 2^20-cell kernel bench - spherocylinders (R = 0.5 um, L ~
+U[2,4] um, shortened where needed to keep overlaps <= 0.05 um) in a dome of
+radius 600 um and height 20 um, tilt ~N(0,15 deg), 3-D model, 5 um bins.
+One step = counting-sort bins + full neighbour-list rebuild + both force
+passes, integration and growth + division (compaction, divide kernel),
+i.e. one cell-update of every cell. Inputs are resident in HBM before the
+timed region; the steps are timed between device synchronisations.
+
+Prints one JSON line (rank 0, one GPU):
+  value        cell-updates/s over the timed steps
+  roofline     the dominant kernel's algorithmic HBM bytes / its mean
+               duration (HIP events on the library's stream)
+  cpu_baseline the oracle restatement (serial C, FP64, same algorithm) on
+               the same colony for a few steps, timed on this host
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s at 1M cells; MG solve ms mixed-vs-FP64 at 1e-10 resid"
+HBM_PEAK_GBS = 8000.0
+DT_H = 1.0 / 3600.0  # 1 s mechanics step, in the reference's time unit (hours)
+GR_BASE = 0.875  # GrowthRate1 such that 1.585 * GR = ln2 / 0.5 h (30 min doubling)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# Algorithmic HBM bytes per cell of each phase (each datum read or written
+# once; neighbour records that the gathers re-read are cache traffic, not
+# algorithmic). kbar = mean reduced list length.
+def phase_bytes(kbar: float) -> dict:
+    return {
+        # cm from p,q (48) + key write (4) + count RMW (8) + key read (4) +
+        # fill RMW (8) + sorted write (4) + per-bin sort read/write (8)
+        "bins": 84.0,
+        # sorted/order/slot (12) + own p,q,L,R (64) + list writes (4 kbar) + count (4)
+        "lists": 80.0 + 4.0 * kbar,
+        # order+count (8) + list (4 kbar) + hot record in/out (128+128) +
+        # type, wall, growth rate (20) + DynFric/StaFric (48) + flag (4)
+        "move_grow": 336.0 + 4.0 * kbar,
+        "compact": 8.0,
+        "divide": 0.0,
+    }
+
+
+def make_workload(n: int, seed: int = 2):
+    from baccolonygrowth_amd.colony import dome_colony, dome_params, height_normal_from_cells
+
+    cells = dome_colony(n, radius=600.0, height=20.0, seed=seed)
+    p = dome_params(cells, bin_size=5.0, max_neighbours=64, extra_cells=max(n // 8, 1024))
+    rng = np.random.default_rng(seed)
+    # cells at uniformly spread points of their cycle: steady division rate
+    gr = 1.5850 * GR_BASE
+    cells["Ldiv"] = cells["Length"] * np.exp(gr * DT_H * rng.uniform(1.0, 1800.0, n))
+    cells["GrowthRate"] = GR_BASE
+    height, normal = height_normal_from_cells(p, cells)
+    wall = (rng.random((p.BoxX, p.BoxY)) - 0.5) * p.wall_rough
+    gr1 = np.full((p.BoxX, p.BoxY, p.BoxZ), GR_BASE)
+    return p, cells, (height, normal, wall, gr1, gr1.copy())
+
+
+def cpu_baseline(p, cells, fields, steps: int) -> dict:
+    """The oracle (serial C restatement of the same algorithm, FP64) on the
+    same colony: `steps` full steps (list rebuild + move/grow + divisions)."""
+    from oracle.oracle import Oracle
+
+    o = Oracle()
+    s = o.sim(p)
+    s.set_cells(cells)
+    s.set_fields(*fields)
+    o.srand(2)
+    n0 = s.n
+    updates = 0
+    t0 = time.perf_counter()
+    t = 0.0
+    for _ in range(steps):
+        updates += s.n
+        s.step(DT_H, t, rebuild=True)
+        t += DT_H
+    el = time.perf_counter() - t0
+    s.close()
+    return {
+        "value": updates / el,
+        "unit": "cell-updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"the same {n0}-cell dome, {steps} full steps incl. list rebuild, serial C oracle ({el:.1f} s)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cells", type=int, default=1 << 20)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is not built.", flush=True)
+        sys.exit(2)
+
+    from baccolonygrowth_amd.simulation import Colony
+
+    t0 = time.time()
+    p, cells, fields = make_workload(args.cells)
+    log(f"[bench] workload: {len(cells)} cells, bins {p.BoxX}x{p.BoxY}x{p.BoxZ}, built in {time.time() - t0:.1f}s")
+    col = Colony(p)
+    col.LoadCells(cells)
+    col.set_fields(*fields)
+    col.srand(2)
+    t = 0.0
+    for _ in range(args.warmup):
+        col.step(DT_H, t, update=True, no_patch=True)
+        t += DT_H
+    log(f"[bench] warmup done ({args.warmup} steps)")
+    phases = {k: 0.0 for k in ("bins", "lists", "move_grow", "compact", "divide")}
+    cell_updates = 0
+    maxlist = 0
+    ndiv = 0
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        cell_updates += col.n
+        ndiv += len(col.step(DT_H, t, update=True, no_patch=True))
+        t += DT_H
+        for k, v in col.last_step_times().items():
+            phases[k] += v
+        maxlist = max(maxlist, col.last_step_stats()["max_list"])
+    el = time.perf_counter() - ts
+    counts, _ = col.neighbours(stride=1)
+    kbar = float(counts.mean())
+    n_end = col.n
+    col.close()
+    value = cell_updates / el
+    ms_step = 1000.0 * el / args.steps
+    avg = {k: v / args.steps for k, v in phases.items()}
+    dom = max(avg, key=avg.get)
+    nmean = cell_updates / args.steps
+    bpc = phase_bytes(kbar)
+    achieved = bpc[dom] * nmean / (avg[dom] * 1e-3) / 1e9
+    log(f"[bench] {args.steps} steps: {ms_step:.3f} ms/step, phases(ms) " + ", ".join(f"{k}={v:.3f}" for k, v in avg.items()))
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] CPU baseline: {args.cpu_steps} oracle steps ...")
+        cpu = cpu_baseline(p, cells, fields, args.cpu_steps)
+        log(f"[bench] CPU baseline {cpu['value']:.3e} cell-updates/s")
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "mechanics step, 2^20-cell 3-D dome (r=600 um, h=20 um), 5 um bins, list rebuild every step",
+            "cells": int(args.cells),
+            "cells_end": int(n_end),
+            "divisions": int(ndiv),
+            "bins": [p.BoxX, p.BoxY, p.BoxZ],
+            "mean_list_length": kbar,
+            "max_list_length": int(maxlist),
+            "phase_ms": avg,
+            "parallelism": "1 GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_cell": bpc[dom],
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

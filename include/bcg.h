@@ -167,8 +167,9 @@ int bcg_last_step_stats(bcg_ctx* ctx, int64_t* wall_draws, int32_t* n_flagged,
                         int32_t* max_list);
 
 /* Per-phase device time of the last bcg_step in milliseconds, from HIP
- * events: [0] bins, [1] lists, [2] move+grow, [3] divisions.               */
-int bcg_last_step_times(bcg_ctx* ctx, float* ms4);
+ * events on the context's stream: [0] bins, [1] lists, [2] the move+grow
+ * kernel alone, [3] division compaction, [4] divisions and list patch.      */
+int bcg_last_step_times(bcg_ctx* ctx, float* ms5);
 
 #ifdef __cplusplus
 }
