@@ -50,6 +50,46 @@ SIGNATURES = {
     "bcg_last_step_times": (ctypes.c_int, [_vp, _fp]),
 }
 
+
+class RD3DParams(ctypes.Structure):
+    """Mirror of `bcg_rd3d_params` (include/bcg.h)."""
+
+    _fields_ = [
+        ("nx", ctypes.c_int32),
+        ("ny", ctypes.c_int32),
+        ("nz", ctypes.c_int32),
+        ("h", ctypes.c_double),
+        ("origin", ctypes.c_double * 3),
+        ("D", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("K", ctypes.c_double),
+        ("c_res", ctypes.c_double),
+        ("nu1", ctypes.c_int32),
+        ("nu2", ctypes.c_int32),
+        ("coarse_sweeps", ctypes.c_int32),
+        ("max_levels", ctypes.c_int32),
+        ("inner", ctypes.c_int32),
+    ]
+
+
+SIGNATURES.update(
+    {
+        "bcg_rd3d_default_params": (ctypes.c_int, [ctypes.POINTER(RD3DParams)]),
+        "bcg_rd3d_create": (ctypes.c_int, [ctypes.POINTER(RD3DParams), ctypes.POINTER(_vp)]),
+        "bcg_rd3d_destroy": (ctypes.c_int, [_vp]),
+        "bcg_rd3d_last_error": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int]),
+        "bcg_rd3d_levels": (ctypes.c_int, [_vp, _ip, _ip, ctypes.c_int32]),
+        "bcg_rd3d_set_field": (ctypes.c_int, [_vp, _dp]),
+        "bcg_rd3d_get_field": (ctypes.c_int, [_vp, _dp]),
+        "bcg_rd3d_set_uptake": (ctypes.c_int, [_vp, _dp]),
+        "bcg_rd3d_get_uptake": (ctypes.c_int, [_vp, _dp]),
+        "bcg_rd3d_deposit": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int32, _ip]),
+        "bcg_rd3d_step": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, _ip, _dp]),
+        "bcg_rd3d_vcycle": (ctypes.c_int, [_vp, ctypes.c_int32, _fp]),
+        "bcg_rd3d_last_times": (ctypes.c_int, [_vp, _fp]),
+    }
+)
+
 _lib = None
 
 

@@ -91,8 +91,9 @@ def _seg_dist(p1, q1, p2, q2):
     return np.sqrt(np.einsum("ij,ij->i", c1 - c2, c1 - c2))
 
 
-def _lattice_sites(rng, radius, height, Rs, a_par, a_perp, a_z, patch=20.0):
-    """Sites (x, y, z, azimuth) of patch-aligned rod lattices inside the cap."""
+def _lattice_sites(rng, radius, height, Rs, a_par, a_perp, a_z, patch=20.0, window=None):
+    """Sites (x, y, z, azimuth) of patch-aligned rod lattices inside the cap
+    (only patches meeting |x|, |y| <= window when a window is given)."""
     pts = []
     npx = int(math.ceil(2 * radius / patch))
     for ix in range(npx):
@@ -101,6 +102,8 @@ def _lattice_sites(rng, radius, height, Rs, a_par, a_perp, a_z, patch=20.0):
             y0 = -radius + iy * patch
             xc, yc = x0 + patch / 2, y0 + patch / 2
             if math.hypot(xc, yc) > radius + patch:
+                continue
+            if window is not None and (x0 > window or x0 + patch < -window or y0 > window or y0 + patch < -window):
                 continue
             th = rng.uniform(0, 2 * np.pi)
             u = np.array([np.cos(th), np.sin(th)])
@@ -254,3 +257,17 @@ def dome_params(cells: np.ndarray, bin_size: float = 5.0, max_neighbours: int = 
     p.maxCells = len(cells) + extra_cells
     p.maxNeighbours = max_neighbours
     return p
+
+
+def dome_window_centres(radius: float, height: float, window: float, seed: int = 3, scale: float = 0.99):
+    """Centres of mass of the cells of a dome (lattice of dome_colony at the
+    given pitch scale) that fall in |x|, |y| <= window: the part of a large
+    colony that a nutrient grid narrower than the colony sees."""
+    rng = np.random.default_rng(seed)
+    Rs = (radius * radius + height * height) / (2 * height)
+    pts = _lattice_sites(rng, radius, height, Rs, 5.0 * scale, 1.45 * scale, 1.5 * scale, window=window)
+    cm = pts[:, :3].copy()
+    cm[:, :2] += rng.uniform(-0.1, 0.1, (len(cm), 2))
+    cm[:, 2] += rng.uniform(0.0, 0.05, len(cm))
+    inside = (np.abs(cm[:, 0]) <= window) & (np.abs(cm[:, 1]) <= window)
+    return cm[inside]

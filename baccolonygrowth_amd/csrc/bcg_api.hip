@@ -51,6 +51,7 @@ struct bcg_ctx {
     int *age_p = nullptr, *age_q = nullptr, *type = nullptr, *ancestor = nullptr;
     int *key = nullptr, *bin_cnt = nullptr, *bin_start = nullptr, *bin_fill = nullptr, *sorted = nullptr;
     int* scan_tot = nullptr;
+    float4 *cmL = nullptr, *cand = nullptr;
     int *nl = nullptr, *nl_count = nullptr, *order = nullptr, *slot_of = nullptr;
     int *divflag = nullptr, *div_ids = nullptr, *div_index_of = nullptr, *ids_tmp = nullptr;
     double *draws = nullptr, *mother_pre = nullptr;
@@ -140,7 +141,7 @@ int build_bins(bcg_ctx* c, int buf, int n) {
     HIPCHK(c, hipMemsetAsync(c->bin_fill, 0, sizeof(int) * c->nbin, c->stream));
     if (n > 0)
         hipLaunchKernelGGL(k_bin_count, dim3(nblk(n, 256)), dim3(256), 0, c->stream, c->hot[buf], n, c->P, c->key,
-                           c->bin_cnt, &c->stats->err);
+                           c->bin_cnt, c->cmL, &c->stats->err);
     int rc = scan(c, c->bin_cnt, c->nbin + 1, c->bin_start, nullptr);
     if (rc) return rc;
     if (n > 0) {
@@ -148,6 +149,7 @@ int build_bins(bcg_ctx* c, int buf, int n) {
                            c->bin_fill, c->sorted);
         hipLaunchKernelGGL(k_bin_sort, dim3(nblk(c->nbin, 256)), dim3(256), 0, c->stream, c->bin_start, c->nbin,
                            c->sorted);
+        hipLaunchKernelGGL(k_bin_cand, dim3(nblk(n, 256)), dim3(256), 0, c->stream, c->sorted, n, c->cmL, c->cand);
     }
     HIPCHK(c, hipGetLastError());
     return 0;
@@ -156,8 +158,8 @@ int build_bins(bcg_ctx* c, int buf, int n) {
 int gather_all(bcg_ctx* c) {
     if (c->N > 0)
         hipLaunchKernelGGL(k_gather_all, dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, c->hot[c->cur], c->N, c->P,
-                           c->bin_start, c->sorted, c->nl, c->nl_count, c->stride, c->cap, c->order, c->slot_of,
-                           &c->stats->maxlist, &c->stats->err);
+                           c->bin_start, c->sorted, c->cand, c->nl, c->nl_count, c->stride, c->cap, c->order,
+                           c->slot_of, &c->stats->maxlist, &c->stats->err);
     HIPCHK(c, hipGetLastError());
     c->lists_valid = true;
     return 0;
@@ -267,6 +269,8 @@ int bcg_create(const bcg_params* p, bcg_ctx** out) {
     A(&c->bin_start, c->nbin + 1);
     A(&c->bin_fill, c->nbin);
     A(&c->sorted, M);
+    A(&c->cmL, M);
+    A(&c->cand, M);
     A(&c->scan_tot, 1024 * 64);
     A(&c->nl, (size_t)c->cap * c->stride);
     A(&c->nl_count, M);
@@ -322,7 +326,7 @@ int bcg_destroy(bcg_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->hot[0],   c->hot[1],    c->time_p,  c->time_q,     c->dynfric,   c->stafric,  c->age_p,
                     c->age_q,    c->type,      c->ancestor, c->key,       c->bin_cnt,   c->bin_start, c->bin_fill,
-                    c->sorted,   c->scan_tot,  c->nl,      c->nl_count,   c->order,     c->slot_of,  c->divflag,
+                    c->sorted,   c->scan_tot, c->cmL, c->cand,  c->nl,      c->nl_count,   c->order,     c->slot_of,  c->divflag,
                     c->div_ids,  c->div_index_of, c->ids_tmp, c->draws,   c->mother_pre, c->height,  c->normal,
                     c->wall,     c->gr1,       c->gr2,     c->Fbuf,       c->Tbuf,      c->stats};
     for (void* q : ptrs)
@@ -496,7 +500,8 @@ int bcg_rebuild_neighbours(bcg_ctx* c, const int32_t* ids, int32_t n) {
         u.erase(std::unique(u.begin(), u.end()), u.end());
         HIPCHK(c, hipMemcpyAsync(c->ids_tmp, u.data(), u.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
         hipLaunchKernelGGL(k_gather_ids, dim3(nblk((long)u.size(), 256)), dim3(256), 0, c->stream, c->hot[c->cur],
-                           c->ids_tmp, (int)u.size(), c->P, c->bin_start, c->sorted, c->nl, c->nl_count, c->stride,
+                           c->ids_tmp, (int)u.size(), c->P, c->bin_start, c->sorted, c->cand, c->nl, c->nl_count,
+                           c->stride,
                            c->cap, c->slot_of, &c->stats->maxlist, &c->stats->err);
         HIPCHK(c, hipGetLastError());
     }

@@ -99,13 +99,14 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_apply(const int* __restrict__ i
 
 // ------------------------------------------------------------- bins ---
 __global__ void k_bin_count(const double* __restrict__ hot, int n, PhysParams P, int* __restrict__ key,
-                            int* __restrict__ cnt, int* __restrict__ err) {
+                            int* __restrict__ cnt, float4* __restrict__ cmL, int* __restrict__ err) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Seg s;
     load_seg(hot, i, s);
+    const d3 cm = cm_of(s);
     int x, y, z;
-    int a = bin_of(P, cm_of(s), x, y, z);
+    int a = bin_of(P, cm, x, y, z);
     if (a < 0) {
         // reported as an error; the cell is filed in the nearest bin so that
         // every later kernel still sees a complete permutation
@@ -116,6 +117,11 @@ __global__ void k_bin_count(const double* __restrict__ hot, int n, PhysParams P,
         a = x * (P.BoxY * P.BoxZ) + y * P.BoxZ + z;
     }
     key[i] = a;
+    // FP32 centre and half segment length |p-q|/2 (rounded up) for the
+    // conservative candidate prefilter of the list build
+    const d3 v = sub(s.q, s.p);
+    const float hl = (float)(0.5 * sqrt(dot(v, v))) * 1.000001f + 1e-6f;
+    cmL[i] = make_float4((float)cm.x, (float)cm.y, (float)cm.z, hl);
     atomicAdd(&cnt[a], 1);
 }
 
@@ -144,12 +150,29 @@ __global__ void k_bin_sort(const int* __restrict__ start, int nbin, int* __restr
     }
 }
 
+// Candidate records in bin order: cand[pos] = cmL[sorted[pos]].
+__global__ void k_bin_cand(const int* __restrict__ sorted, int n, const float4* __restrict__ cmL,
+                           float4* __restrict__ cand) {
+    int pos = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos < n) cand[pos] = cmL[sorted[pos]];
+}
+
 // ------------------------------------------------------------ lists ---
+// Blocks are dispatched round-robin over the 8 XCDs; remap so that each XCD
+// (one L2) owns a contiguous range of bin-ordered slots, i.e. an x-slab.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    constexpr int X = 8;
+    const int per = nb / X, rem = nb % X;
+    const int xcd = b % X, k = b / X;
+    return xcd * per + (xcd < rem ? xcd : rem) + k;
+}
+
 // UniformGrid::GetNeighbours (ABM/UniformGrid.h:141-170) + reduceNeighbours
 // (ABM/Neighbours.cpp:57-93) for the cell in `slot`, against the current bins.
 __device__ __forceinline__ int gather_one(const PhysParams& P, const double* __restrict__ hot, int id, int slot,
                                           const int* __restrict__ start, const int* __restrict__ sorted,
-                                          int* __restrict__ nl, int stride, int cap, int* __restrict__ err) {
+                                          const float4* __restrict__ cand, int* __restrict__ nl, int stride, int cap,
+                                          int* __restrict__ err) {
     Seg A;
     load_seg(hot, id, A);
     d3 cmA = cm_of(A);
@@ -158,6 +181,9 @@ __device__ __forceinline__ int gather_one(const PhysParams& P, const double* __r
         atomicOr(err, ERRBIT_DOMAIN);
         return 0;
     }
+    // j can pass the reference filter only if |cm_j - cm_A| < reach + |p_j - q_j|/2
+    const float reach = (float)(A.L / 2.0 + P.Rc * A.R) + 1e-3f;
+    const float cx = (float)cmA.x, cy = (float)cmA.y, cz = (float)cmA.z;
     int m = 0;
     const int zlo = max(z0 - 1, 0), zhi = min(z0 + 2, P.BoxZ);
     const int ylo = max(y0 - 1, 0), yhi = min(y0 + 2, P.BoxY);
@@ -168,6 +194,10 @@ __device__ __forceinline__ int gather_one(const PhysParams& P, const double* __r
                 const int a = xa * (P.BoxY * P.BoxZ) + ya * P.BoxZ + za;
                 const int e = start[a + 1];
                 for (int k = start[a]; k < e; k++) {
+                    const float4 c = cand[k];
+                    const float dx = c.x - cx, dy = c.y - cy, dz = c.z - cz;
+                    const float rr = reach + c.w;
+                    if (dx * dx + dy * dy + dz * dz > rr * rr) continue;
                     const int j = sorted[k];
                     if (j == id) continue;
                     Seg B;
@@ -187,30 +217,31 @@ __device__ __forceinline__ int gather_one(const PhysParams& P, const double* __r
 
 __global__ void __launch_bounds__(256) k_gather_all(const double* __restrict__ hot, int n, PhysParams P,
                                                     const int* __restrict__ start, const int* __restrict__ sorted,
-                                                    int* __restrict__ nl, int* __restrict__ nl_count, int stride,
-                                                    int cap, int* __restrict__ order, int* __restrict__ slot_of,
+                                                    const float4* __restrict__ cand, int* __restrict__ nl,
+                                                    int* __restrict__ nl_count, int stride, int cap,
+                                                    int* __restrict__ order, int* __restrict__ slot_of,
                                                     int* __restrict__ maxlist, int* __restrict__ err) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int id = sorted[t];
     order[t] = id;
     slot_of[id] = t;
-    int m = gather_one(P, hot, id, t, start, sorted, nl, stride, cap, err);
+    int m = gather_one(P, hot, id, t, start, sorted, cand, nl, stride, cap, err);
     nl_count[t] = m;
     atomicMax(maxlist, m);
 }
 
 __global__ void __launch_bounds__(256) k_gather_ids(const double* __restrict__ hot, const int* __restrict__ ids, int nid,
                                                     PhysParams P, const int* __restrict__ start,
-                                                    const int* __restrict__ sorted, int* __restrict__ nl,
-                                                    int* __restrict__ nl_count, int stride, int cap,
-                                                    const int* __restrict__ slot_of, int* __restrict__ maxlist,
-                                                    int* __restrict__ err) {
+                                                    const int* __restrict__ sorted, const float4* __restrict__ cand,
+                                                    int* __restrict__ nl, int* __restrict__ nl_count, int stride,
+                                                    int cap, const int* __restrict__ slot_of,
+                                                    int* __restrict__ maxlist, int* __restrict__ err) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nid) return;
     const int id = ids[i];
     const int t = slot_of[id];
-    int m = gather_one(P, hot, id, t, start, sorted, nl, stride, cap, err);
+    int m = gather_one(P, hot, id, t, start, sorted, cand, nl, stride, cap, err);
     nl_count[t] = m;
     atomicMax(maxlist, m);
 }
@@ -266,7 +297,7 @@ __global__ void __launch_bounds__(MOVE_BLOCK) k_move_grow(
     double* __restrict__ dynfric, double* __restrict__ stafric, int* __restrict__ divflag,
     unsigned long long* __restrict__ wall_draws, int* __restrict__ err) {
     __shared__ int lds_draws[MOVE_BLOCK / 64];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     int draws = 0;
     if (t < n) {
         const int id = order[t];

@@ -1,0 +1,632 @@
+// mg_api.hip - host side of the 3-D nutrient solver (bcg_rd3d_* in include/bcg.h).
+//
+// Device state of one grid: the nutrient field c (FP64, ping-pong for the
+// refinement update), the previous field, the uptake density q, the FP64
+// right-hand side and coefficient a0 = 1 + dt q/(K + c_prev), and per
+// multigrid level FP64 and FP32 copies of u, f, a0 and a residual buffer.
+//
+// Two solvers of the same implicit step, both to a relative residual tol:
+//   mode 0  FP64 multigrid: V(nu1,nu2) cycles on A c = b, FP64 everywhere.
+//   mode 1  mixed: FP64 iterative refinement; each outer step forms the
+//           residual in FP64 (fused with the update), scales it into FP32
+//           and runs `inner` FP32 V-cycles (red-black smoother and
+//           transfers in FP32, coarsest level solved in FP64).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bcg.h"
+#include "mech_kernels.h"
+#include "mg_kernels.h"
+
+using namespace bcg;
+using namespace bcg::mg;
+
+namespace {
+
+struct Level {
+    int nx, ny, nz;
+    int cx = 0, cy = 0, cz = 0;  // coarsening to the next level
+    double lx, ly, lz;
+    long n() const { return (long)nx * ny * nz; }
+    double *u64 = nullptr, *f64 = nullptr, *a064 = nullptr, *r64 = nullptr;
+    float *u32 = nullptr, *f32 = nullptr, *a032 = nullptr, *r32 = nullptr;
+};
+
+// Deterministic cloud-in-cell deposition: cells sorted by base vertex, each
+// vertex gathers from its 8 base-vertex buckets in a fixed order.
+__global__ void k_dep_key(const double* __restrict__ cm, int n, double ox, double oy, double oz, double inv_h, int nx,
+                          int ny, int nz, int* __restrict__ key, double* __restrict__ frac, int* __restrict__ cnt) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const double gx = (cm[3 * c] - ox) * inv_h, gy = (cm[3 * c + 1] - oy) * inv_h, gz = (cm[3 * c + 2] - oz) * inv_h;
+    const double fx = floor(gx), fy = floor(gy), fz = floor(gz);
+    int k = -1;
+    if (fx >= 0 && fx < nx - 1 && fy >= 0 && fy < ny - 1 && fz >= 0 && fz < nz - 1) {
+        k = (int)(((long)fz * ny + (long)fy) * nx + (long)fx);
+        frac[3 * c] = gx - fx;
+        frac[3 * c + 1] = gy - fy;
+        frac[3 * c + 2] = gz - fz;
+        atomicAdd(&cnt[k], 1);
+    }
+    key[c] = k;
+}
+
+__global__ void k_dep_scatter(const int* __restrict__ key, int n, const int* __restrict__ start, int* __restrict__ fill,
+                              int* __restrict__ sorted) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int k = key[c];
+    if (k < 0) return;
+    sorted[start[k] + atomicAdd(&fill[k], 1)] = c;
+}
+
+__global__ void k_dep_sort(const int* __restrict__ start, long nn, int* __restrict__ sorted) {
+    const long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nn) return;
+    const int s = start[b], e = start[b + 1];
+    for (int i = s + 1; i < e; i++) {
+        const int v = sorted[i];
+        int j = i - 1;
+        while (j >= s && sorted[j] > v) {
+            sorted[j + 1] = sorted[j];
+            j--;
+        }
+        sorted[j + 1] = v;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_dep_gather(const int* __restrict__ start, const int* __restrict__ sorted,
+                                                    const double* __restrict__ frac, const double* __restrict__ rate,
+                                                    int nx, int ny, int nz, double inv_vol, double* __restrict__ q) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= nx || j >= ny) return;
+    double acc = 0.0;
+    for (int dz = 0; dz <= 1; dz++) {
+        const int bk = k - dz;
+        if (bk < 0 || bk >= nz - 1) continue;
+        for (int dy = 0; dy <= 1; dy++) {
+            const int bj = j - dy;
+            if (bj < 0 || bj >= ny - 1) continue;
+            for (int dx = 0; dx <= 1; dx++) {
+                const int bi = i - dx;
+                if (bi < 0 || bi >= nx - 1) continue;
+                const long b = ((long)bk * ny + bj) * nx + bi;
+                for (int s = start[b]; s < start[b + 1]; s++) {
+                    const int c = sorted[s];
+                    const double fx = frac[3 * c], fy = frac[3 * c + 1], fz = frac[3 * c + 2];
+                    const double wx = dx ? fx : 1.0 - fx, wy = dy ? fy : 1.0 - fy, wz = dz ? fz : 1.0 - fz;
+                    acc += rate[c] * (wx * wy * wz);
+                }
+            }
+        }
+    }
+    q[((long)k * ny + j) * nx + i] = acc * inv_vol;
+}
+
+// ||r|| of r = b - A u without storing r (FP64 path convergence check).
+__global__ void __launch_bounds__(256) k_resnorm64(const double* __restrict__ u, const double* __restrict__ b,
+                                                   const double* __restrict__ a0, int nx, int ny, int nz, double lx,
+                                                   double ly, double lz, double* __restrict__ part) {
+    __shared__ double lds[4];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    double rr = 0;
+    if (i < nx && j < ny) {
+        const long sxy = (long)nx * ny;
+        const long p = (long)k * sxy + (long)j * nx + i;
+        double r;
+        if (k == 0) {
+            r = b[p] - u[p];
+        } else {
+            const double s = lx * (u[p + (i > 0 ? -1 : 1)] + u[p + (i < nx - 1 ? 1 : -1)]) +
+                             ly * (u[p + (j > 0 ? -nx : nx)] + u[p + (j < ny - 1 ? nx : -nx)]) +
+                             lz * (u[p - sxy] + u[p + (k < nz - 1 ? sxy : -sxy)]);
+            r = b[p] - ((a0[p] + 2.0 * (lx + ly + lz)) * u[p] - s);
+        }
+        rr = r * r;
+    }
+    for (int o = 32; o > 0; o >>= 1) rr += __shfl_xor(rr, o, 64);
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    if ((tid & 63) == 0) lds[tid >> 6] = rr;
+    __syncthreads();
+    if (tid == 0) part[((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+}
+
+inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
+
+}  // namespace
+
+struct bcg_rd3d {
+    bcg_rd3d_params prm;
+    hipStream_t stream = nullptr;
+    std::vector<Level> L;
+    long N = 0;
+    double *x[2] = {nullptr, nullptr};  // field ping-pong
+    int cur = 0;
+    double *cprev = nullptr, *b = nullptr, *q = nullptr, *a0 = nullptr;
+    float* a0f = nullptr;
+    double* part = nullptr;  // per-block partial sums
+    long npart = 0;
+    double* dnorm = nullptr;
+    double* hnorm = nullptr;  // pinned
+    // deposition scratch
+    int *dkey = nullptr, *dcnt = nullptr, *dstart = nullptr, *dfill = nullptr, *dsorted = nullptr, *scan_tot = nullptr;
+    double *dfrac = nullptr, *dcm = nullptr, *drate = nullptr;
+    int dcap = 0;
+    hipEvent_t ev[4];
+    float last_ms[3] = {0, 0, 0};
+    std::string err;
+};
+
+namespace {
+
+int fail(bcg_rd3d* c, int code, const char* fmt, ...) {
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        c->err = buf;
+    }
+    return code;
+}
+
+#define HIPCHK(c, x)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (x);                                                                      \
+        if (e_ != hipSuccess) return fail(c, BCG_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+int dalloc(bcg_rd3d* c, T** p, size_t n) {
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e != hipSuccess) return fail(c, BCG_ERR_HIP, "hipMalloc(%zu): %s", n * sizeof(T), hipGetErrorString(e));
+    return 0;
+}
+
+dim3 grid2(const Level& v, int bx = 64, int by = 4) { return dim3(nblk(v.nx, bx), nblk(v.ny, by), v.nz); }
+dim3 grid_rb(const Level& v) { return dim3(nblk((v.nx + 1) / 2, 64), nblk(v.ny, 4), v.nz); }
+
+template <class T>
+struct Arr;
+template <>
+struct Arr<double> {
+    static double* u(Level& v) { return v.u64; }
+    static double* f(Level& v) { return v.f64; }
+    static double* a0(Level& v) { return v.a064; }
+    static double* r(Level& v) { return v.r64; }
+};
+template <>
+struct Arr<float> {
+    static float* u(Level& v) { return v.u32; }
+    static float* f(Level& v) { return v.f32; }
+    static float* a0(Level& v) { return v.a032; }
+    static float* r(Level& v) { return v.r32; }
+};
+
+template <class T>
+void smooth(bcg_rd3d* c, Level& v, int sweeps) {
+    for (int s = 0; s < sweeps; s++)
+        for (int color = 0; color < 2; color++)
+            hipLaunchKernelGGL((k_rbgs<T>), grid_rb(v), dim3(64, 4), 0, c->stream, Arr<T>::u(v), Arr<T>::f(v),
+                               Arr<T>::a0(v), v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly, (T)v.lz, color);
+}
+
+void coarse_solve(bcg_rd3d* c, Level& v) {
+    hipLaunchKernelGGL((k_fill<double>), dim3(nblk(v.n(), 256)), dim3(256), 0, c->stream, v.u64, v.n(), 0.0);
+    smooth<double>(c, v, c->prm.coarse_sweeps);
+}
+
+// One V-cycle at level l in precision T on (u_l, f_l); the coarsest level
+// is always solved in FP64.
+template <class T>
+void vcycle(bcg_rd3d* c, int l) {
+    Level& v = c->L[l];
+    const int last = (int)c->L.size() - 1;
+    if (l == last) {
+        if (sizeof(T) == 4) {
+            hipLaunchKernelGGL((k_cast<float, double>), dim3(nblk(v.n(), 256)), dim3(256), 0, c->stream, v.f32, v.f64,
+                               v.n());
+            coarse_solve(c, v);
+            hipLaunchKernelGGL((k_cast<double, float>), dim3(nblk(v.n(), 256)), dim3(256), 0, c->stream, v.u64, v.u32,
+                               v.n());
+        } else {
+            coarse_solve(c, v);
+        }
+        return;
+    }
+    Level& w = c->L[l + 1];
+    smooth<T>(c, v, c->prm.nu1);
+    hipLaunchKernelGGL((k_residual<T>), grid2(v), dim3(64, 4), 0, c->stream, Arr<T>::u(v), Arr<T>::f(v), Arr<T>::a0(v),
+                       Arr<T>::r(v), v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly, (T)v.lz);
+    hipLaunchKernelGGL((k_restrict<T>), grid2(w), dim3(64, 4), 0, c->stream, Arr<T>::r(v), Arr<T>::f(w), v.nx, v.ny, v.nz,
+                       w.nx, w.ny, w.nz, v.cx, v.cy, v.cz);
+    hipLaunchKernelGGL((k_fill<T>), dim3(nblk(w.n(), 256)), dim3(256), 0, c->stream, Arr<T>::u(w), w.n(), (T)0);
+    vcycle<T>(c, l + 1);
+    hipLaunchKernelGGL((k_prolong<T>), grid2(v), dim3(64, 4), 0, c->stream, Arr<T>::u(v), Arr<T>::u(w), v.nx, v.ny, v.nz,
+                       w.nx, w.ny, v.cx, v.cy, v.cz);
+    smooth<T>(c, v, c->prm.nu2);
+}
+
+int read_norm(bcg_rd3d* c, long nparts, double* out) {
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, c->part, (int)nparts, c->dnorm);
+    HIPCHK(c, hipMemcpyAsync(c->hnorm, c->dnorm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *out = std::sqrt(*c->hnorm);
+    return 0;
+}
+
+long nparts_of(const Level& v) {
+    dim3 g = grid2(v);
+    return (long)g.x * g.y * g.z;
+}
+
+// a0 on every level (FP64 and FP32) and the right-hand side from c_prev.
+int setup_system(bcg_rd3d* c) {
+    Level& f = c->L[0];
+    const long n = c->N;
+    hipLaunchKernelGGL(k_monod_a0, dim3(nblk(n, 256)), dim3(256), 0, c->stream, c->q, c->cprev, c->prm.K, c->prm.dt,
+                       f.a064, f.a032, n);
+    hipLaunchKernelGGL(k_rhs, dim3(nblk(n, 256)), dim3(256), 0, c->stream, c->cprev, c->b, c->prm.c_res,
+                       (long)f.nx * f.ny, n);
+    for (size_t l = 0; l + 1 < c->L.size(); l++) {
+        Level& v = c->L[l];
+        Level& w = c->L[l + 1];
+        hipLaunchKernelGGL((k_restrict_a0<double>), grid2(w), dim3(64, 4), 0, c->stream, v.a064, w.a064, v.nx, v.ny, v.nz,
+                           w.nx, w.ny, w.nz, v.cx, v.cy, v.cz);
+        hipLaunchKernelGGL((k_cast<double, float>), dim3(nblk(w.n(), 256)), dim3(256), 0, c->stream, w.a064, w.a032, w.n());
+    }
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bcg_rd3d_default_params(bcg_rd3d_params* p) {
+    if (!p) return BCG_ERR_ARG;
+    memset(p, 0, sizeof *p);
+    p->nx = 257;
+    p->ny = 257;
+    p->nz = 65;
+    p->h = 2.0;
+    p->origin[0] = -256.0;
+    p->origin[1] = -256.0;
+    p->origin[2] = -8.0;
+    p->D = 600.0;
+    p->dt = 10.0;
+    p->K = 0.1;
+    p->c_res = 20.0;
+    p->nu1 = 2;
+    p->nu2 = 2;
+    p->coarse_sweeps = 60;
+    p->max_levels = 12;
+    p->inner = 2;
+    return BCG_OK;
+}
+
+int bcg_rd3d_create(const bcg_rd3d_params* p, bcg_rd3d** out) {
+    if (!p || !out) return BCG_ERR_ARG;
+    *out = nullptr;
+    if (p->nx < 3 || p->ny < 3 || p->nz < 2 || !(p->h > 0) || !(p->D >= 0) || !(p->dt > 0) || p->nu1 < 0 ||
+        p->nu2 < 0 || p->coarse_sweeps < 1 || p->max_levels < 1 || p->inner < 1)
+        return BCG_ERR_ARG;
+    bcg_rd3d* c = new bcg_rd3d();
+    c->prm = *p;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return BCG_ERR_HIP;
+    }
+    // levels: halve every direction whose point count is 2m+1 (m >= 1)
+    int nx = p->nx, ny = p->ny, nz = p->nz;
+    double hx = p->h, hy = p->h, hz = p->h;
+    for (int l = 0; l < p->max_levels; l++) {
+        Level v;
+        v.nx = nx;
+        v.ny = ny;
+        v.nz = nz;
+        v.lx = p->dt * p->D / (hx * hx);
+        v.ly = p->dt * p->D / (hy * hy);
+        v.lz = p->dt * p->D / (hz * hz);
+        v.cx = (nx >= 5 && (nx - 1) % 2 == 0);
+        v.cy = (ny >= 5 && (ny - 1) % 2 == 0);
+        v.cz = (nz >= 3 && (nz - 1) % 2 == 0);
+        const bool more = (v.cx || v.cy || v.cz) && l + 1 < p->max_levels;
+        if (!more) v.cx = v.cy = v.cz = 0;
+        c->L.push_back(v);
+        if (!more) break;
+        if (v.cx) nx = (nx - 1) / 2 + 1, hx *= 2;
+        if (v.cy) ny = (ny - 1) / 2 + 1, hy *= 2;
+        if (v.cz) nz = (nz - 1) / 2 + 1, hz *= 2;
+    }
+    c->N = c->L[0].n();
+    int rc = 0;
+    auto A = [&](auto** ptr, size_t n) {
+        if (!rc) rc = dalloc(c, ptr, n);
+    };
+    const size_t N = (size_t)c->N;
+    A(&c->x[0], N);
+    A(&c->x[1], N);
+    A(&c->cprev, N);
+    A(&c->b, N);
+    A(&c->q, N);
+    for (size_t l = 0; l < c->L.size(); l++) {
+        Level& v = c->L[l];
+        const size_t n = (size_t)v.n();
+        if (l == 0) {
+            v.f64 = c->b;  // the FP64 path solves on the field itself
+        } else {
+            A(&v.u64, n);
+            A(&v.f64, n);
+        }
+        A(&v.a064, n);
+        A(&v.r64, n);
+        A(&v.u32, n);
+        A(&v.f32, n);
+        A(&v.a032, n);
+        A(&v.r32, n);
+    }
+    c->npart = nparts_of(c->L[0]);
+    A(&c->part, (size_t)c->npart);
+    A(&c->dnorm, 1);
+    if (!rc && hipHostMalloc((void**)&c->hnorm, sizeof(double), hipHostMallocDefault) != hipSuccess) rc = BCG_ERR_HIP;
+    for (int i = 0; i < 4 && !rc; i++)
+        if (hipEventCreate(&c->ev[i]) != hipSuccess) rc = BCG_ERR_HIP;
+    if (!rc) {
+        // c = c_res everywhere, no uptake
+        hipLaunchKernelGGL((k_fill<double>), dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, c->x[0], c->N, p->c_res);
+        hipLaunchKernelGGL((k_fill<double>), dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, c->cprev, c->N, p->c_res);
+        hipLaunchKernelGGL((k_fill<double>), dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, c->q, c->N, 0.0);
+        if (hipStreamSynchronize(c->stream) != hipSuccess) rc = BCG_ERR_HIP;
+    }
+    if (rc) {
+        bcg_rd3d_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return BCG_OK;
+}
+
+int bcg_rd3d_destroy(bcg_rd3d* c) {
+    if (!c) return BCG_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    std::vector<void*> ptrs = {c->x[0], c->x[1], c->cprev, c->b, c->q, c->part, c->dnorm, c->dkey, c->dcnt,
+                               c->dstart, c->dfill, c->dsorted, c->scan_tot, c->dfrac, c->dcm, c->drate};
+    for (size_t l = 0; l < c->L.size(); l++) {
+        Level& v = c->L[l];
+        if (l > 0) {
+            ptrs.push_back(v.u64);
+            ptrs.push_back(v.f64);
+        }
+        for (void* q : {(void*)v.a064, (void*)v.r64, (void*)v.u32, (void*)v.f32, (void*)v.a032, (void*)v.r32})
+            ptrs.push_back(q);
+    }
+    for (void* q : ptrs)
+        if (q) hipFree(q);
+    if (c->hnorm) hipHostFree(c->hnorm);
+    for (int i = 0; i < 4; i++)
+        if (c->ev[i]) hipEventDestroy(c->ev[i]);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return BCG_OK;
+}
+
+int bcg_rd3d_last_error(const bcg_rd3d* c, char* buf, int len) {
+    if (!c || !buf || len <= 0) return BCG_ERR_ARG;
+    snprintf(buf, len, "%s", c->err.c_str());
+    return BCG_OK;
+}
+
+int bcg_rd3d_levels(bcg_rd3d* c, int32_t* n, int32_t* dims, int32_t cap) {
+    if (!c || !n) return BCG_ERR_ARG;
+    *n = (int32_t)c->L.size();
+    for (int l = 0; l < *n && l < cap && dims; l++) {
+        dims[3 * l] = c->L[l].nx;
+        dims[3 * l + 1] = c->L[l].ny;
+        dims[3 * l + 2] = c->L[l].nz;
+    }
+    return BCG_OK;
+}
+
+int bcg_rd3d_set_field(bcg_rd3d* c, const double* f) {
+    if (!c || !f) return BCG_ERR_ARG;
+    HIPCHK(c, hipMemcpy(c->x[c->cur], f, sizeof(double) * c->N, hipMemcpyHostToDevice));
+    return BCG_OK;
+}
+
+int bcg_rd3d_get_field(bcg_rd3d* c, double* f) {
+    if (!c || !f) return BCG_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(f, c->x[c->cur], sizeof(double) * c->N, hipMemcpyDeviceToHost));
+    return BCG_OK;
+}
+
+int bcg_rd3d_set_uptake(bcg_rd3d* c, const double* q) {
+    if (!c || !q) return BCG_ERR_ARG;
+    HIPCHK(c, hipMemcpy(c->q, q, sizeof(double) * c->N, hipMemcpyHostToDevice));
+    return BCG_OK;
+}
+
+int bcg_rd3d_get_uptake(bcg_rd3d* c, double* q) {
+    if (!c || !q) return BCG_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(q, c->q, sizeof(double) * c->N, hipMemcpyDeviceToHost));
+    return BCG_OK;
+}
+
+int bcg_rd3d_deposit(bcg_rd3d* c, const double* cm, const double* rate, int32_t n, int32_t* n_inside) {
+    if (!c || n < 0 || (n > 0 && (!cm || !rate))) return BCG_ERR_ARG;
+    Level& v = c->L[0];
+    const long nn = c->N;
+    if (n > c->dcap) {
+        for (void* q : {(void*)c->dkey, (void*)c->dsorted, (void*)c->dfrac, (void*)c->dcm, (void*)c->drate})
+            if (q) hipFree(q);
+        c->dkey = c->dsorted = nullptr;
+        c->dfrac = c->dcm = c->drate = nullptr;
+        int rc = 0;
+        auto A = [&](auto** ptr, size_t m) {
+            if (!rc) rc = dalloc(c, ptr, m);
+        };
+        A(&c->dkey, n);
+        A(&c->dsorted, n);
+        A(&c->dfrac, 3 * (size_t)n);
+        A(&c->dcm, 3 * (size_t)n);
+        A(&c->drate, n);
+        if (rc) return rc;
+        c->dcap = n;
+    }
+    if (!c->dcnt) {
+        int rc = 0;
+        auto A = [&](auto** ptr, size_t m) {
+            if (!rc) rc = dalloc(c, ptr, m);
+        };
+        A(&c->dcnt, nn + 1);
+        A(&c->dstart, nn + 1);
+        A(&c->dfill, nn);
+        A(&c->scan_tot, 1024 * 64);
+        if (rc) return rc;
+    }
+    if (n > 0) {
+        HIPCHK(c, hipMemcpyAsync(c->dcm, cm, 3 * sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->drate, rate, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(c, hipMemsetAsync(c->dcnt, 0, sizeof(int) * (nn + 1), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->dfill, 0, sizeof(int) * nn, c->stream));
+    if (n > 0)
+        hipLaunchKernelGGL(k_dep_key, dim3(nblk(n, 256)), dim3(256), 0, c->stream, c->dcm, n, c->prm.origin[0],
+                           c->prm.origin[1], c->prm.origin[2], 1.0 / c->prm.h, v.nx, v.ny, v.nz, c->dkey, c->dfrac,
+                           c->dcnt);
+    // exclusive scan of the bucket counts (the mechanics' scan kernels)
+    const long ns = nn + 1;
+    const int nb = nblk(ns, SCAN_BLOCK_ITEMS);
+    if (nb > 1024 * 64) return fail(c, BCG_ERR_ARG, "grid too large for the deposition scan");
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, c->stream, c->dcnt, (int)ns, c->scan_tot);
+    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(1024), 0, c->stream, c->scan_tot, nb, (int*)nullptr);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, c->stream, c->dcnt, (int)ns, c->scan_tot, c->dstart);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_dep_scatter, dim3(nblk(n, 256)), dim3(256), 0, c->stream, c->dkey, n, c->dstart, c->dfill,
+                           c->dsorted);
+        hipLaunchKernelGGL(k_dep_sort, dim3(nblk(nn, 256)), dim3(256), 0, c->stream, c->dstart, nn, c->dsorted);
+    }
+    hipLaunchKernelGGL(k_dep_gather, grid2(v), dim3(64, 4), 0, c->stream, c->dstart, c->dsorted, c->dfrac, c->drate,
+                       v.nx, v.ny, v.nz, 1.0 / (c->prm.h * c->prm.h * c->prm.h), c->q);
+    HIPCHK(c, hipGetLastError());
+    if (n_inside) {
+        int tot = 0;
+        HIPCHK(c, hipMemcpyAsync(&tot, c->dstart + nn, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        *n_inside = tot;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return BCG_OK;
+}
+
+int bcg_rd3d_step(bcg_rd3d* c, int32_t mode, double tol, int32_t max_iter, int32_t* iters, double* relres) {
+    if (!c || (mode != 0 && mode != 1) || max_iter < 1 || !(tol > 0)) return BCG_ERR_ARG;
+    Level& f = c->L[0];
+    double* xo = c->x[c->cur];
+    // c_prev <- c ; a0, b from c_prev ; initial guess c_new = c_prev
+    HIPCHK(c, hipMemcpyAsync(c->cprev, xo, sizeof(double) * c->N, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    int rc = setup_system(c);
+    if (rc) return rc;
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    int it = 0;
+    double r0 = 0, rn = 0;
+    const long np = c->npart;
+    if (mode == 0) {
+        f.u64 = xo;
+        hipLaunchKernelGGL(k_resnorm64, grid2(f), dim3(64, 4), 0, c->stream, xo, c->b, f.a064, f.nx, f.ny, f.nz, f.lx,
+                           f.ly, f.lz, c->part);
+        if ((rc = read_norm(c, np, &r0))) return rc;
+        rn = r0;
+        while (it < max_iter && rn > tol * r0) {
+            vcycle<double>(c, 0);
+            hipLaunchKernelGGL(k_resnorm64, grid2(f), dim3(64, 4), 0, c->stream, xo, c->b, f.a064, f.nx, f.ny, f.nz,
+                               f.lx, f.ly, f.lz, c->part);
+            if ((rc = read_norm(c, np, &rn))) return rc;
+            it++;
+        }
+        f.u64 = nullptr;
+    } else {
+        // r0 = b - A x0 (x copied into the other buffer), r32 = r0
+        double* xn = c->x[c->cur ^ 1];
+        hipLaunchKernelGGL(k_ir_update_residual, grid2(f), dim3(64, 4), 0, c->stream, xo, xn, (const float*)nullptr, 0.0,
+                           c->b, f.a064, f.f32, 1.0, f.nx, f.ny, f.nz, f.lx, f.ly, f.lz, c->part);
+        if ((rc = read_norm(c, np, &r0))) return rc;
+        c->cur ^= 1;
+        rn = r0;
+        double scale = 1.0;  // r32 = r * scale
+        while (it < max_iter && rn > tol * r0) {
+            // inner FP32 solve of A e = r32 from e = 0
+            hipLaunchKernelGGL((k_fill<float>), dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, f.u32, c->N, 0.0f);
+            for (int s = 0; s < c->prm.inner; s++) vcycle<float>(c, 0);
+            // x += e / scale ; r = b - A x ; r32 = r * next scale
+            const double rms = rn / std::sqrt((double)c->N);
+            const double next = rms > 0 ? std::ldexp(1.0, -std::ilogb(rms)) : 1.0;
+            xo = c->x[c->cur];
+            xn = c->x[c->cur ^ 1];
+            hipLaunchKernelGGL(k_ir_update_residual, grid2(f), dim3(64, 4), 0, c->stream, xo, xn, f.u32, 1.0 / scale,
+                               c->b, f.a064, f.f32, next, f.nx, f.ny, f.nz, f.lx, f.ly, f.lz, c->part);
+            if ((rc = read_norm(c, np, &rn))) return rc;
+            c->cur ^= 1;
+            scale = next;
+            it++;
+        }
+    }
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    hipEventElapsedTime(&c->last_ms[0], c->ev[0], c->ev[1]);
+    hipEventElapsedTime(&c->last_ms[1], c->ev[1], c->ev[2]);
+    if (iters) *iters = it;
+    if (relres) *relres = r0 > 0 ? rn / r0 : 0.0;
+    if (rn > tol * r0) return fail(c, BCG_ERR_STATE, "no convergence: relative residual %.3e after %d iterations",
+                                   r0 > 0 ? rn / r0 : 0.0, it);
+    return BCG_OK;
+}
+
+int bcg_rd3d_vcycle(bcg_rd3d* c, int32_t mode, float* ms) {
+    if (!c || (mode != 0 && mode != 1)) return BCG_ERR_ARG;
+    Level& f = c->L[0];
+    HIPCHK(c, hipMemcpyAsync(c->cprev, c->x[c->cur], sizeof(double) * c->N, hipMemcpyDeviceToDevice, c->stream));
+    int rc = setup_system(c);
+    if (rc) return rc;
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    if (mode == 0) {
+        f.u64 = c->x[c->cur];
+        vcycle<double>(c, 0);
+        f.u64 = nullptr;
+    } else {
+        hipLaunchKernelGGL((k_cast<double, float>), dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, c->b, f.f32, c->N);
+        hipLaunchKernelGGL((k_fill<float>), dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, f.u32, c->N, 0.0f);
+        vcycle<float>(c, 0);
+    }
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (ms) hipEventElapsedTime(ms, c->ev[0], c->ev[1]);
+    return BCG_OK;
+}
+
+int bcg_rd3d_last_times(bcg_rd3d* c, float* ms2) {
+    if (!c || !ms2) return BCG_ERR_ARG;
+    ms2[0] = c->last_ms[0];
+    ms2[1] = c->last_ms[1];
+    return BCG_OK;
+}
+
+}  // extern "C"

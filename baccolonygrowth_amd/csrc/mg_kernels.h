@@ -1,0 +1,35 @@
+// mg_kernels.h - declarations of the 3-D reaction-diffusion multigrid kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace bcg {
+namespace mg {
+
+template <class T>
+__global__ void k_rbgs(T* u, const T* f, const T* a0, int nx, int ny, int nz, T lx, T ly, T lz, int color);
+template <class T>
+__global__ void k_residual(const T* u, const T* f, const T* a0, T* r, int nx, int ny, int nz, T lx, T ly, T lz);
+template <class T>
+__global__ void k_restrict(const T* r, T* fc, int nx, int ny, int nz, int ncx, int ncy, int ncz, int cx, int cy,
+                           int cz);
+template <class T>
+__global__ void k_prolong(T* u, const T* ec, int nx, int ny, int nz, int ncx, int ncy, int cx, int cy, int cz);
+template <class T>
+__global__ void k_fill(T* a, long n, T v);
+template <class Ts, class Td>
+__global__ void k_cast(const Ts* a, Td* b, long n);
+template <class T>
+__global__ void k_sumsq(const T* x, long n, double* part);
+__global__ void k_sum_parts(const double* part, int n, double* out);
+__global__ void k_ir_update_residual(const double* xo, double* xn, const float* e, double inv_scale, const double* b,
+                                     const double* a0, float* r32, double rscale, int nx, int ny, int nz, double lx,
+                                     double ly, double lz, double* part);
+__global__ void k_monod_a0(const double* q, const double* c, double K, double dt, double* a0, float* a0f, long n);
+__global__ void k_rhs(const double* c, double* b, double c_res, long nxy, long n);
+template <class T>
+__global__ void k_restrict_a0(const T* a0f, T* a0c, int nx, int ny, int nz, int ncx, int ncy, int ncz, int cx, int cy,
+                              int cz);
+
+}  // namespace mg
+}  // namespace bcg

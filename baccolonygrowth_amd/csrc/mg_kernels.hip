@@ -1,0 +1,320 @@
+// mg_kernels.hip - 3-D implicit reaction-diffusion multigrid (gfx950).
+//
+// Operator on a vertex grid (x fastest, then y, then z):
+//   (A u)_p = a0_p u_p - lx (uW + uE) - ly (uS + uN) - lz (uB + uT)
+//             + 2 (lx + ly + lz) u_p,           a0 = 1 + dt*sigma
+// Neumann (mirror) faces at x = 0, nx-1, y = 0, ny-1, z = nz-1; the z = 0
+// rows are Dirichlet (identity). l* = dt*D/h*^2 per direction and level.
+//
+// Kernels are templated on the storage/arithmetic type T (float for the
+// inner solve of the mixed-precision refinement, double for the FP64 path):
+//   rbgs       red-black Gauss-Seidel sweep of one colour; each thread owns
+//              one point of that colour in an x row, so all lanes are busy
+//              and a wave covers a contiguous 512 B (fp32) row segment
+//   residual   r = f - A u
+//   restrict   full weighting (27 points, mirrored at Neumann faces)
+//   prolong    trilinear interpolation added to the fine iterate
+//   norms      per-block FP64 partial sums, reduced in a fixed order
+// plus the FP64 outer-loop kernels of the mixed-precision refinement.
+#include <hip/hip_runtime.h>
+
+#include "mg_kernels.h"
+
+namespace bcg {
+namespace mg {
+
+__device__ __forceinline__ int mirror(int i, int n) { return i < 0 ? -i : (i >= n ? 2 * (n - 1) - i : i); }
+
+template <class T>
+struct Lvl {
+    T* u;
+    const T* f;
+    const T* a0;
+    int nx, ny, nz;
+    T lx, ly, lz;
+};
+
+template <class T>
+__device__ __forceinline__ T neigh_sum(const T* __restrict__ u, int i, int j, int k, int nx, int ny, int nz, T lx, T ly,
+                                       T lz, long& p) {
+    const long sxy = (long)nx * ny;
+    p = (long)k * sxy + (long)j * nx + i;
+    const T w = u[p + (i > 0 ? -1 : 1)];
+    const T e = u[p + (i < nx - 1 ? 1 : -1)];
+    const T s = u[p + (j > 0 ? -nx : nx)];
+    const T n = u[p + (j < ny - 1 ? nx : -nx)];
+    const T b = u[p - sxy];  // k >= 1 here
+    const T t = u[p + (k < nz - 1 ? sxy : -sxy)];
+    return lx * (w + e) + ly * (s + n) + lz * (b + t);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_rbgs(T* __restrict__ u, const T* __restrict__ f, const T* __restrict__ a0,
+                                              int nx, int ny, int nz, T lx, T ly, T lz, int color) {
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (j >= ny) return;
+    const int i = 2 * (blockIdx.x * blockDim.x + threadIdx.x) + ((j + k + color) & 1);
+    if (i >= nx) return;
+    const long sxy = (long)nx * ny;
+    if (k == 0) {
+        const long p = (long)j * nx + i;
+        u[p] = f[p];
+        return;
+    }
+    long p;
+    const T s = neigh_sum<T>(u, i, j, k, nx, ny, nz, lx, ly, lz, p);
+    const T d = a0[p] + T(2) * (lx + ly + lz);
+    u[p] = (f[p] + s) / d;
+    (void)sxy;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_residual(const T* __restrict__ u, const T* __restrict__ f,
+                                                  const T* __restrict__ a0, T* __restrict__ r, int nx, int ny, int nz,
+                                                  T lx, T ly, T lz) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= nx || j >= ny) return;
+    if (k == 0) {
+        const long p = (long)j * nx + i;
+        r[p] = f[p] - u[p];
+        return;
+    }
+    long p;
+    const T s = neigh_sum<T>(u, i, j, k, nx, ny, nz, lx, ly, lz, p);
+    const T d = a0[p] + T(2) * (lx + ly + lz);
+    r[p] = f[p] - (d * u[p] - s);
+}
+
+// Full weighting onto the coarse grid; cx/cy/cz = 1 where that direction
+// is coarsened. Coarse z = 0 rows are Dirichlet: zero correction.
+template <class T>
+__global__ void __launch_bounds__(256) k_restrict(const T* __restrict__ r, T* __restrict__ fc, int nx, int ny, int nz,
+                                                  int ncx, int ncy, int ncz, int cx, int cy, int cz) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    const int K = blockIdx.z;
+    if (I >= ncx || J >= ncy) return;
+    const long pc = ((long)K * ncy + J) * ncx + I;
+    if (K == 0) {
+        fc[pc] = T(0);
+        return;
+    }
+    const int i0 = cx ? 2 * I : I, j0 = cy ? 2 * J : J, k0 = cz ? 2 * K : K;
+    const long sxy = (long)nx * ny;
+    T acc = T(0);
+    for (int c = -cz; c <= cz; c++) {
+        const int kk = mirror(k0 + c, nz);
+        const T wz = (c == 0) ? T(cz ? 0.5 : 1.0) : T(0.25);
+        T accy = T(0);
+        for (int b = -cy; b <= cy; b++) {
+            const int jj = mirror(j0 + b, ny);
+            const T wy = (b == 0) ? T(cy ? 0.5 : 1.0) : T(0.25);
+            const T* row = r + (long)kk * sxy + (long)jj * nx;
+            T accx;
+            if (cx) accx = T(0.25) * row[mirror(i0 - 1, nx)] + T(0.5) * row[i0] + T(0.25) * row[mirror(i0 + 1, nx)];
+            else accx = row[i0];
+            accy += wy * accx;
+        }
+        acc += wz * accy;
+    }
+    fc[pc] = acc;
+}
+
+// u_fine += P e_coarse (trilinear).
+template <class T>
+__global__ void __launch_bounds__(256) k_prolong(T* __restrict__ u, const T* __restrict__ ec, int nx, int ny, int nz,
+                                                 int ncx, int ncy, int cx, int cy, int cz) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= nx || j >= ny) return;
+    int I0, I1, J0, J1, K0, K1;
+    T wi, wj, wk;  // weight of the second point
+    if (cx) {
+        I0 = i >> 1;
+        I1 = (i + 1) >> 1;
+        wi = (i & 1) ? T(0.5) : T(0);
+    } else {
+        I0 = I1 = i;
+        wi = T(0);
+    }
+    if (cy) {
+        J0 = j >> 1;
+        J1 = (j + 1) >> 1;
+        wj = (j & 1) ? T(0.5) : T(0);
+    } else {
+        J0 = J1 = j;
+        wj = T(0);
+    }
+    if (cz) {
+        K0 = k >> 1;
+        K1 = (k + 1) >> 1;
+        wk = (k & 1) ? T(0.5) : T(0);
+    } else {
+        K0 = K1 = k;
+        wk = T(0);
+    }
+    const long scxy = (long)ncx * ncy;
+    auto E = [&](int K, int J, int I) { return ec[(long)K * scxy + (long)J * ncx + I]; };
+    const T e00 = (T(1) - wi) * E(K0, J0, I0) + wi * E(K0, J0, I1);
+    const T e01 = (T(1) - wi) * E(K0, J1, I0) + wi * E(K0, J1, I1);
+    const T e10 = (T(1) - wi) * E(K1, J0, I0) + wi * E(K1, J0, I1);
+    const T e11 = (T(1) - wi) * E(K1, J1, I0) + wi * E(K1, J1, I1);
+    const T e0 = (T(1) - wj) * e00 + wj * e01;
+    const T e1 = (T(1) - wj) * e10 + wj * e11;
+    u[((long)k * ny + j) * nx + i] += (T(1) - wk) * e0 + wk * e1;
+}
+
+template <class T>
+__global__ void k_fill(T* __restrict__ a, long n, T v) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+template <class Ts, class Td>
+__global__ void k_cast(const Ts* __restrict__ a, Td* __restrict__ b, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (Td)a[i];
+}
+
+// Block sums of x^2 (FP64), one partial per block.
+template <class T>
+__global__ void __launch_bounds__(256) k_sumsq(const T* __restrict__ x, long n, double* __restrict__ part) {
+    __shared__ double lds[4];
+    double s = 0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const double v = (double)x[i];
+        s += v * v;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+}
+
+__global__ void __launch_bounds__(256) k_sum_parts(const double* __restrict__ part, int n, double* __restrict__ out) {
+    __shared__ double lds[4];
+    double s = 0;
+    for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+}
+
+// Mixed-precision outer step, fused: x_new = x_old + e/scale (e = 0 when
+// null), then r = b - A x_new in FP64, r32 = r * rscale, and per-block
+// partial sums of r^2. x_new at the neighbours is formed on the fly from
+// x_old and e (ping-pong buffers: no read-after-write between blocks).
+__global__ void __launch_bounds__(256) k_ir_update_residual(const double* __restrict__ xo, double* __restrict__ xn,
+                                                            const float* __restrict__ e, double inv_scale,
+                                                            const double* __restrict__ b, const double* __restrict__ a0,
+                                                            float* __restrict__ r32, double rscale, int nx, int ny,
+                                                            int nz, double lx, double ly, double lz,
+                                                            double* __restrict__ part) {
+    __shared__ double lds[4];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    double rr = 0;
+    if (i < nx && j < ny) {
+        const long sxy = (long)nx * ny;
+        const long p = (long)k * sxy + (long)j * nx + i;
+        auto X = [&](long q) { return e ? xo[q] + (double)e[q] * inv_scale : xo[q]; };
+        const double xc = X(p);
+        xn[p] = xc;
+        double r;
+        if (k == 0) {
+            r = b[p] - xc;
+        } else {
+            const double s = lx * (X(p + (i > 0 ? -1 : 1)) + X(p + (i < nx - 1 ? 1 : -1))) +
+                             ly * (X(p + (j > 0 ? -nx : nx)) + X(p + (j < ny - 1 ? nx : -nx))) +
+                             lz * (X(p - sxy) + X(p + (k < nz - 1 ? sxy : -sxy)));
+            const double d = a0[p] + 2.0 * (lx + ly + lz);
+            r = b[p] - (d * xc - s);
+        }
+        r32[p] = (float)(r * rscale);
+        rr = r * r;
+    }
+    for (int o = 32; o > 0; o >>= 1) rr += __shfl_xor(rr, o, 64);
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    if ((tid & 63) == 0) lds[tid >> 6] = rr;
+    __syncthreads();
+    if (tid == 0) {
+        const long bid = ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        part[bid] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+    }
+}
+
+// a0 = 1 + dt * q / (K + c)   (Monod uptake linearised about c).
+__global__ void k_monod_a0(const double* __restrict__ q, const double* __restrict__ c, double K, double dt,
+                           double* __restrict__ a0, float* __restrict__ a0f, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double a = 1.0 + dt * q[i] / (K + c[i]);
+    a0[i] = a;
+    if (a0f) a0f[i] = (float)a;
+}
+
+// b = c with the Dirichlet rows set to c_res.
+__global__ void k_rhs(const double* __restrict__ c, double* __restrict__ b, double c_res, long nxy, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    b[i] = i < nxy ? c_res : c[i];
+}
+
+// Coarse-level a0 from the fine one: a0_c - 1 = restricted (a0_f - 1).
+template <class T>
+__global__ void __launch_bounds__(256) k_restrict_a0(const T* __restrict__ a0f, T* __restrict__ a0c, int nx, int ny,
+                                                     int nz, int ncx, int ncy, int ncz, int cx, int cy, int cz) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    const int K = blockIdx.z;
+    if (I >= ncx || J >= ncy) return;
+    const int i0 = cx ? 2 * I : I, j0 = cy ? 2 * J : J, k0 = cz ? 2 * K : K;
+    const long sxy = (long)nx * ny;
+    T acc = T(0), wsum = T(0);
+    for (int c = -cz; c <= cz; c++) {
+        const int kk = k0 + c;
+        if (kk < 0 || kk >= nz) continue;
+        const T wz = (c == 0) ? T(cz ? 0.5 : 1.0) : T(0.25);
+        for (int b = -cy; b <= cy; b++) {
+            const int jj = mirror(j0 + b, ny);
+            const T wy = (b == 0) ? T(cy ? 0.5 : 1.0) : T(0.25);
+            for (int a = -cx; a <= cx; a++) {
+                const int ii = mirror(i0 + a, nx);
+                const T wx = (a == 0) ? T(cx ? 0.5 : 1.0) : T(0.25);
+                acc += wz * wy * wx * (a0f[(long)kk * sxy + (long)jj * nx + ii] - T(1));
+                wsum += wz * wy * wx;
+            }
+        }
+    }
+    a0c[((long)K * ncy + J) * ncx + I] = T(1) + acc / wsum;
+}
+
+// ---------------------------------------------------- explicit instances
+template __global__ void k_rbgs<float>(float*, const float*, const float*, int, int, int, float, float, float, int);
+template __global__ void k_rbgs<double>(double*, const double*, const double*, int, int, int, double, double, double,
+                                        int);
+template __global__ void k_residual<float>(const float*, const float*, const float*, float*, int, int, int, float,
+                                           float, float);
+template __global__ void k_residual<double>(const double*, const double*, const double*, double*, int, int, int,
+                                            double, double, double);
+template __global__ void k_restrict<float>(const float*, float*, int, int, int, int, int, int, int, int, int);
+template __global__ void k_restrict<double>(const double*, double*, int, int, int, int, int, int, int, int, int);
+template __global__ void k_prolong<float>(float*, const float*, int, int, int, int, int, int, int, int);
+template __global__ void k_prolong<double>(double*, const double*, int, int, int, int, int, int, int, int);
+template __global__ void k_fill<float>(float*, long, float);
+template __global__ void k_fill<double>(double*, long, double);
+template __global__ void k_cast<float, double>(const float*, double*, long);
+template __global__ void k_cast<double, float>(const double*, float*, long);
+template __global__ void k_sumsq<float>(const float*, long, double*);
+template __global__ void k_sumsq<double>(const double*, long, double*);
+template __global__ void k_restrict_a0<float>(const float*, float*, int, int, int, int, int, int, int, int, int);
+template __global__ void k_restrict_a0<double>(const double*, double*, int, int, int, int, int, int, int, int, int);
+
+}  // namespace mg
+}  // namespace bcg

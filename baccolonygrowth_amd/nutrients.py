@@ -1,0 +1,125 @@
+"""3-D nutrient step on the GPU (bcg_rd3d_* of include/bcg.h).
+
+`Nutrients3D` holds one nutrient grid and solves the implicit step
+    (1 + dt q/(K + c_old)) c_new - dt D Lap_h c_new = c_old
+(Neumann x/y/top faces, Dirichlet c_res at z = 0) by FP64 multigrid
+(mode "fp64") or by FP32 V-cycles inside FP64 iterative refinement
+(mode "mixed"). It is the 3-D counterpart of the reference's planar
+nutrient update (Multigrid.cpp:984-1125); q comes from the cells by
+cloud-in-cell deposition (the role of GetDensity, Compute.cpp:120-253).
+Arrays are [nz][ny][nx] float64 on the host.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from ._lib import RD3DParams, BcgError, _dp, _fp, _ip, _vp, lib
+
+MODES = {"fp64": 0, "mixed": 1}
+
+
+def default_rd3d_params(**kw) -> RD3DParams:
+    p = RD3DParams()
+    rc = lib().bcg_rd3d_default_params(ctypes.byref(p))
+    if rc:
+        raise BcgError(rc)
+    for k, v in kw.items():
+        if k == "origin":
+            for i in range(3):
+                p.origin[i] = v[i]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Nutrients3D:
+    def __init__(self, params: RD3DParams):
+        self.params = params
+        h = _vp()
+        self._check(lib().bcg_rd3d_create(ctypes.byref(params), ctypes.byref(h)), None)
+        self._h = h
+        self.shape = (params.nz, params.ny, params.nx)
+
+    def _check(self, rc, h="self"):
+        if rc:
+            msg = ""
+            hh = self._h if h == "self" else h
+            if hh:
+                buf = ctypes.create_string_buffer(512)
+                lib().bcg_rd3d_last_error(hh, buf, 512)
+                msg = buf.value.decode(errors="replace")
+            raise BcgError(rc, msg)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().bcg_rd3d_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def levels(self):
+        n = ctypes.c_int32()
+        dims = np.zeros(3 * 32, np.int32)
+        self._check(lib().bcg_rd3d_levels(self._h, ctypes.byref(n), dims.ctypes.data_as(_ip), 32))
+        return [tuple(dims[3 * i : 3 * i + 3]) for i in range(n.value)]
+
+    def _arr(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError(f"expected {self.shape}, got {a.shape}")
+        return a
+
+    def set_field(self, c):
+        c = self._arr(c)
+        self._check(lib().bcg_rd3d_set_field(self._h, c.ctypes.data_as(_dp)))
+
+    def field(self):
+        out = np.empty(self.shape)
+        self._check(lib().bcg_rd3d_get_field(self._h, out.ctypes.data_as(_dp)))
+        return out
+
+    def set_uptake(self, q):
+        q = self._arr(q)
+        self._check(lib().bcg_rd3d_set_uptake(self._h, q.ctypes.data_as(_dp)))
+
+    def uptake(self):
+        out = np.empty(self.shape)
+        self._check(lib().bcg_rd3d_get_uptake(self._h, out.ctypes.data_as(_dp)))
+        return out
+
+    def deposit(self, cm, rate) -> int:
+        cm = np.ascontiguousarray(cm, dtype=np.float64).reshape(-1, 3)
+        rate = np.ascontiguousarray(rate, dtype=np.float64).reshape(-1)
+        inside = ctypes.c_int32()
+        self._check(
+            lib().bcg_rd3d_deposit(self._h, cm.ctypes.data_as(_dp), rate.ctypes.data_as(_dp), len(rate), ctypes.byref(inside))
+        )
+        return inside.value
+
+    def step(self, mode: str = "mixed", tol: float = 1e-10, max_iter: int = 100):
+        it = ctypes.c_int32()
+        rr = ctypes.c_double()
+        self._check(lib().bcg_rd3d_step(self._h, MODES[mode], tol, max_iter, ctypes.byref(it), ctypes.byref(rr)))
+        return it.value, rr.value
+
+    def vcycle(self, mode: str = "fp64") -> float:
+        ms = ctypes.c_float()
+        self._check(lib().bcg_rd3d_vcycle(self._h, MODES[mode], ctypes.byref(ms)))
+        return ms.value
+
+    def last_times(self):
+        ms = (ctypes.c_float * 2)()
+        self._check(lib().bcg_rd3d_last_times(self._h, ms))
+        return {"setup": ms[0], "solve": ms[1]}

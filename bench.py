@@ -101,6 +101,77 @@ def cpu_baseline(p, cells, fields, steps: int) -> dict:
     }
 
 
+# 3-D nutrient step (513 x 513 x 129 vertices at 1 um; D = 600 um^2/s,
+# dt = 10 s, K = 0.1 mM, reservoir 20 mM) with the Monod sink of the cells
+# of a 2^22-cell dome of radius 1200 um that fall inside the grid.
+MG_SHAPE = (129, 513, 513)
+MG_H = 1.0
+Q_CELL = 1.66  # uptake per cell, mM um^3 / s (about 1e6 glucose molecules/s)
+
+
+def vcycle_words(levels, nu1: int, nu2: int) -> float:
+    """Algorithmic words moved by one V-cycle, summed over levels: a red-black
+    sweep reads u, f, a0 and writes u (4 words/point), the residual 4, the
+    restriction 1 + 1/8, the prolongation 1/8 + 2, the coarse clear 1/8."""
+    words = 0.0
+    for lv in levels[:-1]:
+        n = float(lv[0]) * lv[1] * lv[2]
+        words += n * (4.0 * (nu1 + nu2) + 4.0 + 1.125 + 2.125 + 0.125)
+    n = float(levels[-1][0]) * levels[-1][1] * levels[-1][2]
+    return words + n * 4.0 * 60  # coarsest: sweeps in FP64
+
+
+def mg_bench(repeats: int = 3) -> dict:
+    from baccolonygrowth_amd.colony import dome_window_centres
+    from baccolonygrowth_amd.nutrients import Nutrients3D, default_rd3d_params
+
+    nz, ny, nx = MG_SHAPE
+    half = (nx - 1) * MG_H / 2
+    cm = dome_window_centres(1200.0, 20.0, half, seed=3)
+    n_total = 1 << 22
+    prm = default_rd3d_params(nx=nx, ny=ny, nz=nz, h=MG_H, D=600.0, dt=10.0, K=0.1, c_res=20.0,
+                              origin=(-half, -half, 0.0), nu1=2, nu2=2, inner=2)
+    res = {}
+    fields = {}
+    with Nutrients3D(prm) as s:
+        inside = s.deposit(cm, np.full(len(cm), Q_CELL))
+        levels = s.levels()
+        c0 = s.field()
+        for mode in ("fp64", "mixed"):
+            times = []
+            for _ in range(repeats):
+                s.set_field(c0)
+                it, rr = s.step(mode, tol=1e-10, max_iter=100)
+                times.append(s.last_times()["solve"])
+            fields[mode] = s.field()
+            res[mode] = {"ms": float(np.median(times)), "iterations": it, "relres": rr}
+            log(f"[bench] MG {mode}: {res[mode]['ms']:.2f} ms, {it} iterations, relres {rr:.2e}")
+        s.set_field(c0)
+        vms = {m: float(np.median([s.vcycle(m) for _ in range(5)])) for m in ("fp64", "mixed")}
+    words = vcycle_words(levels, 2, 2)
+    gbs = {m: words * (8 if m == "fp64" else 4) / (vms[m] * 1e-3) / 1e9 for m in vms}
+    err = float(np.abs(fields["mixed"] - fields["fp64"]).max())
+    return {
+        "grid": [nx, ny, nz],
+        "h_um": MG_H,
+        "levels": [list(map(int, l)) for l in levels],
+        "cells_deposited": int(inside),
+        "cells_total": n_total,
+        "fp64_ms": res["fp64"]["ms"],
+        "mixed_ms": res["mixed"]["ms"],
+        "speedup": res["fp64"]["ms"] / res["mixed"]["ms"],
+        "fp64_iterations": res["fp64"]["iterations"],
+        "mixed_refinements": res["mixed"]["iterations"],
+        "relres": {m: res[m]["relres"] for m in res},
+        "mixed_vs_fp64_max_abs_mM": err,
+        "vcycle_ms": vms,
+        "vcycle_roofline": {
+            m: {"achieved": gbs[m], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs[m] / HBM_PEAK_GBS}
+            for m in gbs
+        },
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +180,7 @@ def main():
     ap.add_argument("--cells", type=int, default=1 << 20)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mg", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is not built.", flush=True)
@@ -158,6 +230,10 @@ def main():
         log(f"[bench] CPU baseline: {args.cpu_steps} oracle steps ...")
         cpu = cpu_baseline(p, cells, fields, args.cpu_steps)
         log(f"[bench] CPU baseline {cpu['value']:.3e} cell-updates/s")
+    mg = None
+    if not args.no_mg:
+        log("[bench] multigrid 513x513x129 ...")
+        mg = mg_bench()
     out = {
         "metric": METRIC,
         "value": value,
@@ -193,6 +269,7 @@ def main():
             "bytes_per_cell": bpc[dom],
         },
         "cpu_baseline": cpu,
+        "mg_solve": mg,
     }
     print(json.dumps(out), flush=True)
 

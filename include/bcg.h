@@ -171,6 +171,64 @@ int bcg_last_step_stats(bcg_ctx* ctx, int64_t* wall_draws, int32_t* n_flagged,
  * kernel alone, [3] division compaction, [4] divisions and list patch.      */
 int bcg_last_step_times(bcg_ctx* ctx, float* ms5);
 
+
+/* ======================================================================
+ * 3-D nutrient step: implicit reaction-diffusion with Monod uptake.
+ *
+ * The reference solves the nutrient field only in its planar x-z model
+ * (UpdateEnvArrayMG_OxygenAcetate, ABM/Multigrid.cpp:984-1125, with the
+ * Crank-Nicolson multigrid of ABM/Multigrid.cpp:5-981 and the Monod colony
+ * chemistry of ABM/Nutrients.cpp:200-506). These entry points are its 3-D
+ * generalisation: one implicit step on a vertex grid (x fastest)
+ *     (1 + dt*q/(K + c_old)) c_new - dt*D*Lap_h c_new = c_old,
+ * Neumann on the x, y and top faces, Dirichlet c = c_res on z = 0 (the
+ * agar reservoir); q is the uptake-rate density deposited from the cells
+ * (the role of GetDensity/ShiftDensity, ABM/Compute.cpp:120-463).
+ * ====================================================================== */
+typedef struct bcg_rd3d_params {
+    int32_t nx, ny, nz;     /* vertices per direction (2^k+1 coarsens fully) */
+    double h;               /* vertex spacing, um                            */
+    double origin[3];       /* position of vertex (0,0,0), um                */
+    double D;               /* diffusivity, um^2/s                           */
+    double dt;              /* time step, s                                  */
+    double K;               /* Monod constant, mM                            */
+    double c_res;           /* Dirichlet value on z = 0, mM                  */
+    int32_t nu1, nu2;       /* red-black Gauss-Seidel pre/post sweeps        */
+    int32_t coarse_sweeps;  /* sweeps on the coarsest level (FP64)           */
+    int32_t max_levels;     /* multigrid levels at most                      */
+    int32_t inner;          /* FP32 V-cycles per refinement step (mode 1)    */
+} bcg_rd3d_params;
+
+typedef struct bcg_rd3d bcg_rd3d;
+
+#define BCG_RD3D_FP64  0   /* FP64 multigrid V-cycles                         */
+#define BCG_RD3D_MIXED 1   /* FP32 V-cycles inside FP64 iterative refinement  */
+
+int bcg_rd3d_default_params(bcg_rd3d_params* p);
+int bcg_rd3d_create(const bcg_rd3d_params* p, bcg_rd3d** out);
+int bcg_rd3d_destroy(bcg_rd3d* ctx);
+int bcg_rd3d_last_error(const bcg_rd3d* ctx, char* buf, int len);
+/* number of multigrid levels and their (nx, ny, nz), cap levels at most */
+int bcg_rd3d_levels(bcg_rd3d* ctx, int32_t* n, int32_t* dims, int32_t cap);
+/* nutrient field [nz][ny][nx], host buffers */
+int bcg_rd3d_set_field(bcg_rd3d* ctx, const double* c);
+int bcg_rd3d_get_field(bcg_rd3d* ctx, double* c);
+/* uptake-rate density q [nz][ny][nx] (rate per um^3), host buffers */
+int bcg_rd3d_set_uptake(bcg_rd3d* ctx, const double* q);
+int bcg_rd3d_get_uptake(bcg_rd3d* ctx, double* q);
+/* q from cells: cloud-in-cell deposition of rate[i] at centre cm[3i..3i+2]
+ * divided by h^3; cells outside the grid are dropped (*n_inside counts the
+ * rest). Deterministic: every vertex sums its cells in a fixed order.     */
+int bcg_rd3d_deposit(bcg_rd3d* ctx, const double* cm, const double* rate, int32_t n, int32_t* n_inside);
+/* One implicit step from the current field (which it replaces), solved to
+ * ||b - A c|| <= tol * ||b - A c_old||. *iters = V-cycles (mode 0) or
+ * refinement steps (mode 1). BCG_ERR_STATE if tol is not reached.          */
+int bcg_rd3d_step(bcg_rd3d* ctx, int32_t mode, double tol, int32_t max_iter, int32_t* iters, double* relres);
+/* One V-cycle of the current step's system (timing); *ms = its duration.  */
+int bcg_rd3d_vcycle(bcg_rd3d* ctx, int32_t mode, float* ms);
+/* device times of the last step, ms: [0] setup (a0, rhs, coarse a0), [1] solve */
+int bcg_rd3d_last_times(bcg_rd3d* ctx, float* ms2);
+
 #ifdef __cplusplus
 }
 #endif
