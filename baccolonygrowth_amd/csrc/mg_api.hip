@@ -545,14 +545,18 @@ int bcg_rd3d_step(bcg_rd3d* c, int32_t mode, double tol, int32_t max_iter, int32
     if (rc) return rc;
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     int it = 0;
-    double r0 = 0, rn = 0;
+    double r0 = 0, rn = 0, bn = 0;
     const long np = c->npart;
+    // the tolerance is relative to ||b||
+    hipLaunchKernelGGL((k_sumsq<double>), dim3(std::min<long>(np, 4096)), dim3(256), 0, c->stream, c->b, c->N, c->part);
+    if ((rc = read_norm(c, std::min<long>(np, 4096), &bn))) return rc;
     if (mode == 0) {
         f.u64 = xo;
         hipLaunchKernelGGL(k_resnorm64, grid2(f), dim3(64, 4), 0, c->stream, xo, c->b, f.a064, f.nx, f.ny, f.nz, f.lx,
                            f.ly, f.lz, c->part);
         if ((rc = read_norm(c, np, &r0))) return rc;
         rn = r0;
+        r0 = bn;
         while (it < max_iter && rn > tol * r0) {
             vcycle<double>(c, 0);
             hipLaunchKernelGGL(k_resnorm64, grid2(f), dim3(64, 4), 0, c->stream, xo, c->b, f.a064, f.nx, f.ny, f.nz,
@@ -569,6 +573,7 @@ int bcg_rd3d_step(bcg_rd3d* c, int32_t mode, double tol, int32_t max_iter, int32
         if ((rc = read_norm(c, np, &r0))) return rc;
         c->cur ^= 1;
         rn = r0;
+        r0 = bn;
         double scale = 1.0;  // r32 = r * scale
         while (it < max_iter && rn > tol * r0) {
             // inner FP32 solve of A e = r32 from e = 0

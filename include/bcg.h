@@ -221,8 +221,9 @@ int bcg_rd3d_get_uptake(bcg_rd3d* ctx, double* q);
  * rest). Deterministic: every vertex sums its cells in a fixed order.     */
 int bcg_rd3d_deposit(bcg_rd3d* ctx, const double* cm, const double* rate, int32_t n, int32_t* n_inside);
 /* One implicit step from the current field (which it replaces), solved to
- * ||b - A c|| <= tol * ||b - A c_old||. *iters = V-cycles (mode 0) or
- * refinement steps (mode 1). BCG_ERR_STATE if tol is not reached.          */
+ * the relative residual ||b - A c|| <= tol * ||b|| starting from c_old.
+ * *iters = V-cycles (mode 0) or refinement steps (mode 1); *relres the
+ * final ||b - A c|| / ||b||. BCG_ERR_STATE if tol is not reached.          */
 int bcg_rd3d_step(bcg_rd3d* ctx, int32_t mode, double tol, int32_t max_iter, int32_t* iters, double* relres);
 /* One V-cycle of the current step's system (timing); *ms = its duration.  */
 int bcg_rd3d_vcycle(bcg_rd3d* ctx, int32_t mode, float* ms);

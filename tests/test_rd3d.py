@@ -2,7 +2,7 @@
 on the CPU, then the HIP multigrid (FP64 and mixed precision) and the
 deposition checked against it on the GPU.
 
-Tolerances: the solve stops at ||b - A c|| <= 1e-10 ||b - A c_old||; the
+Tolerances: the solve stops at ||b - A c|| <= 1e-10 ||b||; the
 solution is compared with a direct sparse solve at 1e-9 relative (the
 residual tolerance times the operator's conditioning on these grids)."""
 import numpy as np
@@ -86,7 +86,7 @@ def test_step_matches_direct_solve(mode, shape):
     lam = dt * D / h**2
     b = rd3d.rhs(c_old, 20.0)
     ref = rd3d.solve_direct(b, sigma, lam, dt)
-    r0 = np.linalg.norm(rd3d.residual(b, c_old, sigma, lam, dt))
+    r0 = np.linalg.norm(b)
     with _solver(shape, h, D, dt, K) as s:
         s.set_field(c_old)
         s.set_uptake(q)
@@ -152,7 +152,7 @@ def test_full_size_257_converges_both_modes():
     sigma = q / (K + c_old)
     lam = dt * D / h**2
     b = rd3d.rhs(c_old, 20.0)
-    r0 = np.linalg.norm(rd3d.residual(b, c_old, sigma, lam, dt))
+    r0 = np.linalg.norm(b)
     out = {}
     for mode in ("fp64", "mixed"):
         with _solver(shape, h, D, dt, K) as s:
