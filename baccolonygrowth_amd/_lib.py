@@ -87,6 +87,13 @@ SIGNATURES.update(
         "bcg_rd3d_step": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, _ip, _dp]),
         "bcg_rd3d_vcycle": (ctypes.c_int, [_vp, ctypes.c_int32, _fp]),
         "bcg_rd3d_last_times": (ctypes.c_int, [_vp, _fp]),
+        "bcg_rd3d_set_state": (ctypes.c_int, [_vp, _dp, _dp]),
+        "bcg_rd3d_set_coeffs": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double]),
+        "bcg_rd3d_adjoint": (
+            ctypes.c_int,
+            [_vp, _dp, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, _dp, _dp, _dp, _dp],
+        ),
+        "bcg_rd3d_gather": (ctypes.c_int, [_vp, _dp, ctypes.c_int32, _dp, _dp]),
     }
 )
 

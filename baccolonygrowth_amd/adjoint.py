@@ -1,0 +1,125 @@
+"""Differentiable nutrient step: torch.autograd.Function over the C ABI.
+
+`nutrient_step(rates, D, K, c_old, solver, cm)` runs one implicit 3-D step
+(bcg_rd3d_step) with the uptake density deposited from per-cell rates at
+centres cm, and returns c_new. Its backward solves the adjoint system with
+the same multigrid (bcg_rd3d_adjoint) and returns gradients with respect to
+the per-cell rates (transpose of the deposition, bcg_rd3d_gather), D, K and
+c_old, so steps chain. `rollout(...)` applies several steps with the field
+checkpointed every k steps; the steps in between are recomputed in the
+backward pass (torch.utils.checkpoint).
+
+Tensors are float64 on the host; the solver keeps its state in HBM.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from ._lib import _dp, lib
+from .nutrients import MODES, Nutrients3D
+
+
+def _np(t: torch.Tensor) -> np.ndarray:
+    return np.ascontiguousarray(t.detach().cpu().numpy(), dtype=np.float64)
+
+
+def adjoint(solver: Nutrients3D, g: np.ndarray, mode: str = "fp64", tol: float = 1e-12, max_iter: int = 200):
+    """(dL/dq, dL/dc_old, dL/dK, dL/dD) of the solver's last step."""
+    g = solver._arr(g)
+    dq = np.empty(solver.shape)
+    dc = np.empty(solver.shape)
+    dK = ctypes.c_double()
+    dD = ctypes.c_double()
+    solver._check(
+        lib().bcg_rd3d_adjoint(
+            solver._h,
+            g.ctypes.data_as(_dp),
+            MODES[mode],
+            tol,
+            max_iter,
+            dq.ctypes.data_as(_dp),
+            dc.ctypes.data_as(_dp),
+            ctypes.byref(dK),
+            ctypes.byref(dD),
+        )
+    )
+    return dq, dc, dK.value, dD.value
+
+
+def gather(solver: Nutrients3D, cm: np.ndarray, field: np.ndarray) -> np.ndarray:
+    cm = np.ascontiguousarray(cm, dtype=np.float64).reshape(-1, 3)
+    field = solver._arr(field)
+    out = np.empty(len(cm))
+    solver._check(
+        lib().bcg_rd3d_gather(solver._h, cm.ctypes.data_as(_dp), len(cm), field.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
+    )
+    return out
+
+
+def set_state(solver: Nutrients3D, c_old: np.ndarray, c_new: np.ndarray) -> None:
+    a, b = solver._arr(c_old), solver._arr(c_new)
+    solver._check(lib().bcg_rd3d_set_state(solver._h, a.ctypes.data_as(_dp), b.ctypes.data_as(_dp)))
+
+
+def set_coeffs(solver: Nutrients3D, D: float, K: float) -> None:
+    solver._check(lib().bcg_rd3d_set_coeffs(solver._h, float(D), float(K)))
+
+
+class NutrientStepFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, rates, D, K, c_old, solver, cm, mode, tol):
+        set_coeffs(solver, D.item(), K.item())
+        solver.deposit(cm, _np(rates))
+        solver.set_field(_np(c_old))
+        solver.step(mode, tol=tol, max_iter=200)
+        c_new = torch.from_numpy(solver.field())
+        ctx.save_for_backward(rates, D, K, c_old, c_new)
+        ctx.solver, ctx.cm, ctx.mode, ctx.tol = solver, cm, mode, tol
+        return c_new
+
+    @staticmethod
+    def backward(ctx, g):
+        rates, D, K, c_old, c_new = ctx.saved_tensors
+        s = ctx.solver
+        # restore this step's state (another step may have run since)
+        set_coeffs(s, D.item(), K.item())
+        s.deposit(ctx.cm, _np(rates))
+        set_state(s, _np(c_old), _np(c_new))
+        dq, dc, dK, dD = adjoint(s, _np(g), ctx.mode, min(ctx.tol, 1e-12))
+        drates = torch.from_numpy(gather(s, ctx.cm, dq))
+        return (
+            drates,
+            torch.tensor(dD, dtype=torch.float64),
+            torch.tensor(dK, dtype=torch.float64),
+            torch.from_numpy(dc),
+            None,
+            None,
+            None,
+            None,
+        )
+
+
+def nutrient_step(rates, D, K, c_old, solver: Nutrients3D, cm, mode: str = "fp64", tol: float = 1e-12):
+    return NutrientStepFn.apply(rates, D, K, c_old, solver, cm, mode, tol)
+
+
+def rollout(rates, D, K, c0, solver: Nutrients3D, cm, steps: int, checkpoint_every: int = 5, mode="fp64", tol=1e-12):
+    """`steps` nutrient steps; only every `checkpoint_every`-th field is kept
+    for the backward pass, the others are recomputed from it."""
+    from torch.utils.checkpoint import checkpoint
+
+    def window(c, n):
+        for _ in range(n):
+            c = nutrient_step(rates, D, K, c, solver, cm, mode, tol)
+        return c
+
+    c = c0
+    done = 0
+    while done < steps:
+        n = min(checkpoint_every, steps - done)
+        c = checkpoint(window, c, n, use_reentrant=False)
+        done += n
+    return c

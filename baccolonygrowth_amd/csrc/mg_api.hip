@@ -33,6 +33,7 @@ namespace {
 struct Level {
     int nx, ny, nz;
     int cx = 0, cy = 0, cz = 0;  // coarsening to the next level
+    double hx, hy, hz;
     double lx, ly, lz;
     long n() const { return (long)nx * ny * nz; }
     double *u64 = nullptr, *f64 = nullptr, *a064 = nullptr, *r64 = nullptr;
@@ -153,7 +154,8 @@ struct bcg_rd3d {
     long N = 0;
     double *x[2] = {nullptr, nullptr};  // field ping-pong
     int cur = 0;
-    double *cprev = nullptr, *b = nullptr, *q = nullptr, *a0 = nullptr;
+    double *cprev = nullptr, *b = nullptr, *q = nullptr, *a0 = nullptr, *save = nullptr, *grad = nullptr;
+    double *gpart = nullptr, *dout = nullptr, *dfield = nullptr;
     float* a0f = nullptr;
     double* part = nullptr;  // per-block partial sums
     long npart = 0;
@@ -338,6 +340,9 @@ int bcg_rd3d_create(const bcg_rd3d_params* p, bcg_rd3d** out) {
         v.nx = nx;
         v.ny = ny;
         v.nz = nz;
+        v.hx = hx;
+        v.hy = hy;
+        v.hz = hz;
         v.lx = p->dt * p->D / (hx * hx);
         v.ly = p->dt * p->D / (hy * hy);
         v.lz = p->dt * p->D / (hz * hz);
@@ -363,6 +368,8 @@ int bcg_rd3d_create(const bcg_rd3d_params* p, bcg_rd3d** out) {
     A(&c->cprev, N);
     A(&c->b, N);
     A(&c->q, N);
+    A(&c->save, N);
+    A(&c->grad, N);
     for (size_t l = 0; l < c->L.size(); l++) {
         Level& v = c->L[l];
         const size_t n = (size_t)v.n();
@@ -381,6 +388,7 @@ int bcg_rd3d_create(const bcg_rd3d_params* p, bcg_rd3d** out) {
     }
     c->npart = nparts_of(c->L[0]);
     A(&c->part, (size_t)c->npart);
+    A(&c->gpart, 2 * (size_t)c->npart);
     A(&c->dnorm, 1);
     if (!rc && hipHostMalloc((void**)&c->hnorm, sizeof(double), hipHostMallocDefault) != hipSuccess) rc = BCG_ERR_HIP;
     for (int i = 0; i < 4 && !rc; i++)
@@ -403,7 +411,8 @@ int bcg_rd3d_create(const bcg_rd3d_params* p, bcg_rd3d** out) {
 int bcg_rd3d_destroy(bcg_rd3d* c) {
     if (!c) return BCG_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
-    std::vector<void*> ptrs = {c->x[0], c->x[1], c->cprev, c->b, c->q, c->part, c->dnorm, c->dkey, c->dcnt,
+    std::vector<void*> ptrs = {c->x[0], c->x[1], c->cprev, c->b, c->q, c->save, c->grad, c->gpart, c->dout, c->dfield,
+                               c->part, c->dnorm, c->dkey, c->dcnt,
                                c->dstart, c->dfill, c->dsorted, c->scan_tot, c->dfrac, c->dcm, c->drate};
     for (size_t l = 0; l < c->L.size(); l++) {
         Level& v = c->L[l];
@@ -534,16 +543,16 @@ int bcg_rd3d_deposit(bcg_rd3d* c, const double* cm, const double* rate, int32_t 
     return BCG_OK;
 }
 
-int bcg_rd3d_step(bcg_rd3d* c, int32_t mode, double tol, int32_t max_iter, int32_t* iters, double* relres) {
-    if (!c || (mode != 0 && mode != 1) || max_iter < 1 || !(tol > 0)) return BCG_ERR_ARG;
+}  // extern "C"
+
+namespace {
+
+// Solve A x = b (the current a0, b) from the initial guess x = c->x[c->cur]
+// to ||b - A x|| <= tol ||b||; the result is left in c->x[c->cur].
+int solve_core(bcg_rd3d* c, int mode, double tol, int max_iter, int* iters, double* relres) {
     Level& f = c->L[0];
     double* xo = c->x[c->cur];
-    // c_prev <- c ; a0, b from c_prev ; initial guess c_new = c_prev
-    HIPCHK(c, hipMemcpyAsync(c->cprev, xo, sizeof(double) * c->N, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    int rc = setup_system(c);
-    if (rc) return rc;
-    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    int rc = 0;
     int it = 0;
     double r0 = 0, rn = 0, bn = 0;
     const long np = c->npart;
@@ -554,8 +563,7 @@ int bcg_rd3d_step(bcg_rd3d* c, int32_t mode, double tol, int32_t max_iter, int32
         f.u64 = xo;
         hipLaunchKernelGGL(k_resnorm64, grid2(f), dim3(64, 4), 0, c->stream, xo, c->b, f.a064, f.nx, f.ny, f.nz, f.lx,
                            f.ly, f.lz, c->part);
-        if ((rc = read_norm(c, np, &r0))) return rc;
-        rn = r0;
+        if ((rc = read_norm(c, np, &rn))) return rc;
         r0 = bn;
         while (it < max_iter && rn > tol * r0) {
             vcycle<double>(c, 0);
@@ -566,13 +574,12 @@ int bcg_rd3d_step(bcg_rd3d* c, int32_t mode, double tol, int32_t max_iter, int32
         }
         f.u64 = nullptr;
     } else {
-        // r0 = b - A x0 (x copied into the other buffer), r32 = r0
+        // r = b - A x0 (x copied into the other buffer), r32 = r
         double* xn = c->x[c->cur ^ 1];
         hipLaunchKernelGGL(k_ir_update_residual, grid2(f), dim3(64, 4), 0, c->stream, xo, xn, (const float*)nullptr, 0.0,
                            c->b, f.a064, f.f32, 1.0, f.nx, f.ny, f.nz, f.lx, f.ly, f.lz, c->part);
-        if ((rc = read_norm(c, np, &r0))) return rc;
+        if ((rc = read_norm(c, np, &rn))) return rc;
         c->cur ^= 1;
-        rn = r0;
         r0 = bn;
         double scale = 1.0;  // r32 = r * scale
         while (it < max_iter && rn > tol * r0) {
@@ -592,15 +599,136 @@ int bcg_rd3d_step(bcg_rd3d* c, int32_t mode, double tol, int32_t max_iter, int32
             it++;
         }
     }
-    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
     HIPCHK(c, hipGetLastError());
+    if (iters) *iters = it;
+    if (relres) *relres = r0 > 0 ? rn / r0 : 0.0;
+    if (rn > tol * r0)
+        return fail(c, BCG_ERR_STATE, "no convergence: relative residual %.3e after %d iterations", r0 > 0 ? rn / r0 : 0.0,
+                    it);
+    return 0;
+}
+
+int upload(bcg_rd3d* c, double** dev, size_t n, const double* host) {
+    if (!*dev) {
+        int rc = dalloc(c, dev, n);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipMemcpyAsync(*dev, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bcg_rd3d_step(bcg_rd3d* c, int32_t mode, double tol, int32_t max_iter, int32_t* iters, double* relres) {
+    if (!c || (mode != 0 && mode != 1) || max_iter < 1 || !(tol > 0)) return BCG_ERR_ARG;
+    // c_prev <- c ; a0, b from c_prev ; initial guess c_new = c_prev
+    HIPCHK(c, hipMemcpyAsync(c->cprev, c->x[c->cur], sizeof(double) * c->N, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    int rc = setup_system(c);
+    if (rc) return rc;
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    rc = solve_core(c, mode, tol, max_iter, iters, relres);
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     hipEventElapsedTime(&c->last_ms[0], c->ev[0], c->ev[1]);
     hipEventElapsedTime(&c->last_ms[1], c->ev[1], c->ev[2]);
-    if (iters) *iters = it;
-    if (relres) *relres = r0 > 0 ? rn / r0 : 0.0;
-    if (rn > tol * r0) return fail(c, BCG_ERR_STATE, "no convergence: relative residual %.3e after %d iterations",
-                                   r0 > 0 ? rn / r0 : 0.0, it);
+    return rc;
+}
+
+int bcg_rd3d_set_state(bcg_rd3d* c, const double* c_old, const double* c_new) {
+    if (!c || !c_old || !c_new) return BCG_ERR_ARG;
+    HIPCHK(c, hipMemcpyAsync(c->cprev, c_old, sizeof(double) * c->N, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->x[c->cur], c_new, sizeof(double) * c->N, hipMemcpyHostToDevice, c->stream));
+    int rc = setup_system(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return BCG_OK;
+}
+
+int bcg_rd3d_set_coeffs(bcg_rd3d* c, double D, double K) {
+    if (!c || !(D >= 0) || !(K > 0)) return BCG_ERR_ARG;
+    c->prm.D = D;
+    c->prm.K = K;
+    for (Level& v : c->L) {
+        v.lx = c->prm.dt * D / (v.hx * v.hx);
+        v.ly = c->prm.dt * D / (v.hy * v.hy);
+        v.lz = c->prm.dt * D / (v.hz * v.hz);
+    }
+    return BCG_OK;
+}
+
+int bcg_rd3d_adjoint(bcg_rd3d* c, const double* g, int32_t mode, double tol, int32_t max_iter, double* dL_dq,
+                     double* dL_dcold, double* dL_dK, double* dL_dD) {
+    if (!c || !g || (mode != 0 && mode != 1) || max_iter < 1 || !(tol > 0)) return BCG_ERR_ARG;
+    Level& f = c->L[0];
+    const size_t N = (size_t)c->N;
+    int rc = upload(c, &c->dfield, N, g);
+    if (rc) return rc;
+    // keep c_new; solve A mu = W^-1 g with zero Dirichlet rows from mu = 0
+    HIPCHK(c, hipMemcpyAsync(c->save, c->x[c->cur], N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL(k_adj_rhs, grid2(f), dim3(64, 4), 0, c->stream, c->dfield, c->b, f.nx, f.ny, f.nz);
+    hipLaunchKernelGGL((k_fill<double>), dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, c->x[c->cur], c->N, 0.0);
+    int it = 0;
+    double rr = 0;
+    rc = solve_core(c, mode, tol, max_iter, &it, &rr);
+    if (rc) return rc;
+    const double* mu = c->x[c->cur];
+    const double h = c->prm.h;
+    hipLaunchKernelGGL(k_adj_grad, grid2(f), dim3(64, 4), 0, c->stream, mu, c->save, c->cprev, c->q, c->prm.K, c->prm.dt,
+                       1.0 / (h * h), f.nx, f.ny, f.nz, c->grad, c->dfield, c->gpart, c->gpart + c->npart);
+    HIPCHK(c, hipGetLastError());
+    if (dL_dq) HIPCHK(c, hipMemcpyAsync(dL_dq, c->grad, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (dL_dcold) HIPCHK(c, hipMemcpyAsync(dL_dcold, c->dfield, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    double gk = 0, gd = 0;
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, c->gpart, (int)c->npart, c->dnorm);
+    HIPCHK(c, hipMemcpyAsync(c->hnorm, c->dnorm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    gk = *c->hnorm;
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, c->gpart + c->npart, (int)c->npart, c->dnorm);
+    HIPCHK(c, hipMemcpyAsync(c->hnorm, c->dnorm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    gd = *c->hnorm;
+    if (dL_dK) *dL_dK = gk;
+    if (dL_dD) *dL_dD = gd;
+    // restore the forward state: c_new and b
+    HIPCHK(c, hipMemcpyAsync(c->x[c->cur], c->save, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL(k_rhs, dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, c->cprev, c->b, c->prm.c_res,
+                       (long)f.nx * f.ny, c->N);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return BCG_OK;
+}
+
+int bcg_rd3d_gather(bcg_rd3d* c, const double* cm, int32_t n, const double* field, double* out) {
+    if (!c || n < 0 || (n > 0 && (!cm || !field || !out))) return BCG_ERR_ARG;
+    if (n == 0) return BCG_OK;
+    Level& f = c->L[0];
+    int rc = upload(c, &c->dfield, (size_t)c->N, field);
+    if (rc) return rc;
+    if (n > c->dcap) {
+        for (void* q : {(void*)c->dkey, (void*)c->dsorted, (void*)c->dfrac, (void*)c->dcm, (void*)c->drate})
+            if (q) hipFree(q);
+        c->dkey = c->dsorted = nullptr;
+        c->dfrac = c->dcm = c->drate = nullptr;
+        auto A = [&](auto** ptr, size_t m) {
+            if (!rc) rc = dalloc(c, ptr, m);
+        };
+        A(&c->dkey, n);
+        A(&c->dsorted, n);
+        A(&c->dfrac, 3 * (size_t)n);
+        A(&c->dcm, 3 * (size_t)n);
+        A(&c->drate, n);
+        if (rc) return rc;
+        c->dcap = n;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->dcm, cm, 3 * sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_cic_gather, dim3(nblk(n, 256)), dim3(256), 0, c->stream, c->dcm, n, c->prm.origin[0],
+                       c->prm.origin[1], c->prm.origin[2], 1.0 / c->prm.h, f.nx, f.ny, f.nz, c->dfield,
+                       1.0 / (c->prm.h * c->prm.h * c->prm.h), c->drate);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(out, c->drate, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return BCG_OK;
 }
 

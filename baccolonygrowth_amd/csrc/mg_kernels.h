@@ -31,5 +31,12 @@ template <class T>
 __global__ void k_restrict_a0(const T* a0f, T* a0c, int nx, int ny, int nz, int ncx, int ncy, int ncz, int cx, int cy,
                               int cz);
 
+__global__ void k_adj_rhs(const double* g, double* b, int nx, int ny, int nz);
+__global__ void k_adj_grad(const double* mu, const double* c1, const double* c0, const double* q, double K, double dt,
+                           double inv_h2, int nx, int ny, int nz, double* dq, double* dc0, double* partK,
+                           double* partD);
+__global__ void k_cic_gather(const double* cm, int n, double ox, double oy, double oz, double inv_h, int nx, int ny,
+                             int nz, const double* field, double inv_vol, double* out);
+
 }  // namespace mg
 }  // namespace bcg

@@ -295,6 +295,104 @@ __global__ void __launch_bounds__(256) k_restrict_a0(const T* __restrict__ a0f, 
     a0c[((long)K * ncy + J) * ncx + I] = T(1) + acc / wsum;
 }
 
+
+// ------------------------------------------------------------- adjoint
+// Control-volume weight of a vertex: 1/2 per Neumann face it lies on
+// (x = 0, nx-1; y = 0, ny-1; z = nz-1). W A is symmetric on the free rows.
+__device__ __forceinline__ double cv_weight(int i, int j, int k, int nx, int ny, int nz) {
+    double w = 1.0;
+    if (i == 0 || i == nx - 1) w *= 0.5;
+    if (j == 0 || j == ny - 1) w *= 0.5;
+    if (k == nz - 1) w *= 0.5;
+    return w;
+}
+
+// rhs of the adjoint solve: W^-1 g on the free rows, 0 on the Dirichlet rows.
+__global__ void __launch_bounds__(256) k_adj_rhs(const double* __restrict__ g, double* __restrict__ b, int nx, int ny,
+                                                 int nz) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= nx || j >= ny) return;
+    const long p = ((long)k * ny + j) * nx + i;
+    b[p] = (k == 0) ? 0.0 : g[p] / cv_weight(i, j, k, nx, ny, nz);
+}
+
+// Parameter gradients from mu (A mu = W^-1 g), lambda = W mu on free rows:
+//   dL/da0   = -lambda c1
+//   dL/dq    = dL/da0 * dt/(K + c0)
+//   dL/dc0   = lambda + dL/da0 * (-dt q/(K + c0)^2)
+//   dL/dK    = sum dL/da0 * (-dt q/(K + c0)^2)          (block partials)
+//   dL/dD    = sum lambda (Lap_h c1) dt/h^2             (block partials)
+__global__ void __launch_bounds__(256) k_adj_grad(const double* __restrict__ mu, const double* __restrict__ c1,
+                                                  const double* __restrict__ c0, const double* __restrict__ q, double K,
+                                                  double dt, double inv_h2, int nx, int ny, int nz,
+                                                  double* __restrict__ dq, double* __restrict__ dc0,
+                                                  double* __restrict__ partK, double* __restrict__ partD) {
+    __shared__ double ldsK[4], ldsD[4];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    double gK = 0, gD = 0;
+    if (i < nx && j < ny) {
+        const long sxy = (long)nx * ny;
+        const long p = (long)k * sxy + (long)j * nx + i;
+        if (k == 0) {
+            if (dq) dq[p] = 0.0;
+            if (dc0) dc0[p] = 0.0;
+        } else {
+            const double lam = cv_weight(i, j, k, nx, ny, nz) * mu[p];
+            const double u = c1[p];
+            const double den = K + c0[p];
+            const double ga0 = -lam * u;
+            const double da0_dc = -dt * q[p] / (den * den);
+            if (dq) dq[p] = ga0 * dt / den;
+            if (dc0) dc0[p] = lam + ga0 * da0_dc;
+            gK = ga0 * da0_dc;
+            const double lap = (c1[p + (i > 0 ? -1 : 1)] + c1[p + (i < nx - 1 ? 1 : -1)]) +
+                               (c1[p + (j > 0 ? -nx : nx)] + c1[p + (j < ny - 1 ? nx : -nx)]) +
+                               (c1[p - sxy] + c1[p + (k < nz - 1 ? sxy : -sxy)]) - 6.0 * u;
+            gD = lam * lap * dt * inv_h2;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        gK += __shfl_xor(gK, o, 64);
+        gD += __shfl_xor(gD, o, 64);
+    }
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    if ((tid & 63) == 0) {
+        ldsK[tid >> 6] = gK;
+        ldsD[tid >> 6] = gD;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const long bid = ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        partK[bid] = (ldsK[0] + ldsK[1]) + (ldsK[2] + ldsK[3]);
+        partD[bid] = (ldsD[0] + ldsD[1]) + (ldsD[2] + ldsD[3]);
+    }
+}
+
+// Transpose of the cloud-in-cell deposition: out[c] = sum_v field[v] w_cv / h^3.
+__global__ void k_cic_gather(const double* __restrict__ cm, int n, double ox, double oy, double oz, double inv_h, int nx,
+                             int ny, int nz, const double* __restrict__ field, double inv_vol, double* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const double gx = (cm[3 * c] - ox) * inv_h, gy = (cm[3 * c + 1] - oy) * inv_h, gz = (cm[3 * c + 2] - oz) * inv_h;
+    const double fx = floor(gx), fy = floor(gy), fz = floor(gz);
+    double acc = 0.0;
+    if (fx >= 0 && fx < nx - 1 && fy >= 0 && fy < ny - 1 && fz >= 0 && fz < nz - 1) {
+        const double ax = gx - fx, ay = gy - fy, az = gz - fz;
+        const long b = ((long)fz * ny + (long)fy) * nx + (long)fx;
+        for (int dz = 0; dz <= 1; dz++)
+            for (int dy = 0; dy <= 1; dy++)
+                for (int dx = 0; dx <= 1; dx++) {
+                    const double w = (dx ? ax : 1.0 - ax) * (dy ? ay : 1.0 - ay) * (dz ? az : 1.0 - az);
+                    acc += field[b + (long)dz * nx * ny + (long)dy * nx + dx] * w;
+                }
+    }
+    out[c] = acc * inv_vol;
+}
+
 // ---------------------------------------------------- explicit instances
 template __global__ void k_rbgs<float>(float*, const float*, const float*, int, int, int, float, float, float, int);
 template __global__ void k_rbgs<double>(double*, const double*, const double*, int, int, int, double, double, double,

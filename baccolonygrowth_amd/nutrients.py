@@ -108,10 +108,16 @@ class Nutrients3D:
         )
         return inside.value
 
-    def step(self, mode: str = "mixed", tol: float = 1e-10, max_iter: int = 100):
+    def step(self, mode: str = "mixed", tol: float = 1e-10, max_iter: int = 100, strict: bool = True):
+        """One implicit step; returns (iterations, ||b - A c|| / ||b||).
+        strict=False keeps the last iterate when tol is not reached (for
+        solves run to the round-off floor)."""
         it = ctypes.c_int32()
         rr = ctypes.c_double()
-        self._check(lib().bcg_rd3d_step(self._h, MODES[mode], tol, max_iter, ctypes.byref(it), ctypes.byref(rr)))
+        rc = lib().bcg_rd3d_step(self._h, MODES[mode], tol, max_iter, ctypes.byref(it), ctypes.byref(rr))
+        if rc == -7 and not strict:
+            return it.value, rr.value
+        self._check(rc)
         return it.value, rr.value
 
     def vcycle(self, mode: str = "fp64") -> float:

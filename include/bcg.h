@@ -225,6 +225,21 @@ int bcg_rd3d_deposit(bcg_rd3d* ctx, const double* cm, const double* rate, int32_
  * *iters = V-cycles (mode 0) or refinement steps (mode 1); *relres the
  * final ||b - A c|| / ||b||. BCG_ERR_STATE if tol is not reached.          */
 int bcg_rd3d_step(bcg_rd3d* ctx, int32_t mode, double tol, int32_t max_iter, int32_t* iters, double* relres);
+/* Make (c_old, c_new) the state of the last step without solving: the
+ * adjoint of a recorded step is then available (checkpoint + recompute).  */
+int bcg_rd3d_set_state(bcg_rd3d* ctx, const double* c_old, const double* c_new);
+/* Change the diffusivity D and the Monod constant K. */
+int bcg_rd3d_set_coeffs(bcg_rd3d* ctx, double D, double K);
+/* Adjoint of the last step: given g = dL/dc_new, solves A^T lambda = g (as
+ * A mu = W^-1 g, lambda = W mu, W the control-volume weights that make W A
+ * symmetric) by the same multigrid, and returns (each may be NULL)
+ *   dL_dq[N]    gradient w.r.t. the uptake density q
+ *   dL_dcold[N] gradient w.r.t. c_old (chains steps backwards)
+ *   *dL_dK, *dL_dD                                                          */
+int bcg_rd3d_adjoint(bcg_rd3d* ctx, const double* g, int32_t mode, double tol, int32_t max_iter,
+                     double* dL_dq, double* dL_dcold, double* dL_dK, double* dL_dD);
+/* Transpose of bcg_rd3d_deposit: out[i] = sum_v field[v] w_iv / h^3.     */
+int bcg_rd3d_gather(bcg_rd3d* ctx, const double* cm, int32_t n, const double* field, double* out);
 /* One V-cycle of the current step's system (timing); *ms = its duration.  */
 int bcg_rd3d_vcycle(bcg_rd3d* ctx, int32_t mode, float* ms);
 /* device times of the last step, ms: [0] setup (a0, rhs, coarse a0), [1] solve */

@@ -127,3 +127,32 @@ def deposit_cic(cm: np.ndarray, q: np.ndarray, shape, h: float, origin) -> np.nd
                 wx = f[:, 0] if dx else 1 - f[:, 0]
                 np.add.at(out, (i0[:, 2] + dz, i0[:, 1] + dy, i0[:, 0] + dx), qq * wx * wy * wz)
     return out / h**3, inside
+
+
+def step_direct(c_old, q, K, D, dt, h, c_res):
+    """One implicit step by direct solve: returns c_new."""
+    sigma = q / (K + c_old)
+    return solve_direct(rhs(c_old, c_res), sigma, dt * D / h**2, dt)
+
+
+def adjoint_direct(c_old, c_new, q, K, D, dt, h, g):
+    """Gradients of L with dL/dc_new = g through one step, from the sparse
+    transpose solve A^T lam = g (no symmetrisation used):
+    returns dL/dq, dL/dc_old, dL/dK, dL/dD."""
+    from scipy.sparse.linalg import spsolve
+
+    shape = c_old.shape
+    sigma = q / (K + c_old)
+    lam_c = dt * D / h**2
+    A = sparse_A(shape, sigma, lam_c, dt)
+    lam = spsolve(A.T.tocsc(), g.ravel()).reshape(shape)
+    free = np.ones(shape, bool)
+    free[0] = False
+    den = K + c_old
+    ga0 = np.where(free, -lam * c_new, 0.0)
+    da0_dc = -dt * q / den**2
+    dq = ga0 * dt / den
+    dc0 = np.where(free, lam + ga0 * da0_dc, 0.0)
+    dK = float((ga0 * da0_dc).sum())
+    dD = float((np.where(free, lam, 0.0) * laplacian(c_new)).sum() * dt / h**2)
+    return dq, dc0, dK, dD

@@ -96,7 +96,7 @@ def test_step_matches_direct_solve(mode, shape):
     err = np.linalg.norm(c - ref) / np.linalg.norm(ref)
     print(f"    iters {it}, reported {rr:.2e}, numpy residual {res:.2e}, error vs direct {err:.2e}", flush=True)
     assert res <= 1.05e-10
-    assert abs(res - rr) <= 1e-3 * rr + 1e-14
+    assert abs(res - rr) <= 0.05 * rr + 1e-13  # both near the FP64 evaluation floor
     assert err <= 1e-9
     assert it < 40
 
@@ -115,9 +115,12 @@ def test_mixed_equals_fp64_and_deterministic():
         if mode in outs:
             assert c.tobytes() == outs[mode].tobytes()  # bitwise repeatable
         outs[mode] = c
-    d = np.abs(outs["mixed"] - outs["fp64"]).max() / np.abs(outs["fp64"]).max()
-    print(f"[gpu] mixed vs fp64 max rel difference {d:.2e}", flush=True)
-    assert d <= 1e-9
+    # ||A^-1|| <= 1 here (A = I + dt*sigma - dt*D*Lap_h with sigma >= 0), so two
+    # iterates with residuals <= tol ||b|| differ by at most ~2 tol ||b||
+    b = rd3d.rhs(c_old, 20.0)
+    d = np.linalg.norm(outs["mixed"] - outs["fp64"]) / np.linalg.norm(b)
+    print(f"[gpu] mixed vs fp64 relative difference {d:.2e}", flush=True)
+    assert d <= 4e-10
 
 
 @pytest.mark.gpu
@@ -163,5 +166,174 @@ def test_full_size_257_converges_both_modes():
         res = np.linalg.norm(rd3d.residual(b, out[mode], sigma, lam, dt)) / r0
         print(f"    {mode}: {it} iterations, residual {res:.2e}", flush=True)
         assert res <= 1.05e-10
-    d = np.abs(out["mixed"] - out["fp64"]).max() / 20.0
-    assert d <= 1e-9
+    d = np.linalg.norm(out["mixed"] - out["fp64"]) / np.linalg.norm(b)
+    print(f"    mixed vs fp64 relative difference {d:.2e}", flush=True)
+    assert d <= 4e-10
+
+
+# ----------------------------------------------------------- adjoint (CPU)
+def _loss_parts(shape, seed=11):
+    rng = np.random.default_rng(seed)
+    c_old, q, _ = _case(shape, seed=seed)
+    target = 20.0 * (1 - 0.3 * rng.random(shape))
+    return c_old, q, target
+
+
+def test_oracle_adjoint_matches_finite_differences():
+    """adjoint_direct (sparse transpose solve) against central differences of
+    the direct step: dL/dK, dL/dD, dL/dq and dL/dc_old at sample vertices."""
+    shape = (5, 6, 7)
+    h, D, dt, K = 1.0, 3.0, 10.0, 0.1
+    c_old, q, target = _loss_parts(shape)
+
+    def L(c_old=c_old, q=q, K=K, D=D):
+        c = rd3d.step_direct(c_old, q, K, D, dt, h, 20.0)
+        return float(((c - target) ** 2).sum())
+
+    c1 = rd3d.step_direct(c_old, q, K, D, dt, h, 20.0)
+    dq, dc0, dK, dD = rd3d.adjoint_direct(c_old, c1, q, K, D, dt, h, 2 * (c1 - target))
+    e = 1e-3  # smaller steps are round-off limited on this tiny grid (truncation here ~e^2)
+    assert dK == pytest.approx((L(K=K * (1 + e)) - L(K=K * (1 - e))) / (2 * e * K), rel=1e-4)
+    assert dD == pytest.approx((L(D=D * (1 + e)) - L(D=D * (1 - e))) / (2 * e * D), rel=1e-4)
+    for idx in [(1, 2, 3), (2, 3, 4), (3, 1, 2), (4, 5, 6)]:
+        for arr, grad in ((q, dq), (c_old, dc0)):
+            a1, a2 = arr.copy(), arr.copy()
+            step = e * max(abs(arr[idx]), 1e-3)
+            a1[idx] += step
+            a2[idx] -= step
+            kw1 = {"q": a1} if arr is q else {"c_old": a1}
+            kw2 = {"q": a2} if arr is q else {"c_old": a2}
+            fd = (L(**kw1) - L(**kw2)) / (2 * step)
+            assert grad[idx] == pytest.approx(fd, rel=1e-5, abs=1e-9)
+
+
+# ----------------------------------------------------------- adjoint (GPU)
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_adjoint_matches_oracle(mode):
+    from baccolonygrowth_amd import adjoint as adj
+
+    shape = (9, 17, 17)
+    h, D, dt, K = 1.0, 20.0, 10.0, 0.1
+    c_old, q, target = _loss_parts(shape, seed=3)
+    c1_ref = rd3d.step_direct(c_old, q, K, D, dt, h, 20.0)
+    g = 2 * (c1_ref - target)
+    ref = rd3d.adjoint_direct(c_old, c1_ref, q, K, D, dt, h, g)
+    print(f"[gpu] adjoint {mode} {shape}", flush=True)
+    with _solver(shape, h, D, dt, K) as s:
+        s.set_field(c_old)
+        s.set_uptake(q)
+        s.step(mode, tol=1e-13, max_iter=200)
+        c1 = s.field()
+        dq, dc0, dK, dD = adj.adjoint(s, g, mode, tol=1e-13)
+        c1_after = s.field()
+    assert np.abs(c1 - c1_ref).max() <= 1e-10 * 20
+    assert c1_after.tobytes() == c1.tobytes()  # forward state restored
+    scale = np.abs(ref[0]).max()
+    assert np.abs(dq - ref[0]).max() <= 1e-8 * scale
+    assert np.abs(dc0 - ref[1]).max() <= 1e-8 * np.abs(ref[1]).max()
+    assert dK == pytest.approx(ref[2], rel=1e-8)
+    assert dD == pytest.approx(ref[3], rel=1e-8)
+
+
+@pytest.mark.gpu
+def test_autograd_rates_D_K_vs_finite_differences():
+    """Gradients of a radial-profile loss with respect to per-cell rates, D
+    and K through torch.autograd, against central differences (h = 1e-6
+    relative) of GPU FP64 solves."""
+    import torch
+
+    from baccolonygrowth_amd.adjoint import nutrient_step
+
+    shape = (9, 33, 33)
+    h, dt = 2.0, 10.0
+    rng = np.random.default_rng(4)
+    n = 3000
+    cm = np.column_stack([rng.uniform(8, 56, n), rng.uniform(8, 56, n), rng.uniform(0.5, 8, n)])
+    base = np.exp(rng.normal(0, 0.3, n)) * 2.0
+    zz, yy, xx = np.meshgrid(np.arange(shape[0]) * h, np.arange(shape[1]) * h, np.arange(shape[2]) * h, indexing="ij")
+    r = np.hypot(xx - 32, yy - 32)
+    target = torch.from_numpy(20.0 * (1 - 0.8 * np.exp(-r / 150.0)))
+    c0 = torch.full(shape, 20.0, dtype=torch.float64)
+    with _solver(shape, h, 600.0, dt, 0.1) as s:
+        from baccolonygrowth_amd.adjoint import set_coeffs
+
+        def loss(rates, D, K):
+            c = nutrient_step(rates, D, K, c0, s, cm, "fp64", 1e-13)
+            return ((c - target) ** 2).sum()
+
+        def loss_fd(rates_np, D, K):
+            # FP64 solve run to the round-off floor (fixed V-cycle count)
+            set_coeffs(s, D, K)
+            s.deposit(cm, rates_np)
+            s.set_field(c0.numpy())
+            s.step("fp64", tol=1e-16, max_iter=40, strict=False)
+            return float(((s.field() - target.numpy()) ** 2).sum())
+
+        rates = torch.tensor(base, requires_grad=True)
+        D = torch.tensor(600.0, dtype=torch.float64, requires_grad=True)
+        K = torch.tensor(0.1, dtype=torch.float64, requires_grad=True)
+        L0 = loss(rates, D, K)
+        L0.backward()
+        e = 1e-6
+        for name, grad in (("D", D.grad.item()), ("K", K.grad.item())):
+            v = {"D": 600.0, "K": 0.1}
+            vp, vm = dict(v), dict(v)
+            vp[name] *= 1 + e
+            vm[name] *= 1 - e
+            fd = (loss_fd(base, **vp) - loss_fd(base, **vm)) / (2 * e * v[name])
+            print(f"    d/d{name}: adjoint {grad:.8e}  fd {fd:.8e}", flush=True)
+            assert grad == pytest.approx(fd, rel=1e-4)
+        for i in (0, 17, 1234):
+            rp, rm = base.copy(), base.copy()
+            rp[i] *= 1 + e
+            rm[i] *= 1 - e
+            fd = (loss_fd(rp, 600.0, 0.1) - loss_fd(rm, 600.0, 0.1)) / (2 * e * base[i])
+            g = rates.grad[i].item()
+            print(f"    d/drate[{i}]: adjoint {g:.8e}  fd {fd:.8e}", flush=True)
+            assert g == pytest.approx(fd, rel=1e-3, abs=1e-6 * np.abs(rates.grad.numpy()).max())
+
+
+@pytest.mark.gpu
+def test_rollout_checkpointed_equals_direct_chain():
+    """4 steps with checkpoints every 2 (recompute in backward) give the
+    same gradients as the oracle's step-by-step adjoint chain."""
+    import torch
+
+    from baccolonygrowth_amd.adjoint import rollout
+
+    shape = (5, 9, 9)
+    h, D, dt, K = 1.0, 5.0, 10.0, 0.1
+    rng = np.random.default_rng(8)
+    n = 200
+    cm = np.column_stack([rng.uniform(1, 7, n), rng.uniform(1, 7, n), rng.uniform(0.5, 3.5, n)])
+    base = rng.random(n) * 0.5
+    target = 20.0 * (1 - 0.2 * rng.random(shape))
+    c0 = np.full(shape, 20.0)
+    # oracle chain
+    q, _ = rd3d.deposit_cic(cm, base, shape, h, (0, 0, 0))
+    cs = [c0]
+    for _ in range(4):
+        cs.append(rd3d.step_direct(cs[-1], q, K, D, dt, h, 20.0))
+    g = 2 * (cs[-1] - target)
+    dq_tot = np.zeros(shape)
+    dK_tot = dD_tot = 0.0
+    for k in range(4, 0, -1):
+        dq, dc0, dK, dD = rd3d.adjoint_direct(cs[k - 1], cs[k], q, K, D, dt, h, g)
+        dq_tot += dq
+        dK_tot += dK
+        dD_tot += dD
+        g = dc0
+    with _solver(shape, h, D, dt, K) as s:
+        from baccolonygrowth_amd.adjoint import gather
+
+        rates = torch.tensor(base, requires_grad=True)
+        Dt = torch.tensor(D, dtype=torch.float64, requires_grad=True)
+        Kt = torch.tensor(K, dtype=torch.float64, requires_grad=True)
+        c = rollout(rates, Dt, Kt, torch.from_numpy(c0), s, cm, steps=4, checkpoint_every=2)
+        ((c - torch.from_numpy(target)) ** 2).sum().backward()
+        drates_ref = gather(s, cm, dq_tot)
+    assert np.abs(c.detach().numpy() - cs[-1]).max() <= 1e-9
+    assert Kt.grad.item() == pytest.approx(dK_tot, rel=1e-7)
+    assert Dt.grad.item() == pytest.approx(dD_tot, rel=1e-7)
+    assert np.abs(rates.grad.numpy() - drates_ref).max() <= 1e-7 * np.abs(drates_ref).max()
