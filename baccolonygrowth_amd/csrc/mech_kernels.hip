@@ -188,26 +188,39 @@ __device__ __forceinline__ int gather_one(const PhysParams& P, const double* __r
     const int zlo = max(z0 - 1, 0), zhi = min(z0 + 2, P.BoxZ);
     const int ylo = max(y0 - 1, 0), yhi = min(y0 + 2, P.BoxY);
     const int xlo = max(x0 - 1, 0), xhi = min(x0 + 2, P.BoxX);
-    for (int za = zlo; za < zhi; za++)
-        for (int ya = ylo; ya < yhi; ya++)
-            for (int xa = xlo; xa < xhi; xa++) {
-                const int a = xa * (P.BoxY * P.BoxZ) + ya * P.BoxZ + za;
-                const int e = start[a + 1];
-                for (int k = start[a]; k < e; k++) {
-                    const float4 c = cand[k];
-                    const float dx = c.x - cx, dy = c.y - cy, dz = c.z - cz;
-                    const float rr = reach + c.w;
-                    if (dx * dx + dy * dy + dz * dz > rr * rr) continue;
-                    const int j = sorted[k];
-                    if (j == id) continue;
-                    Seg B;
-                    load_seg(hot, j, B);
-                    if (keep_neighbour(A, cmA, B, P.Rc)) {
-                        if (m < cap) nl[(size_t)m * stride + slot] = j;
-                        m++;
-                    }
-                }
+    // The 27-box set (the list is a set; its order is free): bins adjacent in
+    // z are adjacent in the sorted order, so each (x, y) column is one range.
+    auto test = [&](int k) {
+        const int j = sorted[k];
+        if (j == id) return;
+        Seg B;
+        load_seg(hot, j, B);
+        if (keep_neighbour(A, cmA, B, P.Rc)) {
+            if (m < cap) nl[(size_t)m * stride + slot] = j;
+            m++;
+        }
+    };
+    auto near = [&](const float4& c) {
+        const float dx = c.x - cx, dy = c.y - cy, dz = c.z - cz;
+        const float rr = reach + c.w;
+        return dx * dx + dy * dy + dz * dz <= rr * rr;
+    };
+    for (int xa = xlo; xa < xhi; xa++)
+        for (int ya = ylo; ya < yhi; ya++) {
+            const int a = xa * (P.BoxY * P.BoxZ) + ya * P.BoxZ;
+            const int k1 = start[a + zhi];
+            int k = start[a + zlo];
+            for (; k + 4 <= k1; k += 4) {
+                const float4 c0 = cand[k], c1 = cand[k + 1], c2 = cand[k + 2], c3 = cand[k + 3];
+                const bool n0 = near(c0), n1 = near(c1), n2 = near(c2), n3 = near(c3);
+                if (n0) test(k);
+                if (n1) test(k + 1);
+                if (n2) test(k + 2);
+                if (n3) test(k + 3);
             }
+            for (; k < k1; k++)
+                if (near(cand[k])) test(k);
+        }
     if (m > cap) {
         atomicOr(err, ERRBIT_NLIST);
         m = cap;
@@ -255,11 +268,20 @@ __device__ __forceinline__ void sum_forces(const PhysParams& P, const Seg& S, co
     F = mk(0, 0, 0);
     T = mk(0, 0, 0);
     const d3 cm = cm_of(S);
+    // the next neighbour's record is requested before this one is used
+    int j = cnt > 0 ? nl[slot] : 0;
+    Seg B;
+    if (cnt > 0) load_seg(hot, j, B);
     for (int k = 0; k < cnt; k++) {
-        const int j = nl[(size_t)k * stride + slot];
-        Seg B;
-        load_seg(hot, j, B);
+        int jn = j;
+        Seg Bn = B;
+        if (k + 1 < cnt) {
+            jn = nl[(size_t)(k + 1) * stride + slot];
+            load_seg(hot, jn, Bn);
+        }
         F_cc_acc(P, S, K, cm, B, hot, j, F, T);
+        j = jn;
+        B = Bn;
     }
     sf = mk(0, 0, 0);
     df = mk(0, 0, 0);

@@ -159,6 +159,9 @@ struct bcg_rd3d {
     float* a0f = nullptr;
     double* part = nullptr;  // per-block partial sums
     long npart = 0;
+    double* part2 = nullptr;  // second-stage partial sums
+    int bottom = 0;           // first level handled by the single-workgroup bottom solver
+    double *bot_u0 = nullptr, *bot_f0 = nullptr;  // FP64 level-0 copies when that level is the bottom
     double* dnorm = nullptr;
     double* hnorm = nullptr;  // pinned
     // deposition scratch
@@ -226,26 +229,37 @@ void smooth(bcg_rd3d* c, Level& v, int sweeps) {
                                Arr<T>::a0(v), v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly, (T)v.lz, color);
 }
 
-void coarse_solve(bcg_rd3d* c, Level& v) {
-    hipLaunchKernelGGL((k_fill<double>), dim3(nblk(v.n(), 256)), dim3(256), 0, c->stream, v.u64, v.n(), 0.0);
-    smooth<double>(c, v, c->prm.coarse_sweeps);
+void bottom_solve(bcg_rd3d* c, double* u0, double* f0) {
+    BotLevels B;
+    B.n = (int)c->L.size() - c->bottom;
+    for (int i = 0; i < B.n; i++) {
+        Level& v = c->L[c->bottom + i];
+        B.L[i] = BotLevel{v.u64, v.f64, v.a064, v.r64, v.nx, v.ny, v.nz, v.cx, v.cy, v.cz, v.lx, v.ly, v.lz};
+    }
+    B.L[0].u = u0;
+    B.L[0].f = f0;
+    hipLaunchKernelGGL(k_bottom, dim3(1), dim3(1024), 0, c->stream, B, c->prm.nu1, c->prm.nu2, c->prm.coarse_sweeps);
 }
 
-// One V-cycle at level l in precision T on (u_l, f_l); the coarsest level
-// is always solved in FP64.
+// One V-cycle at level l in precision T on (u_l, f_l); the levels from
+// c->bottom down are one FP64 single-workgroup launch (on FP64 copies of
+// u and f when T is float).
 template <class T>
 void vcycle(bcg_rd3d* c, int l) {
     Level& v = c->L[l];
-    const int last = (int)c->L.size() - 1;
-    if (l == last) {
+    if (l == c->bottom) {
         if (sizeof(T) == 4) {
-            hipLaunchKernelGGL((k_cast<float, double>), dim3(nblk(v.n(), 256)), dim3(256), 0, c->stream, v.f32, v.f64,
+            double* u64 = l ? v.u64 : c->bot_u0;
+            double* f64 = l ? v.f64 : c->bot_f0;
+            hipLaunchKernelGGL((k_cast<float, double>), dim3(nblk(v.n(), 256)), dim3(256), 0, c->stream, v.f32, f64,
                                v.n());
-            coarse_solve(c, v);
-            hipLaunchKernelGGL((k_cast<double, float>), dim3(nblk(v.n(), 256)), dim3(256), 0, c->stream, v.u64, v.u32,
+            hipLaunchKernelGGL((k_cast<float, double>), dim3(nblk(v.n(), 256)), dim3(256), 0, c->stream, v.u32, u64,
+                               v.n());
+            bottom_solve(c, u64, f64);
+            hipLaunchKernelGGL((k_cast<double, float>), dim3(nblk(v.n(), 256)), dim3(256), 0, c->stream, u64, v.u32,
                                v.n());
         } else {
-            coarse_solve(c, v);
+            bottom_solve(c, v.u64, v.f64);
         }
         return;
     }
@@ -262,8 +276,15 @@ void vcycle(bcg_rd3d* c, int l) {
     smooth<T>(c, v, c->prm.nu2);
 }
 
+int sum_partials(bcg_rd3d* c, const double* part, long nparts) {
+    // fixed two-stage tree: 256 chunk sums, then one block; same order every call
+    hipLaunchKernelGGL(k_sum_chunks, dim3(256), dim3(256), 0, c->stream, part, nparts, c->part2);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, c->part2, 256, c->dnorm);
+    return 0;
+}
+
 int read_norm(bcg_rd3d* c, long nparts, double* out) {
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, c->part, (int)nparts, c->dnorm);
+    sum_partials(c, c->part, nparts);
     HIPCHK(c, hipMemcpyAsync(c->hnorm, c->dnorm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     *out = std::sqrt(*c->hnorm);
@@ -358,6 +379,11 @@ int bcg_rd3d_create(const bcg_rd3d_params* p, bcg_rd3d** out) {
         if (v.cz) nz = (nz - 1) / 2 + 1, hz *= 2;
     }
     c->N = c->L[0].n();
+    // the small coarse levels (at most BOT_MAX_LEVELS, each <= 4096 points)
+    // form the bottom solver
+    c->bottom = (int)c->L.size() - 1;
+    while (c->bottom > 0 && c->L[c->bottom - 1].n() <= 4096 && (int)c->L.size() - (c->bottom - 1) <= BOT_MAX_LEVELS)
+        c->bottom--;
     int rc = 0;
     auto A = [&](auto** ptr, size_t n) {
         if (!rc) rc = dalloc(c, ptr, n);
@@ -388,6 +414,11 @@ int bcg_rd3d_create(const bcg_rd3d_params* p, bcg_rd3d** out) {
     }
     c->npart = nparts_of(c->L[0]);
     A(&c->part, (size_t)c->npart);
+    A(&c->part2, 256);
+    if (c->bottom == 0) {
+        A(&c->bot_u0, N);
+        A(&c->bot_f0, N);
+    }
     A(&c->gpart, 2 * (size_t)c->npart);
     A(&c->dnorm, 1);
     if (!rc && hipHostMalloc((void**)&c->hnorm, sizeof(double), hipHostMallocDefault) != hipSuccess) rc = BCG_ERR_HIP;
@@ -412,7 +443,7 @@ int bcg_rd3d_destroy(bcg_rd3d* c) {
     if (!c) return BCG_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
     std::vector<void*> ptrs = {c->x[0], c->x[1], c->cprev, c->b, c->q, c->save, c->grad, c->gpart, c->dout, c->dfield,
-                               c->part, c->dnorm, c->dkey, c->dcnt,
+                               c->part, c->part2, c->bot_u0, c->bot_f0, c->dnorm, c->dkey, c->dcnt,
                                c->dstart, c->dfill, c->dsorted, c->scan_tot, c->dfrac, c->dcm, c->drate};
     for (size_t l = 0; l < c->L.size(); l++) {
         Level& v = c->L[l];
@@ -682,11 +713,11 @@ int bcg_rd3d_adjoint(bcg_rd3d* c, const double* g, int32_t mode, double tol, int
     if (dL_dq) HIPCHK(c, hipMemcpyAsync(dL_dq, c->grad, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     if (dL_dcold) HIPCHK(c, hipMemcpyAsync(dL_dcold, c->dfield, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     double gk = 0, gd = 0;
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, c->gpart, (int)c->npart, c->dnorm);
+    sum_partials(c, c->gpart, c->npart);
     HIPCHK(c, hipMemcpyAsync(c->hnorm, c->dnorm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     gk = *c->hnorm;
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, c->gpart + c->npart, (int)c->npart, c->dnorm);
+    sum_partials(c, c->gpart + c->npart, c->npart);
     HIPCHK(c, hipMemcpyAsync(c->hnorm, c->dnorm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     gd = *c->hnorm;

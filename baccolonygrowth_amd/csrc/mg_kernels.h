@@ -6,6 +6,21 @@
 namespace bcg {
 namespace mg {
 
+// One level of the single-workgroup bottom solver (FP64 arrays).
+struct BotLevel {
+    double *u, *f, *a0, *r;
+    int nx, ny, nz;
+    int cx, cy, cz;  // coarsening to the next level
+    double lx, ly, lz;
+};
+constexpr int BOT_MAX_LEVELS = 8;
+struct BotLevels {
+    BotLevel L[BOT_MAX_LEVELS];
+    int n;
+};
+__global__ void k_bottom(BotLevels B, int nu1, int nu2, int coarse_sweeps);
+__global__ void k_sum_chunks(const double* part, long n, double* out);
+
 template <class T>
 __global__ void k_rbgs(T* u, const T* f, const T* a0, int nx, int ny, int nz, T lx, T ly, T lz, int color);
 template <class T>

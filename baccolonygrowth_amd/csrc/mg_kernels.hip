@@ -88,20 +88,12 @@ __global__ void __launch_bounds__(256) k_residual(const T* __restrict__ u, const
     r[p] = f[p] - (d * u[p] - s);
 }
 
-// Full weighting onto the coarse grid; cx/cy/cz = 1 where that direction
-// is coarsened. Coarse z = 0 rows are Dirichlet: zero correction.
+// Full weighting at coarse point (I,J,K) (K >= 1); cx/cy/cz = 1 where that
+// direction is coarsened, Neumann faces mirrored.
 template <class T>
-__global__ void __launch_bounds__(256) k_restrict(const T* __restrict__ r, T* __restrict__ fc, int nx, int ny, int nz,
-                                                  int ncx, int ncy, int ncz, int cx, int cy, int cz) {
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
-    const int J = blockIdx.y * blockDim.y + threadIdx.y;
-    const int K = blockIdx.z;
-    if (I >= ncx || J >= ncy) return;
-    const long pc = ((long)K * ncy + J) * ncx + I;
-    if (K == 0) {
-        fc[pc] = T(0);
-        return;
-    }
+__device__ __forceinline__ T restrict_point(const T* __restrict__ r, int nx, int ny, int nz, int I, int J, int K, int cx,
+                                            int cy, int cz) {
+    if (K == 0) return T(0);  // Dirichlet rows: zero correction
     const int i0 = cx ? 2 * I : I, j0 = cy ? 2 * J : J, k0 = cz ? 2 * K : K;
     const long sxy = (long)nx * ny;
     T acc = T(0);
@@ -120,17 +112,23 @@ __global__ void __launch_bounds__(256) k_restrict(const T* __restrict__ r, T* __
         }
         acc += wz * accy;
     }
-    fc[pc] = acc;
+    return acc;
 }
 
-// u_fine += P e_coarse (trilinear).
 template <class T>
-__global__ void __launch_bounds__(256) k_prolong(T* __restrict__ u, const T* __restrict__ ec, int nx, int ny, int nz,
-                                                 int ncx, int ncy, int cx, int cy, int cz) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
-    if (i >= nx || j >= ny) return;
+__global__ void __launch_bounds__(256) k_restrict(const T* __restrict__ r, T* __restrict__ fc, int nx, int ny, int nz,
+                                                  int ncx, int ncy, int ncz, int cx, int cy, int cz) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    const int K = blockIdx.z;
+    if (I >= ncx || J >= ncy) return;
+    fc[((long)K * ncy + J) * ncx + I] = restrict_point<T>(r, nx, ny, nz, I, J, K, cx, cy, cz);
+}
+
+// Trilinear interpolation of the coarse correction at fine point (i,j,k).
+template <class T>
+__device__ __forceinline__ T prolong_point(const T* __restrict__ ec, int ncx, int ncy, int i, int j, int k, int cx, int cy,
+                                           int cz) {
     int I0, I1, J0, J1, K0, K1;
     T wi, wj, wk;  // weight of the second point
     if (cx) {
@@ -165,7 +163,18 @@ __global__ void __launch_bounds__(256) k_prolong(T* __restrict__ u, const T* __r
     const T e11 = (T(1) - wi) * E(K1, J1, I0) + wi * E(K1, J1, I1);
     const T e0 = (T(1) - wj) * e00 + wj * e01;
     const T e1 = (T(1) - wj) * e10 + wj * e11;
-    u[((long)k * ny + j) * nx + i] += (T(1) - wk) * e0 + wk * e1;
+    return (T(1) - wk) * e0 + wk * e1;
+}
+
+// u_fine += P e_coarse.
+template <class T>
+__global__ void __launch_bounds__(256) k_prolong(T* __restrict__ u, const T* __restrict__ ec, int nx, int ny, int nz,
+                                                 int ncx, int ncy, int cx, int cy, int cz) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= nx || j >= ny) return;
+    u[((long)k * ny + j) * nx + i] += prolong_point<T>(ec, ncx, ncy, i, j, k, cx, cy, cz);
 }
 
 template <class T>
@@ -391,6 +400,112 @@ __global__ void k_cic_gather(const double* __restrict__ cm, int n, double ox, do
                 }
     }
     out[c] = acc * inv_vol;
+}
+
+
+// ------------------------------------------------------ bottom solver
+// The small coarse levels run as one workgroup: a whole sub-V-cycle (and
+// the coarsest-level sweeps) in a single launch, FP64, data L2-resident.
+// The point updates repeat the arithmetic of k_rbgs / k_residual /
+// k_restrict / k_prolong.
+__device__ __forceinline__ void bot_rbgs(const BotLevel& v, int color, int nthr, int tid) {
+    const long n = (long)v.nx * v.ny * v.nz;
+    const long sxy = (long)v.nx * v.ny;
+    const double d2 = 2.0 * (v.lx + v.ly + v.lz);
+    for (long p = tid; p < n; p += nthr) {
+        const int k = (int)(p / sxy);
+        const int j = (int)((p - k * sxy) / v.nx);
+        const int i = (int)(p - k * sxy - (long)j * v.nx);
+        if (((i + j + k) & 1) != color) continue;
+        if (k == 0) {
+            v.u[p] = v.f[p];
+            continue;
+        }
+        long q;
+        const double s = neigh_sum<double>(v.u, i, j, k, v.nx, v.ny, v.nz, v.lx, v.ly, v.lz, q);
+        v.u[p] = (v.f[p] + s) / (v.a0[p] + d2);
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_bottom(BotLevels B, int nu1, int nu2, int coarse_sweeps) {
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int last = B.n - 1;
+    for (int l = 0; l < last; l++) {
+        const BotLevel& v = B.L[l];
+        const BotLevel& w = B.L[l + 1];
+        for (int s = 0; s < nu1; s++)
+            for (int color = 0; color < 2; color++) {
+                bot_rbgs(v, color, nthr, tid);
+                __syncthreads();
+            }
+        // residual into r
+        const long n = (long)v.nx * v.ny * v.nz, sxy = (long)v.nx * v.ny;
+        const double d2 = 2.0 * (v.lx + v.ly + v.lz);
+        for (long p = tid; p < n; p += nthr) {
+            const int k = (int)(p / sxy);
+            const int j = (int)((p - k * sxy) / v.nx);
+            const int i = (int)(p - k * sxy - (long)j * v.nx);
+            if (k == 0) {
+                v.r[p] = v.f[p] - v.u[p];
+                continue;
+            }
+            long q;
+            const double sn = neigh_sum<double>(v.u, i, j, k, v.nx, v.ny, v.nz, v.lx, v.ly, v.lz, q);
+            v.r[p] = v.f[p] - ((v.a0[p] + d2) * v.u[p] - sn);
+        }
+        __syncthreads();
+        // restrict r into f of the next level, clear its u
+        const long nc = (long)w.nx * w.ny * w.nz, scxy = (long)w.nx * w.ny;
+        for (long pc = tid; pc < nc; pc += nthr) {
+            const int K = (int)(pc / scxy);
+            const int J = (int)((pc - K * scxy) / w.nx);
+            const int I = (int)(pc - K * scxy - (long)J * w.nx);
+            w.f[pc] = restrict_point<double>(v.r, v.nx, v.ny, v.nz, I, J, K, v.cx, v.cy, v.cz);
+            w.u[pc] = 0.0;
+        }
+        __syncthreads();
+    }
+    {
+        const BotLevel& v = B.L[last];
+        const long n = (long)v.nx * v.ny * v.nz;
+        for (long p = tid; p < n; p += nthr) v.u[p] = 0.0;
+        __syncthreads();
+        for (int s = 0; s < coarse_sweeps; s++)
+            for (int color = 0; color < 2; color++) {
+                bot_rbgs(v, color, nthr, tid);
+                __syncthreads();
+            }
+    }
+    for (int l = last - 1; l >= 0; l--) {
+        const BotLevel& v = B.L[l];
+        const BotLevel& w = B.L[l + 1];
+        const long n = (long)v.nx * v.ny * v.nz, sxy = (long)v.nx * v.ny;
+        for (long p = tid; p < n; p += nthr) {
+            const int k = (int)(p / sxy);
+            const int j = (int)((p - k * sxy) / v.nx);
+            const int i = (int)(p - k * sxy - (long)j * v.nx);
+            v.u[p] += prolong_point<double>(w.u, w.nx, w.ny, i, j, k, v.cx, v.cy, v.cz);
+        }
+        __syncthreads();
+        for (int s = 0; s < nu2; s++)
+            for (int color = 0; color < 2; color++) {
+                bot_rbgs(v, color, nthr, tid);
+                __syncthreads();
+            }
+    }
+}
+
+// Deterministic reduction of many partial sums: fixed chunks per block.
+__global__ void __launch_bounds__(256) k_sum_chunks(const double* __restrict__ part, long n, double* __restrict__ out) {
+    __shared__ double lds[4];
+    const long per = (n + gridDim.x - 1) / gridDim.x;
+    const long lo = (long)blockIdx.x * per, hi = lo + per < n ? lo + per : n;
+    double s = 0;
+    for (long i = lo + threadIdx.x; i < hi; i += 256) s += part[i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
 }
 
 // ---------------------------------------------------- explicit instances
