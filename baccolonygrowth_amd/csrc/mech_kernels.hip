@@ -169,10 +169,18 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 
 // UniformGrid::GetNeighbours (ABM/UniformGrid.h:141-170) + reduceNeighbours
 // (ABM/Neighbours.cpp:57-93) for the cell in `slot`, against the current bins.
+//
+// Two phases so that the FP64 predicate runs converged: the FP32 prefilter
+// scan queues surviving candidate positions in LDS (GQ per thread, column
+// layout queue[e * blockDim + tid]); a full queue, and the end of the scan,
+// flush it through the exact test. Without the queue every lane would run
+// the masked FP64 test whenever any lane of its wave had a survivor.
+constexpr int GQ = 16;
+
 __device__ __forceinline__ int gather_one(const PhysParams& P, const double* __restrict__ hot, int id, int slot,
                                           const int* __restrict__ start, const int* __restrict__ sorted,
                                           const float4* __restrict__ cand, int* __restrict__ nl, int stride, int cap,
-                                          int* __restrict__ err) {
+                                          int* __restrict__ err, int* __restrict__ queue, int qstride) {
     Seg A;
     load_seg(hot, id, A);
     d3 cmA = cm_of(A);
@@ -184,43 +192,51 @@ __device__ __forceinline__ int gather_one(const PhysParams& P, const double* __r
     // j can pass the reference filter only if |cm_j - cm_A| < reach + |p_j - q_j|/2
     const float reach = (float)(A.L / 2.0 + P.Rc * A.R) + 1e-3f;
     const float cx = (float)cmA.x, cy = (float)cmA.y, cz = (float)cmA.z;
-    int m = 0;
-    const int zlo = max(z0 - 1, 0), zhi = min(z0 + 2, P.BoxZ);
-    const int ylo = max(y0 - 1, 0), yhi = min(y0 + 2, P.BoxY);
-    const int xlo = max(x0 - 1, 0), xhi = min(x0 + 2, P.BoxX);
-    // The 27-box set (the list is a set; its order is free): bins adjacent in
-    // z are adjacent in the sorted order, so each (x, y) column is one range.
-    auto test = [&](int k) {
-        const int j = sorted[k];
-        if (j == id) return;
-        Seg B;
-        load_seg(hot, j, B);
-        if (keep_neighbour(A, cmA, B, P.Rc)) {
-            if (m < cap) nl[(size_t)m * stride + slot] = j;
-            m++;
+    int m = 0, nq = 0;
+    auto flush = [&]() {
+        for (int e = 0; e < nq; e++) {
+            const int j = sorted[queue[e * qstride]];
+            Seg B;
+            load_seg(hot, j, B);
+            if (keep_neighbour(A, cmA, B, P.Rc)) {
+                if (m < cap) nl[(size_t)m * stride + slot] = j;
+                m++;
+            }
         }
+        nq = 0;
     };
     auto near = [&](const float4& c) {
         const float dx = c.x - cx, dy = c.y - cy, dz = c.z - cz;
         const float rr = reach + c.w;
         return dx * dx + dy * dy + dz * dz <= rr * rr;
     };
+    auto push = [&](int k) { queue[nq++ * qstride] = k; };
+    const int zlo = max(z0 - 1, 0), zhi = min(z0 + 2, P.BoxZ);
+    const int ylo = max(y0 - 1, 0), yhi = min(y0 + 2, P.BoxY);
+    const int xlo = max(x0 - 1, 0), xhi = min(x0 + 2, P.BoxX);
+    // The 27-box set (the list is a set; its order is free): bins adjacent in
+    // z are adjacent in the sorted order, so each (x, y) column is one range.
     for (int xa = xlo; xa < xhi; xa++)
         for (int ya = ylo; ya < yhi; ya++) {
             const int a = xa * (P.BoxY * P.BoxZ) + ya * P.BoxZ;
             const int k1 = start[a + zhi];
             int k = start[a + zlo];
             for (; k + 4 <= k1; k += 4) {
+                // flush together: a queue full in one lane empties every queue of the wave
+                if (__any(nq > GQ - 4)) flush();
                 const float4 c0 = cand[k], c1 = cand[k + 1], c2 = cand[k + 2], c3 = cand[k + 3];
                 const bool n0 = near(c0), n1 = near(c1), n2 = near(c2), n3 = near(c3);
-                if (n0) test(k);
-                if (n1) test(k + 1);
-                if (n2) test(k + 2);
-                if (n3) test(k + 3);
+                if (n0 && sorted[k] != id) push(k);
+                if (n1 && sorted[k + 1] != id) push(k + 1);
+                if (n2 && sorted[k + 2] != id) push(k + 2);
+                if (n3 && sorted[k + 3] != id) push(k + 3);
             }
-            for (; k < k1; k++)
-                if (near(cand[k])) test(k);
+            for (; k < k1; k++) {
+                if (__any(nq == GQ)) flush();
+                if (near(cand[k]) && sorted[k] != id) push(k);
+            }
         }
+    flush();
     if (m > cap) {
         atomicOr(err, ERRBIT_NLIST);
         m = cap;
@@ -234,12 +250,13 @@ __global__ void __launch_bounds__(256) k_gather_all(const double* __restrict__ h
                                                     int* __restrict__ nl_count, int stride, int cap,
                                                     int* __restrict__ order, int* __restrict__ slot_of,
                                                     int* __restrict__ maxlist, int* __restrict__ err) {
+    __shared__ int queue[GQ * 256];
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int id = sorted[t];
     order[t] = id;
     slot_of[id] = t;
-    int m = gather_one(P, hot, id, t, start, sorted, cand, nl, stride, cap, err);
+    int m = gather_one(P, hot, id, t, start, sorted, cand, nl, stride, cap, err, queue + threadIdx.x, 256);
     nl_count[t] = m;
     atomicMax(maxlist, m);
 }
@@ -250,11 +267,12 @@ __global__ void __launch_bounds__(256) k_gather_ids(const double* __restrict__ h
                                                     int* __restrict__ nl, int* __restrict__ nl_count, int stride,
                                                     int cap, const int* __restrict__ slot_of,
                                                     int* __restrict__ maxlist, int* __restrict__ err) {
+    __shared__ int queue[GQ * 256];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nid) return;
     const int id = ids[i];
     const int t = slot_of[id];
-    int m = gather_one(P, hot, id, t, start, sorted, cand, nl, stride, cap, err);
+    int m = gather_one(P, hot, id, t, start, sorted, cand, nl, stride, cap, err, queue + threadIdx.x, 256);
     nl_count[t] = m;
     atomicMax(maxlist, m);
 }
