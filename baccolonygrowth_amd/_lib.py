@@ -69,6 +69,7 @@ class RD3DParams(ctypes.Structure):
         ("coarse_sweeps", ctypes.c_int32),
         ("max_levels", ctypes.c_int32),
         ("inner", ctypes.c_int32),
+        ("smoother", ctypes.c_int32),
     ]
 
 

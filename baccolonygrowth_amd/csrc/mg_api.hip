@@ -24,6 +24,8 @@
 #include "../../include/bcg.h"
 #include "mech_kernels.h"
 #include "mg_kernels.h"
+#include "rbgs_fused.h"
+#include "zmarch.h"
 
 using namespace bcg;
 using namespace bcg::mg;
@@ -113,36 +115,6 @@ __global__ void __launch_bounds__(256) k_dep_gather(const int* __restrict__ star
     q[((long)k * ny + j) * nx + i] = acc * inv_vol;
 }
 
-// ||r|| of r = b - A u without storing r (FP64 path convergence check).
-__global__ void __launch_bounds__(256) k_resnorm64(const double* __restrict__ u, const double* __restrict__ b,
-                                                   const double* __restrict__ a0, int nx, int ny, int nz, double lx,
-                                                   double ly, double lz, double* __restrict__ part) {
-    __shared__ double lds[4];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
-    double rr = 0;
-    if (i < nx && j < ny) {
-        const long sxy = (long)nx * ny;
-        const long p = (long)k * sxy + (long)j * nx + i;
-        double r;
-        if (k == 0) {
-            r = b[p] - u[p];
-        } else {
-            const double s = lx * (u[p + (i > 0 ? -1 : 1)] + u[p + (i < nx - 1 ? 1 : -1)]) +
-                             ly * (u[p + (j > 0 ? -nx : nx)] + u[p + (j < ny - 1 ? nx : -nx)]) +
-                             lz * (u[p - sxy] + u[p + (k < nz - 1 ? sxy : -sxy)]);
-            r = b[p] - ((a0[p] + 2.0 * (lx + ly + lz)) * u[p] - s);
-        }
-        rr = r * r;
-    }
-    for (int o = 32; o > 0; o >>= 1) rr += __shfl_xor(rr, o, 64);
-    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-    if ((tid & 63) == 0) lds[tid >> 6] = rr;
-    __syncthreads();
-    if (tid == 0) part[((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
-}
-
 inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
 
 }  // namespace
@@ -221,12 +193,44 @@ struct Arr<float> {
     static float* r(Level& v) { return v.r32; }
 };
 
+// Levels of at most this many points go to the single-workgroup bottom
+// solver: one barrier-separated phase there (~1-3 us) is cheaper than one
+// launch of a small kernel.
+constexpr long BOTTOM_MAX_POINTS = 4096;
+
+// Levels of at least this many points use the fused z-marching sweeps;
+// below it the two-launch sweep has more parallelism per launch.
+constexpr long FUSED_MIN_POINTS = 1L << 21;
+constexpr int FUSED_TARGET_BLOCKS = 1536;  // about 6 workgroups per CU
+constexpr int ZM_TARGET_BLOCKS = 2048;     // z-marching passes: 8 per CU, 4 resident
+
+// `sweeps` red-black sweeps on u of level v. Fused sweeps go in pairs
+// u -> r -> u (the residual buffer is free while smoothing); an odd last
+// sweep, and all sweeps of small levels, use one launch per colour.
 template <class T>
 void smooth(bcg_rd3d* c, Level& v, int sweeps) {
-    for (int s = 0; s < sweeps; s++)
+    T* u = Arr<T>::u(v);
+    int s = 0;
+    if (c->prm.smoother == BCG_RD3D_SMOOTH_FUSED && v.n() >= FUSED_MIN_POINTS) {
+        const RbGrid g = rb_grid(v.nx, v.ny, v.nz, FUSED_TARGET_BLOCKS);
+        const dim3 grid(g.tiles_x * g.tiles_y * g.nchunk);
+        T* tmp = Arr<T>::r(v);
+        // plane loads two steps ahead (measured ~4% faster than one step ahead on
+        // the 513 x 513 x 129 level in both precisions)
+        auto sweep = [&](const T* in, T* out) {
+            hipLaunchKernelGGL((k_rbgs2<T, 2>), grid, dim3(RB_THREADS), 0, c->stream, in, out, Arr<T>::f(v),
+                               Arr<T>::a0(v), v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly, (T)v.lz, g.tiles_x, g.tiles_y,
+                               g.kchunk);
+        };
+        for (; s + 2 <= sweeps; s += 2) {
+            sweep(u, tmp);
+            sweep(tmp, u);
+        }
+    }
+    for (; s < sweeps; s++)
         for (int color = 0; color < 2; color++)
-            hipLaunchKernelGGL((k_rbgs<T>), grid_rb(v), dim3(64, 4), 0, c->stream, Arr<T>::u(v), Arr<T>::f(v),
-                               Arr<T>::a0(v), v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly, (T)v.lz, color);
+            hipLaunchKernelGGL((k_rbgs<T>), grid_rb(v), dim3(64, 4), 0, c->stream, u, Arr<T>::f(v), Arr<T>::a0(v),
+                               v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly, (T)v.lz, color);
 }
 
 void bottom_solve(bcg_rd3d* c, double* u0, double* f0) {
@@ -238,7 +242,16 @@ void bottom_solve(bcg_rd3d* c, double* u0, double* f0) {
     }
     B.L[0].u = u0;
     B.L[0].f = f0;
-    hipLaunchKernelGGL(k_bottom, dim3(1), dim3(1024), 0, c->stream, B, c->prm.nu1, c->prm.nu2, c->prm.coarse_sweeps);
+    long pts = 0, nmax = 0;
+    for (int i = 0; i < B.n; i++) {
+        const long n = (long)B.L[i].nx * B.L[i].ny * B.L[i].nz;
+        pts += n;
+        nmax = std::max(nmax, n);
+    }
+    const long bytes = (3 * pts + nmax) * (long)sizeof(double);
+    const int in_lds = bytes <= BOT_LDS_BYTES;
+    hipLaunchKernelGGL(k_bottom, dim3(1), dim3(1024), in_lds ? bytes : 0, c->stream, B, c->prm.nu1, c->prm.nu2,
+                       c->prm.coarse_sweeps, in_lds);
 }
 
 // One V-cycle at level l in precision T on (u_l, f_l); the levels from
@@ -265,8 +278,17 @@ void vcycle(bcg_rd3d* c, int l) {
     }
     Level& w = c->L[l + 1];
     smooth<T>(c, v, c->prm.nu1);
-    hipLaunchKernelGGL((k_residual<T>), grid2(v), dim3(64, 4), 0, c->stream, Arr<T>::u(v), Arr<T>::f(v), Arr<T>::a0(v),
-                       Arr<T>::r(v), v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly, (T)v.lz);
+    // z-marching residual on the large FP64 levels; in FP32 the plain kernel
+    // measured faster (147 vs 139 us on 513 x 513 x 129)
+    if (sizeof(T) == 8 && v.n() >= FUSED_MIN_POINTS) {
+        const ZmGrid g = zm_grid(v.nx, v.ny, v.nz, ZM_TARGET_BLOCKS);
+        hipLaunchKernelGGL((k_zm_residual<T>), dim3(g.blocks()), dim3(ZM_THREADS), 0, c->stream, Arr<T>::u(v),
+                           Arr<T>::f(v), Arr<T>::a0(v), Arr<T>::r(v), v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly, (T)v.lz,
+                           g.tiles_x, g.tiles_y, g.kchunk);
+    } else {
+        hipLaunchKernelGGL((k_residual<T>), grid2(v), dim3(64, 4), 0, c->stream, Arr<T>::u(v), Arr<T>::f(v),
+                           Arr<T>::a0(v), Arr<T>::r(v), v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly, (T)v.lz);
+    }
     hipLaunchKernelGGL((k_restrict<T>), grid2(w), dim3(64, 4), 0, c->stream, Arr<T>::r(v), Arr<T>::f(w), v.nx, v.ny, v.nz,
                        w.nx, w.ny, w.nz, v.cx, v.cy, v.cz);
     hipLaunchKernelGGL((k_fill<T>), dim3(nblk(w.n(), 256)), dim3(256), 0, c->stream, Arr<T>::u(w), w.n(), (T)0);
@@ -335,9 +357,10 @@ int bcg_rd3d_default_params(bcg_rd3d_params* p) {
     p->c_res = 20.0;
     p->nu1 = 2;
     p->nu2 = 2;
-    p->coarse_sweeps = 60;
+    p->coarse_sweeps = 20;
     p->max_levels = 12;
     p->inner = 2;
+    p->smoother = BCG_RD3D_SMOOTH_FUSED;
     return BCG_OK;
 }
 
@@ -345,7 +368,8 @@ int bcg_rd3d_create(const bcg_rd3d_params* p, bcg_rd3d** out) {
     if (!p || !out) return BCG_ERR_ARG;
     *out = nullptr;
     if (p->nx < 3 || p->ny < 3 || p->nz < 2 || !(p->h > 0) || !(p->D >= 0) || !(p->dt > 0) || p->nu1 < 0 ||
-        p->nu2 < 0 || p->coarse_sweeps < 1 || p->max_levels < 1 || p->inner < 1)
+        p->nu2 < 0 || p->coarse_sweeps < 1 || p->max_levels < 1 || p->inner < 1 ||
+        (p->smoother != BCG_RD3D_SMOOTH_FUSED && p->smoother != BCG_RD3D_SMOOTH_SPLIT))
         return BCG_ERR_ARG;
     bcg_rd3d* c = new bcg_rd3d();
     c->prm = *p;
@@ -379,10 +403,11 @@ int bcg_rd3d_create(const bcg_rd3d_params* p, bcg_rd3d** out) {
         if (v.cz) nz = (nz - 1) / 2 + 1, hz *= 2;
     }
     c->N = c->L[0].n();
-    // the small coarse levels (at most BOT_MAX_LEVELS, each <= 4096 points)
+    // the small coarse levels (at most BOT_MAX_LEVELS, each <= BOTTOM_MAX_POINTS)
     // form the bottom solver
     c->bottom = (int)c->L.size() - 1;
-    while (c->bottom > 0 && c->L[c->bottom - 1].n() <= 4096 && (int)c->L.size() - (c->bottom - 1) <= BOT_MAX_LEVELS)
+    while (c->bottom > 0 && c->L[c->bottom - 1].n() <= BOTTOM_MAX_POINTS &&
+           (int)c->L.size() - (c->bottom - 1) <= BOT_MAX_LEVELS)
         c->bottom--;
     int rc = 0;
     auto A = [&](auto** ptr, size_t n) {
@@ -412,7 +437,7 @@ int bcg_rd3d_create(const bcg_rd3d_params* p, bcg_rd3d** out) {
         A(&v.a032, n);
         A(&v.r32, n);
     }
-    c->npart = nparts_of(c->L[0]);
+    c->npart = std::max(nparts_of(c->L[0]), zm_grid(c->L[0].nx, c->L[0].ny, c->L[0].nz, ZM_TARGET_BLOCKS).blocks());
     A(&c->part, (size_t)c->npart);
     A(&c->part2, 256);
     if (c->bottom == 0) {
@@ -586,20 +611,23 @@ int solve_core(bcg_rd3d* c, int mode, double tol, int max_iter, int* iters, doub
     int rc = 0;
     int it = 0;
     double r0 = 0, rn = 0, bn = 0;
-    const long np = c->npart;
+    const ZmGrid zg = zm_grid(f.nx, f.ny, f.nz, ZM_TARGET_BLOCKS);
+    const long np = zg.blocks();  // one partial sum per z-marching workgroup
     // the tolerance is relative to ||b||
-    hipLaunchKernelGGL((k_sumsq<double>), dim3(std::min<long>(np, 4096)), dim3(256), 0, c->stream, c->b, c->N, c->part);
-    if ((rc = read_norm(c, std::min<long>(np, 4096), &bn))) return rc;
+    hipLaunchKernelGGL((k_sumsq<double>), dim3(4096), dim3(256), 0, c->stream, c->b, c->N, c->part);
+    if ((rc = read_norm(c, 4096, &bn))) return rc;
     if (mode == 0) {
         f.u64 = xo;
-        hipLaunchKernelGGL(k_resnorm64, grid2(f), dim3(64, 4), 0, c->stream, xo, c->b, f.a064, f.nx, f.ny, f.nz, f.lx,
-                           f.ly, f.lz, c->part);
+        auto resnorm = [&]() {
+            hipLaunchKernelGGL(k_zm_resnorm, dim3(np), dim3(ZM_THREADS), 0, c->stream, xo, c->b, f.a064, f.nx, f.ny,
+                               f.nz, f.lx, f.ly, f.lz, zg.tiles_x, zg.tiles_y, zg.kchunk, c->part);
+        };
+        resnorm();
         if ((rc = read_norm(c, np, &rn))) return rc;
         r0 = bn;
         while (it < max_iter && rn > tol * r0) {
             vcycle<double>(c, 0);
-            hipLaunchKernelGGL(k_resnorm64, grid2(f), dim3(64, 4), 0, c->stream, xo, c->b, f.a064, f.nx, f.ny, f.nz,
-                               f.lx, f.ly, f.lz, c->part);
+            resnorm();
             if ((rc = read_norm(c, np, &rn))) return rc;
             it++;
         }
@@ -607,8 +635,12 @@ int solve_core(bcg_rd3d* c, int mode, double tol, int max_iter, int* iters, doub
     } else {
         // r = b - A x0 (x copied into the other buffer), r32 = r
         double* xn = c->x[c->cur ^ 1];
-        hipLaunchKernelGGL(k_ir_update_residual, grid2(f), dim3(64, 4), 0, c->stream, xo, xn, (const float*)nullptr, 0.0,
-                           c->b, f.a064, f.f32, 1.0, f.nx, f.ny, f.nz, f.lx, f.ly, f.lz, c->part);
+        auto ir_update = [&](const double* xo_, double* xn_, const float* e, double inv_scale, double rscale) {
+            hipLaunchKernelGGL(k_zm_ir, dim3(np), dim3(ZM_THREADS), 0, c->stream, xo_, xn_, e, inv_scale, c->b, f.a064,
+                               f.f32, rscale, f.nx, f.ny, f.nz, f.lx, f.ly, f.lz, zg.tiles_x, zg.tiles_y, zg.kchunk,
+                               c->part);
+        };
+        ir_update(xo, xn, nullptr, 0.0, 1.0);
         if ((rc = read_norm(c, np, &rn))) return rc;
         c->cur ^= 1;
         r0 = bn;
@@ -622,8 +654,7 @@ int solve_core(bcg_rd3d* c, int mode, double tol, int max_iter, int* iters, doub
             const double next = rms > 0 ? std::ldexp(1.0, -std::ilogb(rms)) : 1.0;
             xo = c->x[c->cur];
             xn = c->x[c->cur ^ 1];
-            hipLaunchKernelGGL(k_ir_update_residual, grid2(f), dim3(64, 4), 0, c->stream, xo, xn, f.u32, 1.0 / scale,
-                               c->b, f.a064, f.f32, next, f.nx, f.ny, f.nz, f.lx, f.ly, f.lz, c->part);
+            ir_update(xo, xn, f.u32, 1.0 / scale, next);
             if ((rc = read_norm(c, np, &rn))) return rc;
             c->cur ^= 1;
             scale = next;

@@ -18,11 +18,23 @@ struct BotLevels {
     BotLevel L[BOT_MAX_LEVELS];
     int n;
 };
-__global__ void k_bottom(BotLevels B, int nu1, int nu2, int coarse_sweeps);
+__global__ void k_bottom(BotLevels B, int nu1, int nu2, int coarse_sweeps, int in_lds);
+constexpr long BOT_LDS_BYTES = 60 * 1024;  // levels kept in LDS when they fit
 __global__ void k_sum_chunks(const double* part, long n, double* out);
 
 template <class T>
 __global__ void k_rbgs(T* u, const T* f, const T* a0, int nx, int ny, int nz, T lx, T ly, T lz, int color);
+template <class T, int AHEAD>
+__global__ void k_rbgs2(const T* uin, T* uout, const T* f, const T* a0, int nx, int ny, int nz, T lx, T ly, T lz,
+                        int tiles_x, int tiles_y, int kchunk);
+template <class T>
+__global__ void k_zm_residual(const T* u, const T* f, const T* a0, T* r, int nx, int ny, int nz, T lx, T ly, T lz,
+                              int tiles_x, int tiles_y, int kchunk);
+__global__ void k_zm_resnorm(const double* u, const double* b, const double* a0, int nx, int ny, int nz, double lx,
+                             double ly, double lz, int tiles_x, int tiles_y, int kchunk, double* part);
+__global__ void k_zm_ir(const double* xo, double* xn, const float* e, double inv_scale, const double* b,
+                        const double* a0, float* r32, double rscale, int nx, int ny, int nz, double lx, double ly,
+                        double lz, int tiles_x, int tiles_y, int kchunk, double* part);
 template <class T>
 __global__ void k_residual(const T* u, const T* f, const T* a0, T* r, int nx, int ny, int nz, T lx, T ly, T lz);
 template <class T>
@@ -37,9 +49,6 @@ __global__ void k_cast(const Ts* a, Td* b, long n);
 template <class T>
 __global__ void k_sumsq(const T* x, long n, double* part);
 __global__ void k_sum_parts(const double* part, int n, double* out);
-__global__ void k_ir_update_residual(const double* xo, double* xn, const float* e, double inv_scale, const double* b,
-                                     const double* a0, float* r32, double rscale, int nx, int ny, int nz, double lx,
-                                     double ly, double lz, double* part);
 __global__ void k_monod_a0(const double* q, const double* c, double K, double dt, double* a0, float* a0f, long n);
 __global__ void k_rhs(const double* c, double* b, double c_res, long nxy, long n);
 template <class T>

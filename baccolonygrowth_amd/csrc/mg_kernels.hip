@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 
 #include "mg_kernels.h"
+#include "rbgs_fused.h"
+#include "zmarch.h"
 
 namespace bcg {
 namespace mg {
@@ -214,50 +216,6 @@ __global__ void __launch_bounds__(256) k_sum_parts(const double* __restrict__ pa
     if (threadIdx.x == 0) *out = (lds[0] + lds[1]) + (lds[2] + lds[3]);
 }
 
-// Mixed-precision outer step, fused: x_new = x_old + e/scale (e = 0 when
-// null), then r = b - A x_new in FP64, r32 = r * rscale, and per-block
-// partial sums of r^2. x_new at the neighbours is formed on the fly from
-// x_old and e (ping-pong buffers: no read-after-write between blocks).
-__global__ void __launch_bounds__(256) k_ir_update_residual(const double* __restrict__ xo, double* __restrict__ xn,
-                                                            const float* __restrict__ e, double inv_scale,
-                                                            const double* __restrict__ b, const double* __restrict__ a0,
-                                                            float* __restrict__ r32, double rscale, int nx, int ny,
-                                                            int nz, double lx, double ly, double lz,
-                                                            double* __restrict__ part) {
-    __shared__ double lds[4];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
-    double rr = 0;
-    if (i < nx && j < ny) {
-        const long sxy = (long)nx * ny;
-        const long p = (long)k * sxy + (long)j * nx + i;
-        auto X = [&](long q) { return e ? xo[q] + (double)e[q] * inv_scale : xo[q]; };
-        const double xc = X(p);
-        xn[p] = xc;
-        double r;
-        if (k == 0) {
-            r = b[p] - xc;
-        } else {
-            const double s = lx * (X(p + (i > 0 ? -1 : 1)) + X(p + (i < nx - 1 ? 1 : -1))) +
-                             ly * (X(p + (j > 0 ? -nx : nx)) + X(p + (j < ny - 1 ? nx : -nx))) +
-                             lz * (X(p - sxy) + X(p + (k < nz - 1 ? sxy : -sxy)));
-            const double d = a0[p] + 2.0 * (lx + ly + lz);
-            r = b[p] - (d * xc - s);
-        }
-        r32[p] = (float)(r * rscale);
-        rr = r * r;
-    }
-    for (int o = 32; o > 0; o >>= 1) rr += __shfl_xor(rr, o, 64);
-    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-    if ((tid & 63) == 0) lds[tid >> 6] = rr;
-    __syncthreads();
-    if (tid == 0) {
-        const long bid = ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-        part[bid] = (lds[0] + lds[1]) + (lds[2] + lds[3]);
-    }
-}
-
 // a0 = 1 + dt * q / (K + c)   (Monod uptake linearised about c).
 __global__ void k_monod_a0(const double* __restrict__ q, const double* __restrict__ c, double K, double dt,
                            double* __restrict__ a0, float* __restrict__ a0f, long n) {
@@ -304,6 +262,117 @@ __global__ void __launch_bounds__(256) k_restrict_a0(const T* __restrict__ a0f, 
     a0c[((long)K * ncy + J) * ncx + I] = T(1) + acc / wsum;
 }
 
+
+// One red-black sweep (both colours) from uin into uout, z-marching; the
+// algorithm and its phases are in rbgs_fused.h. Workgroups are numbered
+// tile-x fastest, then tile-y, then z range, and the hardware workgroup
+// order (round-robin over the 8 XCDs) is remapped so that each XCD runs a
+// contiguous run of tiles, which share their rings through its L2. AHEAD is
+// the number of steps a plane load is in flight.
+template <class T, int AHEAD>
+__global__ void __launch_bounds__(RB_THREADS) k_rbgs2(const T* __restrict__ uin, T* __restrict__ uout,
+                                                      const T* __restrict__ f, const T* __restrict__ a0, int nx, int ny,
+                                                      int nz, T lx, T ly, T lz, int tiles_x, int tiles_y, int kchunk) {
+    __shared__ RbShared<T> sh;
+    constexpr int X = 8;
+    const int nb = gridDim.x, per = nb / X, rem = nb % X;
+    const int xcd = blockIdx.x % X;
+    const int b = xcd * per + (xcd < rem ? xcd : rem) + blockIdx.x / X;
+    const int bx = b % tiles_x, by = (b / tiles_x) % tiles_y, bz = b / (tiles_x * tiles_y);
+    const RbArgs<T> a{uin, uout, f, a0, nx, ny, nz, lx, ly, lz};
+    const int tid = threadIdx.x;
+    RbThread<T> t;
+    RbPlane<T> pa, pb;
+    rb_init(a, bx, by, bz, kchunk, tid, t);
+    rb_prologue(a, sh, t, tid);
+    if (AHEAD == 2 && rb_plane_needed(a, t, t.kfirst + 3)) rb_fetch_plane(a, t, t.kfirst + 3, pa);
+    __syncthreads();
+    rb_prologue_init(sh, t);
+    __syncthreads();
+    auto step = [&](int k, RbPlane<T>& now, RbPlane<T>& later) {
+        rb_fetch_next(a, t, k, AHEAD, later);
+        rb_compute(a, sh, t, k);
+        rb_finish(a, sh, t, k, tid, now);
+        __syncthreads();
+    };
+    if (AHEAD == 2) {
+        for (int k = t.kfirst; k < t.k1; k += 2) {
+            step(k, pa, pb);
+            if (k + 1 < t.k1) step(k + 1, pb, pa);
+        }
+    } else {
+        for (int k = t.kfirst; k < t.k1; k++) step(k, pa, pa);
+    }
+}
+
+// z-marching stencil passes (zmarch.h): one workgroup per ZM_TX x ZM_TY
+// tile and z range, XCD-remapped like k_rbgs2; `part` (when given) gets one
+// sum of squares per workgroup, in workgroup order.
+template <class T, class Op>
+__device__ __forceinline__ void zm_run(const Op& op, int nx, int ny, int nz, int tiles_x, int tiles_y, int kchunk,
+                                       double* __restrict__ part) {
+    __shared__ ZmShared<T> sh;
+    __shared__ double red[ZM_THREADS / 64];
+    constexpr int X = 8;
+    const int nb = gridDim.x, per = nb / X, rem = nb % X;
+    const int xcd = blockIdx.x % X;
+    const int b = xcd * per + (xcd < rem ? xcd : rem) + blockIdx.x / X;
+    const int bx = b % tiles_x, by = (b / tiles_x) % tiles_y, bz = b / (tiles_x * tiles_y);
+    const int tid = threadIdx.x;
+    ZmThread<T> t;
+    typename Op::Pre pa, pb;
+    zm_init<T>(nx, ny, nz, bx, by, bz, kchunk, tid, t);
+    zm_prologue(op, sh, t, pa);
+    __syncthreads();
+    double acc = 0.0;
+    auto step = [&](int k, const typename Op::Pre& cur, typename Op::Pre& next) {
+        zm_fetch(op, t, next, nz, k);
+        acc += zm_point(op, sh, t, cur, nz, k);
+        zm_finish(sh, t, nz, k);
+        __syncthreads();
+    };
+    for (int k = t.k0; k < t.k1; k += 2) {
+        step(k, pa, pb);
+        if (k + 1 < t.k1) step(k + 1, pb, pa);
+    }
+    if (part) {
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if ((tid & 63) == 0) red[tid >> 6] = acc;
+        __syncthreads();
+        if (tid == 0) {
+            double s = 0.0;
+            for (int w = 0; w < ZM_THREADS / 64; w++) s += red[w];
+            part[b] = s;
+        }
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(ZM_THREADS) k_zm_residual(const T* __restrict__ u, const T* __restrict__ f,
+                                                            const T* __restrict__ a0, T* __restrict__ r, int nx, int ny,
+                                                            int nz, T lx, T ly, T lz, int tiles_x, int tiles_y,
+                                                            int kchunk) {
+    const ZmResidual<T> op{u, f, a0, r, (long)nx * ny, lx, ly, lz};
+    zm_run<T>(op, nx, ny, nz, tiles_x, tiles_y, kchunk, nullptr);
+}
+
+__global__ void __launch_bounds__(ZM_THREADS) k_zm_resnorm(const double* __restrict__ u, const double* __restrict__ b,
+                                                           const double* __restrict__ a0, int nx, int ny, int nz,
+                                                           double lx, double ly, double lz, int tiles_x, int tiles_y,
+                                                           int kchunk, double* __restrict__ part) {
+    const ZmResNorm op{u, b, a0, (long)nx * ny, lx, ly, lz};
+    zm_run<double>(op, nx, ny, nz, tiles_x, tiles_y, kchunk, part);
+}
+
+__global__ void __launch_bounds__(ZM_THREADS) k_zm_ir(const double* __restrict__ xo, double* __restrict__ xn,
+                                                      const float* __restrict__ e, double inv_scale,
+                                                      const double* __restrict__ b, const double* __restrict__ a0,
+                                                      float* __restrict__ r32, double rscale, int nx, int ny, int nz,
+                                                      double lx, double ly, double lz, int tiles_x, int tiles_y,
+                                                      int kchunk, double* __restrict__ part) {
+    const ZmIrUpdate op{xo, xn, e, inv_scale, b, a0, r32, rscale, (long)nx * ny, lx, ly, lz};
+    zm_run<double>(op, nx, ny, nz, tiles_x, tiles_y, kchunk, part);
+}
 
 // ------------------------------------------------------------- adjoint
 // Control-volume weight of a vertex: 1/2 per Neumann face it lies on
@@ -405,17 +474,18 @@ __global__ void k_cic_gather(const double* __restrict__ cm, int n, double ox, do
 
 // ------------------------------------------------------ bottom solver
 // The small coarse levels run as one workgroup: a whole sub-V-cycle (and
-// the coarsest-level sweeps) in a single launch, FP64, data L2-resident.
+// the coarsest-level sweeps) in a single launch, FP64, in LDS when the
+// levels fit. Levels hold at most a few thousand points: 32-bit indices.
 // The point updates repeat the arithmetic of k_rbgs / k_residual /
 // k_restrict / k_prolong.
 __device__ __forceinline__ void bot_rbgs(const BotLevel& v, int color, int nthr, int tid) {
-    const long n = (long)v.nx * v.ny * v.nz;
-    const long sxy = (long)v.nx * v.ny;
+    const int n = (int)v.nx * v.ny * v.nz;
+    const int sxy = (int)v.nx * v.ny;
     const double d2 = 2.0 * (v.lx + v.ly + v.lz);
-    for (long p = tid; p < n; p += nthr) {
+    for (int p = tid; p < n; p += nthr) {
         const int k = (int)(p / sxy);
         const int j = (int)((p - k * sxy) / v.nx);
-        const int i = (int)(p - k * sxy - (long)j * v.nx);
+        const int i = (int)(p - k * sxy - (int)j * v.nx);
         if (((i + j + k) & 1) != color) continue;
         if (k == 0) {
             v.u[p] = v.f[p];
@@ -427,24 +497,58 @@ __device__ __forceinline__ void bot_rbgs(const BotLevel& v, int color, int nthr,
     }
 }
 
-__global__ void __launch_bounds__(1024) k_bottom(BotLevels B, int nu1, int nu2, int coarse_sweeps) {
+// in_lds: the levels live in dynamic LDS for the whole launch (u, f, a0 per
+// level and one residual buffer of the largest level; only u of the first
+// level goes back to memory); otherwise they are worked on in place.
+__global__ void __launch_bounds__(1024) k_bottom(BotLevels B, int nu1, int nu2, int coarse_sweeps, int in_lds) {
+    extern __shared__ double smem[];
+    __shared__ BotLevel S[BOT_MAX_LEVELS];
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int last = B.n - 1;
+    if (tid == 0) {
+        int off = 0, rmax = 0;
+        for (int l = 0; l < B.n; l++) {
+            S[l] = B.L[l];
+            const int n = (int)S[l].nx * S[l].ny * S[l].nz;
+            rmax = n > rmax ? n : rmax;
+            if (in_lds) {
+                S[l].u = smem + off;
+                S[l].f = smem + off + n;
+                S[l].a0 = smem + off + 2 * n;
+                off += 3 * n;
+            }
+        }
+        if (in_lds)
+            for (int l = 0; l < B.n; l++) S[l].r = smem + off;
+    }
+    __syncthreads();
+    if (in_lds) {
+        for (int l = 0; l < B.n; l++) {
+            const int n = (int)S[l].nx * S[l].ny * S[l].nz;
+            for (int p = tid; p < n; p += nthr) S[l].a0[p] = B.L[l].a0[p];
+        }
+        const int n0 = (int)S[0].nx * S[0].ny * S[0].nz;
+        for (int p = tid; p < n0; p += nthr) {
+            S[0].u[p] = B.L[0].u[p];
+            S[0].f[p] = B.L[0].f[p];
+        }
+        __syncthreads();
+    }
     for (int l = 0; l < last; l++) {
-        const BotLevel& v = B.L[l];
-        const BotLevel& w = B.L[l + 1];
+        const BotLevel& v = S[l];
+        const BotLevel& w = S[l + 1];
         for (int s = 0; s < nu1; s++)
             for (int color = 0; color < 2; color++) {
                 bot_rbgs(v, color, nthr, tid);
                 __syncthreads();
             }
         // residual into r
-        const long n = (long)v.nx * v.ny * v.nz, sxy = (long)v.nx * v.ny;
+        const int n = (int)v.nx * v.ny * v.nz, sxy = (int)v.nx * v.ny;
         const double d2 = 2.0 * (v.lx + v.ly + v.lz);
-        for (long p = tid; p < n; p += nthr) {
+        for (int p = tid; p < n; p += nthr) {
             const int k = (int)(p / sxy);
             const int j = (int)((p - k * sxy) / v.nx);
-            const int i = (int)(p - k * sxy - (long)j * v.nx);
+            const int i = (int)(p - k * sxy - (int)j * v.nx);
             if (k == 0) {
                 v.r[p] = v.f[p] - v.u[p];
                 continue;
@@ -455,20 +559,20 @@ __global__ void __launch_bounds__(1024) k_bottom(BotLevels B, int nu1, int nu2, 
         }
         __syncthreads();
         // restrict r into f of the next level, clear its u
-        const long nc = (long)w.nx * w.ny * w.nz, scxy = (long)w.nx * w.ny;
-        for (long pc = tid; pc < nc; pc += nthr) {
+        const int nc = (int)w.nx * w.ny * w.nz, scxy = (int)w.nx * w.ny;
+        for (int pc = tid; pc < nc; pc += nthr) {
             const int K = (int)(pc / scxy);
             const int J = (int)((pc - K * scxy) / w.nx);
-            const int I = (int)(pc - K * scxy - (long)J * w.nx);
+            const int I = (int)(pc - K * scxy - (int)J * w.nx);
             w.f[pc] = restrict_point<double>(v.r, v.nx, v.ny, v.nz, I, J, K, v.cx, v.cy, v.cz);
             w.u[pc] = 0.0;
         }
         __syncthreads();
     }
     {
-        const BotLevel& v = B.L[last];
-        const long n = (long)v.nx * v.ny * v.nz;
-        for (long p = tid; p < n; p += nthr) v.u[p] = 0.0;
+        const BotLevel& v = S[last];
+        const int n = (int)v.nx * v.ny * v.nz;
+        for (int p = tid; p < n; p += nthr) v.u[p] = 0.0;
         __syncthreads();
         for (int s = 0; s < coarse_sweeps; s++)
             for (int color = 0; color < 2; color++) {
@@ -477,13 +581,13 @@ __global__ void __launch_bounds__(1024) k_bottom(BotLevels B, int nu1, int nu2, 
             }
     }
     for (int l = last - 1; l >= 0; l--) {
-        const BotLevel& v = B.L[l];
-        const BotLevel& w = B.L[l + 1];
-        const long n = (long)v.nx * v.ny * v.nz, sxy = (long)v.nx * v.ny;
-        for (long p = tid; p < n; p += nthr) {
+        const BotLevel& v = S[l];
+        const BotLevel& w = S[l + 1];
+        const int n = (int)v.nx * v.ny * v.nz, sxy = (int)v.nx * v.ny;
+        for (int p = tid; p < n; p += nthr) {
             const int k = (int)(p / sxy);
             const int j = (int)((p - k * sxy) / v.nx);
-            const int i = (int)(p - k * sxy - (long)j * v.nx);
+            const int i = (int)(p - k * sxy - (int)j * v.nx);
             v.u[p] += prolong_point<double>(w.u, w.nx, w.ny, i, j, k, v.cx, v.cy, v.cz);
         }
         __syncthreads();
@@ -492,6 +596,10 @@ __global__ void __launch_bounds__(1024) k_bottom(BotLevels B, int nu1, int nu2, 
                 bot_rbgs(v, color, nthr, tid);
                 __syncthreads();
             }
+    }
+    if (in_lds) {
+        const int n0 = (int)S[0].nx * S[0].ny * S[0].nz;
+        for (int p = tid; p < n0; p += nthr) B.L[0].u[p] = S[0].u[p];
     }
 }
 
@@ -512,6 +620,14 @@ __global__ void __launch_bounds__(256) k_sum_chunks(const double* __restrict__ p
 template __global__ void k_rbgs<float>(float*, const float*, const float*, int, int, int, float, float, float, int);
 template __global__ void k_rbgs<double>(double*, const double*, const double*, int, int, int, double, double, double,
                                         int);
+template __global__ void k_rbgs2<float, 2>(const float*, float*, const float*, const float*, int, int, int, float,
+                                           float, float, int, int, int);
+template __global__ void k_rbgs2<double, 2>(const double*, double*, const double*, const double*, int, int, int, double,
+                                            double, double, int, int, int);
+template __global__ void k_zm_residual<float>(const float*, const float*, const float*, float*, int, int, int, float,
+                                              float, float, int, int, int);
+template __global__ void k_zm_residual<double>(const double*, const double*, const double*, double*, int, int, int,
+                                               double, double, double, int, int, int);
 template __global__ void k_residual<float>(const float*, const float*, const float*, float*, int, int, int, float,
                                            float, float);
 template __global__ void k_residual<double>(const double*, const double*, const double*, double*, int, int, int,

@@ -20,6 +20,11 @@ from ._lib import RD3DParams, BcgError, _dp, _fp, _ip, _vp, lib
 MODES = {"fp64": 0, "mixed": 1}
 
 
+# bcg_rd3d_params.smoother (include/bcg.h)
+SMOOTH_FUSED = 0  # both colours of a red-black sweep in one z-marching launch
+SMOOTH_SPLIT = 1  # one launch per colour
+
+
 def default_rd3d_params(**kw) -> RD3DParams:
     p = RD3DParams()
     rc = lib().bcg_rd3d_default_params(ctypes.byref(p))

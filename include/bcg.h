@@ -197,12 +197,19 @@ typedef struct bcg_rd3d_params {
     int32_t coarse_sweeps;  /* sweeps on the coarsest level (FP64)           */
     int32_t max_levels;     /* multigrid levels at most                      */
     int32_t inner;          /* FP32 V-cycles per refinement step (mode 1)    */
+    int32_t smoother;       /* BCG_RD3D_SMOOTH_* (default FUSED)             */
 } bcg_rd3d_params;
 
 typedef struct bcg_rd3d bcg_rd3d;
 
 #define BCG_RD3D_FP64  0   /* FP64 multigrid V-cycles                         */
 #define BCG_RD3D_MIXED 1   /* FP32 V-cycles inside FP64 iterative refinement  */
+
+/* Red-black Gauss-Seidel sweeps on the large levels: both colours in one
+ * z-marching launch (FUSED), or one launch per colour (SPLIT). Both give the
+ * same iterates; SPLIT is kept as the plain form to check FUSED against.  */
+#define BCG_RD3D_SMOOTH_FUSED 0
+#define BCG_RD3D_SMOOTH_SPLIT 1
 
 int bcg_rd3d_default_params(bcg_rd3d_params* p);
 int bcg_rd3d_create(const bcg_rd3d_params* p, bcg_rd3d** out);

@@ -124,6 +124,31 @@ def test_mixed_equals_fp64_and_deterministic():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("nu", [(2, 2), (1, 3)])
+def test_fused_sweeps_equal_split_sweeps(nu):
+    """The z-marching two-colour sweep (the default on levels of >= 2^21
+    points) gives the iterates of the one-launch-per-colour sweep, bitwise,
+    in both solvers; odd sweep counts end with a split sweep. 225 x 161 x 65
+    leaves partial 30 x 30 tiles on both x and y edges."""
+    from baccolonygrowth_amd.nutrients import SMOOTH_FUSED, SMOOTH_SPLIT
+
+    shape = (65, 161, 225)
+    print(f"[gpu] rd3d fused vs split sweeps {shape} nu={nu}", flush=True)
+    c_old, q, _ = _case(shape, seed=12)
+    for mode in ("fp64", "mixed"):
+        out = {}
+        for sm in (SMOOTH_FUSED, SMOOTH_SPLIT):
+            with _solver(shape, nu1=nu[0], nu2=nu[1], smoother=sm) as s:
+                s.set_field(c_old)
+                s.set_uptake(q)
+                it, rr = s.step(mode, tol=1e-10, max_iter=60)
+                out[sm] = (s.field(), it, rr)
+        (a, ia, ra), (b, ib, rb) = out[SMOOTH_FUSED], out[SMOOTH_SPLIT]
+        print(f"    {mode}: fused {ia} it {ra:.3e}, split {ib} it {rb:.3e}, max |diff| {np.abs(a - b).max():.3e}", flush=True)
+        assert ia == ib and a.tobytes() == b.tobytes()
+
+
+@pytest.mark.gpu
 def test_deposit_matches_oracle_and_is_deterministic():
     shape = (9, 17, 33)
     rng = np.random.default_rng(5)
