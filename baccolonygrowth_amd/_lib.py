@@ -39,6 +39,7 @@ SIGNATURES = {
     "bcg_get_cells": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, _ip]),
     "bcg_num_cells": (ctypes.c_int, [_vp, _ip]),
     "bcg_set_fields": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
+    "bcg_get_fields": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
     "bcg_srand": (ctypes.c_int, [_vp, ctypes.c_uint32]),
     "bcg_rand": (ctypes.c_int, [_vp, _ip]),
     "bcg_rand_stream": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32, _ip]),
@@ -95,6 +96,73 @@ SIGNATURES.update(
             [_vp, _dp, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, _dp, _dp, _dp, _dp],
         ),
         "bcg_rd3d_gather": (ctypes.c_int, [_vp, _dp, ctypes.c_int32, _dp, _dp]),
+    }
+)
+
+ENV_CELL_FIELDS = [
+    "Carbon", "Waste", "GrowthRate1", "GrowthRate2", "Oxygen", "Acetate",
+    "pH", "Buffer", "Acetic", "Hydrox",
+    "qCarmnt", "qAcemnt", "pAcemnt", "qOxymnt",
+    "GrCarAer", "GrCarAna", "GrAceAer",
+]
+AGA_FIELDS = ["CarbonAgar", "WasteAgar", "OxygenAgar", "AcetateAgar"]
+
+
+class EnvParams(ctypes.Structure):
+    """Mirror of `bcg_env_params` (include/bcg.h)."""
+
+    _fields_ = [
+        (n, ctypes.c_int32)
+        for n in (
+            "BoxX", "BoxY", "BoxZ", "BoxZAgar", "maxLevelsMG", "refinementGridHeight", "FilterLen",
+            "crossFeeding", "NutrientGSI", "InterfaceCondition",
+        )
+    ] + [
+        (n, ctypes.c_double)
+        for n in (
+            "BoxLength", "cellRadius", "density_threshold", "UpdateTime", "Cdt",
+            "KC_aer", "KC_ana", "KW", "KO", "KA", "KI",
+            "C_rate", "W_rate", "Carbon_to_Waste", "Maintenance_rate",
+            "maxGrowthRate1", "maxGrowthRate2",
+            "DiffColony", "DiffAgar", "maxCarbon", "maxO2", "maxO2_Agar",
+            "qCarAer", "qCarAna", "qAceAer", "qO2Car", "qO2Ace", "pAceAer", "pAceAna",
+            "DiffColonyCarbon", "DiffAgarCarbon", "DiffColonyO2", "DiffAgarO2", "DiffColonyAce", "DiffAgarAce",
+            "maxGrowthRateCarAer", "maxGrowthRateCarAna", "maxGrowthRateAceAer",
+            "Kw", "Ka", "Kb", "Cb", "c1",
+            "qCarAernt", "qCarAnant", "qAceAernt", "qO2Carnt", "qO2Acent", "pAceAnant",
+        )
+    ]
+
+
+class Clock(ctypes.Structure):
+    """Mirror of `bcg_clock` (include/bcg.h)."""
+
+    _fields_ = [
+        (n, ctypes.c_double)
+        for n in (
+            "t", "dt", "OutputTime", "UpdateTime", "HeightDensityUpdateTime",
+            "NextOutTime", "NextUpdateTime", "HeightDensityNextUpdateTime",
+        )
+    ] + [(n, ctypes.c_int32) for n in ("OutFlag", "UpdateFlag", "HeightDensityUpdateFlag")]
+
+
+SIGNATURES.update(
+    {
+        "bcg_env_default_params": (ctypes.c_int, [ctypes.POINTER(EnvParams)]),
+        "bcg_env_create": (ctypes.c_int, [ctypes.POINTER(EnvParams), _vp, ctypes.POINTER(_vp)]),
+        "bcg_env_destroy": (ctypes.c_int, [_vp]),
+        "bcg_env_last_error": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int]),
+        "bcg_env_set_state": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "bcg_env_get_state": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "bcg_env_fields": (ctypes.c_int, [_vp, ctypes.c_int32]),
+        "bcg_env_nutrients": (ctypes.c_int, [_vp, _ip]),
+        "bcg_env_get_fields": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, _ip]),
+        "bcg_clock_init": (
+            ctypes.c_int,
+            [ctypes.POINTER(Clock), ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double],
+        ),
+        "bcg_clock_advance": (ctypes.c_int, [ctypes.POINTER(Clock)]),
+        "bcg_sim_step": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(Clock), _ip, _ip, ctypes.c_int32]),
     }
 )
 

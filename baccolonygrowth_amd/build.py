@@ -2,9 +2,9 @@
 
     python -m baccolonygrowth_amd.build
 
-Flags: -ffp-contract=off keeps the mechanics kernels free of FMA contraction
-(every +,-,*,/ rounds as in the reference's x86-64 build); packed FP32 ops are
-disabled for all device code.
+Flags: -ffp-contract=off keeps the mechanics and environment kernels free of
+FMA contraction (every +,-,*,/ rounds as in the reference's x86-64 build);
+packed FP32 ops are disabled for all device code.
 """
 from __future__ import annotations
 
@@ -23,6 +23,8 @@ SOURCES = [
     ("bcg_api.hip", ["-ffp-contract=off"]),
     ("mg_kernels.hip", []),
     ("mg_api.hip", []),
+    ("env_kernels.hip", ["-ffp-contract=off"]),
+    ("env_api.hip", ["-ffp-contract=off"]),
 ]
 
 COMMON = [

@@ -24,48 +24,7 @@
 
 using namespace bcg;
 
-namespace {
-
-struct Stats {
-    unsigned long long wall_draws;
-    int err;
-    int maxlist;
-    int ndiv;
-    int pad;
-};
-
-}  // namespace
-
-struct bcg_ctx {
-    bcg_params prm;
-    PhysParams P;
-    hipStream_t stream = nullptr;
-    int N = 0;
-    int cur = 0;
-    int stride = 0;  // list stride (>= maxCells)
-    int cap = 0;
-    int nbin = 0;
-    bool lists_valid = false;
-    double* hot[2] = {nullptr, nullptr};
-    double *time_p = nullptr, *time_q = nullptr, *dynfric = nullptr, *stafric = nullptr;
-    int *age_p = nullptr, *age_q = nullptr, *type = nullptr, *ancestor = nullptr;
-    int *key = nullptr, *bin_cnt = nullptr, *bin_start = nullptr, *bin_fill = nullptr, *sorted = nullptr;
-    int* scan_tot = nullptr;
-    float4 *cmL = nullptr, *cand = nullptr;
-    int *nl = nullptr, *nl_count = nullptr, *order = nullptr, *slot_of = nullptr;
-    int *divflag = nullptr, *div_ids = nullptr, *div_index_of = nullptr, *ids_tmp = nullptr;
-    double *draws = nullptr, *mother_pre = nullptr;
-    double *height = nullptr, *normal = nullptr, *wall = nullptr, *gr1 = nullptr, *gr2 = nullptr;
-    double *Fbuf = nullptr, *Tbuf = nullptr;
-    Stats* stats = nullptr;  // device
-    Stats* hstats = nullptr; // pinned host
-    GlibcRand rng;
-    hipEvent_t ev[7];
-    float last_ms[6] = {0, 0, 0, 0, 0, 0};
-    Stats last{};
-    int last_flagged = 0;
-    std::string err;
-};
+#include "ctx_internal.h"
 
 namespace {
 
@@ -166,12 +125,12 @@ int gather_all(bcg_ctx* c) {
 }
 
 int reset_stats(bcg_ctx* c) {
-    HIPCHK(c, hipMemsetAsync(c->stats, 0, sizeof(Stats), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->stats, 0, sizeof(BcgStats), c->stream));
     return 0;
 }
 
 int read_stats(bcg_ctx* c) {
-    HIPCHK(c, hipMemcpyAsync(c->hstats, c->stats, sizeof(Stats), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->hstats, c->stats, sizeof(BcgStats), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->last = *c->hstats;
     return 0;
@@ -196,6 +155,14 @@ int set_identity_slots(bcg_ctx* c) {
 }
 
 }  // namespace
+
+int bcg_bin_current(bcg_ctx* c) {
+    int rc = reset_stats(c);
+    if (!rc) rc = build_bins(c, c->cur, c->N);
+    if (!rc) rc = read_stats(c);
+    if (!rc) rc = check_err(c);
+    return rc;
+}
 
 extern "C" {
 
@@ -291,7 +258,7 @@ int bcg_create(const bcg_params* p, bcg_ctx** out) {
     A(&c->Fbuf, 3 * M);
     A(&c->Tbuf, 3 * M);
     A(&c->stats, 1);
-    if (!rc && hipHostMalloc((void**)&c->hstats, sizeof(Stats), hipHostMallocDefault) != hipSuccess)
+    if (!rc && hipHostMalloc((void**)&c->hstats, sizeof(BcgStats), hipHostMallocDefault) != hipSuccess)
         rc = BCG_ERR_HIP;
     for (int i = 0; i < 7 && !rc; i++)
         if (hipEventCreate(&c->ev[i]) != hipSuccess) rc = BCG_ERR_HIP;
@@ -457,6 +424,19 @@ int bcg_set_fields(bcg_ctx* c, const double* height, const double* normal, const
         HIPCHK(c, hipMemcpy(c->wall, wall, sizeof(double) * c->prm.BoxX * c->prm.BoxY, hipMemcpyHostToDevice));
     if (gr1) HIPCHK(c, hipMemcpy(c->gr1, gr1, sizeof(double) * c->nbin, hipMemcpyHostToDevice));
     if (gr2) HIPCHK(c, hipMemcpy(c->gr2, gr2, sizeof(double) * c->nbin, hipMemcpyHostToDevice));
+    return BCG_OK;
+}
+
+int bcg_get_fields(bcg_ctx* c, double* height, double* normal, double* wall, double* gr1, double* gr2) {
+    if (!c) return BCG_ERR_ARG;
+    const size_t nh = (size_t)c->P.nhx * c->P.nhy;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (height) HIPCHK(c, hipMemcpy(height, c->height, nh * sizeof(double), hipMemcpyDeviceToHost));
+    if (normal) HIPCHK(c, hipMemcpy(normal, c->normal, 3 * nh * sizeof(double), hipMemcpyDeviceToHost));
+    if (wall)
+        HIPCHK(c, hipMemcpy(wall, c->wall, sizeof(double) * c->prm.BoxX * c->prm.BoxY, hipMemcpyDeviceToHost));
+    if (gr1) HIPCHK(c, hipMemcpy(gr1, c->gr1, sizeof(double) * c->nbin, hipMemcpyDeviceToHost));
+    if (gr2) HIPCHK(c, hipMemcpy(gr2, c->gr2, sizeof(double) * c->nbin, hipMemcpyDeviceToHost));
     return BCG_OK;
 }
 

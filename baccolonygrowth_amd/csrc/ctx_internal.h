@@ -1,0 +1,56 @@
+// ctx_internal.h - the mechanics context (struct bcg_ctx) shared by the
+// sources of the library: bcg_api.hip owns it, env_api.hip reads its cells
+// and bins and writes the fields it reads.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/bcg.h"
+#include "glibc_rand.h"
+#include "mech_kernels.h"
+
+struct BcgStats {
+    unsigned long long wall_draws;
+    int err;
+    int maxlist;
+    int ndiv;
+    int pad;
+};
+
+struct bcg_ctx {
+    bcg_params prm;
+    bcg::PhysParams P;
+    hipStream_t stream = nullptr;
+    int N = 0;
+    int cur = 0;
+    int stride = 0;  // list stride (>= maxCells)
+    int cap = 0;
+    int nbin = 0;
+    bool lists_valid = false;
+    double* hot[2] = {nullptr, nullptr};
+    double *time_p = nullptr, *time_q = nullptr, *dynfric = nullptr, *stafric = nullptr;
+    int *age_p = nullptr, *age_q = nullptr, *type = nullptr, *ancestor = nullptr;
+    int *key = nullptr, *bin_cnt = nullptr, *bin_start = nullptr, *bin_fill = nullptr, *sorted = nullptr;
+    int* scan_tot = nullptr;
+    float4 *cmL = nullptr, *cand = nullptr;
+    int *nl = nullptr, *nl_count = nullptr, *order = nullptr, *slot_of = nullptr;
+    int *divflag = nullptr, *div_ids = nullptr, *div_index_of = nullptr, *ids_tmp = nullptr;
+    double *draws = nullptr, *mother_pre = nullptr;
+    double *height = nullptr, *normal = nullptr, *wall = nullptr, *gr1 = nullptr, *gr2 = nullptr;
+    double *Fbuf = nullptr, *Tbuf = nullptr;
+    BcgStats* stats = nullptr;  // device
+    BcgStats* hstats = nullptr; // pinned host
+    bcg::GlibcRand rng;
+    hipEvent_t ev[7];
+    float last_ms[6] = {0, 0, 0, 0, 0, 0};
+    BcgStats last{};
+    int last_flagged = 0;
+    std::string err;
+};
+
+
+// Bin the context's current cells (counting sort, ascending IDs per bin);
+// BCG_ERR_OUT_OF_DOMAIN when a centre lies outside the bin grid.
+int bcg_bin_current(bcg_ctx* c);

@@ -1,0 +1,91 @@
+// env_kernels.h - device side of the planar colony environment (bcg_env_*):
+// density and footprint, height and surface normal, and the nutrient update
+// with its nested agar multigrid.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/bcg.h"
+
+namespace bcg {
+namespace env {
+
+constexpr int MG_LEVELS = 8;      // Multigrid(maxLevelsMG, 8, ...) at ABM/Simulation.cpp:150
+constexpr int MAX_NEST = 4;       // nested agar levels (maxLevelsMG) supported
+constexpr int MAX_REF = 8;        // refinementGridHeight supported
+constexpr int ENV_THREADS = 1024; // the nutrient update runs as one workgroup
+
+enum ErrBits {
+    ERR_RANGE = 1,       // a stencil or filter would read outside an array
+    ERR_CELLTYPE = 2,    // a cell of Type other than 1 or 2 (the reference prints "CellType error!")
+    ERR_COLONY_EDGE = 4, // a colony point with x <= 2 (the reference asserts "Need more boxes")
+    ERR_PHANTOM = 8,     // a multigrid read past the end of its array (read as 0; see MG::at)
+};
+
+// Everything the device passes need, by value.
+struct EnvArgs {
+    // grid
+    int BX, BZ, BZA, ML, REF, NHY;  // BoxY = 1; NHY = BoxY * refinementGridHeight
+    double BoxLength, cellRadius, density_threshold, Cdt;
+    // chemistry
+    double KC_aer, KC_ana, KW, KO, KA;
+    double C_rate, W_rate, Carbon_to_Waste, Maintenance_rate, maxGrowthRate2;
+    double DiffColony, DiffAgar, maxCarbon, maxO2, maxO2_Agar;
+    double qCarAer, qCarAna, qAceAer, qO2Car, qO2Ace, pAceAer, pAceAna;
+    double DiffColonyCarbon, DiffAgarCarbon, DiffColonyO2, DiffAgarO2, DiffColonyAce, DiffAgarAce;
+    double maxGrowthRateCarAer, maxGrowthRateCarAna, maxGrowthRateAceAer;
+    double Kw, Ka, Kb, Cb, c1;
+    double qCarAernt, qCarAnant, qAceAernt, qO2Carnt, qO2Acent, pAceAnant;
+    int crossFeeding;
+    int n_outer;  // outer iterations of the nutrient update
+    // state
+    bcg_env_cell* colony[2];          // [0] Environment, [1] oldEnvironment
+    bcg_aga* agar[2][MAX_NEST];       // FieldAgar, oldFieldAgar
+    bcg_aga* wall[2][MAX_NEST];       // FieldWall, oldFieldWall
+    bcg_aga* aga[MG_LEVELS];          // Multigrid::Aga
+    bcg_aga* res[MG_LEVELS];          // Multigrid::AgaRes
+    bcg_aga* err[MG_LEVELS];          // Multigrid::AgaErr
+    int mgz[MG_LEVELS];               // z extent - 1 of each multigrid array
+    // fields (smoothed, shifted to the grid points)
+    const double* inside;             // insideColonyDen [BX][BZ]
+    const double* dshift;             // DensityShiftP
+    const double* d1shift;
+    const double* d2shift;
+    const double* wshift;             // WallDensityShiftP [BX]
+    const int* bounds;                // minx, maxx, miny, maxy, maxz
+    int* errflag;
+};
+
+// density / height passes
+struct FieldArgs {
+    int BX, BZ, REF, NHY, N, FD;
+    double BoxLength, cellRadius, density_threshold;
+    const double* hot;   // mechanics cell records
+    const int* type;
+    const int* start;    // bin starts (bins x-major, z fastest)
+    const int* sorted;   // cell IDs by bin, ascending within a bin
+    double *rough, *rough1, *rough2, *inside;
+    double *rwall, *rwall1, *rwall2;
+    double *rshift, *rshift1, *rshift2, *rwshift, *rwshift1, *rwshift2;
+    double *height, *rheight, *normal;
+    int* bounds;
+    int* errflag;
+};
+
+__global__ void k_env_density(FieldArgs a);
+__global__ void k_env_height(FieldArgs a);
+__global__ void k_env_halo(FieldArgs a);
+__global__ void k_env_pad_bounds(FieldArgs a);
+__global__ void k_env_shift(FieldArgs a);
+__global__ void k_env_box_average(FieldArgs a, const double* rough, const double* rwall, double* out, double* outwall,
+                                  const double* filter3d);
+__global__ void k_env_height_average(FieldArgs a);
+__global__ void k_env_smooth(FieldArgs a, const double* filter1d);
+__global__ void k_env_normal(FieldArgs a);
+__global__ void k_env_init_bounds(FieldArgs a);
+
+__global__ void __launch_bounds__(ENV_THREADS) k_env_nutrients(EnvArgs a);
+__global__ void k_env_growth_to_mech(const bcg_env_cell* colony, int n, double* gr1, double* gr2);
+
+}  // namespace env
+}  // namespace bcg

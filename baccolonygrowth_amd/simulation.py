@@ -101,6 +101,15 @@ class Colony:
         ptrs = [_d(a) for a in (height, normal, wall, gr1, gr2)]
         self._c(lib().bcg_set_fields(self._h, *[p for p, _ in ptrs]))
 
+    def fields(self):
+        """(height, normal, wall, gr1, gr2) as the mechanics sees them."""
+        p = self.params
+        nh = p.BoxX * p.refinementGridHeight * p.BoxY * p.refinementGridHeight
+        nb = p.BoxX * p.BoxY * p.BoxZ
+        out = [np.zeros(nh), np.zeros(3 * nh), np.zeros(p.BoxX * p.BoxY), np.zeros(nb), np.zeros(nb)]
+        self._c(lib().bcg_get_fields(self._h, *[a.ctypes.data_as(_dp) for a in out]))
+        return tuple(out)
+
     # -- RNG
     def srand(self, seed: int):
         self._c(lib().bcg_srand(self._h, seed))

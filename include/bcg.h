@@ -119,6 +119,8 @@ int bcg_num_cells(bcg_ctx* ctx, int32_t* n);
  * Any pointer may be NULL to leave that field unchanged.                    */
 int bcg_set_fields(bcg_ctx* ctx, const double* height, const double* normal,
                    const double* wall, const double* gr1, const double* gr2);
+/* Read the same fields back (what the environment passes wrote).          */
+int bcg_get_fields(bcg_ctx* ctx, double* height, double* normal, double* wall, double* gr1, double* gr2);
 
 /* glibc-compatible rand() stream owned by the context (the reference draws
  * from the process-wide rand(): ABM/CellMD3D.cpp:23, ABM/Forces.cpp:102,
@@ -251,6 +253,122 @@ int bcg_rd3d_gather(bcg_rd3d* ctx, const double* cm, int32_t n, const double* fi
 int bcg_rd3d_vcycle(bcg_rd3d* ctx, int32_t mode, float* ms);
 /* device times of the last step, ms: [0] setup (a0, rhs, coarse a0), [1] solve */
 int bcg_rd3d_last_times(bcg_rd3d* ctx, float* ms2);
+
+/* ---- planar colony environment ------------------------------------------
+ * The fields of the reference's x-z model around the mechanics: cell
+ * density and colony footprint (ABM/Compute.cpp:120-253, 344-605), height
+ * and surface normal (ABM/Compute.cpp:256-341, 470-493, 640-738), and the
+ * nutrient update of the colony, the colony-agar interface and the nested
+ * agar multigrid (ABM/Multigrid.cpp:5-1125, ABM/Nutrients.cpp:200-659,
+ * 806-855). BoxY must be 1 (the reference's planar build). The context is
+ * bound to a mechanics context: it reads its cells and writes the fields the
+ * mechanics reads (Height, Normal, Environment.GrowthRate1/2).              */
+
+/* One colony box of the reference's Environment: `struct LocalEnv`
+ * (ABM/Nutrients.h:11-62), 17 doubles, same order.                         */
+typedef struct bcg_env_cell {
+    double Carbon, Waste, GrowthRate1, GrowthRate2, Oxygen, Acetate;
+    double pH, Buffer, Acetic, Hydrox;
+    double qCarmnt, qAcemnt, pAcemnt, qOxymnt;
+    double GrCarAer, GrCarAna, GrAceAer;
+} bcg_env_cell;
+
+/* One agar / wall point: `struct LocalAga` (ABM/Nutrients.h:64-79).        */
+typedef struct bcg_aga {
+    double CarbonAgar, WasteAgar, OxygenAgar, AcetateAgar;
+} bcg_aga;
+
+/* Constants of the environment; names follow ABM/Constants.cpp (the input
+ * keys of ABM/InputOutput.cpp:481-661 in comments where they differ).      */
+typedef struct bcg_env_params {
+    int32_t BoxX, BoxY, BoxZ;          /* colony boxes (BoxY = 1)            */
+    int32_t BoxZAgar;                  /* "Box_z_agar"                      */
+    int32_t maxLevelsMG;               /* nested agar levels                */
+    int32_t refinementGridHeight;
+    int32_t FilterLen;
+    int32_t crossFeeding;              /* "CrossFeeding"                    */
+    int32_t NutrientGSI;               /* only 0 (Jacobi) is supported      */
+    int32_t InterfaceCondition;        /* only 1 is defined                 */
+    double BoxLength, cellRadius, density_threshold;
+    double UpdateTime;                 /* "Update_Time" (hours)             */
+    double Cdt;
+    double KC_aer, KC_ana, KW, KO, KA, KI;
+    double C_rate, W_rate, Carbon_to_Waste, Maintenance_rate;
+    double maxGrowthRate1, maxGrowthRate2;            /* "Growth_Rate1/2"   */
+    double DiffColony, DiffAgar, maxCarbon;
+    double maxO2;                                     /* "Max_Oxygen"       */
+    double maxO2_Agar;
+    double qCarAer, qCarAna, qAceAer, qO2Car, qO2Ace, pAceAer, pAceAna;
+    double DiffColonyCarbon, DiffAgarCarbon, DiffColonyO2, DiffAgarO2, DiffColonyAce, DiffAgarAce;
+    double maxGrowthRateCarAer, maxGrowthRateCarAna, maxGrowthRateAceAer;
+    double Kw, Ka, Kb, Cb, c1;
+    double qCarAernt, qCarAnant, qAceAernt, qO2Carnt, qO2Acent, pAceAnant;
+} bcg_env_params;
+
+typedef struct bcg_env bcg_env;
+
+/* Reference defaults overlaid with the demo input file (as bcg_default_params). */
+int bcg_env_default_params(bcg_env_params* p);
+/* Replaces the field allocation of main (ABM/CellMD3D.cpp:56-104) and the
+ * Multigrid construction (ABM/Simulation.cpp:150-151). Fields start as the
+ * reference's constructors leave them (LocalEnv(), LocalAga()).            */
+int bcg_env_create(const bcg_env_params* p, bcg_ctx* mech, bcg_env** out);
+int bcg_env_destroy(bcg_env* env);
+int bcg_env_last_error(const bcg_env* env, char* buf, int len);
+/* Nutrient state in the reference layouts (x-major, z fastest):
+ *   colony, colony_old  [BoxX][BoxY][BoxZ]                 Environment, oldEnvironment
+ *   agar, agar_old      [maxLevelsMG][BoxX][BoxY][BoxZAgar] FieldAgar, oldFieldAgar
+ *   wall, wall_old      [maxLevelsMG][BoxX][BoxY]          FieldWall, oldFieldWall
+ * Any pointer may be NULL (set: left unchanged; get: not copied).           */
+int bcg_env_set_state(bcg_env* env, const bcg_env_cell* colony, const bcg_env_cell* colony_old,
+                      const bcg_aga* agar, const bcg_aga* agar_old, const bcg_aga* wall, const bcg_aga* wall_old);
+int bcg_env_get_state(bcg_env* env, bcg_env_cell* colony, bcg_env_cell* colony_old, bcg_aga* agar,
+                      bcg_aga* agar_old, bcg_aga* wall, bcg_aga* wall_old);
+
+#define BCG_ENV_INIT    0  /* the pass before the time loop (ABM/Simulation.cpp:187-211)   */
+#define BCG_ENV_UPDATE  1  /* the UpdateFlag pass (ABM/Simulation.cpp:272-282)            */
+#define BCG_ENV_HEIGHT  2  /* the HeightDensityUpdateFlag pass (ABM/Simulation.cpp:308-312) */
+
+/* Fields from the mechanics context's current cells, on the device:
+ *   INIT, UPDATE: GetDensity, ShiftDensity and six BoxAverage; then for
+ *                 UPDATE Height_Average + Smooth; then GetSurfaceNormal.
+ *   HEIGHT:       GetHeight, Height_Average, Smooth, GetSurfaceNormal.
+ * Height and Normal go to the mechanics context. BCG_ERR_OUT_OF_DOMAIN when
+ * a filter or stencil would read outside the arrays (the reference reads
+ * out of bounds there).                                                     */
+int bcg_env_fields(bcg_env* env, int32_t pass);
+/* UpdateEnvArrayMG_OxygenAcetate (ABM/Multigrid.cpp:984-1125), single
+ * threaded semantics, from the last INIT/UPDATE fields; *count = its return
+ * value. GrowthRate1/2 of the resulting Environment go to the mechanics.   */
+int bcg_env_nutrients(bcg_env* env, int32_t* count);
+/* Field read-back for tests and output, reference layouts (NULL: skip):
+ *   density, density_shift, inside  [BoxX][BoxY][BoxZ]  Density, DensityShiftP, insideColonyDen
+ *   wall_shift [BoxX][BoxY]                              WallDensityShiftP
+ *   bounds[5] = minx, maxx, miny, maxy, maxz (after GetDensity/GetHeight)   */
+int bcg_env_get_fields(bcg_env* env, double* density, double* density_shift, double* inside, double* wall_shift,
+                       int32_t* bounds);
+
+/* ---- time loop cadence ---------------------------------------------------
+ * The flags of the reference time loop (ABM/Simulation.cpp:49-58, 415-426):
+ * set by decrementing doubles, so the periods are what double arithmetic
+ * makes them.                                                               */
+typedef struct bcg_clock {
+    double t, dt;
+    double OutputTime, UpdateTime, HeightDensityUpdateTime;
+    double NextOutTime, NextUpdateTime, HeightDensityNextUpdateTime;
+    int32_t OutFlag, UpdateFlag, HeightDensityUpdateFlag;
+} bcg_clock;
+
+int bcg_clock_init(bcg_clock* c, double t0, double dt, double OutputTime, double UpdateTime);
+/* t += dt, then the three flags for the next step (ABM/Simulation.cpp:415-426). */
+int bcg_clock_advance(bcg_clock* c);
+
+/* One pass of the time loop (ABM/Simulation.cpp:245-427) without output:
+ * UpdateFlag -> fields(UPDATE), nutrients, list rebuild; HeightDensity ->
+ * fields(HEIGHT); move + grow; divisions (lineage as bcg_step); clock
+ * advance. `env` may be NULL (fields stay as set).                          */
+int bcg_sim_step(bcg_ctx* ctx, bcg_env* env, bcg_clock* clock, int32_t* n_div, int32_t* lineage,
+                 int32_t lineage_cap);
 
 #ifdef __cplusplus
 }

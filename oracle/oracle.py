@@ -248,5 +248,71 @@ class RefLib(_Base):
         return self.lib.ref_rand()
 
 
+class RefEnv:
+    """The reference's field and nutrient passes on a RefLib Sim (test
+    infrastructure: ref_env_* of oracle/ref_shim.cpp)."""
+
+    def __init__(self, ref: "RefLib", sim: Sim, env_params):
+        from baccolonygrowth_amd.environment import ENV_DTYPE, AGA_DTYPE
+
+        self.ref, self.sim, self.p = ref, sim, env_params
+        L = ref.lib
+        L.ref_env_create.restype = _vp
+        L.ref_env_create.argtypes = [_vp, _vp]
+        L.ref_env_destroy.argtypes = [_vp]
+        L.ref_env_destroy.restype = None
+        L.ref_env_fields.argtypes = [_vp, ctypes.c_int]
+        L.ref_env_fields.restype = None
+        L.ref_env_nutrients.argtypes = [_vp]
+        L.ref_env_nutrients.restype = ctypes.c_int
+        L.ref_env_get_fields.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
+        L.ref_env_get_fields.restype = None
+        L.ref_env_get_state.argtypes = [_vp] * 7
+        L.ref_env_get_state.restype = None
+        self._env_dtype, self._aga_dtype = ENV_DTYPE, AGA_DTYPE
+        self.h = L.ref_env_create(ctypes.byref(env_params), sim.h)
+
+    def close(self):
+        if self.h:
+            self.ref.lib.ref_env_destroy(self.h)
+            self.h = None
+
+    def fields(self, pass_: int):
+        self.ref.lib.ref_env_fields(self.h, pass_)
+
+    def nutrients(self) -> int:
+        return self.ref.lib.ref_env_nutrients(self.h)
+
+    def get_fields(self):
+        p = self.p
+        nb = p.BoxX * p.BoxZ
+        nh = p.BoxX * p.refinementGridHeight * p.BoxY * p.refinementGridHeight
+        out = {k: np.zeros(nb) for k in ("density", "density_shift", "inside")}
+        out["wall_shift"] = np.zeros(p.BoxX)
+        out["height"] = np.zeros(nh)
+        out["normal"] = np.zeros(3 * nh)
+        b = np.zeros(5, dtype=np.int32)
+        P = lambda a: a.ctypes.data_as(_dp)  # noqa: E731
+        self.ref.lib.ref_env_get_fields(self.h, P(out["density"]), P(out["density_shift"]), P(out["inside"]),
+                                        P(out["wall_shift"]), P(out["height"]), P(out["normal"]),
+                                        b.ctypes.data_as(_ip))
+        out["bounds"] = b
+        return out
+
+    def get_state(self):
+        p = self.p
+        nb, na = p.BoxX * p.BoxZ, p.BoxX * p.BoxZAgar
+        st = {
+            "colony": np.zeros(nb, self._env_dtype),
+            "colony_old": np.zeros(nb, self._env_dtype),
+            "agar": np.zeros(p.maxLevelsMG * na, self._aga_dtype),
+            "agar_old": np.zeros(p.maxLevelsMG * na, self._aga_dtype),
+            "wall": np.zeros(p.maxLevelsMG * p.BoxX, self._aga_dtype),
+            "wall_old": np.zeros(p.maxLevelsMG * p.BoxX, self._aga_dtype),
+        }
+        self.ref.lib.ref_env_get_state(self.h, *[ctypes.c_void_p(a.ctypes.data) for a in st.values()])
+        return st
+
+
 def ref_available() -> bool:
     return os.path.exists(REF_SO)

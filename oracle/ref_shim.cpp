@@ -23,6 +23,9 @@
 #include "UniformGrid.h"
 #include "grow.h"
 #include "tools.h"
+#include "Compute.h"
+#include "Multigrid.h"
+#include "Nutrients.h"
 
 #include "../include/bcg.h"
 
@@ -276,6 +279,246 @@ int ref_sim_step(void* h, double dt, double t, int rebuild, int* lineage, int li
     s->old_cells = tmp;
     s->N = N_cells;
     return done;
+}
+
+// ---------------------------------------------------------------- environment
+// The field and nutrient passes of the reference time loop
+// (ABM/Simulation.cpp:187-211, 269-312) on a RefSim's cells, grid, Height,
+// Normal and Environment. Arrays the reference leaves uninitialised are
+// pinned: BoxAverage outputs outside its window are zeroed after each call,
+// and insideColonyDen gets one extra zero x-plane (the restriction reads
+// x = BoxX at ABM/Multigrid.cpp:272).
+static_assert(sizeof(LocalEnv) == sizeof(bcg_env_cell), "LocalEnv layout");
+static_assert(sizeof(LocalAga) == sizeof(bcg_aga), "LocalAga layout");
+
+static void set_env_globals(const bcg_env_params* p) {
+    BoxX = p->BoxX;
+    BoxY = p->BoxY;
+    BoxZ = p->BoxZ;
+    BoxZAgar = p->BoxZAgar;
+    maxLevelsMG = p->maxLevelsMG;
+    refinementGridHeight = p->refinementGridHeight;
+    FilterLen = p->FilterLen;
+    crossFeeding = p->crossFeeding != 0;
+    NutrientGSI = p->NutrientGSI != 0;
+    InterfaceCondition = p->InterfaceCondition;
+    BoxLength = p->BoxLength;
+    cellRadius = p->cellRadius;
+    density_threshold = p->density_threshold;
+    UpdateTime = p->UpdateTime;
+    Cdt = p->Cdt;
+    KC_aer = p->KC_aer; KC_ana = p->KC_ana; KW = p->KW; KO = p->KO; KA = p->KA; KI = p->KI;
+    C_rate = p->C_rate; W_rate = p->W_rate; Carbon_to_Waste = p->Carbon_to_Waste; Maintenance_rate = p->Maintenance_rate;
+    maxGrowthRate1 = p->maxGrowthRate1; maxGrowthRate2 = p->maxGrowthRate2;
+    DiffColony = p->DiffColony; DiffAgar = p->DiffAgar; maxCarbon = p->maxCarbon; maxO2 = p->maxO2;
+    maxO2_Agar = p->maxO2_Agar;
+    qCarAer = p->qCarAer; qCarAna = p->qCarAna; qAceAer = p->qAceAer; qO2Car = p->qO2Car; qO2Ace = p->qO2Ace;
+    pAceAer = p->pAceAer; pAceAna = p->pAceAna;
+    DiffColonyCarbon = p->DiffColonyCarbon; DiffAgarCarbon = p->DiffAgarCarbon; DiffColonyO2 = p->DiffColonyO2;
+    DiffAgarO2 = p->DiffAgarO2; DiffColonyAce = p->DiffColonyAce; DiffAgarAce = p->DiffAgarAce;
+    maxGrowthRateCarAer = p->maxGrowthRateCarAer; maxGrowthRateCarAna = p->maxGrowthRateCarAna;
+    maxGrowthRateAceAer = p->maxGrowthRateAceAer;
+    Kw = p->Kw; Ka = p->Ka; Kb = p->Kb; Cb = p->Cb; c1 = p->c1;
+    qCarAernt = p->qCarAernt; qCarAnant = p->qCarAnant; qAceAernt = p->qAceAernt; qO2Carnt = p->qO2Carnt;
+    qO2Acent = p->qO2Acent; pAceAnant = p->pAceAnant;
+}
+
+struct RefEnv {
+    bcg_env_params prm;
+    RefSim* sim;
+    EnvArray3D* oldEnv;
+    AgaArray3D *agar[8], *agar_old[8];
+    AgaArray2D *wall[8], *wall_old[8];
+    DoubleArray3D *R, *R1, *R2, *RS, *RS1, *RS2, *D, *D1, *D2, *DS, *DS1, *DS2, *inside;
+    DoubleArray2D *RW, *RW1, *RW2, *RWS, *RWS1, *RWS2, *W, *W1, *W2, *WS, *WS1, *WS2, *RH;
+    DoubleArray2D* Filter;
+    DoubleArray3D* Filter3D;
+    Multigrid* mg;
+    int bounds[5];
+};
+
+static void env_globals(RefEnv* e) {
+    set_globals(&e->sim->prm);
+    set_env_globals(&e->prm);
+}
+
+void* ref_env_create(const bcg_env_params* p, void* sim) {
+    RefEnv* e = new RefEnv();
+    e->prm = *p;
+    e->sim = (RefSim*)sim;
+    env_globals(e);
+    e->oldEnv = new EnvArray3D(BoxX, BoxY, BoxZ);
+    for (int l = 0; l < maxLevelsMG; l++) {
+        e->agar[l] = new AgaArray3D(BoxX, BoxY, BoxZAgar);
+        e->agar_old[l] = new AgaArray3D(BoxX, BoxY, BoxZAgar);
+        e->wall[l] = new AgaArray2D(BoxX, BoxY);
+        e->wall_old[l] = new AgaArray2D(BoxX, BoxY);
+    }
+    // the simulation's Environment starts as constructed (LocalEnv())
+    for (int x = 0; x < BoxX; x++)
+        for (int z = 0; z < BoxZ; z++) e->sim->Env->At(x, 0, z) = LocalEnv();
+    DoubleArray3D** d3[] = {&e->R, &e->R1, &e->R2, &e->RS, &e->RS1, &e->RS2, &e->D, &e->D1, &e->D2,
+                            &e->DS, &e->DS1, &e->DS2};
+    for (auto q : d3) {
+        *q = new DoubleArray3D(BoxX, BoxY, BoxZ);
+        (*q)->Initialize(0.0);
+    }
+    e->inside = new DoubleArray3D(BoxX + 1, BoxY, BoxZ);
+    e->inside->Initialize(0.0);
+    DoubleArray2D** d2[] = {&e->RW, &e->RW1, &e->RW2, &e->RWS, &e->RWS1, &e->RWS2, &e->W, &e->W1, &e->W2,
+                            &e->WS, &e->WS1, &e->WS2};
+    for (auto q : d2) {
+        *q = new DoubleArray2D(BoxX, BoxY);
+        (*q)->Initialize(0.0);
+    }
+    e->RH = new DoubleArray2D(BoxX * refinementGridHeight, BoxY * refinementGridHeight);
+    e->RH->Initialize(cellRadius);
+    // filters exactly as ABM/Simulation.cpp:60-110 builds them
+    e->Filter = new DoubleArray2D(FilterLen, 1);
+    double value = 1.0 / (double)(FilterLen);
+    for (int ii = 0; ii < FilterLen; ii++) e->Filter->Set(ii, 0, value);
+    e->Filter3D = new DoubleArray3D(FilterLen, FilterLen, FilterLen);
+    double norm = 0;
+    for (int ii = 0; ii < FilterLen; ii++)
+        for (int kk = 0; kk < FilterLen; kk++) {
+            value = exp(-(ii - FilterLen / 2) * (ii - FilterLen / 2) - (kk - FilterLen / 2) * (kk - FilterLen / 2));
+            norm = norm + value;
+        }
+    for (int ii = 0; ii < FilterLen; ii++)
+        for (int kk = 0; kk < FilterLen; kk++) {
+            value = exp(-(ii - FilterLen / 2) * (ii - FilterLen / 2) - (kk - FilterLen / 2) * (kk - FilterLen / 2));
+            e->Filter3D->Set(ii, 0, kk, value / norm);
+        }
+    e->mg = new Multigrid(maxLevelsMG, 8, BoxX, BoxY, BoxZAgar);
+    e->mg->setInsideColony(e->inside);
+    memset(e->bounds, 0, sizeof e->bounds);
+    return e;
+}
+
+void ref_env_destroy(void* h) {
+    RefEnv* e = (RefEnv*)h;
+    env_globals(e);
+    delete e->mg;
+    delete e->oldEnv;
+    for (int l = 0; l < e->prm.maxLevelsMG; l++) {
+        delete e->agar[l];
+        delete e->agar_old[l];
+        delete e->wall[l];
+        delete e->wall_old[l];
+    }
+    for (auto q : {e->R, e->R1, e->R2, e->RS, e->RS1, e->RS2, e->D, e->D1, e->D2, e->DS, e->DS1, e->DS2, e->inside})
+        delete q;
+    for (auto q : {e->RW, e->RW1, e->RW2, e->RWS, e->RWS1, e->RWS2, e->W, e->W1, e->W2, e->WS, e->WS1, e->WS2, e->RH})
+        delete q;
+    delete e->Filter;
+    delete e->Filter3D;
+    delete e;
+}
+
+// BoxAverage, then its outputs outside the window it computes set to 0.
+static void box_average(RefEnv* e, DoubleArray3D& rough, DoubleArray3D& out, DoubleArray2D& rwall, DoubleArray2D& wout,
+                        int minx, int maxx, int miny, int maxy, int maxz) {
+    BoxAverage(rough, out, rwall, wout, *e->Filter3D, int((e->Filter3D->m_Size.x) / 2), minx, maxx, miny, maxy, maxz,
+               BoxX, BoxY, BoxZ);
+    for (int x = 0; x < BoxX; x++) {
+        const bool xin = x >= minx && x < maxx;
+        if (!xin) wout.Set(x, 0, 0.0);
+        for (int z = 0; z < BoxZ; z++)
+            if (!xin || z + 1 > maxz) out.Set(x, 0, z, 0.0);
+    }
+}
+
+// pass 0 INIT, 1 UPDATE, 2 HEIGHT (see bcg_env_fields)
+void ref_env_fields(void* h, int pass) {
+    RefEnv* e = (RefEnv*)h;
+    env_globals(e);
+    RefSim* s = e->sim;
+    int minx, maxx, miny, maxy, maxz;
+    const int FilterDim = FilterLen / 2;
+    if (pass == 2) {
+        GetHeight(*e->R, *s->Height, *s->grid, s->old_cells, minx, maxx, miny, maxy, maxz);
+        Height_Average(*s->Height, *e->RH, minx, maxx, miny, maxy);
+        Smooth(*e->RH, *s->Height, *e->Filter, FilterDim, minx, maxx, miny, maxy);
+        GetSurfaceNormal(*s->Normal, *s->Height, minx, maxx, miny, maxy);
+    } else {
+        GetDensity(*e->R, *e->R1, *e->R2, *e->inside, *s->Height, *s->grid, s->old_cells, minx, maxx, miny, maxy, maxz);
+        ShiftDensity(*e->R, *e->R1, *e->R2, *e->RW, *e->RW1, *e->RW2, *e->RS, *e->RS1, *e->RS2, *e->RWS, *e->RWS1,
+                     *e->RWS2, BoxX, BoxY, BoxZ);
+        box_average(e, *e->R, *e->D, *e->RW, *e->W, minx, maxx, miny, maxy, maxz);
+        box_average(e, *e->R1, *e->D1, *e->RW1, *e->W1, minx, maxx, miny, maxy, maxz);
+        box_average(e, *e->R2, *e->D2, *e->RW2, *e->W2, minx, maxx, miny, maxy, maxz);
+        box_average(e, *e->RS, *e->DS, *e->RWS, *e->WS, minx, maxx, miny, maxy, maxz);
+        box_average(e, *e->RS1, *e->DS1, *e->RWS1, *e->WS1, minx, maxx, miny, maxy, maxz);
+        box_average(e, *e->RS2, *e->DS2, *e->RWS2, *e->WS2, minx, maxx, miny, maxy, maxz);
+        if (pass == 1) {
+            Height_Average(*s->Height, *e->RH, minx, maxx, miny, maxy);
+            Smooth(*e->RH, *s->Height, *e->Filter, FilterDim, minx, maxx, miny, maxy);
+        }
+        GetSurfaceNormal(*s->Normal, *s->Height, minx, maxx, miny, maxy);
+    }
+    e->bounds[0] = minx;
+    e->bounds[1] = maxx;
+    e->bounds[2] = miny;
+    e->bounds[3] = maxy;
+    e->bounds[4] = maxz;
+}
+
+int ref_env_nutrients(void* h) {
+    RefEnv* e = (RefEnv*)h;
+    env_globals(e);
+    omp_set_num_threads(1);
+    RefSim* s = e->sim;
+    return UpdateEnvArrayMG_OxygenAcetate(s->Env, e->oldEnv, e->agar, e->agar_old, e->wall, e->wall_old, *e->DS,
+                                          *e->DS1, *e->DS2, *e->WS, *e->WS1, *e->WS2, e->bounds[0], e->bounds[1],
+                                          e->bounds[2], e->bounds[3], e->bounds[4], *s->Height, *e->inside, *e->mg);
+}
+
+void ref_env_get_fields(void* h, double* density, double* dshift, double* inside, double* wshift, double* height,
+                        double* normal, int* bounds) {
+    RefEnv* e = (RefEnv*)h;
+    env_globals(e);
+    for (int x = 0; x < BoxX; x++) {
+        for (int z = 0; z < BoxZ; z++) {
+            const int b = x * BoxZ + z;
+            if (density) density[b] = e->D->Get(x, 0, z);
+            if (dshift) dshift[b] = e->DS->Get(x, 0, z);
+            if (inside) inside[b] = e->inside->Get(x, 0, z);
+        }
+        if (wshift) wshift[x] = e->WS->Get(x, 0);
+    }
+    const int nh = BoxX * refinementGridHeight, ny = BoxY * refinementGridHeight;
+    for (int x = 0; x < nh; x++)
+        for (int y = 0; y < ny; y++) {
+            if (height) height[x * ny + y] = e->sim->Height->Get(x, y);
+            if (normal) {
+                DoubleCoord v = e->sim->Normal->Get(x, y);
+                normal[3 * (x * ny + y)] = v.x;
+                normal[3 * (x * ny + y) + 1] = v.y;
+                normal[3 * (x * ny + y) + 2] = v.z;
+            }
+        }
+    if (bounds) memcpy(bounds, e->bounds, sizeof e->bounds);
+}
+
+void ref_env_get_state(void* h, bcg_env_cell* colony, bcg_env_cell* colony_old, bcg_aga* agar, bcg_aga* agar_old,
+                       bcg_aga* wall, bcg_aga* wall_old) {
+    RefEnv* e = (RefEnv*)h;
+    env_globals(e);
+    for (int x = 0; x < BoxX; x++) {
+        for (int z = 0; z < BoxZ; z++) {
+            if (colony) memcpy(&colony[x * BoxZ + z], &e->sim->Env->At(x, 0, z), sizeof(bcg_env_cell));
+            if (colony_old) memcpy(&colony_old[x * BoxZ + z], &e->oldEnv->At(x, 0, z), sizeof(bcg_env_cell));
+        }
+        for (int l = 0; l < maxLevelsMG; l++) {
+            const size_t na = (size_t)BoxX * BoxZAgar;
+            for (int z = 0; z < BoxZAgar; z++) {
+                if (agar) memcpy(&agar[l * na + x * BoxZAgar + z], &e->agar[l]->At(x, 0, z), sizeof(bcg_aga));
+                if (agar_old) memcpy(&agar_old[l * na + x * BoxZAgar + z], &e->agar_old[l]->At(x, 0, z), sizeof(bcg_aga));
+            }
+            if (wall) memcpy(&wall[l * BoxX + x], &e->wall[l]->At(x, 0), sizeof(bcg_aga));
+            if (wall_old) memcpy(&wall_old[l * BoxX + x], &e->wall_old[l]->At(x, 0), sizeof(bcg_aga));
+        }
+    }
 }
 
 }  // extern "C"

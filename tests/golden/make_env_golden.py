@@ -1,0 +1,136 @@
+"""Write the golden vectors of the planar environment (density, height,
+normal, nutrients) and of a coupled run of the time loop, from the
+reference's own code compiled from source into oracle/_ref/libbcgref.so.
+
+Run in a container that has the reference checkout:
+    make -C oracle ref && python tests/golden/make_env_golden.py
+Output (committed): tests/golden/golden_env.npz - inputs (grid, times,
+seed) and the reference's outputs only. The grid is the demo input's with
+BoxX = 128, BoxZ = 16, BoxZAgar = 64 - every multigrid level keeps an exact
+2:1 ratio in x and at least two z points, so that the reference reads no
+element outside its arrays - and the demo restart state 5 (129 cells).
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from baccolonygrowth_amd.cells import CELL_DTYPE  # noqa: E402
+from baccolonygrowth_amd.colony import synthetic_fields  # noqa: E402
+from baccolonygrowth_amd.environment import env_params_from_file  # noqa: E402
+from helpers import DEMO_INPUT, GOLDEN, demo_params, restart_cells  # noqa: E402
+from oracle.oracle import RefEnv, RefLib  # noqa: E402
+
+GRID = dict(BoxX=128, BoxZ=16)
+AGAR_Z = 64
+FIELDS = ("Carbon", "Waste", "Oxygen", "Acetate", "GrowthRate1", "GrowthRate2", "GrCarAer", "GrCarAna", "GrAceAer",
+          "qCarmnt", "qAcemnt", "pAcemnt", "qOxymnt")
+
+
+def params(update_time: float):
+    p = demo_params()
+    p.BoxX, p.BoxZ = GRID["BoxX"], GRID["BoxZ"]
+    p.maxCells = 4000
+    ep = env_params_from_file(DEMO_INPUT)
+    ep.BoxX, ep.BoxZ, ep.BoxZAgar = GRID["BoxX"], GRID["BoxZ"], AGAR_Z
+    ep.UpdateTime = update_time
+    return p, ep
+
+
+def pack(prefix: str, out: dict, f: dict, st: dict | None, agar: bool = True):
+    """Fields, the colony (both buffers), the interface rows and the finest
+    agar level (its Waste never changes in this path: not stored)."""
+    for k in ("density", "density_shift", "inside", "wall_shift", "height", "normal", "bounds"):
+        if f is not None:
+            out[f"{prefix}_{k}"] = f[k]
+    if st is not None:
+        for k in FIELDS:
+            out[f"{prefix}_colony_{k}"] = st["colony"][k]
+            out[f"{prefix}_colony_old_{k}"] = st["colony_old"][k]
+        na = GRID["BoxX"] * AGAR_Z
+        for k in ("CarbonAgar", "OxygenAgar", "AcetateAgar", "WasteAgar"):
+            if agar and k != "WasteAgar":
+                out[f"{prefix}_agar_{k}"] = st["agar"][k][:na]
+            out[f"{prefix}_wall_{k}"] = st["wall"][k]
+            out[f"{prefix}_wall_old_{k}"] = st["wall_old"][k]
+
+
+def main():
+    ref = RefLib()
+    out = {"grid": np.array([GRID["BoxX"], GRID["BoxZ"], AGAR_Z])}
+    t0, dt, cells = restart_cells(5)
+
+    # 1. passes on a fixed cell state: INIT fields, nutrients, UPDATE fields,
+    #    HEIGHT fields, nutrients again (multigrid state carried over)
+    p, ep = params(1e-4)  # 2 outer iterations per nutrient update
+    sim = ref.sim(p)
+    sim.set_cells(cells)
+    env = RefEnv(ref, sim, ep)
+    env.fields(0)
+    pack("init", out, env.get_fields(), None)
+    out["n1"] = np.array(env.nutrients())
+    pack("nut1", out, None, env.get_state(), agar=False)
+    env.fields(1)
+    pack("update", out, env.get_fields(), None)
+    env.fields(2)
+    pack("height", out, env.get_fields(), None)
+    out["n2"] = np.array(env.nutrients())
+    pack("nut2", out, None, env.get_state())
+    env.close()
+    sim.close()
+
+    # 2. the coupled time loop: init pass, then steps with the clock flags
+    update_time, steps, seed = 1e-3, 25, 11
+    p, ep = params(update_time)
+    rng = np.random.default_rng(seed)
+    cells = cells.copy()
+    sel = rng.random(len(cells)) < 0.1  # these divide within the first steps
+    cells["Ldiv"][sel] = cells["Length"][sel] * (1.0 + rng.uniform(0.0, 5e-5, sel.sum()))
+    out["loop_cells_in"] = np.ascontiguousarray(cells).view(np.uint8)
+    sim = ref.sim(p)
+    sim.set_cells(cells)
+    _, _, wall, _, _ = synthetic_fields(p, cells, seed=seed)
+    sim.set_fields(wall=wall)
+    ref.srand(seed)
+    env = RefEnv(ref, sim, ep)
+    env.fields(0)
+    env.nutrients()
+    t, upd, nxt_u, hd, nxt_h = t0, True, update_time, True, update_time / 10.0
+    lineage = []
+    for _ in range(steps):
+        rebuild = False
+        if upd:
+            env.fields(1)
+            env.nutrients()
+            rebuild = True
+        if hd:
+            env.fields(2)
+        lineage += sim.step(dt, t, rebuild).tolist()
+        t += dt
+        upd = nxt_u <= 0
+        nxt_u = (update_time if upd else nxt_u) - dt
+        hd = nxt_h <= 0
+        nxt_h = (update_time / 10.0 if hd else nxt_h) - dt
+    out["loop_wall"] = wall
+    out["loop_seed"] = np.array(seed)
+    out["loop_steps"] = np.array(steps)
+    out["loop_update_time"] = np.array(update_time)
+    out["loop_cells"] = np.ascontiguousarray(sim.cells()).view(np.uint8)
+    out["loop_lineage"] = np.array(lineage, dtype=np.int32).reshape(-1, 2)
+    out["loop_t"] = np.array(t)
+    pack("loop", out, env.get_fields(), env.get_state())
+    env.close()
+    sim.close()
+    path = os.path.join(GOLDEN, "golden_env.npz")
+    np.savez_compressed(path, **out)
+    print(path, os.path.getsize(path), "bytes;", len(lineage), "divisions in the loop")
+
+
+if __name__ == "__main__":
+    main()
