@@ -6,7 +6,7 @@
 //   colony[2]            Environment, oldEnvironment         [BX][BZ] LocalEnv
 //   agar[2][ML]          FieldAgar, oldFieldAgar             [BX][BZA] LocalAga
 //   wall[2][ML]          FieldWall, oldFieldWall             [BX] LocalAga
-//   aga/res/err[8]       the nested multigrid's arrays       [nx+1][nz+1]
+//   aga/res/err[8]       the nested multigrid's arrays       3 species x [nz+1][nx+1]
 //   rough*, *shift, ...  density fields of GetDensity / ShiftDensity / BoxAverage
 // Height and Normal live in the bound mechanics context (it reads them).
 #include <hip/hip_runtime.h>
@@ -33,10 +33,14 @@ struct bcg_env {
     bcg_env_cell* colony[2] = {nullptr, nullptr};
     bcg_aga* agar[2][MAX_NEST] = {};
     bcg_aga* wall[2][MAX_NEST] = {};
-    bcg_aga* aga[MG_LEVELS] = {};
-    bcg_aga* res[MG_LEVELS] = {};
-    bcg_aga* err[MG_LEVELS] = {};
+    double* aga[MG_LEVELS] = {};
+    double* res[MG_LEVELS] = {};
+    double* err[MG_LEVELS] = {};
     int mgx[MG_LEVELS] = {}, mgz[MG_LEVELS] = {};
+    double* scol[2] = {nullptr, nullptr};   // working copies of the nutrient update
+    double* swall[2] = {nullptr, nullptr};
+    double* srow[2] = {nullptr, nullptr};
+    double *insT = nullptr, *dshT = nullptr, *phantom = nullptr;
     double *rough = nullptr, *rough1 = nullptr, *rough2 = nullptr, *inside = nullptr;
     double *rwall = nullptr, *rwall1 = nullptr, *rwall2 = nullptr;
     double *rshift = nullptr, *rshift1 = nullptr, *rshift2 = nullptr;
@@ -237,6 +241,14 @@ int bcg_env_destroy(bcg_env* e) {
         ptrs.push_back(e->res[l]);
         ptrs.push_back(e->err[l]);
     }
+    for (int b = 0; b < 2; b++) {
+        ptrs.push_back(e->scol[b]);
+        ptrs.push_back(e->swall[b]);
+        ptrs.push_back(e->srow[b]);
+    }
+    ptrs.push_back(e->insT);
+    ptrs.push_back(e->dshT);
+    ptrs.push_back(e->phantom);
     for (void* q : ptrs)
         if (q) hipFree(q);
     delete e;
@@ -282,14 +294,25 @@ int bcg_env_create(const bcg_env_params* p, bcg_ctx* mech, bcg_env** out) {
             nx /= 2;
             nz /= 2;
         }
-        if (nx < 1 || nz < 1) rc = rc ? rc : efail(e, BCG_ERR_ARG, "grid too small for 8 multigrid levels");
+        // nz reaches 0 on the coarsest level when BoxZAgar < 2^(8 - maxLevelsMG)
+        // (the demo input: 32 < 64); the reference allocates that level with
+        // one z point and its stencils then read the next column.
+        if (nx < 1) rc = rc ? rc : efail(e, BCG_ERR_ARG, "grid too small for 8 multigrid levels");
         e->mgx[l] = nx;
         e->mgz[l] = nz;
         const size_t n = (size_t)(nx + 1) * (nz + 1);
-        A(&e->aga[l], n);
-        A(&e->res[l], n);
-        A(&e->err[l], n);
+        A(&e->aga[l], 3 * n);
+        A(&e->res[l], 3 * n);
+        A(&e->err[l], 3 * n);
     }
+    for (int b = 0; b < 2; b++) {
+        A(&e->scol[b], 4 * nb);
+        A(&e->swall[b], (size_t)4 * e->ML * e->BX);
+        A(&e->srow[b], (size_t)4 * e->ML * e->BX);
+    }
+    A(&e->insT, nb);
+    A(&e->dshT, nb);
+    A(&e->phantom, (size_t)3 * ENV_THREADS);
     for (double** q : {&e->rough, &e->rough1, &e->rough2, &e->inside, &e->rshift, &e->rshift1, &e->rshift2, &e->dens,
                        &e->dens1, &e->dens2, &e->dshift, &e->d1shift, &e->d2shift})
         A(q, nb);
@@ -302,6 +325,7 @@ int bcg_env_create(const bcg_env_params* p, bcg_ctx* mech, bcg_env** out) {
     A(&e->bounds, 5);
     A(&e->errflag, 1);
     if (rc) {
+        mech->err = "bcg_env_create: " + e->err_text;  // readable through bcg_last_error
         bcg_env_destroy(e);
         return rc;
     }
@@ -322,10 +346,20 @@ int bcg_env_create(const bcg_env_params* p, bcg_ctx* mech, bcg_env** out) {
                 up(e->wall[b][l], ag.data(), (size_t)e->BX * sizeof(bcg_aga));
             }
         for (int l = 0; l < MG_LEVELS; l++) {
-            std::vector<bcg_aga> m((size_t)(e->mgx[l] + 1) * (e->mgz[l] + 1), default_aga(*p));
-            up(e->aga[l], m.data(), m.size() * sizeof(bcg_aga));
-            up(e->res[l], m.data(), m.size() * sizeof(bcg_aga));
-            up(e->err[l], m.data(), m.size() * sizeof(bcg_aga));
+            // LocalAga() in every point, as three species blocks
+            const size_t n = (size_t)(e->mgx[l] + 1) * (e->mgz[l] + 1);
+            const bcg_aga d = default_aga(*p);
+            std::vector<double> m(3 * n);
+            for (size_t q = 0; q < n; q++) {
+                m[q] = d.CarbonAgar;
+                m[n + q] = d.OxygenAgar;
+                m[2 * n + q] = d.AcetateAgar;
+            }
+            up(e->aga[l], m.data(), m.size() * sizeof(double));
+            up(e->res[l], m.data(), m.size() * sizeof(double));
+            up(e->err[l], m.data(), m.size() * sizeof(double));
+            // the uploads read m before it goes out of scope
+            if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = efail(e, BCG_ERR_HIP, "init sync failed");
         }
         std::vector<double> zero(nb, 0.0);
         for (double* q : {e->rough, e->rough1, e->rough2, e->inside, e->rshift, e->rshift1, e->rshift2, e->dens, e->dens1,
@@ -352,6 +386,7 @@ int bcg_env_create(const bcg_env_params* p, bcg_ctx* mech, bcg_env** out) {
         if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = efail(e, BCG_ERR_HIP, "init sync failed");
     }
     if (rc) {
+        mech->err = "bcg_env_create: " + e->err_text;
         bcg_env_destroy(e);
         return rc;
     }
@@ -528,8 +563,17 @@ int bcg_env_nutrients(bcg_env* e, int32_t* count) {
         a.aga[l] = e->aga[l];
         a.res[l] = e->res[l];
         a.err[l] = e->err[l];
+        a.mgx[l] = e->mgx[l];
         a.mgz[l] = e->mgz[l];
     }
+    for (int b = 0; b < 2; b++) {
+        a.scol[b] = e->scol[b];
+        a.swall[b] = e->swall[b];
+        a.srow[b] = e->srow[b];
+    }
+    a.insT = e->insT;
+    a.dshT = e->dshT;
+    a.phantom = e->phantom;
     a.inside = e->inside;
     a.dshift = e->dshift;
     a.d1shift = e->d1shift;

@@ -13,13 +13,12 @@ namespace env {
 constexpr int MG_LEVELS = 8;      // Multigrid(maxLevelsMG, 8, ...) at ABM/Simulation.cpp:150
 constexpr int MAX_NEST = 4;       // nested agar levels (maxLevelsMG) supported
 constexpr int MAX_REF = 8;        // refinementGridHeight supported
-constexpr int ENV_THREADS = 1024; // the nutrient update runs as one workgroup
+constexpr int ENV_THREADS = 512;  // the nutrient update runs as one workgroup (up to 256 VGPRs a lane)
 
 enum ErrBits {
     ERR_RANGE = 1,       // a stencil or filter would read outside an array
     ERR_CELLTYPE = 2,    // a cell of Type other than 1 or 2 (the reference prints "CellType error!")
     ERR_COLONY_EDGE = 4, // a colony point with x <= 2 (the reference asserts "Need more boxes")
-    ERR_PHANTOM = 8,     // a multigrid read past the end of its array (read as 0; see MG::at)
 };
 
 // Everything the device passes need, by value.
@@ -42,10 +41,22 @@ struct EnvArgs {
     bcg_env_cell* colony[2];          // [0] Environment, [1] oldEnvironment
     bcg_aga* agar[2][MAX_NEST];       // FieldAgar, oldFieldAgar
     bcg_aga* wall[2][MAX_NEST];       // FieldWall, oldFieldWall
-    bcg_aga* aga[MG_LEVELS];          // Multigrid::Aga
-    bcg_aga* res[MG_LEVELS];          // Multigrid::AgaRes
-    bcg_aga* err[MG_LEVELS];          // Multigrid::AgaErr
+    // Multigrid::Aga, AgaRes, AgaErr: per level the species C, O2, acetate
+    // as three blocks of (mgx+1)(mgz+1) values, each [z][x] (x fastest)
+    double* aga[MG_LEVELS];
+    double* res[MG_LEVELS];
+    double* err[MG_LEVELS];
+    int mgx[MG_LEVELS];               // x extent - 1 of each multigrid array
     int mgz[MG_LEVELS];               // z extent - 1 of each multigrid array
+    // working copies during an update, x fastest: colony C, W, O2, acetate
+    // of both buffers [4][BZ][BX]; interface rows and first agar rows of both
+    // buffers [ML][4][BX]; insideColonyDen and DensityShiftP [BZ][BX]
+    double* scol[2];
+    double* swall[2];
+    double* srow[2];
+    double* insT;
+    double* dshT;
+    double* phantom;                  // 3 values per thread: multigrid reads past an array's end
     // fields (smoothed, shifted to the grid points)
     const double* inside;             // insideColonyDen [BX][BZ]
     const double* dshift;             // DensityShiftP

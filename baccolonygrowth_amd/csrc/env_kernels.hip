@@ -292,24 +292,47 @@ __global__ void k_env_growth_to_mech(const bcg_env_cell* colony, int n, double* 
 // ======================================================== nutrient update
 namespace {
 
+constexpr int COARSE_MAX = 64;  // points of the coarsest level relaxed in LDS
+
+// Orders one wave's LDS writes before its later LDS reads (the lanes of a
+// wave run in lockstep; the fences keep the compiler from moving accesses).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Global-memory doubles: data pointers of EnvArgs read from LDS are plain
+// (flat) pointers; cast to this type, their loads and stores are global ones.
+typedef __attribute__((address_space(1))) double gdouble;
+__device__ __forceinline__ gdouble* G(double* p) { return (gdouble*)p; }
+__device__ __forceinline__ const gdouble* G(const double* p) { return (const gdouble*)p; }
+
 struct Geo {  // one multigrid level as the loops of Multigrid.cpp see it
     int nx, nz, nxp, nzp, d;
     double h2;  // BoxLength^2 d^2
 };
 
+// The three multigrid species of one point (the multigrid never touches
+// WasteAgar). Each level stores them as three blocks C, O2, acetate, each
+// [z][x] with x fastest, so that the lanes of a wave, which take
+// neighbouring x, read neighbouring words.
+struct AgaRef {
+    gdouble& CarbonAgar;
+    gdouble& OxygenAgar;
+    gdouble& AcetateAgar;
+};
+
 // Multigrid arrays: (x extent, z extent) per level from the constructor
-// (ABM/Multigrid.cpp:5-33); index x*(nz+1) + z.
+// (ABM/Multigrid.cpp:5-33).
 struct MG {
     const EnvArgs& a;
     int AL;  // agarLevels = maxLevelsMG
     double dt;
-    __device__ MG(const EnvArgs& args, double dt_) : a(args), AL(args.ML), dt(dt_) {}
+    gdouble* phantom;  // this thread's 3 values for reads past the end of an array
+    __device__ MG(const EnvArgs& args, double dt_, gdouble* ph) : a(args), AL(args.ML), dt(dt_), phantom(ph) {}
 
-    __device__ int xext(int l) const {  // x extent - 1
-        int nx = a.BX;
-        for (int i = AL; i <= l; i++) nx /= 2;
-        return nx;
-    }
+    __device__ int xext(int l) const { return a.mgx[l]; }  // x extent - 1
     __device__ Geo geo(int l, int sub) const {  // sub: loop extents BoxX*d0/d/sub
         Geo g;
         g.d = 1 << l;
@@ -321,107 +344,110 @@ struct MG {
         g.h2 = a.BoxLength * a.BoxLength * g.d * g.d;
         return g;
     }
-    // Element (i, k) with the reference's flat index x*(nz+1) + z: a z past
-    // the top of a level reads the next column, as Array3D::Index does. A
-    // flat index outside the array (the reference reads past the end of the
-    // coarsest error array when BoxZAgar < 2^(8 - maxLevelsMG)) reads zero.
-    mutable bcg_aga phantom;
-    __device__ bcg_aga& at(bcg_aga* const* arr, int l, int i, int k) const {
-        const int X = xext(l), Z = a.mgz[l];
-        const long f = (long)i * (Z + 1) + k;
-        if (f < 0 || f >= (long)(X + 1) * (Z + 1)) {
-            flag(a.errflag, ERR_PHANTOM);
-            phantom.CarbonAgar = phantom.WasteAgar = phantom.OxygenAgar = phantom.AcetateAgar = 0.0;
-            return phantom;
+    // Element (i, k) as the reference addresses it, by the flat index
+    // x*(nz+1) + z of Array3D::Index: a z past the top of a level reads the
+    // next column. A flat index outside the array (the reference reads past
+    // the end of the coarsest error array when BoxZAgar < 2^(8 - maxLevelsMG))
+    // reads zero.
+    __device__ __forceinline__ AgaRef at(double* const* arr, int l, int i, int k) const {
+        const int X1 = a.mgx[l] + 1, Z1 = a.mgz[l] + 1, n = X1 * Z1;
+        int p;
+        if (i >= 0 && i < X1 && k >= 0 && k < Z1) {
+            p = k * X1 + i;
+        } else {
+            const int f = i * Z1 + k;
+            if (f < 0 || f >= n) {
+                phantom[0] = phantom[1] = phantom[2] = 0.0;
+                return {phantom[0], phantom[1], phantom[2]};
+            }
+            const int ii = f / Z1;
+            p = (f - ii * Z1) * X1 + ii;
         }
-        return arr[l][f];
+        gdouble* b = G(arr[l]);
+        return {b[p], b[p + n], b[p + 2 * n]};
     }
-    __device__ bcg_aga& A(int l, int i, int k) const { return at(a.aga, l, i, k); }
-    __device__ bcg_aga& R(int l, int i, int k) const { return at(a.res, l, i, k); }
-    __device__ bcg_aga& E(int l, int i, int k) const { return at(a.err, l, i, k); }
+    __device__ __forceinline__ AgaRef A(int l, int i, int k) const { return at(a.aga, l, i, k); }
+    __device__ __forceinline__ AgaRef R(int l, int i, int k) const { return at(a.res, l, i, k); }
+    __device__ __forceinline__ AgaRef E(int l, int i, int k) const { return at(a.err, l, i, k); }
     // insideColonyDen(x, 0, 0); x past the last box reads as 0
-    __device__ double inside0(int x) const { return (x >= 0 && x < a.BX) ? a.inside[x * a.BZ] : 0.0; }
+    __device__ double inside0(int x) const { return (x >= 0 && x < a.BX) ? G(a.insT)[x] : 0.0; }
 };
 
-// (t + (p + q))            relaxation at corners
-__device__ __forceinline__ double g2(double t, double p, double q, double den) { return (t + (p + q)) / den; }
-// (t + e + 0.5 p + 0.5 q)
-__device__ __forceinline__ double g3a(double t, double e, double p, double q, double den) {
-    return (t + e + 0.5 * p + 0.5 * q) / den;
-}
-// (t + 0.5 p + 0.5 q + e)
-__device__ __forceinline__ double g3b(double t, double p, double q, double e, double den) {
-    return (t + 0.5 * p + 0.5 * q + e) / den;
-}
-// (t + 0.5 (p + q + r + s)) spelled term by term
-__device__ __forceinline__ double g4(double t, double p, double q, double r, double s, double den) {
-    return (t + 0.5 * p + 0.5 * q + 0.5 * r + 0.5 * s) / den;
-}
+// One species of one level, [z][x]: element (i, k) at p[k * X1 + i] (global
+// memory, or LDS for the coarsest level).
+template <class T>
+struct Col {
+    T* p;
+    int X1;
+};
 
-// relaxationAgarElementwise (ABM/Multigrid.cpp:161-249).
-__device__ void relax_point(const MG& m, int l, int i, int k, int nx, int nz) {
-    const EnvArgs& a = m.a;
-    const double h2 = a.BoxLength * a.BoxLength * (double)(1 << (2 * l));
-    const int d = 1 << l;
-    const double dt = m.dt;
-    const double DC = a.DiffAgarCarbon, DO = a.DiffAgarO2, DA = a.DiffAgarAce;
-    bcg_aga& e = m.E(l, i, k);
-    const bcg_aga r = m.R(l, i, k);
-    const double tC = r.CarbonAgar * h2 / dt / DC, nC = h2 / dt / DC + 2;
-    const double tA = r.AcetateAgar * h2 / dt / DA, nA = h2 / dt / DA + 2;
-    const bool outer = l >= m.AL - 1;
-    if (k == 0) {
-        const int i0 = min(max((i - nx / 2) * d + a.BX / 2, 0), a.BX - 1);
-        if (a.inside[i0 * a.BZ] > 0) {
-            e.CarbonAgar = r.CarbonAgar;
-            e.OxygenAgar = r.OxygenAgar;
-            e.AcetateAgar = r.AcetateAgar;
-        } else if (i == 0 || i == nx) {
-            if (outer) {
-                const int in = i == 0 ? i + 1 : i - 1;
-                e.CarbonAgar = g2(tC, m.E(l, in, 0).CarbonAgar, m.E(l, i, 1).CarbonAgar, nC);
-                e.OxygenAgar = r.OxygenAgar;
-                e.AcetateAgar = g2(tA, m.E(l, in, 0).AcetateAgar, m.E(l, i, 1).AcetateAgar, nA);
-            }
-        } else {
-            e.CarbonAgar = g3a(tC, m.E(l, i, 1).CarbonAgar, m.E(l, i - 1, 0).CarbonAgar, m.E(l, i + 1, 0).CarbonAgar, nC);
-            e.OxygenAgar = r.OxygenAgar;
-            e.AcetateAgar =
-                g3a(tA, m.E(l, i, 1).AcetateAgar, m.E(l, i - 1, 0).AcetateAgar, m.E(l, i + 1, 0).AcetateAgar, nA);
+// relaxationAgarElementwise (ABM/Multigrid.cpp:161-249) down one column
+// (k ascending) for one species: the reference's branches and operand
+// order, with e(k-1) carried in a register and the loads that do not depend
+// on it (R, the two neighbour columns of the other colour, the old e(k+1))
+// prefetched RING steps ahead. Nothing else writes this column during a
+// colour pass, so the values match the sequential k loop exactly.
+template <class T>
+__device__ __forceinline__ void relax_column(const Col<T> E, const Col<T> R, int i, int nx, int nz, bool isO, double D, double h2,
+                             double dt, bool outer, bool dir0) {
+    const bool edge = (i == 0 || i == nx);
+    if (edge && !outer && !dir0) return;  // nothing in this column changes
+    const int X1 = E.X1;
+    const int il = edge ? (i == 0 ? 1 : i - 1) : i - 1;  // E(i-1, .) or the inner column
+    const int ih = edge ? i : i + 1;                     // E(i+1, .), interior only
+    // e(k+1) past the top is the next column's k = 0 (the flat index), and
+    // past the last column nothing (read as 0)
+    const int inext = min(i + 1, nx);
+    const bool next_ok = i + 1 <= nx;
+    const double nn = h2 / dt / D + 2;
+    constexpr int RING = 6;
+    double qr[RING], ql[RING], qh[RING], qn[RING];
+    // loads without branches (so that they stay in flight across steps):
+    // past the column end and on the edge an in-range element is loaded and
+    // not used
+    auto fetch = [&](int k, int j) {
+        const int kk = min(k, nz);
+        qr[j] = R.p[kk * X1 + i];
+        ql[j] = E.p[kk * X1 + il];
+        qh[j] = E.p[kk * X1 + ih];
+        qn[j] = E.p[kk < nz ? (kk + 1) * X1 + i : inext];
+    };
+#pragma unroll
+    for (int j = 0; j < RING; j++) fetch(j, j);
+    double ek = E.p[i], em1 = 0.0;
+    // steps k0 .. k0+RING-1 use ring slots 0 .. RING-1 (fixed registers: a
+    // load stays in flight until its slot is used RING steps later). The
+    // step body has no branches, so the loads' completion can be counted
+    // exactly; steps past the column end store the last value again.
+    for (int k0 = 0; k0 <= nz; k0 += RING) {
+#pragma unroll
+        for (int j = 0; j < RING; j++) {
+            const int k = k0 + j;
+            const double r = qr[j], L = ql[j], H = qh[j], ekp1 = (k < nz || next_ok) ? qn[j] : 0.0;
+            fetch(k + RING, j);
+            const double t = r * h2 / dt / D;
+            // the numerator of the branch taken, in the reference's order of
+            // operations - corners t + (p + q); edges t + e + p/2 + q/2 or
+            // t + p/2 + q/2 + e; interior t + p/2 + q/2 + r/2 + s/2 - chosen
+            // without branches; then one division
+            const bool bot = k == 0, top = !bot && k == nz;
+            const double n2 = t + (L + (bot ? ekp1 : em1));
+            const double n3e = t + 0.5 * em1 + 0.5 * ekp1 + L;
+            const double e3 = bot ? ekp1 : em1, p3 = bot ? L : H, q3 = bot ? H : L;
+            const double n3 = t + e3 + 0.5 * p3 + 0.5 * q3;
+            const double n4 = t + 0.5 * em1 + 0.5 * ekp1 + 0.5 * L + 0.5 * H;
+            const double num = edge ? ((bot || top) ? n2 : n3e) : ((bot || top) ? n3 : n4);
+            const double q = num / nn;
+            // oxygen keeps r on the boundary rows and columns; a column or
+            // a top point outside the outer level keeps its value
+            const bool keepR = bot ? (dir0 || isO) : (edge && isO);
+            const bool keep = !bot && !outer && (top || edge);
+            const bool live = k <= nz;
+            const double v = live ? (keep ? ek : (keepR ? r : q)) : em1;
+            E.p[min(k, nz) * X1 + i] = v;
+            em1 = v;
+            ek = ekp1;
         }
-    } else if (k == nz) {
-        if (outer) {
-            if (i == 0 || i == nx) {
-                const int in = i == 0 ? i + 1 : i - 1;
-                e.CarbonAgar = g2(tC, m.E(l, in, k).CarbonAgar, m.E(l, i, k - 1).CarbonAgar, nC);
-                e.OxygenAgar = r.OxygenAgar;
-                e.AcetateAgar = g2(tA, m.E(l, in, k).AcetateAgar, m.E(l, i, k - 1).AcetateAgar, nA);
-            } else {
-                const double tO = r.OxygenAgar * h2 / dt / DO, nO = h2 / dt / DO + 2;
-                e.CarbonAgar = g3a(tC, m.E(l, i, k - 1).CarbonAgar, m.E(l, i + 1, k).CarbonAgar,
-                                   m.E(l, i - 1, k).CarbonAgar, nC);
-                e.OxygenAgar = g3a(tO, m.E(l, i, k - 1).OxygenAgar, m.E(l, i + 1, k).OxygenAgar,
-                                   m.E(l, i - 1, k).OxygenAgar, nO);
-                e.AcetateAgar = g3a(tA, m.E(l, i, k - 1).AcetateAgar, m.E(l, i + 1, k).AcetateAgar,
-                                    m.E(l, i - 1, k).AcetateAgar, nA);
-            }
-        }
-    } else if (i == 0 || i == nx) {
-        if (outer) {
-            const int in = i == 0 ? i + 1 : i - 1;
-            e.CarbonAgar = g3b(tC, m.E(l, i, k - 1).CarbonAgar, m.E(l, i, k + 1).CarbonAgar, m.E(l, in, k).CarbonAgar, nC);
-            e.OxygenAgar = r.OxygenAgar;
-            e.AcetateAgar =
-                g3b(tA, m.E(l, i, k - 1).AcetateAgar, m.E(l, i, k + 1).AcetateAgar, m.E(l, in, k).AcetateAgar, nA);
-        }
-    } else {
-        const double tO = r.OxygenAgar * h2 / dt / DO, nO = h2 / dt / DO + 2;
-        e.CarbonAgar = g4(tC, m.E(l, i, k - 1).CarbonAgar, m.E(l, i, k + 1).CarbonAgar, m.E(l, i - 1, k).CarbonAgar,
-                          m.E(l, i + 1, k).CarbonAgar, nC);
-        e.OxygenAgar = g4(tO, m.E(l, i, k - 1).OxygenAgar, m.E(l, i, k + 1).OxygenAgar, m.E(l, i - 1, k).OxygenAgar,
-                          m.E(l, i + 1, k).OxygenAgar, nO);
-        e.AcetateAgar = g4(tA, m.E(l, i, k - 1).AcetateAgar, m.E(l, i, k + 1).AcetateAgar,
-                           m.E(l, i - 1, k).AcetateAgar, m.E(l, i + 1, k).AcetateAgar, nA);
     }
 }
 
@@ -439,17 +465,17 @@ __device__ __forceinline__ double rr_c(double Rp, double S, double D, double dt,
 
 // Species accessors for the three multigrid species.
 struct SpC {
-    __device__ static double& v(bcg_aga& g) { return g.CarbonAgar; }
+    __device__ static gdouble& v(const AgaRef& g) { return g.CarbonAgar; }
 };
 struct SpO {
-    __device__ static double& v(bcg_aga& g) { return g.OxygenAgar; }
+    __device__ static gdouble& v(const AgaRef& g) { return g.OxygenAgar; }
 };
 struct SpA {
-    __device__ static double& v(bcg_aga& g) { return g.AcetateAgar; }
+    __device__ static gdouble& v(const AgaRef& g) { return g.AcetateAgar; }
 };
 
 // restrictionAgaElementwise (ABM/Multigrid.cpp:270-405).
-__device__ void restrict_point(const MG& m, int l, int i, int k, const Geo& g) {
+__device__ __forceinline__ void restrict_point(const MG m, int l, int i, int k, const Geo g) {
     const EnvArgs& a = m.a;
     const double dt = m.dt, h2 = g.h2;
     const int nx = g.nx, nz = g.nz, nxp = g.nxp, nzp = g.nzp;
@@ -457,11 +483,11 @@ __device__ void restrict_point(const MG& m, int l, int i, int k, const Geo& g) {
     const int ip = 2 * i - nx + nxp / 2, kp = 2 * k;
     const int lp = l - 1;
     const double DC = a.DiffAgarCarbon, DO = a.DiffAgarO2, DA = a.DiffAgarAce;
-    bcg_aga& out = m.R(l, i, k);
-    auto E = [&](int ii, int kk) -> bcg_aga& { return m.E(lp, ii, kk); };
-    auto Rp = [&](int ii, int kk) -> bcg_aga& { return m.R(lp, ii, kk); };
-    auto Ap = [&](int ii, int kk) -> bcg_aga& { return m.A(lp, ii, kk); };
-    auto Al = [&](int ii, int kk) -> bcg_aga& { return m.A(l, ii, kk); };
+    const AgaRef out = m.R(l, i, k);
+    auto E = [&](int ii, int kk) -> AgaRef { return m.E(lp, ii, kk); };
+    auto Rp = [&](int ii, int kk) -> AgaRef { return m.R(lp, ii, kk); };
+    auto Ap = [&](int ii, int kk) -> AgaRef { return m.A(lp, ii, kk); };
+    auto Al = [&](int ii, int kk) -> AgaRef { return m.A(l, ii, kk); };
     if (i == 0 || i == nx || k == nz) {
         if (l >= m.AL) {
             if (k == nz && i > 0 && i < nx) {
@@ -611,11 +637,11 @@ __device__ void restrict_point(const MG& m, int l, int i, int k, const Geo& g) {
 }
 
 // interpolationAgaElementwise (ABM/Multigrid.cpp:427-467): bilinear, added.
-__device__ void interp_point(const MG& m, int l, int i, int k, const Geo& g) {
-    bcg_aga& e = m.E(l, i, k);
+__device__ void interp_point(const MG m, int l, int i, int k, const Geo g) {
+    const AgaRef e = m.E(l, i, k);
     const int nx = g.nx, nxp = g.nxp;
     const int ip = nxp / 2 - nx / 4 + (i % 2 == 0 ? i / 2 : (i - 1) / 2);
-    auto C = [&](int ii, int kk) -> bcg_aga& { return m.E(l + 1, ii, kk); };
+    auto C = [&](int ii, int kk) -> AgaRef { return m.E(l + 1, ii, kk); };
     if (i % 2 == 0) {
         if (k % 2 == 0) {
             const int kp = k / 2;
@@ -647,16 +673,16 @@ __device__ void interp_point(const MG& m, int l, int i, int k, const Geo& g) {
 }
 
 // updateSolAgaElementwise (ABM/Multigrid.cpp:539-562): u += e, clamped.
-__device__ void update_sol_point(const MG& m, int l, int i, int k, const Geo& g) {
+__device__ void update_sol_point(const MG m, int l, int i, int k, const Geo g) {
     const EnvArgs& a = m.a;
-    bcg_aga& r = m.R(l, i, k);
+    const AgaRef r = m.R(l, i, k);
     r.CarbonAgar = 0;
     r.OxygenAgar = 0;
     r.AcetateAgar = 0;
     const int ip = 2 * i - g.nx + g.nxp / 2, kp = 2 * k;
-    bcg_aga& e = m.E(l, i, k);
+    const AgaRef e = m.E(l, i, k);
     if (l < m.AL && (l == 0 || ip <= 0 || ip >= g.nxp || kp >= g.nzp)) {
-        bcg_aga& u = m.A(l, i, k);
+        const AgaRef u = m.A(l, i, k);
         u.CarbonAgar += e.CarbonAgar;
         u.OxygenAgar += e.OxygenAgar;
         u.AcetateAgar += e.AcetateAgar;
@@ -671,16 +697,16 @@ __device__ void update_sol_point(const MG& m, int l, int i, int k, const Geo& g)
 
 // updateResAgaElementwise (ABM/Multigrid.cpp:564-689): residual dt D Lap u
 // of the nested levels with the composite-boundary neighbours.
-__device__ void update_res_point(const MG& m, int l, int i, int k, const Geo& g) {
+__device__ __forceinline__ void update_res_point(const MG m, int l, int i, int k, const Geo g) {
     const EnvArgs& a = m.a;
     if (l >= m.AL) return;
     const double dt = m.dt, h2 = g.h2;
     const int nx = g.nx, nz = g.nz, nxp = g.nxp, nzp = g.nzp;
     const int ip = 2 * i - nx + nxp / 2, kp = 2 * k;
     const double DC = a.DiffAgarCarbon, DO = a.DiffAgarO2, DA = a.DiffAgarAce;
-    bcg_aga& out = m.R(l, i, k);
-    auto U = [&](int ii, int kk) -> bcg_aga& { return m.A(l, ii, kk); };
-    auto Up = [&](int ii, int kk) -> bcg_aga& { return m.A(l - 1, ii, kk); };
+    const AgaRef out = m.R(l, i, k);
+    auto U = [&](int ii, int kk) -> AgaRef { return m.A(l, ii, kk); };
+    auto Up = [&](int ii, int kk) -> AgaRef { return m.A(l - 1, ii, kk); };
     if (i == 0 || i == nx) {
         if (l == m.AL - 1) {
             const int s = i == 0 ? 1 : -1;
@@ -784,11 +810,11 @@ __device__ void update_res_point(const MG& m, int l, int i, int k, const Geo& g)
 
 // NestSolAgaElementwise (ABM/Multigrid.cpp:711-724): inject the finer nested
 // level where it covers this one.
-__device__ void nest_point(const MG& m, int l, int i, int k, const Geo& g) {
+__device__ void nest_point(const MG m, int l, int i, int k, const Geo g) {
     const int ip = 2 * i - g.nx + g.nxp / 2, kp = 2 * k;
     if (l < m.AL && ip > 0 && ip < g.nxp && kp < g.nzp) {
-        bcg_aga& u = m.A(l, i, k);
-        const bcg_aga& f = m.A(l - 1, ip, kp);
+        const AgaRef u = m.A(l, i, k);
+        const AgaRef f = m.A(l - 1, ip, kp);
         u.CarbonAgar = f.CarbonAgar;
         u.OxygenAgar = f.OxygenAgar;
         u.AcetateAgar = f.AcetateAgar;
@@ -831,74 +857,93 @@ __device__ double acetate_buffer(const EnvArgs& a, double Cac, bcg_env_cell& p) 
     return (Cac - y);
 }
 
+// Per-update constants of the uptake kinetics (ABM/Nutrients.cpp:244-266).
+struct ChemConst {
+    double mgrCarAers, mgrCarAnas, mgrAceAers, g1s, g2s, g3s;
+};
+__device__ ChemConst chem_const(const EnvArgs& a) {
+    ChemConst c;
+    if (a.maxGrowthRateCarAer != 0.0) {
+        c.mgrCarAers = a.qCarAernt / a.qCarAer;
+        c.g1s = a.KC_aer * (c.mgrCarAers / a.maxGrowthRateCarAer);
+    } else {
+        c.mgrCarAers = 0;
+        c.g1s = 1e6;
+    }
+    if (a.maxGrowthRateCarAna != 0.0) {
+        c.mgrCarAnas = a.qCarAnant / a.qCarAna;
+        c.g2s = a.KC_ana * (c.mgrCarAnas / a.maxGrowthRateCarAna);
+    } else {
+        c.mgrCarAnas = 0;
+        c.g2s = 1e6;
+    }
+    if (a.maxGrowthRateAceAer != 0.0) {
+        c.mgrAceAers = a.qAceAernt / a.qAceAer;
+        c.g3s = a.KA * (c.mgrAceAers / a.maxGrowthRateAceAer);
+    } else {
+        c.mgrAceAers = 0;
+        c.g3s = 1e6;
+    }
+    return c;
+}
+
+// Working copies of the colony (fields C, W, O2, acetate in the order of
+// bcg_aga, each [z][x]) and of interface or agar rows ([field][x]) during an
+// update: the lanes of a wave take neighbouring x.
+enum { FC = 0, FW = 1, FO = 2, FA = 3 };
+struct ColS {
+    gdouble* p;
+    int BX, NB;
+    __device__ gdouble& operator()(int f, int x, int z) const { return p[f * NB + z * BX + x]; }
+};
+struct RowS {
+    gdouble* p;
+    int BX;
+    __device__ gdouble& operator()(int f, int x) const { return p[f * BX + x]; }
+};
+
 // UpdateEnvironment_OxygenAcetate (ABM/Nutrients.cpp:200-506), non-cross-
-// feeding chemistry, one colony box.
-__device__ void colony_point(const EnvArgs& a, bcg_env_cell* Env, bcg_env_cell* prev, const bcg_aga* prevWal, int x,
-                             int z) {
-    const int BZ = a.BZ;
-    const int b = x * BZ + z;
-    const double ins = a.inside[b];
-    auto P = [&](int xx, int zz) -> bcg_env_cell& { return prev[xx * BZ + zz]; };
-    auto I = [&](int xx, int zz) { return a.inside[xx * BZ + zz]; };
-    bcg_env_cell& out = Env[b];
+// feeding chemistry, one colony box. The reference also writes, in every
+// sweep, fields that no later sweep reads: the pathway growth rates and
+// maintenance terms of the box (into Env) and the acetate buffer state
+// (acetateBufferSolve, into the previous buffer; its first call, on the box's
+// own acetate, is overwritten by the second, on the vertex average). Those
+// are computed when `full` is set: in the last two sweeps of an update, whose
+// writes are the ones that remain in the two buffers.
+__device__ __forceinline__ void colony_point(const EnvArgs& a, const ChemConst& cc, const ColS cur, const ColS prev, const RowS pw,
+                             bcg_env_cell* Env, bcg_env_cell* prevEnv, int x, int z, bool full) {
+    const int BX = a.BX;
+    const gdouble* insT = G(a.insT);
+    auto I = [&](int xx, int zz) -> double { return insT[zz * BX + xx]; };
+    const double ins = I(x, z);
     if (ins > 0.5) {
-        const double C = P(x, z).Carbon, W = P(x, z).Waste, A = P(x, z).Acetate, O = P(x, z).Oxygen;
-        (void)acetate_buffer(a, A, P(x, z));
-        const double fC = C / (C + a.KC_aer);
-        const double fW = W / (W + a.KW);
-        (void)fC;
-        (void)fW;
-        (void)O;
-        const double fI = 0;
+        // fC, fW and fI of the box itself enter nothing (fI = 0)
         double Cgr = 0, Wgr = 0, Agr = 0, Ogr = 0;
-        double mgrCarAers, mgrCarAnas, mgrAceAers, g1s, g2s, g3s;
-        if (a.maxGrowthRateCarAer != 0.0) {
-            mgrCarAers = a.qCarAernt / a.qCarAer;
-            g1s = a.KC_aer * (mgrCarAers / a.maxGrowthRateCarAer);
-        } else {
-            mgrCarAers = 0;
-            g1s = 1e6;
-        }
-        if (a.maxGrowthRateCarAna != 0.0) {
-            mgrCarAnas = a.qCarAnant / a.qCarAna;
-            g2s = a.KC_ana * (mgrCarAnas / a.maxGrowthRateCarAna);
-        } else {
-            mgrCarAnas = 0;
-            g2s = 1e6;
-        }
-        if (a.maxGrowthRateAceAer != 0.0) {
-            mgrAceAers = a.qAceAernt / a.qAceAer;
-            g3s = a.KA * (mgrAceAers / a.maxGrowthRateAceAer);
-        } else {
-            mgrAceAers = 0;
-            g3s = 1e6;
-        }
+        const double mgrCarAers = cc.mgrCarAers, mgrCarAnas = cc.mgrCarAnas, mgrAceAers = cc.mgrAceAers;
+        const double g1s = cc.g1s, g2s = cc.g2s, g3s = cc.g3s;
         if (x != 0) {
             // vertex averages of the four boxes (the wall row below z = 0)
             if (z == 0) {
-                const bcg_aga &wm = prevWal[x - 1], &w0 = prevWal[x];
-                Cgr = (wm.CarbonAgar + P(x - 1, z).Carbon + w0.CarbonAgar + P(x, z).Carbon) / 4.0;
-                Wgr = (wm.WasteAgar + P(x - 1, z).Waste + w0.WasteAgar + P(x, z).Waste) / 4.0;
-                Ogr = (wm.OxygenAgar + P(x - 1, z).Oxygen + w0.OxygenAgar + P(x, z).Oxygen) / 4.0;
-                Agr = (wm.AcetateAgar + P(x - 1, z).Acetate + w0.AcetateAgar + P(x, z).Acetate) / 4.0;
+                Cgr = (pw(FC, x - 1) + prev(FC, x - 1, z) + pw(FC, x) + prev(FC, x, z)) / 4.0;
+                Ogr = (pw(FO, x - 1) + prev(FO, x - 1, z) + pw(FO, x) + prev(FO, x, z)) / 4.0;
+                Agr = (pw(FA, x - 1) + prev(FA, x - 1, z) + pw(FA, x) + prev(FA, x, z)) / 4.0;
             } else {
-                Cgr = (P(x - 1, z - 1).Carbon + P(x - 1, z).Carbon + P(x, z - 1).Carbon + P(x, z).Carbon) / 4.0;
-                Wgr = (P(x - 1, z - 1).Waste + P(x - 1, z).Waste + P(x, z - 1).Waste + P(x, z).Waste) / 4.0;
-                Ogr = (P(x - 1, z - 1).Oxygen + P(x - 1, z).Oxygen + P(x, z - 1).Oxygen + P(x, z).Oxygen) / 4.0;
-                Agr = (P(x - 1, z - 1).Acetate + P(x - 1, z).Acetate + P(x, z - 1).Acetate + P(x, z).Acetate) / 4.0;
+                Cgr = (prev(FC, x - 1, z - 1) + prev(FC, x - 1, z) + prev(FC, x, z - 1) + prev(FC, x, z)) / 4.0;
+                Ogr = (prev(FO, x - 1, z - 1) + prev(FO, x - 1, z) + prev(FO, x, z - 1) + prev(FO, x, z)) / 4.0;
+                Agr = (prev(FA, x - 1, z - 1) + prev(FA, x - 1, z) + prev(FA, x, z - 1) + prev(FA, x, z)) / 4.0;
             }
         }
-        (void)acetate_buffer(a, Agr, P(x, z));
+        bcg_env_cell& out = Env[x * a.BZ + z];
+        if (full) (void)acetate_buffer(a, Agr, prevEnv[x * a.BZ + z]);
         const double fCgr_aer = Cgr / (Cgr + a.KC_aer);
         const double fCgr_ana = Cgr / (Cgr + a.KC_ana);
         const double fAgr = Agr / (Agr + a.KA);
         const double fOgr = Ogr / (Ogr + a.KO);
         const double fIgr = 0;
-        (void)fI;
         (void)Wgr;
-        double l1 = smax((a.maxGrowthRateCarAer + mgrCarAers) * fCgr_aer - mgrCarAers, 0.0);
-        double l2 = smax((a.maxGrowthRateCarAna + mgrCarAnas) * fCgr_ana - mgrCarAnas, 0.0);
-        double l3 = smax((a.maxGrowthRateAceAer + mgrAceAers) * fAgr - mgrAceAers, 0.0);
+        const double l1 = smax((a.maxGrowthRateCarAer + mgrCarAers) * fCgr_aer - mgrCarAers, 0.0);
+        const double l2 = smax((a.maxGrowthRateCarAna + mgrCarAnas) * fCgr_ana - mgrCarAnas, 0.0);
+        const double l3 = smax((a.maxGrowthRateAceAer + mgrAceAers) * fAgr - mgrAceAers, 0.0);
         const double inh = smax(1 - fIgr, 0.0);
         const double qO = (a.qO2Car * l1 * fCgr_aer + a.qO2Ace * l3 * (1 - fCgr_aer)) * fOgr * inh +
                           a.qO2Carnt * smin((Cgr / g1s), 1.0) * fCgr_aer * fOgr +
@@ -910,145 +955,241 @@ __device__ void colony_point(const EnvArgs& a, bcg_env_cell* Env, bcg_env_cell* 
                           a.qAceAernt * smin((Agr / g3s), 1.0) * (1 - fCgr_aer) * fOgr;
         const double pA = (a.pAceAer * l1 * fCgr_aer * fOgr + a.pAceAna * l2 * (1 - fOgr)) * inh +
                           a.pAceAnant * smin((Cgr / g2s), 1.0) * (1 - fOgr);
-        out.qCarmnt = a.qCarAernt * smin((Cgr / g1s), 1.0) * fCgr_aer * fOgr + a.qCarAnant * smin((Cgr / g2s), 1.0) * (1 - fOgr);
-        out.qAcemnt = a.qAceAernt * smin((Agr / g3s), 1.0) * (1 - fCgr_aer) * fOgr;
-        out.pAcemnt = a.pAceAnant * smin((Cgr / g2s), 1.0) * (1 - fOgr);
-        out.qOxymnt = a.qO2Carnt * smin((Cgr / g1s), 1.0) * fCgr_aer * fOgr + a.qO2Acent * smin((Agr / g3s), 1.0) * (1 - fCgr_aer) * fOgr;
+        if (full) {
+            out.qCarmnt = a.qCarAernt * smin((Cgr / g1s), 1.0) * fCgr_aer * fOgr + a.qCarAnant * smin((Cgr / g2s), 1.0) * (1 - fOgr);
+            out.qAcemnt = a.qAceAernt * smin((Agr / g3s), 1.0) * (1 - fCgr_aer) * fOgr;
+            out.pAcemnt = a.pAceAnant * smin((Cgr / g2s), 1.0) * (1 - fOgr);
+            out.qOxymnt = a.qO2Carnt * smin((Cgr / g1s), 1.0) * fCgr_aer * fOgr + a.qO2Acent * smin((Agr / g3s), 1.0) * (1 - fCgr_aer) * fOgr;
+            // l1..l3 are recomputed by the reference with the same operands: same values
+            out.GrCarAer = l1 * fCgr_aer * fOgr * inh;
+            out.GrCarAna = l2 * (1 - fOgr) * inh;
+            out.GrAceAer = l3 * (1 - fCgr_aer) * fOgr * inh;
+            out.GrowthRate1 = out.GrCarAer + out.GrCarAna + out.GrAceAer;
+            out.GrowthRate2 = out.GrowthRate1;
+        }
         const double pW = a.Carbon_to_Waste * qC;
         const double qW = 0.0;
-        l1 = smax((a.maxGrowthRateCarAer + mgrCarAers) * fCgr_aer - mgrCarAers, 0.0);
-        l2 = smax((a.maxGrowthRateCarAna + mgrCarAnas) * fCgr_ana - mgrCarAnas, 0.0);
-        l3 = smax((a.maxGrowthRateAceAer + mgrAceAers) * fAgr - mgrAceAers, 0.0);
-        out.GrCarAer = l1 * fCgr_aer * fOgr * inh;
-        out.GrCarAna = l2 * (1 - fOgr) * inh;
-        out.GrAceAer = l3 * (1 - fCgr_aer) * fOgr * inh;
-        out.GrowthRate1 = out.GrCarAer + out.GrCarAna + out.GrAceAer;
-        out.GrowthRate2 = out.GrowthRate1;
         if (!(x > 2)) flag(a.errflag, ERR_COLONY_EDGE);
         const double BL2 = a.BoxLength * a.BoxLength;
+        const double cC = prev(FC, x, z), cW = prev(FW, x, z), cO = prev(FO, x, z), cA = prev(FA, x, z);
         double Czz, Wzz, Azz, Ozz;
-        const bcg_env_cell &c = P(x, z), &up = P(x, z + 1);
         if (z == 0) {
-            const bcg_aga& w = prevWal[x];
-            Wzz = (up.Waste + w.WasteAgar - 2.0 * c.Waste) / BL2;
-            Czz = (up.Carbon + w.CarbonAgar - 2.0 * c.Carbon) / BL2;
-            Azz = (up.Acetate + w.AcetateAgar - 2.0 * c.Acetate) / BL2;
-            Ozz = (up.Oxygen + w.OxygenAgar - 2.0 * c.Oxygen) / BL2;
+            Wzz = (prev(FW, x, z + 1) + pw(FW, x) - 2.0 * cW) / BL2;
+            Czz = (prev(FC, x, z + 1) + pw(FC, x) - 2.0 * cC) / BL2;
+            Azz = (prev(FA, x, z + 1) + pw(FA, x) - 2.0 * cA) / BL2;
+            Ozz = (prev(FO, x, z + 1) + pw(FO, x) - 2.0 * cO) / BL2;
         } else {
-            const bcg_env_cell& dn = P(x, z - 1);
-            Wzz = (up.Waste + dn.Waste - 2.0 * c.Waste) / BL2;
-            Czz = (up.Carbon + dn.Carbon - 2.0 * c.Carbon) / BL2;
-            Azz = (up.Acetate + dn.Acetate - 2.0 * c.Acetate) / BL2;
-            Ozz = (up.Oxygen + dn.Oxygen - 2.0 * c.Oxygen) / BL2;
+            Wzz = (prev(FW, x, z + 1) + prev(FW, x, z - 1) - 2.0 * cW) / BL2;
+            Czz = (prev(FC, x, z + 1) + prev(FC, x, z - 1) - 2.0 * cC) / BL2;
+            Azz = (prev(FA, x, z + 1) + prev(FA, x, z - 1) - 2.0 * cA) / BL2;
+            Ozz = (prev(FO, x, z + 1) + prev(FO, x, z - 1) - 2.0 * cO) / BL2;
         }
-        const bcg_env_cell &xp = P(x + 1, z), &xm = P(x - 1, z);
-        const double Cxx = (xp.Carbon + xm.Carbon - 2.0 * c.Carbon) / BL2;
-        const double Wxx = (xp.Waste + xm.Waste - 2.0 * c.Waste) / BL2;
-        const double Axx = (xp.Acetate + xm.Acetate - 2.0 * c.Acetate) / BL2;
-        const double Oxx = (xp.Oxygen + xm.Oxygen - 2.0 * c.Oxygen) / BL2;
-        const double Dp = a.dshift[b];
-        const double Cnew = c.Carbon + (a.DiffColonyCarbon * (Cxx + Czz) - qC * 0.25 * Dp) * a.Cdt;
-        const double CAcenew = c.Acetate + (a.DiffColonyAce * (Axx + Azz) + pA * 0.25 * Dp - qA * 0.25 * Dp) * a.Cdt;
-        const double Onew = c.Oxygen + (a.DiffColonyO2 * (Oxx + Ozz) - qO * 0.25 * Dp) * a.Cdt;
-        out.Carbon = smax(0.0, smin(Cnew, a.maxCarbon));
-        out.Oxygen = smax(0.0, smin(Onew, a.maxO2));
-        out.Acetate = smax(0.0, CAcenew);
-        const double Wnew = c.Waste + (a.DiffColony * (Wxx + Wzz) + pW * Dp - qW * Dp) * a.Cdt;
-        out.Waste = smax(0.0, Wnew);
+        const double Cxx = (prev(FC, x + 1, z) + prev(FC, x - 1, z) - 2.0 * cC) / BL2;
+        const double Wxx = (prev(FW, x + 1, z) + prev(FW, x - 1, z) - 2.0 * cW) / BL2;
+        const double Axx = (prev(FA, x + 1, z) + prev(FA, x - 1, z) - 2.0 * cA) / BL2;
+        const double Oxx = (prev(FO, x + 1, z) + prev(FO, x - 1, z) - 2.0 * cO) / BL2;
+        const double Dp = G(a.dshT)[z * BX + x];
+        const double Cnew = cC + (a.DiffColonyCarbon * (Cxx + Czz) - qC * 0.25 * Dp) * a.Cdt;
+        const double CAcenew = cA + (a.DiffColonyAce * (Axx + Azz) + pA * 0.25 * Dp - qA * 0.25 * Dp) * a.Cdt;
+        const double Onew = cO + (a.DiffColonyO2 * (Oxx + Ozz) - qO * 0.25 * Dp) * a.Cdt;
+        cur(FC, x, z) = smax(0.0, smin(Cnew, a.maxCarbon));
+        cur(FO, x, z) = smax(0.0, smin(Onew, a.maxO2));
+        cur(FA, x, z) = smax(0.0, CAcenew);
+        const double Wnew = cW + (a.DiffColony * (Wxx + Wzz) + pW * Dp - qW * Dp) * a.Cdt;
+        cur(FW, x, z) = smax(0.0, Wnew);
     } else if (ins > 0.3) {
         // colony surface: mean of the neighbouring colony boxes (x, z; at
         // z = 0 without the box below)
         const bool top = z > 0;
         const int ne = (I(x + 1, z) > 0.5) + (I(x - 1, z) > 0.5) + (I(x, z + 1) > 0.5) + (top ? (I(x, z - 1) > 0.5) : 0);
-        auto mean = [&](double bcg_env_cell::*f) {
-            double s = P(x + 1, z).*f * (I(x + 1, z) > 0.5) + P(x - 1, z).*f * (I(x - 1, z) > 0.5) +
-                       P(x, z + 1).*f * (I(x, z + 1) > 0.5);
-            if (top) s = s + P(x, z - 1).*f * (I(x, z - 1) > 0.5);
+        auto mean = [&](int f) -> double {
+            double s = prev(f, x + 1, z) * (I(x + 1, z) > 0.5) + prev(f, x - 1, z) * (I(x - 1, z) > 0.5) +
+                       prev(f, x, z + 1) * (I(x, z + 1) > 0.5);
+            if (top) s = s + prev(f, x, z - 1) * (I(x, z - 1) > 0.5);
             return s / ne;
         };
-        const double Cnew = mean(&bcg_env_cell::Carbon);
-        const double Wnew = mean(&bcg_env_cell::Waste);
-        const double CAcenew = mean(&bcg_env_cell::Acetate);
-        out.Carbon = smax(0.0, smin(Cnew, a.maxCarbon));
-        out.Oxygen = smax(0.0, smin(a.maxO2, a.maxO2));
-        out.Acetate = smax(0.0, CAcenew);
-        out.Waste = smax(0.0, Wnew);
+        const double Cnew = mean(FC);
+        const double Wnew = mean(FW);
+        const double CAcenew = mean(FA);
+        cur(FC, x, z) = smax(0.0, smin(Cnew, a.maxCarbon));
+        cur(FO, x, z) = smax(0.0, smin(a.maxO2, a.maxO2));
+        cur(FA, x, z) = smax(0.0, CAcenew);
+        cur(FW, x, z) = smax(0.0, Wnew);
     }
 }
 
 // UpdateWallMultigrid_OxygenAcetate (ABM/Nutrients.cpp:509-659) at level l,
-// box x: the colony-agar interface row.
-__device__ void wall_point(const EnvArgs& a, const bcg_env_cell* Env, bcg_aga* const* prevAga, bcg_aga* const* Wal,
-                           bcg_aga* const* prevWal, int l, int x) {
+// box x: the colony-agar interface row. env: the colony being written this
+// sweep; row: the first agar row of the previous buffer; wal / pwal: the
+// interface rows of this and of the previous buffer (levels 4 BX apart).
+__device__ __forceinline__ void wall_point(const EnvArgs& a, const ColS env, double* row, double* wal, double* pwal,
+                                           int l, int x) {
     const int BX = a.BX;
-    bcg_aga& out = Wal[l][x];
-    if (l == 0 && a.inside[x * a.BZ] > 0.5) {
-        const bcg_env_cell& e = Env[x * a.BZ];
-        const bcg_aga& g = prevAga[0][x * a.BZA];
-        const double Cnew = (a.DiffColonyCarbon * e.Carbon + a.DiffAgarCarbon * g.CarbonAgar) / (a.DiffColonyCarbon + a.DiffAgarCarbon);
-        const double Onew = (a.DiffColonyO2 * e.Oxygen + a.DiffAgarO2 * g.OxygenAgar) / (a.DiffColonyO2 + a.DiffAgarO2);
-        out.CarbonAgar = smax(0.0, smin(Cnew, a.maxCarbon));
-        out.OxygenAgar = smax(0.0, smin(Onew, a.maxO2));
-        const double CAcenew = (a.DiffColonyAce * e.Acetate + a.DiffAgarAce * g.AcetateAgar) / (a.DiffColonyAce + a.DiffAgarAce);
-        out.AcetateAgar = smax(0.0, CAcenew);
-        const double Wnew = (a.DiffColony * e.Waste + a.DiffAgar * g.WasteAgar) / (a.DiffColony + a.DiffAgar);
-        out.WasteAgar = smax(0.0, Wnew);
+    const RowS out{G(wal) + l * 4 * BX, BX}, W{G(pwal) + l * 4 * BX, BX};
+    const RowS g{G(row) + l * 4 * BX, BX};
+    if (l == 0 && G(a.insT)[x] > 0.5) {
+        const double eC = env(FC, x, 0), eW = env(FW, x, 0), eO = env(FO, x, 0), eA = env(FA, x, 0);
+        const double Cnew = (a.DiffColonyCarbon * eC + a.DiffAgarCarbon * g(FC, x)) / (a.DiffColonyCarbon + a.DiffAgarCarbon);
+        const double Onew = (a.DiffColonyO2 * eO + a.DiffAgarO2 * g(FO, x)) / (a.DiffColonyO2 + a.DiffAgarO2);
+        out(FC, x) = smax(0.0, smin(Cnew, a.maxCarbon));
+        out(FO, x) = smax(0.0, smin(Onew, a.maxO2));
+        const double CAcenew = (a.DiffColonyAce * eA + a.DiffAgarAce * g(FA, x)) / (a.DiffColonyAce + a.DiffAgarAce);
+        out(FA, x) = smax(0.0, CAcenew);
+        const double Wnew = (a.DiffColony * eW + a.DiffAgar * g(FW, x)) / (a.DiffColony + a.DiffAgar);
+        out(FW, x) = smax(0.0, Wnew);
         return;
     }
     const double hl = a.BoxLength * a.BoxLength * (double)(1 << (2 * l));
-    const bcg_aga* W = prevWal[l];
     // covered interior of a coarser level: the reference leaves the x term
     // uninitialised there (values overwritten by the next copyFromMG); 0 here
     double Cxx = 0, Wxx = 0, Axx = 0;
     if (l > 0 && x >= BX / 4 - 1 && x <= BX * 3 / 4) {
-        const bcg_aga* Wf = prevWal[l - 1];
+        const RowS Wf{G(pwal) + (l - 1) * 4 * BX, BX};
         if (x == BX / 4 - 1) {
-            Wxx = (W[x - 1].WasteAgar + Wf[0].WasteAgar - 2 * W[x].WasteAgar) / hl;
-            Cxx = (W[x - 1].CarbonAgar + Wf[0].CarbonAgar - 2 * W[x].CarbonAgar) / hl;
-            Axx = (W[x - 1].AcetateAgar + Wf[0].AcetateAgar - 2 * W[x].AcetateAgar) / hl;
+            Wxx = (W(FW, x - 1) + Wf(FW, 0) - 2 * W(FW, x)) / hl;
+            Cxx = (W(FC, x - 1) + Wf(FC, 0) - 2 * W(FC, x)) / hl;
+            Axx = (W(FA, x - 1) + Wf(FA, 0) - 2 * W(FA, x)) / hl;
         } else if (x == BX * 3 / 4) {
-            Wxx = (W[x + 1].WasteAgar + Wf[BX - 2].WasteAgar - 2 * W[x].WasteAgar) / hl;
-            Cxx = (W[x + 1].CarbonAgar + Wf[BX - 2].CarbonAgar - 2 * W[x].CarbonAgar) / hl;
-            Axx = (W[x + 1].AcetateAgar + Wf[BX - 2].AcetateAgar - 2 * W[x].AcetateAgar) / hl;
+            Wxx = (W(FW, x + 1) + Wf(FW, BX - 2) - 2 * W(FW, x)) / hl;
+            Cxx = (W(FC, x + 1) + Wf(FC, BX - 2) - 2 * W(FC, x)) / hl;
+            Axx = (W(FA, x + 1) + Wf(FA, BX - 2) - 2 * W(FA, x)) / hl;
         }
     } else if (l < a.ML - 1 && (x == 0 || x == BX - 1)) {
         // coarser level outside: y parity 0 (BoxY = 1), weights 1 and 0
         const int ly = 0;
-        const bcg_aga* Wc = prevWal[l + 1];
+        const RowS Wc{G(pwal) + (l + 1) * 4 * BX, BX};
         const double w1 = 1.0 / (ly + 1), w2 = ly * 1.0 / (ly + 1);
         if (x == 0) {
-            const bcg_aga &c1 = Wc[BX / 4 - 1], &c2 = Wc[BX / 4 - 1];
-            Wxx = (W[x + 2].WasteAgar + w1 * c1.WasteAgar + w2 * c2.WasteAgar - 2 * W[x].WasteAgar) / 4 / hl;
-            Cxx = (W[x + 2].CarbonAgar + w1 * c1.CarbonAgar + w2 * c2.CarbonAgar - 2 * W[x].CarbonAgar) / 4 / hl;
-            Axx = (W[x + 2].AcetateAgar + w1 * c1.AcetateAgar + w2 * c2.AcetateAgar - 2 * W[x].AcetateAgar) / 4 / hl;
+            const int c = BX / 4 - 1;
+            Wxx = (W(FW, x + 2) + w1 * Wc(FW, c) + w2 * Wc(FW, c) - 2 * W(FW, x)) / 4 / hl;
+            Cxx = (W(FC, x + 2) + w1 * Wc(FC, c) + w2 * Wc(FC, c) - 2 * W(FC, x)) / 4 / hl;
+            Axx = (W(FA, x + 2) + w1 * Wc(FA, c) + w2 * Wc(FA, c) - 2 * W(FA, x)) / 4 / hl;
         } else {
-            const bcg_aga &c1 = Wc[BX * 3 / 4], &c2 = Wc[BX * 3 / 4];
-            Wxx = (W[x - 1].WasteAgar + w1 * c1.WasteAgar + w2 * c2.WasteAgar - 2 * W[x].WasteAgar) / hl;
-            Cxx = (W[x - 1].CarbonAgar + w1 * c1.CarbonAgar + w2 * c2.CarbonAgar - 2 * W[x].CarbonAgar) / hl;
-            Axx = (W[x - 1].AcetateAgar + w1 * c1.AcetateAgar + w2 * c2.AcetateAgar - 2 * W[x].AcetateAgar) / hl;
+            const int c = BX * 3 / 4;
+            Wxx = (W(FW, x - 1) + w1 * Wc(FW, c) + w2 * Wc(FW, c) - 2 * W(FW, x)) / hl;
+            Cxx = (W(FC, x - 1) + w1 * Wc(FC, c) + w2 * Wc(FC, c) - 2 * W(FC, x)) / hl;
+            Axx = (W(FA, x - 1) + w1 * Wc(FA, c) + w2 * Wc(FA, c) - 2 * W(FA, x)) / hl;
         }
     } else if (l == a.ML - 1 && (x == 0 || x == BX - 1)) {
         const int n = x == 0 ? x + 1 : x - 1;
-        Wxx = (W[n].WasteAgar - 1.0 * W[x].WasteAgar) / hl;
-        Cxx = (W[n].CarbonAgar - 1.0 * W[x].CarbonAgar) / hl;
-        Axx = (W[n].AcetateAgar - 1.0 * W[x].AcetateAgar) / hl;
+        Wxx = (W(FW, n) - 1.0 * W(FW, x)) / hl;
+        Cxx = (W(FC, n) - 1.0 * W(FC, x)) / hl;
+        Axx = (W(FA, n) - 1.0 * W(FA, x)) / hl;
     } else {
-        Wxx = (W[x - 1].WasteAgar + W[x + 1].WasteAgar - 2 * W[x].WasteAgar) / hl;
-        Cxx = (W[x - 1].CarbonAgar + W[x + 1].CarbonAgar - 2 * W[x].CarbonAgar) / hl;
-        Axx = (W[x - 1].AcetateAgar + W[x + 1].AcetateAgar - 2 * W[x].AcetateAgar) / hl;
+        Wxx = (W(FW, x - 1) + W(FW, x + 1) - 2 * W(FW, x)) / hl;
+        Cxx = (W(FC, x - 1) + W(FC, x + 1) - 2 * W(FC, x)) / hl;
+        Axx = (W(FA, x - 1) + W(FA, x + 1) - 2 * W(FA, x)) / hl;
     }
-    const bcg_aga& g = prevAga[l][x * a.BZA];
-    const double Wzz = 2.0 * (g.WasteAgar - W[x].WasteAgar) / hl;
-    const double Czz = 2.0 * (g.CarbonAgar - W[x].CarbonAgar) / hl;
-    const double Azz = 2.0 * (g.AcetateAgar - W[x].AcetateAgar) / hl;
-    const double Cnew = W[x].CarbonAgar + a.DiffAgarCarbon * (Cxx + Czz) * a.Cdt;
+    const double Wzz = 2.0 * (g(FW, x) - W(FW, x)) / hl;
+    const double Czz = 2.0 * (g(FC, x) - W(FC, x)) / hl;
+    const double Azz = 2.0 * (g(FA, x) - W(FA, x)) / hl;
+    const double Cnew = W(FC, x) + a.DiffAgarCarbon * (Cxx + Czz) * a.Cdt;
     const double Onew = a.maxO2;
-    out.CarbonAgar = smax(0.0, smin(Cnew, a.maxCarbon));
-    out.OxygenAgar = smax(0.0, smin(Onew, a.maxO2));
-    const double CAcenew = W[x].AcetateAgar + a.DiffAgarAce * (Axx + Azz) * a.Cdt;
-    out.AcetateAgar = smax(0.0, CAcenew);
-    const double Wnew = W[x].WasteAgar + a.DiffAgar * (Wxx + Wzz) * a.Cdt;
-    out.WasteAgar = smax(0.0, Wnew);
+    out(FC, x) = smax(0.0, smin(Cnew, a.maxCarbon));
+    out(FO, x) = smax(0.0, smin(Onew, a.maxO2));
+    const double CAcenew = W(FA, x) + a.DiffAgarAce * (Axx + Azz) * a.Cdt;
+    out(FA, x) = smax(0.0, CAcenew);
+    const double Wnew = W(FW, x) + a.DiffAgar * (Wxx + Wzz) * a.Cdt;
+    out(FW, x) = smax(0.0, Wnew);
+}
+
+// relaxationAgarAtLevel (ABM/Multigrid.cpp:135-159): the columns of parity
+// 0, then of parity 1, each column k ascending; a thread per (species,
+// column).
+__device__ __forceinline__ void relax_level(const EnvArgs& a, const MG m, int l, int tid) {
+    const Geo g = m.geo(l, 1);
+    const int X1 = a.mgx[l] + 1, npts = X1 * (a.mgz[l] + 1);
+    const bool outer = l >= m.AL - 1;
+    const double h2 = a.BoxLength * a.BoxLength * (double)(1 << (2 * l));
+    const int d = 1 << l;
+    const gdouble* insT = G(a.insT);
+    gdouble* E = G(a.err[l]);
+    gdouble* R = G(a.res[l]);
+    for (int par = 0; par < 2; par++) {
+        const int ncol = par <= g.nx ? (g.nx - par) / 2 + 1 : 0;
+        for (int w = tid; w < 3 * ncol; w += ENV_THREADS) {
+            const int sp = w / ncol, i = par + 2 * (w - sp * ncol);
+            const int i0 = min(max((i - g.nx / 2) * d + a.BX / 2, 0), a.BX - 1);
+            const double D = sp == 0 ? a.DiffAgarCarbon : (sp == 1 ? a.DiffAgarO2 : a.DiffAgarAce);
+            relax_column(Col<gdouble>{E + sp * npts, X1}, Col<gdouble>{R + sp * npts, X1}, i, g.nx, g.nz, sp == 1, D,
+                         h2, m.dt, outer, insT[i0] > 0);
+        }
+        __syncthreads();
+    }
+}
+
+// The 100 relaxations of the coarsest level (Multigrid::Vcycle): in wave 0,
+// out of LDS, when the level fits (no workgroup barrier between the colour
+// passes).
+__device__ __forceinline__ void coarse_solve(const EnvArgs& a, const MG m, int tid, double* cs_e, double* cs_r) {
+    const int l = MG_LEVELS - 1;
+    const Geo g = m.geo(l, 1);
+    const int X1 = g.nx + 1, npts = X1 * (g.nz + 1);
+    if (npts > COARSE_MAX || 3 * (g.nx / 2 + 1) > 64) {
+        for (int j = 0; j < 100; j++) relax_level(a, m, l, tid);
+        return;
+    }
+    if (tid < 64) {
+        gdouble* E = G(a.err[l]);
+        const gdouble* R = G(a.res[l]);
+        for (int f = tid; f < npts; f += 64)
+            for (int sp = 0; sp < 3; sp++) {
+                cs_e[sp * COARSE_MAX + f] = E[sp * npts + f];
+                cs_r[sp * COARSE_MAX + f] = R[sp * npts + f];
+            }
+        wave_sync();
+        const bool outer = l >= m.AL - 1;
+        const double h2 = a.BoxLength * a.BoxLength * (double)(1 << (2 * l));
+        const int d = 1 << l;
+        const gdouble* insT = G(a.insT);
+        for (int j = 0; j < 100; j++) {
+            for (int par = 0; par < 2; par++) {
+                const int ncol = par <= g.nx ? (g.nx - par) / 2 + 1 : 0;
+                if (tid < 3 * ncol) {
+                    const int sp = tid / ncol, i = par + 2 * (tid - sp * ncol);
+                    const int i0 = min(max((i - g.nx / 2) * d + a.BX / 2, 0), a.BX - 1);
+                    const double D = sp == 0 ? a.DiffAgarCarbon : (sp == 1 ? a.DiffAgarO2 : a.DiffAgarAce);
+                    relax_column(Col<double>{cs_e + sp * COARSE_MAX, X1}, Col<double>{cs_r + sp * COARSE_MAX, X1}, i,
+                                 g.nx, g.nz, sp == 1, D, h2, m.dt, outer, insT[i0] > 0);
+                }
+                wave_sync();
+            }
+        }
+        for (int f = tid; f < npts; f += 64)
+            for (int sp = 0; sp < 3; sp++) E[sp * npts + f] = cs_e[sp * COARSE_MAX + f];
+    }
+    __syncthreads();
+}
+
+// Every point of a level, x fastest across the lanes (the passes that use it
+// write each point from values no other point of the pass writes).
+template <class F>
+__device__ __forceinline__ void over(const Geo g, int tid, F fn) {
+    const int n = (g.nx + 1) * (g.nz + 1);
+    for (int w = tid; w < n; w += ENV_THREADS) {
+        const int k = w / (g.nx + 1);
+        fn(w - k * (g.nx + 1), k);
+    }
+}
+
+// Multigrid::reset's zeroing of the error (all levels) and of the coarse
+// residuals.
+__device__ __forceinline__ void reset_zero(const MG m, int tid) {
+    for (int l = 0; l < MG_LEVELS; l++) {
+        over(m.geo(l, 1), tid, [=](int i, int k) {
+            const AgaRef e = m.E(l, i, k);
+            e.CarbonAgar = 0;
+            e.OxygenAgar = 0;
+            e.AcetateAgar = 0;
+            if (l > 0) {
+                const AgaRef r = m.R(l, i, k);
+                r.CarbonAgar = 0;
+                r.OxygenAgar = 0;
+                r.AcetateAgar = 0;
+            }
+        });
+    }
+    __syncthreads();
 }
 
 }  // namespace
@@ -1056,122 +1197,119 @@ __device__ void wall_point(const EnvArgs& a, const bcg_env_cell* Env, bcg_aga* c
 // The whole UpdateEnvArrayMG_OxygenAcetate in one workgroup.
 __global__ void __launch_bounds__(ENV_THREADS) k_env_nutrients(EnvArgs args) {
     // the pointer tables are indexed by level and buffer at run time: keep
-    // them in LDS rather than in per-lane scratch
+    // them in LDS rather than in per-lane scratch (data pointers are cast to
+    // gdouble* where used, so that accesses are global, not flat)
     __shared__ EnvArgs a;
+    __shared__ double cs_e[3 * COARSE_MAX], cs_r[3 * COARSE_MAX];
     const int tid = threadIdx.x;
     if (tid == 0) a = args;
     __syncthreads();
-    const MG m(a, 100.0);  // dt = 100 (ABM/Multigrid.cpp:993)
+    const ChemConst cc = chem_const(a);
+    const MG m(a, 100.0, G(a.phantom) + 3 * tid);  // dt = 100 (ABM/Multigrid.cpp:993)
     const int AL = a.ML;
-    const int BX = a.BX;
-    auto sync = [] { __syncthreads(); };
-    // level geometry for the loops (sub 1: restriction/update; 2: interpolation)
-    auto G1 = [&](int l) { return m.geo(l, 1); };
-    auto G2 = [&](int l) { return m.geo(l, 2); };
-    // one colour pass of relaxationAgarAtLevel: columns of parity `par`
-    auto relax = [&](int l) {
-        const Geo g = G1(l);
-        for (int par = 0; par < 2; par++) {
-            for (int c = tid; 2 * c + par <= g.nx; c += ENV_THREADS) {
-                const int i = 2 * c + par;
-                for (int k = 0; k <= g.nz; k++) relax_point(m, l, i, k, g.nx, g.nz);
-            }
-            sync();
+    const int BX = a.BX, BZ = a.BZ, NB = BX * BZ, RW = 4 * BX;
+
+    // working copies: inside / density shift [z][x]; colony fields; the
+    // interface rows and the first agar row of each nested level
+    for (int w = tid; w < NB; w += ENV_THREADS) {
+        const int z = w / BX, x = w - z * BX, b = x * BZ + z;
+        G(a.insT)[w] = a.inside[b];
+        G(a.dshT)[w] = a.dshift[b];
+        for (int bb = 0; bb < 2; bb++) {
+            const bcg_env_cell& c = a.colony[bb][b];
+            gdouble* d = G(a.scol[bb]);
+            d[FC * NB + w] = c.Carbon;
+            d[FW * NB + w] = c.Waste;
+            d[FO * NB + w] = c.Oxygen;
+            d[FA * NB + w] = c.Acetate;
         }
-    };
-    auto over = [&](const Geo& g, auto fn) {
-        const int n = (g.nx + 1) * (g.nz + 1);
-        for (int w = tid; w < n; w += ENV_THREADS) fn(w / (g.nz + 1), w % (g.nz + 1));
-    };
-    auto reset_zero = [&]() {
-        for (int l = 0; l < MG_LEVELS; l++) {
-            const Geo g = G1(l);
-            over(g, [&](int i, int k) {
-                bcg_aga& e = m.E(l, i, k);
-                e.CarbonAgar = 0;
-                e.OxygenAgar = 0;
-                e.AcetateAgar = 0;
-                if (l > 0) {
-                    bcg_aga& r = m.R(l, i, k);
-                    r.CarbonAgar = 0;
-                    r.OxygenAgar = 0;
-                    r.AcetateAgar = 0;
-                }
-            });
+    }
+    for (int w = tid; w < AL * BX; w += ENV_THREADS) {
+        const int l = w / BX, x = w - l * BX;
+        for (int bb = 0; bb < 2; bb++) {
+            const bcg_aga& q = a.wall[bb][l][x];
+            const bcg_aga& r = a.agar[bb][l][x * a.BZA];
+            gdouble* dw = G(a.swall[bb]) + l * RW;
+            gdouble* dr = G(a.srow[bb]) + l * RW;
+            dw[FC * BX + x] = q.CarbonAgar;
+            dw[FW * BX + x] = q.WasteAgar;
+            dw[FO * BX + x] = q.OxygenAgar;
+            dw[FA * BX + x] = q.AcetateAgar;
+            dr[FC * BX + x] = r.CarbonAgar;
+            dr[FW * BX + x] = r.WasteAgar;
+            dr[FO * BX + x] = r.OxygenAgar;
+            dr[FA * BX + x] = r.AcetateAgar;
         }
-        sync();
-    };
-    auto update_sol = [&]() {
-        for (int l = 0; l <= AL - 1; l++) {
-            const Geo g = G1(l);
-            over(g, [&](int i, int k) { update_sol_point(m, l, i, k, g); });
-        }
-        sync();
-    };
-    auto update_res = [&]() {
-        for (int l = 0; l <= AL - 1; l++) {
-            const Geo g = G1(l);
-            over(g, [&](int i, int k) { update_res_point(m, l, i, k, g); });
-        }
-        sync();
-    };
+    }
+    __syncthreads();
 
     int count = 0;
     int sw = 0;  // 0: Current = Environment / FieldAgar / FieldWall
     for (int outer = 0; outer < a.n_outer; outer++) {
-        bcg_aga* const* curAgar = a.agar[sw];
-        bcg_aga* const* prevAgar = a.agar[sw ^ 1];
-        bcg_aga* const* curWall = a.wall[sw];
-        bcg_aga* const* prevWall = a.wall[sw ^ 1];
         // --- Multigrid::solve(CurrentAgar, CurrentWall, PreviousAgar, PreviousWall, 3, 100, 100)
         // copyToMG: colony footprint from the wall, and the last column of the
         // outermost nested level from its first
-        for (int w = tid; w < BX - 1; w += ENV_THREADS) {
-            const int i = w + 1;
-            if (a.inside[i * a.BZ] > 0) {
-                bcg_aga& u = m.A(0, i, 0);
-                const bcg_aga& s = curWall[0][i];
-                u.CarbonAgar = s.CarbonAgar;
-                u.OxygenAgar = s.OxygenAgar;
-                u.AcetateAgar = s.AcetateAgar;
+        {
+            const RowS cw{G(a.swall[sw]), BX};
+            const gdouble* insT = G(a.insT);
+            for (int w = tid; w < BX - 1; w += ENV_THREADS) {
+                const int i = w + 1;
+                if (insT[i] > 0) {
+                    const AgaRef u = m.A(0, i, 0);
+                    u.CarbonAgar = cw(FC, i);
+                    u.OxygenAgar = cw(FO, i);
+                    u.AcetateAgar = cw(FA, i);
+                }
             }
         }
         for (int k = tid; k <= a.BZA; k += ENV_THREADS) {
-            bcg_aga& u = m.A(AL - 1, BX, k);
-            const bcg_aga& s = m.A(AL - 1, 0, k);
+            const AgaRef u = m.A(AL - 1, BX, k);
+            const AgaRef s = m.A(AL - 1, 0, k);
             u.CarbonAgar = s.CarbonAgar;
             u.OxygenAgar = s.OxygenAgar;
             u.AcetateAgar = s.AcetateAgar;
         }
-        sync();
-        // setup
-        reset_zero();
-        update_sol();
-        update_res();
+        __syncthreads();
+        // setup: reset, updateSolAga, updateResAga
+        reset_zero(m, tid);
+        for (int l = 0; l <= AL - 1; l++) {
+            const Geo g = m.geo(l, 1);
+            over(g, tid, [=](int i, int k) { update_sol_point(m, l, i, k, g); });
+        }
+        __syncthreads();
+        for (int l = 0; l <= AL - 1; l++) {
+            const Geo g = m.geo(l, 1);
+            over(g, tid, [=](int i, int k) { update_res_point(m, l, i, k, g); });
+        }
+        __syncthreads();
         for (int cyc = 0; cyc < 3; cyc++) {
             for (int l = 0; l < MG_LEVELS - 1; l++) {
-                for (int j = 0; j < 3; j++) relax(l);
-                const Geo g = G1(l + 1);
-                over(g, [&](int i, int k) { restrict_point(m, l + 1, i, k, g); });
-                sync();
+                for (int j = 0; j < 3; j++) relax_level(a, m, l, tid);
+                const Geo g = m.geo(l + 1, 1);
+                over(g, tid, [=](int i, int k) { restrict_point(m, l + 1, i, k, g); });
+                __syncthreads();
             }
-            for (int j = 0; j < 100; j++) relax(MG_LEVELS - 1);
+            coarse_solve(a, m, tid, cs_e, cs_r);
             for (int l = MG_LEVELS - 2; l >= 0; l--) {
-                const Geo g = G2(l);
-                over(g, [&](int i, int k) { interp_point(m, l, i, k, g); });
-                sync();
-                for (int j = 0; j < 3; j++) relax(l);
+                const Geo g = m.geo(l, 2);
+                over(g, tid, [=](int i, int k) { interp_point(m, l, i, k, g); });
+                __syncthreads();
+                for (int j = 0; j < 3; j++) relax_level(a, m, l, tid);
             }
         }
-        // reset
-        update_sol();
+        // reset: updateSolAga, rewriteSolBdryAga, updateResAga, zeroing, NestSolAga
+        for (int l = 0; l <= AL - 1; l++) {
+            const Geo g = m.geo(l, 1);
+            over(g, tid, [=](int i, int k) { update_sol_point(m, l, i, k, g); });
+        }
+        __syncthreads();
         for (int l = 0; l < AL - 1; l++) {
-            const Geo g = G2(l);
+            const Geo g = m.geo(l, 2);
             const int nx = g.nx, nz = g.nz, nx2 = g.nxp;
             for (int i = tid; i <= nx; i += ENV_THREADS) {
                 const int i2 = nx2 / 2 + i / 2 - nx / 4, i2p1 = nx2 / 2 + (i + 1) / 2 - nx / 4;
-                bcg_aga& u = m.A(l, i, nz);
-                const bcg_aga &p = m.A(l + 1, i2, nz / 2), &q = m.A(l + 1, i2p1, nz / 2);
+                const AgaRef u = m.A(l, i, nz);
+                const AgaRef p = m.A(l + 1, i2, nz / 2), q = m.A(l + 1, i2p1, nz / 2);
                 u.CarbonAgar = (p.CarbonAgar + q.CarbonAgar) / 2;
                 u.OxygenAgar = (p.OxygenAgar + q.OxygenAgar) / 2;
                 u.AcetateAgar = (p.AcetateAgar + q.AcetateAgar) / 2;
@@ -1180,73 +1318,112 @@ __global__ void __launch_bounds__(ENV_THREADS) k_env_nutrients(EnvArgs args) {
                 for (int side = 0; side < 2; side++) {
                     const int i = side ? nx : 0;
                     const int i2 = nx2 / 2 + i / 2 - nx / 4;
-                    bcg_aga& u = m.A(l, i, k);
-                    const bcg_aga &p = m.A(l + 1, i2, k / 2), &q = m.A(l + 1, i2, (k + 1) / 2);
+                    const AgaRef u = m.A(l, i, k);
+                    const AgaRef p = m.A(l + 1, i2, k / 2), q = m.A(l + 1, i2, (k + 1) / 2);
                     u.CarbonAgar = (p.CarbonAgar + q.CarbonAgar) / 2;
                     u.OxygenAgar = (p.OxygenAgar + q.OxygenAgar) / 2;
                     u.AcetateAgar = (p.AcetateAgar + q.AcetateAgar) / 2;
                 }
             }
-            sync();
+            __syncthreads();
         }
-        update_res();
-        reset_zero();
+        for (int l = 0; l <= AL - 1; l++) {
+            const Geo g = m.geo(l, 1);
+            over(g, tid, [=](int i, int k) { update_res_point(m, l, i, k, g); });
+        }
+        __syncthreads();
+        reset_zero(m, tid);
         for (int l = 1; l <= AL; l++) {
-            const Geo g = G1(l);
-            over(g, [&](int i, int k) { nest_point(m, l, i, k, g); });
-            sync();
+            const Geo g = m.geo(l, 1);
+            over(g, tid, [=](int i, int k) { nest_point(m, l, i, k, g); });
+            __syncthreads();
         }
-        // copyFromMG into both buffers of the nested levels
+        // copyFromMG into both buffers of the nested levels: the interface
+        // rows (k = 0) here, the agar (k >= 1) when the update ends (nothing
+        // reads it before then but the first agar row, kept here too)
         for (int l = 0; l < AL; l++) {
-            for (int w = tid; w < BX * (a.BZA + 1); w += ENV_THREADS) {
-                const int i = w / (a.BZA + 1), k = w % (a.BZA + 1);
-                const bcg_aga& s = m.A(l, i, k);
-                bcg_aga& d1 = k == 0 ? curWall[l][i] : curAgar[l][i * a.BZA + k - 1];
-                bcg_aga& d2 = k == 0 ? prevWall[l][i] : prevAgar[l][i * a.BZA + k - 1];
-                d1.CarbonAgar = s.CarbonAgar;
-                d1.OxygenAgar = s.OxygenAgar;
-                d1.AcetateAgar = s.AcetateAgar;
-                d2.CarbonAgar = s.CarbonAgar;
-                d2.OxygenAgar = s.OxygenAgar;
-                d2.AcetateAgar = s.AcetateAgar;
+            for (int w = tid; w < 2 * BX; w += ENV_THREADS) {
+                const int k = w / BX, i = w - k * BX;
+                const AgaRef s = m.A(l, i, k);
+                for (int bb = 0; bb < 2; bb++) {
+                    const RowS d{G(k == 0 ? a.swall[bb] : a.srow[bb]) + l * RW, BX};
+                    d(FC, i) = s.CarbonAgar;
+                    d(FO, i) = s.OxygenAgar;
+                    d(FA, i) = s.AcetateAgar;
+                }
             }
         }
-        sync();
+        __syncthreads();
 
-        // --- 200 Jacobi sweeps, direction alternating with the sweep count
+        // --- 200 Jacobi sweeps, direction alternating with the sweep count.
+        // Single-threaded order: z ascending starts with the interface rows
+        // (which then read the colony row of two sweeps ago), z descending
+        // ends with them. The only interface point that reads a colony box
+        // written in the same sweep is (level 0, x), reading box (x, 0): the
+        // thread of box (x, 0) updates both, in that order, so one barrier
+        // per sweep suffices.
         const int minX = a.bounds[0], maxX = a.bounds[1], maxH = a.bounds[4];
         const int ncol = maxX - minX + 1, nrow = maxH + 1;
+        const gdouble* dshT = G(a.dshT);
         for (int j = 0; j < 200; j++) {
             const int dir = (count % 2 == 0) ? 1 : -1;
-            bcg_env_cell* cur = a.colony[sw];
-            bcg_env_cell* prev = a.colony[sw ^ 1];
-            bcg_aga* const* pA = a.agar[sw ^ 1];
-            bcg_aga* const* cW = a.wall[sw];
-            bcg_aga* const* pW = a.wall[sw ^ 1];
-            auto wall_phase = [&]() {
-                for (int w = tid; w < a.ML * BX; w += ENV_THREADS) wall_point(a, cur, pA, cW, pW, w / BX, w % BX);
-                sync();
-            };
-            auto colony_phase = [&]() {
-                for (int w = tid; w < ncol * nrow; w += ENV_THREADS) {
-                    const int x = minX + w / nrow, z = w % nrow;
-                    if (a.dshift[x * a.BZ + z] > DBL_EPSILON) colony_point(a, cur, prev, pW[0], x, z);
-                }
-                sync();
-            };
-            // single-threaded order: z ascending starts with the interface
-            // (which then reads the colony row of two sweeps ago), z
-            // descending ends with it
-            if (dir == 1) {
-                wall_phase();
-                colony_phase();
-            } else {
-                colony_phase();
-                wall_phase();
+            const bool full = outer == a.n_outer - 1 && j >= 198;
+            const ColS cur{G(a.scol[sw]), BX, NB}, prev{G(a.scol[sw ^ 1]), BX, NB};
+            const RowS pw0{G(a.swall[sw ^ 1]), BX};
+            for (int w = tid; w < ncol * nrow; w += ENV_THREADS) {
+                const int z = w / ncol, x = minX + (w - z * ncol);
+                if (z == 0 && dir == 1) wall_point(a, cur, a.srow[sw ^ 1], a.swall[sw], a.swall[sw ^ 1], 0, x);
+                if (dshT[z * BX + x] > DBL_EPSILON)
+                    colony_point(a, cc, cur, prev, pw0, a.colony[sw], a.colony[sw ^ 1], x, z, full);
+                if (z == 0 && dir == -1) wall_point(a, cur, a.srow[sw ^ 1], a.swall[sw], a.swall[sw ^ 1], 0, x);
             }
+            for (int w = tid; w < AL * BX; w += ENV_THREADS) {
+                const int l = w / BX, x = w - l * BX;
+                if (l == 0 && x >= minX && x <= maxX) continue;
+                wall_point(a, cur, a.srow[sw ^ 1], a.swall[sw], a.swall[sw ^ 1], l, x);
+            }
+            __syncthreads();
             sw ^= 1;
             count++;
         }
+    }
+
+    // write the working copies back: colony fields and interface rows of
+    // both buffers, and the agar of the last copyFromMG
+    for (int w = tid; w < NB; w += ENV_THREADS) {
+        const int x = w / BZ, z = w - x * BZ, s = z * BX + x;
+        for (int bb = 0; bb < 2; bb++) {
+            bcg_env_cell& c = a.colony[bb][w];
+            const gdouble* d = G(a.scol[bb]);
+            c.Carbon = d[FC * NB + s];
+            c.Waste = d[FW * NB + s];
+            c.Oxygen = d[FO * NB + s];
+            c.Acetate = d[FA * NB + s];
+        }
+    }
+    for (int w = tid; w < AL * BX; w += ENV_THREADS) {
+        const int l = w / BX, x = w - l * BX;
+        for (int bb = 0; bb < 2; bb++) {
+            bcg_aga& q = a.wall[bb][l][x];
+            const gdouble* dw = G(a.swall[bb]) + l * RW;
+            q.CarbonAgar = dw[FC * BX + x];
+            q.WasteAgar = dw[FW * BX + x];
+            q.OxygenAgar = dw[FO * BX + x];
+            q.AcetateAgar = dw[FA * BX + x];
+        }
+    }
+    if (a.n_outer > 0) {
+        for (int l = 0; l < AL; l++)
+            for (int w = tid; w < BX * a.BZA; w += ENV_THREADS) {
+                const int i = w / a.BZA, k = w - i * a.BZA + 1;
+                const AgaRef s = m.A(l, i, k);
+                for (int bb = 0; bb < 2; bb++) {
+                    bcg_aga& d = a.agar[bb][l][w];
+                    d.CarbonAgar = s.CarbonAgar;
+                    d.OxygenAgar = s.OxygenAgar;
+                    d.AcetateAgar = s.AcetateAgar;
+                }
+            }
     }
 }
 

@@ -103,7 +103,11 @@ class Environment:
         self.colony = colony
         self.p = params
         h = _vp()
-        self._check(lib().bcg_env_create(ctypes.byref(params), colony._h, ctypes.byref(h)), None)
+        rc = lib().bcg_env_create(ctypes.byref(params), colony._h, ctypes.byref(h))
+        if rc:
+            buf = ctypes.create_string_buffer(512)
+            lib().bcg_last_error(colony._h, buf, 512)  # the create error is kept on the mechanics context
+            raise BcgError(rc, buf.value.decode())
         self._h = h
 
     def _check(self, rc, h="self"):

@@ -172,6 +172,80 @@ def mg_bench(repeats: int = 3) -> dict:
     }
 
 
+# Planar nutrient update of the reference's demo input (run_sim/
+# inputFile_20mM_SmallSimulation.txt: 256 x 50 colony boxes, 256 x 32 agar
+# points on 2 nested levels, 8 multigrid levels, Update_Time 0.01 h -> 266
+# outer iterations of 3 V-cycles + 200 colony/wall sweeps) on the demo's
+# restart state 8 (1952 cells).
+ENV_INPUT = os.path.join(ROOT, "tests", "golden", "inputFile_20mM_SmallSimulation.txt")
+ENV_STATE = os.path.join(ROOT, "tests", "golden", "restart_8.npz")
+
+
+def _env_case():
+    from baccolonygrowth_amd.cells import CELL_DTYPE, read_parameters
+    from baccolonygrowth_amd.environment import env_params_from_file
+
+    p, _ = read_parameters(ENV_INPUT)
+    p.maxNeighbours, p.planar, p.maxCells = 96, 1, 5000
+    ep = env_params_from_file(ENV_INPUT)
+    z = np.load(ENV_STATE)
+    cells = np.ascontiguousarray(z["cells"]).view(CELL_DTYPE).reshape(-1).copy()
+    return p, ep, cells
+
+
+def env_bench(repeats: int = 3, cpu: bool = True) -> dict:
+    """One nutrient update (UPDATE density pass + UpdateEnvArrayMG_OxygenAcetate)
+    on the GPU, median of `repeats` from the same starting state, and the
+    reference's own code (compiled from source into oracle/_ref) on one host
+    thread when that library is present."""
+    from baccolonygrowth_amd.environment import ENV_INIT, ENV_UPDATE, Environment
+    from baccolonygrowth_amd.simulation import Colony
+
+    p, ep, cells = _env_case()
+    out = {"grid": {"colony": [ep.BoxX, ep.BoxZ], "agar": [ep.BoxX, ep.BoxZAgar], "nested_levels": ep.maxLevelsMG},
+           "cells": int(len(cells)), "outer_iterations": int(3600 * ep.UpdateTime * 740 / 100)}
+    with Colony(p) as col:
+        col.LoadCells(cells)
+        with Environment(col, ep) as env:
+            env.fields(ENV_INIT)
+            env.nutrients()
+            st0 = env.state()
+            times = []
+            for _ in range(repeats):
+                env.set_state(**st0)
+                t0 = time.perf_counter()
+                env.fields(ENV_UPDATE)
+                sweeps = env.nutrients()
+                times.append(time.perf_counter() - t0)
+    out["gpu_ms"] = 1000.0 * float(np.median(times))
+    out["sweeps"] = int(sweeps)
+    log(f"[bench] planar nutrient update: GPU {out['gpu_ms']:.1f} ms")
+    out["cpu_baseline"] = None
+    if cpu:
+        try:
+            from oracle.oracle import RefEnv, RefLib, ref_available
+        except ImportError:
+            ref_available = None
+        if ref_available is not None and ref_available():
+            ref = RefLib()
+            sim = ref.sim(p)
+            sim.set_cells(cells)
+            renv = RefEnv(ref, sim, ep)
+            renv.fields(ENV_INIT)
+            renv.nutrients()
+            t0 = time.perf_counter()
+            renv.fields(ENV_UPDATE)
+            renv.nutrients()
+            el = time.perf_counter() - t0
+            renv.close()
+            sim.close()
+            out["cpu_baseline"] = {"ms": 1000.0 * el, "cores": 1, "kind": "reference",
+                                   "sample": "one update of the same state, reference sources compiled with g++ -O3"}
+            out["speedup_vs_cpu"] = 1000.0 * el / out["gpu_ms"]
+            log(f"[bench] planar nutrient update: reference CPU {1000.0 * el:.0f} ms")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,6 +255,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mg", action="store_true")
+    ap.add_argument("--no-env", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is not built.", flush=True)
@@ -234,6 +309,10 @@ def main():
     if not args.no_mg:
         log("[bench] multigrid 513x513x129 ...")
         mg = mg_bench()
+    envb = None
+    if not args.no_env:
+        log("[bench] planar nutrient update (demo grid) ...")
+        envb = env_bench(cpu=not args.no_cpu)
     out = {
         "metric": METRIC,
         "value": value,
@@ -270,6 +349,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "mg_solve": mg,
+        "nutrient_update": envb,
     }
     print(json.dumps(out), flush=True)
 
