@@ -41,6 +41,7 @@ struct bcg_env {
     double* swall[2] = {nullptr, nullptr};
     double* srow[2] = {nullptr, nullptr};
     double *insT = nullptr, *dshT = nullptr, *phantom = nullptr;
+    size_t lds_budget = 0;  // dynamic LDS the nutrient kernel may take
     double *rough = nullptr, *rough1 = nullptr, *rough2 = nullptr, *inside = nullptr;
     double *rwall = nullptr, *rwall1 = nullptr, *rwall2 = nullptr;
     double *rshift = nullptr, *rshift1 = nullptr, *rshift2 = nullptr;
@@ -330,6 +331,22 @@ int bcg_env_create(const bcg_env_params* p, bcg_ctx* mech, bcg_env** out) {
         return rc;
     }
     hipStream_t st = mech->stream;
+    {
+        // LDS per workgroup on this device, less the kernel's static LDS
+        int dev = 0, v = 0;
+        hipFuncAttributes fa;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+            hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_env_nutrients)) == hipSuccess &&
+            (size_t)v > fa.sharedSizeBytes)
+            e->lds_budget = std::min((size_t)ENV_LDS_BYTES, (size_t)v - fa.sharedSizeBytes);
+        if (e->lds_budget > 64 * 1024 &&
+            hipFuncSetAttribute(reinterpret_cast<const void*>(&k_env_nutrients),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->lds_budget) != hipSuccess) {
+            (void)hipGetLastError();
+            e->lds_budget = 64 * 1024 - fa.sharedSizeBytes;
+        }
+    }
     auto up = [&](void* dst, const void* src, size_t bytes) {
         if (!rc && hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st) != hipSuccess)
             rc = efail(e, BCG_ERR_HIP, "upload failed");
@@ -581,8 +598,15 @@ int bcg_env_nutrients(bcg_env* e, int32_t* count) {
     a.wshift = e->wshift;
     a.bounds = e->bounds;
     a.errflag = e->errflag;
+    // the colony window [minX-1, maxX+1] x [0, maxH+1] of both buffers in
+    // LDS when it fits
+    const int lw = maxX - minX + 3, lh = maxH + 2;
+    const size_t lds = (size_t)2 * 4 * lw * lh * sizeof(double);
+    a.lds_colony = lds <= e->lds_budget;
+    a.lds_w = lw;
+    a.lds_nf = lw * lh;
     EHIP(e, hipMemsetAsync(e->errflag, 0, sizeof(int), st));
-    hipLaunchKernelGGL(k_env_nutrients, dim3(1), dim3(ENV_THREADS), 0, st, a);
+    hipLaunchKernelGGL(k_env_nutrients, dim3(1), dim3(ENV_THREADS), a.lds_colony ? lds : 0, st, a);
     // the mechanics grows with the caller's Environment (ABM/grow.cpp:26-36)
     const int nb = e->BX * e->BZ;
     hipLaunchKernelGGL(k_env_growth_to_mech, dim3(nblk(nb, 256)), dim3(256), 0, st, e->colony[0], nb, m->gr1, m->gr2);

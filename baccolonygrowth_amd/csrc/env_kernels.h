@@ -14,6 +14,7 @@ constexpr int MG_LEVELS = 8;      // Multigrid(maxLevelsMG, 8, ...) at ABM/Simul
 constexpr int MAX_NEST = 4;       // nested agar levels (maxLevelsMG) supported
 constexpr int MAX_REF = 8;        // refinementGridHeight supported
 constexpr int ENV_THREADS = 512;  // the nutrient update runs as one workgroup (up to 256 VGPRs a lane)
+constexpr int ENV_LDS_BYTES = 144 * 1024;  // dynamic LDS for the colony window of the sweeps
 
 enum ErrBits {
     ERR_RANGE = 1,       // a stencil or filter would read outside an array
@@ -57,6 +58,9 @@ struct EnvArgs {
     double* insT;
     double* dshT;
     double* phantom;                  // 3 values per thread: multigrid reads past an array's end
+    // the sweeps' colony window in dynamic LDS: width maxX - minX + 3 and
+    // lds_nf = width * (maxH + 2) boxes a field (lds_colony = 0: global)
+    int lds_colony, lds_w, lds_nf;
     // fields (smoothed, shifted to the grid points)
     const double* inside;             // insideColonyDen [BX][BZ]
     const double* dshift;             // DensityShiftP

@@ -891,10 +891,14 @@ __device__ ChemConst chem_const(const EnvArgs& a) {
 // bcg_aga, each [z][x]) and of interface or agar rows ([field][x]) during an
 // update: the lanes of a wave take neighbouring x.
 enum { FC = 0, FW = 1, FO = 2, FA = 3 };
+// A colony buffer: field f of box (x, z) at p[f * NF + z * W + x - x0] -
+// the whole grid in global memory (x0 = 0, W = BoxX), or the window the
+// sweeps update plus its one-box halo in LDS.
+template <class T>
 struct ColS {
-    gdouble* p;
-    int BX, NB;
-    __device__ gdouble& operator()(int f, int x, int z) const { return p[f * NB + z * BX + x]; }
+    T* p;
+    int x0, W, NF;
+    __device__ T& operator()(int f, int x, int z) const { return p[f * NF + z * W + x - x0]; }
 };
 struct RowS {
     gdouble* p;
@@ -910,7 +914,8 @@ struct RowS {
 // own acetate, is overwritten by the second, on the vertex average). Those
 // are computed when `full` is set: in the last two sweeps of an update, whose
 // writes are the ones that remain in the two buffers.
-__device__ __forceinline__ void colony_point(const EnvArgs& a, const ChemConst& cc, const ColS cur, const ColS prev, const RowS pw,
+template <class CS>
+__device__ __forceinline__ void colony_point(const EnvArgs& a, const ChemConst& cc, const CS cur, const CS prev, const RowS pw,
                              bcg_env_cell* Env, bcg_env_cell* prevEnv, int x, int z, bool full) {
     const int BX = a.BX;
     const gdouble* insT = G(a.insT);
@@ -1022,7 +1027,8 @@ __device__ __forceinline__ void colony_point(const EnvArgs& a, const ChemConst& 
 // box x: the colony-agar interface row. env: the colony being written this
 // sweep; row: the first agar row of the previous buffer; wal / pwal: the
 // interface rows of this and of the previous buffer (levels 4 BX apart).
-__device__ __forceinline__ void wall_point(const EnvArgs& a, const ColS env, double* row, double* wal, double* pwal,
+template <class CS>
+__device__ __forceinline__ void wall_point(const EnvArgs& a, const CS env, double* row, double* wal, double* pwal,
                                            int l, int x) {
     const int BX = a.BX;
     const RowS out{G(wal) + l * 4 * BX, BX}, W{G(pwal) + l * 4 * BX, BX};
@@ -1192,6 +1198,44 @@ __device__ __forceinline__ void reset_zero(const MG m, int tid) {
     __syncthreads();
 }
 
+// 200 Jacobi sweeps over the colony and the interface rows
+// (UpdateEnvArrayMG_OxygenAcetate, ABM/Multigrid.cpp:1015-1112), direction
+// alternating with the sweep count. Single-threaded order: z ascending starts
+// with the interface rows (which then read the colony row of two sweeps ago),
+// z descending ends with them. The only interface point that reads a colony
+// box written in the same sweep is (level 0, x), reading box (x, 0): the
+// thread of box (x, 0) updates both, in that order, so one barrier per sweep
+// suffices. c0 / c1: the colony buffers (Environment, oldEnvironment).
+template <class T>
+__device__ __forceinline__ void sweeps(const EnvArgs& a, const ChemConst& cc, int outer, int tid, T* c0, T* c1, int x0,
+                                       int W, int NF, int& sw, int& count) {
+    const int BX = a.BX, AL = a.ML;
+    const int minX = a.bounds[0], maxX = a.bounds[1], maxH = a.bounds[4];
+    const int ncol = maxX - minX + 1, nrow = maxH + 1;
+    const gdouble* dshT = G(a.dshT);
+    for (int j = 0; j < 200; j++) {
+        const int dir = (count % 2 == 0) ? 1 : -1;
+        const bool full = outer == a.n_outer - 1 && j >= 198;
+        const ColS<T> cur{sw ? c1 : c0, x0, W, NF}, prev{sw ? c0 : c1, x0, W, NF};
+        const RowS pw0{G(a.swall[sw ^ 1]), BX};
+        for (int w = tid; w < ncol * nrow; w += ENV_THREADS) {
+            const int z = w / ncol, x = minX + (w - z * ncol);
+            if (z == 0 && dir == 1) wall_point(a, cur, a.srow[sw ^ 1], a.swall[sw], a.swall[sw ^ 1], 0, x);
+            if (dshT[z * BX + x] > DBL_EPSILON)
+                colony_point(a, cc, cur, prev, pw0, a.colony[sw], a.colony[sw ^ 1], x, z, full);
+            if (z == 0 && dir == -1) wall_point(a, cur, a.srow[sw ^ 1], a.swall[sw], a.swall[sw ^ 1], 0, x);
+        }
+        for (int w = tid; w < AL * BX; w += ENV_THREADS) {
+            const int l = w / BX, x = w - l * BX;
+            if (l == 0 && x >= minX && x <= maxX) continue;
+            wall_point(a, cur, a.srow[sw ^ 1], a.swall[sw], a.swall[sw ^ 1], l, x);
+        }
+        __syncthreads();
+        sw ^= 1;
+        count++;
+    }
+}
+
 }  // namespace
 
 // The whole UpdateEnvArrayMG_OxygenAcetate in one workgroup.
@@ -1201,6 +1245,7 @@ __global__ void __launch_bounds__(ENV_THREADS) k_env_nutrients(EnvArgs args) {
     // gdouble* where used, so that accesses are global, not flat)
     __shared__ EnvArgs a;
     __shared__ double cs_e[3 * COARSE_MAX], cs_r[3 * COARSE_MAX];
+    extern __shared__ double lds[];  // the colony window of both buffers, when it fits
     const int tid = threadIdx.x;
     if (tid == 0) a = args;
     __syncthreads();
@@ -1239,6 +1284,25 @@ __global__ void __launch_bounds__(ENV_THREADS) k_env_nutrients(EnvArgs args) {
             dr[FW * BX + x] = r.WasteAgar;
             dr[FO * BX + x] = r.OxygenAgar;
             dr[FA * BX + x] = r.AcetateAgar;
+        }
+    }
+    // the window [minX-1, maxX+1] x [0, maxH+1] (the boxes the sweeps update
+    // and the halo they read), fields C, W, O2, acetate, if the launch sized
+    // the LDS for these bounds
+    const bool use_lds = a.lds_colony && a.lds_w == a.bounds[1] - a.bounds[0] + 3 &&
+                         a.lds_nf == a.lds_w * (a.bounds[4] + 2);
+    if (use_lds) {
+        const int x0 = a.bounds[0] - 1, W = a.lds_w, H = a.lds_nf / W;
+        for (int w = tid; w < W * H; w += ENV_THREADS) {
+            const int z = w / W, x = x0 + (w - z * W);
+            for (int bb = 0; bb < 2; bb++) {
+                const bcg_env_cell& c = a.colony[bb][x * BZ + z];
+                double* d = lds + bb * 4 * a.lds_nf;
+                d[FC * a.lds_nf + w] = c.Carbon;
+                d[FW * a.lds_nf + w] = c.Waste;
+                d[FO * a.lds_nf + w] = c.Oxygen;
+                d[FA * a.lds_nf + w] = c.Acetate;
+            }
         }
     }
     __syncthreads();
@@ -1355,50 +1419,39 @@ __global__ void __launch_bounds__(ENV_THREADS) k_env_nutrients(EnvArgs args) {
         }
         __syncthreads();
 
-        // --- 200 Jacobi sweeps, direction alternating with the sweep count.
-        // Single-threaded order: z ascending starts with the interface rows
-        // (which then read the colony row of two sweeps ago), z descending
-        // ends with them. The only interface point that reads a colony box
-        // written in the same sweep is (level 0, x), reading box (x, 0): the
-        // thread of box (x, 0) updates both, in that order, so one barrier
-        // per sweep suffices.
-        const int minX = a.bounds[0], maxX = a.bounds[1], maxH = a.bounds[4];
-        const int ncol = maxX - minX + 1, nrow = maxH + 1;
-        const gdouble* dshT = G(a.dshT);
-        for (int j = 0; j < 200; j++) {
-            const int dir = (count % 2 == 0) ? 1 : -1;
-            const bool full = outer == a.n_outer - 1 && j >= 198;
-            const ColS cur{G(a.scol[sw]), BX, NB}, prev{G(a.scol[sw ^ 1]), BX, NB};
-            const RowS pw0{G(a.swall[sw ^ 1]), BX};
-            for (int w = tid; w < ncol * nrow; w += ENV_THREADS) {
-                const int z = w / ncol, x = minX + (w - z * ncol);
-                if (z == 0 && dir == 1) wall_point(a, cur, a.srow[sw ^ 1], a.swall[sw], a.swall[sw ^ 1], 0, x);
-                if (dshT[z * BX + x] > DBL_EPSILON)
-                    colony_point(a, cc, cur, prev, pw0, a.colony[sw], a.colony[sw ^ 1], x, z, full);
-                if (z == 0 && dir == -1) wall_point(a, cur, a.srow[sw ^ 1], a.swall[sw], a.swall[sw ^ 1], 0, x);
-            }
-            for (int w = tid; w < AL * BX; w += ENV_THREADS) {
-                const int l = w / BX, x = w - l * BX;
-                if (l == 0 && x >= minX && x <= maxX) continue;
-                wall_point(a, cur, a.srow[sw ^ 1], a.swall[sw], a.swall[sw ^ 1], l, x);
-            }
-            __syncthreads();
-            sw ^= 1;
-            count++;
-        }
+        // --- 200 Jacobi sweeps (the colony from LDS when its window fits)
+        if (use_lds)
+            sweeps<double>(a, cc, outer, tid, lds, lds + 4 * a.lds_nf, a.bounds[0] - 1, a.lds_w, a.lds_nf, sw, count);
+        else
+            sweeps<gdouble>(a, cc, outer, tid, G(a.scol[0]), G(a.scol[1]), 0, BX, NB, sw, count);
     }
 
     // write the working copies back: colony fields and interface rows of
     // both buffers, and the agar of the last copyFromMG
-    for (int w = tid; w < NB; w += ENV_THREADS) {
-        const int x = w / BZ, z = w - x * BZ, s = z * BX + x;
-        for (int bb = 0; bb < 2; bb++) {
-            bcg_env_cell& c = a.colony[bb][w];
-            const gdouble* d = G(a.scol[bb]);
-            c.Carbon = d[FC * NB + s];
-            c.Waste = d[FW * NB + s];
-            c.Oxygen = d[FO * NB + s];
-            c.Acetate = d[FA * NB + s];
+    if (use_lds) {
+        const int x0 = a.bounds[0] - 1, W = a.lds_w, H = a.lds_nf / W;
+        for (int w = tid; w < W * H; w += ENV_THREADS) {
+            const int z = w / W, x = x0 + (w - z * W);
+            for (int bb = 0; bb < 2; bb++) {
+                bcg_env_cell& c = a.colony[bb][x * BZ + z];
+                const double* d = lds + bb * 4 * a.lds_nf;
+                c.Carbon = d[FC * a.lds_nf + w];
+                c.Waste = d[FW * a.lds_nf + w];
+                c.Oxygen = d[FO * a.lds_nf + w];
+                c.Acetate = d[FA * a.lds_nf + w];
+            }
+        }
+    } else {
+        for (int w = tid; w < NB; w += ENV_THREADS) {
+            const int x = w / BZ, z = w - x * BZ, s = z * BX + x;
+            for (int bb = 0; bb < 2; bb++) {
+                bcg_env_cell& c = a.colony[bb][w];
+                const gdouble* d = G(a.scol[bb]);
+                c.Carbon = d[FC * NB + s];
+                c.Waste = d[FW * NB + s];
+                c.Oxygen = d[FO * NB + s];
+                c.Acetate = d[FA * NB + s];
+            }
         }
     }
     for (int w = tid; w < AL * BX; w += ENV_THREADS) {

@@ -256,10 +256,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mg", action="store_true")
     ap.add_argument("--no-env", action="store_true")
+    ap.add_argument("--only", choices=("mg", "env"), default=None,
+                    help="run one secondary section alone and print its JSON object (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is not built.", flush=True)
         sys.exit(2)
+
+    if args.only == "env":
+        print(json.dumps(env_bench(cpu=not args.no_cpu)), flush=True)
+        return
+    if args.only == "mg":
+        print(json.dumps(mg_bench()), flush=True)
+        return
 
     from baccolonygrowth_amd.simulation import Colony
 
