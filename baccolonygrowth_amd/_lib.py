@@ -152,6 +152,7 @@ SIGNATURES.update(
         "bcg_env_create": (ctypes.c_int, [ctypes.POINTER(EnvParams), _vp, ctypes.POINTER(_vp)]),
         "bcg_env_destroy": (ctypes.c_int, [_vp]),
         "bcg_env_last_error": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int]),
+        "bcg_env_set_lds_limit": (ctypes.c_int, [_vp, ctypes.c_int64]),
         "bcg_env_set_state": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "bcg_env_get_state": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "bcg_env_fields": (ctypes.c_int, [_vp, ctypes.c_int32]),

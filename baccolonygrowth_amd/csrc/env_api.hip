@@ -41,7 +41,8 @@ struct bcg_env {
     double* swall[2] = {nullptr, nullptr};
     double* srow[2] = {nullptr, nullptr};
     double *insT = nullptr, *dshT = nullptr, *phantom = nullptr;
-    size_t lds_budget = 0;  // dynamic LDS the nutrient kernel may take
+    size_t lds_device = 0;  // dynamic LDS the nutrient kernel can take on this device
+    size_t lds_budget = 0;  // ... and may take (bcg_env_set_lds_limit)
     double *rough = nullptr, *rough1 = nullptr, *rough2 = nullptr, *inside = nullptr;
     double *rwall = nullptr, *rwall1 = nullptr, *rwall2 = nullptr;
     double *rshift = nullptr, *rshift1 = nullptr, *rshift2 = nullptr;
@@ -346,6 +347,7 @@ int bcg_env_create(const bcg_env_params* p, bcg_ctx* mech, bcg_env** out) {
             (void)hipGetLastError();
             e->lds_budget = 64 * 1024 - fa.sharedSizeBytes;
         }
+        e->lds_device = e->lds_budget;
     }
     auto up = [&](void* dst, const void* src, size_t bytes) {
         if (!rc && hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st) != hipSuccess)
@@ -408,6 +410,12 @@ int bcg_env_create(const bcg_env_params* p, bcg_ctx* mech, bcg_env** out) {
         return rc;
     }
     *out = e;
+    return BCG_OK;
+}
+
+int bcg_env_set_lds_limit(bcg_env* e, int64_t bytes) {
+    if (!e || bytes < 0) return BCG_ERR_ARG;
+    e->lds_budget = std::min(e->lds_device, (size_t)bytes);
     return BCG_OK;
 }
 

@@ -171,6 +171,11 @@ class Environment:
         self._check(lib().bcg_env_get_state(self._h, *[ctypes.c_void_p(a.ctypes.data) for a in st.values()]))
         return st
 
+    def set_lds_limit(self, nbytes: int):
+        """Cap the LDS the nutrient update may use for the swept colony boxes
+        (0: global memory); the results do not depend on it."""
+        self._check(lib().bcg_env_set_lds_limit(self._h, int(nbytes)))
+
     def fields(self, pass_: int = ENV_UPDATE):
         self._check(lib().bcg_env_fields(self._h, pass_))
 

@@ -315,6 +315,11 @@ int bcg_env_default_params(bcg_env_params* p);
 int bcg_env_create(const bcg_env_params* p, bcg_ctx* mech, bcg_env** out);
 int bcg_env_destroy(bcg_env* env);
 int bcg_env_last_error(const bcg_env* env, char* buf, int len);
+/* Caps the on-chip memory (LDS) the nutrient update may use for the colony
+ * boxes it sweeps (default: what the device allows, up to 144 KiB; 0: keep
+ * them in global memory). Both placements give the same results. No
+ * counterpart in the reference (a placement choice of this build).        */
+int bcg_env_set_lds_limit(bcg_env* env, int64_t bytes);
 /* Nutrient state in the reference layouts (x-major, z fastest):
  *   colony, colony_old  [BoxX][BoxY][BoxZ]                 Environment, oldEnvironment
  *   agar, agar_old      [maxLevelsMG][BoxX][BoxY][BoxZAgar] FieldAgar, oldFieldAgar
@@ -339,7 +344,12 @@ int bcg_env_get_state(bcg_env* env, bcg_env_cell* colony, bcg_env_cell* colony_o
 int bcg_env_fields(bcg_env* env, int32_t pass);
 /* UpdateEnvArrayMG_OxygenAcetate (ABM/Multigrid.cpp:984-1125), single
  * threaded semantics, from the last INIT/UPDATE fields; *count = its return
- * value. GrowthRate1/2 of the resulting Environment go to the mechanics.   */
+ * value. GrowthRate1/2 of the resulting Environment go to the mechanics.
+ * Multigrid elements are addressed by the reference's flat index, so a z
+ * past the top of a level reads the next column as there; an index past
+ * the end of an array (the coarsest level when BoxZAgar < 2^(8 -
+ * maxLevelsMG), e.g. the demo input) reads 0 where the reference reads
+ * whatever memory follows.                                                  */
 int bcg_env_nutrients(bcg_env* env, int32_t* count);
 /* Field read-back for tests and output, reference layouts (NULL: skip):
  *   density, density_shift, inside  [BoxX][BoxY][BoxZ]  Density, DensityShiftP, insideColonyDen

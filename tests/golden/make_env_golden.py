@@ -30,20 +30,26 @@ from oracle.oracle import RefEnv, RefLib  # noqa: E402
 GRID = dict(BoxX=128, BoxZ=16)
 AGAR_Z = 64
 FIELDS = ("Carbon", "Waste", "Oxygen", "Acetate", "GrowthRate1", "GrowthRate2", "GrCarAer", "GrCarAna", "GrAceAer",
-          "qCarmnt", "qAcemnt", "pAcemnt", "qOxymnt")
+          "qCarmnt", "qAcemnt", "pAcemnt", "qOxymnt", "pH", "Hydrox", "Acetic", "Buffer")
+# one nested agar level (maxLevelsMG = 1, the reference's default): BoxZAgar
+# >= 2^7 keeps z points on every level
+GRID_ML1 = dict(BoxX=128, BoxZ=16, BoxZAgar=128)
 
 
-def params(update_time: float):
+def params(update_time: float, grid=None, agar_z=AGAR_Z, levels=None):
+    grid = grid or GRID
     p = demo_params()
-    p.BoxX, p.BoxZ = GRID["BoxX"], GRID["BoxZ"]
+    p.BoxX, p.BoxZ = grid["BoxX"], grid["BoxZ"]
     p.maxCells = 4000
     ep = env_params_from_file(DEMO_INPUT)
-    ep.BoxX, ep.BoxZ, ep.BoxZAgar = GRID["BoxX"], GRID["BoxZ"], AGAR_Z
+    ep.BoxX, ep.BoxZ, ep.BoxZAgar = grid["BoxX"], grid["BoxZ"], agar_z
+    if levels is not None:
+        ep.maxLevelsMG = levels
     ep.UpdateTime = update_time
     return p, ep
 
 
-def pack(prefix: str, out: dict, f: dict, st: dict | None, agar: bool = True):
+def pack(prefix: str, out: dict, f: dict, st: dict | None, agar: bool = True, na: int | None = None):
     """Fields, the colony (both buffers), the interface rows and the finest
     agar level (its Waste never changes in this path: not stored)."""
     for k in ("density", "density_shift", "inside", "wall_shift", "height", "normal", "bounds"):
@@ -53,7 +59,7 @@ def pack(prefix: str, out: dict, f: dict, st: dict | None, agar: bool = True):
         for k in FIELDS:
             out[f"{prefix}_colony_{k}"] = st["colony"][k]
             out[f"{prefix}_colony_old_{k}"] = st["colony_old"][k]
-        na = GRID["BoxX"] * AGAR_Z
+        na = na or GRID["BoxX"] * AGAR_Z
         for k in ("CarbonAgar", "OxygenAgar", "AcetateAgar", "WasteAgar"):
             if agar and k != "WasteAgar":
                 out[f"{prefix}_agar_{k}"] = st["agar"][k][:na]
@@ -85,7 +91,19 @@ def main():
     env.close()
     sim.close()
 
-    # 2. the coupled time loop: init pass, then steps with the clock flags
+    # 2. one nested agar level: INIT fields and one nutrient update
+    p, ep = params(1e-4, GRID_ML1, GRID_ML1["BoxZAgar"], levels=1)
+    out["ml1_grid"] = np.array([GRID_ML1["BoxX"], GRID_ML1["BoxZ"], GRID_ML1["BoxZAgar"]])
+    sim = ref.sim(p)
+    sim.set_cells(cells)
+    env = RefEnv(ref, sim, ep)
+    env.fields(0)
+    out["ml1_n"] = np.array(env.nutrients())
+    pack("ml1", out, env.get_fields(), env.get_state(), na=GRID_ML1["BoxX"] * GRID_ML1["BoxZAgar"])
+    env.close()
+    sim.close()
+
+    # 3. the coupled time loop: init pass, then steps with the clock flags
     update_time, steps, seed = 1e-3, 25, 11
     p, ep = params(update_time)
     rng = np.random.default_rng(seed)

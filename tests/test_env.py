@@ -5,7 +5,8 @@ run of the time loop, against the reference's own code.
 The expected values are the reference's outputs (tests/golden/golden_env.npz,
 written by tests/golden/make_env_golden.py from oracle/_ref, the reference
 compiled from source) on the demo input with BoxX = 128, BoxZ = 16,
-BoxZAgar = 64 and the demo restart state 5. Arrays the reference leaves
+BoxZAgar = 64 (and, for one nested agar level, BoxZAgar = 128 with
+maxLevelsMG = 1) and the demo restart state 5. Arrays the reference leaves
 uninitialised are pinned to 0 on both sides (BoxAverage outside its window,
 insideColonyDen one box past the last).
 
@@ -26,7 +27,12 @@ from helpers import DEMO_INPUT, GOLDEN, demo_params, restart_cells
 GOLDEN_ENV = os.path.join(GOLDEN, "golden_env.npz")
 FIELDS = ("Carbon", "Waste", "Oxygen", "Acetate", "GrowthRate1", "GrowthRate2", "GrCarAer", "GrCarAna", "GrAceAer",
           "qCarmnt", "qAcemnt", "pAcemnt", "qOxymnt")
+# the acetate buffer state (acetateBufferSolve): Newton iterates, then
+# pH = -log(...)/log(10); device log and the reference's pow(x, 2) (here x*x)
+# may differ in the last bit, so compared at 1e-12
+BUFFER_FIELDS = ("pH", "Hydrox", "Acetic", "Buffer")
 RTOL = 1e-13
+RTOL_BUFFER = 1e-12
 
 
 def say(*a):
@@ -34,14 +40,16 @@ def say(*a):
     sys.stdout.flush()
 
 
-def _params(g, update_time):
+def _params(g, update_time, grid="grid", levels=None):
     from baccolonygrowth_amd.environment import env_params_from_file
 
-    bx, bz, bza = (int(v) for v in g["grid"])
+    bx, bz, bza = (int(v) for v in g[grid])
     p = demo_params()
     p.BoxX, p.BoxZ, p.maxCells = bx, bz, 4000
     ep = env_params_from_file(DEMO_INPUT)
     ep.BoxX, ep.BoxZ, ep.BoxZAgar = bx, bz, bza
+    if levels is not None:
+        ep.maxLevelsMG = levels
     ep.UpdateTime = update_time
     return p, ep
 
@@ -76,6 +84,10 @@ def test_golden_env_fixture_is_consistent():
     c = {k: g[f"nut1_colony_{k}"] for k in ("GrowthRate1", "GrCarAer", "GrCarAna", "GrAceAer")}
     assert np.array_equal(c["GrowthRate1"], c["GrCarAer"] + c["GrCarAna"] + c["GrAceAer"])
     assert len(g["loop_lineage"]) > 0
+    assert g["ml1_agar_CarbonAgar"].shape == (int(g["ml1_grid"][0]) * int(g["ml1_grid"][2]),)
+    # the buffer state is a pH where it was computed and the LocalEnv() default 7 elsewhere
+    ph = g["nut2_colony_pH"]
+    assert np.all((ph > 0) & (ph <= 14))
 
 
 def test_env_params_from_demo_file_equal_defaults():
@@ -160,9 +172,10 @@ def _check_fields(env, col, g, prefix):
 def _check_state(env, g, prefix, agar=True):
     st = env.state()
     worst = 0.0
-    for k in FIELDS:
-        worst = max(worst, _close(st["colony"][k], g[f"{prefix}_colony_{k}"], f"{prefix} colony {k}"))
-        worst = max(worst, _close(st["colony_old"][k], g[f"{prefix}_colony_old_{k}"], f"{prefix} colony_old {k}"))
+    for k in FIELDS + BUFFER_FIELDS:
+        tol = RTOL_BUFFER if k in BUFFER_FIELDS else RTOL
+        worst = max(worst, _close(st["colony"][k], g[f"{prefix}_colony_{k}"], f"{prefix} colony {k}", tol))
+        worst = max(worst, _close(st["colony_old"][k], g[f"{prefix}_colony_old_{k}"], f"{prefix} colony_old {k}", tol))
     na = len(g[f"{prefix}_wall_CarbonAgar"])
     bx = na // env.p.maxLevelsMG
     for k in ("CarbonAgar", "OxygenAgar", "AcetateAgar", "WasteAgar"):
@@ -182,18 +195,24 @@ def _check_state(env, g, prefix, agar=True):
 
 
 @pytest.mark.gpu
-def test_env_passes_match_reference():
+@pytest.mark.parametrize("lds", ["lds", "global"])
+def test_env_passes_match_reference(lds):
+    """INIT fields, nutrients, UPDATE and HEIGHT fields, nutrients again (the
+    multigrid state carried over); the swept colony boxes in LDS or in
+    global memory."""
     from baccolonygrowth_amd.environment import ENV_HEIGHT, ENV_INIT, ENV_UPDATE, Environment
     from baccolonygrowth_amd.simulation import Colony
 
     g = np.load(GOLDEN_ENV)
     p, ep = _params(g, 1e-4)
     _, _, cells = restart_cells(5)
-    say(f"[gpu] env passes: {len(cells)} cells, grid {tuple(g['grid'])}")
+    say(f"[gpu] env passes ({lds}): {len(cells)} cells, grid {tuple(g['grid'])}")
     with Colony(p) as col:
         col.set_cells(cells)
         _start_fields(col, p)
         with Environment(col, ep) as env:
+            if lds == "global":
+                env.set_lds_limit(0)
             env.fields(ENV_INIT)
             _check_fields(env, col, g, "init")
             say("    INIT fields ok")
@@ -211,6 +230,28 @@ def test_env_passes_match_reference():
             # the mechanics grows with the Environment's growth rates
             _, _, _, gr1, _ = col.fields()
             _close(gr1, g["nut2_colony_GrowthRate1"], "mechanics gr1")
+
+
+@pytest.mark.gpu
+def test_one_nested_agar_level_matches_reference():
+    """maxLevelsMG = 1 (the reference's default): the outer level is the
+    finest, other composite-boundary branches."""
+    from baccolonygrowth_amd.environment import ENV_INIT, Environment
+    from baccolonygrowth_amd.simulation import Colony
+
+    g = np.load(GOLDEN_ENV)
+    p, ep = _params(g, 1e-4, grid="ml1_grid", levels=1)
+    _, _, cells = restart_cells(5)
+    say(f"[gpu] one nested level: grid {tuple(g['ml1_grid'])}")
+    with Colony(p) as col:
+        col.set_cells(cells)
+        _start_fields(col, p)
+        with Environment(col, ep) as env:
+            env.fields(ENV_INIT)
+            _check_fields(env, col, g, "ml1")
+            assert env.nutrients() == int(g["ml1_n"])
+            w = _check_state(env, g, "ml1")
+            say(f"    ok (max rel diff {w:.1e})")
 
 
 @pytest.mark.gpu
@@ -253,3 +294,47 @@ def test_time_loop_with_fields_matches_reference():
             _check_fields(env, col, g, "loop")
             w = _check_state(env, g, "loop")
             say(f"    loop ok: {len(lineage)} divisions, env max rel diff {w:.1e}")
+
+
+@pytest.mark.gpu
+def test_demo_grid_update_is_bounded_and_placement_independent():
+    """The demo input itself (256 x 50 boxes, BoxZAgar = 32: the coarsest
+    multigrid level has one z point, and the reference's stencils read past
+    its array there - parity unpinned for that read, 0 here) on the demo
+    restart state 8: concentrations stay within their bounds, the swept boxes
+    in LDS or in global memory give identical states, and a rerun repeats
+    them bit for bit."""
+    from baccolonygrowth_amd.environment import ENV_INIT, ENV_UPDATE, Environment, env_params_from_file
+    from baccolonygrowth_amd.simulation import Colony
+
+    p = demo_params()
+    p.maxCells = 5000
+    ep = env_params_from_file(DEMO_INPUT)
+    ep.UpdateTime = 1e-4
+    _, _, cells = restart_cells(8)
+    say(f"[gpu] demo grid: {len(cells)} cells, {ep.BoxX}x{ep.BoxZ}, agar {ep.BoxZAgar}")
+    states = []
+    for lds in (None, 0, None):
+        with Colony(p) as col:
+            col.set_cells(cells)
+            _start_fields(col, p)
+            with Environment(col, ep) as env:
+                if lds is not None:
+                    env.set_lds_limit(lds)
+                env.fields(ENV_INIT)
+                env.nutrients()
+                env.fields(ENV_UPDATE)
+                env.nutrients()
+                states.append(env.state())
+    for k in states[0]:
+        a = states[0][k].view(np.uint8)
+        assert np.array_equal(a, states[1][k].view(np.uint8)), f"{k}: LDS and global placement differ"
+        assert np.array_equal(a, states[2][k].view(np.uint8)), f"{k}: rerun differs"
+    c, w = states[0]["colony"], states[0]["wall"]
+    assert np.all(np.isfinite(c["Carbon"])) and np.all((c["Carbon"] >= 0) & (c["Carbon"] <= ep.maxCarbon))
+    assert np.all((c["Oxygen"] >= 0) & (c["Oxygen"] <= ep.maxO2))
+    assert np.all(c["Acetate"] >= 0) and np.all(c["Waste"] >= 0)
+    assert np.all((w["CarbonAgar"] >= 0) & (w["CarbonAgar"] <= ep.maxCarbon))
+    gr = c["GrowthRate1"]
+    assert np.all(np.isfinite(gr)) and gr.max() > 0
+    say(f"    ok: growth rate range [{gr.min():.3g}, {gr.max():.3g}]")
