@@ -236,8 +236,8 @@ int bcg_create(const bcg_params* p, bcg_ctx** out) {
     A(&c->bin_start, c->nbin + 1);
     A(&c->bin_fill, c->nbin);
     A(&c->sorted, M);
-    A(&c->cmL, M);
-    A(&c->cand, M);
+    A(&c->cmL, 2 * (size_t)M);   // FP32 end points p, q per cell
+    A(&c->cand, 2 * (size_t)M);
     A(&c->scan_tot, 1024 * 64);
     A(&c->nl, (size_t)c->cap * c->stride);
     A(&c->nl_count, M);

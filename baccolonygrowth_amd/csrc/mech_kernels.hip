@@ -13,6 +13,8 @@
 //             division order by one workgroup
 #include "mech_kernels.h"
 
+#include <climits>
+
 namespace bcg {
 
 // ------------------------------------------------------------- scans ---
@@ -117,11 +119,10 @@ __global__ void k_bin_count(const double* __restrict__ hot, int n, PhysParams P,
         a = x * (P.BoxY * P.BoxZ) + y * P.BoxZ + z;
     }
     key[i] = a;
-    // FP32 centre and half segment length |p-q|/2 (rounded up) for the
-    // conservative candidate prefilter of the list build
-    const d3 v = sub(s.q, s.p);
-    const float hl = (float)(0.5 * sqrt(dot(v, v))) * 1.000001f + 1e-6f;
-    cmL[i] = make_float4((float)cm.x, (float)cm.y, (float)cm.z, hl);
+    // FP32 end points for the conservative candidate prefilter of the list
+    // build, and the bin's z index
+    cmL[2 * i] = make_float4((float)s.p.x, (float)s.p.y, (float)s.p.z, (float)z);
+    cmL[2 * i + 1] = make_float4((float)s.q.x, (float)s.q.y, (float)s.q.z, 0.0f);
     atomicAdd(&cnt[a], 1);
 }
 
@@ -150,11 +151,15 @@ __global__ void k_bin_sort(const int* __restrict__ start, int nbin, int* __restr
     }
 }
 
-// Candidate records in bin order: cand[pos] = cmL[sorted[pos]].
+// Candidate records in bin order: the FP32 end points p, q of cell sorted[pos].
 __global__ void k_bin_cand(const int* __restrict__ sorted, int n, const float4* __restrict__ cmL,
                            float4* __restrict__ cand) {
     int pos = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pos < n) cand[pos] = cmL[sorted[pos]];
+    if (pos < n) {
+        const int j = sorted[pos];
+        cand[2 * pos] = cmL[2 * j];
+        cand[2 * pos + 1] = cmL[2 * j + 1];
+    }
 }
 
 // ------------------------------------------------------------ lists ---
@@ -177,6 +182,54 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 // the masked FP64 test whenever any lane of its wave had a survivor.
 constexpr int GQ = 16;
 
+// The exact test (reduceNeighbours' predicate in FP64) of the queued
+// candidate cells, in queue order; kept ones are appended to the cell's list.
+// The next candidate's record is requested before this one is tested.
+__device__ __forceinline__ int flush_queue(const PhysParams& P, const double* __restrict__ hot, const Seg& A, d3 cmA,
+                                           const int* queue, int qstride, int nq, int* __restrict__ nl, int stride,
+                                           int slot, int cap, int m) {
+    if (nq == 0) return m;
+    int j = queue[0];
+    Seg B;
+    load_seg(hot, j, B);
+    for (int e = 0; e < nq; e++) {
+        const int jn = queue[min(e + 1, nq - 1) * qstride];
+        Seg Bn;
+        load_seg(hot, jn, Bn);
+        if (keep_neighbour(A, cmA, B, P.Rc)) {
+            if (m < cap) nl[(size_t)m * stride + slot] = j;
+            m++;
+        }
+        j = jn;
+        B = Bn;
+    }
+    return m;
+}
+
+// FP32 form of the first test of reduceNeighbours (ABM/Neighbours.cpp:69-73):
+// an end point of j within L_A/2 + Rc R_A of the centre of A. The margin
+// covers the rounding of coordinates of magnitude |c| to FP32 (half an ulp,
+// about 6e-8 |c| each) and of the FP32 squared distance, with room: the
+// exact test decides every pair this keeps.
+struct Pre {
+    float cx, cy, cz, rr;
+    __device__ bool near(const float4 p, const float4 q) const {
+        const float px = p.x - cx, py = p.y - cy, pz = p.z - cz;
+        const float qx = q.x - cx, qy = q.y - cy, qz = q.z - cz;
+        return fminf(px * px + py * py + pz * pz, qx * qx + qy * qy + qz * qz) <= rr;
+    }
+};
+__device__ __forceinline__ Pre prefilter_of(const PhysParams& P, const Seg& A, d3 cmA) {
+    Pre f;
+    f.cx = (float)cmA.x;
+    f.cy = (float)cmA.y;
+    f.cz = (float)cmA.z;
+    const float mag = fmaxf(fmaxf(fabsf(f.cx), fabsf(f.cy)), fabsf(f.cz));
+    const float r = (float)(A.L / 2.0 + P.Rc * A.R) + 1e-3f + 1e-6f * mag;
+    f.rr = r * r * 1.0001f;
+    return f;
+}
+
 __device__ __forceinline__ int gather_one(const PhysParams& P, const double* __restrict__ hot, int id, int slot,
                                           const int* __restrict__ start, const int* __restrict__ sorted,
                                           const float4* __restrict__ cand, int* __restrict__ nl, int stride, int cap,
@@ -189,28 +242,11 @@ __device__ __forceinline__ int gather_one(const PhysParams& P, const double* __r
         atomicOr(err, ERRBIT_DOMAIN);
         return 0;
     }
-    // j can pass the reference filter only if |cm_j - cm_A| < reach + |p_j - q_j|/2
-    const float reach = (float)(A.L / 2.0 + P.Rc * A.R) + 1e-3f;
-    const float cx = (float)cmA.x, cy = (float)cmA.y, cz = (float)cmA.z;
+    const Pre pre = prefilter_of(P, A, cmA);
     int m = 0, nq = 0;
-    auto flush = [&]() {
-        for (int e = 0; e < nq; e++) {
-            const int j = sorted[queue[e * qstride]];
-            Seg B;
-            load_seg(hot, j, B);
-            if (keep_neighbour(A, cmA, B, P.Rc)) {
-                if (m < cap) nl[(size_t)m * stride + slot] = j;
-                m++;
-            }
-        }
-        nq = 0;
-    };
-    auto near = [&](const float4& c) {
-        const float dx = c.x - cx, dy = c.y - cy, dz = c.z - cz;
-        const float rr = reach + c.w;
-        return dx * dx + dy * dy + dz * dz <= rr * rr;
-    };
-    auto push = [&](int k) { queue[nq++ * qstride] = k; };
+    auto flush = [&]() { m = flush_queue(P, hot, A, cmA, queue, qstride, nq, nl, stride, slot, cap, m); nq = 0; };
+    auto near = [&](int k) { return pre.near(cand[2 * k], cand[2 * k + 1]); };
+    auto push = [&](int k) { queue[nq++ * qstride] = sorted[k]; };
     const int zlo = max(z0 - 1, 0), zhi = min(z0 + 2, P.BoxZ);
     const int ylo = max(y0 - 1, 0), yhi = min(y0 + 2, P.BoxY);
     const int xlo = max(x0 - 1, 0), xhi = min(x0 + 2, P.BoxX);
@@ -224,8 +260,7 @@ __device__ __forceinline__ int gather_one(const PhysParams& P, const double* __r
             for (; k + 4 <= k1; k += 4) {
                 // flush together: a queue full in one lane empties every queue of the wave
                 if (__any(nq > GQ - 4)) flush();
-                const float4 c0 = cand[k], c1 = cand[k + 1], c2 = cand[k + 2], c3 = cand[k + 3];
-                const bool n0 = near(c0), n1 = near(c1), n2 = near(c2), n3 = near(c3);
+                const bool n0 = near(k), n1 = near(k + 1), n2 = near(k + 2), n3 = near(k + 3);
                 if (n0 && sorted[k] != id) push(k);
                 if (n1 && sorted[k + 1] != id) push(k + 1);
                 if (n2 && sorted[k + 2] != id) push(k + 2);
@@ -233,7 +268,7 @@ __device__ __forceinline__ int gather_one(const PhysParams& P, const double* __r
             }
             for (; k < k1; k++) {
                 if (__any(nq == GQ)) flush();
-                if (near(cand[k]) && sorted[k] != id) push(k);
+                if (near(k) && sorted[k] != id) push(k);
             }
         }
     flush();
@@ -244,6 +279,87 @@ __device__ __forceinline__ int gather_one(const PhysParams& P, const double* __r
     return m;
 }
 
+// Orders one wave's LDS writes before its later LDS reads (the lanes of a
+// wave run in lockstep; the fences keep the compiler from moving accesses).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
+// The list build of the cells of one wave, when they are consecutive in bin
+// order and share one bin x: the wave walks the union of their 27-bin blocks
+// once, (x, y) column by column, staging 64 candidates at a time in LDS (one
+// coalesced load; every lane then reads the same entry), and each lane keeps
+// the candidates that lie in its own 27 bins, are not itself and pass the
+// FP32 prefilter. A lane meets its
+// candidates in ascending bin order, as in gather_one, so the lists are the
+// same. Lanes with live = false take part in the walk and keep nothing.
+__device__ __forceinline__ int gather_wave(const PhysParams& P, const double* __restrict__ hot, bool live, int id,
+                                           int slot, const Seg& A, d3 cmA, int y0, int z0, int xw, int ylo, int yhi,
+                                           int zlo, int zhi, const int* __restrict__ start,
+                                           const int* __restrict__ sorted, const float4* __restrict__ cand,
+                                           int* __restrict__ nl, int stride, int cap, int* __restrict__ queue,
+                                           int qstride, float4* sp, float4* sq, int* ss) {
+    const Pre pre = prefilter_of(P, A, cmA);
+    int m = 0, nq = 0;
+    auto flush = [&]() { m = flush_queue(P, hot, A, cmA, queue, qstride, nq, nl, stride, slot, cap, m); nq = 0; };
+    const int BYZ = P.BoxY * P.BoxZ;
+    const int lane = __lane_id();
+    const float z0f = (float)z0;
+    for (int xa = max(xw - 1, 0); xa <= min(xw + 1, P.BoxX - 1); xa++)
+        for (int ya = ylo; ya <= yhi; ya++) {
+            const bool ady = live && abs(ya - y0) <= 1;
+            if (!__any(ady)) continue;
+            // the column's bins zlo..zhi are one range of the bin order
+            const int a = __builtin_amdgcn_readfirstlane(xa * BYZ + ya * P.BoxZ);
+            const int k1 = __builtin_amdgcn_readfirstlane(start[a + zhi + 1]);
+            for (int kb = __builtin_amdgcn_readfirstlane(start[a + zlo]); kb < k1; kb += 64) {
+                // stage 64 candidates in LDS with one coalesced load, then walk them
+                const int cnt = min(64, k1 - kb);
+                if (lane < cnt) {
+                    sp[lane] = cand[2 * (kb + lane)];
+                    sq[lane] = cand[2 * (kb + lane) + 1];
+                    ss[lane] = sorted[kb + lane];
+                }
+                wave_lds_sync();
+                int c = 0;
+                for (; c + 4 <= cnt; c += 4) {
+                    if (__any(nq > GQ - 4)) flush();
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        // & rather than &&: every load and test runs, with no branch
+                        const float4 pj = sp[c + u], qj = sq[c + u];
+                        const int sj = ss[c + u];
+                        const bool b = ady & (fabsf(pj.w - z0f) <= 1.0f) & (sj != id) & pre.near(pj, qj);
+                        queue[nq * qstride] = sj;
+                        nq += b;
+                    }
+                }
+                for (; c < cnt; c++) {
+                    if (__any(nq == GQ)) flush();
+                    const float4 pj = sp[c], qj = sq[c];
+                    const int sj = ss[c];
+                    const bool b = ady & (fabsf(pj.w - z0f) <= 1.0f) & (sj != id) & pre.near(pj, qj);
+                    queue[nq * qstride] = sj;
+                    nq += b;
+                }
+                wave_lds_sync();
+            }
+        }
+    flush();
+    return m;
+}
+
 __global__ void __launch_bounds__(256) k_gather_all(const double* __restrict__ hot, int n, PhysParams P,
                                                     const int* __restrict__ start, const int* __restrict__ sorted,
                                                     const float4* __restrict__ cand, int* __restrict__ nl,
@@ -251,12 +367,42 @@ __global__ void __launch_bounds__(256) k_gather_all(const double* __restrict__ h
                                                     int* __restrict__ order, int* __restrict__ slot_of,
                                                     int* __restrict__ maxlist, int* __restrict__ err) {
     __shared__ int queue[GQ * 256];
+    __shared__ float4 stage_p[256], stage_q[256];  // per wave: 64 candidates staged for the walk
+    __shared__ int stage_s[256];
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int id = sorted[t];
+    // every lane of a wave stays to the end (the wave walk uses cross-lane
+    // operations); lanes past the last cell are not live
+    const bool live = t < n;
+    if (!__any(live)) return;
+    const int id = live ? sorted[t] : 0;
+    Seg A;
+    load_seg(hot, id, A);
+    const d3 cmA = cm_of(A);
+    int x0 = 0, y0 = 0, z0 = 0;
+    const bool inside = live && bin_of(P, cmA, x0, y0, z0) >= 0;
+    if (live && !inside) atomicOr(err, ERRBIT_DOMAIN);
+    const bool walk = inside;
+    // one bin column x for the whole wave: walk the union of the blocks
+    const int xmin = wave_min(walk ? x0 : INT_MAX), xmax = wave_max(walk ? x0 : INT_MIN);
+    int m = 0;
+    if (xmin == xmax) {
+        const int ylo = max(wave_min(walk ? y0 : INT_MAX) - 1, 0);
+        const int yhi = min(wave_max(walk ? y0 : INT_MIN) + 1, P.BoxY - 1);
+        const int zlo = max(wave_min(walk ? z0 : INT_MAX) - 1, 0);
+        const int zhi = min(wave_max(walk ? z0 : INT_MIN) + 1, P.BoxZ - 1);
+        const int w = threadIdx.x / 64;
+        m = gather_wave(P, hot, walk, id, t, A, cmA, y0, z0, xmin, ylo, yhi, zlo, zhi, start, sorted, cand, nl,
+                        stride, cap, queue + threadIdx.x, 256, stage_p + 64 * w, stage_q + 64 * w, stage_s + 64 * w);
+    } else if (walk) {
+        m = gather_one(P, hot, id, t, start, sorted, cand, nl, stride, cap, err, queue + threadIdx.x, 256);
+    }
+    if (!live) return;
+    if (m > cap) {
+        atomicOr(err, ERRBIT_NLIST);
+        m = cap;
+    }
     order[t] = id;
     slot_of[id] = t;
-    int m = gather_one(P, hot, id, t, start, sorted, cand, nl, stride, cap, err, queue + threadIdx.x, 256);
     nl_count[t] = m;
     atomicMax(maxlist, m);
 }
