@@ -293,8 +293,21 @@ void vcycle(bcg_rd3d* c, int l) {
                        w.nx, w.ny, w.nz, v.cx, v.cy, v.cz);
     hipLaunchKernelGGL((k_fill<T>), dim3(nblk(w.n(), 256)), dim3(256), 0, c->stream, Arr<T>::u(w), w.n(), (T)0);
     vcycle<T>(c, l + 1);
-    hipLaunchKernelGGL((k_prolong<T>), grid2(v), dim3(64, 4), 0, c->stream, Arr<T>::u(v), Arr<T>::u(w), v.nx, v.ny, v.nz,
-                       w.nx, w.ny, v.cx, v.cy, v.cz);
+    if ((long)v.nx * v.ny * v.nz >= FUSED_MIN_POINTS) {
+        // z-marching: a coarse plane's interpolant serves two fine planes
+        // (short ranges: many columns in flight hide the latency of each
+        // column's load-add-store chain)
+        const long cols = (long)nblk(v.nx, 64) * nblk(v.ny, 4);
+        int nc = (int)((4 * ZM_TARGET_BLOCKS + cols - 1) / cols);
+        nc = std::max(1, std::min(nc, std::max(1, v.nz / 8)));
+        const int kchunk = (v.nz + nc - 1) / nc;
+        hipLaunchKernelGGL((k_zm_prolong<T>), dim3(nblk(v.nx, 64), nblk(v.ny, 4), (v.nz + kchunk - 1) / kchunk),
+                           dim3(256), 0, c->stream, Arr<T>::u(v), Arr<T>::u(w), v.nx, v.ny, v.nz, w.nx, w.ny, v.cx,
+                           v.cy, v.cz, kchunk);
+    } else {
+        hipLaunchKernelGGL((k_prolong<T>), grid2(v), dim3(64, 4), 0, c->stream, Arr<T>::u(v), Arr<T>::u(w), v.nx, v.ny,
+                           v.nz, w.nx, w.ny, v.cx, v.cy, v.cz);
+    }
     smooth<T>(c, v, c->prm.nu2);
 }
 

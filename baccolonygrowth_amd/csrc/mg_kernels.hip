@@ -179,6 +179,79 @@ __global__ void __launch_bounds__(256) k_prolong(T* __restrict__ u, const T* __r
     u[((long)k * ny + j) * nx + i] += prolong_point<T>(ec, ncx, ncy, i, j, k, cx, cy, cz);
 }
 
+// u_fine += P e_coarse, a thread per fine (x, y) column marching up planes
+// [k0, k0 + kchunk): the x-y interpolated coarse value of a coarse plane,
+// the e0 / e1 of prolong_point, is formed once and serves the two fine
+// planes it touches. Same operations as prolong_point.
+template <class T>
+__global__ void __launch_bounds__(256) k_zm_prolong(T* __restrict__ u, const T* __restrict__ ec, int nx, int ny, int nz,
+                                                    int ncx, int ncy, int cx, int cy, int cz, int kchunk) {
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= nx || j >= ny) return;
+    const int k0 = blockIdx.z * kchunk, k1 = min(k0 + kchunk, nz);
+    int I0, I1, J0, J1;
+    T wi, wj;
+    if (cx) {
+        I0 = i >> 1;
+        I1 = (i + 1) >> 1;
+        wi = (i & 1) ? T(0.5) : T(0);
+    } else {
+        I0 = I1 = i;
+        wi = T(0);
+    }
+    if (cy) {
+        J0 = j >> 1;
+        J1 = (j + 1) >> 1;
+        wj = (j & 1) ? T(0.5) : T(0);
+    } else {
+        J0 = J1 = j;
+        wj = T(0);
+    }
+    const long scxy = (long)ncx * ncy;
+    const long o00 = (long)J0 * ncx + I0, o01 = (long)J0 * ncx + I1;
+    const long o10 = (long)J1 * ncx + I0, o11 = (long)J1 * ncx + I1;
+    auto plane = [&](int K) {
+        const T* p = ec + K * scxy;
+        const T r0 = (T(1) - wi) * p[o00] + wi * p[o01];
+        const T r1 = (T(1) - wi) * p[o10] + wi * p[o11];
+        return (T(1) - wj) * r0 + wj * r1;
+    };
+    // two cached coarse planes; K only grows with k
+    int ka = -1, kb = -1;
+    T ea = T(0), eb = T(0);
+    auto val = [&](int K) {
+        if (K == ka) return ea;
+        if (K == kb) return eb;
+        const T v = plane(K);
+        if (ka < kb) {
+            ka = K;
+            ea = v;
+        } else {
+            kb = K;
+            eb = v;
+        }
+        return v;
+    };
+    const long sxy = (long)nx * ny;
+    T* up = u + (long)j * nx + i;
+    for (int k = k0; k < k1; k++) {
+        int K0, K1;
+        T wk;
+        if (cz) {
+            K0 = k >> 1;
+            K1 = (k + 1) >> 1;
+            wk = (k & 1) ? T(0.5) : T(0);
+        } else {
+            K0 = K1 = k;
+            wk = T(0);
+        }
+        const T e0 = val(K0);
+        const T e1 = val(K1);
+        up[k * sxy] += (T(1) - wk) * e0 + wk * e1;
+    }
+}
+
 template <class T>
 __global__ void k_fill(T* __restrict__ a, long n, T v) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -636,6 +709,8 @@ template __global__ void k_restrict<float>(const float*, float*, int, int, int, 
 template __global__ void k_restrict<double>(const double*, double*, int, int, int, int, int, int, int, int, int);
 template __global__ void k_prolong<float>(float*, const float*, int, int, int, int, int, int, int, int);
 template __global__ void k_prolong<double>(double*, const double*, int, int, int, int, int, int, int, int);
+template __global__ void k_zm_prolong<float>(float*, const float*, int, int, int, int, int, int, int, int, int);
+template __global__ void k_zm_prolong<double>(double*, const double*, int, int, int, int, int, int, int, int, int);
 template __global__ void k_fill<float>(float*, long, float);
 template __global__ void k_fill<double>(double*, long, double);
 template __global__ void k_cast<float, double>(const float*, double*, long);
