@@ -56,6 +56,28 @@ def phase_bytes(kbar: float) -> dict:
     }
 
 
+# kernel of each timed phase (the roofline's dominant kernel)
+PHASE_KERNEL = {"bins": "k_bin_count", "lists": "k_gather_all", "move_grow": "k_move_grow",
+                "compact": "k_flag_write", "divide": "k_divide"}
+PROFILE = os.path.join(ROOT, "profiles", "bench_default_kernels.txt")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy calibration) from the rocprofv3 summary of this workload in
+    profiles/, or None when the summary has no row for it."""
+    try:
+        with open(PROFILE) as fh:
+            for line in fh:
+                f = line.split()
+                if len(f) == 7 and f[0] == kernel:
+                    v = (float(f[4]) + float(f[5])) * 1e6
+                    return v if v == v else None
+    except OSError:
+        return None
+    return None
+
+
 def make_workload(n: int, seed: int = 2):
     from baccolonygrowth_amd.colony import dome_colony, dome_params, height_normal_from_cells
 
@@ -353,7 +375,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(PHASE_KERNEL[dom]),
+            "traffic_source": os.path.relpath(PROFILE, ROOT),
             "bytes_per_cell": bpc[dom],
         },
         "cpu_baseline": cpu,
