@@ -105,9 +105,9 @@ int build_bins(bcg_ctx* c, int buf, int n) {
     if (rc) return rc;
     if (n > 0) {
         hipLaunchKernelGGL(k_bin_scatter, dim3(nblk(n, 256)), dim3(256), 0, c->stream, c->key, n, c->bin_start,
-                           c->bin_fill, c->sorted);
-        hipLaunchKernelGGL(k_bin_sort, dim3(nblk(c->nbin, 256)), dim3(256), 0, c->stream, c->bin_start, c->nbin,
-                           c->sorted);
+                           c->bin_fill, c->scattered);
+        hipLaunchKernelGGL(k_bin_rank, dim3(nblk(n, 256)), dim3(256), 0, c->stream, c->scattered, n, c->key,
+                           c->bin_start, c->sorted);
         hipLaunchKernelGGL(k_bin_cand, dim3(nblk(n, 256)), dim3(256), 0, c->stream, c->sorted, n, c->cmL, c->cand);
     }
     HIPCHK(c, hipGetLastError());
@@ -236,6 +236,7 @@ int bcg_create(const bcg_params* p, bcg_ctx** out) {
     A(&c->bin_start, c->nbin + 1);
     A(&c->bin_fill, c->nbin);
     A(&c->sorted, M);
+    A(&c->scattered, M);
     A(&c->cmL, 2 * (size_t)M);   // FP32 end points p, q per cell
     A(&c->cand, 2 * (size_t)M);
     A(&c->scan_tot, 1024 * 64);
@@ -293,7 +294,7 @@ int bcg_destroy(bcg_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->hot[0],   c->hot[1],    c->time_p,  c->time_q,     c->dynfric,   c->stafric,  c->age_p,
                     c->age_q,    c->type,      c->ancestor, c->key,       c->bin_cnt,   c->bin_start, c->bin_fill,
-                    c->sorted,   c->scan_tot, c->cmL, c->cand,  c->nl,      c->nl_count,   c->order,     c->slot_of,  c->divflag,
+                    c->sorted,   c->scattered, c->scan_tot, c->cmL, c->cand,  c->nl,      c->nl_count,   c->order,     c->slot_of,  c->divflag,
                     c->div_ids,  c->div_index_of, c->ids_tmp, c->draws,   c->mother_pre, c->height,  c->normal,
                     c->wall,     c->gr1,       c->gr2,     c->Fbuf,       c->Tbuf,      c->stats};
     for (void* q : ptrs)

@@ -135,20 +135,19 @@ __global__ void k_bin_scatter(const int* __restrict__ key, int n, const int* __r
     sorted[pos] = i;
 }
 
-// Ascending IDs inside each bin (insertion sort; bins hold tens of cells).
-__global__ void k_bin_sort(const int* __restrict__ start, int nbin, int* __restrict__ sorted) {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nbin) return;
-    int s = start[b], e = start[b + 1];
-    for (int i = s + 1; i < e; i++) {
-        int v = sorted[i];
-        int j = i - 1;
-        while (j >= s && sorted[j] > v) {
-            sorted[j + 1] = sorted[j];
-            j--;
-        }
-        sorted[j + 1] = v;
-    }
+// Ascending IDs inside each bin: the cell at position p of the scattered
+// order goes to its bin's start plus the number of cells of the bin with a
+// smaller ID (bins hold tens of cells; a thread per cell).
+__global__ void k_bin_rank(const int* __restrict__ scattered, int n, const int* __restrict__ key,
+                           const int* __restrict__ start, int* __restrict__ sorted) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int id = scattered[p];
+    const int a = key[id];
+    const int s = start[a], e = start[a + 1];
+    int r = 0;
+    for (int q = s; q < e; q++) r += scattered[q] < id;
+    sorted[s + r] = id;
 }
 
 // Candidate records in bin order: the FP32 end points p, q of cell sorted[pos].
