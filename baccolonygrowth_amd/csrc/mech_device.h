@@ -120,11 +120,12 @@ __device__ __forceinline__ int bin_of(const PhysParams& P, d3 pos, int& x, int& 
     return x * (P.BoxY * P.BoxZ) + y * P.BoxZ + z;
 }
 
-// F_cc: ABM/Forces.cpp:13-84, contribution of neighbour B to the force sum of
-// A (returns F and the torque arm r - cmA). B's kinematics are loaded only if
-// the cells touch.
-__device__ __forceinline__ void F_cc_acc(const PhysParams& P, const Seg& A, const Kin& KA, d3 cmA, const Seg& B,
-                                         const double* __restrict__ hot, int idB, d3& Fnet, d3& Tnet) {
+// F_cc: ABM/Forces.cpp:13-84. The force F of neighbour B on A and its torque
+// Tc = (r - cmA) x F about A's centre. The kinematics come from kinA() and
+// kinB(), called only if the cells touch.
+template <class KA_, class KB_>
+__device__ __forceinline__ void F_cc(const PhysParams& P, const Seg& A, d3 cmA, const Seg& B, KA_ kinA, KB_ kinB, d3& F,
+                                     d3& Tc) {
     const double gamma_n = 100.0;
     d3 c1, c2;
     double sigma = A.R + B.R;
@@ -137,11 +138,11 @@ __device__ __forceinline__ void F_cc_acc(const PhysParams& P, const Seg& A, cons
     double delta = dmax(sigma - d, 0.0);
     d3 vn = scl(sub(c2, c1), 1.0 / d);
     double FE_mag = sqrt(A.R * B.R / (A.R + B.R)) * P.k_cc * pow(delta, 3.0 / 2.0);
-    d3 F = scl(vn, -FE_mag);
+    F = scl(vn, -FE_mag);
     d3 r = scl(add(scl(c1, B.R), scl(c2, A.R)), 1 / sigma);
     if (FE_mag > DBL_EPSILON) {
-        Kin KB;
-        load_kin(hot, idB, KB);
+        const Kin KA = kinA();
+        const Kin KB = kinB();
         d3 rcm1 = sub(r, cmA), rcm2 = sub(r, cm_of(B));
         d3 va1 = cross(KA.w, rcm1), va2 = cross(KB.w, rcm2);
         d3 v1 = add(KA.v, va1), v2 = add(KB.v, va2);
@@ -154,9 +155,24 @@ __device__ __forceinline__ void F_cc_acc(const PhysParams& P, const Seg& A, cons
         if (v_mag > 3 * DBL_EPSILON) FDt = scl(dvt, dmin(P.gamma_t * Meff * sqrt(delta), P.cell_mu * FE_mag / v_mag));
         F = add(add(F, FDt), FDn);
     }
-    Fnet = add(Fnet, F);
     r = sub(r, cmA);
-    Tnet = add(Tnet, cross(r, F));
+    Tc = cross(r, F);
+}
+
+// F_cc accumulated into A's running sums in the reference's order.
+__device__ __forceinline__ void F_cc_acc(const PhysParams& P, const Seg& A, const Kin& KA, d3 cmA, const Seg& B,
+                                         const double* __restrict__ hot, int idB, d3& Fnet, d3& Tnet) {
+    d3 F, Tc;
+    F_cc(
+        P, A, cmA, B, [&]() { return KA; },
+        [&]() {
+            Kin KB;
+            load_kin(hot, idB, KB);
+            return KB;
+        },
+        F, Tc);
+    Fnet = add(Fnet, F);
+    Tnet = add(Tnet, Tc);
 }
 
 // One locus of F_cw: ABM/Forces.cpp:98-152 / :157-201.

@@ -423,29 +423,12 @@ __global__ void __launch_bounds__(256) k_gather_ids(const double* __restrict__ h
 }
 
 // -------------------------------------------------------- move + grow ---
-// sum_forces (ABM/Forces.cpp:332-381) of cell S against its list.
-__device__ __forceinline__ void sum_forces(const PhysParams& P, const Seg& S, const Kin& K, const double* __restrict__ hot,
-                                           const int* __restrict__ nl, int stride, int slot, int cnt, double wall_z,
-                                           const double* __restrict__ H, const double* __restrict__ N, d3& F, d3& T,
-                                           d3& sf, d3& df, int& draws) {
-    F = mk(0, 0, 0);
-    T = mk(0, 0, 0);
-    const d3 cm = cm_of(S);
-    // the next neighbour's record is requested before this one is used
-    int j = cnt > 0 ? nl[slot] : 0;
-    Seg B;
-    if (cnt > 0) load_seg(hot, j, B);
-    for (int k = 0; k < cnt; k++) {
-        int jn = j;
-        Seg Bn = B;
-        if (k + 1 < cnt) {
-            jn = nl[(size_t)(k + 1) * stride + slot];
-            load_seg(hot, jn, Bn);
-        }
-        F_cc_acc(P, S, K, cm, B, hot, j, F, T);
-        j = jn;
-        B = Bn;
-    }
+// The rest of sum_forces after the cell-cell loop (ABM/Forces.cpp:349-380):
+// wall contact, viscous drag, surface tension, planar constraint. F and T
+// hold the cell-cell sums on entry.
+__device__ __forceinline__ void finish_forces(const PhysParams& P, const Seg& S, const Kin& K, d3 cm, double wall_z,
+                                              const double* __restrict__ H, const double* __restrict__ N, d3& F, d3& T,
+                                              d3& sf, d3& df, int& draws) {
     sf = mk(0, 0, 0);
     df = mk(0, 0, 0);
     if (dmin(S.q.z, S.p.z) < 1.2 * S.R) {
@@ -474,6 +457,134 @@ __device__ __forceinline__ void sum_forces(const PhysParams& P, const Seg& S, co
     }
 }
 
+// sum_forces (ABM/Forces.cpp:332-381) of cell S against its list.
+__device__ __forceinline__ void sum_forces(const PhysParams& P, const Seg& S, const Kin& K, const double* __restrict__ hot,
+                                           const int* __restrict__ nl, int stride, int slot, int cnt, double wall_z,
+                                           const double* __restrict__ H, const double* __restrict__ N, d3& F, d3& T,
+                                           d3& sf, d3& df, int& draws) {
+    F = mk(0, 0, 0);
+    T = mk(0, 0, 0);
+    const d3 cm = cm_of(S);
+    // the next neighbour's record is requested before this one is used
+    int j = cnt > 0 ? nl[slot] : 0;
+    Seg B;
+    if (cnt > 0) load_seg(hot, j, B);
+    for (int k = 0; k < cnt; k++) {
+        int jn = j;
+        Seg Bn = B;
+        if (k + 1 < cnt) {
+            jn = nl[(size_t)(k + 1) * stride + slot];
+            load_seg(hot, jn, Bn);
+        }
+        F_cc_acc(P, S, K, cm, B, hot, j, F, T);
+        j = jn;
+        B = Bn;
+    }
+    finish_forces(P, S, K, cm, wall_z, H, N, F, T, sf, df, draws);
+}
+
+// Cell state of one block of k_move_grow staged in LDS for its pair phases
+// (structure of arrays, one column per cell of the block).
+struct MoveLds {
+    double seg[8][MOVE_BLOCK];  // p.x p.y p.z q.x q.y q.z L R
+    double kin[6][MOVE_BLOCK];  // v.x v.y v.z w.x w.y w.z
+    double con[6][MOVE_BLOCK];  // one chunk of pair contributions: F.xyz, torque.xyz
+    int off[MOVE_BLOCK + 1];    // exclusive scan of the cells' list lengths
+    int scan[48];
+    int draws[MOVE_BLOCK / 64];
+};
+
+__device__ __forceinline__ void lds_put_seg(MoveLds& L, int c, const Seg& S) {
+    L.seg[0][c] = S.p.x;
+    L.seg[1][c] = S.p.y;
+    L.seg[2][c] = S.p.z;
+    L.seg[3][c] = S.q.x;
+    L.seg[4][c] = S.q.y;
+    L.seg[5][c] = S.q.z;
+    L.seg[6][c] = S.L;
+    L.seg[7][c] = S.R;
+}
+__device__ __forceinline__ void lds_put_kin(MoveLds& L, int c, const Kin& K) {
+    L.kin[0][c] = K.v.x;
+    L.kin[1][c] = K.v.y;
+    L.kin[2][c] = K.v.z;
+    L.kin[3][c] = K.w.x;
+    L.kin[4][c] = K.w.y;
+    L.kin[5][c] = K.w.z;
+}
+__device__ __forceinline__ Seg lds_seg(const MoveLds& L, int c) {
+    Seg S;
+    S.p = mk(L.seg[0][c], L.seg[1][c], L.seg[2][c]);
+    S.q = mk(L.seg[3][c], L.seg[4][c], L.seg[5][c]);
+    S.L = L.seg[6][c];
+    S.R = L.seg[7][c];
+    return S;
+}
+__device__ __forceinline__ Kin lds_kin(const MoveLds& L, int c) {
+    Kin K;
+    K.v = mk(L.kin[0][c], L.kin[1][c], L.kin[2][c]);
+    K.w = mk(L.kin[3][c], L.kin[4][c], L.kin[5][c]);
+    return K;
+}
+
+// The cell-cell sums (the neighbour loop of ABM/Forces.cpp:341-347) of every
+// cell of the block against the segments and kinematics staged in L. The
+// block's (cell, list entry) pairs are spread over its threads, MOVE_BLOCK
+// pairs per chunk: each thread evaluates F_cc for one pair into the chunk
+// buffer, then each cell's own thread adds its pairs in list order. These are
+// the reference's additions in the reference's order, so F and T are
+// bit-identical to the sequential loop, while no lane idles behind a longer
+// list elsewhere in its wave.
+__device__ __forceinline__ void pair_sums(const PhysParams& P, MoveLds& L, const double* __restrict__ hot,
+                                          const int* __restrict__ nl, int stride, int slot0, d3& F, d3& T) {
+    __syncthreads();
+    const int tid = threadIdx.x;
+    const int total = L.off[MOVE_BLOCK];
+    const int lo = L.off[tid], hi = L.off[tid + 1];
+    F = mk(0, 0, 0);
+    T = mk(0, 0, 0);
+    for (int base = 0; base < total; base += MOVE_BLOCK) {
+        const int p = base + tid;
+        if (p < total) {
+            // the owner of pair p: the largest c with off[c] <= p (off[0] = 0)
+            int c = 0;
+#pragma unroll
+            for (int s = MOVE_BLOCK / 2; s > 0; s >>= 1)
+                if (L.off[c + s] <= p) c += s;
+            const int j = nl[(size_t)(p - L.off[c]) * stride + slot0 + c];
+            Seg B;
+            load_seg(hot, j, B);
+            const Seg A = lds_seg(L, c);
+            d3 f, tc;
+            F_cc(
+                P, A, cm_of(A), B, [&]() { return lds_kin(L, c); },
+                [&]() {
+                    Kin KB;
+                    load_kin(hot, j, KB);
+                    return KB;
+                },
+                f, tc);
+            L.con[0][tid] = f.x;
+            L.con[1][tid] = f.y;
+            L.con[2][tid] = f.z;
+            L.con[3][tid] = tc.x;
+            L.con[4][tid] = tc.y;
+            L.con[5][tid] = tc.z;
+        }
+        __syncthreads();
+        const int a = lo > base ? lo : base;
+        const int b = hi < base + MOVE_BLOCK ? hi : base + MOVE_BLOCK;
+        for (int q = a - base; q < b - base; q++) {
+            F = add(F, mk(L.con[0][q], L.con[1][q], L.con[2][q]));
+            T = add(T, mk(L.con[3][q], L.con[4][q], L.con[5][q]));
+        }
+        __syncthreads();
+    }
+}
+
+// MoveCell + GrowCell for every cell (ABM/Simulation.cpp:330-335): integrate
+// (ABM/Integrate.cpp:11-31) with both force sums, then grow. Block b owns the
+// bin-ordered slots [b' * MOVE_BLOCK, (b' + 1) * MOVE_BLOCK), b' its XCD remap.
 __global__ void __launch_bounds__(MOVE_BLOCK) k_move_grow(
     const double* __restrict__ hot_old, double* __restrict__ hot_new, int n, PhysParams P, double dt,
     const int* __restrict__ order, const int* __restrict__ nl, const int* __restrict__ nl_count, int stride,
@@ -481,21 +592,23 @@ __global__ void __launch_bounds__(MOVE_BLOCK) k_move_grow(
     const double* __restrict__ gr1, const double* __restrict__ gr2, const int* __restrict__ type,
     double* __restrict__ dynfric, double* __restrict__ stafric, int* __restrict__ divflag,
     unsigned long long* __restrict__ wall_draws, int* __restrict__ err) {
-    __shared__ int lds_draws[MOVE_BLOCK / 64];
-    const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    __shared__ MoveLds L;
+    const int tid = threadIdx.x;
+    const int slot0 = xcd_block(blockIdx.x, gridDim.x) * MOVE_BLOCK;
+    const int t = slot0 + tid;
     int draws = 0;
+    bool live = false;  // a cell inside the bin grid
+    int id = 0, cnt = 0;
+    double Ldiv = 0, wall_z = 0;
     if (t < n) {
-        const int id = order[t];
+        id = order[t];
         Seg S;
         Kin K;
         load_seg(hot_old, id, S);
         load_kin(hot_old, id, K);
-        const double2 ld = reinterpret_cast<const double2*>(hot_old + (size_t)id * HOT)[7];
-        const double Ldiv = ld.x;
-        d3 cm = cm_of(S);
+        Ldiv = hot_old[(size_t)id * HOT + 14];
         int x, y, z;
-        int a = bin_of(P, cm, x, y, z);  // MoveCell oldAddress: ABM/Cells.cpp:25-26
-        if (a < 0) {
+        if (bin_of(P, cm_of(S), x, y, z) < 0) {  // MoveCell oldAddress: ABM/Cells.cpp:25-26
             // reported; the cell is carried over unchanged and does not divide
             atomicOr(err, ERRBIT_DOMAIN);
             const double2* src = reinterpret_cast<const double2*>(hot_old + (size_t)id * HOT);
@@ -503,70 +616,93 @@ __global__ void __launch_bounds__(MOVE_BLOCK) k_move_grow(
             for (int c = 0; c < HOT / 2; c++) dst[c] = src[c];
             divflag[id] = 0;
         } else {
-            const double wall_z = wall[x * P.BoxY + y];
-            const int cnt = nl_count[t];
-            d3 F, T, sf, df;
-            // integrate: ABM/Integrate.cpp:11-31
-            sum_forces(P, S, K, hot_old, nl, stride, t, cnt, wall_z, H, Nrm, F, T, sf, df, draws);
-            // UpdatePositions: ABM/Integrate.cpp:34-90
-            Kin K1;
-            half_kick(dt, F, T, S, K, K1);
-            Seg S1;
-            S1.p = add(S.p, scl(add(K1.v, cross(K1.w, sub(S.p, cm))), dt));
-            S1.q = add(S.q, scl(add(K1.v, cross(K1.w, sub(S.q, cm))), dt));
-            S1.L = S.L;
-            S1.R = S.R;
-            if (P.planar) {
-                S1.p.y = 0;
-                S1.q.y = 0;
-            }
-            sum_forces(P, S1, K1, hot_old, nl, stride, t, cnt, wall_z, H, Nrm, F, T, sf, df, draws);
-            // UpdateVelocities: ABM/Integrate.cpp:93-127
-            Kin K2;
-            half_kick(dt, F, T, S1, K1, K2);
-            if (P.planar) K2.v.y = 0;
-            // grow: ABM/grow.cpp:17-102 at the MoveCell newAddress (ABM/Cells.cpp:32)
-            int a2 = bin_of(P, cm_of(S1), x, y, z);
-            double GR = 0;
-            if (a2 < 0) {
-                atomicOr(err, ERRBIT_DOMAIN);
-            } else {
-                GR = (P.crossFeeding && type[id] != 1) ? gr2[a2] : gr1[a2];
-                GR = GR * 1.5850;
-                d3 v = sub(S1.q, S1.p);
-                double dL = S1.L * dt * GR;
-                d3 dv = scl(v, dL * 0.5 / S1.L);
-                S1.L += dL;
-                S1.p = sub(S1.p, dv);
-                S1.q = add(S1.q, dv);
-            }
-            double2* o = reinterpret_cast<double2*>(hot_new + (size_t)id * HOT);
-            o[0] = make_double2(S1.p.x, S1.p.y);
-            o[1] = make_double2(S1.p.z, S1.q.x);
-            o[2] = make_double2(S1.q.y, S1.q.z);
-            o[3] = make_double2(K2.v.x, K2.v.y);
-            o[4] = make_double2(K2.v.z, K2.w.x);
-            o[5] = make_double2(K2.w.y, K2.w.z);
-            o[6] = make_double2(S1.L, S1.R);
-            o[7] = make_double2(Ldiv, GR);
-            dynfric[3 * (size_t)id] = df.x;
-            dynfric[3 * (size_t)id + 1] = df.y;
-            dynfric[3 * (size_t)id + 2] = df.z;
-            stafric[3 * (size_t)id] = sf.x;
-            stafric[3 * (size_t)id + 1] = sf.y;
-            stafric[3 * (size_t)id + 2] = sf.z;
-            divflag[id] = (S1.L >= Ldiv) ? 1 : 0;  // GrowCell: ABM/Cells.cpp:48
+            live = true;
+            wall_z = wall[x * P.BoxY + y];
+            cnt = nl_count[t];
+            lds_put_seg(L, tid, S);
+            lds_put_kin(L, tid, K);
         }
+    }
+    int total;
+    const int o = block_excl_scan(cnt, L.scan, &total);
+    L.off[tid] = o;
+    if (tid == 0) L.off[MOVE_BLOCK] = total;
+
+    // pass 1 at the old state
+    d3 F, T, sf, df;
+    pair_sums(P, L, hot_old, nl, stride, slot0, F, T);
+    if (live) {
+        const Seg S = lds_seg(L, tid);
+        const Kin K = lds_kin(L, tid);
+        const d3 cm = cm_of(S);
+        finish_forces(P, S, K, cm, wall_z, H, Nrm, F, T, sf, df, draws);
+        // UpdatePositions: ABM/Integrate.cpp:34-90
+        Kin K1;
+        half_kick(dt, F, T, S, K, K1);
+        Seg S1;
+        S1.p = add(S.p, scl(add(K1.v, cross(K1.w, sub(S.p, cm))), dt));
+        S1.q = add(S.q, scl(add(K1.v, cross(K1.w, sub(S.q, cm))), dt));
+        S1.L = S.L;
+        S1.R = S.R;
+        if (P.planar) {
+            S1.p.y = 0;
+            S1.q.y = 0;
+        }
+        lds_put_seg(L, tid, S1);
+        lds_put_kin(L, tid, K1);
+    }
+    // pass 2 at the half step, against the neighbours' old records
+    pair_sums(P, L, hot_old, nl, stride, slot0, F, T);
+    if (live) {
+        Seg S1 = lds_seg(L, tid);
+        const Kin K1 = lds_kin(L, tid);
+        finish_forces(P, S1, K1, cm_of(S1), wall_z, H, Nrm, F, T, sf, df, draws);
+        // UpdateVelocities: ABM/Integrate.cpp:93-127
+        Kin K2;
+        half_kick(dt, F, T, S1, K1, K2);
+        if (P.planar) K2.v.y = 0;
+        // grow: ABM/grow.cpp:17-102 at the MoveCell newAddress (ABM/Cells.cpp:32)
+        int x, y, z;
+        int a2 = bin_of(P, cm_of(S1), x, y, z);
+        double GR = 0;
+        if (a2 < 0) {
+            atomicOr(err, ERRBIT_DOMAIN);
+        } else {
+            GR = (P.crossFeeding && type[id] != 1) ? gr2[a2] : gr1[a2];
+            GR = GR * 1.5850;
+            d3 v = sub(S1.q, S1.p);
+            double dL = S1.L * dt * GR;
+            d3 dv = scl(v, dL * 0.5 / S1.L);
+            S1.L += dL;
+            S1.p = sub(S1.p, dv);
+            S1.q = add(S1.q, dv);
+        }
+        double2* out = reinterpret_cast<double2*>(hot_new + (size_t)id * HOT);
+        out[0] = make_double2(S1.p.x, S1.p.y);
+        out[1] = make_double2(S1.p.z, S1.q.x);
+        out[2] = make_double2(S1.q.y, S1.q.z);
+        out[3] = make_double2(K2.v.x, K2.v.y);
+        out[4] = make_double2(K2.v.z, K2.w.x);
+        out[5] = make_double2(K2.w.y, K2.w.z);
+        out[6] = make_double2(S1.L, S1.R);
+        out[7] = make_double2(Ldiv, GR);
+        dynfric[3 * (size_t)id] = df.x;
+        dynfric[3 * (size_t)id + 1] = df.y;
+        dynfric[3 * (size_t)id + 2] = df.z;
+        stafric[3 * (size_t)id] = sf.x;
+        stafric[3 * (size_t)id + 1] = sf.y;
+        stafric[3 * (size_t)id + 2] = sf.z;
+        divflag[id] = (S1.L >= Ldiv) ? 1 : 0;  // GrowCell: ABM/Cells.cpp:48
     }
     // block sum of F_cw draws -> one 64-bit atomic (integer: order-free)
     int w = draws;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o, 64);
-    if ((threadIdx.x & 63) == 0) lds_draws[threadIdx.x >> 6] = w;
+    for (int s = 32; s > 0; s >>= 1) w += __shfl_xor(w, s, 64);
+    if ((tid & 63) == 0) L.draws[tid >> 6] = w;
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         int s = 0;
-        for (int i = 0; i < MOVE_BLOCK / 64; i++) s += lds_draws[i];
+        for (int i = 0; i < MOVE_BLOCK / 64; i++) s += L.draws[i];
         if (s) atomicAdd(wall_draws, (unsigned long long)s);
     }
 }
