@@ -67,8 +67,21 @@ __device__ __forceinline__ void load_kin(const double* __restrict__ hot, int id,
     k.w = mk(b.y, c.x, c.y);
 }
 
-// Segment-segment closest points: ABM/Neighbours.cpp:107-160.
-__device__ __forceinline__ double min_distance(const Seg& A, const Seg& B, d3& c1, d3& c2) {
+// sqrt(x) < t (x >= 0, t > 0 normal) exactly as the FP64 expression with a
+// correctly rounded square root decides it: when x is below t^2 (1 - 2^-48)
+// the rounded root is below t(1 - 2^-50) < t, and above t^2 (1 + 2^-48) it
+// exceeds t; only in between is the root taken.
+__device__ __forceinline__ bool sqrt_lt(double x, double t) {
+    const double t2 = t * t;
+    bool r = x < t2 * (1.0 - 0x1p-48);
+    const bool amb = !r & (x <= t2 * (1.0 + 0x1p-48));
+    if (amb) r = sqrt(x) < t;
+    return r;
+}
+
+// Segment-segment closest points: ABM/Neighbours.cpp:107-160 (the squared
+// distance; min_distance takes the root as the reference does).
+__device__ __forceinline__ double min_distance_sq(const Seg& A, const Seg& B, d3& c1, d3& c2) {
     d3 v1 = sub(A.q, A.p), v2 = sub(B.q, B.p), r = sub(A.p, B.p);
     double l1 = dot(v1, v1), l2 = dot(v2, v2), f = dot(v2, r), c = dot(v1, r), b = dot(v1, v2);
     double denom = l1 * l2 - b * b;
@@ -88,19 +101,24 @@ __device__ __forceinline__ double min_distance(const Seg& A, const Seg& B, d3& c
     c1 = add(A.p, scl(v1, s));
     c2 = add(B.p, scl(v2, t));
     d3 cv = sub(c1, c2);
-    return sqrt(dot(cv, cv));
+    return dot(cv, cv);
+}
+__device__ __forceinline__ double min_distance(const Seg& A, const Seg& B, d3& c1, d3& c2) {
+    return sqrt(min_distance_sq(A, B, c1, c2));
 }
 
 // reduceNeighbours predicate for neighbour B of cell A: ABM/Neighbours.cpp:72-88.
+// The square roots are compared through sqrt_lt, which decides each
+// comparison as the reference's sqrt(...) < threshold does.
 __device__ __forceinline__ bool keep_neighbour(const Seg& A, d3 cmA, const Seg& B, double Rc) {
+    const double thr = A.L / 2.0 + Rc * A.R;
     d3 v = sub(B.p, cmA);
-    double dp = sqrt(dot(v, v));
+    const double dp2 = dot(v, v);
     v = sub(B.q, cmA);
-    double dq = sqrt(dot(v, v));
-    if ((dp < (A.L / 2.0 + Rc * A.R)) || (dq < (A.L / 2.0 + Rc * A.R))) {
+    const double dq2 = dot(v, v);
+    if (sqrt_lt(dp2, thr) || sqrt_lt(dq2, thr)) {
         d3 c1, c2;
-        double d = min_distance(A, B, c1, c2);
-        return d < Rc * A.R;
+        return sqrt_lt(min_distance_sq(A, B, c1, c2), Rc * A.R);
     }
     return false;
 }

@@ -100,6 +100,18 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_apply(const int* __restrict__ i
 }
 
 // ------------------------------------------------------------- bins ---
+// Origin of bin column (x, y) for the FP32 prefilter records (any per-column
+// origin does; this one keeps local coordinates within about a bin).
+__device__ __forceinline__ double col_ox(const PhysParams& P, int x) { return (double)(x - P.BoxX / 2) * P.BoxLength; }
+__device__ __forceinline__ double col_oy(const PhysParams& P, int y) { return (double)(y - P.BoxY / 2) * P.BoxLength; }
+
+// FP32 prefilter record of an end point: x, y relative to its column origin
+// (ox, oy), z, and x^2 + y^2 of the rounded local x, y.
+__device__ __forceinline__ float4 local_point(d3 e, double ox, double oy) {
+    const float x = (float)(e.x - ox), y = (float)(e.y - oy);
+    return make_float4(x, y, (float)e.z, (float)((double)x * x + (double)y * y));
+}
+
 __global__ void k_bin_count(const double* __restrict__ hot, int n, PhysParams P, int* __restrict__ key,
                             int* __restrict__ cnt, float4* __restrict__ cmL, int* __restrict__ err) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -119,10 +131,9 @@ __global__ void k_bin_count(const double* __restrict__ hot, int n, PhysParams P,
         a = x * (P.BoxY * P.BoxZ) + y * P.BoxZ + z;
     }
     key[i] = a;
-    // FP32 end points for the conservative candidate prefilter of the list
-    // build, and the bin's z index
-    cmL[2 * i] = make_float4((float)s.p.x, (float)s.p.y, (float)s.p.z, (float)z);
-    cmL[2 * i + 1] = make_float4((float)s.q.x, (float)s.q.y, (float)s.q.z, 0.0f);
+    // FP32 end points for the candidate prefilter of the list build
+    cmL[2 * i] = local_point(s.p, col_ox(P, x), col_oy(P, y));
+    cmL[2 * i + 1] = local_point(s.q, col_ox(P, x), col_oy(P, y));
     atomicAdd(&cnt[a], 1);
 }
 
@@ -179,14 +190,15 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 // layout queue[e * blockDim + tid]); a full queue, and the end of the scan,
 // flush it through the exact test. Without the queue every lane would run
 // the masked FP64 test whenever any lane of its wave had a survivor.
-constexpr int GQ = 16;
+constexpr int GQ = 8;
 
 // The exact test (reduceNeighbours' predicate in FP64) of the queued
 // candidate cells, in queue order; kept ones are appended to the cell's list.
-// The next candidate's record is requested before this one is tested.
-__device__ __forceinline__ int flush_queue(const PhysParams& P, const double* __restrict__ hot, const Seg& A, d3 cmA,
-                                           const int* queue, int qstride, int nq, int* __restrict__ nl, int stride,
-                                           int slot, int cap, int m) {
+// The cell itself, queued like any other candidate, is skipped. The next
+// candidate's record is requested before this one is tested.
+__device__ __forceinline__ int flush_queue(const PhysParams& P, const double* __restrict__ hot, int self, const Seg& A,
+                                           d3 cmA, const int* queue, int qstride, int nq, int* __restrict__ nl,
+                                           int stride, int slot, int cap, int m) {
     if (nq == 0) return m;
     int j = queue[0];
     Seg B;
@@ -195,7 +207,7 @@ __device__ __forceinline__ int flush_queue(const PhysParams& P, const double* __
         const int jn = queue[min(e + 1, nq - 1) * qstride];
         Seg Bn;
         load_seg(hot, jn, Bn);
-        if (keep_neighbour(A, cmA, B, P.Rc)) {
+        if ((j != self) & keep_neighbour(A, cmA, B, P.Rc)) {
             if (m < cap) nl[(size_t)m * stride + slot] = j;
             m++;
         }
@@ -206,27 +218,47 @@ __device__ __forceinline__ int flush_queue(const PhysParams& P, const double* __
 }
 
 // FP32 form of the first test of reduceNeighbours (ABM/Neighbours.cpp:69-73):
-// an end point of j within L_A/2 + Rc R_A of the centre of A. The margin
-// covers the rounding of coordinates of magnitude |c| to FP32 (half an ulp,
-// about 6e-8 |c| each) and of the FP32 squared distance, with room: the
-// exact test decides every pair this keeps.
+// an end point e of the candidate within r = L_A/2 + Rc R_A of the centre c
+// of A. End points are stored by local_point (x, y relative to the origin o
+// of their bin column, w = x^2 + y^2), so that against one column
+//   |e - c|^2 = w - 2 x c'_x - 2 y c'_y + (e_z - c_z)^2 + |c'|^2,  c' = c - o,
+// one subtraction and three FMAs per end point (PreCol::near).
+// Conservative: for an end point within reach, the FP32 rounding of the
+// coordinates and of this sum is a few units of 2^-24 E^2 with
+// E = |c'_x| + |c'_y| + |c_z| + 2r; the threshold adds 1e-6 E^2 to the
+// squared reach, r carries 1e-3 for its own rounding, and the exact FP64
+// test decides every pair this keeps.
 struct Pre {
-    float cx, cy, cz, rr;
+    double cx, cy;  // centre of A
+    float cz, r;    // centre z; reach plus margin
+};
+struct PreCol {
+    float m2x, m2y, cz, thr;
     __device__ bool near(const float4 p, const float4 q) const {
-        const float px = p.x - cx, py = p.y - cy, pz = p.z - cz;
-        const float qx = q.x - cx, qy = q.y - cy, qz = q.z - cz;
-        return fminf(px * px + py * py + pz * pz, qx * qx + qy * qy + qz * qz) <= rr;
+        const float dpz = p.z - cz, dqz = q.z - cz;
+        const float sp = fmaf(dpz, dpz, fmaf(p.y, m2y, fmaf(p.x, m2x, p.w)));
+        const float sq = fmaf(dqz, dqz, fmaf(q.y, m2y, fmaf(q.x, m2x, q.w)));
+        return fminf(sp, sq) <= thr;
     }
 };
 __device__ __forceinline__ Pre prefilter_of(const PhysParams& P, const Seg& A, d3 cmA) {
     Pre f;
-    f.cx = (float)cmA.x;
-    f.cy = (float)cmA.y;
+    f.cx = cmA.x;
+    f.cy = cmA.y;
     f.cz = (float)cmA.z;
-    const float mag = fmaxf(fmaxf(fabsf(f.cx), fabsf(f.cy)), fabsf(f.cz));
-    const float r = (float)(A.L / 2.0 + P.Rc * A.R) + 1e-3f + 1e-6f * mag;
-    f.rr = r * r * 1.0001f;
+    f.r = (float)(A.L / 2.0 + P.Rc * A.R) + 1e-3f;
     return f;
+}
+// The test constants of A against bin column (x, y).
+__device__ __forceinline__ PreCol pre_col(const PhysParams& P, const Pre& f, int x, int y) {
+    const float cx = (float)(f.cx - col_ox(P, x)), cy = (float)(f.cy - col_oy(P, y));
+    const float e = fabsf(cx) + fabsf(cy) + fabsf(f.cz) + 2.0f * f.r;
+    PreCol c;
+    c.m2x = -2.0f * cx;
+    c.m2y = -2.0f * cy;
+    c.cz = f.cz;
+    c.thr = f.r * f.r * 1.0001f - (cx * cx + cy * cy) + 1e-6f * (e * e);
+    return c;
 }
 
 __device__ __forceinline__ int gather_one(const PhysParams& P, const double* __restrict__ hot, int id, int slot,
@@ -243,8 +275,7 @@ __device__ __forceinline__ int gather_one(const PhysParams& P, const double* __r
     }
     const Pre pre = prefilter_of(P, A, cmA);
     int m = 0, nq = 0;
-    auto flush = [&]() { m = flush_queue(P, hot, A, cmA, queue, qstride, nq, nl, stride, slot, cap, m); nq = 0; };
-    auto near = [&](int k) { return pre.near(cand[2 * k], cand[2 * k + 1]); };
+    auto flush = [&]() { m = flush_queue(P, hot, id, A, cmA, queue, qstride, nq, nl, stride, slot, cap, m); nq = 0; };
     auto push = [&](int k) { queue[nq++ * qstride] = sorted[k]; };
     const int zlo = max(z0 - 1, 0), zhi = min(z0 + 2, P.BoxZ);
     const int ylo = max(y0 - 1, 0), yhi = min(y0 + 2, P.BoxY);
@@ -253,6 +284,8 @@ __device__ __forceinline__ int gather_one(const PhysParams& P, const double* __r
     // z are adjacent in the sorted order, so each (x, y) column is one range.
     for (int xa = xlo; xa < xhi; xa++)
         for (int ya = ylo; ya < yhi; ya++) {
+            const PreCol col = pre_col(P, pre, xa, ya);
+            auto near = [&](int k) { return col.near(cand[2 * k], cand[2 * k + 1]); };
             const int a = xa * (P.BoxY * P.BoxZ) + ya * P.BoxZ;
             const int k1 = start[a + zhi];
             int k = start[a + zlo];
@@ -260,14 +293,14 @@ __device__ __forceinline__ int gather_one(const PhysParams& P, const double* __r
                 // flush together: a queue full in one lane empties every queue of the wave
                 if (__any(nq > GQ - 4)) flush();
                 const bool n0 = near(k), n1 = near(k + 1), n2 = near(k + 2), n3 = near(k + 3);
-                if (n0 && sorted[k] != id) push(k);
-                if (n1 && sorted[k + 1] != id) push(k + 1);
-                if (n2 && sorted[k + 2] != id) push(k + 2);
-                if (n3 && sorted[k + 3] != id) push(k + 3);
+                if (n0) push(k);
+                if (n1) push(k + 1);
+                if (n2) push(k + 2);
+                if (n3) push(k + 3);
             }
             for (; k < k1; k++) {
                 if (__any(nq == GQ)) flush();
-                if (near(k) && sorted[k] != id) push(k);
+                if (near(k)) push(k);
             }
         }
     flush();
@@ -298,31 +331,86 @@ __device__ __forceinline__ int wave_max(int v) {
 // The list build of the cells of one wave, when they are consecutive in bin
 // order and share one bin x: the wave walks the union of their 27-bin blocks
 // once, (x, y) column by column, staging 64 candidates at a time in LDS (one
-// coalesced load; every lane then reads the same entry), and each lane keeps
-// the candidates that lie in its own 27 bins, are not itself and pass the
-// FP32 prefilter. A lane meets its
-// candidates in ascending bin order, as in gather_one, so the lists are the
-// same. Lanes with live = false take part in the walk and keep nothing.
+// coalesced load; every lane then reads the same entry). Each lane queues the
+// candidates that lie in its own bins of the column (the position range
+// [klo, khi): its bins z0-1..z0+1, empty when the column is not next to its
+// own) and pass the FP32 prefilter. A lane meets its candidates in ascending
+// bin order, as in gather_one, so the lists are the same. Lanes with
+// live = false take part in the walk and keep nothing.
+//
+// The exact test runs on the whole wave's queues at once: its sum of queue
+// lengths (cell, candidate) pairs are dealt out evenly over the 64 lanes
+// (segments of the lanes' cells in LDS: sa[f * 64 + lane]), each verdict is
+// written back into the queue slot (the id, or -1), and each lane then appends
+// its kept ids in queue order. A lane with a long queue thus no longer holds
+// the other 63 back.
 __device__ __forceinline__ int gather_wave(const PhysParams& P, const double* __restrict__ hot, bool live, int id,
                                            int slot, const Seg& A, d3 cmA, int y0, int z0, int xw, int ylo, int yhi,
                                            int zlo, int zhi, const int* __restrict__ start,
                                            const int* __restrict__ sorted, const float4* __restrict__ cand,
                                            int* __restrict__ nl, int stride, int cap, int* __restrict__ queue,
-                                           int qstride, float4* sp, float4* sq, int* ss) {
+                                           int qstride, float4* sp, float4* sq, int* ss, double* sa,
+                                           unsigned char* own, int* offs, int* sid) {
     const Pre pre = prefilter_of(P, A, cmA);
     int m = 0, nq = 0;
-    auto flush = [&]() { m = flush_queue(P, hot, A, cmA, queue, qstride, nq, nl, stride, slot, cap, m); nq = 0; };
     const int BYZ = P.BoxY * P.BoxZ;
     const int lane = __lane_id();
-    const float z0f = (float)z0;
+    sa[0 * 64 + lane] = A.p.x;
+    sa[1 * 64 + lane] = A.p.y;
+    sa[2 * 64 + lane] = A.p.z;
+    sa[3 * 64 + lane] = A.q.x;
+    sa[4 * 64 + lane] = A.q.y;
+    sa[5 * 64 + lane] = A.q.z;
+    sa[6 * 64 + lane] = A.L;
+    sa[7 * 64 + lane] = A.R;
+    sid[lane] = id;
+    int* const qw = queue - lane;  // entry e of lane l: qw[e * qstride + l]
+    auto flush = [&]() {
+        const int incl = wave_incl_scan(nq);
+        const int tot = __builtin_amdgcn_readlane(incl, 63);
+        if (tot == 0) return;
+        const int off = incl - nq;
+        offs[lane] = off;
+        for (int e = 0; e < nq; e++) own[off + e] = (unsigned char)lane;
+        wave_lds_sync();
+        for (int p = lane; p < tot; p += 64) {
+            const int o = own[p];
+            int* const q = qw + (p - offs[o]) * qstride + o;
+            const int j = *q;
+            Seg B;
+            load_seg(hot, j, B);
+            Seg Ao;
+            Ao.p = mk(sa[0 * 64 + o], sa[1 * 64 + o], sa[2 * 64 + o]);
+            Ao.q = mk(sa[3 * 64 + o], sa[4 * 64 + o], sa[5 * 64 + o]);
+            Ao.L = sa[6 * 64 + o];
+            Ao.R = sa[7 * 64 + o];
+            const bool keep = (j != sid[o]) & keep_neighbour(Ao, cm_of(Ao), B, P.Rc);
+            *q = keep ? j : -1;
+        }
+        wave_lds_sync();
+        for (int e = 0; e < nq; e++) {
+            const int j = queue[e * qstride];
+            if (j >= 0) {
+                if (m < cap) nl[(size_t)m * stride + slot] = j;
+                m++;
+            }
+        }
+        nq = 0;
+        wave_lds_sync();
+    };
+    const int zl = max(z0 - 1, 0), zh = min(z0 + 1, P.BoxZ - 1) + 1;
     for (int xa = max(xw - 1, 0); xa <= min(xw + 1, P.BoxX - 1); xa++)
         for (int ya = ylo; ya <= yhi; ya++) {
             const bool ady = live && abs(ya - y0) <= 1;
             if (!__any(ady)) continue;
             // the column's bins zlo..zhi are one range of the bin order
             const int a = __builtin_amdgcn_readfirstlane(xa * BYZ + ya * P.BoxZ);
+            const int k0 = __builtin_amdgcn_readfirstlane(start[a + zlo]);
             const int k1 = __builtin_amdgcn_readfirstlane(start[a + zhi + 1]);
-            for (int kb = __builtin_amdgcn_readfirstlane(start[a + zlo]); kb < k1; kb += 64) {
+            const int klo = ady ? start[a + zl] : 0;
+            const int khi = ady ? start[a + zh] : 0;
+            const PreCol col = pre_col(P, pre, xa, ya);
+            for (int kb = k0; kb < k1; kb += 64) {
                 // stage 64 candidates in LDS with one coalesced load, then walk them
                 const int cnt = min(64, k1 - kb);
                 if (lane < cnt) {
@@ -331,6 +419,7 @@ __device__ __forceinline__ int gather_wave(const PhysParams& P, const double* __
                     ss[lane] = sorted[kb + lane];
                 }
                 wave_lds_sync();
+                const int lo = klo - kb, hi = khi - kb;  // the lane's range, chunk-relative
                 int c = 0;
                 for (; c + 4 <= cnt; c += 4) {
                     if (__any(nq > GQ - 4)) flush();
@@ -338,18 +427,16 @@ __device__ __forceinline__ int gather_wave(const PhysParams& P, const double* __
                     for (int u = 0; u < 4; u++) {
                         // & rather than &&: every load and test runs, with no branch
                         const float4 pj = sp[c + u], qj = sq[c + u];
-                        const int sj = ss[c + u];
-                        const bool b = ady & (fabsf(pj.w - z0f) <= 1.0f) & (sj != id) & pre.near(pj, qj);
-                        queue[nq * qstride] = sj;
+                        const bool b = (c + u >= lo) & (c + u < hi) & col.near(pj, qj);
+                        queue[nq * qstride] = ss[c + u];
                         nq += b;
                     }
                 }
                 for (; c < cnt; c++) {
                     if (__any(nq == GQ)) flush();
                     const float4 pj = sp[c], qj = sq[c];
-                    const int sj = ss[c];
-                    const bool b = ady & (fabsf(pj.w - z0f) <= 1.0f) & (sj != id) & pre.near(pj, qj);
-                    queue[nq * qstride] = sj;
+                    const bool b = (c >= lo) & (c < hi) & col.near(pj, qj);
+                    queue[nq * qstride] = ss[c];
                     nq += b;
                 }
                 wave_lds_sync();
@@ -368,6 +455,9 @@ __global__ void __launch_bounds__(256) k_gather_all(const double* __restrict__ h
     __shared__ int queue[GQ * 256];
     __shared__ float4 stage_p[256], stage_q[256];  // per wave: 64 candidates staged for the walk
     __shared__ int stage_s[256];
+    __shared__ double stage_a[8 * 256];       // per wave: the lanes' segments
+    __shared__ unsigned char owner[GQ * 256];  // per wave: owner lane of each queued pair
+    __shared__ int qoff[256], qself[256];
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     // every lane of a wave stays to the end (the wave walk uses cross-lane
     // operations); lanes past the last cell are not live
@@ -391,7 +481,8 @@ __global__ void __launch_bounds__(256) k_gather_all(const double* __restrict__ h
         const int zhi = min(wave_max(walk ? z0 : INT_MIN) + 1, P.BoxZ - 1);
         const int w = threadIdx.x / 64;
         m = gather_wave(P, hot, walk, id, t, A, cmA, y0, z0, xmin, ylo, yhi, zlo, zhi, start, sorted, cand, nl,
-                        stride, cap, queue + threadIdx.x, 256, stage_p + 64 * w, stage_q + 64 * w, stage_s + 64 * w);
+                        stride, cap, queue + threadIdx.x, 256, stage_p + 64 * w, stage_q + 64 * w, stage_s + 64 * w,
+                        stage_a + 8 * 64 * w, owner + GQ * 64 * w, qoff + 64 * w, qself + 64 * w);
     } else if (walk) {
         m = gather_one(P, hot, id, t, start, sorted, cand, nl, stride, cap, err, queue + threadIdx.x, 256);
     }
