@@ -190,7 +190,7 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 // layout queue[e * blockDim + tid]); a full queue, and the end of the scan,
 // flush it through the exact test. Without the queue every lane would run
 // the masked FP64 test whenever any lane of its wave had a survivor.
-constexpr int GQ = 8;
+constexpr int GQ = 16;
 
 // The exact test (reduceNeighbours' predicate in FP64) of the queued
 // candidate cells, in queue order; kept ones are appended to the cell's list.
@@ -339,8 +339,8 @@ __device__ __forceinline__ int wave_max(int v) {
 // live = false take part in the walk and keep nothing.
 //
 // The exact test runs on the whole wave's queues at once: its sum of queue
-// lengths (cell, candidate) pairs are dealt out evenly over the 64 lanes
-// (segments of the lanes' cells in LDS: sa[f * 64 + lane]), each verdict is
+// lengths (cell, candidate) pairs are dealt out evenly over the 64 lanes (a
+// lane reads the owner's record again, from cache), each verdict is
 // written back into the queue slot (the id, or -1), and each lane then appends
 // its kept ids in queue order. A lane with a long queue thus no longer holds
 // the other 63 back.
@@ -349,27 +349,24 @@ __device__ __forceinline__ int gather_wave(const PhysParams& P, const double* __
                                            int zlo, int zhi, const int* __restrict__ start,
                                            const int* __restrict__ sorted, const float4* __restrict__ cand,
                                            int* __restrict__ nl, int stride, int cap, int* __restrict__ queue,
-                                           int qstride, float4* sp, float4* sq, int* ss, double* sa,
-                                           unsigned char* own, int* offs, int* sid) {
+                                           int qstride, float4* sp, float4* sq, int* ss, unsigned char* own,
+                                           int* offs, int* sid) {
     const Pre pre = prefilter_of(P, A, cmA);
     int m = 0, nq = 0;
     const int BYZ = P.BoxY * P.BoxZ;
     const int lane = __lane_id();
-    sa[0 * 64 + lane] = A.p.x;
-    sa[1 * 64 + lane] = A.p.y;
-    sa[2 * 64 + lane] = A.p.z;
-    sa[3 * 64 + lane] = A.q.x;
-    sa[4 * 64 + lane] = A.q.y;
-    sa[5 * 64 + lane] = A.q.z;
-    sa[6 * 64 + lane] = A.L;
-    sa[7 * 64 + lane] = A.R;
     sid[lane] = id;
     int* const qw = queue - lane;  // entry e of lane l: qw[e * qstride + l]
     auto flush = [&]() {
-        const int incl = wave_incl_scan(nq);
-        const int tot = __builtin_amdgcn_readlane(incl, 63);
+        // exclusive scan of the queue lengths (0..GQ) by bit planes
+        int off = 0, tot = 0;
+#pragma unroll
+        for (int b = 0; (1 << b) <= GQ; b++) {
+            const unsigned long long v = __ballot((nq >> b) & 1);
+            off += (int)__builtin_amdgcn_mbcnt_hi((unsigned)(v >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)v, 0u)) << b;
+            tot += __popcll(v) << b;
+        }
         if (tot == 0) return;
-        const int off = incl - nq;
         offs[lane] = off;
         for (int e = 0; e < nq; e++) own[off + e] = (unsigned char)lane;
         wave_lds_sync();
@@ -380,10 +377,7 @@ __device__ __forceinline__ int gather_wave(const PhysParams& P, const double* __
             Seg B;
             load_seg(hot, j, B);
             Seg Ao;
-            Ao.p = mk(sa[0 * 64 + o], sa[1 * 64 + o], sa[2 * 64 + o]);
-            Ao.q = mk(sa[3 * 64 + o], sa[4 * 64 + o], sa[5 * 64 + o]);
-            Ao.L = sa[6 * 64 + o];
-            Ao.R = sa[7 * 64 + o];
+            load_seg(hot, sid[o], Ao);
             const bool keep = (j != sid[o]) & keep_neighbour(Ao, cm_of(Ao), B, P.Rc);
             *q = keep ? j : -1;
         }
@@ -455,7 +449,6 @@ __global__ void __launch_bounds__(256) k_gather_all(const double* __restrict__ h
     __shared__ int queue[GQ * 256];
     __shared__ float4 stage_p[256], stage_q[256];  // per wave: 64 candidates staged for the walk
     __shared__ int stage_s[256];
-    __shared__ double stage_a[8 * 256];       // per wave: the lanes' segments
     __shared__ unsigned char owner[GQ * 256];  // per wave: owner lane of each queued pair
     __shared__ int qoff[256], qself[256];
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -482,7 +475,7 @@ __global__ void __launch_bounds__(256) k_gather_all(const double* __restrict__ h
         const int w = threadIdx.x / 64;
         m = gather_wave(P, hot, walk, id, t, A, cmA, y0, z0, xmin, ylo, yhi, zlo, zhi, start, sorted, cand, nl,
                         stride, cap, queue + threadIdx.x, 256, stage_p + 64 * w, stage_q + 64 * w, stage_s + 64 * w,
-                        stage_a + 8 * 64 * w, owner + GQ * 64 * w, qoff + 64 * w, qself + 64 * w);
+                        owner + GQ * 64 * w, qoff + 64 * w, qself + 64 * w);
     } else if (walk) {
         m = gather_one(P, hot, id, t, start, sorted, cand, nl, stride, cap, err, queue + threadIdx.x, 256);
     }
