@@ -3,8 +3,9 @@
 // Each function restates one reference function (file:line relative to
 // AgentBasedModel-BacColonyGrowth/). Expressions keep the reference's
 // evaluation order and the library is built with -ffp-contract=off, so +,-,*,
-// /, sqrt round exactly as in the reference; pow() comes from the device math
-// library and may differ from glibc's by an ulp.
+// /, sqrt round exactly as in the reference. The reference's pow(delta, 3/2)
+// is evaluated as delta * sqrt(delta) (pow15), within an ulp or so of glibc's
+// pow.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -123,6 +124,11 @@ __device__ __forceinline__ bool keep_neighbour(const Seg& A, d3 cmA, const Seg& 
     return false;
 }
 
+// delta^(3/2) for delta >= 0 (the reference's pow(delta, 3.0 / 2.0)): a
+// correctly rounded root and one rounded product, within about one ulp of
+// the exact power, as glibc's pow is.
+__device__ __forceinline__ double pow15(double delta) { return delta * sqrt(delta); }
+
 struct PhysParams {
     double k_cc, k_wc, gamma_t, cell_mu, wall_mu, viscosity, tension, DH, cellRadius, BoxLength, Rc;
     double L_divide, varL, varAngle;
@@ -155,7 +161,7 @@ __device__ __forceinline__ void F_cc(const PhysParams& P, const Seg& A, d3 cmA, 
     double d = min_distance(A, B, c1, c2);
     double delta = dmax(sigma - d, 0.0);
     d3 vn = scl(sub(c2, c1), 1.0 / d);
-    double FE_mag = sqrt(A.R * B.R / (A.R + B.R)) * P.k_cc * pow(delta, 3.0 / 2.0);
+    double FE_mag = sqrt(A.R * B.R / (A.R + B.R)) * P.k_cc * pow15(delta);
     F = scl(vn, -FE_mag);
     d3 r = scl(add(scl(c1, B.R), scl(c2, A.R)), 1 / sigma);
     if (FE_mag > DBL_EPSILON) {
@@ -201,7 +207,7 @@ __device__ __forceinline__ d3 F_cw_locus(const PhysParams& P, const Seg& S, cons
     rr = mk(vert.x, P.planar ? 0.0 : vert.y, wall_z);  // var_pos == 0: no roughness draw term
     double d = vert.z - rr.z;
     double delta = dmax(sigma - d, 0.0);
-    double FE_mag = sqrt(S.R) * P.k_wc * pow(delta, 3.0 / 2.0);
+    double FE_mag = sqrt(S.R) * P.k_wc * pow15(delta);
     if (FE_mag > DBL_EPSILON) {
         d3 FE = mk(0.0, 0.0, FE_mag);
         d3 rcm = sub(rr, cm);
