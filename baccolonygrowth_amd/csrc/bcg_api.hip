@@ -97,18 +97,16 @@ int scan(bcg_ctx* c, const int* in, int n, int* out, int* grand) {
 // Counting sort of cells 0..n-1 of hot[buf] into bins.
 int build_bins(bcg_ctx* c, int buf, int n) {
     HIPCHK(c, hipMemsetAsync(c->bin_cnt, 0, sizeof(int) * (c->nbin + 1), c->stream));
-    HIPCHK(c, hipMemsetAsync(c->bin_fill, 0, sizeof(int) * c->nbin, c->stream));
     if (n > 0)
         hipLaunchKernelGGL(k_bin_count, dim3(nblk(n, 256)), dim3(256), 0, c->stream, c->hot[buf], n, c->P, c->key,
-                           c->bin_cnt, c->cmL, &c->stats->err);
+                           c->bin_cnt, c->bin_slot, c->cmL, &c->stats->err);
     int rc = scan(c, c->bin_cnt, c->nbin + 1, c->bin_start, nullptr);
     if (rc) return rc;
     if (n > 0) {
-        hipLaunchKernelGGL(k_bin_scatter, dim3(nblk(n, 256)), dim3(256), 0, c->stream, c->key, n, c->bin_start,
-                           c->bin_fill, c->scattered);
+        hipLaunchKernelGGL(k_bin_place, dim3(nblk(n, 256)), dim3(256), 0, c->stream, c->key, c->bin_slot, n,
+                           c->bin_start, c->scattered);
         hipLaunchKernelGGL(k_bin_rank, dim3(nblk(n, 256)), dim3(256), 0, c->stream, c->scattered, n, c->key,
-                           c->bin_start, c->sorted);
-        hipLaunchKernelGGL(k_bin_cand, dim3(nblk(n, 256)), dim3(256), 0, c->stream, c->sorted, n, c->cmL, c->cand);
+                           c->bin_start, c->cmL, c->sorted, c->cand);
     }
     HIPCHK(c, hipGetLastError());
     return 0;
@@ -234,7 +232,7 @@ int bcg_create(const bcg_params* p, bcg_ctx** out) {
     A(&c->key, M);
     A(&c->bin_cnt, c->nbin + 1);
     A(&c->bin_start, c->nbin + 1);
-    A(&c->bin_fill, c->nbin);
+    A(&c->bin_slot, M);
     A(&c->sorted, M);
     A(&c->scattered, M);
     A(&c->cmL, 2 * (size_t)M);   // FP32 end points p, q per cell
@@ -293,7 +291,7 @@ int bcg_destroy(bcg_ctx* c) {
     if (!c) return BCG_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->hot[0],   c->hot[1],    c->time_p,  c->time_q,     c->dynfric,   c->stafric,  c->age_p,
-                    c->age_q,    c->type,      c->ancestor, c->key,       c->bin_cnt,   c->bin_start, c->bin_fill,
+                    c->age_q,    c->type,      c->ancestor, c->key,       c->bin_cnt,   c->bin_start, c->bin_slot,
                     c->sorted,   c->scattered, c->scan_tot, c->cmL, c->cand,  c->nl,      c->nl_count,   c->order,     c->slot_of,  c->divflag,
                     c->div_ids,  c->div_index_of, c->ids_tmp, c->draws,   c->mother_pre, c->height,  c->normal,
                     c->wall,     c->gr1,       c->gr2,     c->Fbuf,       c->Tbuf,      c->stats};

@@ -32,7 +32,7 @@ struct bcg_ctx {
     double* hot[2] = {nullptr, nullptr};
     double *time_p = nullptr, *time_q = nullptr, *dynfric = nullptr, *stafric = nullptr;
     int *age_p = nullptr, *age_q = nullptr, *type = nullptr, *ancestor = nullptr;
-    int *key = nullptr, *bin_cnt = nullptr, *bin_start = nullptr, *bin_fill = nullptr, *sorted = nullptr;
+    int *key = nullptr, *bin_cnt = nullptr, *bin_start = nullptr, *bin_slot = nullptr, *sorted = nullptr;
     int* scattered = nullptr;  // bin order before the in-bin ranking
     int* scan_tot = nullptr;
     float4 *cmL = nullptr, *cand = nullptr;

@@ -17,10 +17,11 @@ __global__ void k_scan_reduce(const int* in, int n, int* tot);
 __global__ void k_scan_totals(int* tot, int nb, int* grand);
 __global__ void k_scan_apply(const int* in, int n, const int* tot, int* out);
 
-__global__ void k_bin_count(const double* hot, int n, PhysParams P, int* key, int* cnt, float4* cmL, int* err);
-__global__ void k_bin_cand(const int* sorted, int n, const float4* cmL, float4* cand);
-__global__ void k_bin_scatter(const int* key, int n, const int* start, int* fill, int* sorted);
-__global__ void k_bin_rank(const int* scattered, int n, const int* key, const int* start, int* sorted);
+__global__ void k_bin_count(const double* hot, int n, PhysParams P, int* key, int* cnt, int* slot, float4* cmL,
+                            int* err);
+__global__ void k_bin_place(const int* key, const int* slot, int n, const int* start, int* scattered);
+__global__ void k_bin_rank(const int* scattered, int n, const int* key, const int* start, const float4* cmL,
+                           int* sorted, float4* cand);
 
 __global__ void k_gather_all(const double* hot, int n, PhysParams P, const int* start, const int* sorted,
                              const float4* cand, int* nl, int* nl_count, int stride, int cap, int* order, int* slot_of,

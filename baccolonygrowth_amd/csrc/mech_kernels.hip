@@ -113,7 +113,8 @@ __device__ __forceinline__ float4 local_point(d3 e, double ox, double oy) {
 }
 
 __global__ void k_bin_count(const double* __restrict__ hot, int n, PhysParams P, int* __restrict__ key,
-                            int* __restrict__ cnt, float4* __restrict__ cmL, int* __restrict__ err) {
+                            int* __restrict__ cnt, int* __restrict__ slot, float4* __restrict__ cmL,
+                            int* __restrict__ err) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Seg s;
@@ -134,23 +135,23 @@ __global__ void k_bin_count(const double* __restrict__ hot, int n, PhysParams P,
     // FP32 end points for the candidate prefilter of the list build
     cmL[2 * i] = local_point(s.p, col_ox(P, x), col_oy(P, y));
     cmL[2 * i + 1] = local_point(s.q, col_ox(P, x), col_oy(P, y));
-    atomicAdd(&cnt[a], 1);
+    slot[i] = atomicAdd(&cnt[a], 1);  // arrival order in the bin
 }
 
-__global__ void k_bin_scatter(const int* __restrict__ key, int n, const int* __restrict__ start, int* __restrict__ fill,
-                              int* __restrict__ sorted) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int a = key[i];
-    int pos = start[a] + atomicAdd(&fill[a], 1);
-    sorted[pos] = i;
+// Bin order with the cells of a bin in arrival order.
+__global__ void k_bin_place(const int* __restrict__ key, const int* __restrict__ slot, int n,
+                            const int* __restrict__ start, int* __restrict__ scattered) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) scattered[start[key[i]] + slot[i]] = i;
 }
 
-// Ascending IDs inside each bin: the cell at position p of the scattered
-// order goes to its bin's start plus the number of cells of the bin with a
-// smaller ID (bins hold tens of cells; a thread per cell).
+// Ascending IDs inside each bin: the cell at position p of the arrival order
+// goes to its bin's start plus the number of cells of the bin with a smaller
+// ID (bins hold tens of cells; a thread per cell), with its candidate record
+// (the FP32 end points of the prefilter).
 __global__ void k_bin_rank(const int* __restrict__ scattered, int n, const int* __restrict__ key,
-                           const int* __restrict__ start, int* __restrict__ sorted) {
+                           const int* __restrict__ start, const float4* __restrict__ cmL, int* __restrict__ sorted,
+                           float4* __restrict__ cand) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int id = scattered[p];
@@ -159,17 +160,8 @@ __global__ void k_bin_rank(const int* __restrict__ scattered, int n, const int* 
     int r = 0;
     for (int q = s; q < e; q++) r += scattered[q] < id;
     sorted[s + r] = id;
-}
-
-// Candidate records in bin order: the FP32 end points p, q of cell sorted[pos].
-__global__ void k_bin_cand(const int* __restrict__ sorted, int n, const float4* __restrict__ cmL,
-                           float4* __restrict__ cand) {
-    int pos = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pos < n) {
-        const int j = sorted[pos];
-        cand[2 * pos] = cmL[2 * j];
-        cand[2 * pos + 1] = cmL[2 * j + 1];
-    }
+    cand[2 * (s + r)] = cmL[2 * id];
+    cand[2 * (s + r) + 1] = cmL[2 * id + 1];
 }
 
 // ------------------------------------------------------------ lists ---
