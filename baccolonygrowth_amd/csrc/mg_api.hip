@@ -142,6 +142,10 @@ struct bcg_rd3d {
     int dcap = 0;
     hipEvent_t ev[4];
     float last_ms[3] = {0, 0, 0};
+    // workgroups resident on the device at once, for the z-range counts of
+    // the z-marching launches: k_rbgs2 in FP64 and FP32, the fine-level
+    // passes (residual, residual norm, refinement update), k_zm_prolong
+    long res_rb[2] = {1, 1}, res_zm = 1, res_zp[2] = {1, 1};
     std::string err;
 };
 
@@ -174,7 +178,10 @@ int dalloc(bcg_rd3d* c, T** p, size_t n) {
 }
 
 dim3 grid2(const Level& v, int bx = 64, int by = 4) { return dim3(nblk(v.nx, bx), nblk(v.ny, by), v.nz); }
-dim3 grid_rb(const Level& v) { return dim3(nblk((v.nx + 1) / 2, 64), nblk(v.ny, 4), v.nz); }
+// 1-D launches over 64 x 4 x 1 tiles, remapped in the kernel (slab_tile)
+dim3 slab_grid(int nx, int ny, int nz) { return dim3((unsigned)(nblk(nx, 64) * nblk(ny, 4) * nz)); }
+dim3 slab_grid(const Level& v) { return slab_grid(v.nx, v.ny, v.nz); }
+dim3 grid_rb(const Level& v) { return slab_grid((v.nx + 1) / 2, v.ny, v.nz); }
 
 template <class T>
 struct Arr;
@@ -201,8 +208,16 @@ constexpr long BOTTOM_MAX_POINTS = 4096;
 // Levels of at least this many points use the fused z-marching sweeps;
 // below it the two-launch sweep has more parallelism per launch.
 constexpr long FUSED_MIN_POINTS = 1L << 21;
-constexpr int FUSED_TARGET_BLOCKS = 1536;  // about 6 workgroups per CU
-constexpr int ZM_TARGET_BLOCKS = 2048;     // z-marching passes: 8 per CU, 4 resident
+
+// Workgroups of `kernel` (block size `threads`) resident on the device at once.
+template <class K>
+long resident(K kernel, int threads) {
+    int dev = 0, cus = 0, nb = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, threads, 0) != hipSuccess) return 1;
+    return std::max(1L, (long)nb * cus);
+}
 
 // `sweeps` red-black sweeps on u of level v. Fused sweeps go in pairs
 // u -> r -> u (the residual buffer is free while smoothing); an odd last
@@ -212,7 +227,7 @@ void smooth(bcg_rd3d* c, Level& v, int sweeps) {
     T* u = Arr<T>::u(v);
     int s = 0;
     if (c->prm.smoother == BCG_RD3D_SMOOTH_FUSED && v.n() >= FUSED_MIN_POINTS) {
-        const RbGrid g = rb_grid(v.nx, v.ny, v.nz, FUSED_TARGET_BLOCKS);
+        const RbGrid g = rb_grid(v.nx, v.ny, v.nz, c->res_rb[sizeof(T) == 4]);
         const dim3 grid(g.tiles_x * g.tiles_y * g.nchunk);
         T* tmp = Arr<T>::r(v);
         // plane loads two steps ahead (measured ~4% faster than one step ahead on
@@ -281,15 +296,15 @@ void vcycle(bcg_rd3d* c, int l) {
     // z-marching residual on the large FP64 levels; in FP32 the plain kernel
     // measured faster (147 vs 139 us on 513 x 513 x 129)
     if (sizeof(T) == 8 && v.n() >= FUSED_MIN_POINTS) {
-        const ZmGrid g = zm_grid(v.nx, v.ny, v.nz, ZM_TARGET_BLOCKS);
+        const ZmGrid g = zm_grid(v.nx, v.ny, v.nz, c->res_zm);
         hipLaunchKernelGGL((k_zm_residual<T>), dim3(g.blocks()), dim3(ZM_THREADS), 0, c->stream, Arr<T>::u(v),
                            Arr<T>::f(v), Arr<T>::a0(v), Arr<T>::r(v), v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly, (T)v.lz,
                            g.tiles_x, g.tiles_y, g.kchunk);
     } else {
-        hipLaunchKernelGGL((k_residual<T>), grid2(v), dim3(64, 4), 0, c->stream, Arr<T>::u(v), Arr<T>::f(v),
+        hipLaunchKernelGGL((k_residual<T>), slab_grid(v), dim3(64, 4), 0, c->stream, Arr<T>::u(v), Arr<T>::f(v),
                            Arr<T>::a0(v), Arr<T>::r(v), v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly, (T)v.lz);
     }
-    hipLaunchKernelGGL((k_restrict<T>), grid2(w), dim3(64, 4), 0, c->stream, Arr<T>::r(v), Arr<T>::f(w), v.nx, v.ny, v.nz,
+    hipLaunchKernelGGL((k_restrict<T>), slab_grid(w), dim3(64, 4), 0, c->stream, Arr<T>::r(v), Arr<T>::f(w), v.nx, v.ny, v.nz,
                        w.nx, w.ny, w.nz, v.cx, v.cy, v.cz);
     hipLaunchKernelGGL((k_fill<T>), dim3(nblk(w.n(), 256)), dim3(256), 0, c->stream, Arr<T>::u(w), w.n(), (T)0);
     vcycle<T>(c, l + 1);
@@ -298,14 +313,13 @@ void vcycle(bcg_rd3d* c, int l) {
         // (short ranges: many columns in flight hide the latency of each
         // column's load-add-store chain)
         const long cols = (long)nblk(v.nx, 64) * nblk(v.ny, 4);
-        int nc = (int)((4 * ZM_TARGET_BLOCKS + cols - 1) / cols);
-        nc = std::max(1, std::min(nc, std::max(1, v.nz / 8)));
+        const int nc = z_ranges(cols, v.nz, c->res_zp[sizeof(T) == 4], 1, 8);
         const int kchunk = (v.nz + nc - 1) / nc;
-        hipLaunchKernelGGL((k_zm_prolong<T>), dim3(nblk(v.nx, 64), nblk(v.ny, 4), (v.nz + kchunk - 1) / kchunk),
+        hipLaunchKernelGGL((k_zm_prolong<T>), slab_grid(v.nx, v.ny, (v.nz + kchunk - 1) / kchunk),
                            dim3(256), 0, c->stream, Arr<T>::u(v), Arr<T>::u(w), v.nx, v.ny, v.nz, w.nx, w.ny, v.cx,
                            v.cy, v.cz, kchunk);
     } else {
-        hipLaunchKernelGGL((k_prolong<T>), grid2(v), dim3(64, 4), 0, c->stream, Arr<T>::u(v), Arr<T>::u(w), v.nx, v.ny,
+        hipLaunchKernelGGL((k_prolong<T>), slab_grid(v), dim3(64, 4), 0, c->stream, Arr<T>::u(v), Arr<T>::u(w), v.nx, v.ny,
                            v.nz, w.nx, w.ny, v.cx, v.cy, v.cz);
     }
     smooth<T>(c, v, c->prm.nu2);
@@ -342,7 +356,7 @@ int setup_system(bcg_rd3d* c) {
     for (size_t l = 0; l + 1 < c->L.size(); l++) {
         Level& v = c->L[l];
         Level& w = c->L[l + 1];
-        hipLaunchKernelGGL((k_restrict_a0<double>), grid2(w), dim3(64, 4), 0, c->stream, v.a064, w.a064, v.nx, v.ny, v.nz,
+        hipLaunchKernelGGL((k_restrict_a0<double>), slab_grid(w), dim3(64, 4), 0, c->stream, v.a064, w.a064, v.nx, v.ny, v.nz,
                            w.nx, w.ny, w.nz, v.cx, v.cy, v.cz);
         hipLaunchKernelGGL((k_cast<double, float>), dim3(nblk(w.n(), 256)), dim3(256), 0, c->stream, w.a064, w.a032, w.n());
     }
@@ -416,6 +430,12 @@ int bcg_rd3d_create(const bcg_rd3d_params* p, bcg_rd3d** out) {
         if (v.cz) nz = (nz - 1) / 2 + 1, hz *= 2;
     }
     c->N = c->L[0].n();
+    c->res_rb[0] = resident(k_rbgs2<double, 2>, RB_THREADS);
+    c->res_rb[1] = resident(k_rbgs2<float, 2>, RB_THREADS);
+    c->res_zm = std::min({resident(k_zm_residual<double>, ZM_THREADS), resident(k_zm_resnorm, ZM_THREADS),
+                          resident(k_zm_ir, ZM_THREADS)});
+    c->res_zp[0] = resident(k_zm_prolong<double>, 256);
+    c->res_zp[1] = resident(k_zm_prolong<float>, 256);
     // the small coarse levels (at most BOT_MAX_LEVELS, each <= BOTTOM_MAX_POINTS)
     // form the bottom solver
     c->bottom = (int)c->L.size() - 1;
@@ -450,7 +470,7 @@ int bcg_rd3d_create(const bcg_rd3d_params* p, bcg_rd3d** out) {
         A(&v.a032, n);
         A(&v.r32, n);
     }
-    c->npart = std::max(nparts_of(c->L[0]), zm_grid(c->L[0].nx, c->L[0].ny, c->L[0].nz, ZM_TARGET_BLOCKS).blocks());
+    c->npart = std::max(nparts_of(c->L[0]), zm_grid(c->L[0].nx, c->L[0].ny, c->L[0].nz, c->res_zm).blocks());
     A(&c->part, (size_t)c->npart);
     A(&c->part2, 256);
     if (c->bottom == 0) {
@@ -624,7 +644,7 @@ int solve_core(bcg_rd3d* c, int mode, double tol, int max_iter, int* iters, doub
     int rc = 0;
     int it = 0;
     double r0 = 0, rn = 0, bn = 0;
-    const ZmGrid zg = zm_grid(f.nx, f.ny, f.nz, ZM_TARGET_BLOCKS);
+    const ZmGrid zg = zm_grid(f.nx, f.ny, f.nz, c->res_zm);
     const long np = zg.blocks();  // one partial sum per z-marching workgroup
     // the tolerance is relative to ||b||
     hipLaunchKernelGGL((k_sumsq<double>), dim3(4096), dim3(256), 0, c->stream, c->b, c->N, c->part);

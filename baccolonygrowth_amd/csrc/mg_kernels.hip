@@ -27,6 +27,23 @@ namespace mg {
 
 __device__ __forceinline__ int mirror(int i, int n) { return i < 0 ? -i : (i >= n ? 2 * (n - 1) - i : i); }
 
+// Tile (bx, by, bz) of a 1-D launch over gx x gy x gz tiles of 64 x 4 points
+// (x, y) by one plane (z). The hardware deals workgroups round-robin over the
+// 8 XCDs, each with its own L2; the remap gives XCD x the consecutive tiles
+// [x * per, (x + 1) * per) in x-fastest, then y, then z order - a slab of
+// planes - so that the cache lines a tile shares with its x neighbours (rows
+// are not line-aligned) and the planes it shares with its z neighbours are
+// read through the same L2, shortly after each other.
+__device__ __forceinline__ void slab_tile(int gx, int gy, int& bx, int& by, int& bz) {
+    constexpr int X = 8;
+    const int nb = gridDim.x, per = nb / X, rem = nb % X;
+    const int xcd = blockIdx.x % X;
+    const int b = xcd * per + (xcd < rem ? xcd : rem) + blockIdx.x / X;
+    bx = b % gx;
+    by = (b / gx) % gy;
+    bz = b / (gx * gy);
+}
+
 template <class T>
 struct Lvl {
     T* u;
@@ -53,10 +70,12 @@ __device__ __forceinline__ T neigh_sum(const T* __restrict__ u, int i, int j, in
 template <class T>
 __global__ void __launch_bounds__(256) k_rbgs(T* __restrict__ u, const T* __restrict__ f, const T* __restrict__ a0,
                                               int nx, int ny, int nz, T lx, T ly, T lz, int color) {
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
+    int bx, by, bz;
+    slab_tile(((nx + 1) / 2 + 63) / 64, (ny + 3) / 4, bx, by, bz);
+    const int j = by * blockDim.y + threadIdx.y;
+    const int k = bz;
     if (j >= ny) return;
-    const int i = 2 * (blockIdx.x * blockDim.x + threadIdx.x) + ((j + k + color) & 1);
+    const int i = 2 * (bx * blockDim.x + threadIdx.x) + ((j + k + color) & 1);
     if (i >= nx) return;
     const long sxy = (long)nx * ny;
     if (k == 0) {
@@ -75,9 +94,11 @@ template <class T>
 __global__ void __launch_bounds__(256) k_residual(const T* __restrict__ u, const T* __restrict__ f,
                                                   const T* __restrict__ a0, T* __restrict__ r, int nx, int ny, int nz,
                                                   T lx, T ly, T lz) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
+    int bx, by, bz;
+    slab_tile((nx + 63) / 64, (ny + 3) / 4, bx, by, bz);
+    const int i = bx * blockDim.x + threadIdx.x;
+    const int j = by * blockDim.y + threadIdx.y;
+    const int k = bz;
     if (i >= nx || j >= ny) return;
     if (k == 0) {
         const long p = (long)j * nx + i;
@@ -120,9 +141,11 @@ __device__ __forceinline__ T restrict_point(const T* __restrict__ r, int nx, int
 template <class T>
 __global__ void __launch_bounds__(256) k_restrict(const T* __restrict__ r, T* __restrict__ fc, int nx, int ny, int nz,
                                                   int ncx, int ncy, int ncz, int cx, int cy, int cz) {
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
-    const int J = blockIdx.y * blockDim.y + threadIdx.y;
-    const int K = blockIdx.z;
+    int bx, by, bz;
+    slab_tile((ncx + 63) / 64, (ncy + 3) / 4, bx, by, bz);
+    const int I = bx * blockDim.x + threadIdx.x;
+    const int J = by * blockDim.y + threadIdx.y;
+    const int K = bz;
     if (I >= ncx || J >= ncy) return;
     fc[((long)K * ncy + J) * ncx + I] = restrict_point<T>(r, nx, ny, nz, I, J, K, cx, cy, cz);
 }
@@ -172,9 +195,11 @@ __device__ __forceinline__ T prolong_point(const T* __restrict__ ec, int ncx, in
 template <class T>
 __global__ void __launch_bounds__(256) k_prolong(T* __restrict__ u, const T* __restrict__ ec, int nx, int ny, int nz,
                                                  int ncx, int ncy, int cx, int cy, int cz) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
+    int bx, by, bz;
+    slab_tile((nx + 63) / 64, (ny + 3) / 4, bx, by, bz);
+    const int i = bx * blockDim.x + threadIdx.x;
+    const int j = by * blockDim.y + threadIdx.y;
+    const int k = bz;
     if (i >= nx || j >= ny) return;
     u[((long)k * ny + j) * nx + i] += prolong_point<T>(ec, ncx, ncy, i, j, k, cx, cy, cz);
 }
@@ -186,10 +211,12 @@ __global__ void __launch_bounds__(256) k_prolong(T* __restrict__ u, const T* __r
 template <class T>
 __global__ void __launch_bounds__(256) k_zm_prolong(T* __restrict__ u, const T* __restrict__ ec, int nx, int ny, int nz,
                                                     int ncx, int ncy, int cx, int cy, int cz, int kchunk) {
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    int bx, by, bz;
+    slab_tile((nx + 63) / 64, (ny + 3) / 4, bx, by, bz);
+    const int i = bx * 64 + (threadIdx.x & 63);
+    const int j = by * 4 + (threadIdx.x >> 6);
     if (i >= nx || j >= ny) return;
-    const int k0 = blockIdx.z * kchunk, k1 = min(k0 + kchunk, nz);
+    const int k0 = bz * kchunk, k1 = min(k0 + kchunk, nz);
     int I0, I1, J0, J1;
     T wi, wj;
     if (cx) {
@@ -310,9 +337,11 @@ __global__ void k_rhs(const double* __restrict__ c, double* __restrict__ b, doub
 template <class T>
 __global__ void __launch_bounds__(256) k_restrict_a0(const T* __restrict__ a0f, T* __restrict__ a0c, int nx, int ny,
                                                      int nz, int ncx, int ncy, int ncz, int cx, int cy, int cz) {
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
-    const int J = blockIdx.y * blockDim.y + threadIdx.y;
-    const int K = blockIdx.z;
+    int bx, by, bz;
+    slab_tile((ncx + 63) / 64, (ncy + 3) / 4, bx, by, bz);
+    const int I = bx * blockDim.x + threadIdx.x;
+    const int J = by * blockDim.y + threadIdx.y;
+    const int K = bz;
     if (I >= ncx || J >= ncy) return;
     const int i0 = cx ? 2 * I : I, j0 = cy ? 2 * J : J, k0 = cz ? 2 * K : K;
     const long sxy = (long)nx * ny;

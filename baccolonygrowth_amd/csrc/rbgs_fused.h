@@ -301,14 +301,36 @@ struct RbGrid {
     int tiles_x, tiles_y, nchunk, kchunk;
 };
 
-inline RbGrid rb_grid(int nx, int ny, int nz, int target_blocks) {
+// Number of z ranges of a z-marching launch over `tiles` column tiles and nz
+// planes: the count that minimises rounds x (planes per range + halo), where
+// a round is `resident` workgroups (all that fit on the device at once) and
+// `halo` the extra planes a workgroup steps through below its range; at
+// least `minplanes` planes per range. A launch of 3.2 rounds takes as long
+// as one of 4, so the count is chosen to fill the last round.
+inline int z_ranges(long tiles, int nz, long resident, int halo, int minplanes) {
+    const int maxc = nz / minplanes > 1 ? nz / minplanes : 1;
+    resident = resident > 0 ? resident : 1;
+    int best = 1;
+    long best_cost = -1;
+    for (int nc = 1; nc <= maxc; nc++) {
+        const int kchunk = (nz + nc - 1) / nc;
+        const int used = (nz + kchunk - 1) / kchunk;
+        if (used != nc) continue;  // the same ranges as a smaller count
+        const long rounds = (tiles * nc + resident - 1) / resident;
+        const long cost = rounds * (kchunk + halo);
+        if (best_cost < 0 || cost < best_cost) {
+            best = nc;
+            best_cost = cost;
+        }
+    }
+    return best;
+}
+
+inline RbGrid rb_grid(int nx, int ny, int nz, long resident) {
     RbGrid g;
     g.tiles_x = (nx + RB_TX - 1) / RB_TX;
     g.tiles_y = (ny + RB_TY - 1) / RB_TY;
-    const int tiles = g.tiles_x * g.tiles_y;
-    int nc = (target_blocks + tiles - 1) / tiles;
-    const int maxc = nz / 8 > 1 ? nz / 8 : 1;  // at least 8 planes per range
-    nc = nc < 1 ? 1 : (nc > maxc ? maxc : nc);
+    const int nc = z_ranges((long)g.tiles_x * g.tiles_y, nz, resident, 3, 8);
     g.kchunk = (nz + nc - 1) / nc;
     g.nchunk = (nz + g.kchunk - 1) / g.kchunk;
     return g;

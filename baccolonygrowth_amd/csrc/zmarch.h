@@ -19,6 +19,8 @@
 // the host, one phase for all threads at a time.
 #pragma once
 
+#include "rbgs_fused.h"  // z_ranges, RBGS_HD
+
 #ifndef RBGS_HD
 #define RBGS_HD __host__ __device__
 #endif
@@ -139,14 +141,11 @@ struct ZmGrid {
     long blocks() const { return (long)tiles_x * tiles_y * nchunk; }
 };
 
-inline ZmGrid zm_grid(int nx, int ny, int nz, int target_blocks) {
+inline ZmGrid zm_grid(int nx, int ny, int nz, long resident) {
     ZmGrid g;
     g.tiles_x = (nx + ZM_TX - 1) / ZM_TX;
     g.tiles_y = (ny + ZM_TY - 1) / ZM_TY;
-    const int tiles = g.tiles_x * g.tiles_y;
-    int nc = (target_blocks + tiles - 1) / tiles;
-    const int maxc = nz / 4 > 1 ? nz / 4 : 1;  // at least 4 planes per range
-    nc = nc < 1 ? 1 : (nc > maxc ? maxc : nc);
+    const int nc = z_ranges((long)g.tiles_x * g.tiles_y, nz, resident, 1, 4);
     g.kchunk = (nz + nc - 1) / nc;
     g.nchunk = (nz + g.kchunk - 1) / g.kchunk;
     return g;
