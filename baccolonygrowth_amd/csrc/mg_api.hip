@@ -143,9 +143,10 @@ struct bcg_rd3d {
     hipEvent_t ev[4];
     float last_ms[3] = {0, 0, 0};
     // workgroups resident on the device at once, for the z-range counts of
-    // the z-marching launches: k_rbgs2 in FP64 and FP32, the fine-level
-    // passes (residual, residual norm, refinement update), k_zm_prolong
-    long res_rb[2] = {1, 1}, res_zm = 1, res_zp[2] = {1, 1};
+    // the z-marching launches: k_rbgs2 (FP64, FP32 narrow, FP32 wide tile),
+    // the fine-level passes (residual, residual norm, refinement update),
+    // k_zm_prolong (FP64, FP32)
+    long res_rb[3] = {1, 1, 1}, res_zm = 1, res_zp[2] = {1, 1};
     std::string err;
 };
 
@@ -208,6 +209,7 @@ constexpr long BOTTOM_MAX_POINTS = 4096;
 // Levels of at least this many points use the fused z-marching sweeps;
 // below it the two-launch sweep has more parallelism per launch.
 constexpr long FUSED_MIN_POINTS = 1L << 21;
+constexpr long WIDE_MIN_POINTS = 1L << 24;  // FP32 levels that get the 62 x 30 sweep tile
 
 // Workgroups of `kernel` (block size `threads`) resident on the device at once.
 template <class K>
@@ -227,20 +229,34 @@ void smooth(bcg_rd3d* c, Level& v, int sweeps) {
     T* u = Arr<T>::u(v);
     int s = 0;
     if (c->prm.smoother == BCG_RD3D_SMOOTH_FUSED && v.n() >= FUSED_MIN_POINTS) {
-        const RbGrid g = rb_grid(v.nx, v.ny, v.nz, c->res_rb[sizeof(T) == 4]);
-        const dim3 grid(g.tiles_x * g.tiles_y * g.nchunk);
         T* tmp = Arr<T>::r(v);
         // plane loads two steps ahead (measured ~4% faster than one step ahead on
         // the 513 x 513 x 129 level in both precisions)
-        auto sweep = [&](const T* in, T* out) {
-            hipLaunchKernelGGL((k_rbgs2<T, 2>), grid, dim3(RB_THREADS), 0, c->stream, in, out, Arr<T>::f(v),
-                               Arr<T>::a0(v), v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly, (T)v.lz, g.tiles_x, g.tiles_y,
-                               g.kchunk);
+        auto sweeps2 = [&](auto tile, long resident) {
+            using D = decltype(tile);
+            const RbGrid g = rb_grid<D>(v.nx, v.ny, v.nz, resident);
+            const dim3 grid(g.tiles_x * g.tiles_y * g.nchunk);
+            auto sweep = [&](const T* in, T* out) {
+                hipLaunchKernelGGL((k_rbgs2<T, 2, D>), grid, dim3(D::THREADS), 0, c->stream, in, out, Arr<T>::f(v),
+                                   Arr<T>::a0(v), v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly, (T)v.lz, g.tiles_x, g.tiles_y,
+                                   g.kchunk);
+            };
+            for (; s + 2 <= sweeps; s += 2) {
+                sweep(u, tmp);
+                sweep(tmp, u);
+            }
         };
-        for (; s + 2 <= sweeps; s += 2) {
-            sweep(u, tmp);
-            sweep(tmp, u);
+        // FP32 has the LDS for the wide tile; it pays where there are enough
+        // tiles to fill the device (measured on 513 x 513 x 129: 143 vs 166 us a
+        // sweep; on 257 x 257 x 65 the narrow tile is faster)
+        bool wide = false;
+        if constexpr (sizeof(T) == 4) {
+            if (v.n() >= WIDE_MIN_POINTS) {
+                sweeps2(RbWide{}, c->res_rb[2]);
+                wide = true;
+            }
         }
+        if (!wide) sweeps2(RbNarrow{}, c->res_rb[sizeof(T) == 4]);
     }
     for (; s < sweeps; s++)
         for (int color = 0; color < 2; color++)
@@ -293,9 +309,8 @@ void vcycle(bcg_rd3d* c, int l) {
     }
     Level& w = c->L[l + 1];
     smooth<T>(c, v, c->prm.nu1);
-    // z-marching residual on the large FP64 levels; in FP32 the plain kernel
-    // measured faster (147 vs 139 us on 513 x 513 x 129)
-    if (sizeof(T) == 8 && v.n() >= FUSED_MIN_POINTS) {
+    // z-marching residual on the large levels
+    if (v.n() >= FUSED_MIN_POINTS) {
         const ZmGrid g = zm_grid(v.nx, v.ny, v.nz, c->res_zm);
         hipLaunchKernelGGL((k_zm_residual<T>), dim3(g.blocks()), dim3(ZM_THREADS), 0, c->stream, Arr<T>::u(v),
                            Arr<T>::f(v), Arr<T>::a0(v), Arr<T>::r(v), v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly, (T)v.lz,
@@ -430,8 +445,9 @@ int bcg_rd3d_create(const bcg_rd3d_params* p, bcg_rd3d** out) {
         if (v.cz) nz = (nz - 1) / 2 + 1, hz *= 2;
     }
     c->N = c->L[0].n();
-    c->res_rb[0] = resident(k_rbgs2<double, 2>, RB_THREADS);
-    c->res_rb[1] = resident(k_rbgs2<float, 2>, RB_THREADS);
+    c->res_rb[0] = resident(k_rbgs2<double, 2, RbNarrow>, RbNarrow::THREADS);
+    c->res_rb[1] = resident(k_rbgs2<float, 2, RbNarrow>, RbNarrow::THREADS);
+    c->res_rb[2] = resident(k_rbgs2<float, 2, RbWide>, RbWide::THREADS);
     c->res_zm = std::min({resident(k_zm_residual<double>, ZM_THREADS), resident(k_zm_resnorm, ZM_THREADS),
                           resident(k_zm_ir, ZM_THREADS)});
     c->res_zp[0] = resident(k_zm_prolong<double>, 256);

@@ -24,7 +24,7 @@ __global__ void k_sum_chunks(const double* part, long n, double* out);
 
 template <class T>
 __global__ void k_rbgs(T* u, const T* f, const T* a0, int nx, int ny, int nz, T lx, T ly, T lz, int color);
-template <class T, int AHEAD>
+template <class T, int AHEAD, class D>
 __global__ void k_rbgs2(const T* uin, T* uout, const T* f, const T* a0, int nx, int ny, int nz, T lx, T ly, T lz,
                         int tiles_x, int tiles_y, int kchunk);
 template <class T>

@@ -371,11 +371,11 @@ __global__ void __launch_bounds__(256) k_restrict_a0(const T* __restrict__ a0f, 
 // order (round-robin over the 8 XCDs) is remapped so that each XCD runs a
 // contiguous run of tiles, which share their rings through its L2. AHEAD is
 // the number of steps a plane load is in flight.
-template <class T, int AHEAD>
-__global__ void __launch_bounds__(RB_THREADS) k_rbgs2(const T* __restrict__ uin, T* __restrict__ uout,
+template <class T, int AHEAD, class D>
+__global__ void __launch_bounds__(D::THREADS) k_rbgs2(const T* __restrict__ uin, T* __restrict__ uout,
                                                       const T* __restrict__ f, const T* __restrict__ a0, int nx, int ny,
                                                       int nz, T lx, T ly, T lz, int tiles_x, int tiles_y, int kchunk) {
-    __shared__ RbShared<T> sh;
+    __shared__ RbShared<T, D> sh;
     constexpr int X = 8;
     const int nb = gridDim.x, per = nb / X, rem = nb % X;
     const int xcd = blockIdx.x % X;
@@ -383,7 +383,7 @@ __global__ void __launch_bounds__(RB_THREADS) k_rbgs2(const T* __restrict__ uin,
     const int bx = b % tiles_x, by = (b / tiles_x) % tiles_y, bz = b / (tiles_x * tiles_y);
     const RbArgs<T> a{uin, uout, f, a0, nx, ny, nz, lx, ly, lz};
     const int tid = threadIdx.x;
-    RbThread<T> t;
+    RbThread<T, D> t;
     RbPlane<T> pa, pb;
     rb_init(a, bx, by, bz, kchunk, tid, t);
     rb_prologue(a, sh, t, tid);
@@ -722,10 +722,12 @@ __global__ void __launch_bounds__(256) k_sum_chunks(const double* __restrict__ p
 template __global__ void k_rbgs<float>(float*, const float*, const float*, int, int, int, float, float, float, int);
 template __global__ void k_rbgs<double>(double*, const double*, const double*, int, int, int, double, double, double,
                                         int);
-template __global__ void k_rbgs2<float, 2>(const float*, float*, const float*, const float*, int, int, int, float,
-                                           float, float, int, int, int);
-template __global__ void k_rbgs2<double, 2>(const double*, double*, const double*, const double*, int, int, int, double,
-                                            double, double, int, int, int);
+template __global__ void k_rbgs2<float, 2, RbNarrow>(const float*, float*, const float*, const float*, int, int, int,
+                                                     float, float, float, int, int, int);
+template __global__ void k_rbgs2<float, 2, RbWide>(const float*, float*, const float*, const float*, int, int, int,
+                                                   float, float, float, int, int, int);
+template __global__ void k_rbgs2<double, 2, RbNarrow>(const double*, double*, const double*, const double*, int, int,
+                                                      int, double, double, double, int, int, int);
 template __global__ void k_zm_residual<float>(const float*, const float*, const float*, float*, int, int, int, float,
                                               float, float, int, int, int);
 template __global__ void k_zm_residual<double>(const double*, const double*, const double*, double*, int, int, int,

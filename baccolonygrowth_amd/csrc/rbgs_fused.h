@@ -40,17 +40,27 @@
 namespace bcg {
 namespace mg {
 
-constexpr int RB_TX = 30, RB_TY = 30;                 // output tile
-constexpr int RB_RX = RB_TX + 2, RB_RY = RB_TY + 2;   // red region: tile + 1-point ring
-constexpr int RB_UX = RB_TX + 4, RB_UY = RB_TY + 4;   // old-iterate region: tile + 2-point ring
-constexpr int RB_THREADS = (RB_RX / 2) * RB_RY;       // one thread per x-pair of the red region
-constexpr int RB_PLANE = RB_UY * RB_UX;               // old-iterate values per plane
-static_assert(RB_PLANE > 2 * RB_THREADS && RB_PLANE <= 3 * RB_THREADS, "three plane loads per thread");
+// Output tiles: 30 x 30 (FP64: 44 KB of LDS; FP32 on the smaller levels) and
+// 62 x 30 (FP32 on the finest levels: 43 KB, 1024 threads), where the wider
+// tile spreads the fixed cost of a step over twice the points and trims the
+// ring re-reads.
+template <int TX_, int TY_>
+struct RbTile {
+    static constexpr int TX = TX_, TY = TY_;        // output tile
+    static constexpr int RX = TX + 2, RY = TY + 2;  // red region: tile + 1-point ring
+    static constexpr int UX = TX + 4, UY = TY + 4;  // old-iterate region: tile + 2-point ring
+    static constexpr int THREADS = (RX / 2) * RY;   // one thread per x-pair of the red region
+    static constexpr int PLANE = UY * UX;           // old-iterate values per plane
+    static_assert(PLANE > 2 * THREADS && PLANE <= 3 * THREADS, "three plane loads per thread");
+    static_assert(THREADS <= 1024, "one workgroup");
+};
+using RbNarrow = RbTile<30, 30>;
+using RbWide = RbTile<62, 30>;
 
-template <class T>
+template <class T, class D>
 struct RbShared {
-    T U[3][RB_UY * RB_UX];  // old iterate, planes k, k+1, k+2 (slot = plane mod 3)
-    T R[2][RB_RY * RB_RX];  // new red values, planes k, k+1 (slot = plane mod 2)
+    T U[3][D::UY * D::UX];  // old iterate, planes k, k+1, k+2 (slot = plane mod 3)
+    T R[2][D::RY * D::RX];  // new red values, planes k, k+1 (slot = plane mod 2)
 };
 
 template <class T>
@@ -67,7 +77,7 @@ struct RbArgs {
 // with selects (a runtime-indexed array would live in scratch memory).
 // Global accesses use 32-bit byte offsets from a per-plane base pointer that
 // is the same for the whole workgroup (one scalar base, one vector offset).
-template <class T>
+template <class T, class D>
 struct RbThread {
     int x0, y0, k0, k1, kfirst;  // workgroup-uniform
     int par;                     // (x + y) & 1 of the pair's first column
@@ -117,40 +127,40 @@ RBGS_HD inline const T* rb_at(const T* base, unsigned byte_off) {
 }
 
 // column of the pair that has a red point on plane k+1 (0 or 1)
-template <class T>
-RBGS_HD inline int rb_active(const RbThread<T>& t, int k) {
+template <class T, class D>
+RBGS_HD inline int rb_active(const RbThread<T, D>& t, int k) {
     return (t.par + k + 1) & 1;
 }
 
-template <class T>
-RBGS_HD inline void rb_init(const RbArgs<T>& a, int bx, int by, int bz, int kchunk, int tid, RbThread<T>& t) {
-    t.x0 = bx * RB_TX;
-    t.y0 = by * RB_TY;
+template <class T, class D>
+RBGS_HD inline void rb_init(const RbArgs<T>& a, int bx, int by, int bz, int kchunk, int tid, RbThread<T, D>& t) {
+    t.x0 = bx * D::TX;
+    t.y0 = by * D::TY;
     t.k0 = bz * kchunk;
     t.k1 = t.k0 + kchunk < a.nz ? t.k0 + kchunk : a.nz;
     t.kfirst = t.k0 - 2 > -1 ? t.k0 - 2 : -1;
-    const int px = tid % (RB_RX / 2), ry = tid / (RB_RX / 2);
+    const int px = tid % (D::RX / 2), ry = tid / (D::RX / 2);
     const int y = t.y0 - 1 + ry;
     const int xa = t.x0 - 1 + 2 * px;
     t.par = (xa + y) & 1;
-    t.ui = (ry + 1) * RB_UX + 2 * px + 1;
-    t.ri = ry * RB_RX + 2 * px;
+    t.ui = (ry + 1) * D::UX + 2 * px + 1;
+    t.ri = ry * D::RX + 2 * px;
     const int gy = rb_mirror(y, a.ny);
     t.cb0 = (unsigned)((gy * a.nx + rb_mirror(xa, a.nx)) * (int)sizeof(T));
     t.cb1 = (unsigned)((gy * a.nx + rb_mirror(xa + 1, a.nx)) * (int)sizeof(T));
-    const bool yin = y >= t.y0 && y < t.y0 + RB_TY && y < a.ny;
-    t.in0 = yin && xa >= t.x0 && xa < t.x0 + RB_TX && xa < a.nx;
-    t.in1 = yin && xa + 1 >= t.x0 && xa + 1 < t.x0 + RB_TX && xa + 1 < a.nx;
+    const bool yin = y >= t.y0 && y < t.y0 + D::TY && y < a.ny;
+    t.in0 = yin && xa >= t.x0 && xa < t.x0 + D::TX && xa < a.nx;
+    t.in1 = yin && xa + 1 >= t.x0 && xa + 1 < t.x0 + D::TX && xa + 1 < a.nx;
     t.ob = (t.in0 || t.in1) ? (unsigned)((y * a.nx + xa + 1) * (int)sizeof(T)) : 0u;
     auto src = [&](int e) {
-        const int r = e / RB_UX;
-        const int cx = e - r * RB_UX;
+        const int r = e / D::UX;
+        const int cx = e - r * D::UX;
         return (unsigned)((rb_mirror(t.y0 - 2 + r, a.ny) * a.nx + rb_mirror(t.x0 - 2 + cx, a.nx)) * (int)sizeof(T));
     };
     t.ob0 = src(tid);
-    t.ob1 = src(tid + RB_THREADS);
-    t.has2 = tid + 2 * RB_THREADS < RB_PLANE;
-    t.ob2 = t.has2 ? src(tid + 2 * RB_THREADS) : 0u;
+    t.ob1 = src(tid + D::THREADS);
+    t.has2 = tid + 2 * D::THREADS < D::PLANE;
+    t.ob2 = t.has2 ? src(tid + 2 * D::THREADS) : 0u;
     t.ucol0 = t.ucol1 = t.rcol0 = t.rcol1 = T(0);
     t.cf0 = t.cf1 = t.ca0 = t.ca1 = t.bf = t.ba = T(0);
     t.nf0 = t.nf1 = t.na0 = t.na1 = T(0);
@@ -158,8 +168,8 @@ RBGS_HD inline void rb_init(const RbArgs<T>& a, int bx, int by, int bz, int kchu
 
 // This thread's share of the old iterate of plane p (0 <= p <= nz; plane
 // nz is the mirror of nz-2) on the tile + 2-point ring, into registers.
-template <class T>
-RBGS_HD inline void rb_fetch_plane(const RbArgs<T>& a, const RbThread<T>& t, int p, RbPlane<T>& r) {
+template <class T, class D>
+RBGS_HD inline void rb_fetch_plane(const RbArgs<T>& a, const RbThread<T, D>& t, int p, RbPlane<T>& r) {
     const int pp = p == a.nz ? a.nz - 2 : p;
     const T* src = a.uin + (long)pp * a.nx * a.ny;
     r.v0 = *rb_at(src, t.ob0);
@@ -168,19 +178,19 @@ RBGS_HD inline void rb_fetch_plane(const RbArgs<T>& a, const RbThread<T>& t, int
 }
 
 // ... and from the registers into the LDS slot of plane p.
-template <class T>
-RBGS_HD inline void rb_store_plane(RbShared<T>& sh, const RbThread<T>& t, int p, int tid, const RbPlane<T>& r) {
+template <class T, class D>
+RBGS_HD inline void rb_store_plane(RbShared<T, D>& sh, const RbThread<T, D>& t, int p, int tid, const RbPlane<T>& r) {
     T* dst = sh.U[rb_slot3(p)];
     dst[tid] = r.v0;
-    dst[tid + RB_THREADS] = r.v1;
-    if (t.has2) dst[tid + 2 * RB_THREADS] = r.v2;
+    dst[tid + D::THREADS] = r.v1;
+    if (t.has2) dst[tid + 2 * D::THREADS] = r.v2;
 }
 
 // f, a0 on plane p (0 <= p <= nz-1) at both columns of the pair, into the
 // in-flight registers. Each plane's values are loaded once: the active
 // column's on one step (red point), the other column's on the next (black).
-template <class T>
-RBGS_HD inline void rb_fetch_coef(const RbArgs<T>& a, RbThread<T>& t, int p) {
+template <class T, class D>
+RBGS_HD inline void rb_fetch_coef(const RbArgs<T>& a, RbThread<T, D>& t, int p) {
     const T* fp = a.f + (long)p * a.nx * a.ny;
     const T* ap = a.a0 + (long)p * a.nx * a.ny;
     t.nf0 = *rb_at(fp, t.cb0);
@@ -191,8 +201,8 @@ RBGS_HD inline void rb_fetch_coef(const RbArgs<T>& a, RbThread<T>& t, int p) {
 
 // End of step k: the black point of step k+1 is the other column on plane
 // k+1; the in-flight plane k+2 becomes the current one.
-template <class T>
-RBGS_HD inline void rb_take_coef(RbThread<T>& t, int k) {
+template <class T, class D>
+RBGS_HD inline void rb_take_coef(RbThread<T, D>& t, int k) {
     const int s = rb_active(t, k);
     t.bf = rb_pick(s, t.cf1, t.cf0);
     t.ba = rb_pick(s, t.ca1, t.ca0);
@@ -203,15 +213,15 @@ RBGS_HD inline void rb_take_coef(RbThread<T>& t, int k) {
 }
 
 // Plane p feeds the red points of plane p-1, computed on step p-2.
-template <class T>
-RBGS_HD inline bool rb_plane_needed(const RbArgs<T>& a, const RbThread<T>& t, int p) {
+template <class T, class D>
+RBGS_HD inline bool rb_plane_needed(const RbArgs<T>& a, const RbThread<T, D>& t, int p) {
     return p >= 0 && p <= a.nz && p - 2 < t.k1;
 }
 
 // Before the first step (then a barrier): old planes kfirst .. kfirst+2 (those
 // that exist) into LDS, f and a0 of the first step.
-template <class T>
-RBGS_HD inline void rb_prologue(const RbArgs<T>& a, RbShared<T>& sh, RbThread<T>& t, int tid) {
+template <class T, class D>
+RBGS_HD inline void rb_prologue(const RbArgs<T>& a, RbShared<T, D>& sh, RbThread<T, D>& t, int tid) {
     for (int p = t.kfirst; p <= t.kfirst + 2; p++)
         if (rb_plane_needed(a, t, p)) {
             RbPlane<T> r;
@@ -224,8 +234,8 @@ RBGS_HD inline void rb_prologue(const RbArgs<T>& a, RbShared<T>& sh, RbThread<T>
 
 // After the prologue is visible (then a barrier, as its slots are reused):
 // the old values below each column's first red point.
-template <class T>
-RBGS_HD inline void rb_prologue_init(const RbShared<T>& sh, RbThread<T>& t) {
+template <class T, class D>
+RBGS_HD inline void rb_prologue_init(const RbShared<T, D>& sh, RbThread<T, D>& t) {
     const int s0 = rb_active(t, t.kfirst);  // active at kfirst; the other column at kfirst + 1
     if (t.kfirst >= 0) rb_put(s0, t.ucol0, t.ucol1, sh.U[rb_slot3(t.kfirst)][t.ui + s0]);
     if (t.kfirst + 1 >= 0) rb_put(1 - s0, t.ucol0, t.ucol1, sh.U[rb_slot3(t.kfirst + 1)][t.ui + 1 - s0]);
@@ -234,16 +244,16 @@ RBGS_HD inline void rb_prologue_init(const RbShared<T>& sh, RbThread<T>& t) {
 // Step k, phase 1: issue the loads of f, a0 on plane k+2 (red points of
 // step k+1) and of the old iterate on plane k+2+ahead (into `later`; stored
 // on step k+ahead-1).
-template <class T>
-RBGS_HD inline void rb_fetch_next(const RbArgs<T>& a, RbThread<T>& t, int k, int ahead, RbPlane<T>& later) {
+template <class T, class D>
+RBGS_HD inline void rb_fetch_next(const RbArgs<T>& a, RbThread<T, D>& t, int k, int ahead, RbPlane<T>& later) {
     if (k + 1 < t.k1 && k + 2 <= a.nz - 1) rb_fetch_coef(a, t, k + 2);
     if (rb_plane_needed(a, t, k + 2 + ahead)) rb_fetch_plane(a, t, k + 2 + ahead, later);
 }
 
 // Step k, phase 3 (then a barrier): plane k+3 (`now`, in flight since step
 // k or k-1) into the slot of plane k.
-template <class T>
-RBGS_HD inline void rb_finish(const RbArgs<T>& a, RbShared<T>& sh, RbThread<T>& t, int k, int tid,
+template <class T, class D>
+RBGS_HD inline void rb_finish(const RbArgs<T>& a, RbShared<T, D>& sh, RbThread<T, D>& t, int k, int tid,
                               const RbPlane<T>& now) {
     if (rb_plane_needed(a, t, k + 3)) rb_store_plane(sh, t, k + 3, tid, now);
     rb_take_coef(t, k);
@@ -251,8 +261,8 @@ RBGS_HD inline void rb_finish(const RbArgs<T>& a, RbShared<T>& sh, RbThread<T>& 
 
 // Step k, phase 2: red on plane k+1, black on plane k, write plane k of the
 // new iterate.
-template <class T>
-RBGS_HD inline void rb_compute(const RbArgs<T>& a, RbShared<T>& sh, RbThread<T>& t, int k) {
+template <class T, class D>
+RBGS_HD inline void rb_compute(const RbArgs<T>& a, RbShared<T, D>& sh, RbThread<T, D>& t, int k) {
     const int s = rb_active(t, k);
     const T d2 = T(2) * (a.lx + a.ly + a.lz);
     const bool do_red = k + 1 <= a.nz - 1;
@@ -265,7 +275,7 @@ RBGS_HD inline void rb_compute(const RbArgs<T>& a, RbShared<T>& sh, RbThread<T>&
             red = fr;
         } else {
             const T* Up = sh.U[rb_slot3(k + 1)];
-            const T sum = a.lx * (Up[i - 1] + Up[i + 1]) + a.ly * (Up[i - RB_UX] + Up[i + RB_UX]) +
+            const T sum = a.lx * (Up[i - 1] + Up[i + 1]) + a.ly * (Up[i - D::UX] + Up[i + D::UX]) +
                           a.lz * (rb_pick(s, t.ucol0, t.ucol1) + above);
             red = (fr + sum) / (rb_pick(s, t.ca0, t.ca1) + d2);
         }
@@ -283,7 +293,7 @@ RBGS_HD inline void rb_compute(const RbArgs<T>& a, RbShared<T>& sh, RbThread<T>&
                 const T* Rk = sh.R[k & 1];
                 const int i = t.ri + s;
                 const T above = k == a.nz - 1 ? rown : red;
-                const T sum = a.lx * (Rk[i - 1] + Rk[i + 1]) + a.ly * (Rk[i - RB_RX] + Rk[i + RB_RX]) +
+                const T sum = a.lx * (Rk[i - 1] + Rk[i + 1]) + a.ly * (Rk[i - D::RX] + Rk[i + D::RX]) +
                               a.lz * (rown + above);
                 blk = (t.bf + sum) / (t.ba + d2);
             }
@@ -326,10 +336,11 @@ inline int z_ranges(long tiles, int nz, long resident, int halo, int minplanes) 
     return best;
 }
 
+template <class D>
 inline RbGrid rb_grid(int nx, int ny, int nz, long resident) {
     RbGrid g;
-    g.tiles_x = (nx + RB_TX - 1) / RB_TX;
-    g.tiles_y = (ny + RB_TY - 1) / RB_TY;
+    g.tiles_x = (nx + D::TX - 1) / D::TX;
+    g.tiles_y = (ny + D::TY - 1) / D::TY;
     const int nc = z_ranges((long)g.tiles_x * g.tiles_y, nz, resident, 3, 8);
     g.kchunk = (nz + nc - 1) / nc;
     g.nchunk = (nz + g.kchunk - 1) / g.kchunk;

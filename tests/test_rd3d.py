@@ -124,15 +124,16 @@ def test_mixed_equals_fp64_and_deterministic():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("nu", [(2, 2), (1, 3)])
-def test_fused_sweeps_equal_split_sweeps(nu):
+@pytest.mark.parametrize("nu,shape", [((2, 2), (65, 161, 225)), ((1, 3), (65, 161, 225)), ((2, 2), (129, 353, 369))])
+def test_fused_sweeps_equal_split_sweeps(nu, shape):
     """The z-marching two-colour sweep (the default on levels of >= 2^21
     points) gives the iterates of the one-launch-per-colour sweep, bitwise,
     in both solvers; odd sweep counts end with a split sweep. 225 x 161 x 65
-    leaves partial 30 x 30 tiles on both x and y edges."""
+    leaves partial 30 x 30 tiles on both x and y edges; 369 x 353 x 129
+    (>= 2^24 points) runs its FP32 sweeps on 62 x 30 tiles, partial on both
+    edges too."""
     from baccolonygrowth_amd.nutrients import SMOOTH_FUSED, SMOOTH_SPLIT
 
-    shape = (65, 161, 225)
     print(f"[gpu] rd3d fused vs split sweeps {shape} nu={nu}", flush=True)
     c_old, q, _ = _case(shape, seed=12)
     for mode in ("fp64", "mixed"):
