@@ -223,9 +223,11 @@ long resident(K kernel, int threads) {
 
 // `sweeps` red-black sweeps on u of level v. Fused sweeps go in pairs
 // u -> r -> u (the residual buffer is free while smoothing); an odd last
-// sweep, and all sweeps of small levels, use one launch per colour.
+// sweep, and all sweeps of small levels, use one launch per colour. With
+// `zero`, u starts at zero: the first fused sweep reads no iterate (u need
+// not be cleared); otherwise u is cleared first.
 template <class T>
-void smooth(bcg_rd3d* c, Level& v, int sweeps) {
+void smooth(bcg_rd3d* c, Level& v, int sweeps, bool zero = false) {
     T* u = Arr<T>::u(v);
     int s = 0;
     if (c->prm.smoother == BCG_RD3D_SMOOTH_FUSED && v.n() >= FUSED_MIN_POINTS) {
@@ -242,7 +244,7 @@ void smooth(bcg_rd3d* c, Level& v, int sweeps) {
                                    g.kchunk);
             };
             for (; s + 2 <= sweeps; s += 2) {
-                sweep(u, tmp);
+                sweep(zero && s == 0 ? nullptr : u, tmp);
                 sweep(tmp, u);
             }
         };
@@ -258,6 +260,8 @@ void smooth(bcg_rd3d* c, Level& v, int sweeps) {
         }
         if (!wide) sweeps2(RbNarrow{}, c->res_rb[sizeof(T) == 4]);
     }
+    if (zero && s == 0)
+        hipLaunchKernelGGL((k_fill<T>), dim3(nblk(v.n(), 256)), dim3(256), 0, c->stream, u, v.n(), (T)0);
     for (; s < sweeps; s++)
         for (int color = 0; color < 2; color++)
             hipLaunchKernelGGL((k_rbgs<T>), grid_rb(v), dim3(64, 4), 0, c->stream, u, Arr<T>::f(v), Arr<T>::a0(v),
@@ -285,13 +289,15 @@ void bottom_solve(bcg_rd3d* c, double* u0, double* f0) {
                        c->prm.coarse_sweeps, in_lds);
 }
 
-// One V-cycle at level l in precision T on (u_l, f_l); the levels from
-// c->bottom down are one FP64 single-workgroup launch (on FP64 copies of
-// u and f when T is float).
+// One V-cycle at level l in precision T on (u_l, f_l), from u_l = 0 when
+// `zero`; the levels from c->bottom down are one FP64 single-workgroup launch
+// (on FP64 copies of u and f when T is float).
 template <class T>
-void vcycle(bcg_rd3d* c, int l) {
+void vcycle(bcg_rd3d* c, int l, bool zero = false) {
     Level& v = c->L[l];
     if (l == c->bottom) {
+        if (zero)
+            hipLaunchKernelGGL((k_fill<T>), dim3(nblk(v.n(), 256)), dim3(256), 0, c->stream, Arr<T>::u(v), v.n(), (T)0);
         if (sizeof(T) == 4) {
             double* u64 = l ? v.u64 : c->bot_u0;
             double* f64 = l ? v.f64 : c->bot_f0;
@@ -308,7 +314,7 @@ void vcycle(bcg_rd3d* c, int l) {
         return;
     }
     Level& w = c->L[l + 1];
-    smooth<T>(c, v, c->prm.nu1);
+    smooth<T>(c, v, c->prm.nu1, zero);
     // z-marching residual on the large levels
     if (v.n() >= FUSED_MIN_POINTS) {
         const ZmGrid g = zm_grid(v.nx, v.ny, v.nz, c->res_zm);
@@ -321,8 +327,7 @@ void vcycle(bcg_rd3d* c, int l) {
     }
     hipLaunchKernelGGL((k_restrict<T>), slab_grid(w), dim3(64, 4), 0, c->stream, Arr<T>::r(v), Arr<T>::f(w), v.nx, v.ny, v.nz,
                        w.nx, w.ny, w.nz, v.cx, v.cy, v.cz);
-    hipLaunchKernelGGL((k_fill<T>), dim3(nblk(w.n(), 256)), dim3(256), 0, c->stream, Arr<T>::u(w), w.n(), (T)0);
-    vcycle<T>(c, l + 1);
+    vcycle<T>(c, l + 1, true);
     if ((long)v.nx * v.ny * v.nz >= FUSED_MIN_POINTS) {
         // z-marching: a coarse plane's interpolant serves two fine planes
         // (short ranges: many columns in flight hide the latency of each
@@ -696,8 +701,7 @@ int solve_core(bcg_rd3d* c, int mode, double tol, int max_iter, int* iters, doub
         double scale = 1.0;  // r32 = r * scale
         while (it < max_iter && rn > tol * r0) {
             // inner FP32 solve of A e = r32 from e = 0
-            hipLaunchKernelGGL((k_fill<float>), dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, f.u32, c->N, 0.0f);
-            for (int s = 0; s < c->prm.inner; s++) vcycle<float>(c, 0);
+            for (int s = 0; s < c->prm.inner; s++) vcycle<float>(c, 0, s == 0);
             // x += e / scale ; r = b - A x ; r32 = r * next scale
             const double rms = rn / std::sqrt((double)c->N);
             const double next = rms > 0 ? std::ldexp(1.0, -std::ilogb(rms)) : 1.0;
@@ -856,8 +860,7 @@ int bcg_rd3d_vcycle(bcg_rd3d* c, int32_t mode, float* ms) {
         f.u64 = nullptr;
     } else {
         hipLaunchKernelGGL((k_cast<double, float>), dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, c->b, f.f32, c->N);
-        hipLaunchKernelGGL((k_fill<float>), dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, f.u32, c->N, 0.0f);
-        vcycle<float>(c, 0);
+        vcycle<float>(c, 0, true);
     }
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     HIPCHK(c, hipGetLastError());

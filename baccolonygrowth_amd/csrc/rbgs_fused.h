@@ -167,9 +167,14 @@ RBGS_HD inline void rb_init(const RbArgs<T>& a, int bx, int by, int bz, int kchu
 }
 
 // This thread's share of the old iterate of plane p (0 <= p <= nz; plane
-// nz is the mirror of nz-2) on the tile + 2-point ring, into registers.
+// nz is the mirror of nz-2) on the tile + 2-point ring, into registers. A
+// null uin is the zero iterate (a sweep from a zero initial guess).
 template <class T, class D>
 RBGS_HD inline void rb_fetch_plane(const RbArgs<T>& a, const RbThread<T, D>& t, int p, RbPlane<T>& r) {
+    if (!a.uin) {
+        r.v0 = r.v1 = r.v2 = T(0);
+        return;
+    }
     const int pp = p == a.nz ? a.nz - 2 : p;
     const T* src = a.uin + (long)pp * a.nx * a.ny;
     r.v0 = *rb_at(src, t.ob0);
