@@ -262,20 +262,31 @@ __global__ void __launch_bounds__(256) k_zm_prolong(T* __restrict__ u, const T* 
     };
     const long sxy = (long)nx * ny;
     T* up = u + (long)j * nx + i;
-    for (int k = k0; k < k1; k++) {
-        int K0, K1;
-        T wk;
-        if (cz) {
-            K0 = k >> 1;
-            K1 = (k + 1) >> 1;
-            wk = (k & 1) ? T(0.5) : T(0);
-        } else {
-            K0 = K1 = k;
-            wk = T(0);
+    // four planes at a time, their loads all issued before the first is
+    // used: with one load in flight per thread the pass is latency-bound
+    constexpr int G = 4;
+    for (int kb = k0; kb < k1; kb += G) {
+        T old[G];
+#pragma unroll
+        for (int g = 0; g < G; g++) old[g] = kb + g < k1 ? up[(kb + g) * sxy] : T(0);
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int k = kb + g;
+            if (k >= k1) break;
+            int K0, K1;
+            T wk;
+            if (cz) {
+                K0 = k >> 1;
+                K1 = (k + 1) >> 1;
+                wk = (k & 1) ? T(0.5) : T(0);
+            } else {
+                K0 = K1 = k;
+                wk = T(0);
+            }
+            const T e0 = val(K0);
+            const T e1 = val(K1);
+            up[k * sxy] = old[g] + ((T(1) - wk) * e0 + wk * e1);
         }
-        const T e0 = val(K0);
-        const T e1 = val(K1);
-        up[k * sxy] += (T(1) - wk) * e0 + wk * e1;
     }
 }
 
