@@ -346,7 +346,7 @@ inline RbGrid rb_grid(int nx, int ny, int nz, long resident) {
     RbGrid g;
     g.tiles_x = (nx + D::TX - 1) / D::TX;
     g.tiles_y = (ny + D::TY - 1) / D::TY;
-    const int nc = z_ranges((long)g.tiles_x * g.tiles_y, nz, resident, 3, 8);
+    const int nc = z_ranges((long)g.tiles_x * g.tiles_y, nz, resident, 3, 4);
     g.kchunk = (nz + nc - 1) / nc;
     g.nchunk = (nz + g.kchunk - 1) / g.kchunk;
     return g;
