@@ -268,9 +268,15 @@ void smooth(bcg_rd3d* c, Level& v, int sweeps, bool zero = false) {
                                v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly, (T)v.lz, color);
 }
 
-void bottom_solve(bcg_rd3d* c, double* u0, double* f0) {
+// The bottom levels by the single-workgroup FP64 solver, with level 0 on
+// (u0, f0), or, in an FP32 cycle, read from and written back to (u32, f32)
+// with (u0, f0) as working copies; `zero`: from u = 0.
+void bottom_solve(bcg_rd3d* c, double* u0, double* f0, const float* f32, float* u32, bool zero) {
     BotLevels B;
     B.n = (int)c->L.size() - c->bottom;
+    B.f32 = f32;
+    B.u32 = u32;
+    B.zero = zero ? 1 : 0;
     for (int i = 0; i < B.n; i++) {
         Level& v = c->L[c->bottom + i];
         B.L[i] = BotLevel{v.u64, v.f64, v.a064, v.r64, v.nx, v.ny, v.nz, v.cx, v.cy, v.cz, v.lx, v.ly, v.lz};
@@ -296,21 +302,10 @@ template <class T>
 void vcycle(bcg_rd3d* c, int l, bool zero = false) {
     Level& v = c->L[l];
     if (l == c->bottom) {
-        if (zero)
-            hipLaunchKernelGGL((k_fill<T>), dim3(nblk(v.n(), 256)), dim3(256), 0, c->stream, Arr<T>::u(v), v.n(), (T)0);
-        if (sizeof(T) == 4) {
-            double* u64 = l ? v.u64 : c->bot_u0;
-            double* f64 = l ? v.f64 : c->bot_f0;
-            hipLaunchKernelGGL((k_cast<float, double>), dim3(nblk(v.n(), 256)), dim3(256), 0, c->stream, v.f32, f64,
-                               v.n());
-            hipLaunchKernelGGL((k_cast<float, double>), dim3(nblk(v.n(), 256)), dim3(256), 0, c->stream, v.u32, u64,
-                               v.n());
-            bottom_solve(c, u64, f64);
-            hipLaunchKernelGGL((k_cast<double, float>), dim3(nblk(v.n(), 256)), dim3(256), 0, c->stream, u64, v.u32,
-                               v.n());
-        } else {
-            bottom_solve(c, v.u64, v.f64);
-        }
+        if (sizeof(T) == 4)
+            bottom_solve(c, l ? v.u64 : c->bot_u0, l ? v.f64 : c->bot_f0, v.f32, v.u32, zero);
+        else
+            bottom_solve(c, v.u64, v.f64, nullptr, nullptr, zero);
         return;
     }
     Level& w = c->L[l + 1];

@@ -17,6 +17,11 @@ constexpr int BOT_MAX_LEVELS = 8;
 struct BotLevels {
     BotLevel L[BOT_MAX_LEVELS];
     int n;
+    // level 0 from FP32 arrays (f32, u32; u32 also receives the result)
+    // instead of L[0].f, L[0].u, and/or from u = 0 (zero)
+    const float* f32;
+    float* u32;
+    int zero;
 };
 __global__ void k_bottom(BotLevels B, int nu1, int nu2, int coarse_sweeps, int in_lds);
 constexpr long BOT_LDS_BYTES = 60 * 1024;  // levels kept in LDS when they fit

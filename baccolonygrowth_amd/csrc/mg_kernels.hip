@@ -635,18 +635,22 @@ __global__ void __launch_bounds__(1024) k_bottom(BotLevels B, int nu1, int nu2, 
             for (int l = 0; l < B.n; l++) S[l].r = smem + off;
     }
     __syncthreads();
-    if (in_lds) {
+    if (in_lds)
         for (int l = 0; l < B.n; l++) {
             const int n = (int)S[l].nx * S[l].ny * S[l].nz;
             for (int p = tid; p < n; p += nthr) S[l].a0[p] = B.L[l].a0[p];
         }
-        const int n0 = (int)S[0].nx * S[0].ny * S[0].nz;
+    // level 0 into its working arrays: from the FP32 arrays, from memory into
+    // LDS, and/or from u = 0 (in place otherwise)
+    const int n0 = (int)S[0].nx * S[0].ny * S[0].nz;
+    if (in_lds || B.f32 || B.zero)
         for (int p = tid; p < n0; p += nthr) {
-            S[0].u[p] = B.L[0].u[p];
-            S[0].f[p] = B.L[0].f[p];
+            const double f = B.f32 ? (double)B.f32[p] : B.L[0].f[p];
+            const double u = B.zero ? 0.0 : (B.u32 ? (double)B.u32[p] : B.L[0].u[p]);
+            S[0].f[p] = f;
+            S[0].u[p] = u;
         }
-        __syncthreads();
-    }
+    __syncthreads();
     for (int l = 0; l < last; l++) {
         const BotLevel& v = S[l];
         const BotLevel& w = S[l + 1];
@@ -710,10 +714,10 @@ __global__ void __launch_bounds__(1024) k_bottom(BotLevels B, int nu1, int nu2, 
                 __syncthreads();
             }
     }
-    if (in_lds) {
-        const int n0 = (int)S[0].nx * S[0].ny * S[0].nz;
+    if (B.u32)
+        for (int p = tid; p < n0; p += nthr) B.u32[p] = (float)S[0].u[p];
+    else if (in_lds)
         for (int p = tid; p < n0; p += nthr) B.L[0].u[p] = S[0].u[p];
-    }
 }
 
 // Deterministic reduction of many partial sums: fixed chunks per block.
