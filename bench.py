@@ -43,11 +43,13 @@ def log(*a):
 # algorithmic). kbar = mean reduced list length.
 def phase_bytes(kbar: float) -> dict:
     return {
-        # cm from p,q (48) + key write (4) + count RMW (8) + key read (4) +
-        # fill RMW (8) + sorted write (4) + per-bin sort read/write (8)
-        "bins": 84.0,
-        # sorted/order/slot (12) + own p,q,L,R (64) + list writes (4 kbar) + count (4)
-        "lists": 80.0 + 4.0 * kbar,
+        # count: p,q (48) + key, slot writes (8) + count RMW (8) + FP32 end
+        # points (32); place: key, slot, start (12) + write (4); rank: position,
+        # key, bin bounds (16) + sorted (4) + end points in/out (64); scan (~5)
+        "bins": 201.0,
+        # own p,q,L,R (64) + own id (4) + candidate stream read once (32 + 4)
+        # + order/slot/count writes (12) + list writes (4 kbar)
+        "lists": 116.0 + 4.0 * kbar,
         # order+count (8) + list (4 kbar) + hot record in/out (128+128) +
         # type, wall, growth rate (20) + DynFric/StaFric (48) + flag (4)
         "move_grow": 336.0 + 4.0 * kbar,
