@@ -309,3 +309,89 @@ def test_partial_rebuild_matches_oracle(Colony, oracle):
         cb, lb = col.neighbours()
         assert sorted_lists(ca, la) == sorted_lists(cb, lb)
     o.close()
+
+
+def _threshold_case(seed=7):
+    """Pairs of cells, 20 um apart from every other pair and several hundred um
+    from the origin, each placed at one of the reference predicate's
+    thresholds (ABM/Neighbours.cpp:72-88), to relative offsets from 2e-16 to
+    1e-6 on either side:
+      * B crossing A at 0.8 um (the second test passes) with both end points at
+        distance (L_A/2 + Rc R_A)(1 + d) from the centre of A (the first test
+        decides);
+      * B nearly parallel to A at a distance Rc R_A (1 + d) (the second test
+        decides, the first one passing).
+    The FP32 candidate prefilter must keep every pair the FP64 predicate keeps,
+    and the square roots must be decided as the reference decides them."""
+    from baccolonygrowth_amd.cells import reference_defaults
+    from baccolonygrowth_amd.colony import dome_params
+
+    rng = np.random.default_rng(seed)
+    Rc = reference_defaults().Rc
+    R = 0.5
+    offsets = [s * m for m in (2e-16, 1e-15, 4e-15, 1e-14, 1e-12, 1e-9, 1e-6) for s in (-1.0, 1.0)]
+    cells = []
+
+    def rod(c, u, L):
+        return (c - 0.5 * L * u, c + 0.5 * L * u, L)
+
+    def unit(v):
+        return v / np.linalg.norm(v)
+
+    k = 0
+    for family in ("crossing", "parallel"):
+        for d in offsets:
+            for rep in range(6):
+                c = np.array([420.0 + 20.0 * (k % 12), -640.0 + 20.0 * (k // 12), 1.5 + rng.uniform(0.0, 3.0)])
+                k += 1
+                uA = unit(np.array([np.cos(a := rng.uniform(0, 2 * np.pi)), np.sin(a), rng.normal(0, 0.15)]))
+                LA = rng.uniform(2.0, 4.0)
+                pA, qA, _ = rod(c, uA, LA)
+                LB = rng.uniform(2.0, 4.0)
+                if family == "crossing":
+                    n = unit(np.cross(uA, rng.normal(size=3)))
+                    uB = np.cross(uA, n)
+                    h = 0.8
+                    half = np.sqrt(((LA / 2 + Rc * R) * (1.0 + d)) ** 2 - h * h)
+                    pB, qB = c + n * h - uB * half, c + n * h + uB * half
+                else:
+                    n = unit(np.cross(uA, rng.normal(size=3)))
+                    uB = unit(uA + 1e-3 * unit(np.cross(uA, n)))
+                    cB = c + n * (Rc * R) * (1.0 + d) + uA * rng.uniform(-0.3, 0.3)
+                    pB, qB, _ = rod(cB, uB, LB)
+                for pp, qq, LL in ((pA, qA, LA), (pB, qB, np.linalg.norm(qB - pB))):
+                    cells.append((pp, qq, LL))
+    a = np.zeros(len(cells), CELL_DTYPE)
+    for i, (pp, qq, LL) in enumerate(cells):
+        a[i]["p"] = pp
+        a[i]["q"] = qq
+        a[i]["Length"] = LL
+        a[i]["Radius"] = R
+        a[i]["Ldiv"] = 10.0
+        a[i]["Type"] = 1
+    p = dome_params(a, bin_size=5.0, max_neighbours=16, extra_cells=64)
+    return p, a
+
+
+def test_lists_at_the_predicate_thresholds(Colony, oracle):
+    """Neighbour sets equal the oracle's on pairs placed at the predicate's
+    thresholds far from the origin (FP32 prefilter margin, square roots
+    decided near the threshold)."""
+    p, cells = _threshold_case()
+    say(f"[gpu] threshold pairs: {len(cells) // 2} pairs, bins {p.BoxX}x{p.BoxY}x{p.BoxZ}")
+    o = oracle.sim(p)
+    o.set_cells(cells)
+    o.rebuild()
+    with Colony(p) as col:
+        col.LoadCells(cells)
+        col.getNeighbours()
+        ca, la = o.neighbours(stride=p.maxNeighbours)
+        cb, lb = col.neighbours()
+        assert np.array_equal(ca, cb)
+        assert sorted_lists(ca, la) == sorted_lists(cb, lb)
+    o.close()
+    # the case decides both ways at both thresholds
+    kept = ca.reshape(2, -1, 2)[:, :, 0]  # anchor's list length, per family and pair
+    say(f"    kept: crossing family {int((kept[0] > 0).sum())}/{kept.shape[1]}, parallel family {int((kept[1] > 0).sum())}/{kept.shape[1]}")
+    for fam in kept:
+        assert 0 < (fam > 0).sum() < len(fam)
