@@ -9,7 +9,7 @@ namespace bcg {
 constexpr int ERRBIT_DOMAIN = 1;  // a centre of mass outside the bin grid
 constexpr int ERRBIT_NLIST = 2;   // a reduced list longer than its capacity
 
-constexpr int MOVE_BLOCK = 128;
+constexpr int MOVE_BLOCK = 64;
 constexpr int PATCH_BLOCK = 256;
 constexpr int SCAN_BLOCK_ITEMS = 2048;  // items per block of the scan kernels
 
