@@ -145,8 +145,8 @@ struct bcg_rd3d {
     // workgroups resident on the device at once, for the z-range counts of
     // the z-marching launches: k_rbgs2 (FP64, FP32 narrow, FP32 wide tile),
     // the fine-level passes (residual, residual norm, refinement update),
-    // k_zm_prolong (FP64, FP32)
-    long res_rb[3] = {1, 1, 1}, res_zm = 1, res_zp[2] = {1, 1};
+    // k_zm_prolong and k_zm_restrict (FP64, FP32)
+    long res_rb[3] = {1, 1, 1}, res_zm = 1, res_zp[2] = {1, 1}, res_zr[2] = {1, 1};
     std::string err;
 };
 
@@ -320,8 +320,20 @@ void vcycle(bcg_rd3d* c, int l, bool zero = false) {
         hipLaunchKernelGGL((k_residual<T>), slab_grid(v), dim3(64, 4), 0, c->stream, Arr<T>::u(v), Arr<T>::f(v),
                            Arr<T>::a0(v), Arr<T>::r(v), v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly, (T)v.lz);
     }
-    hipLaunchKernelGGL((k_restrict<T>), slab_grid(w), dim3(64, 4), 0, c->stream, Arr<T>::r(v), Arr<T>::f(w), v.nx, v.ny, v.nz,
-                       w.nx, w.ny, w.nz, v.cx, v.cy, v.cz);
+    if (v.n() >= WIDE_MIN_POINTS) {
+        // z-marching: each fine plane's x-y sum serves both coarse planes it
+        // touches (measured on the finest level only: 65 vs 78 us FP32, 100 vs
+        // 104 us FP64 on 513 x 513 x 129; slower than the plain kernel below)
+        const long cols = (long)nblk(w.nx, 64) * nblk(w.ny, 4);
+        const int nc = z_ranges(cols, w.nz, c->res_zr[sizeof(T) == 4], 1, 4);
+        const int kchunk = (w.nz + nc - 1) / nc;
+        hipLaunchKernelGGL((k_zm_restrict<T>), slab_grid(w.nx, w.ny, (w.nz + kchunk - 1) / kchunk), dim3(256), 0,
+                           c->stream, Arr<T>::r(v), Arr<T>::f(w), v.nx, v.ny, v.nz, w.nx, w.ny, w.nz, v.cx, v.cy, v.cz,
+                           kchunk);
+    } else {
+        hipLaunchKernelGGL((k_restrict<T>), slab_grid(w), dim3(64, 4), 0, c->stream, Arr<T>::r(v), Arr<T>::f(w), v.nx,
+                           v.ny, v.nz, w.nx, w.ny, w.nz, v.cx, v.cy, v.cz);
+    }
     vcycle<T>(c, l + 1, true);
     if ((long)v.nx * v.ny * v.nz >= FUSED_MIN_POINTS) {
         // z-marching: a coarse plane's interpolant serves two fine planes
@@ -452,6 +464,8 @@ int bcg_rd3d_create(const bcg_rd3d_params* p, bcg_rd3d** out) {
                           resident(k_zm_ir, ZM_THREADS)});
     c->res_zp[0] = resident(k_zm_prolong<double>, 256);
     c->res_zp[1] = resident(k_zm_prolong<float>, 256);
+    c->res_zr[0] = resident(k_zm_restrict<double>, 256);
+    c->res_zr[1] = resident(k_zm_restrict<float>, 256);
     // the small coarse levels (at most BOT_MAX_LEVELS, each <= BOTTOM_MAX_POINTS)
     // form the bottom solver
     c->bottom = (int)c->L.size() - 1;

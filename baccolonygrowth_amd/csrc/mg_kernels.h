@@ -48,6 +48,9 @@ __global__ void k_restrict(const T* r, T* fc, int nx, int ny, int nz, int ncx, i
 template <class T>
 __global__ void k_prolong(T* u, const T* ec, int nx, int ny, int nz, int ncx, int ncy, int cx, int cy, int cz);
 template <class T>
+__global__ void k_zm_restrict(const T* r, T* fc, int nx, int ny, int nz, int ncx, int ncy, int ncz, int cx, int cy, int cz,
+                              int kchunk);
+template <class T>
 __global__ void k_zm_prolong(T* u, const T* ec, int nx, int ny, int nz, int ncx, int ncy, int cx, int cy, int cz,
                              int kchunk);
 template <class T>

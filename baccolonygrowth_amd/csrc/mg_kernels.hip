@@ -150,6 +150,63 @@ __global__ void __launch_bounds__(256) k_restrict(const T* __restrict__ r, T* __
     fc[((long)K * ncy + J) * ncx + I] = restrict_point<T>(r, nx, ny, nz, I, J, K, cx, cy, cz);
 }
 
+// restrict_point for coarse planes [K0, K0 + kchunk) of one coarse column
+// (I, J), marching up: the x-y weighted sum of a fine plane (restrict_point's
+// accy) is formed once and serves the two coarse planes it belongs to. Same
+// operations in the same order as restrict_point.
+template <class T>
+__global__ void __launch_bounds__(256) k_zm_restrict(const T* __restrict__ r, T* __restrict__ fc, int nx, int ny, int nz,
+                                                     int ncx, int ncy, int ncz, int cx, int cy, int cz, int kchunk) {
+    int bx, by, bz;
+    slab_tile((ncx + 63) / 64, (ncy + 3) / 4, bx, by, bz);
+    const int I = bx * 64 + (threadIdx.x & 63), J = by * 4 + (threadIdx.x >> 6);
+    if (I >= ncx || J >= ncy) return;
+    const int K0 = bz * kchunk, K1 = min(K0 + kchunk, ncz);
+    const int i0 = cx ? 2 * I : I, j0 = cy ? 2 * J : J;
+    const int im = mirror(i0 - 1, nx), ip = mirror(i0 + 1, nx);
+    long rows[3];
+    for (int b = -cy; b <= cy; b++) rows[b + 1] = (long)mirror(j0 + b, ny) * nx;
+    const long sxy = (long)nx * ny;
+    auto plane = [&](int kk) {
+        const T* base = r + kk * sxy;
+        T accy = T(0);
+#pragma unroll
+        for (int b = -1; b <= 1; b++) {
+            if (b < -cy || b > cy) continue;
+            const T wy = (b == 0) ? T(cy ? 0.5 : 1.0) : T(0.25);
+            const T* row = base + rows[b + 1];
+            T accx;
+            if (cx) accx = T(0.25) * row[im] + T(0.5) * row[i0] + T(0.25) * row[ip];
+            else accx = row[i0];
+            accy += wy * accx;
+        }
+        return accy;
+    };
+    T* out = fc + (long)J * ncx + I;
+    const long scxy = (long)ncx * ncy;
+    int kc = -1;  // fine plane whose sum is cached
+    T pc = T(0);
+    for (int K = K0; K < K1; K++) {
+        if (K == 0) {
+            out[0] = T(0);  // Dirichlet rows: zero correction
+            continue;
+        }
+        const int k0 = cz ? 2 * K : K;
+        T acc = T(0);
+        for (int c = -cz; c <= cz; c++) {
+            const int kk = mirror(k0 + c, nz);
+            const T wz = (c == 0) ? T(cz ? 0.5 : 1.0) : T(0.25);
+            const T pk = kk == kc ? pc : plane(kk);
+            acc += wz * pk;
+            if (c == cz) {
+                kc = kk;
+                pc = pk;
+            }
+        }
+        out[K * scxy] = acc;
+    }
+}
+
 // Trilinear interpolation of the coarse correction at fine point (i,j,k).
 template <class T>
 __device__ __forceinline__ T prolong_point(const T* __restrict__ ec, int ncx, int ncy, int i, int j, int k, int cx, int cy,
@@ -755,6 +812,9 @@ template __global__ void k_restrict<float>(const float*, float*, int, int, int, 
 template __global__ void k_restrict<double>(const double*, double*, int, int, int, int, int, int, int, int, int);
 template __global__ void k_prolong<float>(float*, const float*, int, int, int, int, int, int, int, int);
 template __global__ void k_prolong<double>(double*, const double*, int, int, int, int, int, int, int, int);
+template __global__ void k_zm_restrict<float>(const float*, float*, int, int, int, int, int, int, int, int, int, int);
+template __global__ void k_zm_restrict<double>(const double*, double*, int, int, int, int, int, int, int, int, int,
+                                               int);
 template __global__ void k_zm_prolong<float>(float*, const float*, int, int, int, int, int, int, int, int, int);
 template __global__ void k_zm_prolong<double>(double*, const double*, int, int, int, int, int, int, int, int, int);
 template __global__ void k_fill<float>(float*, long, float);
