@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/bcg.h"
+#include "ctx_internal.h"
 #include "mech_kernels.h"
 #include "mg_kernels.h"
 #include "rbgs_fused.h"
@@ -115,6 +116,50 @@ __global__ void __launch_bounds__(256) k_dep_gather(const int* __restrict__ star
     q[((long)k * ny + j) * nx + i] = acc * inv_vol;
 }
 
+// Sources of the mechanics context's cells: centre (p + q) / 2 and uptake
+// rate x Length (hot record: p [0..2], q [3..5], Length [12]).
+__global__ void k_cells_src(const double* __restrict__ hot, int n, double rate, double* __restrict__ cm,
+                            double* __restrict__ r) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* h = hot + (size_t)i * bcg::HOT;
+    cm[3 * i] = (h[0] + h[3]) * 0.5;
+    cm[3 * i + 1] = (h[1] + h[4]) * 0.5;
+    cm[3 * i + 2] = (h[2] + h[5]) * 0.5;
+    r[i] = rate * h[12];
+}
+
+// Growth rate of every mechanics bin (x * BoxY * BoxZ + y * BoxZ + z) from
+// the field at the bin centre: mu c / (K + c), c trilinear in the grid,
+// positions clamped to it. Bin centres follow the bin address rule
+// (x = int(x/BL + BoxX/2), y = int(y/BL + BoxY/2 + 0.5), z = int(z/BL)).
+__global__ void k_growth_bins(const double* __restrict__ x, int nx, int ny, int nz, double ox, double oy, double oz,
+                              double inv_h, int BoxX, int BoxY, int BoxZ, double BL, double mu, double K,
+                              double* __restrict__ gr) {
+    const long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= (long)BoxX * BoxY * BoxZ) return;
+    const int xb = (int)(b / ((long)BoxY * BoxZ)), yb = (int)((b / BoxZ) % BoxY), zb = (int)(b % BoxZ);
+    const double px = (xb - BoxX / 2 + 0.5) * BL, py = (yb - BoxY / 2) * BL, pz = (zb + 0.5) * BL;
+    auto axis = [](double g, int n, int& i0, double& w) {
+        g = g < 0.0 ? 0.0 : (g > n - 1 ? n - 1 : g);
+        i0 = (int)g;
+        if (i0 > n - 2) i0 = n - 2;
+        w = g - i0;
+    };
+    int i, j, k;
+    double wx, wy, wz;
+    axis((px - ox) * inv_h, nx, i, wx);
+    axis((py - oy) * inv_h, ny, j, wy);
+    axis((pz - oz) * inv_h, nz, k, wz);
+    const long sxy = (long)nx * ny;
+    auto at = [&](int di, int dj, int dk) { return x[(k + dk) * sxy + (long)(j + dj) * nx + (i + di)]; };
+    const double c00 = at(0, 0, 0) * (1 - wx) + at(1, 0, 0) * wx, c10 = at(0, 1, 0) * (1 - wx) + at(1, 1, 0) * wx;
+    const double c01 = at(0, 0, 1) * (1 - wx) + at(1, 0, 1) * wx, c11 = at(0, 1, 1) * (1 - wx) + at(1, 1, 1) * wx;
+    double c = (c00 * (1 - wy) + c10 * wy) * (1 - wz) + (c01 * (1 - wy) + c11 * wy) * wz;
+    c = c > 0.0 ? c : 0.0;
+    gr[b] = mu * c / (K + c);
+}
+
 inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
 
 }  // namespace
@@ -140,7 +185,7 @@ struct bcg_rd3d {
     int *dkey = nullptr, *dcnt = nullptr, *dstart = nullptr, *dfill = nullptr, *dsorted = nullptr, *scan_tot = nullptr;
     double *dfrac = nullptr, *dcm = nullptr, *drate = nullptr;
     int dcap = 0;
-    hipEvent_t ev[4];
+    hipEvent_t ev[5];  // [0..3] timing, [4] ordering against a mechanics context
     float last_ms[3] = {0, 0, 0};
     // workgroups resident on the device at once, for the z-range counts of
     // the z-marching launches: k_rbgs2 (FP64, FP32 narrow, FP32 wide tile),
@@ -510,7 +555,7 @@ int bcg_rd3d_create(const bcg_rd3d_params* p, bcg_rd3d** out) {
     A(&c->gpart, 2 * (size_t)c->npart);
     A(&c->dnorm, 1);
     if (!rc && hipHostMalloc((void**)&c->hnorm, sizeof(double), hipHostMallocDefault) != hipSuccess) rc = BCG_ERR_HIP;
-    for (int i = 0; i < 4 && !rc; i++)
+    for (int i = 0; i < 5 && !rc; i++)
         if (hipEventCreate(&c->ev[i]) != hipSuccess) rc = BCG_ERR_HIP;
     if (!rc) {
         // c = c_res everywhere, no uptake
@@ -545,7 +590,7 @@ int bcg_rd3d_destroy(bcg_rd3d* c) {
     for (void* q : ptrs)
         if (q) hipFree(q);
     if (c->hnorm) hipHostFree(c->hnorm);
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < 5; i++)
         if (c->ev[i]) hipEventDestroy(c->ev[i]);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -595,9 +640,9 @@ int bcg_rd3d_get_uptake(bcg_rd3d* c, double* q) {
     return BCG_OK;
 }
 
-int bcg_rd3d_deposit(bcg_rd3d* c, const double* cm, const double* rate, int32_t n, int32_t* n_inside) {
-    if (!c || n < 0 || (n > 0 && (!cm || !rate))) return BCG_ERR_ARG;
-    Level& v = c->L[0];
+namespace {
+// Room for n sources in the deposition scratch.
+int dep_reserve(bcg_rd3d* c, int n) {
     const long nn = c->N;
     if (n > c->dcap) {
         for (void* q : {(void*)c->dkey, (void*)c->dsorted, (void*)c->dfrac, (void*)c->dcm, (void*)c->drate})
@@ -627,10 +672,13 @@ int bcg_rd3d_deposit(bcg_rd3d* c, const double* cm, const double* rate, int32_t 
         A(&c->scan_tot, 1024 * 64);
         if (rc) return rc;
     }
-    if (n > 0) {
-        HIPCHK(c, hipMemcpyAsync(c->dcm, cm, 3 * sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->drate, rate, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    }
+    return 0;
+}
+
+// Deposition of the n sources in c->dcm, c->drate into q (stream-ordered).
+int dep_run(bcg_rd3d* c, int n, int32_t* n_inside) {
+    Level& v = c->L[0];
+    const long nn = c->N;
     HIPCHK(c, hipMemsetAsync(c->dcnt, 0, sizeof(int) * (nn + 1), c->stream));
     HIPCHK(c, hipMemsetAsync(c->dfill, 0, sizeof(int) * nn, c->stream));
     if (n > 0)
@@ -658,7 +706,51 @@ int bcg_rd3d_deposit(bcg_rd3d* c, const double* cm, const double* rate, int32_t 
         HIPCHK(c, hipStreamSynchronize(c->stream));
         *n_inside = tot;
     }
+    return 0;
+}
+}  // namespace
+
+int bcg_rd3d_deposit(bcg_rd3d* c, const double* cm, const double* rate, int32_t n, int32_t* n_inside) {
+    if (!c || n < 0 || (n > 0 && (!cm || !rate))) return BCG_ERR_ARG;
+    int rc = dep_reserve(c, n);
+    if (rc) return rc;
+    if (n > 0) {
+        HIPCHK(c, hipMemcpyAsync(c->dcm, cm, 3 * sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->drate, rate, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    }
+    if ((rc = dep_run(c, n, n_inside))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return BCG_OK;
+}
+
+int bcg_rd3d_deposit_cells(bcg_rd3d* c, bcg_ctx* cells, double rate, int32_t* n_inside) {
+    if (!c || !cells) return BCG_ERR_ARG;
+    const int n = cells->N;
+    int rc = dep_reserve(c, n);
+    if (rc) return rc;
+    // after the mechanics context's queued work (its current cells)
+    HIPCHK(c, hipEventRecord(c->ev[4], cells->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev[4], 0));
+    if (n > 0)
+        hipLaunchKernelGGL(k_cells_src, dim3(nblk(n, 256)), dim3(256), 0, c->stream, cells->hot[cells->cur], n, rate,
+                           c->dcm, c->drate);
+    return dep_run(c, n, n_inside);
+}
+
+int bcg_rd3d_growth_to_cells(bcg_rd3d* c, bcg_ctx* cells, double mu, double K) {
+    if (!c || !cells || !(K > 0)) return BCG_ERR_ARG;
+    const bcg::PhysParams& P = cells->P;
+    const Level& v = c->L[0];
+    HIPCHK(c, hipEventRecord(c->ev[4], cells->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev[4], 0));
+    const long nb = (long)P.BoxX * P.BoxY * P.BoxZ;
+    hipLaunchKernelGGL(k_growth_bins, dim3(nblk(nb, 256)), dim3(256), 0, c->stream, c->x[c->cur], v.nx, v.ny, v.nz,
+                       c->prm.origin[0], c->prm.origin[1], c->prm.origin[2], 1.0 / c->prm.h, P.BoxX, P.BoxY, P.BoxZ,
+                       P.BoxLength, mu, K, cells->gr1);
+    HIPCHK(c, hipGetLastError());
+    // the context's next work comes after the new growth rates
+    HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+    HIPCHK(c, hipStreamWaitEvent(cells->stream, c->ev[4], 0));
     return BCG_OK;
 }
 

@@ -113,6 +113,21 @@ class Nutrients3D:
         )
         return inside.value
 
+    def deposit_cells(self, colony, rate: float, count: bool = False):
+        """Uptake density from the cells of `colony` (a simulation.Colony on the
+        same device), each depositing rate * Length at its centre; no host
+        copies. Returns the number of cells inside the grid when `count`
+        (which synchronises), else None."""
+        inside = ctypes.c_int32()
+        self._check(lib().bcg_rd3d_deposit_cells(self._h, colony._h, float(rate), ctypes.byref(inside) if count else None))
+        return inside.value if count else None
+
+    def growth_to_cells(self, colony, mu: float, K: float) -> None:
+        """Growth rate of every bin of `colony` = mu c / (K + c) at the bin
+        centre, from the current field (the role of the reference's growth-rate
+        lookup); ordered before the colony's next step."""
+        self._check(lib().bcg_rd3d_growth_to_cells(self._h, colony._h, float(mu), float(K)))
+
     def step(self, mode: str = "mixed", tol: float = 1e-10, max_iter: int = 100, strict: bool = True):
         """One implicit step; returns (iterations, ||b - A c|| / ||b||).
         strict=False keeps the last iterate when tol is not reached (for

@@ -249,6 +249,20 @@ int bcg_rd3d_adjoint(bcg_rd3d* ctx, const double* g, int32_t mode, double tol, i
                      double* dL_dq, double* dL_dcold, double* dL_dK, double* dL_dD);
 /* Transpose of bcg_rd3d_deposit: out[i] = sum_v field[v] w_iv / h^3.     */
 int bcg_rd3d_gather(bcg_rd3d* ctx, const double* cm, int32_t n, const double* field, double* out);
+/* The 3-D step coupled to a mechanics context on the same device, without
+ * host copies. No counterpart in the reference, whose nutrient model is
+ * planar; these take the roles of GetDensity (ABM/Compute.cpp:120-253) and
+ * of the growth-rate lookup of grow (ABM/grow.cpp:17-30):
+ *   bcg_rd3d_deposit_cells: q from the context's current cells, each
+ *     depositing rate * Length at its centre (cloud-in-cell, as
+ *     bcg_rd3d_deposit); *n_inside may be NULL (non-NULL synchronises);
+ *   bcg_rd3d_growth_to_cells: the context's growth rate of every bin
+ *     (GrowthRate1, before the reference's factor 1.5850) = mu c / (K + c),
+ *     c the current field at the bin centre (trilinear, clamped to the grid).
+ * Each runs after the context's queued work; the context's next work runs
+ * after bcg_rd3d_growth_to_cells.                                          */
+int bcg_rd3d_deposit_cells(bcg_rd3d* ctx, bcg_ctx* cells, double rate, int32_t* n_inside);
+int bcg_rd3d_growth_to_cells(bcg_rd3d* ctx, bcg_ctx* cells, double mu, double K);
 /* One V-cycle of the current step's system (timing); *ms = its duration.  */
 int bcg_rd3d_vcycle(bcg_rd3d* ctx, int32_t mode, float* ms);
 /* device times of the last step, ms: [0] setup (a0, rhs, coarse a0), [1] solve */

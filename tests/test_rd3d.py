@@ -363,3 +363,61 @@ def test_rollout_checkpointed_equals_direct_chain():
     assert Kt.grad.item() == pytest.approx(dK_tot, rel=1e-7)
     assert Dt.grad.item() == pytest.approx(dD_tot, rel=1e-7)
     assert np.abs(rates.grad.numpy() - drates_ref).max() <= 1e-7 * np.abs(drates_ref).max()
+
+
+@pytest.mark.gpu
+def test_coupling_to_a_mechanics_context():
+    """bcg_rd3d_deposit_cells equals bcg_rd3d_deposit of the same sources
+    (centre (p + q) / 2, rate * Length), bitwise; bcg_rd3d_growth_to_cells
+    gives mu c / (K + c) at every bin centre (trilinear in the field, clamped
+    to the grid); coupled steps (mechanics, deposition, one FP64 V-cycle,
+    growth rates) run on one device without host copies of the cells."""
+    from baccolonygrowth_amd.colony import dome_colony, dome_params
+    from baccolonygrowth_amd.nutrients import Nutrients3D, default_rd3d_params
+    from baccolonygrowth_amd.simulation import Colony
+
+    print("[gpu] rd3d coupled to a 20000-cell colony", flush=True)
+    cells = dome_colony(20000, radius=110.0, height=12.0, seed=5)
+    p = dome_params(cells, bin_size=5.0, max_neighbours=96, extra_cells=20000)
+    nz, ny, nx, h = 17, 65, 65, 2.0
+    origin = (-64.0, -64.0, 0.0)
+    prm = default_rd3d_params(nx=nx, ny=ny, nz=nz, h=h, D=600.0, dt=10.0, K=0.1, c_res=20.0, origin=origin)
+    rate, mu, K = 0.05, 1.2, 0.1
+    with Colony(p) as col, Nutrients3D(prm) as s:
+        col.LoadCells(cells)
+        inside = s.deposit_cells(col, rate, count=True)
+        q_dev = s.uptake()
+        cm = (cells["p"] + cells["q"]) * 0.5
+        assert s.deposit(cm, rate * cells["Length"]) == inside and 0 < inside < len(cells)
+        assert q_dev.tobytes() == s.uptake().tobytes()
+
+        f = 20.0 * np.random.default_rng(1).random((nz, ny, nx))
+        s.set_field(f)
+        s.growth_to_cells(col, mu, K)
+        gr1 = col.fields()[3]
+        b = np.arange(p.BoxX * p.BoxY * p.BoxZ)
+        xb, yb, zb = b // (p.BoxY * p.BoxZ), (b // p.BoxZ) % p.BoxY, b % p.BoxZ
+        pos = [(xb - p.BoxX // 2 + 0.5) * p.BoxLength, (yb - p.BoxY // 2) * p.BoxLength, (zb + 0.5) * p.BoxLength]
+        idx, w = [], []
+        for g, n in zip([(pos[a] - origin[a]) / h for a in range(3)], (nx, ny, nz)):
+            g = np.clip(g, 0.0, n - 1)
+            i0 = np.minimum(g.astype(int), n - 2)
+            idx.append(i0)
+            w.append(g - i0)
+        (i, j, k), (wx, wy, wz) = idx, w
+        c = sum(f[k + dk, j + dj, i + di] * (wx if di else 1 - wx) * (wy if dj else 1 - wy) * (wz if dk else 1 - wz)
+                for di in (0, 1) for dj in (0, 1) for dk in (0, 1))
+        c = np.maximum(c, 0.0)
+        expect = mu * c / (K + c)
+        assert np.abs(gr1 - expect).max() <= 1e-12 * expect.max()
+
+        s.set_field(np.full((nz, ny, nx), 20.0))
+        n0 = len(col.cells())
+        for step in range(3):
+            col.step(1.0 / 3600, step / 3600, update=True)
+            s.deposit_cells(col, rate)
+            s.vcycle("fp64")
+            s.growth_to_cells(col, mu, K)
+        field = s.field()
+        assert len(col.cells()) >= n0 and np.isfinite(field).all() and field.min() < 20.0
+        assert np.isfinite(col.fields()[3]).all() and col.fields()[3].min() < mu * 20.0 / (K + 20.0)
