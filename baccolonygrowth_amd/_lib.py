@@ -93,6 +93,8 @@ SIGNATURES.update(
         "bcg_rd3d_set_coeffs": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double]),
         "bcg_rd3d_deposit_cells": (ctypes.c_int, [_vp, _vp, ctypes.c_double, _ip]),
         "bcg_rd3d_growth_to_cells": (ctypes.c_int, [_vp, _vp, ctypes.c_double, ctypes.c_double]),
+        "bcg_rd3d_update_cells": (ctypes.c_int, [_vp, _vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                 ctypes.c_int32]),
         "bcg_rd3d_adjoint": (
             ctypes.c_int,
             [_vp, _dp, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, _dp, _dp, _dp, _dp],

@@ -192,6 +192,13 @@ struct bcg_rd3d {
     // the fine-level passes (residual, residual norm, refinement update),
     // k_zm_prolong and k_zm_restrict (FP64, FP32)
     long res_rb[3] = {1, 1, 1}, res_zm = 1, res_zp[2] = {1, 1}, res_zr[2] = {1, 1};
+    // bcg_rd3d_update_cells: its fixed-shape part (set-up, V-cycles, growth
+    // rates) recorded as a HIP graph for one (mechanics context, V-cycle
+    // count) and replayed
+    hipGraphExec_t upd_graph = nullptr;
+    const bcg_ctx* upd_cells = nullptr;
+    int upd_vcycles = 0, upd_cur = -1;
+    double upd_mu = 0, upd_K = 0;
     std::string err;
 };
 
@@ -397,6 +404,12 @@ void vcycle(bcg_rd3d* c, int l, bool zero = false) {
     smooth<T>(c, v, c->prm.nu2);
 }
 
+void drop_graphs(bcg_rd3d* c) {
+    if (c->upd_graph) hipGraphExecDestroy(c->upd_graph);
+    c->upd_graph = nullptr;
+    c->upd_cells = nullptr;
+}
+
 int sum_partials(bcg_rd3d* c, const double* part, long nparts) {
     // fixed two-stage tree: 256 chunk sums, then one block; same order every call
     hipLaunchKernelGGL(k_sum_chunks, dim3(256), dim3(256), 0, c->stream, part, nparts, c->part2);
@@ -575,6 +588,7 @@ int bcg_rd3d_create(const bcg_rd3d_params* p, bcg_rd3d** out) {
 int bcg_rd3d_destroy(bcg_rd3d* c) {
     if (!c) return BCG_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
+    drop_graphs(c);
     std::vector<void*> ptrs = {c->x[0], c->x[1], c->cprev, c->b, c->q, c->save, c->grad, c->gpart, c->dout, c->dfield,
                                c->part, c->part2, c->bot_u0, c->bot_f0, c->dnorm, c->dkey, c->dcnt,
                                c->dstart, c->dfill, c->dsorted, c->scan_tot, c->dfrac, c->dcm, c->drate};
@@ -641,10 +655,13 @@ int bcg_rd3d_get_uptake(bcg_rd3d* c, double* q) {
 }
 
 namespace {
-// Room for n sources in the deposition scratch.
-int dep_reserve(bcg_rd3d* c, int n) {
+// Room for n sources in the deposition scratch, allocated for `want` >= n
+// (reallocation synchronises the device: a colony's scratch is sized for its
+// maxCells once).
+int dep_reserve(bcg_rd3d* c, int n, int want) {
     const long nn = c->N;
     if (n > c->dcap) {
+        n = want > n ? want : n;
         for (void* q : {(void*)c->dkey, (void*)c->dsorted, (void*)c->dfrac, (void*)c->dcm, (void*)c->drate})
             if (q) hipFree(q);
         c->dkey = c->dsorted = nullptr;
@@ -712,7 +729,7 @@ int dep_run(bcg_rd3d* c, int n, int32_t* n_inside) {
 
 int bcg_rd3d_deposit(bcg_rd3d* c, const double* cm, const double* rate, int32_t n, int32_t* n_inside) {
     if (!c || n < 0 || (n > 0 && (!cm || !rate))) return BCG_ERR_ARG;
-    int rc = dep_reserve(c, n);
+    int rc = dep_reserve(c, n, n);
     if (rc) return rc;
     if (n > 0) {
         HIPCHK(c, hipMemcpyAsync(c->dcm, cm, 3 * sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
@@ -726,7 +743,7 @@ int bcg_rd3d_deposit(bcg_rd3d* c, const double* cm, const double* rate, int32_t 
 int bcg_rd3d_deposit_cells(bcg_rd3d* c, bcg_ctx* cells, double rate, int32_t* n_inside) {
     if (!c || !cells) return BCG_ERR_ARG;
     const int n = cells->N;
-    int rc = dep_reserve(c, n);
+    int rc = dep_reserve(c, n, cells->prm.maxCells);
     if (rc) return rc;
     // after the mechanics context's queued work (its current cells)
     HIPCHK(c, hipEventRecord(c->ev[4], cells->stream));
@@ -735,6 +752,62 @@ int bcg_rd3d_deposit_cells(bcg_rd3d* c, bcg_ctx* cells, double rate, int32_t* n_
         hipLaunchKernelGGL(k_cells_src, dim3(nblk(n, 256)), dim3(256), 0, c->stream, cells->hot[cells->cur], n, rate,
                            c->dcm, c->drate);
     return dep_run(c, n, n_inside);
+}
+
+int bcg_rd3d_update_cells(bcg_rd3d* c, bcg_ctx* cells, double rate, double mu, double K, int32_t vcycles) {
+    if (!c || !cells || !(K > 0) || vcycles < 1) return BCG_ERR_ARG;
+    const int n = cells->N;
+    int rc = dep_reserve(c, n, cells->prm.maxCells);
+    if (rc) return rc;
+    // after the mechanics context's queued work: deposition from its cells
+    HIPCHK(c, hipEventRecord(c->ev[4], cells->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev[4], 0));
+    if (n > 0)
+        hipLaunchKernelGGL(k_cells_src, dim3(nblk(n, 256)), dim3(256), 0, c->stream, cells->hot[cells->cur], n, rate,
+                           c->dcm, c->drate);
+    if ((rc = dep_run(c, n, nullptr))) return rc;
+    // the implicit step from the current field and the growth rates: kernel
+    // launches of fixed shape, replayed from a graph
+    Level& f = c->L[0];
+    double* x = c->x[c->cur];
+    const bcg::PhysParams& P = cells->P;
+    auto body = [&]() {
+        hipLaunchKernelGGL((k_cast<double, double>), dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, x, c->cprev, c->N);
+        setup_system(c);
+        f.u64 = x;
+        for (int i = 0; i < vcycles; i++) vcycle<double>(c, 0);
+        f.u64 = nullptr;
+        const long nb = (long)P.BoxX * P.BoxY * P.BoxZ;
+        hipLaunchKernelGGL(k_growth_bins, dim3(nblk(nb, 256)), dim3(256), 0, c->stream, x, f.nx, f.ny, f.nz,
+                           c->prm.origin[0], c->prm.origin[1], c->prm.origin[2], 1.0 / c->prm.h, P.BoxX, P.BoxY,
+                           P.BoxZ, P.BoxLength, mu, K, cells->gr1);
+    };
+    if (c->upd_graph && (c->upd_cells != cells || c->upd_vcycles != vcycles || c->upd_cur != c->cur ||
+                         c->upd_mu != mu || c->upd_K != K))
+        drop_graphs(c);
+    if (!c->upd_graph) {
+        hipGraph_t g = nullptr;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        body();
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (e != hipSuccess) return fail(c, BCG_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+        const hipError_t e2 = hipGraphInstantiate(&c->upd_graph, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e2 != hipSuccess) {
+            c->upd_graph = nullptr;
+            return fail(c, BCG_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e2));
+        }
+        c->upd_cells = cells;
+        c->upd_vcycles = vcycles;
+        c->upd_cur = c->cur;
+        c->upd_mu = mu;
+        c->upd_K = K;
+    }
+    HIPCHK(c, hipGraphLaunch(c->upd_graph, c->stream));
+    // the context's next work comes after the new growth rates
+    HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+    HIPCHK(c, hipStreamWaitEvent(cells->stream, c->ev[4], 0));
+    return BCG_OK;
 }
 
 int bcg_rd3d_growth_to_cells(bcg_rd3d* c, bcg_ctx* cells, double mu, double K) {
@@ -867,6 +940,7 @@ int bcg_rd3d_set_coeffs(bcg_rd3d* c, double D, double K) {
     if (!c || !(D >= 0) || !(K > 0)) return BCG_ERR_ARG;
     c->prm.D = D;
     c->prm.K = K;
+    drop_graphs(c);  // a recorded update holds the old coefficients
     for (Level& v : c->L) {
         v.lx = c->prm.dt * D / (v.hx * v.hx);
         v.ly = c->prm.dt * D / (v.hy * v.hy);

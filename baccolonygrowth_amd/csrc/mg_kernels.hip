@@ -821,6 +821,7 @@ template __global__ void k_fill<float>(float*, long, float);
 template __global__ void k_fill<double>(double*, long, double);
 template __global__ void k_cast<float, double>(const float*, double*, long);
 template __global__ void k_cast<double, float>(const double*, float*, long);
+template __global__ void k_cast<double, double>(const double*, double*, long);
 template __global__ void k_sumsq<float>(const float*, long, double*);
 template __global__ void k_sumsq<double>(const double*, long, double*);
 template __global__ void k_restrict_a0<float>(const float*, float*, int, int, int, int, int, int, int, int, int);

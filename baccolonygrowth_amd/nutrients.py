@@ -128,6 +128,12 @@ class Nutrients3D:
         lookup); ordered before the colony's next step."""
         self._check(lib().bcg_rd3d_growth_to_cells(self._h, colony._h, float(mu), float(K)))
 
+    def update_cells(self, colony, rate: float, mu: float, K: float, vcycles: int = 1) -> None:
+        """One coupled update without host synchronisation: deposition from the
+        cells of `colony`, the implicit step by `vcycles` FP64 V-cycles, the
+        growth rates of its bins (a replayed HIP graph for the fixed part)."""
+        self._check(lib().bcg_rd3d_update_cells(self._h, colony._h, float(rate), float(mu), float(K), int(vcycles)))
+
     def step(self, mode: str = "mixed", tol: float = 1e-10, max_iter: int = 100, strict: bool = True):
         """One implicit step; returns (iterations, ||b - A c|| / ||b||).
         strict=False keeps the last iterate when tol is not reached (for

@@ -64,6 +64,14 @@ PHASE_KERNEL = {"bins": "k_bin_count", "lists": "k_gather_all", "move_grow": "k_
 PROFILE = os.path.join(ROOT, "profiles", "bench_default_kernels.txt")
 
 
+def device_sync() -> None:
+    """Wait for all work queued on the device (every stream)."""
+    import ctypes
+
+    if ctypes.CDLL("libamdhip64.so").hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipDeviceSynchronize failed")
+
+
 def measured_traffic(kernel: str):
     """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by the
     copy calibration) from the rocprofv3 summary of this workload in
@@ -121,7 +129,8 @@ def cpu_baseline(p, cells, fields, steps: int) -> dict:
         "unit": "cell-updates/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"the same {n0}-cell dome, {steps} full steps incl. list rebuild, serial C oracle ({el:.1f} s)",
+        "sample": f"the same {n0}-cell dome, {steps} full mechanics steps incl. list rebuild (without the nutrient "
+        f"V-cycle), serial C oracle ({el:.1f} s)",
     }
 
 
@@ -131,6 +140,11 @@ def cpu_baseline(p, cells, fields, steps: int) -> dict:
 MG_SHAPE = (129, 513, 513)
 MG_H = 1.0
 Q_CELL = 1.66  # uptake per cell, mM um^3 / s (about 1e6 glucose molecules/s)
+# nutrient grid of the coupled step: 257 x 257 x 65 vertices at 2 um, 6 levels,
+# centred on the colony (covers |x|, |y| <= 256 um of the 600 um dome)
+STEP_SHAPE = (65, 257, 257)
+STEP_H = 2.0
+STEP_LEVELS = 6
 
 
 def vcycle_words(levels, nu1: int, nu2: int) -> float:
@@ -294,6 +308,7 @@ def main():
         print(json.dumps(mg_bench()), flush=True)
         return
 
+    from baccolonygrowth_amd.nutrients import Nutrients3D, default_rd3d_params
     from baccolonygrowth_amd.simulation import Colony
 
     t0 = time.time()
@@ -303,11 +318,30 @@ def main():
     col.LoadCells(cells)
     col.set_fields(*fields)
     col.srand(2)
+    # the coupled step: mechanics (list rebuild every step), then the Monod
+    # sink deposited from the cells, one FP64 V-cycle of the implicit
+    # nutrient step (dt = the mechanics step) and the cells' growth rates
+    # from the new field, all on the device
+    nz, ny, nx = STEP_SHAPE
+    half = (nx - 1) * STEP_H / 2
+    nut = Nutrients3D(default_rd3d_params(nx=nx, ny=ny, nz=nz, h=STEP_H, D=600.0, dt=DT_H * 3600.0, K=0.1,
+                                          c_res=20.0, origin=(-half, -half, 0.0), max_levels=STEP_LEVELS))
+    rate = Q_CELL / 3.0  # per um of cell length (mean length 3 um)
+    inside = nut.deposit_cells(col, rate, count=True)
+
+    def coupled_step(t):
+        # the nutrient update is queued behind the mechanics and the next
+        # mechanics step behind it (no host synchronisation between them)
+        nd = len(col.step(DT_H, t, update=True, no_patch=True))
+        nut.update_cells(col, rate, GR_BASE, 0.1, 1)
+        return nd
+
     t = 0.0
     for _ in range(args.warmup):
-        col.step(DT_H, t, update=True, no_patch=True)
+        coupled_step(t)
         t += DT_H
     log(f"[bench] warmup done ({args.warmup} steps)")
+    device_sync()
     phases = {k: 0.0 for k in ("bins", "lists", "move_grow", "compact", "divide")}
     cell_updates = 0
     maxlist = 0
@@ -315,15 +349,18 @@ def main():
     ts = time.perf_counter()
     for _ in range(args.steps):
         cell_updates += col.n
-        ndiv += len(col.step(DT_H, t, update=True, no_patch=True))
+        ndiv += coupled_step(t)
         t += DT_H
         for k, v in col.last_step_times().items():
             phases[k] += v
         maxlist = max(maxlist, col.last_step_stats()["max_list"])
+    device_sync()  # the last nutrient update is still queued
     el = time.perf_counter() - ts
+    vcycle_ms = float(np.median([nut.vcycle("fp64") for _ in range(5)]))  # reported, not part of the timed steps
     counts, _ = col.neighbours(stride=1)
     kbar = float(counts.mean())
     n_end = col.n
+    nut.close()
     col.close()
     value = cell_updates / el
     ms_step = 1000.0 * el / args.steps
@@ -332,7 +369,8 @@ def main():
     nmean = cell_updates / args.steps
     bpc = phase_bytes(kbar)
     achieved = bpc[dom] * nmean / (avg[dom] * 1e-3) / 1e9
-    log(f"[bench] {args.steps} steps: {ms_step:.3f} ms/step, phases(ms) " + ", ".join(f"{k}={v:.3f}" for k, v in avg.items()))
+    log(f"[bench] {args.steps} steps: {ms_step:.3f} ms/step, phases(ms) " + ", ".join(f"{k}={v:.3f}" for k, v in avg.items())
+        + f"; one FP64 V-cycle of the nutrient step alone {vcycle_ms:.3f}")
     cpu = None
     if not args.no_cpu:
         log(f"[bench] CPU baseline: {args.cpu_steps} oracle steps ...")
@@ -360,7 +398,12 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "mechanics step, 2^20-cell 3-D dome (r=600 um, h=20 um), 5 um bins, list rebuild every step",
+            "workload": "coupled step: mechanics of a 2^20-cell 3-D dome (r=600 um, h=20 um, 5 um bins, list "
+            "rebuild every step) + Monod sink deposited from the cells + one FP64 V-cycle of the implicit nutrient "
+            "step on 257x257x65 vertices at 2 um (6 levels) + growth rates from the field",
+            "nutrient_grid": [nx, ny, nz],
+            "cells_in_nutrient_grid": int(inside),
+            "nutrient_vcycle_ms": vcycle_ms,
             "cells": int(args.cells),
             "cells_end": int(n_end),
             "divisions": int(ndiv),

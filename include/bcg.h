@@ -263,6 +263,13 @@ int bcg_rd3d_gather(bcg_rd3d* ctx, const double* cm, int32_t n, const double* fi
  * after bcg_rd3d_growth_to_cells.                                          */
 int bcg_rd3d_deposit_cells(bcg_rd3d* ctx, bcg_ctx* cells, double rate, int32_t* n_inside);
 int bcg_rd3d_growth_to_cells(bcg_rd3d* ctx, bcg_ctx* cells, double mu, double K);
+/* One coupled nutrient update without host synchronisation:
+ * bcg_rd3d_deposit_cells, then the implicit step from the current field by
+ * `vcycles` FP64 V-cycles (no convergence test), then
+ * bcg_rd3d_growth_to_cells. The fixed-shape part (set-up, V-cycles, growth
+ * rates) is recorded as a HIP graph on first use for this (cells, vcycles,
+ * mu, K) and replayed; the results equal those of the separate calls.     */
+int bcg_rd3d_update_cells(bcg_rd3d* ctx, bcg_ctx* cells, double rate, double mu, double K, int32_t vcycles);
 /* One V-cycle of the current step's system (timing); *ms = its duration.  */
 int bcg_rd3d_vcycle(bcg_rd3d* ctx, int32_t mode, float* ms);
 /* device times of the last step, ms: [0] setup (a0, rhs, coarse a0), [1] solve */

@@ -421,3 +421,37 @@ def test_coupling_to_a_mechanics_context():
         field = s.field()
         assert len(col.cells()) >= n0 and np.isfinite(field).all() and field.min() < 20.0
         assert np.isfinite(col.fields()[3]).all() and col.fields()[3].min() < mu * 20.0 / (K + 20.0)
+
+
+@pytest.mark.gpu
+def test_coupled_update_graph_equals_separate_calls():
+    """bcg_rd3d_update_cells (deposition, then set-up, one FP64 V-cycle and
+    growth rates replayed from a recorded HIP graph) gives the field, the
+    growth rates and the cells of the separate calls (deposit_cells, vcycle,
+    growth_to_cells), bitwise, over three coupled steps with mechanics steps
+    in between; mu changes on the third (the graph is recorded again)."""
+    from baccolonygrowth_amd.colony import dome_colony, dome_params
+    from baccolonygrowth_amd.nutrients import Nutrients3D, default_rd3d_params
+    from baccolonygrowth_amd.simulation import Colony
+
+    print("[gpu] rd3d coupled update: graph replay vs separate calls", flush=True)
+    cells = dome_colony(20000, radius=110.0, height=12.0, seed=6)
+    p = dome_params(cells, bin_size=5.0, max_neighbours=96, extra_cells=20000)
+    nz, ny, nx, h = 17, 65, 65, 2.0
+    prm = default_rd3d_params(nx=nx, ny=ny, nz=nz, h=h, D=600.0, dt=1.0, K=0.1, c_res=20.0, origin=(-64.0, -64.0, 0.0))
+    rate, K = 0.5, 0.1
+    with Colony(p) as ca, Colony(p) as cb, Nutrients3D(prm) as sa, Nutrients3D(prm) as sb:
+        for col in (ca, cb):
+            col.LoadCells(cells)
+        for step in range(3):
+            mu = 1.2 if step < 2 else 0.9
+            for col in (ca, cb):
+                col.step(1.0 / 3600, step / 3600, update=True)
+            sa.update_cells(ca, rate, mu, K, 1)
+            sb.deposit_cells(cb, rate)
+            sb.vcycle("fp64")
+            sb.growth_to_cells(cb, mu, K)
+            fa, fb = sa.field(), sb.field()
+            assert fa.tobytes() == fb.tobytes() and fa.min() < 20.0
+            assert ca.fields()[3].tobytes() == cb.fields()[3].tobytes()
+        assert ca.cells().tobytes() == cb.cells().tobytes()
