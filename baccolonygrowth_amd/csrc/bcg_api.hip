@@ -155,6 +155,7 @@ int set_identity_slots(bcg_ctx* c) {
 }  // namespace
 
 int bcg_bin_current(bcg_ctx* c) {
+    HIPCHK(c, ctx_join(c));
     int rc = reset_stats(c);
     if (!rc) rc = build_bins(c, c->cur, c->N);
     if (!rc) rc = read_stats(c);
@@ -261,6 +262,7 @@ int bcg_create(const bcg_params* p, bcg_ctx** out) {
         rc = BCG_ERR_HIP;
     for (int i = 0; i < 7 && !rc; i++)
         if (hipEventCreate(&c->ev[i]) != hipSuccess) rc = BCG_ERR_HIP;
+    if (!rc && hipEventCreateWithFlags(&c->after, hipEventDisableTiming) != hipSuccess) rc = BCG_ERR_HIP;
     if (rc) {
         bcg_destroy(c);
         return rc;
@@ -289,7 +291,10 @@ int bcg_create(const bcg_params* p, bcg_ctx** out) {
 
 int bcg_destroy(bcg_ctx* c) {
     if (!c) return BCG_OK;
-    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->stream) {
+        ctx_join(c);
+        hipStreamSynchronize(c->stream);
+    }
     void* ptrs[] = {c->hot[0],   c->hot[1],    c->time_p,  c->time_q,     c->dynfric,   c->stafric,  c->age_p,
                     c->age_q,    c->type,      c->ancestor, c->key,       c->bin_cnt,   c->bin_start, c->bin_slot,
                     c->sorted,   c->scattered, c->scan_tot, c->cmL, c->cand,  c->nl,      c->nl_count,   c->order,     c->slot_of,  c->divflag,
@@ -300,6 +305,7 @@ int bcg_destroy(bcg_ctx* c) {
     if (c->hstats) hipHostFree(c->hstats);
     for (int i = 0; i < 7; i++)
         if (c->ev[i]) hipEventDestroy(c->ev[i]);
+    if (c->after) hipEventDestroy(c->after);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return BCG_OK;
@@ -314,6 +320,8 @@ int bcg_last_error(const bcg_ctx* c, char* buf, int len) {
 int bcg_set_cells(bcg_ctx* c, const bcg_cell* cells, int32_t n) {
     if (!c || (n > 0 && !cells) || n < 0) return BCG_ERR_ARG;
     if (n > c->prm.maxCells) return fail(c, BCG_ERR_CAPACITY, "%d cells > maxCells %d", n, c->prm.maxCells);
+    HIPCHK(c, ctx_join(c));
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // the copies below are synchronous on the null stream
     std::vector<double> hot((size_t)n * HOT), tp(n), tq(n), df(3 * (size_t)n), sf(3 * (size_t)n);
     std::vector<int> ap(n), aq(n), ty(n), an(n);
     for (int i = 0; i < n; i++) {
@@ -367,6 +375,7 @@ int bcg_get_cells(bcg_ctx* c, bcg_cell* cells, int32_t cap, int32_t* n) {
     if (!c || !n) return BCG_ERR_ARG;
     *n = c->N;
     if (cap < c->N || (c->N > 0 && !cells)) return fail(c, BCG_ERR_ARG, "buffer of %d cells < %d", cap, c->N);
+    HIPCHK(c, ctx_join(c));
     const int N = c->N;
     if (N == 0) return BCG_OK;
     std::vector<double> hot((size_t)N * HOT), tp(N), tq(N), df(3 * (size_t)N), sf(3 * (size_t)N);
@@ -416,6 +425,8 @@ int bcg_num_cells(bcg_ctx* c, int32_t* n) {
 int bcg_set_fields(bcg_ctx* c, const double* height, const double* normal, const double* wall, const double* gr1,
                    const double* gr2) {
     if (!c) return BCG_ERR_ARG;
+    HIPCHK(c, ctx_join(c));
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // the copies below are synchronous on the null stream
     const size_t nh = (size_t)c->P.nhx * c->P.nhy;
     if (height) HIPCHK(c, hipMemcpy(c->height, height, nh * sizeof(double), hipMemcpyHostToDevice));
     if (normal) HIPCHK(c, hipMemcpy(c->normal, normal, 3 * nh * sizeof(double), hipMemcpyHostToDevice));
@@ -428,6 +439,7 @@ int bcg_set_fields(bcg_ctx* c, const double* height, const double* normal, const
 
 int bcg_get_fields(bcg_ctx* c, double* height, double* normal, double* wall, double* gr1, double* gr2) {
     if (!c) return BCG_ERR_ARG;
+    HIPCHK(c, ctx_join(c));
     const size_t nh = (size_t)c->P.nhx * c->P.nhy;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (height) HIPCHK(c, hipMemcpy(height, c->height, nh * sizeof(double), hipMemcpyDeviceToHost));
@@ -462,6 +474,7 @@ int bcg_rand_stream(uint32_t seed, uint64_t skip, int32_t n, int32_t* out) {
 
 int bcg_rebuild_neighbours(bcg_ctx* c, const int32_t* ids, int32_t n) {
     if (!c) return BCG_ERR_ARG;
+    HIPCHK(c, ctx_join(c));
     int rc;
     if ((rc = reset_stats(c))) return rc;
     if ((rc = build_bins(c, c->cur, c->N))) return rc;
@@ -490,6 +503,7 @@ int bcg_rebuild_neighbours(bcg_ctx* c, const int32_t* ids, int32_t n) {
 
 int bcg_get_neighbours(bcg_ctx* c, int32_t* counts, int32_t* lists, int32_t stride) {
     if (!c || !counts || (!lists && stride > 0) || stride < 0) return BCG_ERR_ARG;
+    HIPCHK(c, ctx_join(c));
     const int N = c->N;
     if (N == 0) return BCG_OK;
     std::vector<int> cnt(N), slot(N), nl((size_t)c->cap * N);
@@ -512,6 +526,7 @@ int bcg_get_neighbours(bcg_ctx* c, int32_t* counts, int32_t* lists, int32_t stri
 
 int bcg_sum_forces(bcg_ctx* c, double* F, double* T) {
     if (!c || !F || !T) return BCG_ERR_ARG;
+    HIPCHK(c, ctx_join(c));
     int rc;
     if ((rc = reset_stats(c))) return rc;
     if (c->N > 0)
@@ -546,6 +561,11 @@ int bcg_step(bcg_ctx* c, double dt, double t, int32_t flags, int32_t* n_div, int
         return fail(c, BCG_ERR_STATE, "no neighbour lists: rebuild before stepping");
     }
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    // the growth rates and the other cell buffer may still be in use by work
+    // queued on another stream (a nutrient update): bins and lists above did
+    // not touch them and ran alongside it
+    HIPCHK(c, ctx_join(c));
+    HIPCHK(c, hipEventRecord(c->ev[6], c->stream));
     // MoveCell + GrowCell for every cell (ABM/Simulation.cpp:330-335)
     if (N0 > 0)
         hipLaunchKernelGGL(k_move_grow, dim3(nblk(N0, MOVE_BLOCK)), dim3(MOVE_BLOCK), 0, c->stream, c->hot[in],
@@ -595,7 +615,7 @@ int bcg_step(bcg_ctx* c, double dt, double t, int32_t flags, int32_t* n_div, int
     HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
     if ((rc = read_stats(c))) return rc;
     if ((rc = check_err(c))) return rc;
-    for (int i = 0; i < 5; i++) hipEventElapsedTime(&c->last_ms[i], c->ev[i], c->ev[i + 1]);
+    for (int i = 0; i < 5; i++) hipEventElapsedTime(&c->last_ms[i], c->ev[i == 2 ? 6 : i], c->ev[i + 1]);
     if (lineage && D > 0) {
         std::vector<int> ids(D);
         HIPCHK(c, hipMemcpy(ids.data(), c->div_ids, D * sizeof(int), hipMemcpyDeviceToHost));

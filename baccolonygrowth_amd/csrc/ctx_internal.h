@@ -49,7 +49,19 @@ struct bcg_ctx {
     BcgStats last{};
     int last_flagged = 0;
     std::string err;
+    // work queued on another stream against the context (the 3-D nutrient
+    // update reads the current cells and writes gr1): `after` is recorded
+    // there and the context's stream waits for it before its next kernel
+    // that reads the growth rates or writes cells
+    hipEvent_t after = nullptr;
+    bool after_pending = false;
 };
+
+inline hipError_t ctx_join(bcg_ctx* c) {
+    if (!c->after_pending) return hipSuccess;
+    c->after_pending = false;
+    return hipStreamWaitEvent(c->stream, c->after, 0);
+}
 
 
 // Bin the context's current cells (counting sort, ascending IDs per bin);

@@ -226,7 +226,10 @@ int bcg_env_default_params(bcg_env_params* p) {
 
 int bcg_env_destroy(bcg_env* e) {
     if (!e) return BCG_OK;
-    if (e->mech && e->mech->stream) hipStreamSynchronize(e->mech->stream);
+    if (e->mech && e->mech->stream) {
+        ctx_join(e->mech);
+        hipStreamSynchronize(e->mech->stream);
+    }
     std::vector<void*> ptrs = {e->colony[0], e->colony[1], e->rough,   e->rough1,   e->rough2,   e->inside,
                                e->rwall,     e->rwall1,    e->rwall2,  e->rshift,   e->rshift1,  e->rshift2,
                                e->rwshift,   e->rwshift1,  e->rwshift2, e->dens,    e->dens1,    e->dens2,
@@ -519,6 +522,7 @@ int bcg_env_nutrients(bcg_env* e, int32_t* count) {
                      minX, maxX, maxH, e->BX, e->BZ);
     bcg_ctx* m = e->mech;
     hipStream_t st = m->stream;
+    EHIP(e, ctx_join(m));  // writes the context's growth rates
     EnvArgs a;
     memset(&a, 0, sizeof a);
     a.BX = e->BX;

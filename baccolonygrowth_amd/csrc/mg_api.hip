@@ -655,6 +655,24 @@ int bcg_rd3d_get_uptake(bcg_rd3d* c, double* q) {
 }
 
 namespace {
+// This context's next work follows the mechanics context's queued work (and
+// whatever that context still waits for).
+hipError_t join_cells(bcg_rd3d* c, bcg_ctx* cells) {
+    hipError_t e = ctx_join(cells);
+    if (e == hipSuccess) e = hipEventRecord(c->ev[4], cells->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, c->ev[4], 0);
+    return e;
+}
+
+// The mechanics context's next kernel that reads the growth rates or writes
+// cells follows the work queued here so far; its bins and lists, which do
+// neither, run alongside it.
+hipError_t release_cells(bcg_rd3d* c, bcg_ctx* cells) {
+    hipError_t e = hipEventRecord(cells->after, c->stream);
+    if (e == hipSuccess) cells->after_pending = true;
+    return e;
+}
+
 // Room for n sources in the deposition scratch, allocated for `want` >= n
 // (reallocation synchronises the device: a colony's scratch is sized for its
 // maxCells once).
@@ -745,13 +763,13 @@ int bcg_rd3d_deposit_cells(bcg_rd3d* c, bcg_ctx* cells, double rate, int32_t* n_
     const int n = cells->N;
     int rc = dep_reserve(c, n, cells->prm.maxCells);
     if (rc) return rc;
-    // after the mechanics context's queued work (its current cells)
-    HIPCHK(c, hipEventRecord(c->ev[4], cells->stream));
-    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev[4], 0));
+    HIPCHK(c, join_cells(c, cells));
     if (n > 0)
         hipLaunchKernelGGL(k_cells_src, dim3(nblk(n, 256)), dim3(256), 0, c->stream, cells->hot[cells->cur], n, rate,
                            c->dcm, c->drate);
-    return dep_run(c, n, n_inside);
+    if ((rc = dep_run(c, n, n_inside))) return rc;
+    HIPCHK(c, release_cells(c, cells));
+    return BCG_OK;
 }
 
 int bcg_rd3d_update_cells(bcg_rd3d* c, bcg_ctx* cells, double rate, double mu, double K, int32_t vcycles) {
@@ -759,9 +777,7 @@ int bcg_rd3d_update_cells(bcg_rd3d* c, bcg_ctx* cells, double rate, double mu, d
     const int n = cells->N;
     int rc = dep_reserve(c, n, cells->prm.maxCells);
     if (rc) return rc;
-    // after the mechanics context's queued work: deposition from its cells
-    HIPCHK(c, hipEventRecord(c->ev[4], cells->stream));
-    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev[4], 0));
+    HIPCHK(c, join_cells(c, cells));
     if (n > 0)
         hipLaunchKernelGGL(k_cells_src, dim3(nblk(n, 256)), dim3(256), 0, c->stream, cells->hot[cells->cur], n, rate,
                            c->dcm, c->drate);
@@ -804,9 +820,7 @@ int bcg_rd3d_update_cells(bcg_rd3d* c, bcg_ctx* cells, double rate, double mu, d
         c->upd_K = K;
     }
     HIPCHK(c, hipGraphLaunch(c->upd_graph, c->stream));
-    // the context's next work comes after the new growth rates
-    HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
-    HIPCHK(c, hipStreamWaitEvent(cells->stream, c->ev[4], 0));
+    HIPCHK(c, release_cells(c, cells));
     return BCG_OK;
 }
 
@@ -814,16 +828,13 @@ int bcg_rd3d_growth_to_cells(bcg_rd3d* c, bcg_ctx* cells, double mu, double K) {
     if (!c || !cells || !(K > 0)) return BCG_ERR_ARG;
     const bcg::PhysParams& P = cells->P;
     const Level& v = c->L[0];
-    HIPCHK(c, hipEventRecord(c->ev[4], cells->stream));
-    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev[4], 0));
+    HIPCHK(c, join_cells(c, cells));
     const long nb = (long)P.BoxX * P.BoxY * P.BoxZ;
     hipLaunchKernelGGL(k_growth_bins, dim3(nblk(nb, 256)), dim3(256), 0, c->stream, c->x[c->cur], v.nx, v.ny, v.nz,
                        c->prm.origin[0], c->prm.origin[1], c->prm.origin[2], 1.0 / c->prm.h, P.BoxX, P.BoxY, P.BoxZ,
                        P.BoxLength, mu, K, cells->gr1);
     HIPCHK(c, hipGetLastError());
-    // the context's next work comes after the new growth rates
-    HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
-    HIPCHK(c, hipStreamWaitEvent(cells->stream, c->ev[4], 0));
+    HIPCHK(c, release_cells(c, cells));
     return BCG_OK;
 }
 

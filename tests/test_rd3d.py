@@ -455,3 +455,37 @@ def test_coupled_update_graph_equals_separate_calls():
             assert fa.tobytes() == fb.tobytes() and fa.min() < 20.0
             assert ca.fields()[3].tobytes() == cb.fields()[3].tobytes()
         assert ca.cells().tobytes() == cb.cells().tobytes()
+
+
+@pytest.mark.gpu
+def test_overlapped_coupled_steps_equal_serialised():
+    """Coupled steps queued back to back (each nutrient update runs alongside
+    the next mechanics step's bins and lists, and that step's move/grow waits
+    for the growth rates) give the cells, the growth rates and the field of
+    the same steps with the host waiting for every update, bitwise."""
+    from baccolonygrowth_amd.colony import dome_colony, dome_params
+    from baccolonygrowth_amd.nutrients import Nutrients3D, default_rd3d_params
+    from baccolonygrowth_amd.simulation import Colony
+
+    print("[gpu] rd3d coupled steps: overlapped vs serialised", flush=True)
+    cells = dome_colony(100000, radius=190.0, height=20.0, seed=7)
+    p = dome_params(cells, bin_size=5.0, max_neighbours=96, extra_cells=100000)
+    nz, ny, nx, h = 17, 129, 129, 4.0
+    prm = default_rd3d_params(nx=nx, ny=ny, nz=nz, h=h, D=600.0, dt=1.0, K=0.1, c_res=20.0,
+                              origin=(-256.0, -256.0, 0.0))
+    rate, mu, K = 0.5, 1.2, 0.1
+    with Colony(p) as ca, Colony(p) as cb, Nutrients3D(prm) as sa, Nutrients3D(prm) as sb:
+        for col in (ca, cb):
+            col.LoadCells(cells)
+        for step in range(6):
+            ca.step(1.0 / 3600, step / 3600, update=True)
+            sa.update_cells(ca, rate, mu, K, 2)
+        for step in range(6):
+            cb.step(1.0 / 3600, step / 3600, update=True)
+            sb.update_cells(cb, rate, mu, K, 2)
+            sb.field()  # the host waits for the update
+            cb.fields()
+        fa, fb = sa.field(), sb.field()
+        assert fa.tobytes() == fb.tobytes() and fa.min() < 20.0
+        assert ca.fields()[3].tobytes() == cb.fields()[3].tobytes()
+        assert len(ca.cells()) >= len(cells) and ca.cells().tobytes() == cb.cells().tobytes()
