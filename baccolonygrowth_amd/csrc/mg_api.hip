@@ -45,11 +45,40 @@ struct Level {
 
 // Deterministic cloud-in-cell deposition: cells sorted by base vertex, each
 // vertex gathers from its 8 base-vertex buckets in a fixed order.
-__global__ void k_dep_key(const double* __restrict__ cm, int n, double ox, double oy, double oz, double inv_h, int nx,
-                          int ny, int nz, int* __restrict__ key, double* __restrict__ frac, int* __restrict__ cnt) {
+// Sources given as centres (x, y, z per source) with their rates in `rate`.
+struct PointSources {
+    const double* cm;
+    __device__ void centre(int c, double& x, double& y, double& z, double*) const {
+        x = cm[3 * c];
+        y = cm[3 * c + 1];
+        z = cm[3 * c + 2];
+    }
+};
+
+// Sources of a mechanics context's cells: centre (p + q) / 2 and rate
+// `rate` x Length (hot record: p [0..2], q [3..5], Length [12]), written to
+// `out` for the gather.
+struct CellSources {
+    const double* hot;
+    double rate;
+    __device__ void centre(int c, double& x, double& y, double& z, double* out) const {
+        const double* h = hot + (size_t)c * bcg::HOT;
+        x = (h[0] + h[3]) * 0.5;
+        y = (h[1] + h[4]) * 0.5;
+        z = (h[2] + h[5]) * 0.5;
+        out[c] = rate * h[12];
+    }
+};
+
+template <class Src>
+__global__ void k_dep_key(Src src, int n, double ox, double oy, double oz, double inv_h, int nx, int ny, int nz,
+                          int* __restrict__ key, double* __restrict__ frac, int* __restrict__ cnt,
+                          double* __restrict__ rate_out) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
-    const double gx = (cm[3 * c] - ox) * inv_h, gy = (cm[3 * c + 1] - oy) * inv_h, gz = (cm[3 * c + 2] - oz) * inv_h;
+    double x, y, z;
+    src.centre(c, x, y, z, rate_out);
+    const double gx = (x - ox) * inv_h, gy = (y - oy) * inv_h, gz = (z - oz) * inv_h;
     const double fx = floor(gx), fy = floor(gy), fz = floor(gz);
     int k = -1;
     if (fx >= 0 && fx < nx - 1 && fy >= 0 && fy < ny - 1 && fz >= 0 && fz < nz - 1) {
@@ -86,47 +115,59 @@ __global__ void k_dep_sort(const int* __restrict__ start, long nn, int* __restri
     }
 }
 
+// Each thread walks one vertex column (i, j) up the planes [k0, k0 + kchunk):
+// vertex k sums its 8 base-vertex buckets (planes k, then k-1; rows j, then
+// j-1; columns i, then i-1; each bucket's cells in ascending order). The
+// bucket starts of plane k are kept for the vertex above, so each plane's
+// starts are loaded once.
+constexpr int DEP_KCHUNK = 8;
+
 __global__ void __launch_bounds__(256) k_dep_gather(const int* __restrict__ start, const int* __restrict__ sorted,
                                                     const double* __restrict__ frac, const double* __restrict__ rate,
                                                     int nx, int ny, int nz, double inv_vol, double* __restrict__ q) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
+    const int k0 = blockIdx.z * DEP_KCHUNK;
     if (i >= nx || j >= ny) return;
-    double acc = 0.0;
-    for (int dz = 0; dz <= 1; dz++) {
-        const int bk = k - dz;
-        if (bk < 0 || bk >= nz - 1) continue;
-        for (int dy = 0; dy <= 1; dy++) {
-            const int bj = j - dy;
-            if (bj < 0 || bj >= ny - 1) continue;
-            for (int dx = 0; dx <= 1; dx++) {
-                const int bi = i - dx;
-                if (bi < 0 || bi >= nx - 1) continue;
-                const long b = ((long)bk * ny + bj) * nx + bi;
-                for (int s = start[b]; s < start[b + 1]; s++) {
-                    const int c = sorted[s];
-                    const double fx = frac[3 * c], fy = frac[3 * c + 1], fz = frac[3 * c + 2];
-                    const double wx = dx ? fx : 1.0 - fx, wy = dy ? fy : 1.0 - fy, wz = dz ? fz : 1.0 - fz;
-                    acc += rate[c] * (wx * wy * wz);
+    const int k1 = k0 + DEP_KCHUNK < nz ? k0 + DEP_KCHUNK : nz;
+    // s[dy][0..2]: starts of buckets i-1, i, i+1 on row j-dy of a bucket
+    // plane (an exclusive scan: bucket b is [start[b], start[b+1]))
+    auto row_starts = [&](int bk, int dy, int* r) {
+        const int bj = j - dy;
+        if (bk < 0 || bk >= nz - 1 || bj < 0 || bj >= ny - 1) {
+            r[0] = r[1] = r[2] = 0;
+            return;
+        }
+        const long b = ((long)bk * ny + bj) * nx + i;
+        r[1] = start[b];
+        r[2] = start[b + 1];
+        r[0] = i > 0 ? start[b - 1] : r[1];
+    };
+    int lo[2][3], hi[2][3];  // bucket plane k-1 (lo) and k (hi)
+    row_starts(k0 - 1, 0, lo[0]);
+    row_starts(k0 - 1, 1, lo[1]);
+    for (int k = k0; k < k1; k++) {
+        row_starts(k, 0, hi[0]);
+        row_starts(k, 1, hi[1]);
+        double acc = 0.0;
+        for (int dz = 0; dz <= 1; dz++)
+            for (int dy = 0; dy <= 1; dy++) {
+                const int* r = dz ? lo[dy] : hi[dy];
+                for (int dx = 0; dx <= 1; dx++) {
+                    if (dx == 0 && i >= nx - 1) continue;  // bucket column i exists for i <= nx-2
+                    const int s0 = dx ? r[0] : r[1], s1 = dx ? r[1] : r[2];
+                    for (int s = s0; s < s1; s++) {
+                        const int c = sorted[s];
+                        const double fx = frac[3 * c], fy = frac[3 * c + 1], fz = frac[3 * c + 2];
+                        const double wx = dx ? fx : 1.0 - fx, wy = dy ? fy : 1.0 - fy, wz = dz ? fz : 1.0 - fz;
+                        acc += rate[c] * (wx * wy * wz);
+                    }
                 }
             }
-        }
+        q[((long)k * ny + j) * nx + i] = acc * inv_vol;
+        for (int dy = 0; dy <= 1; dy++)
+            for (int e = 0; e < 3; e++) lo[dy][e] = hi[dy][e];
     }
-    q[((long)k * ny + j) * nx + i] = acc * inv_vol;
-}
-
-// Sources of the mechanics context's cells: centre (p + q) / 2 and uptake
-// rate x Length (hot record: p [0..2], q [3..5], Length [12]).
-__global__ void k_cells_src(const double* __restrict__ hot, int n, double rate, double* __restrict__ cm,
-                            double* __restrict__ r) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double* h = hot + (size_t)i * bcg::HOT;
-    cm[3 * i] = (h[0] + h[3]) * 0.5;
-    cm[3 * i + 1] = (h[1] + h[4]) * 0.5;
-    cm[3 * i + 2] = (h[2] + h[5]) * 0.5;
-    r[i] = rate * h[12];
 }
 
 // Growth rate of every mechanics bin (x * BoxY * BoxZ + y * BoxZ + z) from
@@ -195,6 +236,7 @@ struct bcg_rd3d {
     // bcg_rd3d_update_cells: its fixed-shape part (set-up, V-cycles, growth
     // rates) recorded as a HIP graph for one (mechanics context, V-cycle
     // count) and replayed
+    bool a32_valid = false;  // FP32 copies of a0 match the FP64 ones
     hipGraphExec_t upd_graph = nullptr;
     const bcg_ctx* upd_cells = nullptr;
     int upd_vcycles = 0, upd_cur = -1;
@@ -431,22 +473,35 @@ long nparts_of(const Level& v) {
 }
 
 // a0 on every level (FP64 and FP32) and the right-hand side from c_prev.
-int setup_system(bcg_rd3d* c) {
+// a0 on every level and b of the implicit step from c_prev (from x, copied
+// into c_prev, when x is given); the FP32 copies of a0 that the mixed cycles
+// use only with `f32` (else they are marked stale).
+int setup_system(bcg_rd3d* c, bool f32, const double* x = nullptr) {
     Level& f = c->L[0];
     const long n = c->N;
-    hipLaunchKernelGGL(k_monod_a0, dim3(nblk(n, 256)), dim3(256), 0, c->stream, c->q, c->cprev, c->prm.K, c->prm.dt,
-                       f.a064, f.a032, n);
-    hipLaunchKernelGGL(k_rhs, dim3(nblk(n, 256)), dim3(256), 0, c->stream, c->cprev, c->b, c->prm.c_res,
-                       (long)f.nx * f.ny, n);
+    hipLaunchKernelGGL(k_setup_fine, dim3(nblk(n, 256)), dim3(256), 0, c->stream, x, c->cprev, c->q, c->prm.K,
+                       c->prm.dt, c->prm.c_res, (long)f.nx * f.ny, f.a064, f32 ? f.a032 : nullptr, c->b, n);
     for (size_t l = 0; l + 1 < c->L.size(); l++) {
         Level& v = c->L[l];
         Level& w = c->L[l + 1];
         hipLaunchKernelGGL((k_restrict_a0<double>), slab_grid(w), dim3(64, 4), 0, c->stream, v.a064, w.a064, v.nx, v.ny, v.nz,
                            w.nx, w.ny, w.nz, v.cx, v.cy, v.cz);
-        hipLaunchKernelGGL((k_cast<double, float>), dim3(nblk(w.n(), 256)), dim3(256), 0, c->stream, w.a064, w.a032, w.n());
+        if (f32)
+            hipLaunchKernelGGL((k_cast<double, float>), dim3(nblk(w.n(), 256)), dim3(256), 0, c->stream, w.a064, w.a032,
+                               w.n());
     }
+    c->a32_valid = f32;
     HIPCHK(c, hipGetLastError());
     return 0;
+}
+
+// The FP32 copies of a0 on every level, when the last set-up left them stale.
+void ensure_a32(bcg_rd3d* c) {
+    if (c->a32_valid) return;
+    for (Level& v : c->L)
+        hipLaunchKernelGGL((k_cast<double, float>), dim3(nblk(v.n(), 256)), dim3(256), 0, c->stream, v.a064, v.a032,
+                           v.n());
+    c->a32_valid = true;
 }
 
 }  // namespace
@@ -654,6 +709,8 @@ int bcg_rd3d_get_uptake(bcg_rd3d* c, double* q) {
     return BCG_OK;
 }
 
+}  // extern "C"
+
 namespace {
 // This context's next work follows the mechanics context's queued work (and
 // whatever that context still waits for).
@@ -710,16 +767,18 @@ int dep_reserve(bcg_rd3d* c, int n, int want) {
     return 0;
 }
 
-// Deposition of the n sources in c->dcm, c->drate into q (stream-ordered).
-int dep_run(bcg_rd3d* c, int n, int32_t* n_inside) {
+// Deposition of n sources into q (stream-ordered); their rates are in
+// c->drate (written there by the key kernel for cells).
+template <class Src>
+int dep_run(bcg_rd3d* c, Src src, int n, int32_t* n_inside) {
     Level& v = c->L[0];
     const long nn = c->N;
     HIPCHK(c, hipMemsetAsync(c->dcnt, 0, sizeof(int) * (nn + 1), c->stream));
     HIPCHK(c, hipMemsetAsync(c->dfill, 0, sizeof(int) * nn, c->stream));
     if (n > 0)
-        hipLaunchKernelGGL(k_dep_key, dim3(nblk(n, 256)), dim3(256), 0, c->stream, c->dcm, n, c->prm.origin[0],
+        hipLaunchKernelGGL(k_dep_key<Src>, dim3(nblk(n, 256)), dim3(256), 0, c->stream, src, n, c->prm.origin[0],
                            c->prm.origin[1], c->prm.origin[2], 1.0 / c->prm.h, v.nx, v.ny, v.nz, c->dkey, c->dfrac,
-                           c->dcnt);
+                           c->dcnt, c->drate);
     // exclusive scan of the bucket counts (the mechanics' scan kernels)
     const long ns = nn + 1;
     const int nb = nblk(ns, SCAN_BLOCK_ITEMS);
@@ -732,7 +791,8 @@ int dep_run(bcg_rd3d* c, int n, int32_t* n_inside) {
                            c->dsorted);
         hipLaunchKernelGGL(k_dep_sort, dim3(nblk(nn, 256)), dim3(256), 0, c->stream, c->dstart, nn, c->dsorted);
     }
-    hipLaunchKernelGGL(k_dep_gather, grid2(v), dim3(64, 4), 0, c->stream, c->dstart, c->dsorted, c->dfrac, c->drate,
+    const dim3 gg(nblk(v.nx, 64), nblk(v.ny, 4), nblk(v.nz, DEP_KCHUNK));
+    hipLaunchKernelGGL(k_dep_gather, gg, dim3(64, 4), 0, c->stream, c->dstart, c->dsorted, c->dfrac, c->drate,
                        v.nx, v.ny, v.nz, 1.0 / (c->prm.h * c->prm.h * c->prm.h), c->q);
     HIPCHK(c, hipGetLastError());
     if (n_inside) {
@@ -745,6 +805,8 @@ int dep_run(bcg_rd3d* c, int n, int32_t* n_inside) {
 }
 }  // namespace
 
+extern "C" {
+
 int bcg_rd3d_deposit(bcg_rd3d* c, const double* cm, const double* rate, int32_t n, int32_t* n_inside) {
     if (!c || n < 0 || (n > 0 && (!cm || !rate))) return BCG_ERR_ARG;
     int rc = dep_reserve(c, n, n);
@@ -753,7 +815,7 @@ int bcg_rd3d_deposit(bcg_rd3d* c, const double* cm, const double* rate, int32_t 
         HIPCHK(c, hipMemcpyAsync(c->dcm, cm, 3 * sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->drate, rate, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     }
-    if ((rc = dep_run(c, n, n_inside))) return rc;
+    if ((rc = dep_run(c, PointSources{c->dcm}, n, n_inside))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return BCG_OK;
 }
@@ -764,10 +826,7 @@ int bcg_rd3d_deposit_cells(bcg_rd3d* c, bcg_ctx* cells, double rate, int32_t* n_
     int rc = dep_reserve(c, n, cells->prm.maxCells);
     if (rc) return rc;
     HIPCHK(c, join_cells(c, cells));
-    if (n > 0)
-        hipLaunchKernelGGL(k_cells_src, dim3(nblk(n, 256)), dim3(256), 0, c->stream, cells->hot[cells->cur], n, rate,
-                           c->dcm, c->drate);
-    if ((rc = dep_run(c, n, n_inside))) return rc;
+    if ((rc = dep_run(c, CellSources{cells->hot[cells->cur], rate}, n, n_inside))) return rc;
     HIPCHK(c, release_cells(c, cells));
     return BCG_OK;
 }
@@ -778,18 +837,14 @@ int bcg_rd3d_update_cells(bcg_rd3d* c, bcg_ctx* cells, double rate, double mu, d
     int rc = dep_reserve(c, n, cells->prm.maxCells);
     if (rc) return rc;
     HIPCHK(c, join_cells(c, cells));
-    if (n > 0)
-        hipLaunchKernelGGL(k_cells_src, dim3(nblk(n, 256)), dim3(256), 0, c->stream, cells->hot[cells->cur], n, rate,
-                           c->dcm, c->drate);
-    if ((rc = dep_run(c, n, nullptr))) return rc;
+    if ((rc = dep_run(c, CellSources{cells->hot[cells->cur], rate}, n, nullptr))) return rc;
     // the implicit step from the current field and the growth rates: kernel
     // launches of fixed shape, replayed from a graph
     Level& f = c->L[0];
     double* x = c->x[c->cur];
     const bcg::PhysParams& P = cells->P;
     auto body = [&]() {
-        hipLaunchKernelGGL((k_cast<double, double>), dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, x, c->cprev, c->N);
-        setup_system(c);
+        setup_system(c, false, x);
         f.u64 = x;
         for (int i = 0; i < vcycles; i++) vcycle<double>(c, 0);
         f.u64 = nullptr;
@@ -820,6 +875,7 @@ int bcg_rd3d_update_cells(bcg_rd3d* c, bcg_ctx* cells, double rate, double mu, d
         c->upd_K = K;
     }
     HIPCHK(c, hipGraphLaunch(c->upd_graph, c->stream));
+    c->a32_valid = false;  // the replayed set-up writes FP64 a0 only
     HIPCHK(c, release_cells(c, cells));
     return BCG_OK;
 }
@@ -852,6 +908,7 @@ int solve_core(bcg_rd3d* c, int mode, double tol, int max_iter, int* iters, doub
     double r0 = 0, rn = 0, bn = 0;
     const ZmGrid zg = zm_grid(f.nx, f.ny, f.nz, c->res_zm);
     const long np = zg.blocks();  // one partial sum per z-marching workgroup
+    if (mode == 1) ensure_a32(c);
     // the tolerance is relative to ||b||
     hipLaunchKernelGGL((k_sumsq<double>), dim3(4096), dim3(256), 0, c->stream, c->b, c->N, c->part);
     if ((rc = read_norm(c, 4096, &bn))) return rc;
@@ -924,9 +981,8 @@ extern "C" {
 int bcg_rd3d_step(bcg_rd3d* c, int32_t mode, double tol, int32_t max_iter, int32_t* iters, double* relres) {
     if (!c || (mode != 0 && mode != 1) || max_iter < 1 || !(tol > 0)) return BCG_ERR_ARG;
     // c_prev <- c ; a0, b from c_prev ; initial guess c_new = c_prev
-    HIPCHK(c, hipMemcpyAsync(c->cprev, c->x[c->cur], sizeof(double) * c->N, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    int rc = setup_system(c);
+    int rc = setup_system(c, mode == 1, c->x[c->cur]);
     if (rc) return rc;
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     rc = solve_core(c, mode, tol, max_iter, iters, relres);
@@ -941,7 +997,7 @@ int bcg_rd3d_set_state(bcg_rd3d* c, const double* c_old, const double* c_new) {
     if (!c || !c_old || !c_new) return BCG_ERR_ARG;
     HIPCHK(c, hipMemcpyAsync(c->cprev, c_old, sizeof(double) * c->N, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->x[c->cur], c_new, sizeof(double) * c->N, hipMemcpyHostToDevice, c->stream));
-    int rc = setup_system(c);
+    int rc = setup_system(c, true);
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return BCG_OK;
@@ -1036,8 +1092,7 @@ int bcg_rd3d_gather(bcg_rd3d* c, const double* cm, int32_t n, const double* fiel
 int bcg_rd3d_vcycle(bcg_rd3d* c, int32_t mode, float* ms) {
     if (!c || (mode != 0 && mode != 1)) return BCG_ERR_ARG;
     Level& f = c->L[0];
-    HIPCHK(c, hipMemcpyAsync(c->cprev, c->x[c->cur], sizeof(double) * c->N, hipMemcpyDeviceToDevice, c->stream));
-    int rc = setup_system(c);
+    int rc = setup_system(c, mode == 1, c->x[c->cur]);
     if (rc) return rc;
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     if (mode == 0) {

@@ -60,8 +60,9 @@ __global__ void k_cast(const Ts* a, Td* b, long n);
 template <class T>
 __global__ void k_sumsq(const T* x, long n, double* part);
 __global__ void k_sum_parts(const double* part, int n, double* out);
-__global__ void k_monod_a0(const double* q, const double* c, double K, double dt, double* a0, float* a0f, long n);
 __global__ void k_rhs(const double* c, double* b, double c_res, long nxy, long n);
+__global__ void k_setup_fine(const double* x, double* cprev, const double* q, double K, double dt, double c_res,
+                             long nxy, double* a0, float* a0f, double* b, long n);
 template <class T>
 __global__ void k_restrict_a0(const T* a0f, T* a0c, int nx, int ny, int nz, int ncx, int ncy, int ncz, int cx, int cy,
                               int cz);

@@ -384,14 +384,26 @@ __global__ void __launch_bounds__(256) k_sum_parts(const double* __restrict__ pa
     if (threadIdx.x == 0) *out = (lds[0] + lds[1]) + (lds[2] + lds[3]);
 }
 
-// a0 = 1 + dt * q / (K + c)   (Monod uptake linearised about c).
-__global__ void k_monod_a0(const double* __restrict__ q, const double* __restrict__ c, double K, double dt,
-                           double* __restrict__ a0, float* __restrict__ a0f, long n) {
+// The system of one implicit step from the previous field c, in one pass over
+// the fine grid: a0 = 1 + dt * q / (K + c) (Monod uptake linearised about c;
+// FP32 copy when a0f is given) and b as in k_rhs; with `x` given, c is x and
+// is copied into cprev on the way.
+__global__ void k_setup_fine(const double* __restrict__ x, double* __restrict__ cprev, const double* __restrict__ q,
+                             double K, double dt, double c_res, long nxy, double* __restrict__ a0,
+                             float* __restrict__ a0f, double* __restrict__ b, long n) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double a = 1.0 + dt * q[i] / (K + c[i]);
+    double c;
+    if (x) {
+        c = x[i];
+        cprev[i] = c;
+    } else {
+        c = cprev[i];
+    }
+    const double a = 1.0 + dt * q[i] / (K + c);
     a0[i] = a;
     if (a0f) a0f[i] = (float)a;
+    b[i] = i < nxy ? c_res : c;
 }
 
 // b = c with the Dirichlet rows set to c_res.
@@ -648,15 +660,21 @@ __global__ void k_cic_gather(const double* __restrict__ cm, int n, double ox, do
 // levels fit. Levels hold at most a few thousand points: 32-bit indices.
 // The point updates repeat the arithmetic of k_rbgs / k_residual /
 // k_restrict / k_prolong.
+// One colour: the threads take the points of that colour only (slot m of a
+// row is i = 2m + ((j + k + color) & 1); the last slot of a row of odd
+// length may fall outside it).
 __device__ __forceinline__ void bot_rbgs(const BotLevel& v, int color, int nthr, int tid) {
-    const int n = (int)v.nx * v.ny * v.nz;
-    const int sxy = (int)v.nx * v.ny;
+    const int nx = v.nx, ny = v.ny;
+    const int hx = (nx + 1) >> 1;
+    const int nslot = hx * ny * (int)v.nz;
+    const int sxy = nx * ny;
     const double d2 = 2.0 * (v.lx + v.ly + v.lz);
-    for (int p = tid; p < n; p += nthr) {
-        const int k = (int)(p / sxy);
-        const int j = (int)((p - k * sxy) / v.nx);
-        const int i = (int)(p - k * sxy - (int)j * v.nx);
-        if (((i + j + k) & 1) != color) continue;
+    for (int t = tid; t < nslot; t += nthr) {
+        const int row = t / hx;
+        const int k = row / ny, j = row - k * ny;
+        const int i = 2 * (t - row * hx) + ((j + k + color) & 1);
+        if (i >= nx) continue;
+        const int p = k * sxy + j * nx + i;
         if (k == 0) {
             v.u[p] = v.f[p];
             continue;
@@ -821,7 +839,6 @@ template __global__ void k_fill<float>(float*, long, float);
 template __global__ void k_fill<double>(double*, long, double);
 template __global__ void k_cast<float, double>(const float*, double*, long);
 template __global__ void k_cast<double, float>(const double*, float*, long);
-template __global__ void k_cast<double, double>(const double*, double*, long);
 template __global__ void k_sumsq<float>(const float*, long, double*);
 template __global__ void k_sumsq<double>(const double*, long, double*);
 template __global__ void k_restrict_a0<float>(const float*, float*, int, int, int, int, int, int, int, int, int);

@@ -263,6 +263,31 @@ def test_adjoint_matches_oracle(mode):
 
 
 @pytest.mark.gpu
+def test_mixed_adjoint_after_fp64_step_rebuilds_fp32_operator():
+    """An FP64 step's set-up writes no FP32 copies of a0; a mixed-precision
+    adjoint after it makes them and gives the gradients of a set-up that
+    wrote them (set_state from the same c_old and c_new), bitwise."""
+    from baccolonygrowth_amd import adjoint as adj
+
+    shape = (9, 17, 17)
+    h, D, dt, K = 1.0, 20.0, 10.0, 0.1
+    c_old, q, target = _loss_parts(shape, seed=4)
+    print(f"[gpu] mixed adjoint after an FP64 step {shape}", flush=True)
+    with _solver(shape, h, D, dt, K) as a, _solver(shape, h, D, dt, K) as b:
+        a.set_field(c_old)
+        a.set_uptake(q)
+        a.step("fp64", tol=1e-12, max_iter=200)
+        c1 = a.field()
+        g = 2 * (c1 - target)
+        ra = adj.adjoint(a, g, "mixed", tol=1e-12)
+        b.set_uptake(q)
+        adj.set_state(b, c_old, c1)
+        rb = adj.adjoint(b, g, "mixed", tol=1e-12)
+    assert ra[0].tobytes() == rb[0].tobytes() and ra[1].tobytes() == rb[1].tobytes()
+    assert ra[2] == rb[2] and ra[3] == rb[3]
+
+
+@pytest.mark.gpu
 def test_autograd_rates_D_K_vs_finite_differences():
     """Gradients of a radial-profile loss with respect to per-cell rates, D
     and K through torch.autograd, against central differences (h = 1e-6
