@@ -303,6 +303,7 @@ int bcg_destroy(bcg_ctx* c) {
     for (void* q : ptrs)
         if (q) hipFree(q);
     if (c->hstats) hipHostFree(c->hstats);
+    if (c->hids) hipHostFree(c->hids);
     for (int i = 0; i < 7; i++)
         if (c->ev[i]) hipEventDestroy(c->ev[i]);
     if (c->after) hipEventDestroy(c->after);
@@ -613,16 +614,25 @@ int bcg_step(bcg_ctx* c, double dt, double t, int32_t flags, int32_t* n_div, int
         HIPCHK(c, hipGetLastError());
     }
     HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
+    // the mothers' IDs come back with the stats (one host synchronisation)
+    const int nl = lineage ? std::min(D, lineage_cap) : 0;
+    if (nl > 0) {
+        if (nl > c->hids_cap) {
+            if (c->hids) HIPCHK(c, hipHostFree(c->hids));
+            c->hids = nullptr;
+            c->hids_cap = 0;
+            const int cap = std::max(nl, 1024);
+            HIPCHK(c, hipHostMalloc((void**)&c->hids, sizeof(int) * cap, hipHostMallocDefault));
+            c->hids_cap = cap;
+        }
+        HIPCHK(c, hipMemcpyAsync(c->hids, c->div_ids, nl * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    }
     if ((rc = read_stats(c))) return rc;
     if ((rc = check_err(c))) return rc;
     for (int i = 0; i < 5; i++) hipEventElapsedTime(&c->last_ms[i], c->ev[i == 2 ? 6 : i], c->ev[i + 1]);
-    if (lineage && D > 0) {
-        std::vector<int> ids(D);
-        HIPCHK(c, hipMemcpy(ids.data(), c->div_ids, D * sizeof(int), hipMemcpyDeviceToHost));
-        for (int k = 0; k < D && k < lineage_cap; k++) {
-            lineage[2 * k] = ids[k];
-            lineage[2 * k + 1] = N0 + k;
-        }
+    for (int k = 0; k < nl; k++) {
+        lineage[2 * k] = c->hids[k];
+        lineage[2 * k + 1] = N0 + k;
     }
     c->N = N0 + D;
     c->cur = out;

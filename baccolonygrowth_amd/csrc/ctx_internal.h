@@ -43,6 +43,8 @@ struct bcg_ctx {
     double *Fbuf = nullptr, *Tbuf = nullptr;
     BcgStats* stats = nullptr;  // device
     BcgStats* hstats = nullptr; // pinned host
+    int* hids = nullptr;         // pinned host: the step's dividing mothers (lineage)
+    int hids_cap = 0;
     bcg::GlibcRand rng;
     hipEvent_t ev[7];
     float last_ms[6] = {0, 0, 0, 0, 0, 0};

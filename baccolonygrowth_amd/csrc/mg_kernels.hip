@@ -727,8 +727,8 @@ __global__ void __launch_bounds__(1024) k_bottom(BotLevels B, int nu1, int nu2, 
         }
     __syncthreads();
     for (int l = 0; l < last; l++) {
-        const BotLevel& v = S[l];
-        const BotLevel& w = S[l + 1];
+        const BotLevel v = S[l];  // in registers for the level's phases
+        const BotLevel w = S[l + 1];
         for (int s = 0; s < nu1; s++)
             for (int color = 0; color < 2; color++) {
                 bot_rbgs(v, color, nthr, tid);
@@ -762,7 +762,7 @@ __global__ void __launch_bounds__(1024) k_bottom(BotLevels B, int nu1, int nu2, 
         __syncthreads();
     }
     {
-        const BotLevel& v = S[last];
+        const BotLevel v = S[last];
         const int n = (int)v.nx * v.ny * v.nz;
         for (int p = tid; p < n; p += nthr) v.u[p] = 0.0;
         __syncthreads();
@@ -773,8 +773,8 @@ __global__ void __launch_bounds__(1024) k_bottom(BotLevels B, int nu1, int nu2, 
             }
     }
     for (int l = last - 1; l >= 0; l--) {
-        const BotLevel& v = S[l];
-        const BotLevel& w = S[l + 1];
+        const BotLevel v = S[l];  // in registers for the level's phases
+        const BotLevel w = S[l + 1];
         const int n = (int)v.nx * v.ny * v.nz, sxy = (int)v.nx * v.ny;
         for (int p = tid; p < n; p += nthr) {
             const int k = (int)(p / sxy);

@@ -259,8 +259,11 @@ int bcg_rd3d_gather(bcg_rd3d* ctx, const double* cm, int32_t n, const double* fi
  *   bcg_rd3d_growth_to_cells: the context's growth rate of every bin
  *     (GrowthRate1, before the reference's factor 1.5850) = mu c / (K + c),
  *     c the current field at the bin centre (trilinear, clamped to the grid).
- * Each runs after the context's queued work; the context's next work runs
- * after bcg_rd3d_growth_to_cells.                                          */
+ * Each runs after the context's queued work. The context then waits for
+ * them (stream-ordered, no host synchronisation) before its next kernel that
+ * reads growth rates or writes cells: in bcg_step before move/grow, so that
+ * step's bins and lists run alongside the nutrient work; every other call on
+ * the context waits on entry.                                              */
 int bcg_rd3d_deposit_cells(bcg_rd3d* ctx, bcg_ctx* cells, double rate, int32_t* n_inside);
 int bcg_rd3d_growth_to_cells(bcg_rd3d* ctx, bcg_ctx* cells, double mu, double K);
 /* One coupled nutrient update without host synchronisation:
