@@ -70,14 +70,15 @@ def set_coeffs(solver: Nutrients3D, D: float, K: float) -> None:
 
 class NutrientStepFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, rates, D, K, c_old, solver, cm, mode, tol):
+    def forward(ctx, rates, D, K, c_old, solver, cm, mode, tol, adj_tol):
         set_coeffs(solver, D.item(), K.item())
         solver.deposit(cm, _np(rates))
         solver.set_field(_np(c_old))
         solver.step(mode, tol=tol, max_iter=200)
         c_new = torch.from_numpy(solver.field())
         ctx.save_for_backward(rates, D, K, c_old, c_new)
-        ctx.solver, ctx.cm, ctx.mode, ctx.tol = solver, cm, mode, tol
+        ctx.solver, ctx.cm, ctx.mode = solver, cm, mode
+        ctx.adj_tol = adj_tol if adj_tol is not None else min(tol, 1e-12)
         return c_new
 
     @staticmethod
@@ -88,7 +89,7 @@ class NutrientStepFn(torch.autograd.Function):
         set_coeffs(s, D.item(), K.item())
         s.deposit(ctx.cm, _np(rates))
         set_state(s, _np(c_old), _np(c_new))
-        dq, dc, dK, dD = adjoint(s, _np(g), ctx.mode, min(ctx.tol, 1e-12))
+        dq, dc, dK, dD = adjoint(s, _np(g), ctx.mode, ctx.adj_tol)
         drates = torch.from_numpy(gather(s, ctx.cm, dq))
         return (
             drates,
@@ -99,11 +100,15 @@ class NutrientStepFn(torch.autograd.Function):
             None,
             None,
             None,
+            None,
         )
 
 
-def nutrient_step(rates, D, K, c_old, solver: Nutrients3D, cm, mode: str = "fp64", tol: float = 1e-12):
-    return NutrientStepFn.apply(rates, D, K, c_old, solver, cm, mode, tol)
+def nutrient_step(rates, D, K, c_old, solver: Nutrients3D, cm, mode: str = "fp64", tol: float = 1e-12,
+                  adj_tol: float | None = None):
+    """One implicit step as a torch.autograd.Function; the adjoint solve of
+    its backward pass goes to `adj_tol` (default min(tol, 1e-12))."""
+    return NutrientStepFn.apply(rates, D, K, c_old, solver, cm, mode, tol, adj_tol)
 
 
 def rollout(rates, D, K, c0, solver: Nutrients3D, cm, steps: int, checkpoint_every: int = 5, mode="fp64", tol=1e-12):
@@ -122,4 +127,24 @@ def rollout(rates, D, K, c0, solver: Nutrients3D, cm, steps: int, checkpoint_eve
         n = min(checkpoint_every, steps - done)
         c = checkpoint(window, c, n, use_reentrant=False)
         done += n
+    return c
+
+
+def rollout_seq(rates_seq, D, K, c0, solver: Nutrients3D, cm_seq, checkpoint_every: int = 5, mode="fp64", tol=1e-12,
+                adj_tol=None):
+    """One nutrient step per (rates, cm) pair of the sequences: the sources of
+    each update (the cells move and divide between updates); only every
+    `checkpoint_every`-th field is kept for the backward pass, the others are
+    recomputed from it."""
+    from torch.utils.checkpoint import checkpoint
+
+    def window(c, lo, hi):
+        for k in range(lo, hi):
+            c = nutrient_step(rates_seq[k], D, K, c, solver, cm_seq[k], mode, tol, adj_tol)
+        return c
+
+    c = c0
+    n = len(rates_seq)
+    for lo in range(0, n, checkpoint_every):
+        c = checkpoint(window, c, lo, min(lo + checkpoint_every, n), use_reentrant=False)
     return c

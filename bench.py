@@ -72,6 +72,16 @@ def device_sync() -> None:
         raise RuntimeError("hipDeviceSynchronize failed")
 
 
+def device_used_bytes() -> int:
+    """Device memory in use (total - free, hipMemGetInfo)."""
+    import ctypes
+
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    if ctypes.CDLL("libamdhip64.so").hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) != 0:
+        raise RuntimeError("hipMemGetInfo failed")
+    return total.value - free.value
+
+
 def measured_traffic(kernel: str):
     """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by the
     copy calibration) from the rocprofv3 summary of this workload in
@@ -88,10 +98,10 @@ def measured_traffic(kernel: str):
     return None
 
 
-def make_workload(n: int, seed: int = 2):
+def make_workload(n: int, seed: int = 2, radius: float = 600.0):
     from baccolonygrowth_amd.colony import dome_colony, dome_params, height_normal_from_cells
 
-    cells = dome_colony(n, radius=600.0, height=20.0, seed=seed)
+    cells = dome_colony(n, radius=radius, height=20.0, seed=seed)
     p = dome_params(cells, bin_size=5.0, max_neighbours=64, extra_cells=max(n // 8, 1024))
     rng = np.random.default_rng(seed)
     # cells at uniformly spread points of their cycle: steady division rate
@@ -219,6 +229,151 @@ ENV_INPUT = os.path.join(ROOT, "tests", "golden", "inputFile_20mM_SmallSimulatio
 ENV_STATE = os.path.join(ROOT, "tests", "golden", "restart_8.npz")
 
 
+RUN_CELLS = 1 << 22
+RUN_RADIUS = 1200.0
+
+
+def radial_target(shape, h: float, origin) -> np.ndarray:
+    """c(r) = 20 mM (1 - 0.8 exp(-r / 150 um)), r the distance from the
+    colony axis, on every plane of the grid."""
+    nz, ny, nx = shape
+    x = origin[0] + h * np.arange(nx)
+    y = origin[1] + h * np.arange(ny)
+    r = np.sqrt(x[None, :] ** 2 + y[:, None] ** 2)
+    return np.ascontiguousarray(np.broadcast_to(20.0 * (1.0 - 0.8 * np.exp(-r / 150.0)), shape))
+
+
+def coupled_run(steps: int = 200, update_every: int = 10, ckpt: int = 5) -> dict:
+    """The full coupled run: 2^22 cells from the dome scaled to r = 1200 um,
+    FP64 mechanics (1 s steps, lists rebuilt every step) with a nutrient
+    update every `update_every` steps on 513 x 513 x 129 at 1 um (deposition
+    from the cells, the implicit 10 s step by the mixed-precision solver to
+    1e-10, growth rates from the field). Then the loss ||c - c_target||^2 of
+    the last field is backpropagated through all the updates (checkpoints
+    every `ckpt`, recomputation in between; forward and adjoint solves to
+    1e-11) to D, K and the cells' rates, checked by central differences
+    (h = 1e-6 relative)."""
+    import torch
+
+    from baccolonygrowth_amd.adjoint import rollout_seq
+    from baccolonygrowth_amd.nutrients import Nutrients3D, default_rd3d_params
+    from baccolonygrowth_amd.simulation import Colony
+
+    used0 = device_used_bytes()
+    t0 = time.time()
+    p, cells, fields = make_workload(RUN_CELLS, radius=RUN_RADIUS)
+    log(f"[run] {len(cells)} cells (r = {RUN_RADIUS:.0f} um), built in {time.time() - t0:.1f}s")
+    nz, ny, nx = MG_SHAPE
+    half = (nx - 1) * MG_H / 2
+    origin = (-half, -half, 0.0)
+    D0, K0 = 600.0, 0.1
+    prm = default_rd3d_params(nx=nx, ny=ny, nz=nz, h=MG_H, D=D0, dt=update_every * DT_H * 3600.0, K=K0, c_res=20.0,
+                              origin=origin, nu1=2, nu2=2, inner=2)
+    rate = Q_CELL / 3.0  # per um of cell length
+    lo, hi = np.array(origin), np.array(origin) + MG_H * (np.array([nx, ny, nz]) - 1)
+    seq_cm, seq_rate, iters = [], [], []
+    with Colony(p) as col, Nutrients3D(prm) as s:
+        col.LoadCells(cells)
+        col.set_fields(*fields)
+        col.srand(2)
+        c_init = s.field()
+        peak = device_used_bytes()
+        t = 0.0
+        cell_updates = 0
+        excl = 0.0
+        device_sync()
+        ts = time.perf_counter()
+        for k in range(steps):
+            cell_updates += col.n
+            col.step(DT_H, t, update=True, no_patch=True)
+            t += DT_H
+            if (k + 1) % update_every:
+                continue
+            s.deposit_cells(col, rate)
+            it, _ = s.step("mixed", tol=1e-10, max_iter=100)
+            iters.append(it)
+            s.growth_to_cells(col, GR_BASE, K0)
+            # this update's sources for the backward pass (not timed)
+            device_sync()
+            a = time.perf_counter()
+            cc = col.cells()
+            cm = (cc["p"] + cc["q"]) * 0.5
+            g = np.floor((cm - lo) / MG_H)
+            inside = np.all((g >= 0) & (g < np.array([nx, ny, nz]) - 1), axis=1)
+            seq_cm.append(np.ascontiguousarray(cm[inside]))
+            seq_rate.append(rate * cc["Length"][inside])
+            peak = max(peak, device_used_bytes())
+            del cc
+            device_sync()
+            excl += time.perf_counter() - a
+        device_sync()
+        el = time.perf_counter() - ts - excl
+        n_end = col.n
+        c_run = s.field()
+        log(f"[run] {steps} steps, {len(iters)} nutrient updates: {el:.2f} s, {cell_updates / el:.3e} cell-updates/s")
+
+        target = torch.from_numpy(radial_target(MG_SHAPE, MG_H, origin))
+        base = [torch.from_numpy(r) for r in seq_rate]
+        c0 = torch.from_numpy(c_init)
+
+        def loss_of(D, K, a):
+            # 1e-11: the mixed solver's floor on 34M points is ~4e-12
+            c = rollout_seq([a * r for r in base], D, K, c0, s, seq_cm, checkpoint_every=ckpt, mode="mixed", tol=1e-11,
+                            adj_tol=1e-11)
+            return c, ((c - target) ** 2).sum()
+
+        Dt = torch.tensor(D0, dtype=torch.float64, requires_grad=True)
+        Kt = torch.tensor(K0, dtype=torch.float64, requires_grad=True)
+        at = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+        ta = time.perf_counter()
+        c_fin, L = loss_of(Dt, Kt, at)
+        L.backward()
+        t_adj = time.perf_counter() - ta
+        peak = max(peak, device_used_bytes())
+        grads = {"D": Dt.grad.item(), "K": Kt.grad.item(), "rate_factor": at.grad.item()}
+        log(f"[run] forward + backward through {len(iters)} updates: {t_adj:.1f} s; dL/dD {grads['D']:.6e}, "
+            f"dL/dK {grads['K']:.6e}, dL/d(rate factor) {grads['rate_factor']:.6e}")
+        # central differences at two relative steps: 1e-6, and 1e-4 where the
+        # solves' 1e-11 tolerance (a loss change of ~0.1 against ~25 for D at
+        # 1e-6) no longer dominates
+        fd = {1e-6: {}, 1e-4: {}}
+        with torch.no_grad():
+            for rel_h in fd:
+                for name, v0 in (("D", D0), ("K", K0), ("rate_factor", 1.0)):
+                    vals = []
+                    for sgn in (1.0, -1.0):
+                        args_ = {"D": D0, "K": K0, "rate_factor": 1.0}
+                        args_[name] = v0 * (1.0 + sgn * rel_h)
+                        _, Lv = loss_of(*(torch.tensor(args_[k], dtype=torch.float64) for k in ("D", "K", "rate_factor")))
+                        vals.append(float(Lv))
+                    fd[rel_h][name] = (vals[0] - vals[1]) / (2.0 * rel_h * v0)
+                    log(f"[run] dL/d{name}: adjoint {grads[name]:.8e}, central difference (h {rel_h:g}) "
+                        f"{fd[rel_h][name]:.8e}")
+    err = {h: {k: abs(grads[k] - fd[h][k]) / abs(fd[h][k]) for k in grads} for h in fd}
+    return {
+        "cells": RUN_CELLS,
+        "cells_end": int(n_end),
+        "radius_um": RUN_RADIUS,
+        "mechanics_steps": steps,
+        "nutrient_updates": len(iters),
+        "grid": [nx, ny, nz],
+        "mixed_refinements_per_update": iters,
+        "seconds": el,
+        "cell_updates_per_s": cell_updates / el,
+        "ms_per_mechanics_step_incl_updates": 1000.0 * el / steps,
+        "cells_in_grid_last_update": int(len(seq_cm[-1])),
+        "run_vs_recomputed_field_max_abs_mM": float(np.abs(c_fin.detach().numpy() - c_run).max()),
+        "loss": float(L.detach()),
+        "gradient": grads,
+        "central_difference": {f"h={h:g}": fd[h] for h in fd},
+        "gradient_rel_error": {f"h={h:g}": err[h] for h in err},
+        "adjoint_s": t_adj,
+        "checkpoint_every": ckpt,
+        "device_memory_peak_gb": peak / 1e9,
+        "device_memory_before_gb": used0 / 1e9,
+    }
+
+
 def _env_case():
     from baccolonygrowth_amd.cells import CELL_DTYPE, read_parameters
     from baccolonygrowth_amd.environment import env_params_from_file
@@ -294,7 +449,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mg", action="store_true")
     ap.add_argument("--no-env", action="store_true")
-    ap.add_argument("--only", choices=("mg", "env"), default=None,
+    ap.add_argument("--only", choices=("mg", "env", "coupled"), default=None,
                     help="run one secondary section alone and print its JSON object (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -306,6 +461,9 @@ def main():
         return
     if args.only == "mg":
         print(json.dumps(mg_bench()), flush=True)
+        return
+    if args.only == "coupled":
+        print(json.dumps({"coupled_run": coupled_run()}), flush=True)
         return
 
     from baccolonygrowth_amd.nutrients import Nutrients3D, default_rd3d_params

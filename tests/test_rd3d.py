@@ -391,6 +391,51 @@ def test_rollout_checkpointed_equals_direct_chain():
 
 
 @pytest.mark.gpu
+def test_rollout_with_sources_per_step_equals_direct_chain():
+    """4 steps whose sources change every step (cells that move), with
+    checkpoints every 3 (a partial last window), give the gradients of the
+    oracle's step-by-step adjoint chain: D, K and each step's rates."""
+    import torch
+
+    from baccolonygrowth_amd.adjoint import gather, rollout_seq
+
+    shape = (5, 9, 9)
+    h, D, dt, K = 1.0, 5.0, 10.0, 0.1
+    rng = np.random.default_rng(9)
+    n = 150
+    cms = [np.column_stack([rng.uniform(1, 7, n), rng.uniform(1, 7, n), rng.uniform(0.5, 3.5, n)]) for _ in range(4)]
+    bases = [rng.random(n) * 0.5 for _ in range(4)]
+    target = 20.0 * (1 - 0.2 * rng.random(shape))
+    c0 = np.full(shape, 20.0)
+    qs = [rd3d.deposit_cic(cms[k], bases[k], shape, h, (0, 0, 0))[0] for k in range(4)]
+    cs = [c0]
+    for k in range(4):
+        cs.append(rd3d.step_direct(cs[-1], qs[k], K, D, dt, h, 20.0))
+    g = 2 * (cs[-1] - target)
+    dqs = [None] * 4
+    dK_tot = dD_tot = 0.0
+    for k in range(3, -1, -1):
+        dq, dc0, dK, dD = rd3d.adjoint_direct(cs[k], cs[k + 1], qs[k], K, D, dt, h, g)
+        dqs[k] = dq
+        dK_tot += dK
+        dD_tot += dD
+        g = dc0
+    print("[gpu] rollout with per-step sources", flush=True)
+    with _solver(shape, h, D, dt, K) as s:
+        rates = [torch.tensor(b, requires_grad=True) for b in bases]
+        Dt = torch.tensor(D, dtype=torch.float64, requires_grad=True)
+        Kt = torch.tensor(K, dtype=torch.float64, requires_grad=True)
+        c = rollout_seq(rates, Dt, Kt, torch.from_numpy(c0), s, cms, checkpoint_every=3)
+        ((c - torch.from_numpy(target)) ** 2).sum().backward()
+        refs = [gather(s, cms[k], dqs[k]) for k in range(4)]
+    assert np.abs(c.detach().numpy() - cs[-1]).max() <= 1e-9
+    assert Kt.grad.item() == pytest.approx(dK_tot, rel=1e-7)
+    assert Dt.grad.item() == pytest.approx(dD_tot, rel=1e-7)
+    for k in range(4):
+        assert np.abs(rates[k].grad.numpy() - refs[k]).max() <= 1e-7 * np.abs(refs[k]).max()
+
+
+@pytest.mark.gpu
 def test_coupling_to_a_mechanics_context():
     """bcg_rd3d_deposit_cells equals bcg_rd3d_deposit of the same sources
     (centre (p + q) / 2, rate * Length), bitwise; bcg_rd3d_growth_to_cells
