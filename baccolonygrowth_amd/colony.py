@@ -248,15 +248,74 @@ def dome_params(cells: np.ndarray, bin_size: float = 5.0, max_neighbours: int = 
     p.crossFeeding = 0
     p.planar = 0
     p.refinementGridHeight = 1
-    p.BoxLength = bin_size
-    ext = np.abs(np.concatenate([cells["p"][:, :2], cells["q"][:, :2]])).max() + 4 * bin_size
-    nb = 2 * int(math.ceil(ext / bin_size)) + 2
-    p.BoxX = p.BoxY = nb
+    half = np.abs(np.concatenate([cells["p"][:, :2], cells["q"][:, :2]])).max()
     top = max(cells["p"][:, 2].max(), cells["q"][:, 2].max())
+    return _with_bins(p, half, top, len(cells) + extra_cells, bin_size, max_neighbours)
+
+
+def _with_bins(p: Params, half: float, top: float, max_cells: int, bin_size: float, max_neighbours: int) -> Params:
+    """Bins of `bin_size` over |x|, |y| <= half and z <= top, with a margin of
+    4 bins."""
+    p.BoxLength = bin_size
+    nb = 2 * int(math.ceil((half + 4 * bin_size) / bin_size)) + 2
+    p.BoxX = p.BoxY = nb
     p.BoxZ = int(math.ceil((top + 4 * bin_size) / bin_size))
-    p.maxCells = len(cells) + extra_cells
+    p.maxCells = max_cells
     p.maxNeighbours = max_neighbours
     return p
+
+
+def growing_params(half: float, top: float, max_cells: int, bin_size: float = 5.0, max_neighbours: int = 64) -> Params:
+    """The parameters of dome_params for a colony that will grow to fill
+    |x|, |y| <= half and z <= top (bins sized for the final colony)."""
+    p = dome_params(small_colony(1), bin_size, max_neighbours)
+    return _with_bins(p, half, top, max_cells, bin_size, max_neighbours)
+
+
+def _segment_distance(p1, q1, p2, q2) -> float:
+    """Distance between the segments p1-q1 and p2-q2 (closest points by
+    clamping the line parameters to [0, 1], each refined against the other)."""
+    d1, d2, r = q1 - p1, q2 - p2, p1 - p2
+    a, e, f = d1 @ d1, d2 @ d2, d2 @ r
+    c, b = d1 @ r, d1 @ d2
+    den = a * e - b * b
+    s = np.clip((b * f - c * e) / den, 0.0, 1.0) if den > 1e-12 else 0.0
+    t = (b * s + f) / e
+    if t < 0.0:
+        t, s = 0.0, np.clip(-c / a, 0.0, 1.0)
+    elif t > 1.0:
+        t, s = 1.0, np.clip((b - c) / a, 0.0, 1.0)
+    return float(np.linalg.norm(p1 + s * d1 - (p2 + t * d2)))
+
+
+def small_colony(n: int = 16, disk: float = 5.0, seed: int = 1, z: float = 0.5, radius: float = 0.5) -> np.ndarray:
+    """A founding colony: n rods on the agar plane (centres uniform in the
+    disk r <= `disk` at height z, azimuth U[0, 2pi), length U[2, 4] um,
+    division length twice the length), placed one by one and redrawn until
+    every pair of axes is at least 2 radii apart."""
+    rng = np.random.default_rng(seed)
+    placed = []
+    tries = 0
+    while len(placed) < n:
+        tries += 1
+        if tries > 200000:
+            raise RuntimeError(f"placed only {len(placed)} of {n} rods")
+        r, phi = disk * np.sqrt(rng.random()), rng.uniform(0.0, 2.0 * np.pi)
+        th, L = rng.uniform(0.0, 2.0 * np.pi), rng.uniform(2.0, 4.0)
+        cm = np.array([r * np.cos(phi), r * np.sin(phi), z])
+        half = 0.5 * L * np.array([np.cos(th), np.sin(th), 0.0])
+        seg = (cm + half, cm - half, L)
+        if all(_segment_distance(seg[0], seg[1], o[0], o[1]) >= 2.0 * radius for o in placed):
+            placed.append(seg)
+    cells = np.zeros(n, CELL_DTYPE)
+    cells["p"] = [s_[0] for s_ in placed]
+    cells["q"] = [s_[1] for s_ in placed]
+    cells["Length"] = [s_[2] for s_ in placed]
+    cells["Radius"] = radius
+    cells["Ldiv"] = 2.0 * cells["Length"]
+    cells["Type"] = 1
+    cells["Ancestor"] = np.arange(1, n + 1)
+    return cells
 
 
 def dome_window_centres(radius: float, height: float, window: float, seed: int = 3, scale: float = 0.99):

@@ -284,6 +284,10 @@ constexpr long BOTTOM_MAX_POINTS = 4096;
 constexpr long FUSED_MIN_POINTS = 1L << 21;
 constexpr long WIDE_MIN_POINTS = 1L << 24;  // FP32 levels that get the 62 x 30 sweep tile
 
+// Blocks of the ||b||^2 reduction (k_sumsq): one partial sum each in c->part,
+// which holds at least this many.
+constexpr int SUMSQ_BLOCKS = 4096;
+
 // Workgroups of `kernel` (block size `threads`) resident on the device at once.
 template <class K>
 long resident(K kernel, int threads) {
@@ -593,7 +597,7 @@ int bcg_rd3d_create(const bcg_rd3d_params* p, bcg_rd3d** out) {
         A(&v.r32, n);
     }
     c->npart = std::max(nparts_of(c->L[0]), zm_grid(c->L[0].nx, c->L[0].ny, c->L[0].nz, c->res_zm).blocks());
-    A(&c->part, (size_t)c->npart);
+    A(&c->part, (size_t)std::max(c->npart, (long)SUMSQ_BLOCKS));
     A(&c->part2, 256);
     if (c->bottom == 0) {
         A(&c->bot_u0, N);
@@ -888,8 +892,8 @@ int solve_core(bcg_rd3d* c, int mode, double tol, int max_iter, int* iters, doub
     const long np = zg.blocks();  // one partial sum per z-marching workgroup
     if (mode == 1) ensure_a32(c);
     // the tolerance is relative to ||b||
-    hipLaunchKernelGGL((k_sumsq<double>), dim3(4096), dim3(256), 0, c->stream, c->b, c->N, c->part);
-    if ((rc = read_norm(c, 4096, &bn))) return rc;
+    hipLaunchKernelGGL((k_sumsq<double>), dim3(SUMSQ_BLOCKS), dim3(256), 0, c->stream, c->b, c->N, c->part);
+    if ((rc = read_norm(c, SUMSQ_BLOCKS, &bn))) return rc;
     if (mode == 0) {
         f.u64 = xo;
         auto resnorm = [&]() {

@@ -229,6 +229,57 @@ ENV_INPUT = os.path.join(ROOT, "tests", "golden", "inputFile_20mM_SmallSimulatio
 ENV_STATE = os.path.join(ROOT, "tests", "golden", "restart_8.npz")
 
 
+SMALL_SHAPE = (17, 65, 65)
+SMALL_H = 2.0
+
+
+def small_run(until: int = 5000, max_steps: int = 40000, update_every: int = 10) -> dict:
+    """The founding-colony run, FP64: 16 rods in a 5 um disk on the agar,
+    1 s mechanics steps (lists rebuilt every step), a nutrient update every
+    `update_every` steps on 65 x 65 x 17 at 2 um with 4 multigrid levels
+    (deposition from the cells, the implicit step solved to 1e-10 in FP64,
+    growth rates), until `until` cells."""
+    from baccolonygrowth_amd.colony import growing_params, small_colony
+    from baccolonygrowth_amd.nutrients import Nutrients3D, default_rd3d_params
+    from baccolonygrowth_amd.simulation import Colony
+
+    cells = small_colony(16, seed=1)
+    p = growing_params(128.0, 24.0, until + 4096)
+    nz, ny, nx = SMALL_SHAPE
+    half = (nx - 1) * SMALL_H / 2
+    prm = default_rd3d_params(nx=nx, ny=ny, nz=nz, h=SMALL_H, D=600.0, dt=update_every * DT_H * 3600.0, K=0.1,
+                              c_res=20.0, origin=(-half, -half, 0.0), max_levels=4)
+    rate = Q_CELL / 3.0
+    with Colony(p) as col, Nutrients3D(prm) as s:
+        col.LoadCells(cells)
+        col.srand(1)
+        s.growth_to_cells(col, GR_BASE, 0.1)
+        t, k, updates, cell_updates, its = 0.0, 0, 0, 0, []
+        device_sync()
+        ts = time.perf_counter()
+        while col.n < until and k < max_steps:
+            cell_updates += col.n
+            col.step(DT_H, t, update=True, no_patch=True)
+            t += DT_H
+            k += 1
+            if k % update_every == 0:
+                s.deposit_cells(col, rate)
+                it, rr = s.step("fp64", tol=1e-10, max_iter=100)
+                its.append(it)
+                s.growth_to_cells(col, GR_BASE, 0.1)
+                updates += 1
+            if k % 2000 == 0:
+                log(f"[small] step {k}: {col.n} cells")
+        device_sync()
+        el = time.perf_counter() - ts
+        n_end = col.n
+        cmin = float(s.field().min())
+    log(f"[small] {n_end} cells after {k} steps ({k / 3600:.2f} h, {updates} nutrient updates): {el:.2f} s")
+    return {"cells_start": 16, "cells_end": int(n_end), "steps": k, "hours": k / 3600.0, "nutrient_updates": updates,
+            "grid": [nx, ny, nz], "fp64_iterations_mean": float(np.mean(its)) if its else 0.0, "seconds": el,
+            "cell_updates_per_s": cell_updates / el, "field_min_mM": cmin}
+
+
 RUN_CELLS = 1 << 22
 RUN_RADIUS = 1200.0
 
@@ -449,7 +500,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mg", action="store_true")
     ap.add_argument("--no-env", action="store_true")
-    ap.add_argument("--only", choices=("mg", "env", "coupled"), default=None,
+    ap.add_argument("--only", choices=("mg", "env", "coupled", "small"), default=None,
                     help="run one secondary section alone and print its JSON object (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -461,6 +512,9 @@ def main():
         return
     if args.only == "mg":
         print(json.dumps(mg_bench()), flush=True)
+        return
+    if args.only == "small":
+        print(json.dumps({"small_run": small_run()}), flush=True)
         return
     if args.only == "coupled":
         print(json.dumps({"coupled_run": coupled_run()}), flush=True)
