@@ -280,6 +280,94 @@ def small_run(until: int = 5000, max_steps: int = 40000, update_every: int = 10)
             "cell_updates_per_s": cell_updates / el, "field_min_mM": cmin}
 
 
+def adjoint_run(n: int = 1 << 18) -> dict:
+    """Gradients of one implicit nutrient step on 257 x 257 x 65 at 2 um with
+    the sink of n cells (a dome of radius 300 um, rates = base x
+    lognormal(0, 0.3)): loss sum (c - c*(r))^2, c* = 20 (1 - 0.8 exp(-r/150))
+    mM, with respect to every cell's rate, D and K through
+    torch.autograd; checked by central differences (h = 1e-6, 1e-4 and 1e-2
+    relative; the loss difference summed from the two fields' difference)
+    for D, K and the three cells of largest gradient, and the
+    mixed-precision gradient against the FP64 one."""
+    import torch
+
+    from baccolonygrowth_amd.adjoint import nutrient_step
+    from baccolonygrowth_amd.colony import dome_colony
+    from baccolonygrowth_amd.nutrients import Nutrients3D, default_rd3d_params
+
+    cells = dome_colony(n, radius=300.0, height=20.0, seed=4)
+    rng = np.random.default_rng(4)
+    cm = np.ascontiguousarray((cells["p"] + cells["q"]) * 0.5)
+    base = Q_CELL / 3.0 * cells["Length"] * rng.lognormal(0.0, 0.3, n)
+    nz, ny, nx = STEP_SHAPE
+    half = (nx - 1) * STEP_H / 2
+    origin = (-half, -half, 0.0)
+    D0, K0 = 600.0, 0.1
+    prm = default_rd3d_params(nx=nx, ny=ny, nz=nz, h=STEP_H, D=D0, dt=10.0, K=K0, c_res=20.0, origin=origin,
+                              max_levels=STEP_LEVELS)
+    target = torch.from_numpy(radial_target(STEP_SHAPE, STEP_H, origin))
+    c0 = torch.full(STEP_SHAPE, 20.0, dtype=torch.float64)
+    out = {"cells": n, "grid": [nx, ny, nz]}
+    with Nutrients3D(prm) as s:
+        inside = s.deposit(cm, base)
+
+        def loss(rates, D, K, mode):
+            c = nutrient_step(rates, D, K, c0, s, cm, mode, 1e-12 if mode == "fp64" else 1e-11)
+            return ((c - target) ** 2).sum()
+
+        grads = {}
+        for mode in ("fp64", "mixed"):
+            r = torch.tensor(base, requires_grad=True)
+            Dt = torch.tensor(D0, dtype=torch.float64, requires_grad=True)
+            Kt = torch.tensor(K0, dtype=torch.float64, requires_grad=True)
+            ta = time.perf_counter()
+            L = loss(r, Dt, Kt, mode)
+            tf = time.perf_counter()
+            L.backward()
+            tb = time.perf_counter()
+            grads[mode] = (r.grad.numpy().copy(), Dt.grad.item(), Kt.grad.item())
+            out[f"{mode}_forward_s"] = tf - ta
+            out[f"{mode}_backward_s"] = tb - tf
+            log(f"[adjoint] {mode}: forward {tf - ta:.2f} s, backward {tb - tf:.2f} s (host copies included)")
+        gr, gD, gK = grads["fp64"]
+        out["mixed_vs_fp64_rel"] = {
+            "rates": float(np.abs(grads["mixed"][0] - gr).max() / np.abs(gr).max()),
+            "D": abs(grads["mixed"][1] - gD) / abs(gD),
+            "K": abs(grads["mixed"][2] - gK) / abs(gK),
+        }
+        pick = np.argsort(-np.abs(gr))[:3]
+
+        def field(rates, D, K):
+            return nutrient_step(torch.from_numpy(rates), torch.tensor(D, dtype=torch.float64),
+                                 torch.tensor(K, dtype=torch.float64), c0, s, cm, "fp64", 1e-12)
+
+        def dloss(a, b):
+            # L(a) - L(b) as sum (a - b)(a + b - 2 c*): no cancellation of the two ~1e8 sums
+            return float(((a - b) * (a + b - 2.0 * target)).sum())
+
+        fd = {}
+        with torch.no_grad():
+            # one cell's rate moves the field by little more than the solves'
+            # 1e-12 residual at h = 1e-6: larger steps (second-order central
+            # differences) lift the signal above it
+            for h in (1e-6, 1e-4, 1e-2):
+                e = {}
+                for name, v0, g in (("D", D0, gD), ("K", K0, gK)):
+                    up = field(base, D0 * (1 + h) if name == "D" else D0, K0 * (1 + h) if name == "K" else K0)
+                    dn = field(base, D0 * (1 - h) if name == "D" else D0, K0 * (1 - h) if name == "K" else K0)
+                    e[name] = abs(dloss(up, dn) / (2 * h * v0) - g) / abs(g)
+                for i in pick:
+                    ru, rd = base.copy(), base.copy()
+                    ru[i] *= 1 + h
+                    rd[i] *= 1 - h
+                    e[f"rate[{int(i)}]"] = abs(dloss(field(ru, D0, K0), field(rd, D0, K0)) / (2 * h * base[i]) - gr[i]) / abs(gr[i])
+                fd[f"h={h:g}"] = e
+                log(f"[adjoint] central differences h={h:g}: " + ", ".join(f"{k} {v:.2e}" for k, v in e.items()))
+    out.update({"cells_in_grid": int(inside), "gradient": {"D": gD, "K": gK, "rates_max_abs": float(np.abs(gr).max())},
+                "fd_rel_error": fd})
+    return out
+
+
 RUN_CELLS = 1 << 22
 RUN_RADIUS = 1200.0
 
@@ -500,7 +588,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mg", action="store_true")
     ap.add_argument("--no-env", action="store_true")
-    ap.add_argument("--only", choices=("mg", "env", "coupled", "small"), default=None,
+    ap.add_argument("--only", choices=("mg", "env", "coupled", "small", "adjoint"), default=None,
                     help="run one secondary section alone and print its JSON object (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -512,6 +600,9 @@ def main():
         return
     if args.only == "mg":
         print(json.dumps(mg_bench()), flush=True)
+        return
+    if args.only == "adjoint":
+        print(json.dumps({"adjoint_run": adjoint_run()}), flush=True)
         return
     if args.only == "small":
         print(json.dumps({"small_run": small_run()}), flush=True)
