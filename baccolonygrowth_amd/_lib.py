@@ -46,6 +46,7 @@ SIGNATURES = {
     "bcg_rebuild_neighbours": (ctypes.c_int, [_vp, _ip, ctypes.c_int32]),
     "bcg_get_neighbours": (ctypes.c_int, [_vp, _ip, _ip, ctypes.c_int32]),
     "bcg_sum_forces": (ctypes.c_int, [_vp, _dp, _dp]),
+    "bcg_mean_stress": (ctypes.c_int, [_vp, _dp, _dp]),
     "bcg_step": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int32, _ip, _ip, ctypes.c_int32]),
     "bcg_last_step_stats": (ctypes.c_int, [_vp, _i64p, _ip, _ip]),
     "bcg_last_step_times": (ctypes.c_int, [_vp, _fp]),

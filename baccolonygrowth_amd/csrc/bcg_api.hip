@@ -257,6 +257,7 @@ int bcg_create(const bcg_params* p, bcg_ctx** out) {
     A(&c->gr2, c->nbin);
     A(&c->Fbuf, 3 * M);
     A(&c->Tbuf, 3 * M);
+    A(&c->Sbuf, 5 * M);
     A(&c->stats, 1);
     if (!rc && hipHostMalloc((void**)&c->hstats, sizeof(BcgStats), hipHostMallocDefault) != hipSuccess)
         rc = BCG_ERR_HIP;
@@ -299,7 +300,7 @@ int bcg_destroy(bcg_ctx* c) {
                     c->age_q,    c->type,      c->ancestor, c->key,       c->bin_cnt,   c->bin_start, c->bin_slot,
                     c->sorted,   c->scattered, c->scan_tot, c->cmL, c->cand,  c->nl,      c->nl_count,   c->order,     c->slot_of,  c->divflag,
                     c->div_ids,  c->div_index_of, c->ids_tmp, c->draws,   c->mother_pre, c->height,  c->normal,
-                    c->wall,     c->gr1,       c->gr2,     c->Fbuf,       c->Tbuf,      c->stats};
+                    c->wall,     c->gr1,       c->gr2,     c->Fbuf,       c->Tbuf,      c->Sbuf,      c->stats};
     for (void* q : ptrs)
         if (q) hipFree(q);
     if (c->hstats) hipHostFree(c->hstats);
@@ -540,6 +541,28 @@ int bcg_sum_forces(bcg_ctx* c, double* F, double* T) {
     if (c->N > 0) {
         HIPCHK(c, hipMemcpy(F, c->Fbuf, 3 * sizeof(double) * c->N, hipMemcpyDeviceToHost));
         HIPCHK(c, hipMemcpy(T, c->Tbuf, 3 * sizeof(double) * c->N, hipMemcpyDeviceToHost));
+    }
+    return BCG_OK;
+}
+
+int bcg_mean_stress(bcg_ctx* c, double* stress, double* fnet) {
+    if (!c || (c->N > 0 && (!stress || !fnet))) return BCG_ERR_ARG;
+    HIPCHK(c, ctx_join(c));
+    if (!c->lists_valid && c->N > 0) return fail(c, BCG_ERR_STATE, "no neighbour lists: rebuild first");
+    int rc;
+    if ((rc = reset_stats(c))) return rc;
+    if (c->N > 0)
+        hipLaunchKernelGGL(k_mean_stress, dim3(nblk(c->N, MOVE_BLOCK)), dim3(MOVE_BLOCK), 0, c->stream,
+                           c->hot[c->cur], c->N, c->P, c->order, c->nl, c->nl_count, c->stride, c->height, c->normal,
+                           c->wall, c->Sbuf, c->Fbuf, &c->stats->wall_draws, &c->stats->err);
+    HIPCHK(c, hipGetLastError());
+    if ((rc = read_stats(c))) return rc;
+    if ((rc = check_err(c))) return rc;
+    // every F_cw call took one draw of the stream (ABM/Forces.cpp:102)
+    c->rng.discard(c->last.wall_draws);
+    if (c->N > 0) {
+        HIPCHK(c, hipMemcpy(stress, c->Sbuf, 5 * sizeof(double) * c->N, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(fnet, c->Fbuf, 3 * sizeof(double) * c->N, hipMemcpyDeviceToHost));
     }
     return BCG_OK;
 }

@@ -41,6 +41,7 @@ struct bcg_ctx {
     double *draws = nullptr, *mother_pre = nullptr;
     double *height = nullptr, *normal = nullptr, *wall = nullptr, *gr1 = nullptr, *gr2 = nullptr;
     double *Fbuf = nullptr, *Tbuf = nullptr;
+    double* Sbuf = nullptr;  // mean_stress output, 5 per cell
     BcgStats* stats = nullptr;  // device
     BcgStats* hstats = nullptr; // pinned host
     int* hids = nullptr;         // pinned host: the step's dividing mothers (lineage)

@@ -144,12 +144,12 @@ __device__ __forceinline__ int bin_of(const PhysParams& P, d3 pos, int& x, int& 
     return x * (P.BoxY * P.BoxZ) + y * P.BoxZ + z;
 }
 
-// F_cc: ABM/Forces.cpp:13-84. The force F of neighbour B on A and its torque
-// Tc = (r - cmA) x F about A's centre. The kinematics come from kinA() and
-// kinB(), called only if the cells touch.
+// F_cc: ABM/Forces.cpp:13-84. The force F of neighbour B on A and its point
+// of action relative to A's centre, r - cmA. The kinematics come from kinA()
+// and kinB(), called only if the cells touch.
 template <class KA_, class KB_>
-__device__ __forceinline__ void F_cc(const PhysParams& P, const Seg& A, d3 cmA, const Seg& B, KA_ kinA, KB_ kinB, d3& F,
-                                     d3& Tc) {
+__device__ __forceinline__ void F_cc_r(const PhysParams& P, const Seg& A, d3 cmA, const Seg& B, KA_ kinA, KB_ kinB,
+                                       d3& F, d3& r) {
     const double gamma_n = 100.0;
     d3 c1, c2;
     double sigma = A.R + B.R;
@@ -163,7 +163,7 @@ __device__ __forceinline__ void F_cc(const PhysParams& P, const Seg& A, d3 cmA, 
     d3 vn = scl(sub(c2, c1), 1.0 / d);
     double FE_mag = sqrt(A.R * B.R / (A.R + B.R)) * P.k_cc * pow15(delta);
     F = scl(vn, -FE_mag);
-    d3 r = scl(add(scl(c1, B.R), scl(c2, A.R)), 1 / sigma);
+    r = scl(add(scl(c1, B.R), scl(c2, A.R)), 1 / sigma);
     if (FE_mag > DBL_EPSILON) {
         const Kin KA = kinA();
         const Kin KB = kinB();
@@ -180,6 +180,14 @@ __device__ __forceinline__ void F_cc(const PhysParams& P, const Seg& A, d3 cmA, 
         F = add(add(F, FDt), FDn);
     }
     r = sub(r, cmA);
+}
+
+// ... and its torque Tc = (r - cmA) x F about A's centre.
+template <class KA_, class KB_>
+__device__ __forceinline__ void F_cc(const PhysParams& P, const Seg& A, d3 cmA, const Seg& B, KA_ kinA, KB_ kinB, d3& F,
+                                     d3& Tc) {
+    d3 r;
+    F_cc_r(P, A, cmA, B, kinA, kinB, F, r);
     Tc = cross(r, F);
 }
 

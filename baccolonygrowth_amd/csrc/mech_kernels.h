@@ -37,6 +37,9 @@ __global__ void k_move_grow(const double* hot_old, double* hot_new, int n, PhysP
 __global__ void k_sum_forces(const double* hot, int n, PhysParams P, const int* order, const int* nl,
                              const int* nl_count, int stride, const double* H, const double* Nrm, const double* wall,
                              double* Fo, double* To, int* err);
+__global__ void k_mean_stress(const double* hot, int n, PhysParams P, const int* order, const int* nl,
+                              const int* nl_count, int stride, const double* H, const double* Nrm, const double* wall,
+                              double* S5, double* F3, unsigned long long* wall_draws, int* err);
 
 __global__ void k_flag_count(const int* flag, int n, int* tot);
 __global__ void k_flag_write(const int* flag, int n, const int* off, int* ids);

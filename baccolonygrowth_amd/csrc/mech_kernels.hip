@@ -813,6 +813,109 @@ __global__ void __launch_bounds__(MOVE_BLOCK) k_sum_forces(const double* __restr
     To[3 * (size_t)id + 2] = T.z;
 }
 
+// mean_stress (ABM/Compute.cpp:18-117) of every cell at the current state:
+// the diagonal of its stress tensor from the contact forces of its
+// neighbours (F_cc, in list order), of the agar (F_cw, one rand() draw per
+// call, counted) and of the surface tension, each times its point of action
+// relative to the centre, over the cell volume; the radial and azimuthal
+// components; and the net force. S5 = (xx, yy, zz, rr, tt), by cell ID.
+__global__ void __launch_bounds__(MOVE_BLOCK) k_mean_stress(const double* __restrict__ hot, int n, PhysParams P,
+                                                            const int* __restrict__ order, const int* __restrict__ nl,
+                                                            const int* __restrict__ nl_count, int stride,
+                                                            const double* __restrict__ H, const double* __restrict__ Nrm,
+                                                            const double* __restrict__ wall, double* __restrict__ S5,
+                                                            double* __restrict__ F3,
+                                                            unsigned long long* __restrict__ wall_draws,
+                                                            int* __restrict__ err) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int id = order[t];
+    Seg S;
+    Kin K;
+    load_seg(hot, id, S);
+    load_kin(hot, id, K);
+    const d3 cm = cm_of(S);
+    int x, y, z;
+    if (bin_of(P, cm, x, y, z) < 0) {
+        atomicOr(err, ERRBIT_DOMAIN);
+        return;
+    }
+    const double V = S.L * kPI * S.R * S.R + 4.0 / 3.0 * kPI * S.R * S.R * S.R;
+    double xx = 0, yy = 0, zz = 0;
+    d3 Fn = mk(0, 0, 0);
+    const int cnt = nl_count[t];
+    for (int k = 0; k < cnt; k++) {
+        const int j = nl[(size_t)k * stride + t];
+        Seg B;
+        load_seg(hot, j, B);
+        d3 F, r;
+        F_cc_r(
+            P, S, cm, B, [&]() { return K; },
+            [&]() {
+                Kin KB;
+                load_kin(hot, j, KB);
+                return KB;
+            },
+            F, r);
+        Fn = add(Fn, F);
+        xx -= F.x * r.x;
+        yy -= F.y * r.y;
+        zz -= F.z * r.z;
+    }
+    if (dmin(S.q.z, S.p.z) < 1.2 * S.R) {
+        const double L = S.L + 4.0 / 3 * S.R;
+        const double M = L * S.R * S.R * kPI;
+        d3 r1, r2, sf = mk(0, 0, 0), df = mk(0, 0, 0);
+        const double wall_z = wall[x * P.BoxY + y];
+        const d3 F1 = F_cw_locus(P, S, K, cm, S.p, wall_z, M, r1, sf, df);
+        const d3 F2 = F_cw_locus(P, S, K, cm, S.q, wall_z, M, r2, sf, df);
+        atomicAdd(wall_draws, 1ull);
+        Fn = add(Fn, F1);
+        r1 = sub(r1, cm);
+        xx -= F1.x * r1.x;
+        yy -= F1.y * r1.y;
+        zz -= F1.z * r1.z;
+        Fn = add(Fn, F2);
+        r2 = sub(r2, cm);
+        xx -= F2.x * r2.x;
+        yy -= F2.y * r2.y;
+        zz -= F2.z * r2.z;
+    }
+    d3 Fs, Ts;
+    F_surf_tension(P, S, cm, H, Nrm, Fs, Ts);
+    Fn = add(Fn, Fs);
+    d3 r = sub(S.p, cm);
+    r.z += S.R;
+    d3 r2 = sub(S.q, cm);
+    r2.z += S.R;
+    xx -= Fs.x * r.x * 0.5 - Fs.x * r2.x * 0.5;
+    yy -= Fs.y * r.y * 0.5 - Fs.y * r2.y * 0.5;
+    zz -= Fs.z * r.z * 0.5 - Fs.z * r2.z * 0.5;
+    xx /= V;
+    yy /= V;
+    zz /= V;
+    double rr, tt;
+    const double Lcm = sqrt(cm.x * cm.x + cm.y * cm.y);
+    if (Lcm > DBL_EPSILON) {
+        const double s = 1.0 / Lcm;
+        const double ux = cm.x * s, uy = cm.y * s;
+        rr = xx * ux + yy * uy;
+        tt = -xx * uy + yy * ux;
+    } else {
+        rr = xx + yy;
+        tt = 0.0;
+    }
+    double* o = S5 + 5 * (size_t)id;
+    o[0] = xx;
+    o[1] = yy;
+    o[2] = zz;
+    o[3] = rr;
+    o[4] = tt;
+    F3[3 * (size_t)id] = Fn.x;
+    F3[3 * (size_t)id + 1] = Fn.y;
+    F3[3 * (size_t)id + 2] = Fn.z;
+}
+
 // ----------------------------------------------------------- compact ---
 // Per-block count of flagged cells (wave ballot + popcount).
 __global__ void __launch_bounds__(256) k_flag_count(const int* __restrict__ flag, int n, int* __restrict__ tot) {

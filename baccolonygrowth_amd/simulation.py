@@ -143,6 +143,16 @@ class Colony:
         self._c(lib().bcg_sum_forces(self._h, F.ctypes.data_as(_dp), T.ctypes.data_as(_dp)))
         return F, T
 
+    def mean_stress(self):
+        """mean_stress of every cell as the reference's output step computes
+        it: (stress[N, 5] = xx, yy, zz, rr, tt; Fnet[N, 3]). Takes the rand()
+        draws of its F_cw calls."""
+        n = self.n
+        S = np.zeros((n, 5))
+        F = np.zeros((n, 3))
+        self._c(lib().bcg_mean_stress(self._h, S.ctypes.data_as(_dp), F.ctypes.data_as(_dp)))
+        return S, F
+
     def step(self, dt: float, t: float, update: bool = False, no_patch: bool = False, lineage_cap: int = 1 << 16):
         """One mechanics step; returns the (parent, daughter) pairs."""
         flags = (STEP_REBUILD if update else 0) | (STEP_NO_PATCH if no_patch else 0)

@@ -143,6 +143,13 @@ int bcg_get_neighbours(bcg_ctx* ctx, int32_t* counts, int32_t* lists, int32_t st
 /* sum_forces (ABM/Forces.cpp:332-381) on the current cells against their
  * current lists: F[3N], T[3N]. Diagnostic: consumes no rand() draws.       */
 int bcg_sum_forces(bcg_ctx* ctx, double* F, double* T);
+/* mean_stress (ABM/Compute.cpp:18-117), as the output step calls it for
+ * every cell (ABM/Simulation.cpp:320-326), on the current cells against
+ * their current lists: stress[5N] = (xx, yy, zz, rr, tt) per cell, the
+ * diagonal of the contact stress tensor over the cell volume and its radial
+ * and azimuthal components; fnet[3N] the net contact force. Each F_cw call
+ * takes one rand() draw of the context's stream, as in the reference.      */
+int bcg_mean_stress(bcg_ctx* ctx, double* stress, double* fnet);
 
 /* One mechanics step = the body of the reference time loop after the field
  * update (ABM/Simulation.cpp:330-413):

@@ -630,6 +630,74 @@ void or_sim_sum_forces(or_sim* s, double* F, double* T) {
     s->wall_draws = saved;
 }
 
+/* mean_stress: ABM/Compute.cpp:18-117, every cell as the output step calls
+ * it (ABM/Simulation.cpp:320-326); F_cw takes its rand() draws.            */
+void or_sim_mean_stress(or_sim* s, double* S5, double* F3) {
+    const bcg_params* pr = &s->p;
+    for (int i = 0; i < s->N; i++) {
+        const bcg_cell* c = &s->old_cells[i];
+        v3 cm = cmof(c);
+        int x, y, z;
+        get_address(pr, cm, &x, &y, &z);
+        double R = c->Radius;
+        double V = c->Length * PI_ * R * R + 4.0 / 3.0 * PI_ * R * R * R;
+        double xx = 0, yy = 0, zz = 0;
+        v3 Fn = mk(0, 0, 0), F, F2, r, r2, T, sf, df;
+        double d;
+        const int* nl = s->nl + (size_t)i * s->cap;
+        for (int k = 0; k < s->nl_count[i]; k++) {
+            F_cc(pr, c, &s->old_cells[nl[k]], &F, &r, &d);
+            Fn = vadd(Fn, F);
+            r = vsub(r, cm);
+            xx -= F.x * r.x;
+            yy -= F.y * r.y;
+            zz -= F.z * r.z;
+        }
+        if (dmin(c->q[2], c->p[2]) < 1.2 * R) {
+            double wall_z = s->f.wall[x * pr->BoxY + y];
+            F_cw(pr, c, wall_z, &F, &F2, &r, &r2, &sf, &df);
+            Fn = vadd(Fn, F);
+            r = vsub(r, cm);
+            xx -= F.x * r.x;
+            yy -= F.y * r.y;
+            zz -= F.z * r.z;
+            Fn = vadd(Fn, F2);
+            r2 = vsub(r2, cm);
+            xx -= F2.x * r2.x;
+            yy -= F2.y * r2.y;
+            zz -= F2.z * r2.z;
+        }
+        F_surf_tension(pr, c, &s->f, &F, &T);
+        Fn = vadd(Fn, F);
+        r = vsub(P(c), cm);
+        r.z += R;
+        r2 = vsub(Q(c), cm);
+        r2.z += R;
+        xx -= F.x * r.x * 0.5 - F.x * r2.x * 0.5;
+        yy -= F.y * r.y * 0.5 - F.y * r2.y * 0.5;
+        zz -= F.z * r.z * 0.5 - F.z * r2.z * 0.5;
+        xx /= V;
+        yy /= V;
+        zz /= V;
+        double rr, tt, Lcm = sqrt(cm.x * cm.x + cm.y * cm.y);
+        if (Lcm > DBL_EPSILON) {
+            double ux = cm.x * (1.0 / Lcm), uy = cm.y * (1.0 / Lcm);
+            rr = xx * ux + yy * uy;
+            tt = -xx * uy + yy * ux;
+        } else {
+            rr = xx + yy;
+            tt = 0.0;
+        }
+        double* o = S5 + 5 * (size_t)i;
+        o[0] = xx;
+        o[1] = yy;
+        o[2] = zz;
+        o[3] = rr;
+        o[4] = tt;
+        st(F3 + 3 * i, Fn);
+    }
+}
+
 /* One step of ABM/Simulation.cpp:330-413 (single thread). */
 int or_sim_step(or_sim* s, double dt, double t, int rebuild, int* lineage, int lineage_cap) {
     const bcg_params* pr = &s->p;

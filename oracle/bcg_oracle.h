@@ -44,6 +44,7 @@ int or_sim_rebuild_ids(or_sim* s, const int* ids, int n);
 void or_sim_get_neighbours(or_sim* s, int* counts, int* lists, int stride);
 void or_sim_get_boxes(or_sim* s, int* counts, int* ids, int stride);
 void or_sim_sum_forces(or_sim* s, double* F, double* T);
+void or_sim_mean_stress(or_sim* s, double* S5, double* F3);
 /* returns divisions performed (>= 0) or a BCG_ERR_* code */
 int or_sim_step(or_sim* s, double dt, double t, int rebuild, int* lineage, int lineage_cap);
 int64_t or_sim_wall_draws(or_sim* s);

@@ -144,6 +144,16 @@ class Sim:
         self.o._f("sim_sum_forces")(self.h, F.ctypes.data_as(_dp), T.ctypes.data_as(_dp))
         return F, T
 
+
+    def mean_stress(self):
+        """mean_stress of every cell (stress[N, 5], Fnet[N, 3]); takes the
+        rand() draws of its F_cw calls."""
+        n = self.n
+        S = np.zeros((n, 5))
+        F = np.zeros((n, 3))
+        self.o._f("sim_mean_stress")(self.h, S.ctypes.data_as(_dp), F.ctypes.data_as(_dp))
+        return S, F
+
     def step(self, dt: float, t: float, rebuild: bool, lineage_cap: int = 100000):
         lin = np.zeros((lineage_cap, 2), np.int32)
         nd = self.o._f("sim_step")(
@@ -169,6 +179,7 @@ def _declare(lib, prefix):
     f("sim_get_neighbours").argtypes = [_vp, _ip, _ip, ctypes.c_int]
     f("sim_get_boxes").argtypes = [_vp, _ip, _ip, ctypes.c_int]
     f("sim_sum_forces").argtypes = [_vp, _dp, _dp]
+    f("sim_mean_stress").argtypes = [_vp, _dp, _dp]
     f("sim_step").argtypes = [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _ip, ctypes.c_int]
     f("sim_step").restype = ctypes.c_int
     f("F_cc").argtypes = [ctypes.POINTER(Params), _vp, _vp, _dp]

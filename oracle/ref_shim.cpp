@@ -16,6 +16,7 @@
 #include "Array.h"
 #include "Cell.h"
 #include "Cells.h"
+#include "Compute.h"
 #include "Constants.h"
 #include "Forces.h"
 #include "Integrate.h"
@@ -240,6 +241,20 @@ void ref_sim_sum_forces(void* h, double* F, double* T) {
                    *s->Wall, sf, df);
         F[3 * i] = f.x; F[3 * i + 1] = f.y; F[3 * i + 2] = f.z;
         T[3 * i] = t.x; T[3 * i + 1] = t.y; T[3 * i + 2] = t.z;
+    }
+}
+
+// mean_stress (ABM/Compute.cpp:18-117) of every cell, as the output step
+// calls it (ABM/Simulation.cpp:320-326).
+void ref_sim_mean_stress(void* h, double* S5, double* F3) {
+    RefSim* s = (RefSim*)h;
+    set_globals(&s->prm);
+    for (int i = 0; i < s->N; i++) {
+        Tensor st;
+        DoubleCoord fn;
+        mean_stress(s->old_cells[i], s->old_cells, s->NL[i], *s->grid, *s->Wall, *s->Height, *s->Normal, st, fn);
+        S5[5 * i] = st.xx; S5[5 * i + 1] = st.yy; S5[5 * i + 2] = st.zz; S5[5 * i + 3] = st.rr; S5[5 * i + 4] = st.tt;
+        F3[3 * i] = fn.x; F3[3 * i + 1] = fn.y; F3[3 * i + 2] = fn.z;
     }
 }
 

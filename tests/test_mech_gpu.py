@@ -395,3 +395,35 @@ def test_lists_at_the_predicate_thresholds(Colony, oracle):
     say(f"    kept: crossing family {int((kept[0] > 0).sum())}/{kept.shape[1]}, parallel family {int((kept[1] > 0).sum())}/{kept.shape[1]}")
     for fam in kept:
         assert 0 < (fam > 0).sum() < len(fam)
+
+
+@pytest.mark.parametrize("case", ["planar", "3-D"])
+def test_mean_stress_vs_oracle(Colony, oracle, case):
+    """mean_stress of every cell (the output step's stress tensor and net
+    force) against the oracle, on a planar state and on the 3-D dome; the
+    F_cw draws leave both streams at the same position."""
+    if case == "planar":
+        p, t, dt, cells, fields = planar_case(5, seed=56, divide_frac=0.2)
+    else:
+        p, cells, fields = _dome_case(n=8000, seed=3)
+    say(f"[gpu] mean_stress {case}, {len(cells)} cells")
+    o = oracle.sim(p)
+    o.set_cells(cells)
+    o.set_fields(*fields)
+    o.rebuild()
+    oracle.srand(7)
+    Sa, Fa = o.mean_stress()
+    ra = oracle.rand()
+    o.close()
+    with Colony(p) as col:
+        col.LoadCells(cells)
+        col.set_fields(*fields)
+        col.getNeighbours()
+        col.srand(7)
+        Sb, Fb = col.mean_stress()
+        rb = col.rand()
+    assert ra == rb
+    for name, a, b in (("stress", Sa, Sb), ("Fnet", Fa, Fb)):
+        err = np.abs(b - a).max() / np.abs(a).max()
+        say(f"    {case} {name} max rel error {err:.2e}")
+        assert err <= 1e-10

@@ -180,6 +180,29 @@ def test_oracle_vs_reference_library(oracle, reflib):
     assert ra == rb
 
 
+def test_mean_stress_oracle_vs_reference(oracle, reflib):
+    """mean_stress of every cell, as the output step computes it: the
+    restatement and the reference code agree bit for bit, and take the same
+    rand() draws (one per F_cw call)."""
+    p, t, dt, cells, fields = planar_case(5, seed=56, divide_frac=0.2)
+    out = []
+    for lib in (oracle, reflib):
+        s = lib.sim(p)
+        s.set_cells(cells)
+        s.set_fields(*fields)
+        s.rebuild()
+        lib.srand(7)
+        S, F = s.mean_stress()
+        out.append((S, F, lib.rand()))
+        s.close()
+    (Sa, Fa, ra), (Sb, Fb, rb) = out
+    assert Sa.tobytes() == Sb.tobytes() and Fa.tobytes() == Fb.tobytes()
+    assert ra == rb
+    assert np.abs(Sa[:, :3]).max() > 0 and np.abs(Fa).max() > 0
+    oracle.srand(7)
+    assert ra != oracle.rand()  # the F_cw calls took draws
+
+
 # ------------------------------------------- 3-D generalisation pinning ---
 def _embed_3d(p, cells, fields, ny=5):
     """A planar state inside a 3-D grid (BoxY = ny) with y-constant fields."""
