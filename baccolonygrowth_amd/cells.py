@@ -228,6 +228,28 @@ def load_restart(path: str):
     return float(t), float(dt), cells
 
 
+# One line of the Cells/<k>.txt output (ABM/InputOutput.cpp:799-804): t, ID,
+# Type, p, q, Length, the stress diagonal xx yy zz, Velocity, GrowthRate,
+# DynFric, StaFric, time_p, time_q, age_p, age_q, Ancestor.
+CELLS_LINE = ("%4.4f %d %d %4.4f %4.4f %4.4f %4.4f %4.4f %4.4f %4.4f %4.4f %4.4E %4.4E %4.4E %4.4E %4.4E %4.4E "
+              "%4.4f %4.4f %4.4f %4.4f %4.4f %4.4f %4.4f %4.4f %d %d %d\n")
+
+
+def write_cells_text(path: str, t: float, cells: np.ndarray, stress: np.ndarray) -> None:
+    """The output step's cell file (ABM/Simulation.cpp:320-326 with
+    ABM/InputOutput.cpp:799-804): one line per cell in ID order, with the
+    stress diagonal of mean_stress (stress[:, 0:3], e.g. from
+    Colony.mean_stress)."""
+    with open(path, "w") as fh:
+        for i, c in enumerate(cells):
+            p, q, v = c["p"], c["q"], c["Velocity"]
+            df, sf = c["DynFric"], c["StaFric"]
+            fh.write(CELLS_LINE % (t, i, c["Type"], p[0], p[1], p[2], q[0], q[1], q[2], c["Length"], stress[i, 0],
+                                   stress[i, 1], stress[i, 2], v[0], v[1], v[2], c["GrowthRate"], df[0], df[1], df[2],
+                                   sf[0], sf[1], sf[2], c["time_p"], c["time_q"], c["age_p"], c["age_q"],
+                                   c["Ancestor"]))
+
+
 def save_restart(path: str, cells: np.ndarray, t: float, dt: float) -> None:
     """Write a Restart binary as SaveCells does (InputOutput.cpp:434-448)."""
     cells = np.ascontiguousarray(cells, dtype=CELL_DTYPE)

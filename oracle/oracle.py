@@ -154,6 +154,14 @@ class Sim:
         self.o._f("sim_mean_stress")(self.h, S.ctypes.data_as(_dp), F.ctypes.data_as(_dp))
         return S, F
 
+    def output_cells(self, path: str, t: float) -> None:
+        """The reference's own cell file of an output step (reference
+        library only)."""
+        f = self.o._f("sim_output_cells")
+        f.argtypes = [_vp, ctypes.c_char_p, ctypes.c_double]
+        if f(self.h, path.encode(), ctypes.c_double(t)) != 0:
+            raise OSError(f"cannot write {path}")
+
     def step(self, dt: float, t: float, rebuild: bool, lineage_cap: int = 100000):
         lin = np.zeros((lineage_cap, 2), np.int32)
         nd = self.o._f("sim_step")(

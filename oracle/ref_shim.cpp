@@ -16,7 +16,6 @@
 #include "Array.h"
 #include "Cell.h"
 #include "Cells.h"
-#include "Compute.h"
 #include "Constants.h"
 #include "Forces.h"
 #include "Integrate.h"
@@ -25,6 +24,7 @@
 #include "grow.h"
 #include "tools.h"
 #include "Compute.h"
+#include "InputOutput.h"
 #include "Multigrid.h"
 #include "Nutrients.h"
 
@@ -256,6 +256,23 @@ void ref_sim_mean_stress(void* h, double* S5, double* F3) {
         S5[5 * i] = st.xx; S5[5 * i + 1] = st.yy; S5[5 * i + 2] = st.zz; S5[5 * i + 3] = st.rr; S5[5 * i + 4] = st.tt;
         F3[3 * i] = fn.x; F3[3 * i + 1] = fn.y; F3[3 * i + 2] = fn.z;
     }
+}
+
+// The output step's cell file: mean_stress and Output of every cell
+// (ABM/Simulation.cpp:320-326) into `path`.
+int ref_sim_output_cells(void* h, const char* path, double t) {
+    RefSim* s = (RefSim*)h;
+    set_globals(&s->prm);
+    FILE* f = fopen(path, "w");
+    if (!f) return -1;
+    for (int i = 0; i < s->N; i++) {
+        Tensor st;
+        DoubleCoord fn;
+        mean_stress(s->old_cells[i], s->old_cells, s->NL[i], *s->grid, *s->Wall, *s->Height, *s->Normal, st, fn);
+        Output(f, i, t, s->old_cells[i], st);
+    }
+    fclose(f);
+    return 0;
 }
 
 // One step, ABM/Simulation.cpp:261-413 without the field update and output.

@@ -203,6 +203,31 @@ def test_mean_stress_oracle_vs_reference(oracle, reflib):
     assert ra != oracle.rand()  # the F_cw calls took draws
 
 
+def test_cells_text_output_matches_reference(oracle, reflib, tmp_path):
+    """The output step's cell file written from the oracle's cells and
+    mean_stress equals the one the reference's own mean_stress and Output
+    write for the same state, byte for byte."""
+    from baccolonygrowth_amd.cells import write_cells_text
+
+    p, t, dt, cells, fields = planar_case(5, seed=57, divide_frac=0.2)
+    sims = [oracle.sim(p), reflib.sim(p)]
+    for s, lib in zip(sims, (oracle, reflib)):
+        s.set_cells(cells)
+        s.set_fields(*fields)
+        s.rebuild()
+        lib.srand(11)  # one libc stream: seeded before each library's step
+        s.step(dt, t, rebuild=False)  # velocities and frictions of a step
+    ref_path, ours_path = tmp_path / "ref.txt", tmp_path / "ours.txt"
+    sims[1].output_cells(str(ref_path), t + dt)
+    S, _ = sims[0].mean_stress()
+    write_cells_text(str(ours_path), t + dt, sims[0].cells(), S)
+    ref, ours = ref_path.read_bytes(), ours_path.read_bytes()
+    assert ref.count(b"\n") == sims[1].n >= len(cells) and len(ref.split(b"\n")[0].split()) == 28
+    assert ours == ref
+    for s in sims:
+        s.close()
+
+
 # ------------------------------------------- 3-D generalisation pinning ---
 def _embed_3d(p, cells, fields, ny=5):
     """A planar state inside a 3-D grid (BoxY = ny) with y-constant fields."""
