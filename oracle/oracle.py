@@ -299,6 +299,14 @@ class RefEnv:
     def fields(self, pass_: int):
         self.ref.lib.ref_env_fields(self.h, pass_)
 
+    def output(self, density_path: str, env_path: str) -> None:
+        """The reference's own density and environment files."""
+        f = self.ref.lib.ref_env_output
+        f.argtypes = [_vp, ctypes.c_char_p, ctypes.c_char_p]
+        f.restype = ctypes.c_int
+        if f(self.h, density_path.encode(), env_path.encode()) != 0:
+            raise OSError("cannot write the output files")
+
     def nutrients(self) -> int:
         return self.ref.lib.ref_env_nutrients(self.h)
 

@@ -427,6 +427,22 @@ void* ref_env_create(const bcg_env_params* p, void* sim) {
     return e;
 }
 
+// The output step's density and environment files (ABM/Simulation.cpp:198,
+// 221): the reference's own writers.
+int ref_env_output(void* h, const char* density_path, const char* env_path) {
+    RefEnv* e = (RefEnv*)h;
+    env_globals(e);
+    FILE* f = fopen(density_path, "w");
+    if (!f) return -1;
+    e->DS->Output(f, 1);
+    fclose(f);
+    f = fopen(env_path, "w");
+    if (!f) return -1;
+    e->sim->Env->Output(f, 1);
+    fclose(f);
+    return 0;
+}
+
 void ref_env_destroy(void* h) {
     RefEnv* e = (RefEnv*)h;
     env_globals(e);

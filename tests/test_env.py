@@ -149,6 +149,34 @@ def test_reference_reproduces_golden_env():
     sim.close()
 
 
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(GOLDEN), "..", "oracle", "_ref", "libbcgref.so")),
+                    reason="oracle/_ref (the reference compiled from source) is not built here")
+def test_density_and_env_files_match_reference(tmp_path):
+    """The output step's density and environment files written from the
+    state arrays equal the reference's own writers' files, byte for byte."""
+    from baccolonygrowth_amd.environment import write_density_text, write_env_text
+    from oracle.oracle import RefEnv, RefLib
+
+    g = np.load(GOLDEN_ENV)
+    p, ep = _params(g, 1e-4)
+    ref = RefLib()
+    _, _, cells = restart_cells(5)
+    sim = ref.sim(p)
+    sim.set_cells(cells)
+    env = RefEnv(ref, sim, ep)
+    env.fields(0)
+    env.nutrients()
+    env.output(str(tmp_path / "ref_density.txt"), str(tmp_path / "ref_env.txt"))
+    f, st = env.get_fields(), env.get_state()
+    write_density_text(str(tmp_path / "density.txt"), f["density_shift"], ep.BoxX, ep.BoxZ)
+    write_env_text(str(tmp_path / "env.txt"), st["colony"], ep.BoxX, ep.BoxZ)
+    env.close()
+    sim.close()
+    for a, b in (("ref_density.txt", "density.txt"), ("ref_env.txt", "env.txt")):
+        ra, rb = (tmp_path / a).read_bytes(), (tmp_path / b).read_bytes()
+        assert len(ra) > 1000 and ra == rb, a
+
+
 # ------------------------------------------------------------------ GPU
 def _start_fields(col, p):
     """Height = cellRadius, normal = (0, 0, -1): the reference's initial
