@@ -467,8 +467,20 @@ int setup_system(bcg_rd3d* c, bool f32, const double* x = nullptr) {
     for (size_t l = 0; l + 1 < c->L.size(); l++) {
         Level& v = c->L[l];
         Level& w = c->L[l + 1];
-        hipLaunchKernelGGL((k_restrict_a0<double>), slab_grid(w), dim3(64, 4), 0, c->stream, v.a064, w.a064, v.nx, v.ny, v.nz,
-                           w.nx, w.ny, w.nz, v.cx, v.cy, v.cz);
+        auto launch = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, slab_grid(w), dim3(64, 4), 0, c->stream, v.a064, w.a064, v.nx, v.ny, v.nz, w.nx,
+                               w.ny, w.nz);
+        };
+        switch ((v.cx ? 4 : 0) | (v.cy ? 2 : 0) | (v.cz ? 1 : 0)) {
+            case 0: launch(k_restrict_a0<double, 0, 0, 0>); break;
+            case 1: launch(k_restrict_a0<double, 0, 0, 1>); break;
+            case 2: launch(k_restrict_a0<double, 0, 1, 0>); break;
+            case 3: launch(k_restrict_a0<double, 0, 1, 1>); break;
+            case 4: launch(k_restrict_a0<double, 1, 0, 0>); break;
+            case 5: launch(k_restrict_a0<double, 1, 0, 1>); break;
+            case 6: launch(k_restrict_a0<double, 1, 1, 0>); break;
+            default: launch(k_restrict_a0<double, 1, 1, 1>); break;
+        }
         if (f32)
             hipLaunchKernelGGL((k_cast<double, float>), dim3(nblk(w.n(), 256)), dim3(256), 0, c->stream, w.a064, w.a032,
                                w.n());

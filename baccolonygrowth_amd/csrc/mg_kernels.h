@@ -63,9 +63,10 @@ __global__ void k_sum_parts(const double* part, int n, double* out);
 __global__ void k_rhs(const double* c, double* b, double c_res, long nxy, long n);
 __global__ void k_setup_fine(const double* x, double* cprev, const double* q, double K, double dt, double c_res,
                              long nxy, double* a0, float* a0f, double* b, long n);
-template <class T>
-__global__ void k_restrict_a0(const T* a0f, T* a0c, int nx, int ny, int nz, int ncx, int ncy, int ncz, int cx, int cy,
-                              int cz);
+// Coarsening factors (0 or 1 per axis) as template arguments: the 27-point
+// loop unrolls.
+template <class T, int cx, int cy, int cz>
+__global__ void k_restrict_a0(const T* a0f, T* a0c, int nx, int ny, int nz, int ncx, int ncy, int ncz);
 
 __global__ void k_adj_rhs(const double* g, double* b, int nx, int ny, int nz);
 __global__ void k_adj_grad(const double* mu, const double* c1, const double* c0, const double* q, double K, double dt,

@@ -414,9 +414,9 @@ __global__ void k_rhs(const double* __restrict__ c, double* __restrict__ b, doub
 }
 
 // Coarse-level a0 from the fine one: a0_c - 1 = restricted (a0_f - 1).
-template <class T>
+template <class T, int cx, int cy, int cz>
 __global__ void __launch_bounds__(256) k_restrict_a0(const T* __restrict__ a0f, T* __restrict__ a0c, int nx, int ny,
-                                                     int nz, int ncx, int ncy, int ncz, int cx, int cy, int cz) {
+                                                     int nz, int ncx, int ncy, int ncz) {
     int bx, by, bz;
     slab_tile((ncx + 63) / 64, (ncy + 3) / 4, bx, by, bz);
     const int I = bx * blockDim.x + threadIdx.x;
@@ -426,13 +426,16 @@ __global__ void __launch_bounds__(256) k_restrict_a0(const T* __restrict__ a0f, 
     const int i0 = cx ? 2 * I : I, j0 = cy ? 2 * J : J, k0 = cz ? 2 * K : K;
     const long sxy = (long)nx * ny;
     T acc = T(0), wsum = T(0);
+#pragma unroll
     for (int c = -cz; c <= cz; c++) {
         const int kk = k0 + c;
         if (kk < 0 || kk >= nz) continue;
         const T wz = (c == 0) ? T(cz ? 0.5 : 1.0) : T(0.25);
+#pragma unroll
         for (int b = -cy; b <= cy; b++) {
             const int jj = mirror(j0 + b, ny);
             const T wy = (b == 0) ? T(cy ? 0.5 : 1.0) : T(0.25);
+#pragma unroll
             for (int a = -cx; a <= cx; a++) {
                 const int ii = mirror(i0 + a, nx);
                 const T wx = (a == 0) ? T(cx ? 0.5 : 1.0) : T(0.25);
@@ -841,8 +844,17 @@ template __global__ void k_cast<float, double>(const float*, double*, long);
 template __global__ void k_cast<double, float>(const double*, float*, long);
 template __global__ void k_sumsq<float>(const float*, long, double*);
 template __global__ void k_sumsq<double>(const double*, long, double*);
-template __global__ void k_restrict_a0<float>(const float*, float*, int, int, int, int, int, int, int, int, int);
-template __global__ void k_restrict_a0<double>(const double*, double*, int, int, int, int, int, int, int, int, int);
+#define BCG_RESTRICT_A0(CX, CY, CZ) \
+    template __global__ void k_restrict_a0<double, CX, CY, CZ>(const double*, double*, int, int, int, int, int, int);
+BCG_RESTRICT_A0(0, 0, 0)
+BCG_RESTRICT_A0(0, 0, 1)
+BCG_RESTRICT_A0(0, 1, 0)
+BCG_RESTRICT_A0(0, 1, 1)
+BCG_RESTRICT_A0(1, 0, 0)
+BCG_RESTRICT_A0(1, 0, 1)
+BCG_RESTRICT_A0(1, 1, 0)
+BCG_RESTRICT_A0(1, 1, 1)
+#undef BCG_RESTRICT_A0
 
 }  // namespace mg
 }  // namespace bcg
