@@ -330,3 +330,24 @@ def dome_window_centres(radius: float, height: float, window: float, seed: int =
     cm[:, 2] += rng.uniform(0.0, 0.05, len(cm))
     inside = (np.abs(cm[:, 0]) <= window) & (np.abs(cm[:, 1]) <= window)
     return cm[inside]
+
+
+def bench_dome(n: int = 1 << 20, seed: int = 2, radius: float = 600.0, gr_base: float = 0.875,
+               dt_h: float = 1.0 / 3600.0):
+    """The kernel-bench workload: dome_colony(n, radius, height 20 um), 5 um
+    bins, lists of at most 64, room for n/8 daughters; every cell at a
+    uniformly spread point of its cycle (division length = length x
+    exp(1.585 gr_base dt_h U[1, 1800]), a steady division rate of 1 s
+    steps), growth rate gr_base everywhere, height and normal from the
+    cells and a rough wall. Returns (params, cells, (height, normal, wall,
+    gr1, gr2))."""
+    cells = dome_colony(n, radius=radius, height=20.0, seed=seed)
+    p = dome_params(cells, bin_size=5.0, max_neighbours=64, extra_cells=max(n // 8, 1024))
+    rng = np.random.default_rng(seed)
+    gr = 1.5850 * gr_base
+    cells["Ldiv"] = cells["Length"] * np.exp(gr * dt_h * rng.uniform(1.0, 1800.0, n))
+    cells["GrowthRate"] = gr_base
+    height, normal = height_normal_from_cells(p, cells)
+    wall = (rng.random((p.BoxX, p.BoxY)) - 0.5) * p.wall_rough
+    gr1 = np.full((p.BoxX, p.BoxY, p.BoxZ), gr_base)
+    return p, cells, (height, normal, wall, gr1, gr1.copy())

@@ -155,6 +155,7 @@ int set_identity_slots(bcg_ctx* c) {
 }  // namespace
 
 int bcg_bin_current(bcg_ctx* c) {
+    if (int rc_ = ctx_check_failed(c)) return rc_;
     HIPCHK(c, ctx_join(c));
     int rc = reset_stats(c);
     if (!rc) rc = build_bins(c, c->cur, c->N);
@@ -350,6 +351,7 @@ int bcg_set_cells(bcg_ctx* c, const bcg_cell* cells, int32_t n) {
     }
     c->cur = 0;
     c->N = n;
+    c->failed = false;
     auto up = [&](void* d, const void* h, size_t b) { return hipMemcpy(d, h, b, hipMemcpyHostToDevice); };
     if (n > 0) {
         HIPCHK(c, up(c->hot[0], hot.data(), hot.size() * sizeof(double)));
@@ -375,6 +377,7 @@ int bcg_set_cells(bcg_ctx* c, const bcg_cell* cells, int32_t n) {
 
 int bcg_get_cells(bcg_ctx* c, bcg_cell* cells, int32_t cap, int32_t* n) {
     if (!c || !n) return BCG_ERR_ARG;
+    if (int rc_ = ctx_check_failed(c)) return rc_;
     *n = c->N;
     if (cap < c->N || (c->N > 0 && !cells)) return fail(c, BCG_ERR_ARG, "buffer of %d cells < %d", cap, c->N);
     HIPCHK(c, ctx_join(c));
@@ -476,6 +479,7 @@ int bcg_rand_stream(uint32_t seed, uint64_t skip, int32_t n, int32_t* out) {
 
 int bcg_rebuild_neighbours(bcg_ctx* c, const int32_t* ids, int32_t n) {
     if (!c) return BCG_ERR_ARG;
+    if (int rc_ = ctx_check_failed(c)) return rc_;
     HIPCHK(c, ctx_join(c));
     int rc;
     if ((rc = reset_stats(c))) return rc;
@@ -505,6 +509,7 @@ int bcg_rebuild_neighbours(bcg_ctx* c, const int32_t* ids, int32_t n) {
 
 int bcg_get_neighbours(bcg_ctx* c, int32_t* counts, int32_t* lists, int32_t stride) {
     if (!c || !counts || (!lists && stride > 0) || stride < 0) return BCG_ERR_ARG;
+    if (int rc_ = ctx_check_failed(c)) return rc_;
     HIPCHK(c, ctx_join(c));
     const int N = c->N;
     if (N == 0) return BCG_OK;
@@ -528,6 +533,7 @@ int bcg_get_neighbours(bcg_ctx* c, int32_t* counts, int32_t* lists, int32_t stri
 
 int bcg_sum_forces(bcg_ctx* c, double* F, double* T) {
     if (!c || !F || !T) return BCG_ERR_ARG;
+    if (int rc_ = ctx_check_failed(c)) return rc_;
     HIPCHK(c, ctx_join(c));
     int rc;
     if ((rc = reset_stats(c))) return rc;
@@ -547,6 +553,7 @@ int bcg_sum_forces(bcg_ctx* c, double* F, double* T) {
 
 int bcg_mean_stress(bcg_ctx* c, double* stress, double* fnet) {
     if (!c || (c->N > 0 && (!stress || !fnet))) return BCG_ERR_ARG;
+    if (int rc_ = ctx_check_failed(c)) return rc_;
     HIPCHK(c, ctx_join(c));
     if (!c->lists_valid && c->N > 0) return fail(c, BCG_ERR_STATE, "no neighbour lists: rebuild first");
     int rc;
@@ -567,9 +574,15 @@ int bcg_mean_stress(bcg_ctx* c, double* stress, double* fnet) {
     return BCG_OK;
 }
 
-int bcg_step(bcg_ctx* c, double dt, double t, int32_t flags, int32_t* n_div, int32_t* lineage, int32_t lineage_cap) {
-    if (!c) return BCG_ERR_ARG;
-    if (n_div) *n_div = 0;
+}  // extern "C"
+
+namespace {
+
+// The body of bcg_step; *started is set once the step has written state that
+// a failure cannot take back (move/grow's friction accumulators, the other
+// cell buffer, then the rng position and the lists).
+int step_body(bcg_ctx* c, double dt, double t, int32_t flags, int32_t* n_div, int32_t* lineage, int32_t lineage_cap,
+              bool* started) {
     int rc;
     const int N0 = c->N;
     const int in = c->cur, out = c->cur ^ 1;
@@ -590,6 +603,7 @@ int bcg_step(bcg_ctx* c, double dt, double t, int32_t flags, int32_t* n_div, int
     // not touch them and ran alongside it
     HIPCHK(c, ctx_join(c));
     HIPCHK(c, hipEventRecord(c->ev[6], c->stream));
+    *started = true;
     // MoveCell + GrowCell for every cell (ABM/Simulation.cpp:330-335)
     if (N0 > 0)
         hipLaunchKernelGGL(k_move_grow, dim3(nblk(N0, MOVE_BLOCK)), dim3(MOVE_BLOCK), 0, c->stream, c->hot[in],
@@ -661,6 +675,25 @@ int bcg_step(bcg_ctx* c, double dt, double t, int32_t flags, int32_t* n_div, int
     c->cur = out;
     if (n_div) *n_div = D;
     return BCG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bcg_step(bcg_ctx* c, double dt, double t, int32_t flags, int32_t* n_div, int32_t* lineage, int32_t lineage_cap) {
+    if (!c) return BCG_ERR_ARG;
+    if (n_div) *n_div = 0;
+    if (int rc = ctx_check_failed(c)) return rc;
+    bool started = false;
+    const int rc = step_body(c, dt, t, flags, n_div, lineage, lineage_cap, &started);
+    if (rc && started) {
+        // the friction accumulators and possibly the rng, the other buffer and
+        // the lists have moved on: nothing but bcg_set_cells may follow
+        c->failed = true;
+        c->lists_valid = false;
+    }
+    return rc;
 }
 
 int bcg_last_step_stats(bcg_ctx* c, int64_t* wall_draws, int32_t* n_flagged, int32_t* max_list) {

@@ -29,6 +29,9 @@ struct bcg_ctx {
     int cap = 0;
     int nbin = 0;
     bool lists_valid = false;
+    // set when a step fails after its divisions began (the rng, the cell
+    // buffers and the lists no longer agree); cleared by bcg_set_cells
+    bool failed = false;
     double* hot[2] = {nullptr, nullptr};
     double *time_p = nullptr, *time_q = nullptr, *dynfric = nullptr, *stafric = nullptr;
     int *age_p = nullptr, *age_q = nullptr, *type = nullptr, *ancestor = nullptr;
@@ -59,6 +62,13 @@ struct bcg_ctx {
     hipEvent_t after = nullptr;
     bool after_pending = false;
 };
+
+// A context whose last step failed part-way accepts only bcg_set_cells.
+inline int ctx_check_failed(bcg_ctx* c) {
+    if (!c->failed) return 0;
+    c->err = "a step failed after its divisions began: upload cells (bcg_set_cells) before further use";
+    return BCG_ERR_STATE;
+}
 
 inline hipError_t ctx_join(bcg_ctx* c) {
     if (!c->after_pending) return hipSuccess;

@@ -522,6 +522,7 @@ int bcg_env_nutrients(bcg_env* e, int32_t* count) {
                      minX, maxX, maxH, e->BX, e->BZ);
     bcg_ctx* m = e->mech;
     hipStream_t st = m->stream;
+    if (int rc_ = ctx_check_failed(m)) return efail(e, rc_, "%s", m->err.c_str());
     EHIP(e, ctx_join(m));  // writes the context's growth rates
     EnvArgs a;
     memset(&a, 0, sizeof a);

@@ -816,6 +816,7 @@ int bcg_rd3d_deposit(bcg_rd3d* c, const double* cm, const double* rate, int32_t 
 
 int bcg_rd3d_deposit_cells(bcg_rd3d* c, bcg_ctx* cells, double rate, int32_t* n_inside) {
     if (!c || !cells) return BCG_ERR_ARG;
+    if (int rc_ = ctx_check_failed(cells)) return fail(c, rc_, "%s", cells->err.c_str());
     const int n = cells->N;
     int rc = dep_reserve(c, n, cells->prm.maxCells);
     if (rc) return rc;
@@ -827,6 +828,7 @@ int bcg_rd3d_deposit_cells(bcg_rd3d* c, bcg_ctx* cells, double rate, int32_t* n_
 
 int bcg_rd3d_update_cells(bcg_rd3d* c, bcg_ctx* cells, double rate, double mu, double K, int32_t vcycles) {
     if (!c || !cells || !(K > 0) || vcycles < 1) return BCG_ERR_ARG;
+    if (int rc_ = ctx_check_failed(cells)) return fail(c, rc_, "%s", cells->err.c_str());
     const int n = cells->N;
     int rc = dep_reserve(c, n, cells->prm.maxCells);
     if (rc) return rc;
@@ -876,6 +878,7 @@ int bcg_rd3d_update_cells(bcg_rd3d* c, bcg_ctx* cells, double rate, double mu, d
 
 int bcg_rd3d_growth_to_cells(bcg_rd3d* c, bcg_ctx* cells, double mu, double K) {
     if (!c || !cells || !(K > 0)) return BCG_ERR_ARG;
+    if (int rc_ = ctx_check_failed(cells)) return fail(c, rc_, "%s", cells->err.c_str());
     const bcg::PhysParams& P = cells->P;
     const Level& v = c->L[0];
     HIPCHK(c, join_cells(c, cells));
@@ -1021,10 +1024,24 @@ int bcg_rd3d_adjoint(bcg_rd3d* c, const double* g, int32_t mode, double tol, int
     HIPCHK(c, hipMemcpyAsync(c->save, c->x[c->cur], N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     hipLaunchKernelGGL(k_adj_rhs, grid2(f), dim3(64, 4), 0, c->stream, c->dfield, c->b, f.nx, f.ny, f.nz);
     hipLaunchKernelGGL((k_fill<double>), dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, c->x[c->cur], c->N, 0.0);
+    // the forward state (c_new in x, the step's b) comes back on every exit
+    auto restore = [&]() -> int {
+        HIPCHK(c, hipMemcpyAsync(c->x[c->cur], c->save, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        hipLaunchKernelGGL(k_rhs, dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, c->cprev, c->b, c->prm.c_res,
+                           (long)f.nx * f.ny, c->N);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return 0;
+    };
     int it = 0;
     double rr = 0;
     rc = solve_core(c, mode, tol, max_iter, &it, &rr);
-    if (rc) return rc;
+    if (rc) {
+        const std::string msg = c->err;
+        restore();
+        c->err = "adjoint solve: " + msg;
+        return rc;
+    }
     const double* mu = c->x[c->cur];
     const double h = c->prm.h;
     hipLaunchKernelGGL(k_adj_grad, grid2(f), dim3(64, 4), 0, c->stream, mu, c->save, c->cprev, c->q, c->prm.K, c->prm.dt,
@@ -1043,12 +1060,7 @@ int bcg_rd3d_adjoint(bcg_rd3d* c, const double* g, int32_t mode, double tol, int
     gd = *c->hnorm;
     if (dL_dK) *dL_dK = gk;
     if (dL_dD) *dL_dD = gd;
-    // restore the forward state: c_new and b
-    HIPCHK(c, hipMemcpyAsync(c->x[c->cur], c->save, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    hipLaunchKernelGGL(k_rhs, dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, c->cprev, c->b, c->prm.c_res,
-                       (long)f.nx * f.ny, c->N);
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return BCG_OK;
+    return restore();
 }
 
 int bcg_rd3d_gather(bcg_rd3d* c, const double* cm, int32_t n, const double* field, double* out) {

@@ -99,19 +99,9 @@ def measured_traffic(kernel: str):
 
 
 def make_workload(n: int, seed: int = 2, radius: float = 600.0):
-    from baccolonygrowth_amd.colony import dome_colony, dome_params, height_normal_from_cells
+    from baccolonygrowth_amd.colony import bench_dome
 
-    cells = dome_colony(n, radius=radius, height=20.0, seed=seed)
-    p = dome_params(cells, bin_size=5.0, max_neighbours=64, extra_cells=max(n // 8, 1024))
-    rng = np.random.default_rng(seed)
-    # cells at uniformly spread points of their cycle: steady division rate
-    gr = 1.5850 * GR_BASE
-    cells["Ldiv"] = cells["Length"] * np.exp(gr * DT_H * rng.uniform(1.0, 1800.0, n))
-    cells["GrowthRate"] = GR_BASE
-    height, normal = height_normal_from_cells(p, cells)
-    wall = (rng.random((p.BoxX, p.BoxY)) - 0.5) * p.wall_rough
-    gr1 = np.full((p.BoxX, p.BoxY, p.BoxZ), GR_BASE)
-    return p, cells, (height, normal, wall, gr1, gr1.copy())
+    return bench_dome(n, seed=seed, radius=radius, gr_base=GR_BASE, dt_h=DT_H)
 
 
 def cpu_baseline(p, cells, fields, steps: int) -> dict:
@@ -280,6 +270,12 @@ def small_run(until: int = 5000, max_steps: int = 40000, update_every: int = 10)
             "cell_updates_per_s": cell_updates / el, "field_min_mM": cmin}
 
 
+# V-cycles of the finite-difference forward solves: enough to reach the
+# FP64 round-off floor (the FP64 multigrid contracts the residual ~15x per
+# cycle on these grids)
+FD_VCYCLES = 30
+
+
 def adjoint_run(n: int = 1 << 18) -> dict:
     """Gradients of one implicit nutrient step on 257 x 257 x 65 at 2 um with
     the sink of n cells (a dome of radius 300 um, rates = base x
@@ -291,7 +287,7 @@ def adjoint_run(n: int = 1 << 18) -> dict:
     mixed-precision gradient against the FP64 one."""
     import torch
 
-    from baccolonygrowth_amd.adjoint import nutrient_step
+    from baccolonygrowth_amd.adjoint import nutrient_step, set_coeffs
     from baccolonygrowth_amd.colony import dome_colony
     from baccolonygrowth_amd.nutrients import Nutrients3D, default_rd3d_params
 
@@ -337,9 +333,17 @@ def adjoint_run(n: int = 1 << 18) -> dict:
         }
         pick = np.argsort(-np.abs(gr))[:3]
 
+        c0_np = c0.numpy()
+
         def field(rates, D, K):
-            return nutrient_step(torch.from_numpy(rates), torch.tensor(D, dtype=torch.float64),
-                                 torch.tensor(K, dtype=torch.float64), c0, s, cm, "fp64", 1e-12)
+            # FP64 solve run to the round-off floor (a fixed count of V-cycles,
+            # no tolerance test): the field moves by ~1e-6 relative at h = 1e-6,
+            # far above that floor but not far above a 1e-12 stopping residual
+            set_coeffs(s, D, K)
+            s.deposit(cm, rates)
+            s.set_field(c0_np)
+            s.step("fp64", tol=1e-16, max_iter=FD_VCYCLES, strict=False)
+            return torch.from_numpy(s.field())
 
         def dloss(a, b):
             # L(a) - L(b) as sum (a - b)(a + b - 2 c*): no cancellation of the two ~1e8 sums
@@ -347,9 +351,6 @@ def adjoint_run(n: int = 1 << 18) -> dict:
 
         fd = {}
         with torch.no_grad():
-            # one cell's rate moves the field by little more than the solves'
-            # 1e-12 residual at h = 1e-6: larger steps (second-order central
-            # differences) lift the signal above it
             for h in (1e-6, 1e-4, 1e-2):
                 e = {}
                 for name, v0, g in (("D", D0, gD), ("K", K0, gK)):
@@ -472,20 +473,35 @@ def coupled_run(steps: int = 200, update_every: int = 10, ckpt: int = 5) -> dict
         grads = {"D": Dt.grad.item(), "K": Kt.grad.item(), "rate_factor": at.grad.item()}
         log(f"[run] forward + backward through {len(iters)} updates: {t_adj:.1f} s; dL/dD {grads['D']:.6e}, "
             f"dL/dK {grads['K']:.6e}, dL/d(rate factor) {grads['rate_factor']:.6e}")
-        # central differences at two relative steps: 1e-6, and 1e-4 where the
-        # solves' 1e-11 tolerance (a loss change of ~0.1 against ~25 for D at
-        # 1e-6) no longer dominates
+        # central differences at two relative steps, 1e-6 and 1e-4; each
+        # forward of the 20 updates solved to the FP64 round-off floor (a
+        # fixed V-cycle count), and the loss difference summed from the two
+        # final fields' difference (no cancellation of the two large sums)
+        from baccolonygrowth_amd.adjoint import set_coeffs
+
+        tgt = target.numpy()
+
+        def fd_field(D, K, a):
+            set_coeffs(s, D, K)
+            c = c_init
+            for k in range(len(seq_rate)):
+                s.deposit(seq_cm[k], a * seq_rate[k])
+                s.set_field(c)
+                s.step("fp64", tol=1e-16, max_iter=FD_VCYCLES, strict=False)
+                c = s.field()
+            return c
+
         fd = {1e-6: {}, 1e-4: {}}
         with torch.no_grad():
             for rel_h in fd:
                 for name, v0 in (("D", D0), ("K", K0), ("rate_factor", 1.0)):
-                    vals = []
+                    fields_pm = []
                     for sgn in (1.0, -1.0):
                         args_ = {"D": D0, "K": K0, "rate_factor": 1.0}
                         args_[name] = v0 * (1.0 + sgn * rel_h)
-                        _, Lv = loss_of(*(torch.tensor(args_[k], dtype=torch.float64) for k in ("D", "K", "rate_factor")))
-                        vals.append(float(Lv))
-                    fd[rel_h][name] = (vals[0] - vals[1]) / (2.0 * rel_h * v0)
+                        fields_pm.append(fd_field(args_["D"], args_["K"], args_["rate_factor"]))
+                    up, dn = fields_pm
+                    fd[rel_h][name] = float(((up - dn) * (up + dn - 2.0 * tgt)).sum()) / (2.0 * rel_h * v0)
                     log(f"[run] dL/d{name}: adjoint {grads[name]:.8e}, central difference (h {rel_h:g}) "
                         f"{fd[rel_h][name]:.8e}")
     err = {h: {k: abs(grads[k] - fd[h][k]) / abs(fd[h][k]) for k in grads} for h in fd}

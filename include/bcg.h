@@ -4,8 +4,27 @@
  *
  * Every entry point returns an int status (BCG_OK == 0); no C++ exception
  * crosses this boundary. Host buffers belong to the caller, device buffers to
- * the context. One context per host thread; all work is ordered on the
- * context's HIP stream and every call returns after that work has finished.
+ * the context. One context per host thread; each context orders its work on
+ * its own HIP stream.
+ *
+ * Completion. A call that takes or returns host buffers, or returns a count
+ * or status computed on the device, returns after its work has finished.
+ * Three calls return before their device work finishes, by design:
+ * bcg_rd3d_deposit_cells (when n_inside is NULL), bcg_rd3d_growth_to_cells
+ * and bcg_rd3d_update_cells. They are queued on the nutrient context's
+ * stream behind the mechanics context's queued work; the mechanics context
+ * then waits for them on the device (no host synchronisation): bcg_step
+ * before its move/grow kernel, so that step's bins and lists run alongside
+ * the nutrient work; every other call on the mechanics context on entry.
+ * A call on the nutrient context that reads its field back
+ * (bcg_rd3d_get_field) waits for them too.
+ *
+ * Failure. A bcg_step that fails after its move/grow kernel was queued
+ * (list overflow or an out-of-domain daughter during the list patch) leaves
+ * the friction accumulators, the rand() position and the lists advanced but
+ * the cell count and buffers not swapped: the context is then marked failed
+ * and every call that reads its cells returns BCG_ERR_STATE until
+ * bcg_set_cells uploads a state again.
  *
  * The reference has no plugin or FFI layer: its seam is a set of free C++
  * functions that take extern globals as parameters. Each entry point below

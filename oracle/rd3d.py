@@ -8,7 +8,11 @@ problem and is pinned by mathematics, not by reference vectors:
   * a direct sparse solve (scipy.sparse.linalg.spsolve) on small grids;
   * the operator's invariants: constants are in the kernel of the
     diffusion part; the symmetrised operator W A is symmetric;
-  * manufactured solutions (b := A u* for a chosen u*).
+  * manufactured solutions (b := A u* for a chosen u*);
+  * the conjugate-gradient form of the step and of its adjoint (for grids
+    where the factorisation is slow) against the direct solves;
+  * the cloud-in-cell deposition by conservation of the total rate and of
+    its first moment (exact for linear weights).
 
 The step, on a vertex grid of n = (nx, ny, nz) points with spacing h:
     (1 + dt*sigma) c_new - lam * Lap_h c_new = c_old,   lam = dt*D/h^2
@@ -129,23 +133,70 @@ def deposit_cic(cm: np.ndarray, q: np.ndarray, shape, h: float, origin) -> np.nd
     return out / h**3, inside
 
 
-def step_direct(c_old, q, K, D, dt, h, c_res):
-    """One implicit step by direct solve: returns c_new."""
+def _free_system(shape, sigma, lam, dt):
+    """(S, w, free, A): the rows of A that are not Dirichlet, restricted to
+    the free unknowns and scaled by the control-volume weights w, S = W_f
+    A_ff, which is symmetric positive definite (symmetric by construction of
+    W; positive diagonal dominance from 1 + dt sigma > 0)."""
+    A = sparse_A(shape, sigma, lam, dt)
+    free = np.ones(shape, bool)
+    free[0] = False
+    f = free.ravel()
+    w = control_volume_weights(shape).ravel()[f]
+    S = (A[f][:, f]).multiply(w[:, None]).tocsr()
+    return S, w, f, A
+
+
+def solve_cg(b, sigma, lam, dt, rtol=1e-14):
+    """The same system as solve_direct by conjugate gradients on the
+    symmetrised free block (for grids where the sparse factorisation is too
+    slow); Dirichlet values are b there."""
+    from scipy.sparse.linalg import cg
+
+    S, w, f, A = _free_system(b.shape, sigma, lam, dt)
+    x = b.ravel().copy()
+    fixed = x.copy()
+    fixed[f] = 0.0
+    rhs_f = w * (b.ravel()[f] - (A @ fixed)[f])
+    xf, info = cg(S, rhs_f, x0=b.ravel()[f], rtol=rtol, maxiter=20000)
+    if info != 0:
+        raise RuntimeError(f"cg did not converge (info={info})")
+    x[f] = xf
+    return x.reshape(b.shape)
+
+
+def step_direct(c_old, q, K, D, dt, h, c_res, solver="direct"):
+    """One implicit step by direct solve (or solver="cg": conjugate
+    gradients to 1e-14): returns c_new."""
     sigma = q / (K + c_old)
+    if solver == "cg":
+        return solve_cg(rhs(c_old, c_res), sigma, dt * D / h**2, dt)
     return solve_direct(rhs(c_old, c_res), sigma, dt * D / h**2, dt)
 
 
-def adjoint_direct(c_old, c_new, q, K, D, dt, h, g):
+def adjoint_direct(c_old, c_new, q, K, D, dt, h, g, solver="direct"):
     """Gradients of L with dL/dc_new = g through one step, from the sparse
-    transpose solve A^T lam = g (no symmetrisation used):
+    transpose solve A^T lam = g (no symmetrisation used; solver="cg": the
+    free block of the transpose through the symmetrised system,
+    A_ff^T = S W_f^-1, by conjugate gradients to 1e-14):
     returns dL/dq, dL/dc_old, dL/dK, dL/dD."""
-    from scipy.sparse.linalg import spsolve
+    from scipy.sparse.linalg import cg, spsolve
 
     shape = c_old.shape
     sigma = q / (K + c_old)
     lam_c = dt * D / h**2
-    A = sparse_A(shape, sigma, lam_c, dt)
-    lam = spsolve(A.T.tocsc(), g.ravel()).reshape(shape)
+    if solver == "cg":
+        # only the free rows of lam enter the gradients below
+        S, w, f, _ = _free_system(shape, sigma, lam_c, dt)
+        mu, info = cg(S, g.ravel()[f], rtol=1e-14, maxiter=20000)
+        if info != 0:
+            raise RuntimeError(f"cg did not converge (info={info})")
+        lam = np.zeros(g.size)
+        lam[f] = w * mu
+        lam = lam.reshape(shape)
+    else:
+        A = sparse_A(shape, sigma, lam_c, dt)
+        lam = spsolve(A.T.tocsc(), g.ravel()).reshape(shape)
     free = np.ones(shape, bool)
     free[0] = False
     den = K + c_old

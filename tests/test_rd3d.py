@@ -67,6 +67,48 @@ def test_deposit_cic_conserves_and_localises():
     assert q1[1, 3, 2] == 2.0 and q1.sum() == 2.0
 
 
+def test_deposit_cic_first_moment():
+    """Linear (cloud-in-cell) weights reproduce linear functions: for every
+    axis, sum_v q_v x_v h^3 = sum_i rate_i x_i exactly (to rounding) for
+    cells inside the grid, whatever their position in a cell."""
+    rng = np.random.default_rng(1)
+    shape, h, origin = (7, 9, 11), 2.5, (-3.0, 4.0, 1.0)
+    nz, ny, nx = shape
+    hi = np.array(origin) + h * (np.array([nx, ny, nz]) - 1)
+    cm = rng.uniform(np.array(origin) - 2.0, hi + 2.0, (4000, 3))
+    rate = rng.lognormal(0.0, 0.5, 4000)
+    q, inside = rd3d.deposit_cic(cm, rate, shape, h, origin)
+    assert 0 < inside.sum() < len(cm)
+    z, y, x = np.meshgrid(*(origin[a] + h * np.arange(n) for a, n in ((2, nz), (1, ny), (0, nx))), indexing="ij")
+    for axis, coord in ((0, x), (1, y), (2, z)):
+        lhs = (q * coord).sum() * h**3
+        rhs = (rate[inside] * cm[inside, axis]).sum()
+        assert abs(lhs - rhs) <= 1e-11 * np.abs(rate[inside] * cm[inside, axis]).sum(), axis
+    # a dropped index or a swapped weight breaks the moments: a cell at a
+    # fractional position off the lattice moves the first moment by its offset
+    q1, _ = rd3d.deposit_cic(np.array([[origin[0] + 1.3 * h, origin[1] + 2.6 * h, origin[2] + 0.2 * h]]),
+                             np.array([1.0]), shape, h, origin)
+    assert (q1 * x).sum() * h**3 == pytest.approx(origin[0] + 1.3 * h, rel=1e-13)
+    assert (q1 * y).sum() * h**3 == pytest.approx(origin[1] + 2.6 * h, rel=1e-13)
+    assert (q1 * z).sum() * h**3 == pytest.approx(origin[2] + 0.2 * h, rel=1e-13)
+
+
+def test_cg_step_and_adjoint_equal_direct():
+    """The conjugate-gradient forms (used on grids too large to factorise)
+    against the direct sparse solves: the step and every adjoint output."""
+    shape = (6, 7, 9)
+    h, D, dt, K = 1.5, 30.0, 10.0, 0.1
+    c_old, q, target = _loss_parts(shape, seed=5)
+    a = rd3d.step_direct(c_old, q, K, D, dt, h, 20.0)
+    b = rd3d.step_direct(c_old, q, K, D, dt, h, 20.0, solver="cg")
+    assert np.abs(a - b).max() <= 1e-12 * 20
+    g = 2 * (a - target)
+    ra = rd3d.adjoint_direct(c_old, a, q, K, D, dt, h, g)
+    rb = rd3d.adjoint_direct(c_old, a, q, K, D, dt, h, g, solver="cg")
+    for u, v in zip(ra, rb):
+        assert np.abs(np.asarray(u) - np.asarray(v)).max() <= 1e-11 * max(np.abs(np.asarray(u)).max(), 1e-300)
+
+
 # ------------------------------------------------------------- GPU tests
 def _solver(shape, h=1.0, D=600.0, dt=10.0, K=0.1, **kw):
     from baccolonygrowth_amd.nutrients import Nutrients3D, default_rd3d_params
