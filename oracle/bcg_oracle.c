@@ -16,6 +16,7 @@
 
 #include <float.h>
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -68,8 +69,28 @@ int32_t or_glibc_rand(or_glibc_rng* g) {
     return (int32_t)(v >> 1);
 }
 
+/* The stream the oracle draws from: glibc's rand() restated (above), owned
+ * by the oracle so that no other code in the process (runtimes that call
+ * rand() themselves) can advance it. Starts as srand(1), as libc does.    */
+static or_glibc_rng g_stream;
+static int g_stream_seeded = 0;
+
+void or_srand(uint32_t seed) {
+    or_glibc_srand(&g_stream, seed);
+    g_stream_seeded = 1;
+}
+
+int32_t or_rand(void) {
+    if (!g_stream_seeded) or_srand(1);
+    return or_glibc_rand(&g_stream);
+}
+
+static void rand_discard(int64_t n) {
+    for (int64_t k = 0; k < n; k++) (void)or_rand();
+}
+
 /* A draw as the reference uses it: ((float)rand()/RAND_MAX - 0.5). */
-static double draw_centered(void) { return (double)((float)rand() / RAND_MAX) - 0.5; }
+static double draw_centered(void) { return (double)((float)or_rand() / RAND_MAX) - 0.5; }
 
 /* ----------------------------------------------------- min_distance --- */
 /* ABM/Neighbours.cpp:95-104 */
@@ -170,14 +191,21 @@ static v3 F_cw_locus(const bcg_params* pr, const bcg_cell* c, v3 vert, double wa
     return mk(0.0, 0.0, 0.0);
 }
 
-/* F_cw: ABM/Forces.cpp:88-204. The first locus draws one rand() (:102). */
-static void F_cw(const bcg_params* pr, const bcg_cell* c, double Wall_z, v3* F1, v3* F2, v3* r1, v3* r2, v3* sf,
-                 v3* df) {
+/* F_cw: ABM/Forces.cpp:88-204. The first locus draws one rand() (:102).
+ * With draw == 0 the draw is left to the caller (its value is multiplied by
+ * var_pos, so only var_pos == 0 may skip it).                             */
+static void F_cw_d(const bcg_params* pr, const bcg_cell* c, double Wall_z, v3* F1, v3* F2, v3* r1, v3* r2, v3* sf,
+                   v3* df, int draw) {
     double L = c->Length + 4.0 / 3 * c->Radius;
     double M = L * c->Radius * c->Radius * PI_;
-    double wx = c->p[0] + draw_centered() * pr->var_pos;
+    double wx = c->p[0] + (draw ? draw_centered() : 0.0) * pr->var_pos;
     *F1 = F_cw_locus(pr, c, P(c), wx, Wall_z, M, r1, sf, df);
     *F2 = F_cw_locus(pr, c, Q(c), c->q[0], Wall_z, M, r2, sf, df);
+}
+
+static void F_cw(const bcg_params* pr, const bcg_cell* c, double Wall_z, v3* F1, v3* F2, v3* r1, v3* r2, v3* sf,
+                 v3* df) {
+    F_cw_d(pr, c, Wall_z, F1, F2, r1, r2, sf, df, 1);
 }
 
 void or_F_cw(const bcg_params* p, const bcg_cell* c, double wall_z, double* o) {
@@ -326,7 +354,7 @@ static int box_remove(or_sim* s, int a, int id) {
 
 /* UniformGrid::GetNeighbours (ABM/UniformGrid.h:141-170) followed by
  * reduceNeighbours (ABM/Neighbours.cpp:57-93) for one cell.               */
-static int gather(or_sim* s, const bcg_cell* cells, int id) {
+static int gather_into(or_sim* s, const bcg_cell* cells, int id, int* scratch) {
     const bcg_params* pr = &s->p;
     int x0, y0, z0;
     const bcg_cell* c = &cells[id];
@@ -340,14 +368,14 @@ static int gather(or_sim* s, const bcg_cell* cells, int id) {
                 for (int k = 0; k < s->box_count[a]; k++)
                     if (b[k] != id) {
                         if (n >= s->scratch_cap) return BCG_ERR_NEIGHBOUR_OVERFLOW;
-                        s->scratch[n++] = b[k];
+                        scratch[n++] = b[k];
                     }
             }
     v3 cm = cmof(c);
     int m = 0;
     const double reach = c->Length / 2.0 + pr->Rc * c->Radius;
     for (int k = 0; k < n; k++) {
-        const bcg_cell* o = &cells[s->scratch[k]];
+        const bcg_cell* o = &cells[scratch[k]];
         v3 v = vsub(P(o), cm);
         double dp = sqrt(vdot(v, v));
         v = vsub(Q(o), cm);
@@ -357,7 +385,7 @@ static int gather(or_sim* s, const bcg_cell* cells, int id) {
             min_distance(c, o, &d, &c1, &c2);
             if (d < pr->Rc * c->Radius) {
                 if (m >= s->cap) return BCG_ERR_NEIGHBOUR_OVERFLOW;
-                s->nl[(size_t)id * s->cap + m++] = s->scratch[k];
+                s->nl[(size_t)id * s->cap + m++] = scratch[k];
             }
         }
     }
@@ -365,9 +393,11 @@ static int gather(or_sim* s, const bcg_cell* cells, int id) {
     return 0;
 }
 
+static int gather(or_sim* s, const bcg_cell* cells, int id) { return gather_into(s, cells, id, s->scratch); }
+
 /* sum_forces: ABM/Forces.cpp:332-381 */
-static void sum_forces(or_sim* s, const bcg_cell* c, const bcg_cell* cells, const int* nl, int nn, int wx, int wy,
-                       v3* Fnet, v3* Tnet, v3* sf, v3* df) {
+static void sum_forces_d(or_sim* s, const bcg_cell* c, const bcg_cell* cells, const int* nl, int nn, int wx, int wy,
+                         v3* Fnet, v3* Tnet, v3* sf, v3* df, int64_t* draws, int draw) {
     const bcg_params* pr = &s->p;
     *Fnet = mk(0, 0, 0);
     *Tnet = mk(0, 0, 0);
@@ -384,8 +414,8 @@ static void sum_forces(or_sim* s, const bcg_cell* c, const bcg_cell* cells, cons
     *df = mk(0, 0, 0);
     if (dmin(c->q[2], c->p[2]) < 1.2 * c->Radius) {
         double wall_z = s->f.wall[wx * pr->BoxY + wy];
-        F_cw(pr, c, wall_z, &F, &F2, &r, &r2, sf, df);
-        s->wall_draws++;
+        F_cw_d(pr, c, wall_z, &F, &F2, &r, &r2, sf, df, draw);
+        (*draws)++;
         *Fnet = vadd(*Fnet, F);
         r = vsub(r, cm);
         *Tnet = vadd(*Tnet, vcross(r, F));
@@ -405,6 +435,11 @@ static void sum_forces(or_sim* s, const bcg_cell* c, const bcg_cell* cells, cons
         Tnet->z = 0;
         Fnet->y = 0;
     }
+}
+
+static void sum_forces(or_sim* s, const bcg_cell* c, const bcg_cell* cells, const int* nl, int nn, int wx, int wy,
+                       v3* Fnet, v3* Tnet, v3* sf, v3* df) {
+    sum_forces_d(s, c, cells, nl, nn, wx, wy, Fnet, Tnet, sf, df, &s->wall_draws, 1);
 }
 
 /* Half kick of UpdatePositions / UpdateVelocities: ABM/Integrate.cpp:47-65
@@ -622,7 +657,7 @@ void or_sim_sum_forces(or_sim* s, double* F, double* T) {
         int x, y, z;
         get_address(&s->p, cmof(&s->old_cells[i]), &x, &y, &z);
         v3 f, t, sf, df;
-        /* diagnostic pass: F_cw still takes its libc rand() draws */
+        /* diagnostic pass: F_cw still takes its rand() draws */
         sum_forces(s, &s->old_cells[i], s->old_cells, s->nl + (size_t)i * s->cap, s->nl_count[i], x, y, &f, &t, &sf, &df);
         st(F + 3 * i, f);
         st(T + 3 * i, t);
@@ -768,5 +803,120 @@ int or_sim_step(or_sim* s, double dt, double t, int rebuild, int* lineage, int l
     s->new_cells = s->old_cells;
     s->old_cells = tmp;
     s->N = N;
+    return done;
+}
+
+/* ---------------------------------------------- OpenMP form of the step ---
+ * The same step with the per-cell work on all threads: the list rebuild
+ * (each thread its own gather buffer; the boxes do not change while it
+ * runs) and MoveCell + GrowCell (each cell reads the old buffer and its own
+ * list, writes only its new record). What the reference does in order stays
+ * in order on one thread, in ascending ID: the box moves (the boxes'
+ * contents order depends on it), the dividing list, the wall-contact draws
+ * (taken after the loop: with var_pos == 0 their values are not used), and
+ * the division loop. The result equals or_sim_step bit for bit.            */
+void or_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int or_get_threads(void) { return omp_get_max_threads(); }
+
+int or_sim_step_par(or_sim* s, double dt, double t, int rebuild, int* lineage, int lineage_cap) {
+    const bcg_params* pr = &s->p;
+    if (pr->var_pos != 0.0) return BCG_ERR_UNSUPPORTED;
+    const int N = s->N;
+    int err = 0;
+    if (rebuild) {
+#pragma omp parallel reduction(| : err)
+        {
+            int* scratch = (int*)malloc(sizeof(int) * s->scratch_cap);
+#pragma omp for schedule(dynamic, 256)
+            for (int i = 0; i < N; i++) {
+                int rc = gather_into(s, s->old_cells, i, scratch);
+                if (rc) err |= 1 << (-rc);
+            }
+            free(scratch);
+        }
+        if (err) {
+            for (int c = 1; c < 16; c++)
+                if (err & (1 << c)) return -c;
+        }
+    }
+    int* olda = (int*)malloc(sizeof(int) * (N > 0 ? N : 1));
+    int* newa = (int*)malloc(sizeof(int) * (N > 0 ? N : 1));
+    int64_t draws = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : draws) reduction(| : err)
+    for (int id = 0; id < N; id++) {
+        const bcg_cell* o = &s->old_cells[id];
+        bcg_cell* n = &s->new_cells[id];
+        int x, y, z;
+        olda[id] = get_address(pr, cmof(o), &x, &y, &z);
+        if (olda[id] < 0) {
+            err |= 1 << (-BCG_ERR_OUT_OF_DOMAIN);
+            continue;
+        }
+        v3 F, T, sf, df;
+        bcg_cell temp;
+        const int* nl = s->nl + (size_t)id * s->cap;
+        sum_forces_d(s, o, s->old_cells, nl, s->nl_count[id], x, y, &F, &T, &sf, &df, &draws, 0);
+        update_positions(pr->planar, dt, F, T, o, &temp);
+        sum_forces_d(s, &temp, s->old_cells, nl, s->nl_count[id], x, y, &F, &T, &sf, &df, &draws, 0);
+        update_velocities(pr->planar, dt, F, T, &temp, n);
+        st(n->DynFric, df);
+        st(n->StaFric, sf);
+        newa[id] = get_address(pr, cmof(n), NULL, NULL, NULL);
+        if (newa[id] < 0) {
+            err |= 1 << (-BCG_ERR_OUT_OF_DOMAIN);
+            continue;
+        }
+        int rc = grow(s, dt, n);
+        if (rc) err |= 1 << (-rc);
+    }
+    int rc = 0;
+    if (err) {
+        for (int c = 1; c < 16 && !rc; c++)
+            if (err & (1 << c)) rc = -c;
+    }
+    int numDivide = 0;
+    for (int id = 0; id < N && !rc; id++) {
+        if (newa[id] != olda[id]) {
+            rc = box_remove(s, olda[id], id);
+            if (!rc) rc = box_add(s, newa[id], id);
+        }
+        if (s->new_cells[id].Length >= s->new_cells[id].Ldiv) s->dividing[numDivide++] = id;
+    }
+    free(olda);
+    free(newa);
+    if (rc) return rc;
+    s->wall_draws += draws;
+    rand_discard(draws);
+    /* divisions and list patch, in order (as or_sim_step) */
+    int Nn = N, done = 0;
+    for (int k = 0; k < numDivide; k++) {
+        if (Nn > (pr->maxCells - 1)) break;
+        const int id = s->dividing[k];
+        bcg_cell* m = &s->new_cells[id];
+        bcg_cell* d = &s->new_cells[Nn];
+        memset(d, 0, sizeof *d);
+        rc = box_remove(s, get_address(pr, cmof(m), NULL, NULL, NULL), id);
+        if (rc) return rc;
+        divide(pr, m, d, t);
+        rc = box_add(s, get_address(pr, cmof(m), NULL, NULL, NULL), id);
+        if (!rc) rc = box_add(s, get_address(pr, cmof(d), NULL, NULL, NULL), Nn);
+        if (rc) return rc;
+        if (done < lineage_cap) { lineage[2 * done] = id; lineage[2 * done + 1] = Nn; }
+        done++;
+        Nn++;
+        int len = s->nl_count[id];
+        s->idlist[0] = id;
+        memcpy(s->idlist + 1, s->nl + (size_t)id * s->cap, sizeof(int) * len);
+        s->idlist[len + 1] = Nn - 1;
+        len += 2;
+        for (int i = 0; i < len; i++) {
+            rc = gather(s, s->new_cells, s->idlist[i]);
+            if (rc) return rc;
+        }
+    }
+    bcg_cell* tmp = s->new_cells;
+    s->new_cells = s->old_cells;
+    s->old_cells = tmp;
+    s->N = Nn;
     return done;
 }

@@ -1,7 +1,7 @@
 /*
  * bcg_oracle.h - CPU restatement of the reference mechanics (test
  * infrastructure: the checker for the HIP path, and the timed CPU "port"
- * baseline). Plain C99, serial, FP64, no FMA contraction.
+ * baseline). Plain C99 (one OpenMP form of the step), FP64, no FMA contraction.
  *
  * Parity status of each function (how it is pinned):
  *   or_min_distance, or_F_cc, or_F_cw, or_sum_forces, or_sim_* (planar):
@@ -48,6 +48,16 @@ void or_sim_mean_stress(or_sim* s, double* S5, double* F3);
 /* returns divisions performed (>= 0) or a BCG_ERR_* code */
 int or_sim_step(or_sim* s, double dt, double t, int rebuild, int* lineage, int lineage_cap);
 int64_t or_sim_wall_draws(or_sim* s);
+/* The same step with its per-cell work on OpenMP threads (timed CPU
+ * baseline); bit-identical to or_sim_step. var_pos must be 0. */
+int or_sim_step_par(or_sim* s, double dt, double t, int rebuild, int* lineage, int lineage_cap);
+void or_set_threads(int n);
+int or_get_threads(void);
+
+/* The oracle's own rand() stream (glibc's algorithm, below): every draw of
+ * F_cw and divide comes from it; starts as after srand(1). */
+void or_srand(uint32_t seed);
+int32_t or_rand(void);
 
 /* glibc TYPE_3 rand() restated from its published description */
 typedef struct or_glibc_rng { int32_t r[34]; int32_t i; } or_glibc_rng;

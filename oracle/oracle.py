@@ -162,6 +162,14 @@ class Sim:
         if f(self.h, path.encode(), ctypes.c_double(t)) != 0:
             raise OSError(f"cannot write {path}")
 
+    def step_par(self, dt: float, t: float, rebuild: bool, lineage_cap: int = 100000):
+        """The OpenMP form of step (C oracle only); the same result."""
+        lin = np.zeros((lineage_cap, 2), np.int32)
+        nd = self.o.lib.or_sim_step_par(self.h, dt, t, int(bool(rebuild)), lin.ctypes.data_as(_ip), lineage_cap)
+        if nd < 0:
+            raise RuntimeError(f"step rc={nd}")
+        return lin[: min(nd, lineage_cap)].copy()
+
     def step(self, dt: float, t: float, rebuild: bool, lineage_cap: int = 100000):
         lin = np.zeros((lineage_cap, 2), np.int32)
         nd = self.o._f("sim_step")(
@@ -210,8 +218,12 @@ class Oracle(_Base):
         self.lib.or_sim_wall_draws.argtypes = [_vp]
         self.lib.or_sim_wall_draws.restype = ctypes.c_int64
         self.lib.or_min_distance.argtypes = [_vp, _vp, _dp]
-        self.lib.srand.argtypes = [ctypes.c_uint]
-        self.lib.rand.restype = ctypes.c_int
+        self.lib.or_srand.argtypes = [ctypes.c_uint32]
+        self.lib.or_rand.restype = ctypes.c_int32
+        self.lib.or_sim_step_par.argtypes = [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _ip, ctypes.c_int]
+        self.lib.or_sim_step_par.restype = ctypes.c_int
+        self.lib.or_set_threads.argtypes = [ctypes.c_int]
+        self.lib.or_get_threads.restype = ctypes.c_int
 
     def _pre(self):
         return ()
@@ -220,11 +232,16 @@ class Oracle(_Base):
         return Sim(self, p)
 
     def srand(self, seed: int):
-        """Seed the process C library rand() (the oracle draws from it)."""
-        self.lib.srand(seed)
+        """Seed the oracle's own rand() stream (glibc's algorithm, kept apart
+        from the process C library so that nothing else can advance it)."""
+        self.lib.or_srand(seed)
 
     def rand(self) -> int:
-        return self.lib.rand()
+        return self.lib.or_rand()
+
+    def set_threads(self, n: int) -> None:
+        """OpenMP threads of Sim.step_par."""
+        self.lib.or_set_threads(n)
 
     def glibc_stream(self, seed: int, n: int) -> np.ndarray:
         class G(ctypes.Structure):
@@ -252,7 +269,23 @@ class RefLib(_Base):
         for name in ("sim_set_cells", "sim_rebuild", "sim_rebuild_ids", "sim_set_fields"):
             getattr(self.lib, "ref_" + name).restype = None
         self.lib.ref_rand.restype = ctypes.c_int
+        self.lib.ref_set_threads.argtypes = [ctypes.c_int]
+        self.lib.ref_sim_step_omp.argtypes = [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _ip, ctypes.c_int]
+        self.lib.ref_sim_step_omp.restype = ctypes.c_int
         self._p = Params()
+
+    def set_threads(self, n: int) -> None:
+        """OpenMP threads of the reference's parallel regions (1 for parity)."""
+        self.lib.ref_set_threads(n)
+
+    def step_omp(self, sim: "Sim", dt: float, t: float, rebuild: bool) -> int:
+        """The reference's step with its threaded move/grow loop (timing only:
+        the division order then depends on the threads)."""
+        lin = np.zeros((1, 2), np.int32)
+        nd = self.lib.ref_sim_step_omp(sim.h, dt, t, int(bool(rebuild)), lin.ctypes.data_as(_ip), 0)
+        if nd < 0:
+            raise RuntimeError(f"step rc={nd}")
+        return nd
 
     def _pre(self):
         return (ctypes.byref(self._p),)

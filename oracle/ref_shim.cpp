@@ -74,9 +74,16 @@ struct RefSim {
     int* IDlist;
 };
 
+// OpenMP threads of the reference's own parallel regions (the nutrient
+// sweeps of ABM/Multigrid.cpp and the move loop of ref_sim_step_omp); 1 for
+// parity runs, where the reference's threaded sweeps race.
+static int g_threads = 1;
+
 extern "C" {
 
-void ref_set_params(const bcg_params* p) { set_globals(p); omp_set_num_threads(1); }
+void ref_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+
+void ref_set_params(const bcg_params* p) { set_globals(p); omp_set_num_threads(g_threads); }
 void ref_srand(unsigned s) { srand(s); }
 int ref_rand(void) { return rand(); }
 
@@ -113,7 +120,7 @@ void ref_F_cw(const bcg_params* p, const bcg_cell* c, double wall_z, double* out
 
 void* ref_sim_create(const bcg_params* p) {
     set_globals(p);
-    omp_set_num_threads(1);
+    omp_set_num_threads(g_threads);
     RefSim* s = new RefSim();
     s->prm = *p;
     s->N = 0;
@@ -283,6 +290,50 @@ int ref_sim_step(void* h, double dt, double t, int rebuild, int* lineage, int li
     int N_cells = s->N;
     if (rebuild) getNeighbours(s->old_cells, N_cells, *s->grid, s->NL, s->maxNeighbours);
     int numDivide = 0;
+    for (int cellID = 0; cellID < N_cells; cellID++) {
+        MoveCell(cellID, *s->grid, s->old_cells, s->new_cells, s->NL[cellID], dt, *s->Height, *s->Normal,
+                 *s->Wall);
+        GrowCell(s->new_cells[cellID], cellID, dt, s->dividing, numDivide, *s->Env, NULL, *s->grid);
+    }
+    int done = 0;
+    for (int cellCount = 0; cellCount < numDivide; cellCount++) {
+        if (N_cells > (maxCells - 1)) break;
+        const int cellID = s->dividing[cellCount];
+        DivideCell(cellID, N_cells, s->new_cells, *s->grid, s->NL[cellID], *s->Wall, *s->Height, *s->Normal, t);
+        if (done < lineage_cap) {
+            lineage[2 * done] = cellID;
+            lineage[2 * done + 1] = N_cells;
+        }
+        done++;
+        N_cells++;
+        int IDlen = s->NL[cellID][0];
+        memcpy(s->IDlist, s->NL[cellID], (IDlen + 1) * sizeof(int));
+        s->IDlist[0] = cellID;
+        s->IDlist[IDlen + 1] = N_cells - 1;
+        IDlen += 2;
+        getNeighbours(s->new_cells, *s->grid, s->NL, s->maxNeighbours, s->IDlist, IDlen);
+    }
+    Cell* tmp = s->new_cells;
+    s->new_cells = s->old_cells;
+    s->old_cells = tmp;
+    s->N = N_cells;
+    return done;
+}
+
+// The same step with the move/grow loop on OpenMP threads, as the
+// reference's time loop runs it (ABM/Simulation.cpp:330-335: a static
+// parallel for over MoveCell + GrowCell, whose bin moves and dividing-list
+// appends take critical sections, ABM/Cells.cpp:34,50). The dividing order,
+// and with it the division loop, then depends on thread timing: for timing
+// only.
+int ref_sim_step_omp(void* h, double dt, double t, int rebuild, int* lineage, int lineage_cap) {
+    RefSim* s = (RefSim*)h;
+    set_globals(&s->prm);
+    omp_set_num_threads(g_threads);
+    int N_cells = s->N;
+    if (rebuild) getNeighbours(s->old_cells, N_cells, *s->grid, s->NL, s->maxNeighbours);
+    int numDivide = 0;
+#pragma omp parallel for schedule(static)
     for (int cellID = 0; cellID < N_cells; cellID++) {
         MoveCell(cellID, *s->grid, s->old_cells, s->new_cells, s->NL[cellID], dt, *s->Height, *s->Normal,
                  *s->Wall);
@@ -514,7 +565,7 @@ void ref_env_fields(void* h, int pass) {
 int ref_env_nutrients(void* h) {
     RefEnv* e = (RefEnv*)h;
     env_globals(e);
-    omp_set_num_threads(1);
+    omp_set_num_threads(g_threads);
     RefSim* s = e->sim;
     return UpdateEnvArrayMG_OxygenAcetate(s->Env, e->oldEnv, e->agar, e->agar_old, e->wall, e->wall_old, *e->DS,
                                           *e->DS1, *e->DS2, *e->WS, *e->WS1, *e->WS2, e->bounds[0], e->bounds[1],
