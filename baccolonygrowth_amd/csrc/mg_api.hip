@@ -18,6 +18,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -811,6 +812,102 @@ int bcg_rd3d_deposit(bcg_rd3d* c, const double* cm, const double* rate, int32_t 
     }
     if ((rc = dep_run(c, PointSources{c->dcm}, n, n_inside))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return BCG_OK;
+}
+
+}  // extern "C"
+
+// ---- deposition without allocation ---------------------------------------
+// Scratch of bcg_rd3d_deposit_noalloc: one array per device, held until the
+// process ends. Host code never passes `noalloc_scratch` itself to a kernel
+// (on the host it names a stand-in, not device memory): it uses the device
+// address that hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char noalloc_scratch[BCG_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex noalloc_mutex;
+void* noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of this device's scratch array (looked up once per device,
+// with that device current).
+int noalloc_scratch_of(bcg_rd3d* c, unsigned char** out) {
+    int dev = 0;
+    HIPCHK(c, hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(c, BCG_ERR_ARG, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(noalloc_mutex);
+    if (!noalloc_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(noalloc_scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(c, BCG_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        noalloc_ptr[dev] = ptr;
+    }
+    *out = (unsigned char*)noalloc_ptr[dev];
+    return 0;
+}
+
+__global__ void k_fill_int(int* __restrict__ a, long n, int v) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+}  // namespace
+
+extern "C" {
+
+int bcg_rd3d_deposit_noalloc(bcg_rd3d* c, const double* d_cm, const double* d_rate, int32_t n, double* d_q,
+                             void* stream) {
+    if (!c || n < 0 || !d_q || (n > 0 && (!d_cm || !d_rate))) return BCG_ERR_ARG;
+    Level& v = c->L[0];
+    const long nn = c->N;
+    const long ns = nn + 1;
+    const int nb = nblk(ns, SCAN_BLOCK_ITEMS);
+    // scratch layout: key[n], frac[3n], sorted[n], cnt[nn+1], start[nn+1], fill[nn], scan totals[nb]
+    const size_t o_key = 0;
+    const size_t o_frac = o_key + align256(sizeof(int) * (size_t)n);
+    const size_t o_sorted = o_frac + align256(sizeof(double) * 3 * (size_t)n);
+    const size_t o_cnt = o_sorted + align256(sizeof(int) * (size_t)n);
+    const size_t o_start = o_cnt + align256(sizeof(int) * (size_t)ns);
+    const size_t o_fill = o_start + align256(sizeof(int) * (size_t)ns);
+    const size_t o_tot = o_fill + align256(sizeof(int) * (size_t)nn);
+    const size_t need = o_tot + align256(sizeof(int) * (size_t)nb);
+    if (need > (size_t)BCG_NOALLOC_SCRATCH_BYTES)
+        return fail(c, BCG_ERR_CAPACITY,
+                    "deposition of %d sources on %ld vertices needs %zu bytes of scratch, more than the %d bytes "
+                    "(BCG_NOALLOC_SCRATCH_BYTES) this entry point has",
+                    n, nn, need, (int)BCG_NOALLOC_SCRATCH_BYTES);
+    unsigned char* base = nullptr;
+    int rc = noalloc_scratch_of(c, &base);
+    if (rc) return rc;
+    int* key = (int*)(base + o_key);
+    double* frac = (double*)(base + o_frac);
+    int* sorted = (int*)(base + o_sorted);
+    int* cnt = (int*)(base + o_cnt);
+    int* start = (int*)(base + o_start);
+    int* fillc = (int*)(base + o_fill);
+    int* tot = (int*)(base + o_tot);
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    // counters start at zero (a fill kernel: no memset node if the caller captures)
+    hipLaunchKernelGGL(k_fill_int, dim3(nblk(ns, 256)), dim3(256), 0, st, cnt, ns, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3(nblk(nn, 256)), dim3(256), 0, st, fillc, nn, 0);
+    if (n > 0)
+        hipLaunchKernelGGL(k_dep_key<PointSources>, dim3(nblk(n, 256)), dim3(256), 0, st, PointSources{d_cm}, n,
+                           c->prm.origin[0], c->prm.origin[1], c->prm.origin[2], 1.0 / c->prm.h, v.nx, v.ny, v.nz, key,
+                           frac, cnt, (double*)nullptr);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, st, cnt, (int)ns, tot);
+    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(1024), 0, st, tot, nb, (int*)nullptr);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, st, cnt, (int)ns, tot, start);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_dep_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, key, n, start, fillc, sorted);
+        hipLaunchKernelGGL(k_dep_sort, dim3(nblk(nn, 256)), dim3(256), 0, st, start, nn, sorted);
+    }
+    hipLaunchKernelGGL(k_dep_gather, grid2(v), dim3(64, 4), 0, st, start, sorted, frac, d_rate, v.nx, v.ny, v.nz,
+                       1.0 / (c->prm.h * c->prm.h * c->prm.h), d_q);
+    HIPCHK(c, hipGetLastError());
     return BCG_OK;
 }
 

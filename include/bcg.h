@@ -273,6 +273,24 @@ int bcg_rd3d_set_coeffs(bcg_rd3d* ctx, double D, double K);
  *   *dL_dK, *dL_dD                                                          */
 int bcg_rd3d_adjoint(bcg_rd3d* ctx, const double* g, int32_t mode, double tol, int32_t max_iter,
                      double* dL_dq, double* dL_dcold, double* dL_dK, double* dL_dD);
+/* bcg_rd3d_deposit without any allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Inputs and output are DEVICE pointers: d_cm[3n] centres,
+ * d_rate[n] rates, d_q[nz*ny*nx] receives the uptake density (the same
+ * values, bit for bit, that bcg_rd3d_deposit leaves in the context). The
+ * context is used for its grid only; its own uptake field is not touched.
+ * The work is queued on `stream` (a hipStream_t; NULL: the context's
+ * stream) and the call returns without waiting for it. Its scratch (keys,
+ * fractions, bucket counts and starts) is one static device array per GPU
+ * of BCG_NOALLOC_SCRATCH_BYTES (16 MiB, held until the process ends, e.g.
+ * 2^18 sources on 129 x 129 x 33 vertices); larger cases return
+ * BCG_ERR_CAPACITY. Every call on a GPU shares that array: calls that
+ * overlap on one GPU (other streams, other threads) race and must be
+ * serialised by the caller.                                                 */
+#define BCG_NOALLOC_SCRATCH_BYTES (16 << 20)
+int bcg_rd3d_deposit_noalloc(bcg_rd3d* ctx, const double* d_cm, const double* d_rate, int32_t n, double* d_q,
+                             void* stream);
 /* Transpose of bcg_rd3d_deposit: out[i] = sum_v field[v] w_iv / h^3.     */
 int bcg_rd3d_gather(bcg_rd3d* ctx, const double* cm, int32_t n, const double* field, double* out);
 /* The 3-D step coupled to a mechanics context on the same device, without
