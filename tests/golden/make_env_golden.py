@@ -3,9 +3,10 @@ normal, nutrients) and of a coupled run of the time loop, from the
 reference's own code compiled from source into oracle/_ref/libbcgref.so.
 
 Run in a container that has the reference checkout:
-    make -C oracle ref && python tests/golden/make_env_golden.py
-Output (committed): tests/golden/golden_env.npz - inputs (grid, times,
-seed) and the reference's outputs only. The grid is the demo input's with
+    make -C oracle ref && python tests/golden/make_env_golden.py [large]
+Output (committed): tests/golden/golden_env.npz and golden_env_large.npz
+(the reference's second shipped input, below; "large" writes only that one)
+- inputs (grid, times, seed) and the reference's outputs only. The grid is the demo input's with
 BoxX = 128, BoxZ = 16, BoxZAgar = 64 - every multigrid level keeps an exact
 2:1 ratio in x and at least two z points, so that the reference reads no
 element outside its arrays - and the demo restart state 5 (129 cells).
@@ -148,7 +149,92 @@ def main():
     path = os.path.join(GOLDEN, "golden_env.npz")
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), "bytes;", len(lineage), "divisions in the loop")
+    large(ref)
+
+
+# ---------------------------------------------------------------- large grid
+# The reference's second shipped input (run_sim/inputFile_20mM_LargeSimulation
+# .txt: 1520 x 100 colony boxes, Box_z_agar 512, 3 nested agar levels) with
+# Update_Time 1e-4 h (2 outer iterations per update instead of 266), on the
+# demo restart state 8 (1952 cells). Every multigrid level keeps z points
+# (512 >= 2^(8 - 3)), so the reference reads nothing outside its arrays.
+# The agar levels (1521 x 513 points each) are stored as strided samples and
+# seeded weighted sums; the colony as the window around the colony, which
+# holds every point the update changes (the rest keeps LocalEnv()).
+LARGE_INPUT = os.path.join(GOLDEN, "inputFile_20mM_LargeSimulation.txt")
+LARGE_UPDATE_TIME = 1e-4
+LARGE_STRIDE = (8, 16)  # agar samples: every 8th z, every 16th x
+ALL_FIELDS = FIELDS + ("pH", "Hydrox", "Acetic", "Buffer")
+
+
+def large_params():
+    from baccolonygrowth_amd.cells import read_parameters
+
+    p, _ = read_parameters(LARGE_INPUT)
+    p.maxNeighbours, p.planar, p.maxCells = 96, 1, 4000
+    ep = env_params_from_file(LARGE_INPUT)
+    ep.UpdateTime = LARGE_UPDATE_TIME
+    return p, ep
+
+
+def agar_weights(n: int, k: int) -> np.ndarray:
+    """Weights of the k-th checksum of an n-point agar array."""
+    return np.random.default_rng(1000 + k).random(n)
+
+
+def pack_large(prefix: str, out: dict, ep, st: dict, window) -> None:
+    x0, x1 = window
+    bx, bz, bza, ml = ep.BoxX, ep.BoxZ, ep.BoxZAgar, ep.maxLevelsMG
+    wc = agar_weights(bx * bz, 2)
+    for k in ALL_FIELDS:
+        for b in ("colony", "colony_old"):
+            out[f"{prefix}_{b}_{k}"] = st[b][k].reshape(bx, bz)[x0:x1]
+            # the whole array (outside the window too) as a weighted sum
+            out[f"{prefix}_{b}_{k}_sums"] = np.array([st[b][k] @ wc, np.abs(st[b][k]).sum()])
+    for k in ("CarbonAgar", "OxygenAgar", "AcetateAgar", "WasteAgar"):
+        for w in ("wall", "wall_old"):
+            out[f"{prefix}_{w}_{k}"] = st[w][k]
+        if k == "WasteAgar":
+            continue
+        a = st["agar"][k].reshape(ml, bx, bza)
+        out[f"{prefix}_agar_{k}_sample"] = a[:, :: LARGE_STRIDE[1], :: LARGE_STRIDE[0]]
+        w0, w1 = agar_weights(bx * bza, 0), agar_weights(bx * bza, 1)
+        out[f"{prefix}_agar_{k}_sums"] = np.array([[a[l].ravel() @ w0, a[l].ravel() @ w1, np.abs(a[l]).sum()]
+                                                   for l in range(ml)])
+
+
+def large(ref):
+    from helpers import restart_cells as rc
+
+    p, ep = large_params()
+    _, _, cells = rc(8)
+    out = {"grid": np.array([ep.BoxX, ep.BoxZ, ep.BoxZAgar, ep.maxLevelsMG]),
+           "update_time": np.array(LARGE_UPDATE_TIME), "restart": np.array(8)}
+    sim = ref.sim(p)
+    sim.set_cells(cells)
+    env = RefEnv(ref, sim, ep)
+    env.fields(0)
+    f = env.get_fields()
+    for k in ("density_shift", "inside", "wall_shift", "bounds"):
+        out[f"init_{k}"] = f[k]
+    b = f["bounds"]
+    window = (max(int(b[0]) - 8, 0), min(int(b[1]) + 9, ep.BoxX))
+    out["window"] = np.array(window)
+    out["n1"] = np.array(env.nutrients())
+    pack_large("nut1", out, ep, env.get_state(), window)
+    env.fields(1)
+    env.fields(2)
+    out["n2"] = np.array(env.nutrients())
+    pack_large("nut2", out, ep, env.get_state(), window)
+    env.close()
+    sim.close()
+    path = os.path.join(GOLDEN, "golden_env_large.npz")
+    np.savez_compressed(path, **out)
+    print(path, os.path.getsize(path), "bytes")
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["large"]:
+        large(RefLib())
+    else:
+        main()

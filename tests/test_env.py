@@ -366,3 +366,126 @@ def test_demo_grid_update_is_bounded_and_placement_independent():
     gr = c["GrowthRate1"]
     assert np.all(np.isfinite(gr)) and gr.max() > 0
     say(f"    ok: growth rate range [{gr.min():.3g}, {gr.max():.3g}]")
+
+
+# ------------------------------------------------ the reference's large input
+GOLDEN_LARGE = os.path.join(GOLDEN, "golden_env_large.npz")
+LARGE_INPUT = os.path.join(GOLDEN, "inputFile_20mM_LargeSimulation.txt")
+LARGE_STRIDE = (8, 16)  # the fixture's agar samples: every 8th z, every 16th x
+
+
+def _large_params(g):
+    """run_sim/inputFile_20mM_LargeSimulation.txt as the reference reads it,
+    with the fixture's Update_Time (2 outer iterations an update)."""
+    from baccolonygrowth_amd.cells import read_parameters
+    from baccolonygrowth_amd.environment import env_params_from_file
+
+    p, _ = read_parameters(LARGE_INPUT)
+    p.maxNeighbours, p.planar, p.maxCells = 96, 1, 4000
+    ep = env_params_from_file(LARGE_INPUT)
+    ep.UpdateTime = float(g["update_time"])
+    return p, ep
+
+
+def _agar_weights(n, k):
+    return np.random.default_rng(1000 + k).random(n)
+
+
+def _check_large_state(st, g, ep, prefix):
+    bx, bz, bza, ml = ep.BoxX, ep.BoxZ, ep.BoxZAgar, ep.maxLevelsMG
+    x0, x1 = (int(v) for v in g["window"])
+    worst = 0.0
+    wc = _agar_weights(bx * bz, 2)
+    for k in FIELDS + BUFFER_FIELDS:
+        tol = RTOL_BUFFER if k in BUFFER_FIELDS else RTOL
+        for b in ("colony", "colony_old"):
+            full = st[b][k].reshape(bx, bz)
+            worst = max(worst, _close(full[x0:x1], g[f"{prefix}_{b}_{k}"], f"{prefix} {b} {k}", tol))
+            ws, asum = g[f"{prefix}_{b}_{k}_sums"]
+            assert abs(st[b][k] @ wc - ws) <= 10 * tol * max(asum, 1e-300), (prefix, b, k)
+    for k in ("CarbonAgar", "OxygenAgar", "AcetateAgar", "WasteAgar"):
+        for w in ("wall", "wall_old"):
+            got, exp = st[w][k], g[f"{prefix}_{w}_{k}"]
+            keep = np.ones(len(exp), bool)  # as _check_state: the coarser levels' middle is unpinned
+            for lv in range(1, ml):
+                keep[lv * bx + bx // 4: lv * bx + bx * 3 // 4] = False
+            worst = max(worst, _close(got[keep], exp[keep], f"{prefix} {w} {k}"))
+        if k == "WasteAgar":
+            continue
+        a = st["agar"][k].reshape(ml, bx, bza)
+        worst = max(worst, _close(a[:, :: LARGE_STRIDE[1], :: LARGE_STRIDE[0]], g[f"{prefix}_agar_{k}_sample"],
+                                  f"{prefix} agar {k} samples"))
+        sums = g[f"{prefix}_agar_{k}_sums"]
+        w0, w1 = _agar_weights(bx * bza, 0), _agar_weights(bx * bza, 1)
+        for lv in range(ml):
+            got = np.array([a[lv].ravel() @ w0, a[lv].ravel() @ w1])
+            # each term within 1e-13 of its value: the sums within 1e-13 of sum |x|
+            assert np.all(np.abs(got - sums[lv, :2]) <= 1e-12 * sums[lv, 2]), (prefix, k, lv, got, sums[lv])
+    return worst
+
+
+def test_golden_env_large_fixture_is_consistent():
+    g = np.load(GOLDEN_LARGE)
+    bx, bz, bza, ml = (int(v) for v in g["grid"])
+    assert (bx, bz, bza, ml) == (1520, 100, 512, 3)
+    assert bza >= 2 ** (8 - ml)  # every multigrid level keeps z points
+    x0, x1 = (int(v) for v in g["window"])
+    b = g["init_bounds"]
+    assert x0 <= b[0] - 2 and x1 >= b[1] + 3  # the window holds every swept box
+    assert int(g["n1"]) == int(3600 * float(g["update_time"]) * 740 / 100) * 200 == int(g["n2"])
+    assert g["nut1_colony_Carbon"].shape == (x1 - x0, bz)
+    assert g["nut2_agar_CarbonAgar_sample"].shape == (ml, (bx + 15) // 16, (bza + 7) // 8)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(GOLDEN), "..", "oracle", "_ref", "libbcgref.so")),
+                    reason="oracle/_ref (the reference compiled from source) is not built here")
+def test_reference_reproduces_golden_env_large():
+    from baccolonygrowth_amd.environment import ENV_DTYPE  # noqa: F401
+    from oracle.oracle import RefEnv, RefLib
+
+    g = np.load(GOLDEN_LARGE)
+    p, ep = _large_params(g)
+    ref = RefLib()
+    _, _, cells = restart_cells(int(g["restart"]))
+    sim = ref.sim(p)
+    sim.set_cells(cells)
+    env = RefEnv(ref, sim, ep)
+    env.fields(0)
+    assert np.array_equal(env.get_fields()["density_shift"], g["init_density_shift"])
+    assert env.nutrients() == int(g["n1"])
+    _check_large_state(env.get_state(), g, ep, "nut1")
+    env.close()
+    sim.close()
+
+
+@pytest.mark.gpu
+def test_large_simulation_grid_matches_reference():
+    """The reference's second shipped input: 1520 x 100 colony boxes, three
+    nested 1521 x 513 agar levels and five coarser ones, on the demo restart
+    state 8 - INIT fields, a nutrient update, UPDATE and HEIGHT fields, a
+    second update with the multigrid state carried over."""
+    from baccolonygrowth_amd.environment import ENV_HEIGHT, ENV_INIT, ENV_UPDATE, Environment
+    from baccolonygrowth_amd.simulation import Colony
+
+    g = np.load(GOLDEN_LARGE)
+    p, ep = _large_params(g)
+    _, _, cells = restart_cells(int(g["restart"]))
+    say(f"[gpu] LargeSimulation grid {ep.BoxX}x{ep.BoxZ}, agar {ep.BoxZAgar} x {ep.maxLevelsMG} levels, "
+        f"{len(cells)} cells")
+    with Colony(p) as col:
+        col.set_cells(cells)
+        _start_fields(col, p)
+        with Environment(col, ep) as env:
+            env.fields(ENV_INIT)
+            f = env.get_fields()
+            assert np.array_equal(f["bounds"], g["init_bounds"])
+            for k in ("density_shift", "inside", "wall_shift"):
+                _close(f[k], g[f"init_{k}"], f"init {k}")
+            assert env.nutrients() == int(g["n1"])
+            w = _check_large_state(env.state(), g, ep, "nut1")
+            say(f"    update 1 ok (max rel diff {w:.1e})")
+            env.fields(ENV_UPDATE)
+            env.fields(ENV_HEIGHT)
+            assert env.nutrients() == int(g["n2"])
+            w = _check_large_state(env.state(), g, ep, "nut2")
+            say(f"    update 2 ok (max rel diff {w:.1e})")
