@@ -101,6 +101,7 @@ SIGNATURES.update(
             [_vp, _dp, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, _dp, _dp, _dp, _dp],
         ),
         "bcg_rd3d_gather": (ctypes.c_int, [_vp, _dp, ctypes.c_int32, _dp, _dp]),
+        "bcg_rd3d_solve": (ctypes.c_int, [_vp, ctypes.c_int32, _dp, ctypes.c_double, ctypes.c_int32, _dp, _ip, _dp]),
         "bcg_rd3d_deposit_noalloc": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, _vp, _vp]),
     }
 )

@@ -148,3 +148,71 @@ def rollout_seq(rates_seq, D, K, c0, solver: Nutrients3D, cm_seq, checkpoint_eve
     for lo in range(0, n, checkpoint_every):
         c = checkpoint(window, c, lo, min(lo + checkpoint_every, n), use_reentrant=False)
     return c
+
+
+def laplacian_h2(u: np.ndarray) -> np.ndarray:
+    """h^2 times the 7-point Laplacian of the step's operator on [z][y][x]
+    arrays: mirror ghosts on the x and y faces and on the top face (rows of
+    the bottom face are Dirichlet; callers set them themselves)."""
+    p = np.pad(u, 1, mode="reflect")
+    out = p[1:-1, 1:-1, 2:] + p[1:-1, 1:-1, :-2]
+    out += p[1:-1, 2:, 1:-1]
+    out += p[1:-1, :-2, 1:-1]
+    out += p[2:, 1:-1, 1:-1]
+    out += p[:-2, 1:-1, 1:-1]
+    out -= 6.0 * u
+    return out
+
+
+def loss_difference(solver: Nutrients3D, cm_seq, rates_minus, drates, c0, target, D_minus: float, D_plus: float,
+                    K_minus: float, K_plus: float, max_iter: int = 30) -> float:
+    """L(+) - L(-) for L = sum (c_n - target)^2, c_n the field after one
+    implicit FP64 step per source set (cm_seq[k], rates), from c0; (-) has
+    diffusivity D_minus, Monod constant K_minus and rates rates_minus[k], (+)
+    has D_plus, K_plus and rates_minus[k] + drates[k] (drates[k] may be None).
+
+    Evaluated in difference form: with A-/A+ the two steps' operators, the
+    difference d = c+ - c- of every step solves exactly
+        A+_k d_k = d_{k-1} + (A-_k - A+_k) c-_k,
+        A-_k - A+_k = diag(a0-_k - a0+_k) + (lam+ - lam-) Lap_h,
+    a0 = 1 + dt q / (K + c_prev), lam = dt D / h^2, with a0- - a0+ formed from
+    the differences (dq, dK, d_{k-1}) directly. Subtracting two separately
+    solved fields instead loses the difference below the solves' round-off
+    floor (the operator's terms are ~lam |c|, lam ~ 1e3); here each d_k is
+    solved to round-off relative to itself, so a central difference at
+    h = 1e-6 is as accurate as at h = 1e-2, less the h^2 truncation."""
+    prm = solver.params
+    dt, h = prm.dt, prm.h
+    dlam = dt * (D_plus - D_minus) / (h * h)
+    dK = K_plus - K_minus
+    dn = np.ascontiguousarray(c0, dtype=np.float64)
+    d = np.zeros_like(dn)
+    for k in range(len(cm_seq)):
+        cm = cm_seq[k]
+        # (-) step from dn
+        set_coeffs(solver, D_minus, K_minus)
+        solver.deposit(cm, rates_minus[k])
+        q_minus = solver.uptake()
+        solver.set_field(dn)
+        solver.step("fp64", tol=1e-16, max_iter=max_iter, strict=False)
+        dn_new = solver.field()
+        if drates is not None and drates[k] is not None:
+            solver.deposit(cm, drates[k])
+            dq = solver.uptake()
+            rates_plus = rates_minus[k] + drates[k]
+        else:
+            dq = 0.0
+            rates_plus = rates_minus[k]
+        up_prev = dn + d
+        km = K_minus + dn
+        da0 = dt * (q_minus * (dK + d) - dq * km) / (km * (K_plus + up_prev))
+        rhs = d + da0 * dn_new + dlam * laplacian_h2(dn_new)
+        rhs[0] = 0.0
+        # (+) operator: its coefficients, sources and c_prev = up_prev
+        set_coeffs(solver, D_plus, K_plus)
+        solver.deposit(cm, rates_plus)
+        set_state(solver, up_prev, dn_new)
+        d, _, _ = solver.solve(rhs, "fp64", tol=1e-16, max_iter=max_iter, strict=False)
+        dn = dn_new
+    t = np.asarray(target, dtype=np.float64)
+    return float((d * (2.0 * dn + d - 2.0 * t)).sum())

@@ -1160,6 +1160,35 @@ int bcg_rd3d_adjoint(bcg_rd3d* c, const double* g, int32_t mode, double tol, int
     return restore();
 }
 
+int bcg_rd3d_solve(bcg_rd3d* c, int32_t mode, const double* rhs, double tol, int32_t max_iter, double* x,
+                   int32_t* iters, double* relres) {
+    if (!c || !rhs || !x || (mode != 0 && mode != 1) || max_iter < 1 || !(tol > 0)) return BCG_ERR_ARG;
+    Level& f = c->L[0];
+    const size_t N = (size_t)c->N;
+    int rc = upload(c, &c->dfield, N, rhs);
+    if (rc) return rc;
+    // keep the field; b <- rhs; solve from x = 0
+    HIPCHK(c, hipMemcpyAsync(c->save, c->x[c->cur], N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->b, c->dfield, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL((k_fill<double>), dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, c->x[c->cur], c->N, 0.0);
+    int it = 0;
+    double rr = 0;
+    rc = solve_core(c, mode, tol, max_iter, &it, &rr);
+    const std::string msg = c->err;
+    // the last iterate goes out whether or not tol was reached
+    HIPCHK(c, hipMemcpyAsync(x, c->x[c->cur], N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    // the step's state comes back: the field and its right-hand side
+    HIPCHK(c, hipMemcpyAsync(c->x[c->cur], c->save, N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL(k_rhs, dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, c->cprev, c->b, c->prm.c_res,
+                       (long)f.nx * f.ny, c->N);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (iters) *iters = it;
+    if (relres) *relres = rr;
+    if (rc) c->err = msg;
+    return rc;
+}
+
 int bcg_rd3d_gather(bcg_rd3d* c, const double* cm, int32_t n, const double* field, double* out) {
     if (!c || n < 0 || (n > 0 && (!cm || !field || !out))) return BCG_ERR_ARG;
     if (n == 0) return BCG_OK;

@@ -146,6 +146,21 @@ class Nutrients3D:
         self._check(rc)
         return it.value, rr.value
 
+    def solve(self, rhs, mode: str = "fp64", tol: float = 1e-12, max_iter: int = 100, strict: bool = True):
+        """x with A x = rhs for the operator of the last step (or set_state),
+        from x = 0; returns (x, iterations, ||rhs - A x|| / ||rhs||). The
+        field and the step's right-hand side are unchanged. strict=False
+        keeps the last iterate when tol is not reached."""
+        rhs = self._arr(rhs)
+        x = np.empty(self.shape)
+        it = ctypes.c_int32()
+        rr = ctypes.c_double()
+        rc = lib().bcg_rd3d_solve(self._h, MODES[mode], rhs.ctypes.data_as(_dp), tol, max_iter, x.ctypes.data_as(_dp),
+                                  ctypes.byref(it), ctypes.byref(rr))
+        if not (rc == -7 and not strict):
+            self._check(rc)
+        return x, it.value, rr.value
+
     def vcycle(self, mode: str = "fp64") -> float:
         ms = ctypes.c_float()
         self._check(lib().bcg_rd3d_vcycle(self._h, MODES[mode], ctypes.byref(ms)))

@@ -318,12 +318,11 @@ def adjoint_run(n: int = 1 << 18) -> dict:
     lognormal(0, 0.3)): loss sum (c - c*(r))^2, c* = 20 (1 - 0.8 exp(-r/150))
     mM, with respect to every cell's rate, D and K through
     torch.autograd; checked by central differences (h = 1e-6, 1e-4 and 1e-2
-    relative; the loss difference summed from the two fields' difference)
-    for D, K and the three cells of largest gradient, and the
-    mixed-precision gradient against the FP64 one."""
+    relative, evaluated in difference form) for D, K and the three cells of
+    largest gradient, and the mixed-precision gradient against the FP64 one."""
     import torch
 
-    from baccolonygrowth_amd.adjoint import nutrient_step, set_coeffs
+    from baccolonygrowth_amd.adjoint import loss_difference, nutrient_step
     from baccolonygrowth_amd.colony import dome_colony
     from baccolonygrowth_amd.nutrients import Nutrients3D, default_rd3d_params
 
@@ -370,36 +369,28 @@ def adjoint_run(n: int = 1 << 18) -> dict:
         pick = np.argsort(-np.abs(gr))[:3]
 
         c0_np = c0.numpy()
-
-        def field(rates, D, K):
-            # FP64 solve run to the round-off floor (a fixed count of V-cycles,
-            # no tolerance test): the field moves by ~1e-6 relative at h = 1e-6,
-            # far above that floor but not far above a 1e-12 stopping residual
-            set_coeffs(s, D, K)
-            s.deposit(cm, rates)
-            s.set_field(c0_np)
-            s.step("fp64", tol=1e-16, max_iter=FD_VCYCLES, strict=False)
-            return torch.from_numpy(s.field())
-
-        def dloss(a, b):
-            # L(a) - L(b) as sum (a - b)(a + b - 2 c*): no cancellation of the two ~1e8 sums
-            return float(((a - b) * (a + b - 2.0 * target)).sum())
+        tgt = target.numpy()
 
         fd = {}
-        with torch.no_grad():
-            for h in (1e-6, 1e-4, 1e-2):
-                e = {}
-                for name, v0, g in (("D", D0, gD), ("K", K0, gK)):
-                    up = field(base, D0 * (1 + h) if name == "D" else D0, K0 * (1 + h) if name == "K" else K0)
-                    dn = field(base, D0 * (1 - h) if name == "D" else D0, K0 * (1 - h) if name == "K" else K0)
-                    e[name] = abs(dloss(up, dn) / (2 * h * v0) - g) / abs(g)
-                for i in pick:
-                    ru, rd = base.copy(), base.copy()
-                    ru[i] *= 1 + h
-                    rd[i] *= 1 - h
-                    e[f"rate[{int(i)}]"] = abs(dloss(field(ru, D0, K0), field(rd, D0, K0)) / (2 * h * base[i]) - gr[i]) / abs(gr[i])
-                fd[f"h={h:g}"] = e
-                log(f"[adjoint] central differences h={h:g}: " + ", ".join(f"{k} {v:.2e}" for k, v in e.items()))
+        # central differences in difference form (loss_difference): each
+        # step's field difference solved directly, so that h = 1e-6 is not
+        # lost below the solves' round-off floor
+        for h in (1e-6, 1e-4, 1e-2):
+            e = {}
+            for name, v0, g in (("D", D0, gD), ("K", K0, gK)):
+                Dm, Dp = (D0 * (1 - h), D0 * (1 + h)) if name == "D" else (D0, D0)
+                Km, Kp = (K0 * (1 - h), K0 * (1 + h)) if name == "K" else (K0, K0)
+                dL = loss_difference(s, [cm], [base], None, c0_np, tgt, Dm, Dp, Km, Kp, max_iter=FD_VCYCLES)
+                e[name] = abs(dL / (2 * h * v0) - g) / abs(g)
+            for i in pick:
+                rm = base.copy()
+                rm[i] *= 1 - h
+                dr = np.zeros_like(base)
+                dr[i] = 2 * h * base[i]
+                dL = loss_difference(s, [cm], [rm], [dr], c0_np, tgt, D0, D0, K0, K0, max_iter=FD_VCYCLES)
+                e[f"rate[{int(i)}]"] = abs(dL / (2 * h * base[i]) - gr[i]) / abs(gr[i])
+            fd[f"h={h:g}"] = e
+            log(f"[adjoint] central differences h={h:g}: " + ", ".join(f"{k} {v:.2e}" for k, v in e.items()))
     out.update({"cells_in_grid": int(inside), "gradient": {"D": gD, "K": gK, "rates_max_abs": float(np.abs(gr).max())},
                 "fd_rel_error": fd})
     return out
@@ -428,7 +419,7 @@ def coupled_run(steps: int = 200, update_every: int = 10, ckpt: int = 5) -> dict
     the last field is backpropagated through all the updates (checkpoints
     every `ckpt`, recomputation in between; forward and adjoint solves to
     1e-11) to D, K and the cells' rates, checked by central differences
-    (h = 1e-6 relative)."""
+    (h = 1e-6 and 1e-4 relative, evaluated in difference form)."""
     import torch
 
     from baccolonygrowth_amd.adjoint import rollout_seq
@@ -509,37 +500,27 @@ def coupled_run(steps: int = 200, update_every: int = 10, ckpt: int = 5) -> dict
         grads = {"D": Dt.grad.item(), "K": Kt.grad.item(), "rate_factor": at.grad.item()}
         log(f"[run] forward + backward through {len(iters)} updates: {t_adj:.1f} s; dL/dD {grads['D']:.6e}, "
             f"dL/dK {grads['K']:.6e}, dL/d(rate factor) {grads['rate_factor']:.6e}")
-        # central differences at two relative steps, 1e-6 and 1e-4; each
-        # forward of the 20 updates solved to the FP64 round-off floor (a
-        # fixed V-cycle count), and the loss difference summed from the two
-        # final fields' difference (no cancellation of the two large sums)
-        from baccolonygrowth_amd.adjoint import set_coeffs
+        # central differences at two relative steps, 1e-6 and 1e-4, of the
+        # FP64 chain of the 20 updates in difference form (loss_difference:
+        # each update's field difference solved directly, to round-off
+        # relative to itself)
+        from baccolonygrowth_amd.adjoint import loss_difference
 
         tgt = target.numpy()
-
-        def fd_field(D, K, a):
-            set_coeffs(s, D, K)
-            c = c_init
-            for k in range(len(seq_rate)):
-                s.deposit(seq_cm[k], a * seq_rate[k])
-                s.set_field(c)
-                s.step("fp64", tol=1e-16, max_iter=FD_VCYCLES, strict=False)
-                c = s.field()
-            return c
-
         fd = {1e-6: {}, 1e-4: {}}
-        with torch.no_grad():
-            for rel_h in fd:
-                for name, v0 in (("D", D0), ("K", K0), ("rate_factor", 1.0)):
-                    fields_pm = []
-                    for sgn in (1.0, -1.0):
-                        args_ = {"D": D0, "K": K0, "rate_factor": 1.0}
-                        args_[name] = v0 * (1.0 + sgn * rel_h)
-                        fields_pm.append(fd_field(args_["D"], args_["K"], args_["rate_factor"]))
-                    up, dn = fields_pm
-                    fd[rel_h][name] = float(((up - dn) * (up + dn - 2.0 * tgt)).sum()) / (2.0 * rel_h * v0)
-                    log(f"[run] dL/d{name}: adjoint {grads[name]:.8e}, central difference (h {rel_h:g}) "
-                        f"{fd[rel_h][name]:.8e}")
+        for rel_h in fd:
+            for name, v0 in (("D", D0), ("K", K0), ("rate_factor", 1.0)):
+                Dm, Dp = (D0 * (1 - rel_h), D0 * (1 + rel_h)) if name == "D" else (D0, D0)
+                Km, Kp = (K0 * (1 - rel_h), K0 * (1 + rel_h)) if name == "K" else (K0, K0)
+                if name == "rate_factor":
+                    rm = [r * (1 - rel_h) for r in seq_rate]
+                    dr = [r * (2 * rel_h) for r in seq_rate]
+                else:
+                    rm, dr = seq_rate, None
+                dL = loss_difference(s, seq_cm, rm, dr, c_init, tgt, Dm, Dp, Km, Kp, max_iter=FD_VCYCLES)
+                fd[rel_h][name] = dL / (2.0 * rel_h * v0)
+                log(f"[run] dL/d{name}: adjoint {grads[name]:.8e}, central difference (h {rel_h:g}) "
+                    f"{fd[rel_h][name]:.8e}")
     err = {h: {k: abs(grads[k] - fd[h][k]) / abs(fd[h][k]) for k in grads} for h in fd}
     return {
         "cells": RUN_CELLS,

@@ -291,6 +291,16 @@ int bcg_rd3d_adjoint(bcg_rd3d* ctx, const double* g, int32_t mode, double tol, i
 #define BCG_NOALLOC_SCRATCH_BYTES (16 << 20)
 int bcg_rd3d_deposit_noalloc(bcg_rd3d* ctx, const double* d_cm, const double* d_rate, int32_t n, double* d_q,
                              void* stream);
+/* The system of the last set-up (bcg_rd3d_step or bcg_rd3d_set_state: the
+ * same a0 and D) for another right-hand side: A x = rhs from x = 0, to
+ * ||rhs - A x|| <= tol ||rhs|| (Dirichlet rows: x = rhs). x[N] (host)
+ * receives the last iterate, also when tol is not reached (BCG_ERR_STATE);
+ * the context's field and right-hand side are left as they were. No
+ * counterpart in the reference: it serves sensitivity solves, e.g. the
+ * difference of two steps whose operators differ, A1 (c1 - c2) =
+ * (A2 - A1) c2, solved to the precision of the difference itself.         */
+int bcg_rd3d_solve(bcg_rd3d* ctx, int32_t mode, const double* rhs, double tol, int32_t max_iter, double* x,
+                   int32_t* iters, double* relres);
 /* Transpose of bcg_rd3d_deposit: out[i] = sum_v field[v] w_iv / h^3.     */
 int bcg_rd3d_gather(bcg_rd3d* ctx, const double* cm, int32_t n, const double* field, double* out);
 /* The 3-D step coupled to a mechanics context on the same device, without

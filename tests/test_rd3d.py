@@ -601,3 +601,76 @@ def test_overlapped_coupled_steps_equal_serialised():
         assert fa.tobytes() == fb.tobytes() and fa.min() < 20.0
         assert ca.fields()[3].tobytes() == cb.fields()[3].tobytes()
         assert len(ca.cells()) >= len(cells) and ca.cells().tobytes() == cb.cells().tobytes()
+
+
+@pytest.mark.gpu
+def test_difference_form_central_differences():
+    """loss_difference (each step's field difference solved directly,
+    bcg_rd3d_solve) equals the difference of separately solved losses where
+    that is accurate (h = 1e-2), and at h = 1e-6 its central differences
+    match the adjoint gradients of a 3-step chain with per-step sources
+    (D, K, a global rate factor, one cell's rate) to 1e-7."""
+    import torch
+
+    from baccolonygrowth_amd.adjoint import loss_difference, rollout_seq, set_coeffs
+
+    shape = (9, 17, 17)
+    h, D0, dt, K0 = 2.0, 600.0, 10.0, 0.1
+    rng = np.random.default_rng(12)
+    n = 400
+    cms = [np.column_stack([rng.uniform(4, 28, n), rng.uniform(4, 28, n), rng.uniform(0.5, 12, n)]) for _ in range(3)]
+    bases = [rng.lognormal(0.0, 0.3, n) * 3.0 for _ in range(3)]
+    target = 20.0 * (1 - 0.3 * rng.random(shape))
+    c0 = np.full(shape, 20.0)
+    print("[gpu] difference-form central differences", flush=True)
+    with _solver(shape, h, D0, dt, K0) as s:
+        def plain_loss(D, K, a, rates=None):
+            set_coeffs(s, D, K)
+            c = c0
+            for k in range(3):
+                s.deposit(cms[k], a * (bases[k] if rates is None or k != 1 else rates))
+                s.set_field(c)
+                s.step("fp64", tol=1e-16, max_iter=30, strict=False)
+                c = s.field()
+            return float(((c - target) ** 2).sum())
+
+        def diff(name, eps, i=7):
+            Dm, Dp, Km, Kp = D0, D0, K0, K0
+            rm, dr = list(bases), [None, None, None]
+            if name == "D":
+                Dm, Dp = D0 * (1 - eps), D0 * (1 + eps)
+            elif name == "K":
+                Km, Kp = K0 * (1 - eps), K0 * (1 + eps)
+            elif name == "a":
+                rm = [b * (1 - eps) for b in bases]
+                dr = [b * (2 * eps) for b in bases]
+            else:  # one cell's rate in the second step
+                r = bases[1].copy()
+                r[i] *= 1 - eps
+                rm[1] = r
+                dr[1] = np.zeros(n)
+                dr[1][i] = 2 * eps * bases[1][i]
+            return loss_difference(s, cms, rm, dr, c0, target, Dm, Dp, Km, Kp)
+
+        # h = 1e-2: both forms are accurate; they agree
+        e = 1e-2
+        for name, args in (("D", ((D0 * (1 + e), K0, 1.0), (D0 * (1 - e), K0, 1.0))),
+                           ("K", ((D0, K0 * (1 + e), 1.0), (D0, K0 * (1 - e), 1.0))),
+                           ("a", ((D0, K0, 1 + e), (D0, K0, 1 - e)))):
+            ref = plain_loss(*args[0]) - plain_loss(*args[1])
+            got = diff(name, e)
+            assert got == pytest.approx(ref, rel=1e-8), name
+        # the adjoint gradients of the same chain
+        rates = [torch.tensor(b, requires_grad=True) for b in bases]
+        Dt = torch.tensor(D0, dtype=torch.float64, requires_grad=True)
+        Kt = torch.tensor(K0, dtype=torch.float64, requires_grad=True)
+        c = rollout_seq(rates, Dt, Kt, torch.from_numpy(c0), s, cms, checkpoint_every=2, mode="fp64", tol=1e-12)
+        ((c - torch.from_numpy(target)) ** 2).sum().backward()
+        g = {"D": Dt.grad.item() * D0, "K": Kt.grad.item() * K0,
+             "a": sum(float((r.grad * torch.from_numpy(b)).sum()) for r, b in zip(rates, bases)),
+             "r": rates[1].grad[7].item() * bases[1][7]}
+        e = 1e-6
+        for name in ("D", "K", "a", "r"):
+            fd = diff(name, e) / (2 * e)
+            print(f"    {name}: adjoint {g[name]:.12e}  central difference (h 1e-6) {fd:.12e}", flush=True)
+            assert fd == pytest.approx(g[name], rel=1e-7), name
