@@ -49,6 +49,8 @@ SIGNATURES = {
     "bcg_mean_stress": (ctypes.c_int, [_vp, _dp, _dp]),
     "bcg_step": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int32, _ip, _ip, ctypes.c_int32]),
     "bcg_last_step_stats": (ctypes.c_int, [_vp, _i64p, _ip, _ip]),
+    "bcg_step_out": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int32, _vp, ctypes.c_int32, _ip,
+                                    _ip, ctypes.c_int32]),
     "bcg_last_step_times": (ctypes.c_int, [_vp, _fp]),
 }
 
@@ -160,6 +162,7 @@ SIGNATURES.update(
         "bcg_env_destroy": (ctypes.c_int, [_vp]),
         "bcg_env_last_error": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int]),
         "bcg_env_set_lds_limit": (ctypes.c_int, [_vp, ctypes.c_int64]),
+        "bcg_env_set_execution": (ctypes.c_int, [_vp, ctypes.c_int32]),
         "bcg_env_set_state": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "bcg_env_get_state": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "bcg_env_fields": (ctypes.c_int, [_vp, ctypes.c_int32]),
@@ -171,6 +174,8 @@ SIGNATURES.update(
         ),
         "bcg_clock_advance": (ctypes.c_int, [ctypes.POINTER(Clock)]),
         "bcg_sim_step": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(Clock), _ip, _ip, ctypes.c_int32]),
+        "bcg_sim_prepare": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(Clock)]),
+        "bcg_sim_advance": (ctypes.c_int, [_vp, ctypes.POINTER(Clock), _vp, ctypes.c_int32, _ip, _ip, ctypes.c_int32]),
     }
 )
 

@@ -375,19 +375,18 @@ int bcg_set_cells(bcg_ctx* c, const bcg_cell* cells, int32_t n) {
     return check_err(c);
 }
 
-int bcg_get_cells(bcg_ctx* c, bcg_cell* cells, int32_t cap, int32_t* n) {
-    if (!c || !n) return BCG_ERR_ARG;
-    if (int rc_ = ctx_check_failed(c)) return rc_;
-    *n = c->N;
-    if (cap < c->N || (c->N > 0 && !cells)) return fail(c, BCG_ERR_ARG, "buffer of %d cells < %d", cap, c->N);
-    HIPCHK(c, ctx_join(c));
-    const int N = c->N;
+}  // extern "C"
+
+namespace {
+
+// Cells 0..N-1 of hot buffer `buf` with the cold fields, in the reference's
+// record (the context's stream is idle when this is called).
+int download_cells(bcg_ctx* c, int buf, int N, bcg_cell* cells) {
     if (N == 0) return BCG_OK;
     std::vector<double> hot((size_t)N * HOT), tp(N), tq(N), df(3 * (size_t)N), sf(3 * (size_t)N);
     std::vector<int> ap(N), aq(N), ty(N), an(N);
-    HIPCHK(c, hipStreamSynchronize(c->stream));
     auto dn = [&](void* h, const void* d, size_t b) { return hipMemcpy(h, d, b, hipMemcpyDeviceToHost); };
-    HIPCHK(c, dn(hot.data(), c->hot[c->cur], hot.size() * sizeof(double)));
+    HIPCHK(c, dn(hot.data(), c->hot[buf], hot.size() * sizeof(double)));
     HIPCHK(c, dn(tp.data(), c->time_p, N * sizeof(double)));
     HIPCHK(c, dn(tq.data(), c->time_q, N * sizeof(double)));
     HIPCHK(c, dn(df.data(), c->dynfric, df.size() * sizeof(double)));
@@ -419,6 +418,20 @@ int bcg_get_cells(bcg_ctx* c, bcg_cell* cells, int32_t cap, int32_t* n) {
         s.Ancestor = an[i];
     }
     return BCG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bcg_get_cells(bcg_ctx* c, bcg_cell* cells, int32_t cap, int32_t* n) {
+    if (!c || !n) return BCG_ERR_ARG;
+    if (int rc_ = ctx_check_failed(c)) return rc_;
+    *n = c->N;
+    if (cap < c->N || (c->N > 0 && !cells)) return fail(c, BCG_ERR_ARG, "buffer of %d cells < %d", cap, c->N);
+    HIPCHK(c, ctx_join(c));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return download_cells(c, c->cur, c->N, cells);
 }
 
 int bcg_num_cells(bcg_ctx* c, int32_t* n) {
@@ -582,7 +595,7 @@ namespace {
 // a failure cannot take back (move/grow's friction accumulators, the other
 // cell buffer, then the rng position and the lists).
 int step_body(bcg_ctx* c, double dt, double t, int32_t flags, int32_t* n_div, int32_t* lineage, int32_t lineage_cap,
-              bool* started) {
+              bcg_cell* moved, bool* started) {
     int rc;
     const int N0 = c->N;
     const int in = c->cur, out = c->cur ^ 1;
@@ -621,6 +634,9 @@ int step_body(bcg_ctx* c, double dt, double t, int32_t flags, int32_t* n_div, in
     HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
     if ((rc = read_stats(c))) return rc;
     if ((rc = check_err(c))) return rc;
+    // the moved and grown cells before the divisions (what the reference's
+    // output step saves, ABM/Simulation.cpp:349)
+    if (moved && (rc = download_cells(c, out, N0, moved))) return rc;
     // draws: every F_cw call took one (values unused: var_pos == 0), then
     // 9 per division in ascending mother ID (ABM/Simulation.cpp:380-404)
     c->rng.discard(c->last.wall_draws);
@@ -686,10 +702,25 @@ int bcg_step(bcg_ctx* c, double dt, double t, int32_t flags, int32_t* n_div, int
     if (n_div) *n_div = 0;
     if (int rc = ctx_check_failed(c)) return rc;
     bool started = false;
-    const int rc = step_body(c, dt, t, flags, n_div, lineage, lineage_cap, &started);
+    const int rc = step_body(c, dt, t, flags, n_div, lineage, lineage_cap, nullptr, &started);
     if (rc && started) {
         // the friction accumulators and possibly the rng, the other buffer and
         // the lists have moved on: nothing but bcg_set_cells may follow
+        c->failed = true;
+        c->lists_valid = false;
+    }
+    return rc;
+}
+
+int bcg_step_out(bcg_ctx* c, double dt, double t, int32_t flags, bcg_cell* moved, int32_t moved_cap, int32_t* n_div,
+                 int32_t* lineage, int32_t lineage_cap) {
+    if (!c) return BCG_ERR_ARG;
+    if (n_div) *n_div = 0;
+    if (int rc = ctx_check_failed(c)) return rc;
+    if (moved && moved_cap < c->N) return fail(c, BCG_ERR_ARG, "buffer of %d cells < %d", moved_cap, c->N);
+    bool started = false;
+    const int rc = step_body(c, dt, t, flags, n_div, lineage, lineage_cap, moved, &started);
+    if (rc && started) {
         c->failed = true;
         c->lists_valid = false;
     }

@@ -36,6 +36,7 @@ struct bcg_env {
     double* aga[MG_LEVELS] = {};
     double* res[MG_LEVELS] = {};
     double* err[MG_LEVELS] = {};
+    double* rt[MG_LEVELS] = {};
     int mgx[MG_LEVELS] = {}, mgz[MG_LEVELS] = {};
     double* scol[2] = {nullptr, nullptr};   // working copies of the nutrient update
     double* swall[2] = {nullptr, nullptr};
@@ -43,6 +44,8 @@ struct bcg_env {
     double *insT = nullptr, *dshT = nullptr, *phantom = nullptr;
     size_t lds_device = 0;  // dynamic LDS the nutrient kernel can take on this device
     size_t lds_budget = 0;  // ... and may take (bcg_env_set_lds_limit)
+    size_t lds_device_wg = 0;  // the same for the one-workgroup sweeps kernel
+    int exec = 0;              // BCG_ENV_EXEC_* (bcg_env_set_execution)
     double *rough = nullptr, *rough1 = nullptr, *rough2 = nullptr, *inside = nullptr;
     double *rwall = nullptr, *rwall1 = nullptr, *rwall2 = nullptr;
     double *rshift = nullptr, *rshift1 = nullptr, *rshift2 = nullptr;
@@ -54,6 +57,8 @@ struct bcg_env {
     double *filter3d = nullptr, *filter1d = nullptr;
     int* bounds = nullptr;   // device
     int* errflag = nullptr;  // device
+    EnvArgs* dargs = nullptr;  // device copy of the update's arguments (phase kernels)
+    EnvArgs* hargs = nullptr;  // pinned staging of it
     int hbounds[5] = {0, 0, 0, 0, 0};
     bool have_fields = false;
     std::string err_text;
@@ -153,9 +158,159 @@ int read_flags(bcg_env* e, int* flags) {
     return 0;
 }
 
+// x extent of multigrid level l as the update's loops see it
+// (Multigrid.cpp's BoxX * 2^agarLevels / 2^l / 2, capped at BoxX).
+int level_nx(const bcg_env* e, int l) {
+    const long v = (long)e->BX * (1L << e->ML) / (1L << l) / 2;
+    return (int)std::min((long)e->BX, v);
+}
+
+// The update as phase launches on the mechanics stream (the order of
+// k_env_nutrients, one launch per barrier there). Levels below `lc` are
+// wide (more column walks a colour than one workgroup runs at once): their
+// relaxation passes, restriction and interpolation are device-wide
+// launches; levels from lc down run in one workgroup per V-cycle.
+struct Phased {
+    bcg_env* e;
+    const EnvArgs& a;
+    const EnvArgs* da;  // device copy of a (the grid-wide phase kernels)
+    hipStream_t st;
+    int lc;
+    bool wg_sweeps;
+    size_t wg_lds;
+    size_t coarse_lds;  // dynamic LDS of the tail's coarse solve (0: the one-wave or global path)
+    dim3 pts(long n) const {
+        return dim3((unsigned)std::max(1L, std::min((long)PH_MAX_BLOCKS, (n + PH_THREADS - 1) / PH_THREADS)));
+    }
+    long level_points(int l) const { return (long)(level_nx(e, l) + 1) * (a.mgz[l] + 1); }
+    void outer(bool last) const {
+        const int AL = a.ML, BX = a.BX;
+        const dim3 T(PH_THREADS);
+        long all = 0;
+        for (int l = 0; l < MG_LEVELS; l++) all = std::max(all, (long)(a.mgx[l] + 1) * (a.mgz[l] + 1));
+        hipLaunchKernelGGL(k_nut_copy_to_mg, pts(std::max(BX, a.BZA + 1)), T, 0, st, da, 0);
+        hipLaunchKernelGGL(k_nut_reset_zero, pts(all), T, 0, st, da);
+        hipLaunchKernelGGL(k_nut_update_sol, pts(all), T, 0, st, da);
+        hipLaunchKernelGGL(k_nut_update_res, pts(all), T, 0, st, da);
+        hipLaunchKernelGGL(k_nut_rterm, pts(3 * level_points(0)), T, 0, st, da, 0);
+        auto relax3 = [&](int l) {
+            const int nx = level_nx(e, l);
+            const long walks = 3L * (nx / 2 + 1);
+            const dim3 g((unsigned)((walks + PH_RELAX_THREADS - 1) / PH_RELAX_THREADS));
+            for (int j = 0; j < 3; j++)
+                for (int par = 0; par < 2; par++)
+                    hipLaunchKernelGGL(k_nut_relax, g, dim3(PH_RELAX_THREADS), 0, st, da, l, par);
+        };
+        for (int cyc = 0; cyc < 3; cyc++) {
+            for (int l = 0; l < lc; l++) {
+                relax3(l);
+                hipLaunchKernelGGL(k_nut_restrict, pts(level_points(l + 1)), T, 0, st, da, l + 1);
+                hipLaunchKernelGGL(k_nut_rterm, pts(3 * level_points(l + 1)), T, 0, st, da, l + 1);
+            }
+            hipLaunchKernelGGL(k_nut_vcycle_tail, dim3(1), dim3(ENV_THREADS), coarse_lds, st, a, lc,
+                               coarse_lds ? 1 : 0);
+            for (int l = lc - 1; l >= 0; l--) {
+                hipLaunchKernelGGL(k_nut_interp, pts(level_points(l)), T, 0, st, da, l);
+                relax3(l);
+            }
+        }
+        hipLaunchKernelGGL(k_nut_update_sol, pts(all), T, 0, st, da);
+        for (int l = 0; l < AL - 1; l++)
+            hipLaunchKernelGGL(k_nut_rewrite, pts(std::max(level_nx(e, l) + 1, 2 * a.BZA)), T, 0, st, da, l);
+        hipLaunchKernelGGL(k_nut_update_res, pts(all), T, 0, st, da);
+        hipLaunchKernelGGL(k_nut_reset_zero, pts(all), T, 0, st, da);
+        for (int l = 1; l <= AL; l++) hipLaunchKernelGGL(k_nut_nest, pts(level_points(l)), T, 0, st, da, l);
+        hipLaunchKernelGGL(k_nut_copy_from_mg, pts((long)AL * 2 * BX), T, 0, st, da);
+        if (wg_sweeps) {
+            hipLaunchKernelGGL(k_nut_sweeps_wg, dim3(1), dim3(ENV_THREADS), wg_lds, st, a, last ? 1 : 0);
+        } else {
+            const long items = (long)(e->hbounds[1] - e->hbounds[0] + 1) * (e->hbounds[4] + 1) + (long)AL * BX;
+            for (int j = 0; j < 200; j++)
+                hipLaunchKernelGGL(k_nut_sweep, pts(items), T, 0, st, da, (last && j >= 198) ? 1 : 0,
+                                   (j & 1) ? -1 : 1, j & 1);
+        }
+    }
+};
+
+int nutrients_phased(bcg_env* e, const EnvArgs& a) {
+    hipStream_t st = e->mech->stream;
+    // first narrow level: it and every coarser one fit one workgroup a colour
+    int lc = MG_LEVELS - 1;
+    while (lc > 0 && 3L * (level_nx(e, lc - 1) / 2 + 1) <= ENV_THREADS) lc--;
+    // sweeps in one workgroup when one round of it covers a sweep's boxes and
+    // rows (or when asked); device-wide launches otherwise
+    const long items = (long)(e->hbounds[1] - e->hbounds[0] + 1) * (e->hbounds[4] + 1) + (long)e->ML * e->BX;
+    bool wg = items <= ENV_THREADS;
+    if (e->exec & BCG_ENV_EXEC_WG_SWEEPS) wg = true;
+    if (e->exec & BCG_ENV_EXEC_GRID_SWEEPS) wg = false;
+    EnvArgs aa = a;
+    size_t lds = 0;
+    if (wg) {
+        const size_t need = (size_t)2 * 4 * aa.lds_nf * sizeof(double);
+        aa.lds_colony = need <= std::min(e->lds_budget, e->lds_device_wg);
+        lds = aa.lds_colony ? need : 0;
+    } else {
+        aa.lds_colony = 0;
+    }
+    // the coarsest level in LDS for the tail's coarse solve when the one-wave
+    // path (at most 64 points and 64 column walks) cannot take it
+    size_t coarse_lds = 0;
+    {
+        const int l = MG_LEVELS - 1;
+        const long npts = (long)(e->mgx[l] + 1) * (e->mgz[l] + 1);
+        const int nx = level_nx(e, l);
+        const bool wave = npts <= COARSE_WAVE_POINTS && 3 * (nx / 2 + 1) <= 64;
+        const size_t need = (size_t)3 * 3 * npts * sizeof(double);
+        if (!wave && need <= (size_t)COARSE_LDS_BYTES) coarse_lds = need;
+    }
+    *e->hargs = aa;
+    EHIP(e, hipMemcpyAsync(e->dargs, e->hargs, sizeof(EnvArgs), hipMemcpyHostToDevice, st));
+    const Phased P{e, aa, e->dargs, st, lc, wg, lds, coarse_lds};
+    hipLaunchKernelGGL(k_nut_begin, P.pts(std::max((long)e->BX * e->BZ, (long)e->ML * e->BX)), dim3(PH_THREADS), 0, st,
+                       P.da);
+    const int n = aa.n_outer;
+    if (n >= 2 && !(e->exec & BCG_ENV_EXEC_NO_GRAPH)) {
+        // outer iterations 0 .. n-2 are the same launches: record one, replay
+        hipGraph_t g = nullptr;
+        hipGraphExec_t x = nullptr;
+        EHIP(e, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        P.outer(false);
+        const hipError_t ec = hipStreamEndCapture(st, &g);
+        if (ec != hipSuccess) return efail(e, BCG_ERR_HIP, "graph capture: %s", hipGetErrorString(ec));
+        const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (ei != hipSuccess) return efail(e, BCG_ERR_HIP, "graph instantiate: %s", hipGetErrorString(ei));
+        hipError_t el = hipSuccess;
+        for (int o = 0; o < n - 1 && el == hipSuccess; o++) el = hipGraphLaunch(x, st);
+        // the executable graph may go once its launches are queued
+        const hipError_t es = hipStreamSynchronize(st);
+        hipGraphExecDestroy(x);
+        if (el != hipSuccess) return efail(e, BCG_ERR_HIP, "graph launch: %s", hipGetErrorString(el));
+        EHIP(e, es);
+    } else {
+        for (int o = 0; o < n - 1; o++) P.outer(false);
+    }
+    if (n >= 1) P.outer(true);
+    hipLaunchKernelGGL(k_nut_end, P.pts(std::max((long)e->BX * std::max(e->BZ, e->BZA), (long)e->ML * e->BX)),
+                       dim3(PH_THREADS), 0, st, P.da);
+    EHIP(e, hipGetLastError());
+    // the staging buffer is reused by the next update: its copy must have run
+    EHIP(e, hipStreamSynchronize(st));
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+int bcg_env_set_execution(bcg_env* e, int32_t flags) {
+    if (!e || (flags & ~(BCG_ENV_EXEC_ONE_WORKGROUP | BCG_ENV_EXEC_NO_GRAPH | BCG_ENV_EXEC_WG_SWEEPS |
+                         BCG_ENV_EXEC_GRID_SWEEPS)) ||
+        ((flags & BCG_ENV_EXEC_WG_SWEEPS) && (flags & BCG_ENV_EXEC_GRID_SWEEPS)))
+        return BCG_ERR_ARG;
+    e->exec = flags;
+    return BCG_OK;
+}
 
 int bcg_env_default_params(bcg_env_params* p) {
     if (!p) return BCG_ERR_ARG;
@@ -245,6 +400,7 @@ int bcg_env_destroy(bcg_env* e) {
         ptrs.push_back(e->aga[l]);
         ptrs.push_back(e->res[l]);
         ptrs.push_back(e->err[l]);
+        ptrs.push_back(e->rt[l]);
     }
     for (int b = 0; b < 2; b++) {
         ptrs.push_back(e->scol[b]);
@@ -254,8 +410,10 @@ int bcg_env_destroy(bcg_env* e) {
     ptrs.push_back(e->insT);
     ptrs.push_back(e->dshT);
     ptrs.push_back(e->phantom);
+    ptrs.push_back(e->dargs);
     for (void* q : ptrs)
         if (q) hipFree(q);
+    if (e->hargs) hipHostFree(e->hargs);
     delete e;
     return BCG_OK;
 }
@@ -309,6 +467,7 @@ int bcg_env_create(const bcg_env_params* p, bcg_ctx* mech, bcg_env** out) {
         A(&e->aga[l], 3 * n);
         A(&e->res[l], 3 * n);
         A(&e->err[l], 3 * n);
+        A(&e->rt[l], 3 * n);
     }
     for (int b = 0; b < 2; b++) {
         A(&e->scol[b], 4 * nb);
@@ -317,7 +476,7 @@ int bcg_env_create(const bcg_env_params* p, bcg_ctx* mech, bcg_env** out) {
     }
     A(&e->insT, nb);
     A(&e->dshT, nb);
-    A(&e->phantom, (size_t)3 * ENV_THREADS);
+    A(&e->phantom, (size_t)3 * PHANTOM_THREADS);
     for (double** q : {&e->rough, &e->rough1, &e->rough2, &e->inside, &e->rshift, &e->rshift1, &e->rshift2, &e->dens,
                        &e->dens1, &e->dens2, &e->dshift, &e->d1shift, &e->d2shift})
         A(q, nb);
@@ -329,6 +488,9 @@ int bcg_env_create(const bcg_env_params* p, bcg_ctx* mech, bcg_env** out) {
     A(&e->filter1d, (size_t)p->FilterLen);
     A(&e->bounds, 5);
     A(&e->errflag, 1);
+    A(&e->dargs, 1);
+    if (!rc && hipHostMalloc((void**)&e->hargs, sizeof(EnvArgs), hipHostMallocDefault) != hipSuccess)
+        rc = efail(e, BCG_ERR_HIP, "hipHostMalloc failed");
     if (rc) {
         mech->err = "bcg_env_create: " + e->err_text;  // readable through bcg_last_error
         bcg_env_destroy(e);
@@ -351,6 +513,17 @@ int bcg_env_create(const bcg_env_params* p, bcg_ctx* mech, bcg_env** out) {
             e->lds_budget = 64 * 1024 - fa.sharedSizeBytes;
         }
         e->lds_device = e->lds_budget;
+        // the one-workgroup sweeps kernel of the phased update
+        if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&k_nut_sweeps_wg)) == hipSuccess &&
+            (size_t)v > fa.sharedSizeBytes) {
+            e->lds_device_wg = std::min((size_t)ENV_LDS_BYTES, (size_t)v - fa.sharedSizeBytes);
+            if (e->lds_device_wg > 64 * 1024 &&
+                hipFuncSetAttribute(reinterpret_cast<const void*>(&k_nut_sweeps_wg),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->lds_device_wg) != hipSuccess) {
+                (void)hipGetLastError();
+                e->lds_device_wg = 64 * 1024 - fa.sharedSizeBytes;
+            }
+        }
     }
     auto up = [&](void* dst, const void* src, size_t bytes) {
         if (!rc && hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st) != hipSuccess)
@@ -380,6 +553,7 @@ int bcg_env_create(const bcg_env_params* p, bcg_ctx* mech, bcg_env** out) {
             up(e->aga[l], m.data(), m.size() * sizeof(double));
             up(e->res[l], m.data(), m.size() * sizeof(double));
             up(e->err[l], m.data(), m.size() * sizeof(double));
+            up(e->rt[l], m.data(), m.size() * sizeof(double));
             // the uploads read m before it goes out of scope
             if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = efail(e, BCG_ERR_HIP, "init sync failed");
         }
@@ -593,6 +767,7 @@ int bcg_env_nutrients(bcg_env* e, int32_t* count) {
         a.aga[l] = e->aga[l];
         a.res[l] = e->res[l];
         a.err[l] = e->err[l];
+        a.rt[l] = e->rt[l];
         a.mgx[l] = e->mgx[l];
         a.mgz[l] = e->mgz[l];
     }
@@ -619,7 +794,12 @@ int bcg_env_nutrients(bcg_env* e, int32_t* count) {
     a.lds_w = lw;
     a.lds_nf = lw * lh;
     EHIP(e, hipMemsetAsync(e->errflag, 0, sizeof(int), st));
-    hipLaunchKernelGGL(k_env_nutrients, dim3(1), dim3(ENV_THREADS), a.lds_colony ? lds : 0, st, a);
+    if (e->exec & BCG_ENV_EXEC_ONE_WORKGROUP) {
+        hipLaunchKernelGGL(k_env_nutrients, dim3(1), dim3(ENV_THREADS), a.lds_colony ? lds : 0, st, a);
+    } else {
+        int rc = nutrients_phased(e, a);
+        if (rc) return rc;
+    }
     // the mechanics grows with the caller's Environment (ABM/grow.cpp:26-36)
     const int nb = e->BX * e->BZ;
     hipLaunchKernelGGL(k_env_growth_to_mech, dim3(nblk(nb, 256)), dim3(256), 0, st, e->colony[0], nb, m->gr1, m->gr2);
@@ -676,22 +856,35 @@ int bcg_clock_advance(bcg_clock* c) {
     return BCG_OK;
 }
 
-int bcg_sim_step(bcg_ctx* ctx, bcg_env* env, bcg_clock* clock, int32_t* n_div, int32_t* lineage, int32_t lineage_cap) {
+int bcg_sim_prepare(bcg_ctx* ctx, bcg_env* env, const bcg_clock* clock) {
     if (!ctx || !clock) return BCG_ERR_ARG;
     if (env && env->mech != ctx) return BCG_ERR_ARG;
-    int32_t flags = 0;
     int rc;
     if (clock->UpdateFlag) {
         if (env) {
             if ((rc = bcg_env_fields(env, BCG_ENV_UPDATE))) return rc;
             if ((rc = bcg_env_nutrients(env, nullptr))) return rc;
         }
-        flags |= BCG_STEP_REBUILD;  // getNeighbours (ABM/Simulation.cpp:290)
+        // getNeighbours (ABM/Simulation.cpp:290)
+        if ((rc = bcg_rebuild_neighbours(ctx, nullptr, -1))) return rc;
     }
     if (clock->HeightDensityUpdateFlag && env)
         if ((rc = bcg_env_fields(env, BCG_ENV_HEIGHT))) return rc;
-    if ((rc = bcg_step(ctx, clock->dt, clock->t, flags, n_div, lineage, lineage_cap))) return rc;
+    return BCG_OK;
+}
+
+int bcg_sim_advance(bcg_ctx* ctx, bcg_clock* clock, bcg_cell* moved, int32_t moved_cap, int32_t* n_div,
+                    int32_t* lineage, int32_t lineage_cap) {
+    if (!ctx || !clock) return BCG_ERR_ARG;
+    int rc = bcg_step_out(ctx, clock->dt, clock->t, 0, moved, moved_cap, n_div, lineage, lineage_cap);
+    if (rc) return rc;
     return bcg_clock_advance(clock);
+}
+
+int bcg_sim_step(bcg_ctx* ctx, bcg_env* env, bcg_clock* clock, int32_t* n_div, int32_t* lineage, int32_t lineage_cap) {
+    int rc = bcg_sim_prepare(ctx, env, clock);
+    if (rc) return rc;
+    return bcg_sim_advance(ctx, clock, nullptr, 0, n_div, lineage, lineage_cap);
 }
 
 }  // extern "C"

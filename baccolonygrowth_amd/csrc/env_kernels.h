@@ -47,6 +47,9 @@ struct EnvArgs {
     double* aga[MG_LEVELS];
     double* res[MG_LEVELS];
     double* err[MG_LEVELS];
+    // the relaxation's residual term r h^2 / dt / D of every point (same
+    // layout as res), formed once each time the residual is
+    double* rt[MG_LEVELS];
     int mgx[MG_LEVELS];               // x extent - 1 of each multigrid array
     int mgz[MG_LEVELS];               // z extent - 1 of each multigrid array
     // working copies during an update, x fastest: colony C, W, O2, acetate
@@ -100,6 +103,32 @@ __global__ void k_env_normal(FieldArgs a);
 __global__ void k_env_init_bounds(FieldArgs a);
 
 __global__ void __launch_bounds__(ENV_THREADS) k_env_nutrients(EnvArgs a);
+
+// The same update in phases (one launch per barrier of k_env_nutrients).
+constexpr int PH_THREADS = 256;      // block of the grid-wide phase kernels
+constexpr int PH_MAX_BLOCKS = 1024;  // their grids are capped here (grid-stride loops)
+constexpr int PH_RELAX_THREADS = 64; // block of the relaxation passes (one column walk a lane)
+// per-thread slots of EnvArgs::phantom: every thread of any phase launch
+constexpr int PHANTOM_THREADS = PH_THREADS * PH_MAX_BLOCKS;
+__global__ void __launch_bounds__(PH_THREADS) k_nut_begin(const EnvArgs* a);
+__global__ void __launch_bounds__(PH_THREADS) k_nut_copy_to_mg(const EnvArgs* a, int sw);
+__global__ void __launch_bounds__(PH_THREADS) k_nut_reset_zero(const EnvArgs* a);
+__global__ void __launch_bounds__(PH_THREADS) k_nut_update_sol(const EnvArgs* a);
+__global__ void __launch_bounds__(PH_THREADS) k_nut_update_res(const EnvArgs* a);
+__global__ void __launch_bounds__(PH_RELAX_THREADS) k_nut_relax(const EnvArgs* a, int l, int par);
+__global__ void __launch_bounds__(PH_THREADS) k_nut_restrict(const EnvArgs* a, int l);
+__global__ void __launch_bounds__(PH_THREADS) k_nut_interp(const EnvArgs* a, int l);
+__global__ void __launch_bounds__(PH_THREADS) k_nut_rterm(const EnvArgs* a, int l);
+__global__ void __launch_bounds__(ENV_THREADS) k_nut_vcycle_tail(EnvArgs a, int lc, int coarse_lds);
+// the coarsest level in LDS for the tail's coarse solve (E, R and RT), up to this size
+constexpr int COARSE_LDS_BYTES = 60 * 1024;
+constexpr int COARSE_WAVE_POINTS = 64;  // ... unless the one-wave path takes it
+__global__ void __launch_bounds__(PH_THREADS) k_nut_rewrite(const EnvArgs* a, int l);
+__global__ void __launch_bounds__(PH_THREADS) k_nut_nest(const EnvArgs* a, int l);
+__global__ void __launch_bounds__(PH_THREADS) k_nut_copy_from_mg(const EnvArgs* a);
+__global__ void __launch_bounds__(PH_THREADS) k_nut_sweep(const EnvArgs* a, int full, int dir, int sw);
+__global__ void __launch_bounds__(ENV_THREADS) k_nut_sweeps_wg(EnvArgs a, int last);
+__global__ void __launch_bounds__(PH_THREADS) k_nut_end(const EnvArgs* a);
 __global__ void k_env_growth_to_mech(const bcg_env_cell* colony, int n, double* gr1, double* gr2);
 
 }  // namespace env

@@ -293,6 +293,14 @@ __global__ void k_env_growth_to_mech(const bcg_env_cell* colony, int n, double* 
 namespace {
 
 constexpr int COARSE_MAX = 64;  // points of the coarsest level relaxed in LDS
+// Loads in flight per column walk of the relaxation: the one-workgroup
+// passes (many walks a lane, L2-resident levels) and the device-wide passes
+// on the wide levels (one walk a lane, deep columns that stream from HBM).
+constexpr int RING_WG = 6;
+#ifndef BCG_RING_WIDE
+#define BCG_RING_WIDE 16
+#endif
+constexpr int RING_WIDE = BCG_RING_WIDE;
 
 // Orders one wave's LDS writes before its later LDS reads (the lanes of a
 // wave run in lockstep; the fences keep the compiler from moving accesses).
@@ -384,12 +392,13 @@ struct Col {
 // relaxationAgarElementwise (ABM/Multigrid.cpp:161-249) down one column
 // (k ascending) for one species: the reference's branches and operand
 // order, with e(k-1) carried in a register and the loads that do not depend
-// on it (R, the two neighbour columns of the other colour, the old e(k+1))
-// prefetched RING steps ahead. Nothing else writes this column during a
+// on it (R, its term RT = r h^2 / dt / D, the two neighbour columns of the
+// other colour, the old e(k+1)) prefetched RING steps ahead; nn = h^2 / dt /
+// D + 2. Nothing else writes this column during a
 // colour pass, so the values match the sequential k loop exactly.
-template <class T>
-__device__ __forceinline__ void relax_column(const Col<T> E, const Col<T> R, int i, int nx, int nz, bool isO, double D, double h2,
-                             double dt, bool outer, bool dir0) {
+template <int RING, class T>
+__device__ __forceinline__ void relax_column(const Col<T> E, const Col<T> R, const Col<T> RT, int i, int nx, int nz,
+                                             bool isO, double nn, bool outer, bool dir0) {
     const bool edge = (i == 0 || i == nx);
     if (edge && !outer && !dir0) return;  // nothing in this column changes
     const int X1 = E.X1;
@@ -399,15 +408,14 @@ __device__ __forceinline__ void relax_column(const Col<T> E, const Col<T> R, int
     // past the last column nothing (read as 0)
     const int inext = min(i + 1, nx);
     const bool next_ok = i + 1 <= nx;
-    const double nn = h2 / dt / D + 2;
-    constexpr int RING = 6;
-    double qr[RING], ql[RING], qh[RING], qn[RING];
+    double qr[RING], qt[RING], ql[RING], qh[RING], qn[RING];
     // loads without branches (so that they stay in flight across steps):
     // past the column end and on the edge an in-range element is loaded and
     // not used
     auto fetch = [&](int k, int j) {
         const int kk = min(k, nz);
         qr[j] = R.p[kk * X1 + i];
+        qt[j] = RT.p[kk * X1 + i];
         ql[j] = E.p[kk * X1 + il];
         qh[j] = E.p[kk * X1 + ih];
         qn[j] = E.p[kk < nz ? (kk + 1) * X1 + i : inext];
@@ -423,9 +431,11 @@ __device__ __forceinline__ void relax_column(const Col<T> E, const Col<T> R, int
 #pragma unroll
         for (int j = 0; j < RING; j++) {
             const int k = k0 + j;
-            const double r = qr[j], L = ql[j], H = qh[j], ekp1 = (k < nz || next_ok) ? qn[j] : 0.0;
+            const double r = qr[j], t = qt[j], L = ql[j], H = qh[j], ekp1 = (k < nz || next_ok) ? qn[j] : 0.0;
             fetch(k + RING, j);
-            const double t = r * h2 / dt / D;
+            // keep the loads here: the scheduler would otherwise sink them to
+            // the step before their use, and every step would wait for memory
+            __builtin_amdgcn_sched_barrier(0);
             // the numerator of the branch taken, in the reference's order of
             // operations - corners t + (p + q); edges t + e + p/2 + q/2 or
             // t + p/2 + q/2 + e; interior t + p/2 + q/2 + r/2 + s/2 - chosen
@@ -1099,27 +1109,58 @@ __device__ __forceinline__ void wall_point(const EnvArgs& a, const CS env, doubl
     out(FW, x) = smax(0.0, Wnew);
 }
 
-// relaxationAgarAtLevel (ABM/Multigrid.cpp:135-159): the columns of parity
-// 0, then of parity 1, each column k ascending; a thread per (species,
-// column).
-__device__ __forceinline__ void relax_level(const EnvArgs& a, const MG m, int l, int tid) {
+// Diffusivity of multigrid species sp (C, O2, acetate) and the
+// relaxation's constants of level l.
+__device__ __forceinline__ double species_D(const EnvArgs& a, int sp) {
+    return sp == 0 ? a.DiffAgarCarbon : (sp == 1 ? a.DiffAgarO2 : a.DiffAgarAce);
+}
+__device__ __forceinline__ double level_h2(const EnvArgs& a, int l) {
+    return a.BoxLength * a.BoxLength * (double)(1 << (2 * l));
+}
+
+// The residual term of the relaxation, r h^2 / dt / D, of every point of
+// level l (the reference forms it in each relaxation; the same operations
+// give the same value), entries t0, t0 + stride, ...
+__device__ __forceinline__ void rterm_level(const EnvArgs& a, const MG m, int l, int t0, int stride) {
+    const int npts = (a.mgx[l] + 1) * (a.mgz[l] + 1);
+    const double h2 = level_h2(a, l), dt = m.dt;
+    const gdouble* R = G(a.res[l]);
+    gdouble* T = G(a.rt[l]);
+    for (int f = t0; f < 3 * npts; f += stride) {
+        const int sp = f / npts;
+        T[f] = R[f] * h2 / dt / species_D(a, sp);
+    }
+}
+
+// One colour pass of relaxationAgarAtLevel (ABM/Multigrid.cpp:135-159): the
+// columns of parity `par`, each k ascending; (species, column) pairs t0,
+// t0 + stride, ... of this thread. Columns of one parity read only columns
+// of the other: the pass may be spread over any number of threads.
+template <int RING>
+__device__ __forceinline__ void relax_pass(const EnvArgs& a, const MG m, int l, int par, int t0, int stride) {
     const Geo g = m.geo(l, 1);
     const int X1 = a.mgx[l] + 1, npts = X1 * (a.mgz[l] + 1);
     const bool outer = l >= m.AL - 1;
-    const double h2 = a.BoxLength * a.BoxLength * (double)(1 << (2 * l));
+    const double h2 = level_h2(a, l);
     const int d = 1 << l;
     const gdouble* insT = G(a.insT);
     gdouble* E = G(a.err[l]);
     gdouble* R = G(a.res[l]);
+    gdouble* T = G(a.rt[l]);
+    const int ncol = par <= g.nx ? (g.nx - par) / 2 + 1 : 0;
+    for (int w = t0; w < 3 * ncol; w += stride) {
+        const int sp = w / ncol, i = par + 2 * (w - sp * ncol);
+        const int i0 = min(max((i - g.nx / 2) * d + a.BX / 2, 0), a.BX - 1);
+        const double nn = h2 / m.dt / species_D(a, sp) + 2;
+        relax_column<RING>(Col<gdouble>{E + sp * npts, X1}, Col<gdouble>{R + sp * npts, X1},
+                           Col<gdouble>{T + sp * npts, X1}, i, g.nx, g.nz, sp == 1, nn, outer, insT[i0] > 0);
+    }
+}
+
+// relaxationAgarAtLevel: parity 0, then parity 1, in one workgroup.
+__device__ __forceinline__ void relax_level(const EnvArgs& a, const MG m, int l, int tid) {
     for (int par = 0; par < 2; par++) {
-        const int ncol = par <= g.nx ? (g.nx - par) / 2 + 1 : 0;
-        for (int w = tid; w < 3 * ncol; w += ENV_THREADS) {
-            const int sp = w / ncol, i = par + 2 * (w - sp * ncol);
-            const int i0 = min(max((i - g.nx / 2) * d + a.BX / 2, 0), a.BX - 1);
-            const double D = sp == 0 ? a.DiffAgarCarbon : (sp == 1 ? a.DiffAgarO2 : a.DiffAgarAce);
-            relax_column(Col<gdouble>{E + sp * npts, X1}, Col<gdouble>{R + sp * npts, X1}, i, g.nx, g.nz, sp == 1, D,
-                         h2, m.dt, outer, insT[i0] > 0);
-        }
+        relax_pass<RING_WG>(a, m, l, par, tid, ENV_THREADS);
         __syncthreads();
     }
 }
@@ -1127,7 +1168,8 @@ __device__ __forceinline__ void relax_level(const EnvArgs& a, const MG m, int l,
 // The 100 relaxations of the coarsest level (Multigrid::Vcycle): in wave 0,
 // out of LDS, when the level fits (no workgroup barrier between the colour
 // passes).
-__device__ __forceinline__ void coarse_solve(const EnvArgs& a, const MG m, int tid, double* cs_e, double* cs_r) {
+__device__ __forceinline__ void coarse_solve(const EnvArgs& a, const MG m, int tid, double* cs_e, double* cs_r,
+                                             double* cs_t) {
     const int l = MG_LEVELS - 1;
     const Geo g = m.geo(l, 1);
     const int X1 = g.nx + 1, npts = X1 * (g.nz + 1);
@@ -1138,14 +1180,16 @@ __device__ __forceinline__ void coarse_solve(const EnvArgs& a, const MG m, int t
     if (tid < 64) {
         gdouble* E = G(a.err[l]);
         const gdouble* R = G(a.res[l]);
+        const gdouble* T = G(a.rt[l]);
         for (int f = tid; f < npts; f += 64)
             for (int sp = 0; sp < 3; sp++) {
                 cs_e[sp * COARSE_MAX + f] = E[sp * npts + f];
                 cs_r[sp * COARSE_MAX + f] = R[sp * npts + f];
+                cs_t[sp * COARSE_MAX + f] = T[sp * npts + f];
             }
         wave_sync();
         const bool outer = l >= m.AL - 1;
-        const double h2 = a.BoxLength * a.BoxLength * (double)(1 << (2 * l));
+        const double h2 = level_h2(a, l);
         const int d = 1 << l;
         const gdouble* insT = G(a.insT);
         for (int j = 0; j < 100; j++) {
@@ -1154,9 +1198,10 @@ __device__ __forceinline__ void coarse_solve(const EnvArgs& a, const MG m, int t
                 if (tid < 3 * ncol) {
                     const int sp = tid / ncol, i = par + 2 * (tid - sp * ncol);
                     const int i0 = min(max((i - g.nx / 2) * d + a.BX / 2, 0), a.BX - 1);
-                    const double D = sp == 0 ? a.DiffAgarCarbon : (sp == 1 ? a.DiffAgarO2 : a.DiffAgarAce);
-                    relax_column(Col<double>{cs_e + sp * COARSE_MAX, X1}, Col<double>{cs_r + sp * COARSE_MAX, X1}, i,
-                                 g.nx, g.nz, sp == 1, D, h2, m.dt, outer, insT[i0] > 0);
+                    const double nn = h2 / m.dt / species_D(a, sp) + 2;
+                    relax_column<RING_WG>(Col<double>{cs_e + sp * COARSE_MAX, X1}, Col<double>{cs_r + sp * COARSE_MAX, X1},
+                                          Col<double>{cs_t + sp * COARSE_MAX, X1}, i, g.nx, g.nz, sp == 1, nn, outer,
+                                          insT[i0] > 0);
                 }
                 wave_sync();
             }
@@ -1164,6 +1209,48 @@ __device__ __forceinline__ void coarse_solve(const EnvArgs& a, const MG m, int t
         for (int f = tid; f < npts; f += 64)
             for (int sp = 0; sp < 3; sp++) E[sp * npts + f] = cs_e[sp * COARSE_MAX + f];
     }
+    __syncthreads();
+}
+
+// The 100 relaxations of the coarsest level by the whole workgroup out of
+// LDS (each species [z][x]), for
+// levels too large for the one-wave path of coarse_solve (lds: 3 x 3 x npts
+// doubles, E, R, then RT).
+__device__ __forceinline__ void coarse_solve_lds(const EnvArgs& a, const MG m, int tid, double* lds) {
+    const int l = MG_LEVELS - 1;
+    const Geo g = m.geo(l, 1);
+    const int X1 = a.mgx[l] + 1, npts = X1 * (a.mgz[l] + 1);
+    double* cs_e = lds;
+    double* cs_r = lds + 3 * npts;
+    double* cs_t = lds + 6 * npts;
+    gdouble* E = G(a.err[l]);
+    const gdouble* R = G(a.res[l]);
+    const gdouble* T = G(a.rt[l]);
+    for (int f = tid; f < 3 * npts; f += ENV_THREADS) {
+        cs_e[f] = E[f];
+        cs_r[f] = R[f];
+        cs_t[f] = T[f];
+    }
+    __syncthreads();
+    const bool outer = l >= m.AL - 1;
+    const double h2 = level_h2(a, l);
+    const int d = 1 << l;
+    const gdouble* insT = G(a.insT);
+    for (int j = 0; j < 100; j++) {
+        for (int par = 0; par < 2; par++) {
+            const int ncol = par <= g.nx ? (g.nx - par) / 2 + 1 : 0;
+            for (int w = tid; w < 3 * ncol; w += ENV_THREADS) {
+                const int sp = w / ncol, i = par + 2 * (w - sp * ncol);
+                const int i0 = min(max((i - g.nx / 2) * d + a.BX / 2, 0), a.BX - 1);
+                const double nn = h2 / m.dt / species_D(a, sp) + 2;
+                relax_column<RING_WG>(Col<double>{cs_e + sp * npts, X1}, Col<double>{cs_r + sp * npts, X1},
+                                      Col<double>{cs_t + sp * npts, X1}, i, g.nx, g.nz, sp == 1, nn, outer,
+                                      insT[i0] > 0);
+            }
+            __syncthreads();
+        }
+    }
+    for (int f = tid; f < 3 * npts; f += ENV_THREADS) E[f] = cs_e[f];
     __syncthreads();
 }
 
@@ -1206,30 +1293,42 @@ __device__ __forceinline__ void reset_zero(const MG m, int tid) {
 // box written in the same sweep is (level 0, x), reading box (x, 0): the
 // thread of box (x, 0) updates both, in that order, so one barrier per sweep
 // suffices. c0 / c1: the colony buffers (Environment, oldEnvironment).
+// One sweep, the boxes and rows t0, t0 + stride, ... of this thread: the
+// colony window first (index w < ncol * nrow), then the interface rows of
+// every nested level outside it. sw: which buffer is Current; dir: +1 when
+// the sweep count is even.
 template <class T>
-__device__ __forceinline__ void sweeps(const EnvArgs& a, const ChemConst& cc, int outer, int tid, T* c0, T* c1, int x0,
-                                       int W, int NF, int& sw, int& count) {
+__device__ __forceinline__ void sweep_body(const EnvArgs& a, const ChemConst& cc, bool full, int dir, int sw, T* c0,
+                                           T* c1, int x0, int W, int NF, int t0, int stride) {
     const int BX = a.BX, AL = a.ML;
     const int minX = a.bounds[0], maxX = a.bounds[1], maxH = a.bounds[4];
     const int ncol = maxX - minX + 1, nrow = maxH + 1;
     const gdouble* dshT = G(a.dshT);
-    for (int j = 0; j < 200; j++) {
-        const int dir = (count % 2 == 0) ? 1 : -1;
-        const bool full = outer == a.n_outer - 1 && j >= 198;
-        const ColS<T> cur{sw ? c1 : c0, x0, W, NF}, prev{sw ? c0 : c1, x0, W, NF};
-        const RowS pw0{G(a.swall[sw ^ 1]), BX};
-        for (int w = tid; w < ncol * nrow; w += ENV_THREADS) {
+    const ColS<T> cur{sw ? c1 : c0, x0, W, NF}, prev{sw ? c0 : c1, x0, W, NF};
+    const RowS pw0{G(a.swall[sw ^ 1]), BX};
+    const int nw = ncol * nrow;
+    for (int w = t0; w < nw + AL * BX; w += stride) {
+        if (w < nw) {
             const int z = w / ncol, x = minX + (w - z * ncol);
             if (z == 0 && dir == 1) wall_point(a, cur, a.srow[sw ^ 1], a.swall[sw], a.swall[sw ^ 1], 0, x);
             if (dshT[z * BX + x] > DBL_EPSILON)
                 colony_point(a, cc, cur, prev, pw0, a.colony[sw], a.colony[sw ^ 1], x, z, full);
             if (z == 0 && dir == -1) wall_point(a, cur, a.srow[sw ^ 1], a.swall[sw], a.swall[sw ^ 1], 0, x);
-        }
-        for (int w = tid; w < AL * BX; w += ENV_THREADS) {
-            const int l = w / BX, x = w - l * BX;
+        } else {
+            const int q = w - nw, l = q / BX, x = q - l * BX;
             if (l == 0 && x >= minX && x <= maxX) continue;
             wall_point(a, cur, a.srow[sw ^ 1], a.swall[sw], a.swall[sw ^ 1], l, x);
         }
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void sweeps(const EnvArgs& a, const ChemConst& cc, int outer, int tid, T* c0, T* c1, int x0,
+                                       int W, int NF, int& sw, int& count) {
+    for (int j = 0; j < 200; j++) {
+        const int dir = (count % 2 == 0) ? 1 : -1;
+        const bool full = outer == a.n_outer - 1 && j >= 198;
+        sweep_body(a, cc, full, dir, sw, c0, c1, x0, W, NF, tid, ENV_THREADS);
         __syncthreads();
         sw ^= 1;
         count++;
@@ -1244,7 +1343,7 @@ __global__ void __launch_bounds__(ENV_THREADS) k_env_nutrients(EnvArgs args) {
     // them in LDS rather than in per-lane scratch (data pointers are cast to
     // gdouble* where used, so that accesses are global, not flat)
     __shared__ EnvArgs a;
-    __shared__ double cs_e[3 * COARSE_MAX], cs_r[3 * COARSE_MAX];
+    __shared__ double cs_e[3 * COARSE_MAX], cs_r[3 * COARSE_MAX], cs_t[3 * COARSE_MAX];
     extern __shared__ double lds[];  // the colony window of both buffers, when it fits
     const int tid = threadIdx.x;
     if (tid == 0) a = args;
@@ -1346,14 +1445,18 @@ __global__ void __launch_bounds__(ENV_THREADS) k_env_nutrients(EnvArgs args) {
             over(g, tid, [=](int i, int k) { update_res_point(m, l, i, k, g); });
         }
         __syncthreads();
+        rterm_level(a, m, 0, tid, ENV_THREADS);
+        __syncthreads();
         for (int cyc = 0; cyc < 3; cyc++) {
             for (int l = 0; l < MG_LEVELS - 1; l++) {
                 for (int j = 0; j < 3; j++) relax_level(a, m, l, tid);
                 const Geo g = m.geo(l + 1, 1);
                 over(g, tid, [=](int i, int k) { restrict_point(m, l + 1, i, k, g); });
                 __syncthreads();
+                rterm_level(a, m, l + 1, tid, ENV_THREADS);
+                __syncthreads();
             }
-            coarse_solve(a, m, tid, cs_e, cs_r);
+            coarse_solve(a, m, tid, cs_e, cs_r, cs_t);
             for (int l = MG_LEVELS - 2; l >= 0; l--) {
                 const Geo g = m.geo(l, 2);
                 over(g, tid, [=](int i, int k) { interp_point(m, l, i, k, g); });
@@ -1468,6 +1571,336 @@ __global__ void __launch_bounds__(ENV_THREADS) k_env_nutrients(EnvArgs args) {
     if (a.n_outer > 0) {
         for (int l = 0; l < AL; l++)
             for (int w = tid; w < BX * a.BZA; w += ENV_THREADS) {
+                const int i = w / a.BZA, k = w - i * a.BZA + 1;
+                const AgaRef s = m.A(l, i, k);
+                for (int bb = 0; bb < 2; bb++) {
+                    bcg_aga& d = a.agar[bb][l][w];
+                    d.CarbonAgar = s.CarbonAgar;
+                    d.OxygenAgar = s.OxygenAgar;
+                    d.AcetateAgar = s.AcetateAgar;
+                }
+            }
+    }
+}
+
+// ------------------------------------------------ the update in phases
+// UpdateEnvArrayMG_OxygenAcetate as a sequence of launches, in the order of
+// the one-workgroup kernel above: every barrier of that kernel is a launch
+// boundary here, so each phase computes exactly what the same phase
+// computes there, spread over the device. The host (bcg_env_nutrients)
+// records one outer iteration as a graph and replays it.
+
+namespace {
+
+__device__ __forceinline__ int gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ __forceinline__ int gstride() { return gridDim.x * blockDim.x; }
+
+// Every point of a level across the grid, x fastest.
+template <class F>
+__device__ __forceinline__ void over_grid(const Geo g, F fn) {
+    const int n = (g.nx + 1) * (g.nz + 1);
+    for (int w = gtid(); w < n; w += gstride()) {
+        const int k = w / (g.nx + 1);
+        fn(w - k * (g.nx + 1), k);
+    }
+}
+
+__device__ __forceinline__ MG grid_mg(const EnvArgs& a) { return MG(a, 100.0, G(a.phantom) + 3 * gtid()); }
+
+// The phase kernels read their arguments from one device copy per update:
+// the pointer tables are indexed by level at run time, which from memory is
+// a uniform load (by value, the struct would be copied to scratch memory).
+
+}  // namespace
+
+// Working copies (as the start of k_env_nutrients): insideColonyDen and
+// DensityShiftP [z][x], colony fields C, W, O2, acetate of both buffers, the
+// interface rows and first agar rows of both buffers.
+__global__ void __launch_bounds__(PH_THREADS) k_nut_begin(const EnvArgs* __restrict__ args) {
+    const EnvArgs& a = *args;
+    const int BX = a.BX, BZ = a.BZ, NB = BX * BZ, RW = 4 * BX, AL = a.ML;
+    for (int w = gtid(); w < NB; w += gstride()) {
+        const int z = w / BX, x = w - z * BX, b = x * BZ + z;
+        G(a.insT)[w] = a.inside[b];
+        G(a.dshT)[w] = a.dshift[b];
+        for (int bb = 0; bb < 2; bb++) {
+            const bcg_env_cell& c = a.colony[bb][b];
+            gdouble* d = G(a.scol[bb]);
+            d[FC * NB + w] = c.Carbon;
+            d[FW * NB + w] = c.Waste;
+            d[FO * NB + w] = c.Oxygen;
+            d[FA * NB + w] = c.Acetate;
+        }
+    }
+    for (int w = gtid(); w < AL * BX; w += gstride()) {
+        const int l = w / BX, x = w - l * BX;
+        for (int bb = 0; bb < 2; bb++) {
+            const bcg_aga& q = a.wall[bb][l][x];
+            const bcg_aga& r = a.agar[bb][l][x * a.BZA];
+            gdouble* dw = G(a.swall[bb]) + l * RW;
+            gdouble* dr = G(a.srow[bb]) + l * RW;
+            dw[FC * BX + x] = q.CarbonAgar;
+            dw[FW * BX + x] = q.WasteAgar;
+            dw[FO * BX + x] = q.OxygenAgar;
+            dw[FA * BX + x] = q.AcetateAgar;
+            dr[FC * BX + x] = r.CarbonAgar;
+            dr[FW * BX + x] = r.WasteAgar;
+            dr[FO * BX + x] = r.OxygenAgar;
+            dr[FA * BX + x] = r.AcetateAgar;
+        }
+    }
+}
+
+// copyToMG: the colony footprint from the wall; the last column of the
+// outermost nested level from its first (the two touch different points).
+__global__ void __launch_bounds__(PH_THREADS) k_nut_copy_to_mg(const EnvArgs* __restrict__ args, int sw) {
+    const EnvArgs& a = *args;
+    const MG m = grid_mg(a);
+    const int BX = a.BX, AL = a.ML;
+    const RowS cw{G(a.swall[sw]), BX};
+    const gdouble* insT = G(a.insT);
+    for (int w = gtid(); w < BX - 1; w += gstride()) {
+        const int i = w + 1;
+        if (insT[i] > 0) {
+            const AgaRef u = m.A(0, i, 0);
+            u.CarbonAgar = cw(FC, i);
+            u.OxygenAgar = cw(FO, i);
+            u.AcetateAgar = cw(FA, i);
+        }
+    }
+    for (int k = gtid(); k <= a.BZA; k += gstride()) {
+        const AgaRef u = m.A(AL - 1, BX, k);
+        const AgaRef s = m.A(AL - 1, 0, k);
+        u.CarbonAgar = s.CarbonAgar;
+        u.OxygenAgar = s.OxygenAgar;
+        u.AcetateAgar = s.AcetateAgar;
+    }
+}
+
+// Multigrid::reset's zeroing: the error on every level, the residual on the
+// coarse ones.
+__global__ void __launch_bounds__(PH_THREADS) k_nut_reset_zero(const EnvArgs* __restrict__ args) {
+    const EnvArgs& a = *args;
+    const MG m = grid_mg(a);
+    for (int l = 0; l < MG_LEVELS; l++) {
+        over_grid(m.geo(l, 1), [=](int i, int k) {
+            const AgaRef e = m.E(l, i, k);
+            e.CarbonAgar = 0;
+            e.OxygenAgar = 0;
+            e.AcetateAgar = 0;
+            if (l > 0) {
+                const AgaRef r = m.R(l, i, k);
+                r.CarbonAgar = 0;
+                r.OxygenAgar = 0;
+                r.AcetateAgar = 0;
+            }
+        });
+    }
+}
+
+// updateSolAga of every nested level (each reads and writes its own level).
+__global__ void __launch_bounds__(PH_THREADS) k_nut_update_sol(const EnvArgs* __restrict__ args) {
+    const EnvArgs& a = *args;
+    const MG m = grid_mg(a);
+    for (int l = 0; l <= a.ML - 1; l++) {
+        const Geo g = m.geo(l, 1);
+        over_grid(g, [=](int i, int k) { update_sol_point(m, l, i, k, g); });
+    }
+}
+
+// updateResAga of every nested level (writes residuals, reads solutions).
+__global__ void __launch_bounds__(PH_THREADS) k_nut_update_res(const EnvArgs* __restrict__ args) {
+    const EnvArgs& a = *args;
+    const MG m = grid_mg(a);
+    for (int l = 0; l <= a.ML - 1; l++) {
+        const Geo g = m.geo(l, 1);
+        over_grid(g, [=](int i, int k) { update_res_point(m, l, i, k, g); });
+    }
+}
+
+// One colour pass of the relaxation of level l.
+__global__ void __launch_bounds__(PH_RELAX_THREADS) k_nut_relax(const EnvArgs* __restrict__ args, int l, int par) {
+    const EnvArgs& a = *args;
+    relax_pass<RING_WIDE>(a, grid_mg(a), l, par, gtid(), gstride());
+}
+
+// Level l's residual restricted from level l - 1.
+__global__ void __launch_bounds__(PH_THREADS) k_nut_restrict(const EnvArgs* __restrict__ args, int l) {
+    const EnvArgs& a = *args;
+    const MG m = grid_mg(a);
+    const Geo g = m.geo(l, 1);
+    over_grid(g, [=](int i, int k) { restrict_point(m, l, i, k, g); });
+}
+
+// Level l + 1's error interpolated onto level l.
+__global__ void __launch_bounds__(PH_THREADS) k_nut_interp(const EnvArgs* __restrict__ args, int l) {
+    const EnvArgs& a = *args;
+    const MG m = grid_mg(a);
+    const Geo g = m.geo(l, 2);
+    over_grid(g, [=](int i, int k) { interp_point(m, l, i, k, g); });
+}
+
+// The relaxation's residual term of level l (after its residual changed).
+__global__ void __launch_bounds__(PH_THREADS) k_nut_rterm(const EnvArgs* __restrict__ args, int l) {
+    const EnvArgs& a = *args;
+    rterm_level(a, grid_mg(a), l, gtid(), gstride());
+}
+
+// The V-cycle from level lc down and back up to it, in one workgroup (the
+// levels below lc are narrow: a barrier costs less than a launch there).
+__global__ void __launch_bounds__(ENV_THREADS) k_nut_vcycle_tail(EnvArgs args, int lc, int coarse_lds) {
+    __shared__ EnvArgs a;
+    __shared__ double cs_e[3 * COARSE_MAX], cs_r[3 * COARSE_MAX], cs_t[3 * COARSE_MAX];
+    extern __shared__ double lds[];  // the coarsest level, when coarse_lds
+    const int tid = threadIdx.x;
+    if (tid == 0) a = args;
+    __syncthreads();
+    const MG m(a, 100.0, G(a.phantom) + 3 * tid);
+    for (int l = lc; l < MG_LEVELS - 1; l++) {
+        for (int j = 0; j < 3; j++) relax_level(a, m, l, tid);
+        const Geo g = m.geo(l + 1, 1);
+        over(g, tid, [=](int i, int k) { restrict_point(m, l + 1, i, k, g); });
+        __syncthreads();
+        rterm_level(a, m, l + 1, tid, ENV_THREADS);
+        __syncthreads();
+    }
+    if (coarse_lds)
+        coarse_solve_lds(a, m, tid, lds);
+    else
+        coarse_solve(a, m, tid, cs_e, cs_r, cs_t);
+    for (int l = MG_LEVELS - 2; l >= lc; l--) {
+        const Geo g = m.geo(l, 2);
+        over(g, tid, [=](int i, int k) { interp_point(m, l, i, k, g); });
+        __syncthreads();
+        for (int j = 0; j < 3; j++) relax_level(a, m, l, tid);
+    }
+}
+
+// rewriteSolBdryAga of level l from level l + 1.
+__global__ void __launch_bounds__(PH_THREADS) k_nut_rewrite(const EnvArgs* __restrict__ args, int l) {
+    const EnvArgs& a = *args;
+    const MG m = grid_mg(a);
+    const Geo g = m.geo(l, 2);
+    const int nx = g.nx, nz = g.nz, nx2 = g.nxp;
+    for (int i = gtid(); i <= nx; i += gstride()) {
+        const int i2 = nx2 / 2 + i / 2 - nx / 4, i2p1 = nx2 / 2 + (i + 1) / 2 - nx / 4;
+        const AgaRef u = m.A(l, i, nz);
+        const AgaRef p = m.A(l + 1, i2, nz / 2), q = m.A(l + 1, i2p1, nz / 2);
+        u.CarbonAgar = (p.CarbonAgar + q.CarbonAgar) / 2;
+        u.OxygenAgar = (p.OxygenAgar + q.OxygenAgar) / 2;
+        u.AcetateAgar = (p.AcetateAgar + q.AcetateAgar) / 2;
+    }
+    for (int w = gtid(); w < 2 * nz; w += gstride()) {
+        const int side = w / nz, k = w - side * nz;
+        const int i = side ? nx : 0;
+        const int i2 = nx2 / 2 + i / 2 - nx / 4;
+        const AgaRef u = m.A(l, i, k);
+        const AgaRef p = m.A(l + 1, i2, k / 2), q = m.A(l + 1, i2, (k + 1) / 2);
+        u.CarbonAgar = (p.CarbonAgar + q.CarbonAgar) / 2;
+        u.OxygenAgar = (p.OxygenAgar + q.OxygenAgar) / 2;
+        u.AcetateAgar = (p.AcetateAgar + q.AcetateAgar) / 2;
+    }
+}
+
+// NestSolAga of level l from level l - 1.
+__global__ void __launch_bounds__(PH_THREADS) k_nut_nest(const EnvArgs* __restrict__ args, int l) {
+    const EnvArgs& a = *args;
+    const MG m = grid_mg(a);
+    const Geo g = m.geo(l, 1);
+    over_grid(g, [=](int i, int k) { nest_point(m, l, i, k, g); });
+}
+
+// copyFromMG into both buffers' interface rows (k = 0) and first agar rows.
+__global__ void __launch_bounds__(PH_THREADS) k_nut_copy_from_mg(const EnvArgs* __restrict__ args) {
+    const EnvArgs& a = *args;
+    const MG m = grid_mg(a);
+    const int BX = a.BX, RW = 4 * BX;
+    for (int w = gtid(); w < a.ML * 2 * BX; w += gstride()) {
+        const int l = w / (2 * BX), r = w - l * 2 * BX, k = r / BX, i = r - k * BX;
+        const AgaRef s = m.A(l, i, k);
+        for (int bb = 0; bb < 2; bb++) {
+            const RowS d{G(k == 0 ? a.swall[bb] : a.srow[bb]) + l * RW, BX};
+            d(FC, i) = s.CarbonAgar;
+            d(FO, i) = s.OxygenAgar;
+            d(FA, i) = s.AcetateAgar;
+        }
+    }
+}
+
+// One sweep over the whole device, the colony in global memory.
+__global__ void __launch_bounds__(PH_THREADS) k_nut_sweep(const EnvArgs* __restrict__ args, int full, int dir, int sw) {
+    const EnvArgs& a = *args;
+    const ChemConst cc = chem_const(a);
+    const int NB = a.BX * a.BZ;
+    sweep_body<gdouble>(a, cc, full != 0, dir, sw, G(a.scol[0]), G(a.scol[1]), 0, a.BX, NB, gtid(), gstride());
+}
+
+// The 200 sweeps of one outer iteration in one workgroup, with the colony
+// window [minX-1, maxX+1] x [0, maxH+1] in LDS when the launch sized it
+// (loaded from and stored back to the global working copy).
+__global__ void __launch_bounds__(ENV_THREADS) k_nut_sweeps_wg(EnvArgs args, int last) {
+    __shared__ EnvArgs a;
+    extern __shared__ double lds[];
+    const int tid = threadIdx.x;
+    if (tid == 0) a = args;
+    __syncthreads();
+    const ChemConst cc = chem_const(a);
+    const int BX = a.BX, NB = BX * a.BZ;
+    const bool use_lds = a.lds_colony && a.lds_w == a.bounds[1] - a.bounds[0] + 3 &&
+                         a.lds_nf == a.lds_w * (a.bounds[4] + 2);
+    int sw = 0, count = 0;  // 200 sweeps an outer iteration: both return to these
+    const int outer = last ? a.n_outer - 1 : 0;
+    if (use_lds) {
+        const int x0 = a.bounds[0] - 1, W = a.lds_w, H = a.lds_nf / W;
+        for (int w = tid; w < W * H; w += ENV_THREADS) {
+            const int z = w / W, x = x0 + (w - z * W);
+            for (int bb = 0; bb < 2; bb++)
+                for (int f = 0; f < 4; f++) lds[(bb * 4 + f) * a.lds_nf + w] = G(a.scol[bb])[f * NB + z * BX + x];
+        }
+        __syncthreads();
+        sweeps<double>(a, cc, outer, tid, lds, lds + 4 * a.lds_nf, x0, W, a.lds_nf, sw, count);
+        for (int w = tid; w < W * H; w += ENV_THREADS) {
+            const int z = w / W, x = x0 + (w - z * W);
+            for (int bb = 0; bb < 2; bb++)
+                for (int f = 0; f < 4; f++) G(a.scol[bb])[f * NB + z * BX + x] = lds[(bb * 4 + f) * a.lds_nf + w];
+        }
+    } else {
+        sweeps<gdouble>(a, cc, outer, tid, G(a.scol[0]), G(a.scol[1]), 0, BX, NB, sw, count);
+    }
+}
+
+// Write the working copies back (as the end of k_env_nutrients, from the
+// global working copy): colony fields and interface rows of both buffers,
+// and the agar of the last copyFromMG.
+__global__ void __launch_bounds__(PH_THREADS) k_nut_end(const EnvArgs* __restrict__ args) {
+    const EnvArgs& a = *args;
+    const MG m = grid_mg(a);
+    const int BX = a.BX, BZ = a.BZ, NB = BX * BZ, RW = 4 * BX, AL = a.ML;
+    for (int w = gtid(); w < NB; w += gstride()) {
+        const int x = w / BZ, z = w - x * BZ, s = z * BX + x;
+        for (int bb = 0; bb < 2; bb++) {
+            bcg_env_cell& c = a.colony[bb][w];
+            const gdouble* d = G(a.scol[bb]);
+            c.Carbon = d[FC * NB + s];
+            c.Waste = d[FW * NB + s];
+            c.Oxygen = d[FO * NB + s];
+            c.Acetate = d[FA * NB + s];
+        }
+    }
+    for (int w = gtid(); w < AL * BX; w += gstride()) {
+        const int l = w / BX, x = w - l * BX;
+        for (int bb = 0; bb < 2; bb++) {
+            bcg_aga& q = a.wall[bb][l][x];
+            const gdouble* dw = G(a.swall[bb]) + l * RW;
+            q.CarbonAgar = dw[FC * BX + x];
+            q.WasteAgar = dw[FW * BX + x];
+            q.OxygenAgar = dw[FO * BX + x];
+            q.AcetateAgar = dw[FA * BX + x];
+        }
+    }
+    if (a.n_outer > 0) {
+        for (int l = 0; l < AL; l++)
+            for (int w = gtid(); w < BX * a.BZA; w += gstride()) {
                 const int i = w / a.BZA, k = w - i * a.BZA + 1;
                 const AgaRef s = m.A(l, i, k);
                 for (int bb = 0; bb < 2; bb++) {

@@ -1,6 +1,7 @@
 """Planar colony environment on the GPU: density, height and normal fields,
 and the nutrient update (the bcg_env_* C ABI), plus the time-loop clock and
-one pass of the reference time loop (bcg_clock_*, bcg_sim_step).
+one pass of the reference time loop (bcg_clock_*, bcg_sim_step, or
+bcg_sim_prepare + bcg_sim_advance around the output step).
 
 Mirrors the reference's field/nutrient seam: GetDensity / ShiftDensity /
 BoxAverage / Height_Average / Smooth / GetSurfaceNormal
@@ -49,6 +50,13 @@ def default_env_params(**kw) -> EnvParams:
     for k, v in kw.items():
         setattr(p, k, v)
     return p
+
+
+# bcg_env_set_execution flags (include/bcg.h)
+EXEC_ONE_WORKGROUP = 1  # the whole update in one workgroup
+EXEC_NO_GRAPH = 2  # every phase launched directly
+EXEC_WG_SWEEPS = 4  # an outer iteration's sweeps in one workgroup
+EXEC_GRID_SWEEPS = 8  # one device-wide launch per sweep
 
 
 def env_params_from_file(path: str) -> EnvParams:
@@ -176,6 +184,11 @@ class Environment:
         (0: global memory); the results do not depend on it."""
         self._check(lib().bcg_env_set_lds_limit(self._h, int(nbytes)))
 
+    def set_execution(self, flags: int):
+        """EXEC_* flags: how the nutrient update runs; the results do not
+        depend on them."""
+        self._check(lib().bcg_env_set_execution(self._h, int(flags)))
+
     def fields(self, pass_: int = ENV_UPDATE):
         self._check(lib().bcg_env_fields(self._h, pass_))
 
@@ -210,6 +223,41 @@ def advance(c: Clock) -> None:
         raise BcgError(rc)
 
 
+def _raise(colony, env, rc):
+    msgs = []
+    for h, fn in ((colony._h, lib().bcg_last_error), (env._h if env is not None else None, lib().bcg_env_last_error)):
+        if h:
+            buf = ctypes.create_string_buffer(512)
+            fn(h, buf, 512)
+            if buf.value:
+                msgs.append(buf.value.decode(errors="replace"))
+    raise BcgError(rc, "; ".join(msgs))
+
+
+def sim_prepare(colony, env, c: Clock) -> None:
+    """The time loop's pass up to its output point (bcg_sim_prepare): the
+    UpdateFlag fields, nutrients and list rebuild, the HeightDensity fields."""
+    rc = lib().bcg_sim_prepare(colony._h, env._h if env is not None else None, ctypes.byref(c))
+    if rc:
+        _raise(colony, env, rc)
+
+
+def sim_advance(colony, c: Clock, moved: bool = False, lineage_cap: int = 4096):
+    """The rest of the pass (bcg_sim_advance): move + grow, divisions, clock.
+    Returns (lineage pairs, the moved cells before the divisions or None)."""
+    from .cells import CELL_DTYPE
+
+    nd = ctypes.c_int32(0)
+    lin = np.zeros(2 * lineage_cap, dtype=np.int32)
+    mv = np.zeros(colony.n, CELL_DTYPE) if moved else None
+    rc = lib().bcg_sim_advance(colony._h, ctypes.byref(c), mv.ctypes.data_as(ctypes.c_void_p) if moved else None,
+                               len(mv) if moved else 0, ctypes.byref(nd),
+                               lin.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), lineage_cap)
+    if rc:
+        _raise(colony, None, rc)
+    return lin[: 2 * min(nd.value, lineage_cap)].reshape(-1, 2), mv
+
+
 def sim_step(colony, env, c: Clock, lineage_cap: int = 4096):
     """One pass of the reference time loop (bcg_sim_step); returns the
     (parent, daughter) pairs of this step's divisions."""
@@ -218,14 +266,7 @@ def sim_step(colony, env, c: Clock, lineage_cap: int = 4096):
     rc = lib().bcg_sim_step(colony._h, env._h if env is not None else None, ctypes.byref(c), ctypes.byref(nd),
                             lin.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), lineage_cap)
     if rc:
-        msgs = []
-        for h, fn in ((colony._h, lib().bcg_last_error), (env._h if env is not None else None, lib().bcg_env_last_error)):
-            if h:
-                buf = ctypes.create_string_buffer(512)
-                fn(h, buf, 512)
-                if buf.value:
-                    msgs.append(buf.value.decode(errors="replace"))
-        raise BcgError(rc, "; ".join(msgs))
+        _raise(colony, env, rc)
     return lin[: 2 * min(nd.value, lineage_cap)].reshape(-1, 2)
 
 

@@ -625,6 +625,84 @@ def env_bench(repeats: int = 3, cpu: bool = True) -> dict:
     return out
 
 
+LARGE_INPUT = os.path.join(ROOT, "tests", "golden", "inputFile_20mM_LargeSimulation.txt")
+
+
+def env_bench_large(cpu: bool = True, cpu_outer: int = 2) -> dict:
+    """The reference's second shipped input (1520 x 100 colony boxes, Box_z_agar
+    512, 3 nested agar levels, Update_Time 0.01 h: 266 outer iterations of 3
+    V-cycles + 200 sweeps) on the demo restart state 8: one full nutrient
+    update on the GPU, and the reference's own code on `cpu_outer` outer
+    iterations (the per-iteration work is the same in every iteration) on one
+    host thread and on all of them. Reported per outer iteration and per
+    update."""
+    from baccolonygrowth_amd.cells import CELL_DTYPE, read_parameters
+    from baccolonygrowth_amd.environment import ENV_INIT, ENV_UPDATE, Environment, env_params_from_file
+    from baccolonygrowth_amd.simulation import Colony
+
+    p, _ = read_parameters(LARGE_INPUT)
+    p.maxNeighbours, p.planar, p.maxCells = 96, 1, 4000
+    ep = env_params_from_file(LARGE_INPUT)
+    z = np.load(ENV_STATE)
+    cells = np.ascontiguousarray(z["cells"]).view(CELL_DTYPE).reshape(-1).copy()
+    n_outer = int(3600 * ep.UpdateTime * 740 / 100)
+    out = {"grid": {"colony": [ep.BoxX, ep.BoxZ], "agar": [ep.BoxX, ep.BoxZAgar], "nested_levels": ep.maxLevelsMG},
+           "cells": int(len(cells)), "outer_iterations": n_outer}
+    with Colony(p) as col:
+        col.LoadCells(cells)
+        with Environment(col, ep) as env:
+            env.fields(ENV_INIT)
+            env.nutrients()
+            env.fields(ENV_UPDATE)
+            device_sync()
+            t0 = time.perf_counter()
+            env.nutrients()
+            el = time.perf_counter() - t0
+    out["gpu_ms"] = 1000.0 * el
+    out["gpu_ms_per_outer_iteration"] = 1000.0 * el / n_outer
+    log(f"[bench] LargeSimulation nutrient update: GPU {1000.0 * el:.0f} ms ({n_outer} outer iterations)")
+    out["cpu_baseline"] = None
+    if cpu:
+        try:
+            from oracle.oracle import RefEnv, RefLib, ref_available
+        except ImportError:
+            ref_available = None
+        if ref_available is not None and ref_available():
+            ref = RefLib()
+            ep_cpu = env_params_from_file(LARGE_INPUT)
+            ep_cpu.UpdateTime = (cpu_outer + 0.5) * 100 / (3600 * 740)
+
+            def ref_update(threads):
+                ref.set_threads(threads)
+                sim = ref.sim(p)
+                sim.set_cells(cells)
+                renv = RefEnv(ref, sim, ep_cpu)
+                renv.fields(ENV_INIT)
+                renv.nutrients()
+                renv.fields(ENV_UPDATE)
+                t0 = time.perf_counter()
+                renv.nutrients()
+                el = time.perf_counter() - t0
+                renv.close()
+                sim.close()
+                return el / cpu_outer
+
+            one = ref_update(1)
+            threads = host_threads()
+            many = ref_update(threads)
+            ref.set_threads(1)
+            out["cpu_baseline"] = {
+                "ms_per_outer_iteration": 1000.0 * many, "cores": threads, "kind": "reference",
+                "one_thread_ms_per_outer_iteration": 1000.0 * one,
+                "sample": f"{cpu_outer} outer iterations of the same update, the reference's sources compiled with "
+                "g++ -O3 -fopenmp (its threaded sweeps race; timing only)", "cpu_model": cpu_model()}
+            out["speedup_vs_cpu"] = many / (el / n_outer)
+            out["speedup_vs_cpu_one_thread"] = one / (el / n_outer)
+            log(f"[bench] LargeSimulation reference CPU per outer iteration: {1000.0 * one:.0f} ms (1 thread), "
+                f"{1000.0 * many:.0f} ms ({threads} threads); GPU {1000.0 * el / n_outer:.1f} ms")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -637,7 +715,7 @@ def main():
     ap.add_argument("--no-env", action="store_true")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the founding-colony, adjoint and full coupled configurations")
-    ap.add_argument("--only", choices=("mg", "env", "coupled", "small", "adjoint"), default=None,
+    ap.add_argument("--only", choices=("mg", "env", "env_large", "coupled", "small", "adjoint"), default=None,
                     help="run one secondary section alone and print its JSON object (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -646,6 +724,9 @@ def main():
 
     if args.only == "env":
         print(json.dumps(env_bench(cpu=not args.no_cpu)), flush=True)
+        return
+    if args.only == "env_large":
+        print(json.dumps(env_bench_large(cpu=not args.no_cpu)), flush=True)
         return
     if args.only == "mg":
         print(json.dumps(mg_bench()), flush=True)
@@ -732,10 +813,12 @@ def main():
     if not args.no_mg:
         log("[bench] multigrid 513x513x129 ...")
         mg = mg_bench()
-    envb = None
+    envb = envl = None
     if not args.no_env:
         log("[bench] planar nutrient update (demo grid) ...")
         envb = env_bench(cpu=not args.no_cpu)
+        log("[bench] planar nutrient update (LargeSimulation grid) ...")
+        envl = env_bench_large(cpu=not args.no_cpu)
     extra = {}
     if not args.no_configs:
         # the other configurations of the benchmark, each measured in full; a
@@ -792,6 +875,7 @@ def main():
         "cpu_baseline": cpu,
         "mg_solve": mg,
         "nutrient_update": envb,
+        "nutrient_update_large": envl,
     }
     out.update(extra)
     print(json.dumps(out), flush=True)

@@ -189,6 +189,14 @@ int bcg_mean_stress(bcg_ctx* ctx, double* stress, double* fnet);
 int bcg_step(bcg_ctx* ctx, double dt, double t, int32_t flags,
              int32_t* n_div, int32_t* lineage, int32_t lineage_cap);
 
+/* bcg_step that also copies, when `moved` is non-NULL, the moved and grown
+ * cells before the divisions (N of them, N the count before the step) to
+ * moved[moved_cap]: the array the reference's output step saves to its
+ * Restart file at that point (SaveCells of new_cells, ABM/Simulation.cpp:349).
+ * Adds one host synchronisation and the copy.                              */
+int bcg_step_out(bcg_ctx* ctx, double dt, double t, int32_t flags, bcg_cell* moved, int32_t moved_cap,
+                 int32_t* n_div, int32_t* lineage, int32_t lineage_cap);
+
 /* Counters of the last bcg_step: wall-contact draws (calls of F_cw), cells
  * that reached Ldiv, and maximum reduced list length seen.                 */
 int bcg_last_step_stats(bcg_ctx* ctx, int64_t* wall_draws, int32_t* n_flagged,
@@ -398,6 +406,18 @@ int bcg_env_last_error(const bcg_env* env, char* buf, int len);
  * them in global memory). Both placements give the same results. No
  * counterpart in the reference (a placement choice of this build).        */
 int bcg_env_set_lds_limit(bcg_env* env, int64_t bytes);
+/* How the nutrient update runs (no counterpart in the reference: placement
+ * choices of this build; every choice gives the same results, bit for bit).
+ * Default 0: each barrier of the update is a launch boundary; the
+ * relaxation passes, restrictions and prolongations of the wide multigrid
+ * levels and the grid-wide passes use the whole device; the narrow levels of
+ * each V-cycle run in one workgroup; the outer iterations are replayed from
+ * one recorded HIP graph.                                                   */
+#define BCG_ENV_EXEC_ONE_WORKGROUP 1  /* the whole update in one workgroup  */
+#define BCG_ENV_EXEC_NO_GRAPH      2  /* launch every phase directly        */
+#define BCG_ENV_EXEC_WG_SWEEPS     4  /* the 200 sweeps of an outer iteration in one workgroup */
+#define BCG_ENV_EXEC_GRID_SWEEPS   8  /* one device-wide launch per sweep   */
+int bcg_env_set_execution(bcg_env* env, int32_t flags);
 /* Nutrient state in the reference layouts (x-major, z fastest):
  *   colony, colony_old  [BoxX][BoxY][BoxZ]                 Environment, oldEnvironment
  *   agar, agar_old      [maxLevelsMG][BoxX][BoxY][BoxZAgar] FieldAgar, oldFieldAgar
@@ -451,10 +471,23 @@ int bcg_clock_init(bcg_clock* c, double t0, double dt, double OutputTime, double
 /* t += dt, then the three flags for the next step (ABM/Simulation.cpp:415-426). */
 int bcg_clock_advance(bcg_clock* c);
 
-/* One pass of the time loop (ABM/Simulation.cpp:245-427) without output:
- * UpdateFlag -> fields(UPDATE), nutrients, list rebuild; HeightDensity ->
- * fields(HEIGHT); move + grow; divisions (lineage as bcg_step); clock
- * advance. `env` may be NULL (fields stay as set).                          */
+/* One pass of the time loop (ABM/Simulation.cpp:254-427), split at its
+ * output point so that a driver's output calls go where the reference has
+ * them:
+ *   bcg_sim_prepare: UpdateFlag -> fields(UPDATE), nutrients and the list
+ *     rebuild (getNeighbours, :290); HeightDensityUpdateFlag ->
+ *     fields(HEIGHT) (:296-312). At OutFlag the driver then writes the cell
+ *     file from bcg_mean_stress and bcg_get_cells (:313-327; the F_cw
+ *     draws of mean_stress come before the step's, as there).
+ *   bcg_sim_advance: move + grow; at OutFlag `moved` receives the moved
+ *     cells before the divisions, which the reference saves as its Restart
+ *     state (:349; NULL: skipped); divisions (lineage as bcg_step); the
+ *     buffer swap and the clock advance (:407-426).
+ * bcg_sim_step = bcg_sim_prepare + bcg_sim_advance without output. `env`
+ * may be NULL (fields stay as set).                                         */
+int bcg_sim_prepare(bcg_ctx* ctx, bcg_env* env, const bcg_clock* clock);
+int bcg_sim_advance(bcg_ctx* ctx, bcg_clock* clock, bcg_cell* moved, int32_t moved_cap, int32_t* n_div,
+                    int32_t* lineage, int32_t lineage_cap);
 int bcg_sim_step(bcg_ctx* ctx, bcg_env* env, bcg_clock* clock, int32_t* n_div, int32_t* lineage,
                  int32_t lineage_cap);
 

@@ -274,6 +274,20 @@ class RefLib(_Base):
         self.lib.ref_sim_step_omp.restype = ctypes.c_int
         self._p = Params()
 
+    def step_split(self, sim: "Sim", dt: float, t: float, rebuild: bool, lineage_cap: int = 100000):
+        """The reference's step that also returns the moved and grown cells
+        before the divisions (what its output step saves): (lineage, moved)."""
+        L = self.lib
+        L.ref_sim_step_split.argtypes = [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _vp, _ip, ctypes.c_int]
+        L.ref_sim_step_split.restype = ctypes.c_int
+        moved = np.zeros(sim.n, CELL_DTYPE)
+        lin = np.zeros((lineage_cap, 2), np.int32)
+        nd = L.ref_sim_step_split(sim.h, dt, t, int(bool(rebuild)), _cells_ptr(moved), lin.ctypes.data_as(_ip),
+                                  lineage_cap)
+        if nd < 0:
+            raise RuntimeError(f"step rc={nd}")
+        return lin[: min(nd, lineage_cap)].copy(), moved
+
     def set_threads(self, n: int) -> None:
         """OpenMP threads of the reference's parallel regions (1 for parity)."""
         self.lib.ref_set_threads(n)

@@ -320,6 +320,46 @@ int ref_sim_step(void* h, double dt, double t, int rebuild, int* lineage, int li
     return done;
 }
 
+// ref_sim_step that also copies the moved and grown cells before the
+// divisions (N of them) to `moved`: the array the output step saves as its
+// Restart state (SaveCells(new_cells, ...), ABM/Simulation.cpp:349).
+int ref_sim_step_split(void* h, double dt, double t, int rebuild, bcg_cell* moved, int* lineage, int lineage_cap) {
+    RefSim* s = (RefSim*)h;
+    set_globals(&s->prm);
+    int N_cells = s->N;
+    if (rebuild) getNeighbours(s->old_cells, N_cells, *s->grid, s->NL, s->maxNeighbours);
+    int numDivide = 0;
+    for (int cellID = 0; cellID < N_cells; cellID++) {
+        MoveCell(cellID, *s->grid, s->old_cells, s->new_cells, s->NL[cellID], dt, *s->Height, *s->Normal,
+                 *s->Wall);
+        GrowCell(s->new_cells[cellID], cellID, dt, s->dividing, numDivide, *s->Env, NULL, *s->grid);
+    }
+    memcpy((void*)moved, (const void*)s->new_cells, sizeof(Cell) * N_cells);
+    int done = 0;
+    for (int cellCount = 0; cellCount < numDivide; cellCount++) {
+        if (N_cells > (maxCells - 1)) break;
+        const int cellID = s->dividing[cellCount];
+        DivideCell(cellID, N_cells, s->new_cells, *s->grid, s->NL[cellID], *s->Wall, *s->Height, *s->Normal, t);
+        if (done < lineage_cap) {
+            lineage[2 * done] = cellID;
+            lineage[2 * done + 1] = N_cells;
+        }
+        done++;
+        N_cells++;
+        int IDlen = s->NL[cellID][0];
+        memcpy(s->IDlist, s->NL[cellID], (IDlen + 1) * sizeof(int));
+        s->IDlist[0] = cellID;
+        s->IDlist[IDlen + 1] = N_cells - 1;
+        IDlen += 2;
+        getNeighbours(s->new_cells, *s->grid, s->NL, s->maxNeighbours, s->IDlist, IDlen);
+    }
+    Cell* tmp = s->new_cells;
+    s->new_cells = s->old_cells;
+    s->old_cells = tmp;
+    s->N = N_cells;
+    return done;
+}
+
 // The same step with the move/grow loop on OpenMP threads, as the
 // reference's time loop runs it (ABM/Simulation.cpp:330-335: a static
 // parallel for over MoveCell + GrowCell, whose bin moves and dividing-list

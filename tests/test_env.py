@@ -489,3 +489,43 @@ def test_large_simulation_grid_matches_reference():
             assert env.nutrients() == int(g["n2"])
             w = _check_large_state(env.state(), g, ep, "nut2")
             say(f"    update 2 ok (max rel diff {w:.1e})")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("grid", ["demo", "large"])
+def test_execution_modes_give_identical_updates(grid):
+    """The nutrient update in phases (the default: graph-replayed outer
+    iterations, device-wide passes on the wide levels), launched directly,
+    with the sweeps in one workgroup or device-wide, and as one workgroup:
+    four outer iterations (the recorded graph replayed three times, with
+    host synchronisations between the calls), bit for bit the same."""
+    from baccolonygrowth_amd.environment import (EXEC_GRID_SWEEPS, EXEC_NO_GRAPH, EXEC_ONE_WORKGROUP, EXEC_WG_SWEEPS,
+                                                 ENV_INIT, ENV_UPDATE, Environment, env_params_from_file)
+    from baccolonygrowth_amd.simulation import Colony
+
+    if grid == "demo":
+        p = demo_params()
+        p.maxCells = 5000
+        ep = env_params_from_file(DEMO_INPUT)
+    else:
+        p, ep = _large_params(np.load(GOLDEN_LARGE))
+    ep.UpdateTime = 4.5 * 100 / (3600 * 740)  # 4 outer iterations
+    _, _, cells = restart_cells(8)
+    say(f"[gpu] execution modes on the {grid} grid: {len(cells)} cells")
+    states = {}
+    for flags in (0, EXEC_NO_GRAPH | EXEC_GRID_SWEEPS, EXEC_WG_SWEEPS, EXEC_ONE_WORKGROUP):
+        with Colony(p) as col:
+            col.set_cells(cells)
+            _start_fields(col, p)
+            with Environment(col, ep) as env:
+                env.set_execution(flags)
+                env.fields(ENV_INIT)
+                assert env.nutrients() == 800
+                env.fields(ENV_UPDATE)
+                env.nutrients()
+                states[flags] = env.state()
+    ref = states[0]
+    for flags, st in states.items():
+        for k in ref:
+            assert np.array_equal(st[k].view(np.uint8), ref[k].view(np.uint8)), f"flags {flags}: {k} differs"
+    say("    all modes equal")
