@@ -3,10 +3,11 @@ normal, nutrients) and of a coupled run of the time loop, from the
 reference's own code compiled from source into oracle/_ref/libbcgref.so.
 
 Run in a container that has the reference checkout:
-    make -C oracle ref && python tests/golden/make_env_golden.py [large]
-Output (committed): tests/golden/golden_env.npz and golden_env_large.npz
-(the reference's second shipped input, below; "large" writes only that one)
-- inputs (grid, times, seed) and the reference's outputs only. The grid is the demo input's with
+    make -C oracle ref && python tests/golden/make_env_golden.py [large|output_loop]
+Output (committed): tests/golden/golden_env.npz, golden_env_large.npz (the
+reference's second shipped input) and golden_output_loop.npz (a time loop
+with output steps); "large" or "output_loop" writes only that one - inputs
+(grid, times, seed) and the reference's outputs only. The grid is the demo input's with
 BoxX = 128, BoxZ = 16, BoxZAgar = 64 - every multigrid level keeps an exact
 2:1 ratio in x and at least two z points, so that the reference reads no
 element outside its arrays - and the demo restart state 5 (129 cells).
@@ -150,6 +151,7 @@ def main():
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), "bytes;", len(lineage), "divisions in the loop")
     large(ref)
+    output_loop(ref)
 
 
 # ---------------------------------------------------------------- large grid
@@ -233,8 +235,72 @@ def large(ref):
     print(path, os.path.getsize(path), "bytes")
 
 
+# ------------------------------------------------------ output steps in the loop
+# The time loop with output steps in the reference's order
+# (ABM/Simulation.cpp:254-427): UpdateFlag fields, nutrients and the list
+# rebuild; HeightDensity fields; at OutFlag mean_stress of every cell (the
+# cell file; its F_cw draws come first), then move + grow, the moved cells
+# before the divisions (the Restart file), divisions. OutputTime equals
+# Update_Time, so output steps fall on update steps (11 and 21).
+def output_loop(ref):
+    update_time, steps, seed = 1e-3, 25, 13
+    p, ep = params(update_time)
+    t0, dt, cells = restart_cells(5)
+    rng = np.random.default_rng(seed)
+    cells = cells.copy()
+    sel = rng.random(len(cells)) < 0.1
+    cells["Ldiv"][sel] = cells["Length"][sel] * (1.0 + rng.uniform(0.0, 5e-5, sel.sum()))
+    out = {"cells_in": np.ascontiguousarray(cells).view(np.uint8), "seed": np.array(seed), "steps": np.array(steps),
+           "update_time": np.array(update_time), "output_time": np.array(update_time)}
+    sim = ref.sim(p)
+    sim.set_cells(cells)
+    _, _, wall, _, _ = synthetic_fields(p, cells, seed=seed)
+    sim.set_fields(wall=wall)
+    ref.srand(seed)
+    env = RefEnv(ref, sim, ep)
+    env.fields(0)
+    env.nutrients()
+    t = t0
+    upd, nxt_u, hd, nxt_h, outf, nxt_o = True, update_time, True, update_time / 10.0, False, update_time
+    lineage, out_steps = [], []
+    for k in range(steps):
+        if upd:
+            env.fields(1)
+            env.nutrients()
+            sim.rebuild()
+        if hd:
+            env.fields(2)
+        if outf:
+            S, F = sim.mean_stress()
+            out[f"out{k}_stress"], out[f"out{k}_fnet"] = S, F
+        lin, moved = ref.step_split(sim, dt, t, False)
+        if outf:
+            out[f"out{k}_moved"] = np.ascontiguousarray(moved).view(np.uint8)
+            out_steps.append(k)
+        lineage += lin.tolist()
+        t += dt
+        outf = nxt_o <= 0
+        nxt_o = (update_time if outf else nxt_o) - dt
+        upd = nxt_u <= 0
+        nxt_u = (update_time if upd else nxt_u) - dt
+        hd = nxt_h <= 0
+        nxt_h = (update_time / 10.0 if hd else nxt_h) - dt
+    out["wall"] = wall
+    out["out_steps"] = np.array(out_steps)
+    out["cells"] = np.ascontiguousarray(sim.cells()).view(np.uint8)
+    out["lineage"] = np.array(lineage, dtype=np.int32).reshape(-1, 2)
+    out["next_draws"] = np.array([ref.rand() for _ in range(4)])
+    env.close()
+    sim.close()
+    path = os.path.join(GOLDEN, "golden_output_loop.npz")
+    np.savez_compressed(path, **out)
+    print(path, os.path.getsize(path), "bytes; output steps", out_steps, ";", len(lineage), "divisions")
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["large"]:
         large(RefLib())
+    elif sys.argv[1:] == ["output_loop"]:
+        output_loop(RefLib())
     else:
         main()

@@ -529,3 +529,64 @@ def test_execution_modes_give_identical_updates(grid):
         for k in ref:
             assert np.array_equal(st[k].view(np.uint8), ref[k].view(np.uint8)), f"flags {flags}: {k} differs"
     say("    all modes equal")
+
+
+GOLDEN_OUTLOOP = os.path.join(GOLDEN, "golden_output_loop.npz")
+
+
+@pytest.mark.gpu
+def test_time_loop_with_output_steps_matches_reference():
+    """The time loop split at its output point (bcg_sim_prepare, then at
+    OutFlag the cell file's mean_stress, then bcg_sim_advance with the moved
+    cells that the Restart file saves), with output steps falling on update
+    steps, against the reference's own loop order: stress and net force of
+    every cell, the saved cells, lineage, final cells and the rand()
+    position."""
+    from baccolonygrowth_amd.environment import ENV_INIT, Environment, clock, sim_advance, sim_prepare
+    from baccolonygrowth_amd.simulation import Colony
+
+    g = np.load(GOLDEN_OUTLOOP)
+    update_time = float(g["update_time"])
+    p, ep = _params(np.load(GOLDEN_ENV), update_time)
+    t0, dt, _ = restart_cells(5)
+    cells = np.ascontiguousarray(g["cells_in"]).view(CELL_DTYPE).reshape(-1).copy()
+    out_steps = [int(k) for k in g["out_steps"]]
+    say(f"[gpu] time loop with output steps {out_steps}: {len(cells)} cells")
+    with Colony(p) as col:
+        col.set_cells(cells)
+        _start_fields(col, p)
+        col.set_fields(wall=g["wall"])
+        col.srand(int(g["seed"]))
+        with Environment(col, ep) as env:
+            env.fields(ENV_INIT)
+            env.nutrients()
+            c = clock(t0, dt, float(g["output_time"]), update_time)
+            lineage, seen = [], []
+            for k in range(int(g["steps"])):
+                out = bool(c.OutFlag)
+                assert out == (k in out_steps), k
+                sim_prepare(col, env, c)
+                if out:
+                    S, F = col.mean_stress()
+                    for name, got, exp in (("stress", S, g[f"out{k}_stress"]), ("fnet", F, g[f"out{k}_fnet"])):
+                        err = np.abs(got - exp).max() / np.abs(exp).max()
+                        assert err <= 1e-10, (k, name, err)
+                lin, moved = sim_advance(col, c, moved=out)
+                lineage += lin.tolist()
+                if out:
+                    exp = np.ascontiguousarray(g[f"out{k}_moved"]).view(CELL_DTYPE).reshape(-1)
+                    assert len(moved) == len(exp)
+                    for f in ("p", "q", "Length"):
+                        d = np.abs(moved[f] - exp[f]).max()
+                        assert d <= 1e-10 * max(np.abs(exp[f]).max(), 1.0), (k, f, d)
+                    seen.append(k)
+            assert seen == out_steps
+            assert np.array_equal(np.array(lineage, np.int32).reshape(-1, 2), g["lineage"])
+            got = col.cells()
+            exp = np.ascontiguousarray(g["cells"]).view(CELL_DTYPE).reshape(-1)
+            assert len(got) == len(exp)
+            for f in ("p", "q"):
+                d = np.abs(got[f] - exp[f]).max()
+                assert d <= 1e-10 * max(np.abs(exp[f]).max(), 1.0), (f, d)
+            assert [col.rand() for _ in range(4)] == g["next_draws"].tolist()
+            say(f"    ok: {len(lineage)} divisions, output steps {seen}")
