@@ -200,6 +200,8 @@ def read_parameters(path: str, base: Params | None = None):
                 if name == "CrossFeeding":
                     v = 1 if v else 0
                 setattr(p, _KEYS_I[name], v)
+            elif name == "Restart_dir":
+                extra[name] = value  # strcpy of the token (InputOutput.cpp:642-643)
             elif name == "Box_Dim":
                 p.BoxX = p.BoxY = _atoi(value)
             else:
@@ -235,12 +237,15 @@ CELLS_LINE = ("%4.4f %d %d %4.4f %4.4f %4.4f %4.4f %4.4f %4.4f %4.4f %4.4f %4.4E
               "%4.4f %4.4f %4.4f %4.4f %4.4f %4.4f %4.4f %4.4f %d %d %d\n")
 
 
-def write_cells_text(path: str, t: float, cells: np.ndarray, stress: np.ndarray) -> None:
+def write_cells_text(target, t: float, cells: np.ndarray, stress: np.ndarray) -> None:
     """The output step's cell file (ABM/Simulation.cpp:320-326 with
     ABM/InputOutput.cpp:799-804): one line per cell in ID order, with the
     stress diagonal of mean_stress (stress[:, 0:3], e.g. from
-    Colony.mean_stress)."""
-    with open(path, "w") as fh:
+    Colony.mean_stress). `target` is a path (written anew) or an open text
+    file (written at its position)."""
+    own = not hasattr(target, "write")
+    fh = open(target, "w") if own else target
+    try:
         for i, c in enumerate(cells):
             p, q, v = c["p"], c["q"], c["Velocity"]
             df, sf = c["DynFric"], c["StaFric"]
@@ -248,6 +253,10 @@ def write_cells_text(path: str, t: float, cells: np.ndarray, stress: np.ndarray)
                                    stress[i, 1], stress[i, 2], v[0], v[1], v[2], c["GrowthRate"], df[0], df[1], df[2],
                                    sf[0], sf[1], sf[2], c["time_p"], c["time_q"], c["age_p"], c["age_q"],
                                    c["Ancestor"]))
+        fh.flush()
+    finally:
+        if own:
+            fh.close()
 
 
 def save_restart(path: str, cells: np.ndarray, t: float, dt: float) -> None:
@@ -256,3 +265,34 @@ def save_restart(path: str, cells: np.ndarray, t: float, dt: float) -> None:
     with open(path, "wb") as fh:
         fh.write(np.array([t, dt], "<f8").tobytes())
         fh.write(cells.tobytes())
+
+
+RAND_MAX = 2147483647  # glibc
+
+
+def first_cells(p: Params, colony_size: int = 1, rand=None) -> np.ndarray:
+    """The initial colony without a restart, AddFirstCells
+    (InputOutput.cpp:294-390): one rod of length L_divide/2 - R lying along x
+    through the origin at height R, Ldiv = L + L_divide/2, Type 1, Ancestor 1,
+    at rest. Each placement attempt takes one rand() draw (its random radius
+    is not used); `rand` is the draw function of the run's stream.
+
+    The reference places every cell of a colony at the same point, so a
+    second cell always overlaps the first and the reference gives up after
+    10000 attempts ("Unable to generate initial cells!"); only
+    IniColonySize 1 starts a run, and other sizes raise ValueError."""
+    if int(colony_size) != 1:
+        raise ValueError(f"IniColonySize {colony_size}: the reference cannot place more than one initial cell")
+    c = np.zeros(1, CELL_DTYPE)
+    L = p.L_divide * 0.5 - p.cellRadius
+    c["Length"] = L
+    c["Radius"] = p.cellRadius
+    if rand is not None:
+        rand()
+    c["p"] = (L / 2.0, 0.0, p.cellRadius)
+    c["q"] = (-L / 2.0, 0.0, p.cellRadius)
+    c["Ancestor"] = 1
+    c["Ldiv"] = L + p.L_divide / 2
+    c["Type"] = 1
+    return c
+

@@ -15,6 +15,7 @@ import ctypes
 import numpy as np
 
 from ._lib import AGA_FIELDS, ENV_CELL_FIELDS, BcgError, Clock, EnvParams, _vp, lib
+from .output import ENV_OUTPUT_FIELDS, write_density_text, write_env_text  # noqa: F401 (output-step writers)
 
 ENV_DTYPE = np.dtype([(f, "<f8") for f in ENV_CELL_FIELDS])  # struct LocalEnv, 136 B
 AGA_DTYPE = np.dtype([(f, "<f8") for f in AGA_FIELDS])  # struct LocalAga, 32 B
@@ -268,37 +269,3 @@ def sim_step(colony, env, c: Clock, lineage_cap: int = 4096):
     if rc:
         _raise(colony, env, rc)
     return lin[: 2 * min(nd.value, lineage_cap)].reshape(-1, 2)
-
-
-# Fields of the environment file, in the order its writer prints them
-# (Array3D<LocalEnv>::Output, ABM/Array.h:931-1100).
-ENV_OUTPUT_FIELDS = ("Carbon", "Acetate", "Waste", "Oxygen", "GrowthRate1", "GrowthRate2", "GrCarAer", "GrCarAna",
-                     "GrAceAer", "pH", "Hydrox", "Buffer", "Acetic", "qCarmnt", "qAcemnt", "pAcemnt", "qOxymnt")
-
-
-def _planes(values, BX: int, BY: int, BZ: int, blank: bool):
-    """Rows of "%6E\t" values, x fastest, y rows, z planes, from a flat
-    x-major array (x * BY * BZ + y * BZ + z); a blank line after each plane
-    when `blank`."""
-    out = []
-    for zi in range(BZ):
-        for yi in range(BY):
-            out.append("".join("%6E\t" % values[(xi * BY + yi) * BZ + zi] for xi in range(BX)) + "\n")
-        if blank:
-            out.append("\n")
-    return out
-
-
-def write_density_text(path: str, density_shift: np.ndarray, BX: int, BZ: int, BY: int = 1) -> None:
-    """The output step's density file, DensityShiftP.Output(Files.density, 1)
-    (ABM/Simulation.cpp:198, 355; Array3D<double>::Output, ABM/Array.h:568-583)."""
-    with open(path, "w") as fh:
-        fh.writelines(_planes(density_shift, BX, BY, BZ, True))
-
-
-def write_env_text(path: str, colony: np.ndarray, BX: int, BZ: int, BY: int = 1) -> None:
-    """The output step's environment file, Environment.Output(Files.env, 1)
-    (ABM/Simulation.cpp:221, 362): one block per field of ENV_OUTPUT_FIELDS."""
-    with open(path, "w") as fh:
-        for f in ENV_OUTPUT_FIELDS:
-            fh.writelines(_planes(colony[f], BX, BY, BZ, False))

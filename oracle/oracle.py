@@ -292,6 +292,70 @@ class RefLib(_Base):
         """OpenMP threads of the reference's parallel regions (1 for parity)."""
         self.lib.ref_set_threads(n)
 
+    # -- the reference's file writers and readers (ref_shim.cpp, file I/O)
+    def write_agar(self, path: str, agar: np.ndarray, BX: int, BY: int, BZA: int, levels: int, n: int = 1) -> None:
+        a = np.ascontiguousarray(agar)
+        L = self.lib
+        L.ref_write_agar.argtypes = [ctypes.c_char_p, _vp] + [ctypes.c_int] * 5
+        if L.ref_write_agar(path.encode(), a.ctypes.data, BX, BY, BZA, levels, n):
+            raise OSError(path)
+
+    def write_wall(self, path: str, wall: np.ndarray, BX: int, BY: int, levels: int) -> None:
+        w = np.ascontiguousarray(wall)
+        L = self.lib
+        L.ref_write_wall.argtypes = [ctypes.c_char_p, _vp] + [ctypes.c_int] * 3
+        if L.ref_write_wall(path.encode(), w.ctypes.data, BX, BY, levels):
+            raise OSError(path)
+
+    def read_pair(self, kind: str, path: str, a: np.ndarray, b: np.ndarray, *dims) -> None:
+        """Two Input calls on one stream into a then b (kind agar, wall or
+        env; dims BX, BY[, BZ]); both arrays are read and written in place."""
+        fn = getattr(self.lib, f"ref_read_{kind}_pair")
+        fn.argtypes = [ctypes.c_char_p, _vp, _vp] + [ctypes.c_int] * len(dims)
+        assert a.flags.c_contiguous and b.flags.c_contiguous
+        if fn(path.encode(), a.ctypes.data, b.ctypes.data, *dims):
+            raise OSError(path)
+
+    def save_cells(self, path: str, cells: np.ndarray, t: float, dt: float) -> None:
+        c = np.ascontiguousarray(cells, dtype=CELL_DTYPE)
+        L = self.lib
+        L.ref_save_cells.argtypes = [ctypes.c_char_p, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        if L.ref_save_cells(path.encode(), c.ctypes.data, len(c), t, dt):
+            raise OSError(path)
+
+    def add_first_cells(self, sim: "Sim", size: int = 1, radius: float = 8.0) -> int:
+        """AddFirstCells into `sim` (size 1 only)."""
+        f = self.lib.ref_sim_add_first_cells
+        f.argtypes = [_vp, ctypes.c_int, ctypes.c_double]
+        n = f(sim.h, size, radius)
+        if n < 0:
+            raise ValueError(size)
+        return n
+
+    def rough_wall(self, p: Params) -> np.ndarray:
+        """The rough wall's expression over this library's rand() stream."""
+        w = np.zeros(p.BoxX * p.BoxY)
+        self.lib.ref_rough_wall.argtypes = [ctypes.POINTER(Params), _vp]
+        self.lib.ref_rough_wall.restype = None
+        self.lib.ref_rough_wall(ctypes.byref(p), w.ctypes.data)
+        return w
+
+    def sim_rough_wall(self, sim: "Sim") -> None:
+        """The run's rough wall into `sim` (rand() draws)."""
+        self.lib.ref_sim_rough_wall.argtypes = [_vp]
+        self.lib.ref_sim_rough_wall.restype = None
+        self.lib.ref_sim_rough_wall(sim.h)
+
+    def load_cells(self, sim: "Sim", path: str):
+        """LoadCells into `sim`: (t, dt, number of cells)."""
+        L = self.lib
+        L.ref_sim_load_cells.argtypes = [_vp, ctypes.c_char_p, _dp, _dp]
+        t, dt = ctypes.c_double(0), ctypes.c_double(0)
+        n = L.ref_sim_load_cells(sim.h, path.encode(), ctypes.byref(t), ctypes.byref(dt))
+        if n < 0:
+            raise OSError(path)
+        return t.value, dt.value, n
+
     def step_omp(self, sim: "Sim", dt: float, t: float, rebuild: bool) -> int:
         """The reference's step with its threaded move/grow loop (timing only:
         the division order then depends on the threads)."""
@@ -356,6 +420,20 @@ class RefEnv:
 
     def nutrients(self) -> int:
         return self.ref.lib.ref_env_nutrients(self.h)
+
+    def append(self, agar_path: str, wall_path: str) -> None:
+        """The reference's own AgarField and WallField files of the state."""
+        f = self.ref.lib.ref_env_append
+        f.argtypes = [_vp, ctypes.c_char_p, ctypes.c_char_p]
+        if f(self.h, agar_path.encode(), wall_path.encode()) != 0:
+            raise OSError("cannot write the field files")
+
+    def field_restart(self, dirname: str, index: int) -> None:
+        """The reference's field restart reads from <dirname>/.../<index>.txt."""
+        f = self.ref.lib.ref_env_field_restart
+        f.argtypes = [_vp, ctypes.c_char_p, ctypes.c_int]
+        if f(self.h, dirname.encode(), index) != 0:
+            raise OSError(dirname)
 
     def get_fields(self):
         p = self.p

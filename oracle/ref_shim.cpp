@@ -9,6 +9,7 @@
 // the reference's MoveCell / GrowCell / DivideCell / getNeighbours doing the
 // work. Field arrays are injected instead of computed.
 #include <omp.h>
+#include <fstream>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -658,6 +659,200 @@ void ref_env_get_state(void* h, bcg_env_cell* colony, bcg_env_cell* colony_old, 
             if (wall_old) memcpy(&wall_old[l * BoxX + x], &e->wall_old[l]->At(x, 0), sizeof(bcg_aga));
         }
     }
+}
+
+// ---------------------------------------------------------------- file I/O
+// The reference's own writers and readers of its output files, on arrays in
+// the library's layouts (agar of level l at l * BX * BY * BZA, x-major; wall
+// of level l at l * BX * BY). The arrays are built with the current env
+// globals (LocalAga() reads maxCarbon and maxO2), then overwritten.
+
+// FieldAgar[level]->Append(f, n) for every level (ABM/Array.h:1447-1494)
+int ref_write_agar(const char* path, const bcg_aga* data, int BX, int BY, int BZA, int levels, int n) {
+    FILE* f = fopen(path, "w");
+    if (!f) return -1;
+    const size_t na = (size_t)BX * BY * BZA;
+    for (int l = 0; l < levels; l++) {
+        AgaArray3D a(BX, BY, BZA);
+        for (int x = 0; x < BX; x++)
+            for (int y = 0; y < BY; y++)
+                for (int z = 0; z < BZA; z++)
+                    memcpy(&a.At(x, y, z), &data[l * na + ((size_t)x * BY + y) * BZA + z], sizeof(LocalAga));
+        a.Append(f, n);
+    }
+    fclose(f);
+    return 0;
+}
+
+// FieldWall[level]->Append(f) for every level (ABM/Array.h:419-457)
+int ref_write_wall(const char* path, const bcg_aga* data, int BX, int BY, int levels) {
+    FILE* f = fopen(path, "w");
+    if (!f) return -1;
+    for (int l = 0; l < levels; l++) {
+        AgaArray2D w(BX, BY);
+        for (int x = 0; x < BX; x++)
+            for (int y = 0; y < BY; y++) memcpy(&w.At(x, y), &data[(size_t)l * BX * BY + x * BY + y], sizeof(LocalAga));
+        w.Append(f);
+    }
+    fclose(f);
+    return 0;
+}
+
+// Two Input calls on one stream, as the field restart makes them
+// (ABM/CellMD3D.cpp:105-125): a then b are read from `path`; both hold the
+// values to keep where the file gives none.
+int ref_read_agar_pair(const char* path, bcg_aga* a, bcg_aga* b, int BX, int BY, int BZA) {
+    std::ifstream in(path);
+    if (!in) return -1;
+    bcg_aga* io[2] = {a, b};
+    for (int k = 0; k < 2; k++) {
+        AgaArray3D arr(BX, BY, BZA);
+        for (int x = 0; x < BX; x++)
+            for (int y = 0; y < BY; y++)
+                for (int z = 0; z < BZA; z++)
+                    memcpy(&arr.At(x, y, z), &io[k][((size_t)x * BY + y) * BZA + z], sizeof(LocalAga));
+        arr.Input(in);
+        for (int x = 0; x < BX; x++)
+            for (int y = 0; y < BY; y++)
+                for (int z = 0; z < BZA; z++)
+                    memcpy(&io[k][((size_t)x * BY + y) * BZA + z], &arr.At(x, y, z), sizeof(LocalAga));
+    }
+    return 0;
+}
+
+int ref_read_wall_pair(const char* path, bcg_aga* a, bcg_aga* b, int BX, int BY) {
+    std::ifstream in(path);
+    if (!in) return -1;
+    bcg_aga* io[2] = {a, b};
+    for (int k = 0; k < 2; k++) {
+        AgaArray2D arr(BX, BY);
+        for (int x = 0; x < BX; x++)
+            for (int y = 0; y < BY; y++) memcpy(&arr.At(x, y), &io[k][x * BY + y], sizeof(LocalAga));
+        arr.Input(in);
+        for (int x = 0; x < BX; x++)
+            for (int y = 0; y < BY; y++) memcpy(&io[k][x * BY + y], &arr.At(x, y), sizeof(LocalAga));
+    }
+    return 0;
+}
+
+int ref_read_env_pair(const char* path, bcg_env_cell* a, bcg_env_cell* b, int BX, int BY, int BZ) {
+    std::ifstream in(path);
+    if (!in) return -1;
+    bcg_env_cell* io[2] = {a, b};
+    for (int k = 0; k < 2; k++) {
+        EnvArray3D arr(BX, BY, BZ);
+        for (int x = 0; x < BX; x++)
+            for (int y = 0; y < BY; y++)
+                for (int z = 0; z < BZ; z++)
+                    memcpy(&arr.At(x, y, z), &io[k][((size_t)x * BY + y) * BZ + z], sizeof(LocalEnv));
+        arr.Input(in);
+        for (int x = 0; x < BX; x++)
+            for (int y = 0; y < BY; y++)
+                for (int z = 0; z < BZ; z++)
+                    memcpy(&io[k][((size_t)x * BY + y) * BZ + z], &arr.At(x, y, z), sizeof(LocalEnv));
+    }
+    return 0;
+}
+
+// SaveCells into a new file (ABM/InputOutput.cpp:434-448)
+int ref_save_cells(const char* path, const bcg_cell* cells, int n, double t, double dt) {
+    FILE* f = fopen(path, "w");
+    if (!f) return -1;
+    SaveCells(f, (Cell*)cells, n, t, dt);
+    fclose(f);
+    return 0;
+}
+
+// LoadCells into the simulation (ABM/InputOutput.cpp:394-431): cells, their
+// grid insertion in ID order, t and dt. -1 if the file holds more cells than
+// the simulation's capacity (the reference does not check).
+int ref_sim_load_cells(void* h, const char* path, double* t, double* dt) {
+    RefSim* s = (RefSim*)h;
+    set_globals(&s->prm);
+    FILE* f = fopen(path, "rb");
+    if (!f) return -1;
+    fseek(f, 0, SEEK_END);
+    long size = ftell(f);
+    fclose(f);
+    if (size < 16 || (size - 16) / (long)sizeof(Cell) > s->prm.maxCells) return -1;
+    s->N = LoadCells((char*)path, s->old_cells, *s->grid, *t, *dt);
+    return s->N;
+}
+
+// AddFirstCells (ABM/InputOutput.cpp:294-390) for one colony of `size`
+// cells into the simulation; size 1 only (a larger colony never places its
+// second cell and the reference exits).
+int ref_sim_add_first_cells(void* h, int size, double radius) {
+    RefSim* s = (RefSim*)h;
+    set_globals(&s->prm);
+    if (size != 1) return -1;
+    Inputs ini;
+    ini.ColonyNumber = 1;
+    ini.ColonySeparation = 0;
+    ini.ColonyRadius = radius;
+    ini.ColonySize = size;
+    ini.Fraction = 0;
+    s->N = AddFirstCells(s->old_cells, L_divide, cellRadius, *s->grid, ini);
+    return s->N;
+}
+
+// The rough wall's expression (ABM/Simulation.cpp:160-168), compiled with the
+// reference's flags: BoxX * BoxY draws of this library's rand().
+void ref_rough_wall(const bcg_params* p, double* out) {
+    set_globals(p);
+    for (int ii = 0; ii < BoxX; ii++)
+        for (int jj = 0; jj < BoxY; jj++) out[ii * BoxY + jj] = ((float)rand() / RAND_MAX - 0.5) * wall_rough;
+}
+
+// The field part of a restart (ABM/CellMD3D.cpp:105-125) into the
+// environment's arrays: Environment and oldEnvironment from one stream of
+// <dir>/Environment/<index>.txt; every level reopens AgarField and WallField.
+int ref_env_field_restart(void* h, const char* dir, int index) {
+    RefEnv* e = (RefEnv*)h;
+    env_globals(e);
+    char env_name[600], agar_name[600], wal_name[600];
+    snprintf(env_name, sizeof env_name, "%s/Environment/%d.txt", dir, index);
+    snprintf(agar_name, sizeof agar_name, "%s/AgarField/%d.txt", dir, index);
+    snprintf(wal_name, sizeof wal_name, "%s/WallField/%d.txt", dir, index);
+    std::ifstream infile2(env_name);
+    if (!infile2) return -1;
+    e->sim->Env->Input(infile2);
+    e->oldEnv->Input(infile2);
+    for (int i = 0; i < maxLevelsMG; i++) {
+        std::ifstream infile(agar_name);
+        std::ifstream infile3(wal_name);
+        if (!infile || !infile3) return -1;
+        e->agar[i]->Input(infile);
+        e->agar_old[i]->Input(infile);
+        e->wall[i]->Input(infile3);
+        e->wall_old[i]->Input(infile3);
+    }
+    return 0;
+}
+
+// The AgarField and WallField files of the environment's state
+// (ABM/Simulation.cpp:222-226): every level appended to each.
+int ref_env_append(void* h, const char* agar_path, const char* wall_path) {
+    RefEnv* e = (RefEnv*)h;
+    env_globals(e);
+    FILE* fa = fopen(agar_path, "w");
+    FILE* fw = fopen(wall_path, "w");
+    if (!fa || !fw) return -1;
+    for (int level = 0; level < maxLevelsMG; level++) {
+        e->agar[level]->Append(fa, 1);
+        e->wall[level]->Append(fw);
+    }
+    fclose(fa);
+    fclose(fw);
+    return 0;
+}
+
+// The rough wall of the run (ABM/Simulation.cpp:160-168) into the simulation.
+void ref_sim_rough_wall(void* h) {
+    RefSim* s = (RefSim*)h;
+    set_globals(&s->prm);
+    for (int ii = 0; ii < BoxX; ii++)
+        for (int jj = 0; jj < BoxY; jj++) s->Wall->Set(ii, jj, ((float)rand() / RAND_MAX - 0.5) * wall_rough);
 }
 
 }  // extern "C"

@@ -3,10 +3,11 @@ normal, nutrients) and of a coupled run of the time loop, from the
 reference's own code compiled from source into oracle/_ref/libbcgref.so.
 
 Run in a container that has the reference checkout:
-    make -C oracle ref && python tests/golden/make_env_golden.py [large|output_loop]
+    make -C oracle ref && python tests/golden/make_env_golden.py [large|output_loop|driver]
 Output (committed): tests/golden/golden_env.npz, golden_env_large.npz (the
-reference's second shipped input) and golden_output_loop.npz (a time loop
-with output steps); "large" or "output_loop" writes only that one - inputs
+reference's second shipped input), golden_output_loop.npz (a time loop
+with output steps) and golden_driver.npz (a restart run of the executable's
+sequence, with its output files); an argument writes only that one - inputs
 (grid, times, seed) and the reference's outputs only. The grid is the demo input's with
 BoxX = 128, BoxZ = 16, BoxZAgar = 64 - every multigrid level keeps an exact
 2:1 ratio in x and at least two z points, so that the reference reads no
@@ -26,7 +27,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 from baccolonygrowth_amd.cells import CELL_DTYPE  # noqa: E402
 from baccolonygrowth_amd.colony import synthetic_fields  # noqa: E402
 from baccolonygrowth_amd.environment import env_params_from_file  # noqa: E402
-from helpers import DEMO_INPUT, GOLDEN, demo_params, restart_cells  # noqa: E402
+from helpers import DEMO_INPUT, GOLDEN, demo_params, driver_input, restart_cells  # noqa: E402
 from oracle.oracle import RefEnv, RefLib  # noqa: E402
 
 GRID = dict(BoxX=128, BoxZ=16)
@@ -297,10 +298,113 @@ def output_loop(ref):
     print(path, os.path.getsize(path), "bytes; output steps", out_steps, ";", len(lineage), "divisions")
 
 
+def driver_run(ref):
+    """A restart run as the reference's executable makes it, from the
+    reference's own functions: LoadCells and the field restart from files the
+    reference wrote (restart state 5 after one nutrient update), rand()
+    seeded, the rough wall, the initial pass, then the loop with its output
+    steps (cell files, saved cells) and lineage."""
+    import lzma
+    import tempfile
+
+    from baccolonygrowth_amd.cells import read_parameters
+    from baccolonygrowth_amd.environment import env_params_from_file
+
+    steps, seed = 25, 29
+    t0, dt, cells = restart_cells(5)
+    rng = np.random.default_rng(seed)
+    cells = cells.copy()
+    sel = rng.random(len(cells)) < 0.1
+    cells["Ldiv"][sel] = cells["Length"][sel] * (1.0 + rng.uniform(0.0, 5e-5, sel.sum()))
+    t_max = t0 + (steps - 0.5) * dt
+    out = {"seed": np.array(seed), "steps": np.array(steps), "t_max": np.array(t_max)}
+    with tempfile.TemporaryDirectory() as tmp:
+        rs = os.path.join(tmp, "rs")
+        for d in ("Restart", "Environment", "AgarField", "WallField"):
+            os.makedirs(os.path.join(rs, d))
+        inp = os.path.join(tmp, "input.txt")
+        driver_input(inp, rs, t_max)
+        p, _ = read_parameters(inp)
+        p.maxNeighbours = 96
+        ep = env_params_from_file(inp)
+        ref.save_cells(os.path.join(rs, "Restart", "5.txt"), cells, t0, dt)
+        # the field files of the restart: the reference's state after one update
+        sim0 = ref.sim(p)
+        sim0.set_cells(restart_cells(5)[2])
+        env0 = RefEnv(ref, sim0, ep)
+        env0.fields(0)
+        env0.nutrients()
+        env0.output(os.path.join(tmp, "unused_density.txt"), os.path.join(rs, "Environment", "5.txt"))
+        env0.append(os.path.join(rs, "AgarField", "5.txt"), os.path.join(rs, "WallField", "5.txt"))
+        env0.close()
+        sim0.close()
+        for d in ("Restart", "Environment", "AgarField", "WallField"):
+            raw = open(os.path.join(rs, d, "5.txt"), "rb").read()
+            out[f"rs_{d}"] = np.frombuffer(lzma.compress(raw), np.uint8)
+        # the run
+        sim = ref.sim(p)
+        ref.srand(seed)
+        t, dt2, n = ref.load_cells(sim, os.path.join(rs, "Restart", "5.txt"))
+        assert (t, dt2, n) == (t0, dt, len(cells))
+        env = RefEnv(ref, sim, ep)
+        env.field_restart(rs, 5)
+        ref.sim_rough_wall(sim)
+        env.fields(0)
+        env.nutrients()
+        update_time = ep.UpdateTime
+        upd, nxt_u, hd, nxt_h, outf, nxt_o = True, update_time, True, update_time / 10.0, False, 1e-3
+        lineage, out_steps = [], []
+        k = 0
+        while t <= t_max:
+            if upd:
+                env.fields(1)
+                env.nutrients()
+                sim.rebuild()
+            if hd:
+                env.fields(2)
+            if outf:
+                path = os.path.join(tmp, f"cells_{k}.txt")
+                sim.output_cells(path, t)
+                out[f"out{len(out_steps)}_cells_text"] = np.frombuffer(lzma.compress(open(path, "rb").read()), np.uint8)
+            lin, moved = ref.step_split(sim, dt, t, False)
+            if outf:
+                out[f"out{len(out_steps)}_moved"] = np.ascontiguousarray(moved).view(np.uint8)
+                out[f"out{len(out_steps)}_t"] = np.array(t)
+                out_steps.append(k)
+                # the output step's field files (the last step's are kept)
+                env.output(os.path.join(tmp, "d.txt"), os.path.join(tmp, "e.txt"))
+                env.append(os.path.join(tmp, "a.txt"), os.path.join(tmp, "w.txt"))
+                for name, f in (("Density", "d.txt"), ("Environment", "e.txt"), ("AgarField", "a.txt"),
+                                ("WallField", "w.txt")):
+                    out[f"last_{name}"] = np.frombuffer(lzma.compress(open(os.path.join(tmp, f), "rb").read()),
+                                                        np.uint8)
+            lineage += lin.tolist()
+            t += dt
+            k += 1
+            outf = nxt_o <= 0
+            nxt_o = (1e-3 if outf else nxt_o) - dt
+            upd = nxt_u <= 0
+            nxt_u = (update_time if upd else nxt_u) - dt
+            hd = nxt_h <= 0
+            nxt_h = (update_time / 10.0 if hd else nxt_h) - dt
+        env.close()
+        out["steps_run"] = np.array(k)
+        out["out_steps"] = np.array(out_steps)
+        out["cells"] = np.ascontiguousarray(sim.cells()).view(np.uint8)
+        out["lineage"] = np.array(lineage, dtype=np.int32).reshape(-1, 2)
+        out["next_draws"] = np.array([ref.rand() for _ in range(4)])
+        sim.close()
+    path = os.path.join(GOLDEN, "golden_driver.npz")
+    np.savez_compressed(path, **out)
+    print(path, os.path.getsize(path), "bytes; steps", k, "output steps", out_steps, ";", len(lineage), "divisions")
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["large"]:
         large(RefLib())
     elif sys.argv[1:] == ["output_loop"]:
         output_loop(RefLib())
+    elif sys.argv[1:] == ["driver"]:
+        driver_run(RefLib())
     else:
         main()

@@ -43,3 +43,24 @@ def planar_case(k: int, seed: int, divide_frac: float = 0.05):
 def sorted_lists(counts, lists):
     """Neighbour lists as sorted tuples (set comparison independent of order)."""
     return [tuple(sorted(lists[i, : counts[i]].tolist())) for i in range(len(counts))]
+
+
+# the restart run of the driver test: the demo input on the reduced grid,
+# update and output every 0.001 h, restart index 5
+DRIVER_OVERRIDES = {"Box_x": "128", "Box_z": "16", "Box_z_agar": "64", "MaxCells": "4000", "Update_Time": "0.001",
+                    "Output_Time": "0.001", "Restart": "1", "RestartIndex": "5"}
+
+
+def driver_input(path: str, restart_dir: str, t_max: float) -> None:
+    """The demo input with the driver run's grid, times and restart lines."""
+    over = dict(DRIVER_OVERRIDES, Restart_dir=restart_dir, t_max=repr(t_max))
+    out = []
+    with open(DEMO_INPUT) as fh:
+        for line in fh:
+            parts = line.split()
+            if parts and parts[0] in over:
+                line = f"{parts[0]}\t{over.pop(parts[0])}\n"
+            out.append(line)
+    out += [f"{k}\t{v}\n" for k, v in over.items()]
+    with open(path, "w") as fh:
+        fh.writelines(out)

@@ -261,6 +261,42 @@ def test_out_of_domain_is_an_error(Colony):
         assert e.value.code == -3
 
 
+def test_failed_step_is_sticky_until_set_cells(Colony):
+    """A step that fails after its move has run (a cell pushed out of the
+    bin grid: 512 um is the +x edge of the demo grid) leaves the context
+    failed: every later call that uses its cells reports BCG_ERR_STATE
+    until bcg_set_cells, after which it steps normally."""
+    from baccolonygrowth_amd._lib import BcgError
+
+    p = demo_params()
+    p.maxCells = 16
+    edge = p.BoxX / 2 * p.BoxLength
+    c = np.zeros(2, CELL_DTYPE)
+    for i, cx in enumerate((edge - 0.05, edge - 0.75)):  # overlapping end to end: A is pushed across the edge
+        c["p"][i] = (cx + 0.5, 0, 0.5)
+        c["q"][i] = (cx - 0.5, 0, 0.5)
+    c["Length"], c["Radius"], c["Ldiv"], c["Type"], c["Ancestor"] = 1.0, 0.5, 10.0, 1, 1
+    nh = p.BoxX * p.refinementGridHeight
+    normal = np.tile([0.0, 0.0, -1.0], nh)
+    with Colony(p) as col:
+        col.LoadCells(c)
+        col.set_fields(height=np.full(nh, 0.5), normal=normal, wall=np.zeros(p.BoxX))
+        with pytest.raises(BcgError) as e:
+            col.step(1e-4, 0.0, update=True)
+        assert e.value.code == -3
+        for call in (col.cells, lambda: col.step(1e-4, 0.0, update=True), col.getNeighbours, col.mean_stress,
+                     col.sum_forces):
+            with pytest.raises(BcgError) as e:
+                call()
+            assert e.value.code == -7
+        safe = c.copy()
+        safe["p"][:, 0] -= 100.0
+        safe["q"][:, 0] -= 100.0
+        col.LoadCells(safe)
+        col.step(1e-4, 0.0, update=True)
+        assert col.n == 2 and np.all(np.isfinite(col.cells()["p"]))
+
+
 def test_neighbour_overflow_is_an_error(Colony):
     from baccolonygrowth_amd._lib import BcgError
 

@@ -305,6 +305,33 @@ def test_adjoint_matches_oracle(mode):
 
 
 @pytest.mark.gpu
+def test_failed_adjoint_keeps_the_forward_state():
+    """An adjoint solve that cannot converge (one iteration to 1e-14) fails
+    and leaves the forward state as it was: the field is still c_new and the
+    next step equals the one a fresh solver makes from c_new."""
+    from baccolonygrowth_amd import adjoint as adj
+    from baccolonygrowth_amd._lib import BcgError
+
+    shape = (9, 17, 17)
+    h, D, dt, K = 1.0, 20.0, 10.0, 0.1
+    c_old, q, target = _loss_parts(shape, seed=5)
+    print(f"[gpu] failed adjoint {shape}", flush=True)
+    with _solver(shape, h, D, dt, K) as a, _solver(shape, h, D, dt, K) as b:
+        a.set_field(c_old)
+        a.set_uptake(q)
+        a.step("fp64", tol=1e-12, max_iter=200)
+        c1 = a.field()
+        with pytest.raises(BcgError):
+            adj.adjoint(a, 2 * (c1 - target), "fp64", tol=1e-14, max_iter=1)
+        assert a.field().tobytes() == c1.tobytes()
+        a.step("fp64", tol=1e-12, max_iter=200)
+        b.set_field(c1)
+        b.set_uptake(q)
+        b.step("fp64", tol=1e-12, max_iter=200)
+        assert a.field().tobytes() == b.field().tobytes()
+
+
+@pytest.mark.gpu
 def test_mixed_adjoint_after_fp64_step_rebuilds_fp32_operator():
     """An FP64 step's set-up writes no FP32 copies of a0; a mixed-precision
     adjoint after it makes them and gives the gradients of a set-up that
