@@ -135,12 +135,15 @@ struct PhysParams {
     int BoxX, BoxY, BoxZ, ref, nhx, nhy, planar, crossFeeding;
 };
 
-// UniformGrid::GetAddress (ABM/UniformGrid.h:87-100); -1 when outside.
+// UniformGrid::GetAddress (ABM/UniformGrid.h:87-100); -1 when outside. A
+// NaN coordinate is outside too: x86-64 converts it to INT_MIN (outside),
+// the GPU's conversion to 0, which would pass it as box 0.
 __device__ __forceinline__ int bin_of(const PhysParams& P, d3 pos, int& x, int& y, int& z) {
     x = (int)(pos.x / P.BoxLength + P.BoxX / 2);
     y = (int)(pos.y / P.BoxLength + P.BoxY / 2 + 0.5);
     z = (int)(pos.z / P.BoxLength);
-    if (!((x < P.BoxX) & (y < P.BoxY) & (z < P.BoxZ) & (x >= 0) & (y >= 0) & (z >= 0))) return -1;
+    const bool nan = (pos.x != pos.x) | (pos.y != pos.y) | (pos.z != pos.z);
+    if (nan | !((x < P.BoxX) & (y < P.BoxY) & (z < P.BoxZ) & (x >= 0) & (y >= 0) & (z >= 0))) return -1;
     return x * (P.BoxY * P.BoxZ) + y * P.BoxZ + z;
 }
 

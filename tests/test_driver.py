@@ -85,9 +85,10 @@ def _numbers(raw: bytes):
     return [np.array([float(v) for v in ln.split()]) for ln in lines]
 
 
-def _same_text(got: bytes, exp: bytes, rtol: float, atol: float, label: str):
+def _same_text(got: bytes, exp: bytes, rtol: float, atol: float, label: str, skip=None):
     """Same lines and tokens; numbers equal to the printed precision (a
-    last-digit difference allowed where the values differ at round-off)."""
+    last-digit difference allowed where the values differ at round-off).
+    skip(line) gives a slice of a line's values not compared."""
     if got == exp:
         return 1.0
     a, b = _numbers(got), _numbers(exp)
@@ -95,9 +96,19 @@ def _same_text(got: bytes, exp: bytes, rtol: float, atol: float, label: str):
     same = 0
     for i, (x, y) in enumerate(zip(a, b)):
         assert x.shape == y.shape, (label, i)
-        assert np.allclose(x, y, rtol=rtol, atol=atol), (label, i, np.abs(x - y).max())
+        keep = np.ones(len(x), bool)
+        if skip is not None and skip(i) is not None:
+            keep[skip(i)] = False
+        assert np.allclose(x[keep], y[keep], rtol=rtol, atol=atol), (label, i, np.abs(x - y)[keep].max())
         same += np.array_equal(x, y)
     return same / len(a)
+
+
+def _wall_skip(bx: int):
+    """WallField rows of levels >= 1 (four rows a level when BoxY = 1): the
+    x range that the finer level covers holds a term the reference leaves
+    uninitialised (its value is overwritten in every outer iteration)."""
+    return lambda row: slice(bx // 4, bx * 3 // 4) if row >= 4 else None
 
 
 @pytest.mark.gpu
@@ -132,9 +143,10 @@ def test_driver_restart_run_matches_reference(tmp_path):
                           lzma.decompress(g[f"out{k}_cells_text"].tobytes()), 1e-4, 1.01e-4, f"Cells/{k}")
         assert frac >= 0.98, frac
     last = n_out - 1
+    bx = read_parameters(str(inp))[0].BoxX
     for name in ("Density", "Environment", "AgarField", "WallField"):
         _same_text((out / name / f"{last}.txt").read_bytes(), lzma.decompress(g[f"last_{name}"].tobytes()),
-                   1.5e-6, 1e-300, f"{name}/{last}")
+                   1.5e-6, 1e-300, f"{name}/{last}", _wall_skip(bx) if name == "WallField" else None)
     got = s["final_cells"]
     exp = np.ascontiguousarray(g["cells"]).view(CELL_DTYPE).reshape(-1)
     assert len(got) == len(exp)

@@ -262,17 +262,18 @@ def test_out_of_domain_is_an_error(Colony):
 
 
 def test_failed_step_is_sticky_until_set_cells(Colony):
-    """A step that fails after its move has run (a cell pushed out of the
-    bin grid: 512 um is the +x edge of the demo grid) leaves the context
-    failed: every later call that uses its cells reports BCG_ERR_STATE
-    until bcg_set_cells, after which it steps normally."""
+    """A step that fails after its move has run (a cell pushed across the
+    +x edge of the demo grid, 512 um, by a neighbour 1.2 um behind it: about
+    2.7e-4 um in one step) leaves the context failed: every later call that
+    uses its cells reports BCG_ERR_STATE until bcg_set_cells, after which it
+    steps normally."""
     from baccolonygrowth_amd._lib import BcgError
 
     p = demo_params()
     p.maxCells = 16
     edge = p.BoxX / 2 * p.BoxLength
     c = np.zeros(2, CELL_DTYPE)
-    for i, cx in enumerate((edge - 0.05, edge - 0.75)):  # overlapping end to end: A is pushed across the edge
+    for i, cx in enumerate((edge - 1e-5, edge - 1e-5 - 1.2)):
         c["p"][i] = (cx + 0.5, 0, 0.5)
         c["q"][i] = (cx - 0.5, 0, 0.5)
     c["Length"], c["Radius"], c["Ldiv"], c["Type"], c["Ancestor"] = 1.0, 0.5, 10.0, 1, 1
