@@ -5,6 +5,7 @@ first call's contents, so whatever must start at zero has to be cleared by
 the call itself). The test's device buffers come from the HIP runtime
 (hipMalloc through ctypes): the test allocates, the entry point does not."""
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -32,7 +33,11 @@ class _Hip:
     H2D, D2H = 1, 2
 
     def __init__(self):
-        self.L = ctypes.CDLL("libamdhip64.so")
+        # the runtime libbcg.so is bound to (by the soname it names, after
+        # loading it): a bare "libamdhip64.so" can resolve to another copy
+        # that a library loaded earlier in the process brought in (torch's)
+        lib()
+        self.L = ctypes.CDLL("libamdhip64.so.7", mode=getattr(os, "RTLD_NOLOAD", 4) | ctypes.RTLD_GLOBAL)
         self.L.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
         self.L.hipFree.argtypes = [ctypes.c_void_p]
         self.L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
