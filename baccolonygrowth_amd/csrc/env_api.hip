@@ -179,6 +179,7 @@ struct Phased {
     bool wg_sweeps;
     size_t wg_lds;
     size_t coarse_lds;  // dynamic LDS of the tail's coarse solve (0: the one-wave or global path)
+    int coarse_mode;    // 2: a wave per species, 1: the workgroup, 0: global
     dim3 pts(long n) const {
         return dim3((unsigned)std::max(1L, std::min((long)PH_MAX_BLOCKS, (n + PH_THREADS - 1) / PH_THREADS)));
     }
@@ -207,8 +208,7 @@ struct Phased {
                 hipLaunchKernelGGL(k_nut_restrict, pts(level_points(l + 1)), T, 0, st, da, l + 1);
                 hipLaunchKernelGGL(k_nut_rterm, pts(3 * level_points(l + 1)), T, 0, st, da, l + 1);
             }
-            hipLaunchKernelGGL(k_nut_vcycle_tail, dim3(1), dim3(ENV_THREADS), coarse_lds, st, a, lc,
-                               coarse_lds ? 1 : 0);
+            hipLaunchKernelGGL(k_nut_vcycle_tail, dim3(1), dim3(ENV_THREADS), coarse_lds, st, a, lc, coarse_mode);
             for (int l = lc - 1; l >= 0; l--) {
                 hipLaunchKernelGGL(k_nut_interp, pts(level_points(l)), T, 0, st, da, l);
                 relax3(l);
@@ -235,8 +235,11 @@ struct Phased {
 int nutrients_phased(bcg_env* e, const EnvArgs& a) {
     hipStream_t st = e->mech->stream;
     // first narrow level: it and every coarser one fit one workgroup a colour
+    // and are small enough for the workgroup's transfers
     int lc = MG_LEVELS - 1;
-    while (lc > 0 && 3L * (level_nx(e, lc - 1) / 2 + 1) <= ENV_THREADS) lc--;
+    while (lc > 0 && 3L * (level_nx(e, lc - 1) / 2 + 1) <= ENV_THREADS &&
+           (long)(level_nx(e, lc - 1) + 1) * (e->mgz[lc - 1] + 1) <= TAIL_MAX_POINTS)
+        lc--;
     // sweeps in one workgroup when one round of it covers a sweep's boxes and
     // rows (or when asked); device-wide launches otherwise
     const long items = (long)(e->hbounds[1] - e->hbounds[0] + 1) * (e->hbounds[4] + 1) + (long)e->ML * e->BX;
@@ -252,20 +255,24 @@ int nutrients_phased(bcg_env* e, const EnvArgs& a) {
     } else {
         aa.lds_colony = 0;
     }
-    // the coarsest level in LDS for the tail's coarse solve when the one-wave
-    // path (at most 64 points and 64 column walks) cannot take it
+    // the coarsest level in LDS for the tail's coarse solve: one wave per
+    // species when its columns fit a wave, else the whole workgroup; the
+    // one-wave global path when neither fits
     size_t coarse_lds = 0;
+    int coarse_mode = 0;
     {
         const int l = MG_LEVELS - 1;
         const long npts = (long)(e->mgx[l] + 1) * (e->mgz[l] + 1);
         const int nx = level_nx(e, l);
-        const bool wave = npts <= COARSE_WAVE_POINTS && 3 * (nx / 2 + 1) <= 64;
         const size_t need = (size_t)3 * 3 * npts * sizeof(double);
-        if (!wave && need <= (size_t)COARSE_LDS_BYTES) coarse_lds = need;
+        if (need <= (size_t)COARSE_LDS_BYTES) {
+            coarse_lds = need;
+            coarse_mode = nx + 1 <= 64 ? 2 : 1;
+        }
     }
     *e->hargs = aa;
     EHIP(e, hipMemcpyAsync(e->dargs, e->hargs, sizeof(EnvArgs), hipMemcpyHostToDevice, st));
-    const Phased P{e, aa, e->dargs, st, lc, wg, lds, coarse_lds};
+    const Phased P{e, aa, e->dargs, st, lc, wg, lds, coarse_lds, coarse_mode};
     hipLaunchKernelGGL(k_nut_begin, P.pts(std::max((long)e->BX * e->BZ, (long)e->ML * e->BX)), dim3(PH_THREADS), 0, st,
                        P.da);
     const int n = aa.n_outer;

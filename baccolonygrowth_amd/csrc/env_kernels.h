@@ -119,10 +119,16 @@ __global__ void __launch_bounds__(PH_RELAX_THREADS) k_nut_relax(const EnvArgs* a
 __global__ void __launch_bounds__(PH_THREADS) k_nut_restrict(const EnvArgs* a, int l);
 __global__ void __launch_bounds__(PH_THREADS) k_nut_interp(const EnvArgs* a, int l);
 __global__ void __launch_bounds__(PH_THREADS) k_nut_rterm(const EnvArgs* a, int l);
-__global__ void __launch_bounds__(ENV_THREADS) k_nut_vcycle_tail(EnvArgs a, int lc, int coarse_lds);
+// coarse_mode: 2 coarsest level a wave per species, 1 the workgroup (both out
+// of dynamic LDS), 0 the one-wave or global path
+__global__ void __launch_bounds__(ENV_THREADS) k_nut_vcycle_tail(EnvArgs a, int lc, int coarse_mode);
+// the V-cycle tail (one workgroup) starts at the first level whose column
+// walks of a colour fit the workgroup and whose points are at most this many:
+// on wider levels its restriction and interpolation, a few points a thread,
+// wait on memory with only the workgroup's waves to hide it
+constexpr long TAIL_MAX_POINTS = 4096;
 // the coarsest level in LDS for the tail's coarse solve (E, R and RT), up to this size
 constexpr int COARSE_LDS_BYTES = 60 * 1024;
-constexpr int COARSE_WAVE_POINTS = 64;  // ... unless the one-wave path takes it
 __global__ void __launch_bounds__(PH_THREADS) k_nut_rewrite(const EnvArgs* a, int l);
 __global__ void __launch_bounds__(PH_THREADS) k_nut_nest(const EnvArgs* a, int l);
 __global__ void __launch_bounds__(PH_THREADS) k_nut_copy_from_mg(const EnvArgs* a);

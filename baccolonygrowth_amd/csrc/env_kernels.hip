@@ -1254,6 +1254,54 @@ __device__ __forceinline__ void coarse_solve_lds(const EnvArgs& a, const MG m, i
     __syncthreads();
 }
 
+// The 100 relaxations of the coarsest level with one wave per species (the
+// three error equations do not couple) and one lane per column, out of LDS
+// (lds: E, R, RT of the three species, 3 x 3 x npts doubles): the colour
+// passes of a species need only its own wave's synchronisation, and each
+// lane's column constants are formed once. Requires nx + 1 <= 64.
+__device__ __forceinline__ void coarse_solve_waves(const EnvArgs& a, const MG m, int tid, double* lds) {
+    const int l = MG_LEVELS - 1;
+    const Geo g = m.geo(l, 1);
+    const int X1 = a.mgx[l] + 1, npts = X1 * (a.mgz[l] + 1);
+    double* cs_e = lds;
+    double* cs_r = lds + 3 * npts;
+    double* cs_t = lds + 6 * npts;
+    gdouble* E = G(a.err[l]);
+    const gdouble* R = G(a.res[l]);
+    const gdouble* T = G(a.rt[l]);
+    for (int f = tid; f < 3 * npts; f += ENV_THREADS) {
+        cs_e[f] = E[f];
+        cs_r[f] = R[f];
+        cs_t[f] = T[f];
+    }
+    __syncthreads();
+    const int sp = tid / 64, i = tid - 64 * sp;
+    if (sp < 3) {
+        const bool col = i <= g.nx;
+        const bool outer = l >= m.AL - 1;
+        const int d = 1 << l;
+        const int i0 = min(max((i - g.nx / 2) * d + a.BX / 2, 0), a.BX - 1);
+        const bool dir0 = col && G(a.insT)[i0] > 0;
+        const double nn = level_h2(a, l) / m.dt / species_D(a, sp) + 2;
+        const Col<double> ce{cs_e + sp * npts, X1}, cr{cs_r + sp * npts, X1}, ct{cs_t + sp * npts, X1};
+        for (int j = 0; j < 100; j++) {
+            for (int par = 0; par < 2; par++) {
+                // a one-row level takes no ring of loads ahead
+                if (col && (i & 1) == par) {
+                    if (g.nz == 0)
+                        relax_column<1>(ce, cr, ct, i, g.nx, g.nz, sp == 1, nn, outer, dir0);
+                    else
+                        relax_column<RING_WG>(ce, cr, ct, i, g.nx, g.nz, sp == 1, nn, outer, dir0);
+                }
+                wave_sync();
+            }
+        }
+    }
+    __syncthreads();
+    for (int f = tid; f < 3 * npts; f += ENV_THREADS) E[f] = cs_e[f];
+    __syncthreads();
+}
+
 // Every point of a level, x fastest across the lanes (the passes that use it
 // write each point from values no other point of the pass writes).
 template <class F>
@@ -1748,10 +1796,10 @@ __global__ void __launch_bounds__(PH_THREADS) k_nut_rterm(const EnvArgs* __restr
 
 // The V-cycle from level lc down and back up to it, in one workgroup (the
 // levels below lc are narrow: a barrier costs less than a launch there).
-__global__ void __launch_bounds__(ENV_THREADS) k_nut_vcycle_tail(EnvArgs args, int lc, int coarse_lds) {
+__global__ void __launch_bounds__(ENV_THREADS) k_nut_vcycle_tail(EnvArgs args, int lc, int coarse_mode) {
     __shared__ EnvArgs a;
     __shared__ double cs_e[3 * COARSE_MAX], cs_r[3 * COARSE_MAX], cs_t[3 * COARSE_MAX];
-    extern __shared__ double lds[];  // the coarsest level, when coarse_lds
+    extern __shared__ double lds[];  // the coarsest level, when coarse_mode > 0
     const int tid = threadIdx.x;
     if (tid == 0) a = args;
     __syncthreads();
@@ -1764,7 +1812,9 @@ __global__ void __launch_bounds__(ENV_THREADS) k_nut_vcycle_tail(EnvArgs args, i
         rterm_level(a, m, l + 1, tid, ENV_THREADS);
         __syncthreads();
     }
-    if (coarse_lds)
+    if (coarse_mode == 2)
+        coarse_solve_waves(a, m, tid, lds);
+    else if (coarse_mode == 1)
         coarse_solve_lds(a, m, tid, lds);
     else
         coarse_solve(a, m, tid, cs_e, cs_r, cs_t);
