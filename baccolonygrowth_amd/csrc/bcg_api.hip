@@ -113,10 +113,13 @@ int build_bins(bcg_ctx* c, int buf, int n) {
 }
 
 int gather_all(bcg_ctx* c) {
-    if (c->N > 0)
-        hipLaunchKernelGGL(k_gather_all, dim3(nblk(c->N, 256)), dim3(256), 0, c->stream, c->hot[c->cur], c->N, c->P,
-                           c->bin_start, c->sorted, c->cand, c->nl, c->nl_count, c->stride, c->cap, c->order,
-                           c->slot_of, &c->stats->maxlist, &c->stats->err);
+    if (c->N > 0) {
+        const int nb = nblk(c->N, 256);
+        hipLaunchKernelGGL(k_gather_all, dim3(nb), dim3(256), 0, c->stream, c->hot[c->cur], c->N, c->P, c->bin_start,
+                           c->sorted, c->cand, c->nl, c->nl_count, c->stride, c->cap, c->order, c->slot_of,
+                           c->part_max, &c->stats->err);
+        hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, c->stream, c->part_max, nb, &c->stats->maxlist);
+    }
     HIPCHK(c, hipGetLastError());
     c->lists_valid = true;
     return 0;
@@ -206,6 +209,8 @@ int bcg_create(const bcg_params* p, bcg_ctx** out) {
         return BCG_ERR_ARG;
     if (p->var_pos != 0.0) return BCG_ERR_UNSUPPORTED;  // F_cw draw values would enter the forces
     if ((long)p->BoxX * p->BoxY * p->BoxZ > (1L << 30)) return BCG_ERR_ARG;
+    // the list build's queue entries hold bin-order positions in QK_BITS bits
+    if (p->maxCells > QK_MASK + 1) return BCG_ERR_ARG;
     bcg_ctx* c = new bcg_ctx();
     c->prm = *p;
     c->P = phys(*p);
@@ -238,7 +243,7 @@ int bcg_create(const bcg_params* p, bcg_ctx** out) {
     A(&c->sorted, M);
     A(&c->scattered, M);
     A(&c->cmL, 2 * (size_t)M);   // FP32 end points p, q per cell
-    A(&c->cand, 2 * (size_t)M);
+    A(&c->cand, 2 * ((size_t)M + WALK_GROUP));  // + the walk's read-ahead
     A(&c->scan_tot, 1024 * 64);
     A(&c->nl, (size_t)c->cap * c->stride);
     A(&c->nl_count, M);
@@ -258,6 +263,8 @@ int bcg_create(const bcg_params* p, bcg_ctx** out) {
     A(&c->gr2, c->nbin);
     A(&c->Fbuf, 3 * M);
     A(&c->Tbuf, 3 * M);
+    A(&c->part_max, (size_t)nblk(M, 256));
+    A(&c->part_draws, (size_t)nblk(M, std::min(MOVE_BLOCK, 64)));  // per block or per wave
     A(&c->Sbuf, 5 * M);
     A(&c->stats, 1);
     if (!rc && hipHostMalloc((void**)&c->hstats, sizeof(BcgStats), hipHostMallocDefault) != hipSuccess)
@@ -301,7 +308,8 @@ int bcg_destroy(bcg_ctx* c) {
                     c->age_q,    c->type,      c->ancestor, c->key,       c->bin_cnt,   c->bin_start, c->bin_slot,
                     c->sorted,   c->scattered, c->scan_tot, c->cmL, c->cand,  c->nl,      c->nl_count,   c->order,     c->slot_of,  c->divflag,
                     c->div_ids,  c->div_index_of, c->ids_tmp, c->draws,   c->mother_pre, c->height,  c->normal,
-                    c->wall,     c->gr1,       c->gr2,     c->Fbuf,       c->Tbuf,      c->Sbuf,      c->stats};
+                    c->wall,     c->gr1,       c->gr2,     c->Fbuf,       c->Tbuf,      c->Sbuf,      c->stats,
+                    c->part_max, c->part_draws};
     for (void* q : ptrs)
         if (q) hipFree(q);
     if (c->hstats) hipHostFree(c->hstats);
@@ -571,10 +579,14 @@ int bcg_mean_stress(bcg_ctx* c, double* stress, double* fnet) {
     if (!c->lists_valid && c->N > 0) return fail(c, BCG_ERR_STATE, "no neighbour lists: rebuild first");
     int rc;
     if ((rc = reset_stats(c))) return rc;
-    if (c->N > 0)
+    if (c->N > 0) {
+        const int nw = nblk(c->N, 64);
+        HIPCHK(c, hipMemsetAsync(c->part_draws, 0, sizeof(unsigned) * nw, c->stream));
         hipLaunchKernelGGL(k_mean_stress, dim3(nblk(c->N, MOVE_BLOCK)), dim3(MOVE_BLOCK), 0, c->stream,
                            c->hot[c->cur], c->N, c->P, c->order, c->nl, c->nl_count, c->stride, c->height, c->normal,
-                           c->wall, c->Sbuf, c->Fbuf, &c->stats->wall_draws, &c->stats->err);
+                           c->wall, c->Sbuf, c->Fbuf, c->part_draws, &c->stats->err);
+        hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(1024), 0, c->stream, c->part_draws, nw, &c->stats->wall_draws);
+    }
     HIPCHK(c, hipGetLastError());
     if ((rc = read_stats(c))) return rc;
     if ((rc = check_err(c))) return rc;
@@ -618,11 +630,14 @@ int step_body(bcg_ctx* c, double dt, double t, int32_t flags, int32_t* n_div, in
     HIPCHK(c, hipEventRecord(c->ev[6], c->stream));
     *started = true;
     // MoveCell + GrowCell for every cell (ABM/Simulation.cpp:330-335)
-    if (N0 > 0)
-        hipLaunchKernelGGL(k_move_grow, dim3(nblk(N0, MOVE_BLOCK)), dim3(MOVE_BLOCK), 0, c->stream, c->hot[in],
-                           c->hot[out], N0, c->P, dt, c->order, c->nl, c->nl_count, c->stride, c->height, c->normal,
-                           c->wall, c->gr1, c->gr2, c->type, c->dynfric, c->stafric, c->divflag,
-                           &c->stats->wall_draws, &c->stats->err);
+    if (N0 > 0) {
+        const int nb = nblk(N0, MOVE_BLOCK);
+        hipLaunchKernelGGL(k_move_grow, dim3(nb), dim3(MOVE_BLOCK), 0, c->stream, c->hot[in], c->hot[out], N0, c->P,
+                           dt, c->order, c->nl, c->nl_count, c->stride, c->height, c->normal, c->wall, c->gr1, c->gr2,
+                           c->type, c->dynfric, c->stafric, c->divflag, c->part_draws, &c->stats->err);
+        hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(1024), 0, c->stream, c->part_draws, nb,
+                           &c->stats->wall_draws);
+    }
     HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
     if (N0 > 0) {
         const int nb = nblk(N0, 256);

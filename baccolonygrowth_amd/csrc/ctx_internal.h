@@ -38,13 +38,15 @@ struct bcg_ctx {
     int *key = nullptr, *bin_cnt = nullptr, *bin_start = nullptr, *bin_slot = nullptr, *sorted = nullptr;
     int* scattered = nullptr;  // bin order before the in-bin ranking
     int* scan_tot = nullptr;
-    float4 *cmL = nullptr, *cand = nullptr;
+    float4 *cmL = nullptr, *cand = nullptr;  // FP32 end points by ID, by bin-order slot
     int *nl = nullptr, *nl_count = nullptr, *order = nullptr, *slot_of = nullptr;
     int *divflag = nullptr, *div_ids = nullptr, *div_index_of = nullptr, *ids_tmp = nullptr;
     double *draws = nullptr, *mother_pre = nullptr;
     double *height = nullptr, *normal = nullptr, *wall = nullptr, *gr1 = nullptr, *gr2 = nullptr;
     double *Fbuf = nullptr, *Tbuf = nullptr;
     double* Sbuf = nullptr;  // mean_stress output, 5 per cell
+    int* part_max = nullptr;        // per block of the list build: its longest list
+    unsigned* part_draws = nullptr; // per block of move/grow: its F_cw draws
     BcgStats* stats = nullptr;  // device
     BcgStats* hstats = nullptr; // pinned host
     int* hids = nullptr;         // pinned host: the step's dividing mothers (lineage)

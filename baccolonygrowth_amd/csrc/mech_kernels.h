@@ -12,6 +12,15 @@ constexpr int ERRBIT_NLIST = 2;   // a reduced list longer than its capacity
 constexpr int MOVE_BLOCK = 64;
 constexpr int PATCH_BLOCK = 256;
 constexpr int SCAN_BLOCK_ITEMS = 2048;  // items per block of the scan kernels
+// The list build reads candidate records in groups of this many from any
+// position below the cell count: the candidate array holds as many records
+// beyond its last cell (never used for a decision).
+constexpr int WALK_GROUP = 32;
+// Queue entries of the list build: the candidate's position in bin order (28
+// bits, so at most 2^28 cells) and its bin column relative to the owner's,
+// (dx + 1) * 3 + (dy + 1).
+constexpr int QK_BITS = 28;
+constexpr int QK_MASK = (1 << QK_BITS) - 1;
 
 __global__ void k_scan_reduce(const int* in, int n, int* tot);
 __global__ void k_scan_totals(int* tot, int nb, int* grand);
@@ -24,8 +33,10 @@ __global__ void k_bin_rank(const int* scattered, int n, const int* key, const in
                            int* sorted, float4* cand);
 
 __global__ void k_gather_all(const double* hot, int n, PhysParams P, const int* start, const int* sorted,
-                             const float4* cand, int* nl, int* nl_count, int stride, int cap, int* order, int* slot_of,
-                             int* maxlist, int* err);
+                             const float4* cand, int* nl, int* nl_count, int stride, int cap, int* order,
+                             int* slot_of, int* block_max, int* err);
+__global__ void k_reduce_max(const int* part, int n, int* out);
+__global__ void k_reduce_sum(const unsigned* part, int n, unsigned long long* out);
 __global__ void k_gather_ids(const double* hot, const int* ids, int nid, PhysParams P, const int* start,
                              const int* sorted, const float4* cand, int* nl, int* nl_count, int stride, int cap,
                              const int* slot_of, int* maxlist, int* err);
@@ -33,13 +44,13 @@ __global__ void k_gather_ids(const double* hot, const int* ids, int nid, PhysPar
 __global__ void k_move_grow(const double* hot_old, double* hot_new, int n, PhysParams P, double dt, const int* order,
                             const int* nl, const int* nl_count, int stride, const double* H, const double* Nrm,
                             const double* wall, const double* gr1, const double* gr2, const int* type,
-                            double* dynfric, double* stafric, int* divflag, unsigned long long* wall_draws, int* err);
+                            double* dynfric, double* stafric, int* divflag, unsigned* block_draws, int* err);
 __global__ void k_sum_forces(const double* hot, int n, PhysParams P, const int* order, const int* nl,
                              const int* nl_count, int stride, const double* H, const double* Nrm, const double* wall,
                              double* Fo, double* To, int* err);
 __global__ void k_mean_stress(const double* hot, int n, PhysParams P, const int* order, const int* nl,
                               const int* nl_count, int stride, const double* H, const double* Nrm, const double* wall,
-                              double* S5, double* F3, unsigned long long* wall_draws, int* err);
+                              double* S5, double* F3, unsigned* wave_draws, int* err);
 
 __global__ void k_flag_count(const int* flag, int n, int* tot);
 __global__ void k_flag_write(const int* flag, int n, const int* off, int* ids);

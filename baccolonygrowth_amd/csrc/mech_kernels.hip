@@ -12,6 +12,7 @@
 //   patch     the reference's post-division list patch, replayed in
 //             division order by one workgroup
 #include "mech_kernels.h"
+#include "pair_class.h"
 
 #include <climits>
 
@@ -132,9 +133,10 @@ __global__ void k_bin_count(const double* __restrict__ hot, int n, PhysParams P,
         a = x * (P.BoxY * P.BoxZ) + y * P.BoxZ + z;
     }
     key[i] = a;
-    // FP32 end points for the candidate prefilter of the list build
-    cmL[2 * i] = local_point(s.p, col_ox(P, x), col_oy(P, y));
-    cmL[2 * i + 1] = local_point(s.q, col_ox(P, x), col_oy(P, y));
+    // FP32 end points for the list build (its walk and pair classification)
+    const double ox = col_ox(P, x), oy = col_oy(P, y);
+    cmL[2 * i] = local_point(s.p, ox, oy);
+    cmL[2 * i + 1] = local_point(s.q, ox, oy);
     slot[i] = atomicAdd(&cnt[a], 1);  // arrival order in the bin
 }
 
@@ -148,7 +150,7 @@ __global__ void k_bin_place(const int* __restrict__ key, const int* __restrict__
 // Ascending IDs inside each bin: the cell at position p of the arrival order
 // goes to its bin's start plus the number of cells of the bin with a smaller
 // ID (bins hold tens of cells; a thread per cell), with its candidate record
-// (the FP32 end points of the prefilter).
+// (the FP32 end points).
 __global__ void k_bin_rank(const int* __restrict__ scattered, int n, const int* __restrict__ key,
                            const int* __restrict__ start, const float4* __restrict__ cmL, int* __restrict__ sorted,
                            float4* __restrict__ cand) {
@@ -226,10 +228,12 @@ struct Pre {
 };
 struct PreCol {
     float m2x, m2y, cz, thr;
+    // (each FMA takes at most one operand from the record, which the walk
+    // holds in scalar registers)
     __device__ bool near(const float4 p, const float4 q) const {
         const float dpz = p.z - cz, dqz = q.z - cz;
-        const float sp = fmaf(dpz, dpz, fmaf(p.y, m2y, fmaf(p.x, m2x, p.w)));
-        const float sq = fmaf(dqz, dqz, fmaf(q.y, m2y, fmaf(q.x, m2x, q.w)));
+        const float sp = fmaf(p.y, m2y, fmaf(p.x, m2x, fmaf(dpz, dpz, p.w)));
+        const float sq = fmaf(q.y, m2y, fmaf(q.x, m2x, fmaf(dqz, dqz, q.w)));
         return fminf(sp, sq) <= thr;
     }
 };
@@ -252,6 +256,13 @@ __device__ __forceinline__ PreCol pre_col(const PhysParams& P, const Pre& f, int
     c.thr = f.r * f.r * 1.0001f - (cx * cx + cy * cy) + 1e-6f * (e * e);
     return c;
 }
+
+
+// v_rcp_f32: the approximate reciprocal of the FP32 pair classification.
+struct DevRcp {
+    __device__ float operator()(float x) const { return __builtin_amdgcn_rcpf(x); }
+};
+
 
 __device__ __forceinline__ int gather_one(const PhysParams& P, const double* __restrict__ hot, int id, int slot,
                                           const int* __restrict__ start, const int* __restrict__ sorted,
@@ -320,34 +331,38 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
-// The list build of the cells of one wave, when they are consecutive in bin
-// order and share one bin x: the wave walks the union of their 27-bin blocks
-// once, (x, y) column by column, staging 64 candidates at a time in LDS (one
-// coalesced load; every lane then reads the same entry). Each lane queues the
-// candidates that lie in its own bins of the column (the position range
-// [klo, khi): its bins z0-1..z0+1, empty when the column is not next to its
-// own) and pass the FP32 prefilter. A lane meets its candidates in ascending
-// bin order, as in gather_one, so the lists are the same. Lanes with
-// live = false take part in the walk and keep nothing.
+// The list build of the cells of one wave (consecutive in bin order): the
+// wave walks the union of their 27-bin blocks once, (x, y) column by column,
+// WALK_GROUP candidates at a time. A group's
+// records are the same for every lane, so they come by scalar loads, and each
+// lane tests them all against its FP32 prefilter, shifting the verdicts into
+// a bit mask (one add with carry a candidate). The mask is then cut to the
+// lane's own bins of the column (the position range [klo, khi): its bins
+// z0-1..z0+1, empty when the column is not next to its own (x0, y0)) and its set bits
+// are queued in ascending order. A lane meets its candidates in ascending bin
+// order, as in gather_one, so the lists are the same. Lanes with live = false
+// take part in the walk and keep nothing.
 //
 // The exact test runs on the whole wave's queues at once: its sum of queue
-// lengths (cell, candidate) pairs are dealt out evenly over the 64 lanes (a
-// lane reads the owner's record again, from cache), each verdict is
-// written back into the queue slot (the id, or -1), and each lane then appends
-// its kept ids in queue order. A lane with a long queue thus no longer holds
-// the other 63 back.
-__device__ __forceinline__ int gather_wave(const PhysParams& P, const double* __restrict__ hot, bool live, int id,
-                                           int slot, const Seg& A, d3 cmA, int y0, int z0, int xw, int ylo, int yhi,
-                                           int zlo, int zhi, const int* __restrict__ start,
-                                           const int* __restrict__ sorted, const float4* __restrict__ cand,
-                                           int* __restrict__ nl, int stride, int cap, int* __restrict__ queue,
-                                           int qstride, float4* sp, float4* sq, int* ss, unsigned char* own,
-                                           int* offs, int* sid) {
-    const Pre pre = prefilter_of(P, A, cmA);
-    int m = 0, nq = 0;
+// lengths (cell, candidate) pairs are dealt out evenly over the 64 lanes and
+// decided from the FP32 candidate records (classify_pair: the owner's end
+// points from LDS, the candidate's from the stream just walked). Each verdict
+// is written back into the queue slot (the candidate's id, -1, or for the few
+// pairs the FP32 values leave open the marker -2 - k), and each lane then
+// appends its kept ids and markers in queue order. After the walk, lanes with
+// markers settle them with the reference's FP64 predicate and close the gaps,
+// so the FP64 registers are not live during the walk.
+__device__ __forceinline__ int gather_wave(const PhysParams& P, const double* __restrict__ hot, bool live, int slot,
+                                           int x0, int y0, int z0, int xlo, int xhi, int ylo, int yhi, int zlo, int zhi,
+                                           const int* __restrict__ start, const int* __restrict__ sorted,
+                                           const float4* __restrict__ cand, int* __restrict__ nl, int stride, int cap,
+                                           int* __restrict__ queue, int qstride, unsigned char* own, int* offs,
+                                           const float4* op, const float4* oq, const float2* orr, const Pre& pre) {
+    int m = 0, nq = 0, namb = 0;
     const int BYZ = P.BoxY * P.BoxZ;
     const int lane = __lane_id();
-    sid[lane] = id;
+    const int base = slot - lane;  // the wave's first position in bin order
+    const float BL = (float)P.BoxLength;
     int* const qw = queue - lane;  // entry e of lane l: qw[e * qstride + l]
     auto flush = [&]() {
         // exclusive scan of the queue lengths (0..GQ) by bit planes
@@ -365,29 +380,35 @@ __device__ __forceinline__ int gather_wave(const PhysParams& P, const double* __
         for (int p = lane; p < tot; p += 64) {
             const int o = own[p];
             int* const q = qw + (p - offs[o]) * qstride + o;
-            const int j = *q;
-            Seg B;
-            load_seg(hot, j, B);
-            Seg Ao;
-            load_seg(hot, sid[o], Ao);
-            const bool keep = (j != sid[o]) & keep_neighbour(Ao, cm_of(Ao), B, P.Rc);
-            *q = keep ? j : -1;
+            const unsigned e = (unsigned)*q;
+            const int k = (int)(e & QK_MASK);
+            const int code = (int)(e >> QK_BITS);
+            const int dx = code / 3 - 1, dy = code % 3 - 1;
+            const float4 bp = cand[2 * k], bq = cand[2 * k + 1];
+            const int j = sorted[k];
+            int cls = 0;
+            if (k != base + o) {  // the cell itself is skipped
+                const float2 r = orr[o];
+                cls = classify_pair(op[o], oq[o], r.x, r.y, bp, bq, (float)dx * BL, (float)dy * BL, DevRcp{});
+            }
+            *q = cls > 0 ? j : (cls < 0 ? -2 - k : -1);
         }
         wave_lds_sync();
         for (int e = 0; e < nq; e++) {
             const int j = queue[e * qstride];
-            if (j >= 0) {
+            if (j != -1) {
                 if (m < cap) nl[(size_t)m * stride + slot] = j;
                 m++;
+                namb += j < -1;
             }
         }
         nq = 0;
         wave_lds_sync();
     };
     const int zl = max(z0 - 1, 0), zh = min(z0 + 1, P.BoxZ - 1) + 1;
-    for (int xa = max(xw - 1, 0); xa <= min(xw + 1, P.BoxX - 1); xa++)
+    for (int xa = xlo; xa <= xhi; xa++)
         for (int ya = ylo; ya <= yhi; ya++) {
-            const bool ady = live && abs(ya - y0) <= 1;
+            const bool ady = live && (abs(xa - x0) <= 1) & (abs(ya - y0) <= 1);
             if (!__any(ady)) continue;
             // the column's bins zlo..zhi are one range of the bin order
             const int a = __builtin_amdgcn_readfirstlane(xa * BYZ + ya * P.BoxZ);
@@ -396,90 +417,165 @@ __device__ __forceinline__ int gather_wave(const PhysParams& P, const double* __
             const int klo = ady ? start[a + zl] : 0;
             const int khi = ady ? start[a + zh] : 0;
             const PreCol col = pre_col(P, pre, xa, ya);
-            for (int kb = k0; kb < k1; kb += 64) {
-                // stage 64 candidates in LDS with one coalesced load, then walk them
-                const int cnt = min(64, k1 - kb);
-                if (lane < cnt) {
-                    sp[lane] = cand[2 * (kb + lane)];
-                    sq[lane] = cand[2 * (kb + lane) + 1];
-                    ss[lane] = sorted[kb + lane];
+            const int qcode = ady ? (((xa - x0 + 1) * 3 + (ya - y0 + 1)) << QK_BITS) : 0;
+            for (int g = k0; g < k1; g += WALK_GROUP) {
+                // the group's records are the same for every lane: scalar loads
+                const float4* __restrict__ cg = cand + 2 * (size_t)g;
+                // bit WALK_GROUP-1-u: candidate g+u passed the prefilter
+                unsigned bits = 0;
+#pragma unroll 8
+                for (int u = 0; u < WALK_GROUP; u++) {
+                    unsigned carry;
+                    bits = __builtin_addc(bits, bits, col.near(cg[2 * u], cg[2 * u + 1]) ? 1u : 0u, &carry);
                 }
-                wave_lds_sync();
-                const int lo = klo - kb, hi = khi - kb;  // the lane's range, chunk-relative
-                int c = 0;
-                for (; c + 4 <= cnt; c += 4) {
-                    if (__any(nq > GQ - 4)) flush();
-#pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        // & rather than &&: every load and test runs, with no branch
-                        const float4 pj = sp[c + u], qj = sq[c + u];
-                        const bool b = (c + u >= lo) & (c + u < hi) & col.near(pj, qj);
-                        queue[nq * qstride] = ss[c + u];
-                        nq += b;
+                // only the lane's own bins: positions [klo, khi) of the group
+                const int lo = min(max(klo - g, 0), WALK_GROUP), hi = min(max(khi - g, 0), WALK_GROUP);
+                const unsigned below_hi = hi >= WALK_GROUP ? ~0u : ~(~0u >> hi);  // bits of u < hi
+                const unsigned below_lo = lo >= WALK_GROUP ? ~0u : ~(~0u >> lo);  // bits of u < lo
+                bits &= below_hi & ~below_lo;
+                // queue the survivors in ascending order: room is made once a
+                // group (a lane with more than a queue's worth fills and
+                // flushes in turns), then each lane pushes its own
+                if (!__any(bits != 0)) continue;
+                const int qb = qcode + g;
+                auto push = [&]() {
+                    const int u = __builtin_clz(bits);
+                    bits &= ~(0x80000000u >> u);
+                    queue[nq * qstride] = qb + u;
+                    nq++;
+                };
+                if (__any(nq + __popc(bits) > GQ)) {
+                    flush();
+                    while (__any(__popc(bits) > GQ)) {
+                        for (int e = 0; e < GQ && bits != 0; e++) push();
+                        flush();
                     }
                 }
-                for (; c < cnt; c++) {
-                    if (__any(nq == GQ)) flush();
-                    const float4 pj = sp[c], qj = sq[c];
-                    const bool b = (c >= lo) & (c < hi) & col.near(pj, qj);
-                    queue[nq * qstride] = ss[c];
-                    nq += b;
-                }
-                wave_lds_sync();
+                while (bits != 0) push();
             }
         }
     flush();
+    if (namb > 0) {
+        // the FP64 predicate for the pairs the FP32 records left open, in
+        // list order, closing the gaps of the rejected ones
+        Seg A;
+        load_seg(hot, sorted[slot], A);
+        const d3 cmA = cm_of(A);
+        const int stored = min(m, cap);
+        int w = 0;
+        for (int e = 0; e < stored; e++) {
+            int j = nl[(size_t)e * stride + slot];
+            if (j < -1) {
+                j = sorted[-2 - j];
+                Seg B;
+                load_seg(hot, j, B);
+                if (!keep_neighbour(A, cmA, B, P.Rc)) continue;
+            }
+            nl[(size_t)w++ * stride + slot] = j;
+        }
+        m = w + (m - stored);  // a list past its capacity stays past it (an error)
+    }
     return m;
 }
 
+// The lists of the cells in bin-order slots [0, n) (order, slot_of and
+// nl_count by slot); n <= QK_MASK + 1 (the queue entries' position field).
 __global__ void __launch_bounds__(256) k_gather_all(const double* __restrict__ hot, int n, PhysParams P,
                                                     const int* __restrict__ start, const int* __restrict__ sorted,
                                                     const float4* __restrict__ cand, int* __restrict__ nl,
                                                     int* __restrict__ nl_count, int stride, int cap,
                                                     int* __restrict__ order, int* __restrict__ slot_of,
-                                                    int* __restrict__ maxlist, int* __restrict__ err) {
+                                                    int* __restrict__ block_max, int* __restrict__ err) {
     __shared__ int queue[GQ * 256];
-    __shared__ float4 stage_p[256], stage_q[256];  // per wave: 64 candidates staged for the walk
-    __shared__ int stage_s[256];
     __shared__ unsigned char owner[GQ * 256];  // per wave: owner lane of each queued pair
-    __shared__ int qoff[256], qself[256];
+    __shared__ int qoff[256];
+    __shared__ float4 own_p[256], own_q[256];  // each cell's FP32 end points (its candidate record)
+    __shared__ float2 own_r[256];              // each cell's L/2 + Rc R and Rc R
+    __shared__ int wave_lmax[4];
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int lane = __lane_id(), w = threadIdx.x / 64;
     // every lane of a wave stays to the end (the wave walk uses cross-lane
     // operations); lanes past the last cell are not live
     const bool live = t < n;
-    if (!__any(live)) return;
     const int id = live ? sorted[t] : 0;
-    Seg A;
-    load_seg(hot, id, A);
-    const d3 cmA = cm_of(A);
     int x0 = 0, y0 = 0, z0 = 0;
-    const bool inside = live && bin_of(P, cmA, x0, y0, z0) >= 0;
+    Pre pre;
+    bool inside;
+    {
+        Seg A;
+        load_seg(hot, id, A);
+        const d3 cmA = cm_of(A);
+        inside = live && bin_of(P, cmA, x0, y0, z0) >= 0;
+        pre = prefilter_of(P, A, cmA);
+        if (live) {
+            own_p[threadIdx.x] = cand[2 * t];
+            own_q[threadIdx.x] = cand[2 * t + 1];
+            own_r[threadIdx.x] = make_float2((float)(A.L / 2.0 + P.Rc * A.R), (float)(P.Rc * A.R));
+        }
+    }
     if (live && !inside) atomicOr(err, ERRBIT_DOMAIN);
     const bool walk = inside;
-    // one bin column x for the whole wave: walk the union of the blocks
-    const int xmin = wave_min(walk ? x0 : INT_MAX), xmax = wave_max(walk ? x0 : INT_MIN);
+    // the union of the lanes' blocks
+    const int xmin = wave_min(walk ? x0 : INT_MAX);
     int m = 0;
-    if (xmin == xmax) {
-        const int ylo = max(wave_min(walk ? y0 : INT_MAX) - 1, 0);
-        const int yhi = min(wave_max(walk ? y0 : INT_MIN) + 1, P.BoxY - 1);
-        const int zlo = max(wave_min(walk ? z0 : INT_MAX) - 1, 0);
-        const int zhi = min(wave_max(walk ? z0 : INT_MIN) + 1, P.BoxZ - 1);
-        const int w = threadIdx.x / 64;
-        m = gather_wave(P, hot, walk, id, t, A, cmA, y0, z0, xmin, ylo, yhi, zlo, zhi, start, sorted, cand, nl,
-                        stride, cap, queue + threadIdx.x, 256, stage_p + 64 * w, stage_q + 64 * w, stage_s + 64 * w,
-                        owner + GQ * 64 * w, qoff + 64 * w, qself + 64 * w);
-    } else if (walk) {
-        m = gather_one(P, hot, id, t, start, sorted, cand, nl, stride, cap, err, queue + threadIdx.x, 256);
+    if (xmin != INT_MAX) {  // (no lane inside: nothing to walk)
+        // wave-uniform bounds (readfirstlane: the walk's loops and its record
+        // addresses are then uniform, so the records come by scalar loads)
+        const int xlo = __builtin_amdgcn_readfirstlane(max(xmin - 1, 0));
+        const int xhi = __builtin_amdgcn_readfirstlane(min(wave_max(walk ? x0 : INT_MIN) + 1, P.BoxX - 1));
+        const int ylo = __builtin_amdgcn_readfirstlane(max(wave_min(walk ? y0 : INT_MAX) - 1, 0));
+        const int yhi = __builtin_amdgcn_readfirstlane(min(wave_max(walk ? y0 : INT_MIN) + 1, P.BoxY - 1));
+        const int zlo = __builtin_amdgcn_readfirstlane(max(wave_min(walk ? z0 : INT_MAX) - 1, 0));
+        const int zhi = __builtin_amdgcn_readfirstlane(min(wave_max(walk ? z0 : INT_MIN) + 1, P.BoxZ - 1));
+        wave_lds_sync();  // the owners' records are in LDS
+        m = gather_wave(P, hot, walk, t, x0, y0, z0, xlo, xhi, ylo, yhi, zlo, zhi, start, sorted, cand, nl, stride,
+                        cap, queue + threadIdx.x, 256, owner + GQ * 64 * w, qoff + 64 * w,
+                        own_p + 64 * w, own_q + 64 * w, own_r + 64 * w, pre);
     }
-    if (!live) return;
     if (m > cap) {
         atomicOr(err, ERRBIT_NLIST);
         m = cap;
     }
-    order[t] = id;
-    slot_of[id] = t;
-    nl_count[t] = m;
-    atomicMax(maxlist, m);
+    if (live) {
+        order[t] = id;
+        slot_of[id] = t;
+        nl_count[t] = m;
+    }
+    // the block's longest list (k_reduce_max folds the blocks' values; one
+    // device atomic per wave on a single address cost ~0.15 ms at 2^20 cells)
+    const int wm = wave_max(live ? m : 0);
+    if (lane == 0) wave_lmax[w] = wm;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        block_max[blockIdx.x] = max(max(wave_lmax[0], wave_lmax[1]), max(wave_lmax[2], wave_lmax[3]));
+}
+
+// Folds per-block values into one statistic (one workgroup, one atomic).
+__global__ void __launch_bounds__(1024) k_reduce_max(const int* __restrict__ part, int n, int* __restrict__ out) {
+    __shared__ int red[16];
+    int v = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v = max(v, part[i]);
+    v = wave_max(v);
+    if (__lane_id() == 0) red[threadIdx.x / 64] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x / 64); w++) v = max(v, red[w]);
+        atomicMax(out, v);
+    }
+}
+__global__ void __launch_bounds__(1024) k_reduce_sum(const unsigned* __restrict__ part, int n,
+                                                     unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long red[16];
+    unsigned long long v = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v += part[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (__lane_id() == 0) red[threadIdx.x / 64] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x / 64); w++) v += red[w];
+        if (v) atomicAdd(out, v);
+    }
 }
 
 __global__ void __launch_bounds__(256) k_gather_ids(const double* __restrict__ hot, const int* __restrict__ ids, int nid,
@@ -667,7 +763,7 @@ __global__ void __launch_bounds__(MOVE_BLOCK) k_move_grow(
     const double* __restrict__ H, const double* __restrict__ Nrm, const double* __restrict__ wall,
     const double* __restrict__ gr1, const double* __restrict__ gr2, const int* __restrict__ type,
     double* __restrict__ dynfric, double* __restrict__ stafric, int* __restrict__ divflag,
-    unsigned long long* __restrict__ wall_draws, int* __restrict__ err) {
+    unsigned* __restrict__ block_draws, int* __restrict__ err) {
     __shared__ MoveLds L;
     const int tid = threadIdx.x;
     const int slot0 = xcd_block(blockIdx.x, gridDim.x) * MOVE_BLOCK;
@@ -770,7 +866,7 @@ __global__ void __launch_bounds__(MOVE_BLOCK) k_move_grow(
         stafric[3 * (size_t)id + 2] = sf.z;
         divflag[id] = (S1.L >= Ldiv) ? 1 : 0;  // GrowCell: ABM/Cells.cpp:48
     }
-    // block sum of F_cw draws -> one 64-bit atomic (integer: order-free)
+    // the block's count of F_cw draws (integers: the fold's order is free)
     int w = draws;
 #pragma unroll
     for (int s = 32; s > 0; s >>= 1) w += __shfl_xor(w, s, 64);
@@ -779,7 +875,7 @@ __global__ void __launch_bounds__(MOVE_BLOCK) k_move_grow(
     if (tid == 0) {
         int s = 0;
         for (int i = 0; i < MOVE_BLOCK / 64; i++) s += L.draws[i];
-        if (s) atomicAdd(wall_draws, (unsigned long long)s);
+        block_draws[blockIdx.x] = (unsigned)s;  // folded by k_reduce_sum
     }
 }
 
@@ -825,7 +921,7 @@ __global__ void __launch_bounds__(MOVE_BLOCK) k_mean_stress(const double* __rest
                                                             const double* __restrict__ H, const double* __restrict__ Nrm,
                                                             const double* __restrict__ wall, double* __restrict__ S5,
                                                             double* __restrict__ F3,
-                                                            unsigned long long* __restrict__ wall_draws,
+                                                            unsigned* __restrict__ wave_draws,
                                                             int* __restrict__ err) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -843,6 +939,7 @@ __global__ void __launch_bounds__(MOVE_BLOCK) k_mean_stress(const double* __rest
     const double V = S.L * kPI * S.R * S.R + 4.0 / 3.0 * kPI * S.R * S.R * S.R;
     double xx = 0, yy = 0, zz = 0;
     d3 Fn = mk(0, 0, 0);
+    bool drew = false;
     const int cnt = nl_count[t];
     for (int k = 0; k < cnt; k++) {
         const int j = nl[(size_t)k * stride + t];
@@ -869,7 +966,7 @@ __global__ void __launch_bounds__(MOVE_BLOCK) k_mean_stress(const double* __rest
         const double wall_z = wall[x * P.BoxY + y];
         const d3 F1 = F_cw_locus(P, S, K, cm, S.p, wall_z, M, r1, sf, df);
         const d3 F2 = F_cw_locus(P, S, K, cm, S.q, wall_z, M, r2, sf, df);
-        atomicAdd(wall_draws, 1ull);
+        drew = true;
         Fn = add(Fn, F1);
         r1 = sub(r1, cm);
         xx -= F1.x * r1.x;
@@ -914,6 +1011,10 @@ __global__ void __launch_bounds__(MOVE_BLOCK) k_mean_stress(const double* __rest
     F3[3 * (size_t)id] = Fn.x;
     F3[3 * (size_t)id + 1] = Fn.y;
     F3[3 * (size_t)id + 2] = Fn.z;
+    // the wave's count of F_cw draws, by its first remaining lane (the
+    // caller zeroes the counts; k_reduce_sum folds them)
+    const unsigned long long d = __ballot(drew);
+    if (__lane_id() == __builtin_ctzll(__ballot(1))) wave_draws[t / 64] = (unsigned)__popcll(d);
 }
 
 // ----------------------------------------------------------- compact ---

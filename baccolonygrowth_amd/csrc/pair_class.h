@@ -1,0 +1,88 @@
+// pair_class.h - FP32 decision of the neighbour-list predicate for one pair.
+//
+// reduceNeighbours (ABM/Neighbours.cpp:72-88) keeps neighbour B of cell A when
+// an end point of B lies within re = L_A/2 + Rc R_A of A's centre and the
+// segment distance is below rd = Rc R_A, both compared in FP64. The list build
+// decides most pairs from the FP32 candidate records instead, and hands the
+// rest to the FP64 predicate. The function is host-callable so that a CPU
+// harness can check it against the oracle on many pairs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+namespace bcg {
+
+// 1 keep, 0 reject, -1 undecided (the FP64 predicate decides).
+// a_p, a_q: A's end points in the frame of A's bin column (x, y relative to the
+// column origin, z as stored); b_p, b_q: B's end points in the frame of B's
+// column, moved into A's by (sx, sy); re, rd as above; rcp: a reciprocal,
+// exact or approximate (the bounds below hold for any parameters it yields).
+//
+// End points: |e - c|^2 from the differences. Segment distance: any (s, t) in
+// [0,1]^2 gives the upper bound f(s, t) = |A(s) - B(t)|^2, and since f is a
+// convex quadratic, f(s, t) + the minimum over the box of its gradient's
+// linear term,
+//   f + min(-g_s s, g_s (1 - s)) + min(-g_t t, g_t (1 - t)),
+// is a lower bound of its minimum over the box. Near the closest points both
+// bounds are tight (the gradient vanishes or points out of the box there),
+// also for near-parallel segments whose FP32 parameters are poorly
+// determined.
+//
+// Tolerances: 1e-4 of the squared thresholds (the FP64 reference's rounding
+// and its root comparisons are far below it) plus 1e-5 E^2, E the largest
+// coordinate magnitude of the four end points; the FP32 rounding of the stored
+// coordinates and of these sums is a few tens of 2^-24 E^2.
+template <class Rcp>
+__host__ __device__ inline int classify_pair(float4 a_p, float4 a_q, float re, float rd, float4 b_p, float4 b_q,
+                                             float sx, float sy, Rcp rcp) {
+    // straight-line code (no early exits): a wave runs it for 64 pairs at once
+    b_p.x += sx;
+    b_p.y += sy;
+    b_q.x += sx;
+    b_q.y += sy;
+    const float E = fmaxf(fmaxf(fmaxf(fmaxf(fabsf(a_p.x), fabsf(a_p.y)), fmaxf(fabsf(a_p.z), fabsf(a_q.x))),
+                                fmaxf(fmaxf(fabsf(a_q.y), fabsf(a_q.z)), fmaxf(fabsf(b_p.x), fabsf(b_p.y)))),
+                          fmaxf(fmaxf(fabsf(b_p.z), fabsf(b_q.x)), fmaxf(fabsf(b_q.y), fabsf(b_q.z))));
+    const float tol = 1e-5f * E * E;
+    // first test: an end point of B within re of A's centre
+    const float cx = 0.5f * (a_p.x + a_q.x), cy = 0.5f * (a_p.y + a_q.y), cz = 0.5f * (a_p.z + a_q.z);
+    const float px = b_p.x - cx, py = b_p.y - cy, pz = b_p.z - cz;
+    const float qx = b_q.x - cx, qy = b_q.y - cy, qz = b_q.z - cz;
+    const float e2 = fminf(px * px + py * py + pz * pz, qx * qx + qy * qy + qz * qz);
+    const float re2 = re * re;
+    const bool end_no = e2 > re2 * 1.0001f + tol;
+    const bool end_yes = e2 < re2 * 0.9999f - tol;
+    // second test: the segment distance below rd. (s, t): the line-line
+    // parameter s0, t optimal for it and clamped, s optimal for that t and
+    // clamped (the closest points when the segments' closest pair is interior
+    // or has t at an end; near them otherwise, which only loosens the bounds)
+    const float v1x = a_q.x - a_p.x, v1y = a_q.y - a_p.y, v1z = a_q.z - a_p.z;
+    const float v2x = b_q.x - b_p.x, v2y = b_q.y - b_p.y, v2z = b_q.z - b_p.z;
+    const float rx = a_p.x - b_p.x, ry = a_p.y - b_p.y, rz = a_p.z - b_p.z;
+    const float l1 = v1x * v1x + v1y * v1y + v1z * v1z, l2 = v2x * v2x + v2y * v2y + v2z * v2z;
+    // (near-)degenerate segments: the FP64 predicate decides, whatever it does
+    const bool degenerate = !(l1 > 1e-4f * tol) || !(l2 > 1e-4f * tol);
+    const float f = v2x * rx + v2y * ry + v2z * rz, c = v1x * rx + v1y * ry + v1z * rz;
+    const float b = v1x * v2x + v1y * v2y + v1z * v2z;
+    const float den = l1 * l2 - b * b;
+    auto clamp01 = [](float x) { return fmaxf(0.0f, fminf(1.0f, x)); };  // NaN -> 1
+    const float s0 = den > 0.0f ? clamp01((b * f - c * l2) * rcp(den)) : 0.0f;
+    const float t = clamp01((b * s0 + f) * rcp(l2));
+    const float s = clamp01((b * t - c) * rcp(l1));
+    const float wx = rx + s * v1x - t * v2x, wy = ry + s * v1y - t * v2y, wz = rz + s * v1z - t * v2z;
+    const float ub = wx * wx + wy * wy + wz * wz;
+    const float gs = 2.0f * (wx * v1x + wy * v1y + wz * v1z);
+    const float gt = -2.0f * (wx * v2x + wy * v2y + wz * v2z);
+    const float lb = ub + fminf(-gs * s, gs * (1.0f - s)) + fminf(-gt * t, gt * (1.0f - t));
+    const float rd2 = rd * rd;
+    const bool dist_no = lb > rd2 * 1.0001f + tol;
+    const bool dist_yes = ub < rd2 * 0.9999f - tol;
+    if (end_no) return 0;
+    if (degenerate) return -1;
+    if (dist_no) return 0;
+    if (end_yes && dist_yes) return 1;
+    return -1;
+}
+
+}  // namespace bcg

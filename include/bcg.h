@@ -97,7 +97,7 @@ typedef struct bcg_params {
     double BoxLength;          /* bin edge; the reference uses L_divide + 2*Radius */
     int32_t BoxX, BoxY, BoxZ;  /* "Box_x", "Box_y", "Box_z" */
     int32_t refinementGridHeight;
-    int32_t maxCells;          /* "MaxCells": capacity of the cell buffers          */
+    int32_t maxCells;          /* "MaxCells": capacity of the cell buffers (<= 2^28) */
     int32_t maxNeighbours;     /* capacity of one reduced neighbour list            */
     int32_t crossFeeding;      /* "CrossFeeding": Type 2 grows with GrowthRate2     */
     int32_t planar;            /* 1: reference x-z model (y forced to 0),

@@ -352,6 +352,28 @@ static int box_remove(or_sim* s, int a, int id) {
     return 0;
 }
 
+/* reduceNeighbours' test of one pair (ABM/Neighbours.cpp:72-88): 1 when
+ * neighbour o stays in the list of cell c.                                 */
+static int keep_neighbour(const bcg_cell* c, const bcg_cell* o, double Rc) {
+    v3 cm = cmof(c);
+    const double reach = c->Length / 2.0 + Rc * c->Radius;
+    v3 v = vsub(P(o), cm);
+    double dp = sqrt(vdot(v, v));
+    v = vsub(Q(o), cm);
+    double dq = sqrt(vdot(v, v));
+    if ((dp < reach) || (dq < reach)) {
+        double d; v3 c1, c2;
+        min_distance(c, o, &d, &c1, &c2);
+        return d < Rc * c->Radius;
+    }
+    return 0;
+}
+
+/* The test of pairs (a[i], b[i]), i < n. */
+void or_keep_batch(const bcg_cell* a, const bcg_cell* b, int n, double Rc, int* out) {
+    for (int i = 0; i < n; i++) out[i] = keep_neighbour(&a[i], &b[i], Rc);
+}
+
 /* UniformGrid::GetNeighbours (ABM/UniformGrid.h:141-170) followed by
  * reduceNeighbours (ABM/Neighbours.cpp:57-93) for one cell.               */
 static int gather_into(or_sim* s, const bcg_cell* cells, int id, int* scratch) {
@@ -371,22 +393,11 @@ static int gather_into(or_sim* s, const bcg_cell* cells, int id, int* scratch) {
                         scratch[n++] = b[k];
                     }
             }
-    v3 cm = cmof(c);
     int m = 0;
-    const double reach = c->Length / 2.0 + pr->Rc * c->Radius;
     for (int k = 0; k < n; k++) {
-        const bcg_cell* o = &cells[scratch[k]];
-        v3 v = vsub(P(o), cm);
-        double dp = sqrt(vdot(v, v));
-        v = vsub(Q(o), cm);
-        double dq = sqrt(vdot(v, v));
-        if ((dp < reach) || (dq < reach)) {
-            double d; v3 c1, c2;
-            min_distance(c, o, &d, &c1, &c2);
-            if (d < pr->Rc * c->Radius) {
-                if (m >= s->cap) return BCG_ERR_NEIGHBOUR_OVERFLOW;
-                s->nl[(size_t)id * s->cap + m++] = scratch[k];
-            }
+        if (keep_neighbour(c, &cells[scratch[k]], pr->Rc)) {
+            if (m >= s->cap) return BCG_ERR_NEIGHBOUR_OVERFLOW;
+            s->nl[(size_t)id * s->cap + m++] = scratch[k];
         }
     }
     s->nl_count[id] = m;

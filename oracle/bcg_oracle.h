@@ -28,6 +28,8 @@ extern "C" {
 typedef struct or_sim or_sim;
 
 void or_min_distance(const bcg_cell* a, const bcg_cell* b, double* out7);
+/* reduceNeighbours' test of pairs (a[i], b[i]): 1 when b[i] stays in a[i]'s list */
+void or_keep_batch(const bcg_cell* a, const bcg_cell* b, int n, double Rc, int* out);
 void or_F_cc(const bcg_params* p, const bcg_cell* a, const bcg_cell* b, double* out7);
 /* wall contact at both vertices; consumes one rand() draw like the reference */
 void or_F_cw(const bcg_params* p, const bcg_cell* c, double wall_z, double* out18);

@@ -243,6 +243,17 @@ class Oracle(_Base):
         """OpenMP threads of Sim.step_par."""
         self.lib.or_set_threads(n)
 
+    def keep_batch(self, a: np.ndarray, b: np.ndarray, Rc: float) -> np.ndarray:
+        """reduceNeighbours' test of the pairs (a[i], b[i]): 1 when b[i] stays
+        in a[i]'s list."""
+        a = np.ascontiguousarray(a)
+        b = np.ascontiguousarray(b)
+        assert len(a) == len(b)
+        out = np.zeros(len(a), np.int32)
+        self.lib.or_keep_batch.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_double, _ip]
+        self.lib.or_keep_batch(_cells_ptr(a), _cells_ptr(b), len(a), Rc, out.ctypes.data_as(_ip))
+        return out
+
     def glibc_stream(self, seed: int, n: int) -> np.ndarray:
         class G(ctypes.Structure):
             _fields_ = [("r", ctypes.c_int32 * 34), ("i", ctypes.c_int32)]

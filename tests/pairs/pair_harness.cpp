@@ -1,0 +1,36 @@
+// Host harness of the list build's FP32 pair classification (test
+// infrastructure): the classifier the kernels use, compiled for the host, over
+// a batch of pairs. Reciprocal mode 0 is exact; 1 and 2 round it one ulp up or
+// down, as an approximate reciprocal may; 3 is 30 % low, which moves the
+// closest-point parameters far off (the bounds must hold for any parameters).
+#include "../../baccolonygrowth_amd/csrc/pair_class.h"
+
+#include <math.h>
+
+namespace {
+struct HostRcp {
+    int mode;
+    float operator()(float x) const {
+        const float r = 1.0f / x;
+        if (mode == 1) return nextafterf(r, INFINITY);
+        if (mode == 2) return nextafterf(r, -INFINITY);
+        if (mode == 3) return 0.7f * r;
+        return r;
+    }
+};
+}  // namespace
+
+extern "C" {
+// rec: n x 16 floats (a_p, a_q, b_p, b_q as float4); thr: n x 2 (re, rd);
+// shift: n x 2 (sx, sy); out: n verdicts.
+int pc_classify_batch(int n, const float* rec, const float* thr, const float* shift, int mode, int* out) {
+    const HostRcp rcp{mode};
+    for (int i = 0; i < n; i++) {
+        const float* r = rec + 16 * (long)i;
+        const float4 ap = make_float4(r[0], r[1], r[2], r[3]), aq = make_float4(r[4], r[5], r[6], r[7]);
+        const float4 bp = make_float4(r[8], r[9], r[10], r[11]), bq = make_float4(r[12], r[13], r[14], r[15]);
+        out[i] = bcg::classify_pair(ap, aq, thr[2 * i], thr[2 * i + 1], bp, bq, shift[2 * i], shift[2 * i + 1], rcp);
+    }
+    return 0;
+}
+}
