@@ -16,6 +16,7 @@ constexpr int SCAN_BLOCK_ITEMS = 2048;  // items per block of the scan kernels
 // position below the cell count: the candidate array holds as many records
 // beyond its last cell (never used for a decision).
 constexpr int WALK_GROUP = 32;
+constexpr int WALK_STEP = 8;  // candidates a step of the list build's walk (divides WALK_GROUP)
 // Queue entries of the list build: the candidate's position in bin order (28
 // bits, so at most 2^28 cells) and its bin column relative to the owner's,
 // (dx + 1) * 3 + (dy + 1).

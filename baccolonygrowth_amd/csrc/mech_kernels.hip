@@ -421,13 +421,19 @@ __device__ __forceinline__ int gather_wave(const PhysParams& P, const double* __
             for (int g = k0; g < k1; g += WALK_GROUP) {
                 // the group's records are the same for every lane: scalar loads
                 const float4* __restrict__ cg = cand + 2 * (size_t)g;
-                // bit WALK_GROUP-1-u: candidate g+u passed the prefilter
+                // the group's candidates in steps of WALK_STEP (a short last
+                // group of a column stops at the next multiple of the step)
+                const int n = min(WALK_GROUP, (k1 - g + WALK_STEP - 1) / WALK_STEP * WALK_STEP);
                 unsigned bits = 0;
-#pragma unroll 8
-                for (int u = 0; u < WALK_GROUP; u++) {
-                    unsigned carry;
-                    bits = __builtin_addc(bits, bits, col.near(cg[2 * u], cg[2 * u + 1]) ? 1u : 0u, &carry);
+                for (int u0 = 0; u0 < n; u0 += WALK_STEP) {
+#pragma unroll
+                    for (int u = u0; u < u0 + WALK_STEP; u++) {
+                        unsigned carry;
+                        bits = __builtin_addc(bits, bits, col.near(cg[2 * u], cg[2 * u + 1]) ? 1u : 0u, &carry);
+                    }
                 }
+                // bit WALK_GROUP-1-u: candidate g+u passed the prefilter
+                if (n < WALK_GROUP) bits <<= WALK_GROUP - n;
                 // only the lane's own bins: positions [klo, khi) of the group
                 const int lo = min(max(klo - g, 0), WALK_GROUP), hi = min(max(khi - g, 0), WALK_GROUP);
                 const unsigned below_hi = hi >= WALK_GROUP ? ~0u : ~(~0u >> hi);  // bits of u < hi
