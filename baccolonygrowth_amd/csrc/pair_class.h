@@ -33,6 +33,12 @@ namespace bcg {
 // and its root comparisons are far below it) plus 1e-5 E^2, E the largest
 // coordinate magnitude of the four end points; the FP32 rounding of the stored
 // coordinates and of these sums is a few tens of 2^-24 E^2.
+// FMA helpers: the bounds below carry their own margins, so contraction is
+// free here (fmaf rounds once on the host and on the device alike).
+__host__ __device__ inline float dot3(float ax, float ay, float az, float bx, float by, float bz) {
+    return fmaf(az, bz, fmaf(ay, by, ax * bx));
+}
+
 template <class Rcp>
 __host__ __device__ inline int classify_pair(float4 a_p, float4 a_q, float re, float rd, float4 b_p, float4 b_q,
                                              float sx, float sy, Rcp rcp) {
@@ -49,10 +55,10 @@ __host__ __device__ inline int classify_pair(float4 a_p, float4 a_q, float re, f
     const float cx = 0.5f * (a_p.x + a_q.x), cy = 0.5f * (a_p.y + a_q.y), cz = 0.5f * (a_p.z + a_q.z);
     const float px = b_p.x - cx, py = b_p.y - cy, pz = b_p.z - cz;
     const float qx = b_q.x - cx, qy = b_q.y - cy, qz = b_q.z - cz;
-    const float e2 = fminf(px * px + py * py + pz * pz, qx * qx + qy * qy + qz * qz);
+    const float e2 = fminf(dot3(px, py, pz, px, py, pz), dot3(qx, qy, qz, qx, qy, qz));
     const float re2 = re * re;
-    const bool end_no = e2 > re2 * 1.0001f + tol;
-    const bool end_yes = e2 < re2 * 0.9999f - tol;
+    const bool end_no = e2 > fmaf(re2, 1.0001f, tol);
+    const bool end_yes = e2 < fmaf(re2, 0.9999f, -tol);
     // second test: the segment distance below rd. (s, t): the line-line
     // parameter s0, t optimal for it and clamped, s optimal for that t and
     // clamped (the closest points when the segments' closest pair is interior
@@ -60,24 +66,26 @@ __host__ __device__ inline int classify_pair(float4 a_p, float4 a_q, float re, f
     const float v1x = a_q.x - a_p.x, v1y = a_q.y - a_p.y, v1z = a_q.z - a_p.z;
     const float v2x = b_q.x - b_p.x, v2y = b_q.y - b_p.y, v2z = b_q.z - b_p.z;
     const float rx = a_p.x - b_p.x, ry = a_p.y - b_p.y, rz = a_p.z - b_p.z;
-    const float l1 = v1x * v1x + v1y * v1y + v1z * v1z, l2 = v2x * v2x + v2y * v2y + v2z * v2z;
+    const float l1 = dot3(v1x, v1y, v1z, v1x, v1y, v1z), l2 = dot3(v2x, v2y, v2z, v2x, v2y, v2z);
     // (near-)degenerate segments: the FP64 predicate decides, whatever it does
     const bool degenerate = !(l1 > 1e-4f * tol) || !(l2 > 1e-4f * tol);
-    const float f = v2x * rx + v2y * ry + v2z * rz, c = v1x * rx + v1y * ry + v1z * rz;
-    const float b = v1x * v2x + v1y * v2y + v1z * v2z;
-    const float den = l1 * l2 - b * b;
+    const float f = dot3(v2x, v2y, v2z, rx, ry, rz), c = dot3(v1x, v1y, v1z, rx, ry, rz);
+    const float b = dot3(v1x, v1y, v1z, v2x, v2y, v2z);
+    const float den = fmaf(l1, l2, -b * b);
     auto clamp01 = [](float x) { return fmaxf(0.0f, fminf(1.0f, x)); };  // NaN -> 1
-    const float s0 = den > 0.0f ? clamp01((b * f - c * l2) * rcp(den)) : 0.0f;
-    const float t = clamp01((b * s0 + f) * rcp(l2));
-    const float s = clamp01((b * t - c) * rcp(l1));
-    const float wx = rx + s * v1x - t * v2x, wy = ry + s * v1y - t * v2y, wz = rz + s * v1z - t * v2z;
-    const float ub = wx * wx + wy * wy + wz * wz;
-    const float gs = 2.0f * (wx * v1x + wy * v1y + wz * v1z);
-    const float gt = -2.0f * (wx * v2x + wy * v2y + wz * v2z);
-    const float lb = ub + fminf(-gs * s, gs * (1.0f - s)) + fminf(-gt * t, gt * (1.0f - t));
+    const float s0 = den > 0.0f ? clamp01(fmaf(b, f, -c * l2) * rcp(den)) : 0.0f;
+    const float t = clamp01(fmaf(b, s0, f) * rcp(l2));
+    const float s = clamp01(fmaf(b, t, -c) * rcp(l1));
+    const float wx = fmaf(-t, v2x, fmaf(s, v1x, rx));
+    const float wy = fmaf(-t, v2y, fmaf(s, v1y, ry));
+    const float wz = fmaf(-t, v2z, fmaf(s, v1z, rz));
+    const float ub = dot3(wx, wy, wz, wx, wy, wz);
+    const float gs = 2.0f * dot3(wx, wy, wz, v1x, v1y, v1z);
+    const float gt = -2.0f * dot3(wx, wy, wz, v2x, v2y, v2z);
+    const float lb = ub + fminf(-gs * s, fmaf(-gs, s, gs)) + fminf(-gt * t, fmaf(-gt, t, gt));
     const float rd2 = rd * rd;
-    const bool dist_no = lb > rd2 * 1.0001f + tol;
-    const bool dist_yes = ub < rd2 * 0.9999f - tol;
+    const bool dist_no = lb > fmaf(rd2, 1.0001f, tol);
+    const bool dist_yes = ub < fmaf(rd2, 0.9999f, -tol);
     if (end_no) return 0;
     if (degenerate) return -1;
     if (dist_no) return 0;
