@@ -90,7 +90,7 @@ def measured_traffic(kernel: str):
         with open(PROFILE) as fh:
             for line in fh:
                 f = line.split()
-                if len(f) == 7 and f[0] == kernel:
+                if len(f) >= 7 and f[0] == kernel:
                     v = (float(f[4]) + float(f[5])) * 1e6
                     return v if v == v else None
     except OSError:
