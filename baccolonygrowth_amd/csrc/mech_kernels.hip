@@ -663,6 +663,10 @@ __device__ __forceinline__ void sum_forces(const PhysParams& P, const Seg& S, co
 
 // Cell state of one block of k_move_grow staged in LDS for its pair phases
 // (structure of arrays, one column per cell of the block).
+// Pairs tested for contact at once (a super-chunk of PAIR_SUPER chunks).
+constexpr int PAIR_SUPER = 8;
+static_assert(MOVE_BLOCK == 64, "the pair dealing of pair_sums is one wave");
+
 struct MoveLds {
     double seg[8][MOVE_BLOCK];  // p.x p.y p.z q.x q.y q.z L R
     double kin[6][MOVE_BLOCK];  // v.x v.y v.z w.x w.y w.z
@@ -670,6 +674,8 @@ struct MoveLds {
     int off[MOVE_BLOCK + 1];    // exclusive scan of the cells' list lengths
     int scan[48];
     int draws[MOVE_BLOCK / 64];
+    int cp_j[PAIR_SUPER * MOVE_BLOCK];            // a super-chunk's pairs that may touch: neighbour id,
+    unsigned char cp_c[PAIR_SUPER * MOVE_BLOCK];  // and owning cell, in pair order
 };
 
 __device__ __forceinline__ void lds_put_seg(MoveLds& L, int c, const Seg& S) {
@@ -707,9 +713,13 @@ __device__ __forceinline__ Kin lds_kin(const MoveLds& L, int c) {
 
 // The cell-cell sums (the neighbour loop of ABM/Forces.cpp:341-347) of every
 // cell of the block against the segments and kinematics staged in L. The
-// block's (cell, list entry) pairs are spread over its threads, MOVE_BLOCK
-// pairs per chunk: each thread evaluates F_cc for one pair into the chunk
-// buffer, then each cell's own thread adds its pairs in list order. These are
+// block's (cell, list entry) pairs are spread over its threads, a super-chunk
+// of PAIR_SUPER * MOVE_BLOCK pairs at a time. Every pair is first tested in
+// FP32 for "surely not in contact" (surely_apart: F_cc would then add only
+// signed zeros, which leave a sum that started at +0 unchanged). The others,
+// about one in six on a dense colony, are compacted in pair order and
+// evaluated by F_cc, one per thread, MOVE_BLOCK at a time into the chunk
+// buffer; each cell's own thread then adds its pairs in list order. These are
 // the reference's additions in the reference's order, so F and T are
 // bit-identical to the sequential loop, while no lane idles behind a longer
 // list elsewhere in its wave.
@@ -721,42 +731,86 @@ __device__ __forceinline__ void pair_sums(const PhysParams& P, MoveLds& L, const
     const int lo = L.off[tid], hi = L.off[tid + 1];
     F = mk(0, 0, 0);
     T = mk(0, 0, 0);
-    for (int base = 0; base < total; base += MOVE_BLOCK) {
-        const int p = base + tid;
-        if (p < total) {
-            // the owner of pair p: the largest c with off[c] <= p (off[0] = 0)
-            int c = 0;
+    auto owner = [&](int p) {  // the largest c with off[c] <= p (off[0] = 0)
+        int c = 0;
 #pragma unroll
-            for (int s = MOVE_BLOCK / 2; s > 0; s >>= 1)
-                if (L.off[c + s] <= p) c += s;
-            const int j = nl[(size_t)(p - L.off[c]) * stride + slot0 + c];
-            Seg B;
-            load_seg(hot, j, B);
-            const Seg A = lds_seg(L, c);
-            d3 f, tc;
-            F_cc(
-                P, A, cm_of(A), B, [&]() { return lds_kin(L, c); },
-                [&]() {
-                    Kin KB;
-                    load_kin(hot, j, KB);
-                    return KB;
-                },
-                f, tc);
-            L.con[0][tid] = f.x;
-            L.con[1][tid] = f.y;
-            L.con[2][tid] = f.z;
-            L.con[3][tid] = tc.x;
-            L.con[4][tid] = tc.y;
-            L.con[5][tid] = tc.z;
+        for (int s = MOVE_BLOCK / 2; s > 0; s >>= 1)
+            if (L.off[c + s] <= p) c += s;
+        return c;
+    };
+    constexpr int SC = PAIR_SUPER * MOVE_BLOCK;
+    for (int sbase = 0; sbase < total; sbase += SC) {
+        unsigned long long touch[PAIR_SUPER];
+        int nc = 0;
+#pragma unroll
+        for (int r = 0; r < PAIR_SUPER; r++) {
+            const int p = sbase + r * MOVE_BLOCK + tid;
+            bool t = false;
+            int c = 0, j = 0;
+            if (p < total) {
+                c = owner(p);
+                j = nl[(size_t)(p - L.off[c]) * stride + slot0 + c];
+                Seg B;
+                load_seg(hot, j, B);
+                const Seg A = lds_seg(L, c);
+                const d3 o = cm_of(A);
+                auto rel = [&](d3 e) { return make_float4((float)(e.x - o.x), (float)(e.y - o.y), (float)(e.z - o.z), 0.f); };
+                t = !surely_apart(rel(A.p), rel(A.q), rel(B.p), rel(B.q), (float)(A.R + B.R), DevRcp{});
+            }
+            touch[r] = __ballot(t);
+            if (t) {
+                const int k = nc + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(touch[r] >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((unsigned)touch[r], 0u));
+                L.cp_j[k] = j;
+                L.cp_c[k] = (unsigned char)c;
+            }
+            nc += __popcll(touch[r]);
         }
+        // this cell's pairs of the super-chunk that may touch: [cs, ce) of the compact list
+        auto before = [&](int p) {  // pairs that may touch in [sbase, p), sbase <= p <= sbase + SC
+            const int q = p - sbase;
+            int m = 0;
+#pragma unroll
+            for (int r = 0; r < PAIR_SUPER; r++) {
+                const int bq = min(max(q - r * MOVE_BLOCK, 0), MOVE_BLOCK);
+                m += __popcll(bq >= 64 ? touch[r] : (touch[r] & ((1ull << bq) - 1ull)));
+            }
+            return m;
+        };
+        const int cs = before(min(max(lo, sbase), sbase + SC)), ce = before(min(max(hi, sbase), sbase + SC));
         __syncthreads();
-        const int a = lo > base ? lo : base;
-        const int b = hi < base + MOVE_BLOCK ? hi : base + MOVE_BLOCK;
-        for (int q = a - base; q < b - base; q++) {
-            F = add(F, mk(L.con[0][q], L.con[1][q], L.con[2][q]));
-            T = add(T, mk(L.con[3][q], L.con[4][q], L.con[5][q]));
+        for (int base = 0; base < nc; base += MOVE_BLOCK) {
+            const int k = base + tid;
+            if (k < nc) {
+                const int c = L.cp_c[k], j = L.cp_j[k];
+                Seg B;
+                load_seg(hot, j, B);
+                const Seg A = lds_seg(L, c);
+                d3 f, tc;
+                F_cc(
+                    P, A, cm_of(A), B, [&]() { return lds_kin(L, c); },
+                    [&]() {
+                        Kin KB;
+                        load_kin(hot, j, KB);
+                        return KB;
+                    },
+                    f, tc);
+                L.con[0][tid] = f.x;
+                L.con[1][tid] = f.y;
+                L.con[2][tid] = f.z;
+                L.con[3][tid] = tc.x;
+                L.con[4][tid] = tc.y;
+                L.con[5][tid] = tc.z;
+            }
+            __syncthreads();
+            const int a = cs > base ? cs : base;
+            const int b = ce < base + MOVE_BLOCK ? ce : base + MOVE_BLOCK;
+            for (int q = a - base; q < b - base; q++) {
+                F = add(F, mk(L.con[0][q], L.con[1][q], L.con[2][q]));
+                T = add(T, mk(L.con[3][q], L.con[4][q], L.con[5][q]));
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 

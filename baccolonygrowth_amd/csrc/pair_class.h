@@ -1,4 +1,5 @@
-// pair_class.h - FP32 decision of the neighbour-list predicate for one pair.
+// pair_class.h - FP32 decisions about a pair of cells: the neighbour-list
+// predicate (classify_pair) and "surely not in contact" (surely_apart).
 //
 // reduceNeighbours (ABM/Neighbours.cpp:72-88) keeps neighbour B of cell A when
 // an end point of B lies within re = L_A/2 + Rc R_A of A's centre and the
@@ -39,36 +40,26 @@ __host__ __device__ inline float dot3(float ax, float ay, float az, float bx, fl
     return fmaf(az, bz, fmaf(ay, by, ax * bx));
 }
 
+__host__ __device__ inline float coord_max(float4 a_p, float4 a_q, float4 b_p, float4 b_q) {
+    return fmaxf(fmaxf(fmaxf(fmaxf(fabsf(a_p.x), fabsf(a_p.y)), fmaxf(fabsf(a_p.z), fabsf(a_q.x))),
+                       fmaxf(fmaxf(fabsf(a_q.y), fabsf(a_q.z)), fmaxf(fabsf(b_p.x), fabsf(b_p.y)))),
+                 fmaxf(fmaxf(fabsf(b_p.z), fabsf(b_q.x)), fmaxf(fabsf(b_q.y), fabsf(b_q.z))));
+}
+
+// Upper and lower bounds (ub, lb) of the squared distance between segments
+// A = [a_p, a_q] and B = [b_p, b_q] (see above): (s, t) is the line-line
+// parameter s0, t optimal for it and clamped, s optimal for that t and
+// clamped (the closest points when the segments' closest pair is interior or
+// has t at an end; near them otherwise, which only loosens the bounds).
+// Returns false for (near-)degenerate segments (no bounds).
 template <class Rcp>
-__host__ __device__ inline int classify_pair(float4 a_p, float4 a_q, float re, float rd, float4 b_p, float4 b_q,
-                                             float sx, float sy, Rcp rcp) {
-    // straight-line code (no early exits): a wave runs it for 64 pairs at once
-    b_p.x += sx;
-    b_p.y += sy;
-    b_q.x += sx;
-    b_q.y += sy;
-    const float E = fmaxf(fmaxf(fmaxf(fmaxf(fabsf(a_p.x), fabsf(a_p.y)), fmaxf(fabsf(a_p.z), fabsf(a_q.x))),
-                                fmaxf(fmaxf(fabsf(a_q.y), fabsf(a_q.z)), fmaxf(fabsf(b_p.x), fabsf(b_p.y)))),
-                          fmaxf(fmaxf(fabsf(b_p.z), fabsf(b_q.x)), fmaxf(fabsf(b_q.y), fabsf(b_q.z))));
-    const float tol = 1e-5f * E * E;
-    // first test: an end point of B within re of A's centre
-    const float cx = 0.5f * (a_p.x + a_q.x), cy = 0.5f * (a_p.y + a_q.y), cz = 0.5f * (a_p.z + a_q.z);
-    const float px = b_p.x - cx, py = b_p.y - cy, pz = b_p.z - cz;
-    const float qx = b_q.x - cx, qy = b_q.y - cy, qz = b_q.z - cz;
-    const float e2 = fminf(dot3(px, py, pz, px, py, pz), dot3(qx, qy, qz, qx, qy, qz));
-    const float re2 = re * re;
-    const bool end_no = e2 > fmaf(re2, 1.0001f, tol);
-    const bool end_yes = e2 < fmaf(re2, 0.9999f, -tol);
-    // second test: the segment distance below rd. (s, t): the line-line
-    // parameter s0, t optimal for it and clamped, s optimal for that t and
-    // clamped (the closest points when the segments' closest pair is interior
-    // or has t at an end; near them otherwise, which only loosens the bounds)
+__host__ __device__ inline bool seg_dist2_bounds(float4 a_p, float4 a_q, float4 b_p, float4 b_q, float tol, Rcp rcp,
+                                                 float& ub, float& lb) {
     const float v1x = a_q.x - a_p.x, v1y = a_q.y - a_p.y, v1z = a_q.z - a_p.z;
     const float v2x = b_q.x - b_p.x, v2y = b_q.y - b_p.y, v2z = b_q.z - b_p.z;
     const float rx = a_p.x - b_p.x, ry = a_p.y - b_p.y, rz = a_p.z - b_p.z;
     const float l1 = dot3(v1x, v1y, v1z, v1x, v1y, v1z), l2 = dot3(v2x, v2y, v2z, v2x, v2y, v2z);
-    // (near-)degenerate segments: the FP64 predicate decides, whatever it does
-    const bool degenerate = !(l1 > 1e-4f * tol) || !(l2 > 1e-4f * tol);
+    const bool ok = (l1 > 1e-4f * tol) && (l2 > 1e-4f * tol);
     const float f = dot3(v2x, v2y, v2z, rx, ry, rz), c = dot3(v1x, v1y, v1z, rx, ry, rz);
     const float b = dot3(v1x, v1y, v1z, v2x, v2y, v2z);
     const float den = fmaf(l1, l2, -b * b);
@@ -79,18 +70,55 @@ __host__ __device__ inline int classify_pair(float4 a_p, float4 a_q, float re, f
     const float wx = fmaf(-t, v2x, fmaf(s, v1x, rx));
     const float wy = fmaf(-t, v2y, fmaf(s, v1y, ry));
     const float wz = fmaf(-t, v2z, fmaf(s, v1z, rz));
-    const float ub = dot3(wx, wy, wz, wx, wy, wz);
+    ub = dot3(wx, wy, wz, wx, wy, wz);
     const float gs = 2.0f * dot3(wx, wy, wz, v1x, v1y, v1z);
     const float gt = -2.0f * dot3(wx, wy, wz, v2x, v2y, v2z);
-    const float lb = ub + fminf(-gs * s, fmaf(-gs, s, gs)) + fminf(-gt * t, fmaf(-gt, t, gt));
+    lb = ub + fminf(-gs * s, fmaf(-gs, s, gs)) + fminf(-gt * t, fmaf(-gt, t, gt));
+    return ok;
+}
+
+template <class Rcp>
+__host__ __device__ inline int classify_pair(float4 a_p, float4 a_q, float re, float rd, float4 b_p, float4 b_q,
+                                             float sx, float sy, Rcp rcp) {
+    // straight-line code (no early exits): a wave runs it for 64 pairs at once
+    b_p.x += sx;
+    b_p.y += sy;
+    b_q.x += sx;
+    b_q.y += sy;
+    const float E = coord_max(a_p, a_q, b_p, b_q);
+    const float tol = 1e-5f * E * E;
+    // first test: an end point of B within re of A's centre
+    const float cx = 0.5f * (a_p.x + a_q.x), cy = 0.5f * (a_p.y + a_q.y), cz = 0.5f * (a_p.z + a_q.z);
+    const float px = b_p.x - cx, py = b_p.y - cy, pz = b_p.z - cz;
+    const float qx = b_q.x - cx, qy = b_q.y - cy, qz = b_q.z - cz;
+    const float e2 = fminf(dot3(px, py, pz, px, py, pz), dot3(qx, qy, qz, qx, qy, qz));
+    const float re2 = re * re;
+    const bool end_no = e2 > fmaf(re2, 1.0001f, tol);
+    const bool end_yes = e2 < fmaf(re2, 0.9999f, -tol);
+    // second test: the segment distance below rd
+    float ub, lb;
+    const bool ok = seg_dist2_bounds(a_p, a_q, b_p, b_q, tol, rcp, ub, lb);
     const float rd2 = rd * rd;
     const bool dist_no = lb > fmaf(rd2, 1.0001f, tol);
     const bool dist_yes = ub < fmaf(rd2, 0.9999f, -tol);
     if (end_no) return 0;
-    if (degenerate) return -1;
+    if (!ok) return -1;  // (near-)degenerate segments: the FP64 predicate decides, whatever it does
     if (dist_no) return 0;
     if (end_yes && dist_yes) return 1;
     return -1;
+}
+
+// True when the segments' FP64 distance (ABM/Neighbours.cpp:107-160) is surely
+// at least sigma = R_A + R_B, i.e. the cells do not touch and F_cc (ABM/Forces.cpp:
+// 13-84) adds only signed zeros. End points relative to a common origin near the
+// pair (FP32 of the FP64 differences); margins as in classify_pair.
+template <class Rcp>
+__host__ __device__ inline bool surely_apart(float4 a_p, float4 a_q, float4 b_p, float4 b_q, float sigma, Rcp rcp) {
+    const float E = coord_max(a_p, a_q, b_p, b_q);
+    const float tol = 1e-5f * E * E;
+    float ub, lb;
+    const bool ok = seg_dist2_bounds(a_p, a_q, b_p, b_q, tol, rcp, ub, lb);
+    return ok && lb > fmaf(sigma * sigma, 1.0001f, tol);
 }
 
 }  // namespace bcg

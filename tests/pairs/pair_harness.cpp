@@ -33,4 +33,17 @@ int pc_classify_batch(int n, const float* rec, const float* thr, const float* sh
     }
     return 0;
 }
+
+// surely_apart over a batch: rec as above (end points relative to a common
+// origin), sigma: n values.
+int pc_apart_batch(int n, const float* rec, const float* sigma, int mode, int* out) {
+    const HostRcp rcp{mode};
+    for (int i = 0; i < n; i++) {
+        const float* r = rec + 16 * (long)i;
+        const float4 ap = make_float4(r[0], r[1], r[2], r[3]), aq = make_float4(r[4], r[5], r[6], r[7]);
+        const float4 bp = make_float4(r[8], r[9], r[10], r[11]), bq = make_float4(r[12], r[13], r[14], r[15]);
+        out[i] = bcg::surely_apart(ap, aq, bp, bq, sigma[i], rcp) ? 1 : 0;
+    }
+    return 0;
+}
 }

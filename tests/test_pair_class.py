@@ -1,6 +1,8 @@
-"""The list build's FP32 pair classification (pair_class.h, compiled for the
-host by tests/pairs/Makefile) against the oracle's restatement of
-reduceNeighbours' test (ABM/Neighbours.cpp:72-88).
+"""The FP32 pair decisions of pair_class.h, compiled for the host by
+tests/pairs/Makefile: the list build's classification against the oracle's
+restatement of reduceNeighbours' test (ABM/Neighbours.cpp:72-88), and the
+force gather's "surely not in contact" against the oracle's min_distance
+(ABM/Neighbours.cpp:107-160).
 
 Whenever the classifier decides a pair it must decide it as the FP64 predicate
 does; the pairs it leaves open go to the FP64 predicate in the kernel. Checked
@@ -32,6 +34,7 @@ def harness():
     lib = ctypes.CDLL(LIB)
     fp = ctypes.POINTER(ctypes.c_float)
     lib.pc_classify_batch.argtypes = [ctypes.c_int, fp, fp, fp, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]
+    lib.pc_apart_batch.argtypes = [ctypes.c_int, fp, fp, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]
     return lib
 
 
@@ -254,3 +257,71 @@ def test_a_planted_sign_error_is_caught(harness, oracle):
     wrong = ref.copy()
     wrong[decided[:3]] ^= 1
     assert np.any((got >= 0) & (got != wrong))
+
+
+# ------------------------------------------------------------ contact ---
+def apart(lib, a, b, mode):
+    """surely_apart on pairs (a[i], b[i]) with end points relative to a's
+    centre, as the force gather forms them (FP64 differences, then FP32)."""
+    o = 0.5 * (a["p"] + a["q"])
+    recs = np.zeros((len(a), 16), np.float32)
+    for k, (cells, e) in enumerate(((a, "p"), (a, "q"), (b, "p"), (b, "q"))):
+        recs[:, 4 * k:4 * k + 3] = (cells[e] - o).astype(np.float32)
+    sigma = np.ascontiguousarray((a["Radius"] + b["Radius"]).astype(np.float32))
+    out = np.zeros(len(a), np.int32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    lib.pc_apart_batch(len(a), np.ascontiguousarray(recs).ctypes.data_as(fp), sigma.ctypes.data_as(fp), mode,
+                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    return out
+
+
+def oracle_distance(a, b):
+    o = Oracle()
+    return np.array([o.min_distance(a[k:k + 1], b[k:k + 1])[0] for k in range(len(a))])
+
+
+def check_apart(lib, a, b, min_skipped=None):
+    d = oracle_distance(a, b)
+    sigma = a["Radius"] + b["Radius"]
+    for mode in (3, 1, 2, 0):
+        got = apart(lib, a, b, mode)
+        bad = np.nonzero((got == 1) & (d < sigma))[0]
+        assert len(bad) == 0, f"mode {mode}: {len(bad)} touching pairs called apart, e.g. d={d[bad[:3]]}"
+        if min_skipped is not None and mode != 3:
+            frac = got[d >= sigma * 1.01].mean()
+            assert frac >= min_skipped, f"mode {mode}: only {frac:.3f} of the clearly separate pairs called apart"
+    return d
+
+
+def test_contact_test_on_dome_neighbours(harness):
+    """Neighbour-like pairs (within Rc R): no touching pair is called apart,
+    and almost every clearly separate one is."""
+    rng = np.random.default_rng(61)
+    a, b = dome_pairs(rng, 30_000, reach=3.0)
+    check_apart(harness, a, b, min_skipped=0.99)
+
+
+@pytest.mark.parametrize("rel", [1e-2, 1e-5, 1e-9])
+def test_contact_at_the_touching_distance(harness, rel):
+    """Segment distances at (R_A + R_B)(1 +- rel)."""
+    rng = np.random.default_rng(71)
+    a, b = dome_pairs(rng, 3000, reach=3.0)
+    target = 2 * RADIUS * (1.0 + rel * rng.choice([-1.0, 1.0], len(a)))
+    a, b = at_distance(a, b, target)
+    check_apart(harness, a, b)
+
+
+def test_contact_near_parallel_and_crossing(harness):
+    rng = np.random.default_rng(81)
+    n = 5000
+    a, _ = dome_pairs(rng, n)
+    n = len(a)
+    ua = unit(a["q"] - a["p"])
+    ang = 10.0 ** rng.uniform(-8, -2, n)
+    ub = unit(ua + ang[:, None] * unit(np.cross(ua, rng.normal(size=(n, 3)))))
+    side = unit(np.cross(ua, rng.normal(size=(n, 3))))
+    dist = 2 * RADIUS * rng.uniform(0.9, 1.1, n)
+    dist[: n // 10] = 0.0  # crossing
+    ca = 0.5 * (a["p"] + a["q"])
+    b = cells_of(ca + dist[:, None] * side + rng.uniform(-3, 3, n)[:, None] * ua, ub, rng.uniform(2, 4, n))
+    check_apart(harness, a, b)
