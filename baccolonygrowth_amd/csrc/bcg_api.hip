@@ -85,11 +85,10 @@ PhysParams phys(const bcg_params& p) {
 
 // Exclusive scan of n ints (in -> out); *grand (device) receives the total.
 int scan(bcg_ctx* c, const int* in, int n, int* out, int* grand) {
-    const int nb = nblk(n, SCAN_BLOCK_ITEMS);
+    const int nb = nblk(n, TSCAN_ITEMS);
     if (nb > 1024 * 64) return fail(c, BCG_ERR_ARG, "scan of %d items too large", n);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, c->stream, in, n, c->scan_tot);
-    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(1024), 0, c->stream, c->scan_tot, nb, grand);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, c->stream, in, n, c->scan_tot, out);
+    hipLaunchKernelGGL(k_tscan_sums, dim3(nb), dim3(TSCAN_T), 0, c->stream, in, n, c->scan_tot);
+    hipLaunchKernelGGL(k_tscan_apply, dim3(nb), dim3(TSCAN_T), 0, c->stream, in, n, c->scan_tot, out, grand);
     HIPCHK(c, hipGetLastError());
     return 0;
 }

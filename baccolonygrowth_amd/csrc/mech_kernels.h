@@ -26,6 +26,10 @@ constexpr int QK_MASK = (1 << QK_BITS) - 1;
 __global__ void k_scan_reduce(const int* in, int n, int* tot);
 __global__ void k_scan_totals(int* tot, int nb, int* grand);
 __global__ void k_scan_apply(const int* in, int n, const int* tot, int* out);
+// the two-launch scan (k_tscan_sums, then k_tscan_apply): tiles of TSCAN_ITEMS
+constexpr int TSCAN_T = 256, TSCAN_ITEMS = 4096;
+__global__ void k_tscan_sums(const int* in, int n, int* tot);
+__global__ void k_tscan_apply(const int* in, int n, const int* tot, int* out, int* grand);
 
 __global__ void k_bin_count(const double* hot, int n, PhysParams P, int* key, int* cnt, int* slot, float4* cmL,
                             int* err);

@@ -100,6 +100,90 @@ __global__ void __launch_bounds__(SCAN_T) k_scan_apply(const int* __restrict__ i
         }
 }
 
+// Two-launch scan: tile sums, then every tile adds the sums of the tiles
+// before it (at most a few thousand, read from L2) and scans its own items.
+// A thread owns 16 consecutive items, read and written as four 16-byte
+// vectors when the arrays are 16-byte aligned. In place (in == out) is
+// allowed: a tile reads all its items before it writes any.
+constexpr int TSCAN_I = TSCAN_ITEMS / TSCAN_T;
+static_assert(TSCAN_I == 16, "four 16-byte vectors a thread");
+
+__device__ __forceinline__ void tscan_load(const int* __restrict__ in, int n, int base, bool vec, int (&v)[TSCAN_I]) {
+    if (vec && base + TSCAN_I <= n) {
+        const int4* p = reinterpret_cast<const int4*>(in + base);
+#pragma unroll
+        for (int q = 0; q < TSCAN_I / 4; q++) {
+            const int4 x = p[q];
+            v[4 * q] = x.x;
+            v[4 * q + 1] = x.y;
+            v[4 * q + 2] = x.z;
+            v[4 * q + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < TSCAN_I; i++) v[i] = base + i < n ? in[base + i] : 0;
+    }
+}
+
+// Sum of one int per thread over the block (blockDim = TSCAN_T).
+__device__ __forceinline__ int block_sum(int s, int* lds) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
+    __syncthreads();
+    int t = 0;
+#pragma unroll
+    for (int w = 0; w < TSCAN_T / 64; w++) t += lds[w];
+    __syncthreads();
+    return t;
+}
+
+__global__ void __launch_bounds__(TSCAN_T) k_tscan_sums(const int* __restrict__ in, int n, int* __restrict__ tot) {
+    __shared__ int lds[TSCAN_T / 64];
+    const bool vec = (reinterpret_cast<uintptr_t>(in) & 15) == 0;
+    int v[TSCAN_I];
+    tscan_load(in, n, blockIdx.x * TSCAN_ITEMS + threadIdx.x * TSCAN_I, vec, v);
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < TSCAN_I; i++) s += v[i];
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) tot[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(TSCAN_T) k_tscan_apply(const int* in, int n, const int* __restrict__ tot, int* out,
+                                                         int* __restrict__ grand) {
+    __shared__ int lds[96];
+    const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+    const int b = blockIdx.x;
+    int p = 0;
+    for (int i = threadIdx.x; i < b; i += TSCAN_T) p += tot[i];
+    const int prefix = block_sum(p, lds);
+    const int base = b * TSCAN_ITEMS + threadIdx.x * TSCAN_I;
+    int v[TSCAN_I];
+    tscan_load(in, n, base, vec, v);
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < TSCAN_I; i++) s += v[i];
+    int total;
+    int ex = prefix + block_excl_scan(s, lds, &total);
+#pragma unroll
+    for (int i = 0; i < TSCAN_I; i++) {
+        const int x = v[i];
+        v[i] = ex;
+        ex += x;
+    }
+    if (vec && base + TSCAN_I <= n) {
+        int4* q = reinterpret_cast<int4*>(out + base);
+#pragma unroll
+        for (int j = 0; j < TSCAN_I / 4; j++) q[j] = make_int4(v[4 * j], v[4 * j + 1], v[4 * j + 2], v[4 * j + 3]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < TSCAN_I; i++)
+            if (base + i < n) out[base + i] = v[i];
+    }
+    if (grand && b == (int)gridDim.x - 1 && threadIdx.x == 0) *grand = prefix + total;
+}
+
 // ------------------------------------------------------------- bins ---
 // Origin of bin column (x, y) for the FP32 prefilter records (any per-column
 // origin does; this one keeps local coordinates within about a bin).

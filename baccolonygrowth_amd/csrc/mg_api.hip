@@ -777,11 +777,11 @@ int dep_run(bcg_rd3d* c, Src src, int n, int32_t* n_inside) {
                            c->dcnt, c->drate);
     // exclusive scan of the bucket counts (the mechanics' scan kernels)
     const long ns = nn + 1;
-    const int nb = nblk(ns, SCAN_BLOCK_ITEMS);
+    const int nb = nblk(ns, TSCAN_ITEMS);
     if (nb > 1024 * 64) return fail(c, BCG_ERR_ARG, "grid too large for the deposition scan");
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, c->stream, c->dcnt, (int)ns, c->scan_tot);
-    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(1024), 0, c->stream, c->scan_tot, nb, (int*)nullptr);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, c->stream, c->dcnt, (int)ns, c->scan_tot, c->dstart);
+    hipLaunchKernelGGL(k_tscan_sums, dim3(nb), dim3(TSCAN_T), 0, c->stream, c->dcnt, (int)ns, c->scan_tot);
+    hipLaunchKernelGGL(k_tscan_apply, dim3(nb), dim3(TSCAN_T), 0, c->stream, c->dcnt, (int)ns, c->scan_tot, c->dstart,
+                       (int*)nullptr);
     if (n > 0) {
         hipLaunchKernelGGL(k_dep_scatter, dim3(nblk(n, 256)), dim3(256), 0, c->stream, c->dkey, n, c->dstart, c->dfill,
                            c->dsorted);
