@@ -229,8 +229,8 @@ int bcg_create(const bcg_params* p, bcg_ctx** out) {
     A(&c->hot[1], M * HOT);
     A(&c->time_p, M);
     A(&c->time_q, M);
-    A(&c->dynfric, 3 * M);
-    A(&c->stafric, 3 * M);
+    A(&c->fric, 6 * M);
+    A(&c->fric_id, M);
     A(&c->age_p, M);
     A(&c->age_q, M);
     A(&c->type, M);
@@ -248,7 +248,7 @@ int bcg_create(const bcg_params* p, bcg_ctx** out) {
     A(&c->nl_count, M);
     A(&c->order, M);
     A(&c->slot_of, M);
-    A(&c->divflag, M);
+    A(&c->divbits, (size_t)nblk(M, 32));
     A(&c->div_ids, M);
     A(&c->div_index_of, M);
     A(&c->ids_tmp, M);
@@ -286,6 +286,7 @@ int bcg_create(const bcg_params* p, bcg_ctx** out) {
     std::vector<int> neg(M, -1);
     hipMemcpy(c->div_index_of, neg.data(), M * sizeof(int), hipMemcpyHostToDevice);
     hipMemset(c->nl_count, 0, M * sizeof(int));
+    hipMemset(c->divbits, 0, nblk(M, 32) * sizeof(unsigned));
     hipMemset(c->hot[0], 0, M * HOT * sizeof(double));
     hipMemset(c->hot[1], 0, M * HOT * sizeof(double));
     c->rng.seed(1);
@@ -303,9 +304,9 @@ int bcg_destroy(bcg_ctx* c) {
         ctx_join(c);
         hipStreamSynchronize(c->stream);
     }
-    void* ptrs[] = {c->hot[0],   c->hot[1],    c->time_p,  c->time_q,     c->dynfric,   c->stafric,  c->age_p,
+    void* ptrs[] = {c->hot[0],   c->hot[1],    c->time_p,  c->time_q,     c->fric,      c->fric_id,  c->age_p,
                     c->age_q,    c->type,      c->ancestor, c->key,       c->bin_cnt,   c->bin_start, c->bin_slot,
-                    c->sorted,   c->scattered, c->scan_tot, c->cmL, c->cand,  c->nl,      c->nl_count,   c->order,     c->slot_of,  c->divflag,
+                    c->sorted,   c->scattered, c->scan_tot, c->cmL, c->cand,  c->nl,      c->nl_count,   c->order,     c->slot_of,  c->divbits,
                     c->div_ids,  c->div_index_of, c->ids_tmp, c->draws,   c->mother_pre, c->height,  c->normal,
                     c->wall,     c->gr1,       c->gr2,     c->Fbuf,       c->Tbuf,      c->Sbuf,      c->stats,
                     c->part_max, c->part_draws};
@@ -332,8 +333,8 @@ int bcg_set_cells(bcg_ctx* c, const bcg_cell* cells, int32_t n) {
     if (n > c->prm.maxCells) return fail(c, BCG_ERR_CAPACITY, "%d cells > maxCells %d", n, c->prm.maxCells);
     HIPCHK(c, ctx_join(c));
     HIPCHK(c, hipStreamSynchronize(c->stream));  // the copies below are synchronous on the null stream
-    std::vector<double> hot((size_t)n * HOT), tp(n), tq(n), df(3 * (size_t)n), sf(3 * (size_t)n);
-    std::vector<int> ap(n), aq(n), ty(n), an(n);
+    std::vector<double> hot((size_t)n * HOT), tp(n), tq(n), fr(6 * (size_t)n);
+    std::vector<int> ap(n), aq(n), ty(n), an(n), ident(n);
     for (int i = 0; i < n; i++) {
         const bcg_cell& s = cells[i];
         double* h = &hot[(size_t)i * HOT];
@@ -342,9 +343,10 @@ int bcg_set_cells(bcg_ctx* c, const bcg_cell* cells, int32_t n) {
             h[3 + k] = s.q[k];
             h[6 + k] = s.Velocity[k];
             h[9 + k] = s.AngularVelocity[k];
-            df[3 * i + k] = s.DynFric[k];
-            sf[3 * i + k] = s.StaFric[k];
+            fr[6 * (size_t)i + k] = s.DynFric[k];
+            fr[6 * (size_t)i + 3 + k] = s.StaFric[k];
         }
+        ident[i] = i;
         h[12] = s.Length;
         h[13] = s.Radius;
         h[14] = s.Ldiv;
@@ -364,14 +366,15 @@ int bcg_set_cells(bcg_ctx* c, const bcg_cell* cells, int32_t n) {
         HIPCHK(c, up(c->hot[0], hot.data(), hot.size() * sizeof(double)));
         HIPCHK(c, up(c->time_p, tp.data(), n * sizeof(double)));
         HIPCHK(c, up(c->time_q, tq.data(), n * sizeof(double)));
-        HIPCHK(c, up(c->dynfric, df.data(), df.size() * sizeof(double)));
-        HIPCHK(c, up(c->stafric, sf.data(), sf.size() * sizeof(double)));
+        HIPCHK(c, up(c->fric, fr.data(), fr.size() * sizeof(double)));
+        HIPCHK(c, up(c->fric_id, ident.data(), n * sizeof(int)));
         HIPCHK(c, up(c->age_p, ap.data(), n * sizeof(int)));
         HIPCHK(c, up(c->age_q, aq.data(), n * sizeof(int)));
         HIPCHK(c, up(c->type, ty.data(), n * sizeof(int)));
         HIPCHK(c, up(c->ancestor, an.data(), n * sizeof(int)));
     }
     HIPCHK(c, hipMemset(c->nl_count, 0, sizeof(int) * c->prm.maxCells));
+    HIPCHK(c, hipMemset(c->divbits, 0, sizeof(unsigned) * nblk(c->prm.maxCells, 32)));
     c->lists_valid = true;  // empty lists, as InitializeNeighbourList leaves them
     int rc = set_identity_slots(c);
     if (rc) return rc;
@@ -390,18 +393,27 @@ namespace {
 // record (the context's stream is idle when this is called).
 int download_cells(bcg_ctx* c, int buf, int N, bcg_cell* cells) {
     if (N == 0) return BCG_OK;
-    std::vector<double> hot((size_t)N * HOT), tp(N), tq(N), df(3 * (size_t)N), sf(3 * (size_t)N);
-    std::vector<int> ap(N), aq(N), ty(N), an(N);
+    std::vector<double> hot((size_t)N * HOT), tp(N), tq(N), fr(6 * (size_t)N);
+    std::vector<int> ap(N), aq(N), ty(N), an(N), fid(N);
     auto dn = [&](void* h, const void* d, size_t b) { return hipMemcpy(h, d, b, hipMemcpyDeviceToHost); };
     HIPCHK(c, dn(hot.data(), c->hot[buf], hot.size() * sizeof(double)));
     HIPCHK(c, dn(tp.data(), c->time_p, N * sizeof(double)));
     HIPCHK(c, dn(tq.data(), c->time_q, N * sizeof(double)));
-    HIPCHK(c, dn(df.data(), c->dynfric, df.size() * sizeof(double)));
-    HIPCHK(c, dn(sf.data(), c->stafric, sf.size() * sizeof(double)));
+    HIPCHK(c, dn(fr.data(), c->fric, fr.size() * sizeof(double)));
+    HIPCHK(c, dn(fid.data(), c->fric_id, N * sizeof(int)));
     HIPCHK(c, dn(ap.data(), c->age_p, N * sizeof(int)));
     HIPCHK(c, dn(aq.data(), c->age_q, N * sizeof(int)));
     HIPCHK(c, dn(ty.data(), c->type, N * sizeof(int)));
     HIPCHK(c, dn(an.data(), c->ancestor, N * sizeof(int)));
+    // frictions are held by slot: slot t belongs to cell fid[t]
+    for (int t = 0; t < N; t++) {
+        const int i = fid[t];
+        if (i < 0 || i >= N) return fail(c, BCG_ERR_HIP, "friction slot %d names cell %d", t, i);
+        for (int k = 0; k < 3; k++) {
+            cells[i].DynFric[k] = fr[6 * (size_t)t + k];
+            cells[i].StaFric[k] = fr[6 * (size_t)t + 3 + k];
+        }
+    }
     for (int i = 0; i < N; i++) {
         bcg_cell& s = cells[i];
         const double* h = &hot[(size_t)i * HOT];
@@ -410,8 +422,6 @@ int download_cells(bcg_ctx* c, int buf, int N, bcg_cell* cells) {
             s.q[k] = h[3 + k];
             s.Velocity[k] = h[6 + k];
             s.AngularVelocity[k] = h[9 + k];
-            s.DynFric[k] = df[3 * i + k];
-            s.StaFric[k] = sf[3 * i + k];
         }
         s.Length = h[12];
         s.Radius = h[13];
@@ -633,16 +643,16 @@ int step_body(bcg_ctx* c, double dt, double t, int32_t flags, int32_t* n_div, in
         const int nb = nblk(N0, MOVE_BLOCK);
         hipLaunchKernelGGL(k_move_grow, dim3(nb), dim3(MOVE_BLOCK), 0, c->stream, c->hot[in], c->hot[out], N0, c->P,
                            dt, c->order, c->nl, c->nl_count, c->stride, c->height, c->normal, c->wall, c->gr1, c->gr2,
-                           c->type, c->dynfric, c->stafric, c->divflag, c->part_draws, &c->stats->err);
+                           c->type, c->fric, c->fric_id, c->divbits, c->part_draws, &c->stats->err);
         hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(1024), 0, c->stream, c->part_draws, nb,
                            &c->stats->wall_draws);
     }
     HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
     if (N0 > 0) {
-        const int nb = nblk(N0, 256);
-        hipLaunchKernelGGL(k_flag_count, dim3(nb), dim3(256), 0, c->stream, c->divflag, N0, c->ids_tmp);
+        const int nw = nblk(N0, 32), nb = nblk(nw, 256);
+        hipLaunchKernelGGL(k_flag_count, dim3(nb), dim3(256), 0, c->stream, c->divbits, nw, c->ids_tmp);
         if ((rc = scan(c, c->ids_tmp, nb, c->ids_tmp, &c->stats->ndiv))) return rc;
-        hipLaunchKernelGGL(k_flag_write, dim3(nb), dim3(256), 0, c->stream, c->divflag, N0, c->ids_tmp, c->div_ids);
+        hipLaunchKernelGGL(k_flag_write, dim3(nb), dim3(256), 0, c->stream, c->divbits, nw, c->ids_tmp, c->div_ids);
         HIPCHK(c, hipGetLastError());
     }
     HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
@@ -663,7 +673,7 @@ int step_body(bcg_ctx* c, double dt, double t, int32_t flags, int32_t* n_div, in
         HIPCHK(c, hipMemcpyAsync(c->draws, u.data(), u.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
         hipLaunchKernelGGL(k_divide, dim3(nblk(D, 64)), dim3(64), 0, c->stream, c->hot[out], N0, D, c->div_ids,
                            c->draws, c->P, t, c->mother_pre, c->div_index_of, c->time_p, c->time_q, c->age_p,
-                           c->age_q, c->type, c->ancestor, c->dynfric, c->stafric, c->order, c->slot_of,
+                           c->age_q, c->type, c->ancestor, c->fric, c->fric_id, c->order, c->slot_of,
                            c->nl_count);
         HIPCHK(c, hipGetLastError());
         if (!(flags & 2)) {

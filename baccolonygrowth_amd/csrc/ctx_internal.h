@@ -33,14 +33,19 @@ struct bcg_ctx {
     // buffers and the lists no longer agree); cleared by bcg_set_cells
     bool failed = false;
     double* hot[2] = {nullptr, nullptr};
-    double *time_p = nullptr, *time_q = nullptr, *dynfric = nullptr, *stafric = nullptr;
+    double *time_p = nullptr, *time_q = nullptr;
+    // DynFric, StaFric of the cell in each slot of the last move (6 doubles a
+    // slot) and that cell's ID
+    double* fric = nullptr;
+    int* fric_id = nullptr;
     int *age_p = nullptr, *age_q = nullptr, *type = nullptr, *ancestor = nullptr;
     int *key = nullptr, *bin_cnt = nullptr, *bin_start = nullptr, *bin_slot = nullptr, *sorted = nullptr;
     int* scattered = nullptr;  // bin order before the in-bin ranking
     int* scan_tot = nullptr;
     float4 *cmL = nullptr, *cand = nullptr;  // FP32 end points by ID, by bin-order slot
     int *nl = nullptr, *nl_count = nullptr, *order = nullptr, *slot_of = nullptr;
-    int *divflag = nullptr, *div_ids = nullptr, *div_index_of = nullptr, *ids_tmp = nullptr;
+    unsigned* divbits = nullptr;  // division flags, a bit per cell ID (zero between steps)
+    int *div_ids = nullptr, *div_index_of = nullptr, *ids_tmp = nullptr;
     double *draws = nullptr, *mother_pre = nullptr;
     double *height = nullptr, *normal = nullptr, *wall = nullptr, *gr1 = nullptr, *gr2 = nullptr;
     double *Fbuf = nullptr, *Tbuf = nullptr;

@@ -49,7 +49,7 @@ __global__ void k_gather_ids(const double* hot, const int* ids, int nid, PhysPar
 __global__ void k_move_grow(const double* hot_old, double* hot_new, int n, PhysParams P, double dt, const int* order,
                             const int* nl, const int* nl_count, int stride, const double* H, const double* Nrm,
                             const double* wall, const double* gr1, const double* gr2, const int* type,
-                            double* dynfric, double* stafric, int* divflag, unsigned* block_draws, int* err);
+                            double* fric, int* fric_id, unsigned* divbits, unsigned* block_draws, int* err);
 __global__ void k_sum_forces(const double* hot, int n, PhysParams P, const int* order, const int* nl,
                              const int* nl_count, int stride, const double* H, const double* Nrm, const double* wall,
                              double* Fo, double* To, int* err);
@@ -57,12 +57,12 @@ __global__ void k_mean_stress(const double* hot, int n, PhysParams P, const int*
                               const int* nl_count, int stride, const double* H, const double* Nrm, const double* wall,
                               double* S5, double* F3, unsigned* wave_draws, int* err);
 
-__global__ void k_flag_count(const int* flag, int n, int* tot);
-__global__ void k_flag_write(const int* flag, int n, const int* off, int* ids);
+__global__ void k_flag_count(const unsigned* bits, int nw, int* tot);
+__global__ void k_flag_write(unsigned* bits, int nw, const int* off, int* ids);
 
 __global__ void k_divide(double* hot, int n0, int D, const int* div_ids, const double* u, PhysParams P, double t,
                          double* mother_pre, int* div_index_of, double* time_p, double* time_q, int* age_p,
-                         int* age_q, int* type, int* ancestor, double* dynfric, double* stafric, int* order,
+                         int* age_q, int* type, int* ancestor, double* fric, int* fric_id, int* order,
                          int* slot_of, int* nl_count);
 
 // Position of cell j as the reference's grid and cell array hold it just

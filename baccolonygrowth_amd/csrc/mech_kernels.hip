@@ -906,7 +906,7 @@ __global__ void __launch_bounds__(MOVE_BLOCK) k_move_grow(
     const int* __restrict__ order, const int* __restrict__ nl, const int* __restrict__ nl_count, int stride,
     const double* __restrict__ H, const double* __restrict__ Nrm, const double* __restrict__ wall,
     const double* __restrict__ gr1, const double* __restrict__ gr2, const int* __restrict__ type,
-    double* __restrict__ dynfric, double* __restrict__ stafric, int* __restrict__ divflag,
+    double* __restrict__ fric, int* __restrict__ fric_id, unsigned* __restrict__ divbits,
     unsigned* __restrict__ block_draws, int* __restrict__ err) {
     __shared__ MoveLds L;
     const int tid = threadIdx.x;
@@ -930,7 +930,9 @@ __global__ void __launch_bounds__(MOVE_BLOCK) k_move_grow(
             const double2* src = reinterpret_cast<const double2*>(hot_old + (size_t)id * HOT);
             double2* dst = reinterpret_cast<double2*>(hot_new + (size_t)id * HOT);
             for (int c = 0; c < HOT / 2; c++) dst[c] = src[c];
-            divflag[id] = 0;
+            double2* fo = reinterpret_cast<double2*>(fric + 6 * (size_t)t);
+            fo[0] = fo[1] = fo[2] = make_double2(0.0, 0.0);
+            fric_id[t] = id;
         } else {
             live = true;
             wall_z = wall[x * P.BoxY + y];
@@ -1002,13 +1004,13 @@ __global__ void __launch_bounds__(MOVE_BLOCK) k_move_grow(
         out[5] = make_double2(K2.w.y, K2.w.z);
         out[6] = make_double2(S1.L, S1.R);
         out[7] = make_double2(Ldiv, GR);
-        dynfric[3 * (size_t)id] = df.x;
-        dynfric[3 * (size_t)id + 1] = df.y;
-        dynfric[3 * (size_t)id + 2] = df.z;
-        stafric[3 * (size_t)id] = sf.x;
-        stafric[3 * (size_t)id + 1] = sf.y;
-        stafric[3 * (size_t)id + 2] = sf.z;
-        divflag[id] = (S1.L >= Ldiv) ? 1 : 0;  // GrowCell: ABM/Cells.cpp:48
+        // DynFric, StaFric in slot order (coalesced), with the cell's ID
+        double2* fo = reinterpret_cast<double2*>(fric + 6 * (size_t)t);
+        fo[0] = make_double2(df.x, df.y);
+        fo[1] = make_double2(df.z, sf.x);
+        fo[2] = make_double2(sf.y, sf.z);
+        fric_id[t] = id;
+        if (S1.L >= Ldiv) atomicOr(&divbits[id >> 5], 1u << (id & 31));  // GrowCell: ABM/Cells.cpp:48
     }
     // the block's count of F_cw draws (integers: the fold's order is free)
     int w = draws;
@@ -1162,31 +1164,31 @@ __global__ void __launch_bounds__(MOVE_BLOCK) k_mean_stress(const double* __rest
 }
 
 // ----------------------------------------------------------- compact ---
-// Per-block count of flagged cells (wave ballot + popcount).
-__global__ void __launch_bounds__(256) k_flag_count(const int* __restrict__ flag, int n, int* __restrict__ tot) {
+// Dividing cells from the flag bitmap (a word of 32 IDs a thread): the
+// number per block, then the IDs in ascending order (block offset + rank in
+// the block); the second pass clears the words it read.
+__global__ void __launch_bounds__(256) k_flag_count(const unsigned* __restrict__ bits, int nw, int* __restrict__ tot) {
     __shared__ int lds[4];
     const int i = blockIdx.x * 256 + threadIdx.x;
-    unsigned long long b = __ballot(i < n && flag[i] != 0);
-    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = __popcll(b);
+    int c = i < nw ? __popc(bits[i]) : 0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = c;
     __syncthreads();
     if (threadIdx.x == 0) tot[blockIdx.x] = lds[0] + lds[1] + lds[2] + lds[3];
 }
 
-// Write flagged IDs in ascending order: block offset + wave offset + lane rank.
-__global__ void __launch_bounds__(256) k_flag_write(const int* __restrict__ flag, int n, const int* __restrict__ off,
+__global__ void __launch_bounds__(256) k_flag_write(unsigned* __restrict__ bits, int nw, const int* __restrict__ off,
                                                     int* __restrict__ ids) {
-    __shared__ int lds[4];
+    __shared__ int lds[96];
     const int i = blockIdx.x * 256 + threadIdx.x;
-    const bool f = i < n && flag[i] != 0;
-    unsigned long long b = __ballot(f);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) lds[w] = __popcll(b);
-    __syncthreads();
-    int woff = 0;
-    for (int k = 0; k < w; k++) woff += lds[k];
-    if (f) {
-        unsigned long long below = lane ? (b & ((~0ull) >> (64 - lane))) : 0ull;
-        ids[off[blockIdx.x] + woff + __popcll(below)] = i;
+    unsigned w = i < nw ? bits[i] : 0u;
+    int o = off[blockIdx.x] + block_excl_scan(__popc(w), lds, nullptr);
+    if (w) bits[i] = 0u;
+    while (w) {
+        const int b = __builtin_ctz(w);
+        ids[o++] = (i << 5) + b;
+        w &= w - 1;
     }
 }
 
@@ -1198,7 +1200,7 @@ __global__ void k_divide(double* __restrict__ hot, int n0, int D, const int* __r
                          const double* __restrict__ u, PhysParams P, double t, double* __restrict__ mother_pre,
                          int* __restrict__ div_index_of, double* __restrict__ time_p, double* __restrict__ time_q,
                          int* __restrict__ age_p, int* __restrict__ age_q, int* __restrict__ type,
-                         int* __restrict__ ancestor, double* __restrict__ dynfric, double* __restrict__ stafric,
+                         int* __restrict__ ancestor, double* __restrict__ fric, int* __restrict__ fric_id,
                          int* __restrict__ order, int* __restrict__ slot_of, int* __restrict__ nl_count) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= D) return;
@@ -1251,10 +1253,8 @@ __global__ void k_divide(double* __restrict__ hot, int n0, int D, const int* __r
     ancestor[d] = ancestor[m];
     const double mLdiv = P.L_divide / 2 + P.cellRadius + mL;
     const double dLdiv = P.L_divide / 2 + P.cellRadius + dLen;
-    for (int c = 0; c < 3; c++) {
-        dynfric[3 * (size_t)d + c] = 0;
-        stafric[3 * (size_t)d + c] = 0;
-    }
+    for (int c = 0; c < 6; c++) fric[6 * (size_t)d + c] = 0;  // the daughter's slot is its ID
+    fric_id[d] = d;
     hm[0] = mp.x; hm[1] = mp.y; hm[2] = mp.z;
     hm[3] = mq.x; hm[4] = mq.y; hm[5] = mq.z;
     hm[9] = 0; hm[10] = 0; hm[11] = 0;

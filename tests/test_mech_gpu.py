@@ -59,7 +59,7 @@ def _check_cells(a, b, label):
     for f in ("p", "q"):
         d = np.abs(a[f] - b[f]).max()
         assert d <= POS_RTOL * max(np.abs(b[f]).max(), 1.0), (label, f, d)
-    for f in ("Velocity", "AngularVelocity"):
+    for f in ("Velocity", "AngularVelocity", "DynFric", "StaFric"):
         d = np.abs(a[f] - b[f]).max()
         assert d <= 1e-8 * max(np.abs(b[f]).max(), 1.0), (label, f, d)
 
