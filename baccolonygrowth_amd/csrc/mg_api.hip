@@ -101,6 +101,17 @@ __global__ void k_dep_scatter(const int* __restrict__ key, int n, const int* __r
     sorted[start[k] + atomicAdd(&fill[k], 1)] = c;
 }
 
+// The same placement from the bucket counts themselves, counted back down
+// to zero (the next deposition needs no clearing pass).
+__global__ void k_dep_scatter_dec(const int* __restrict__ key, int n, const int* __restrict__ start,
+                                  int* __restrict__ cnt, int* __restrict__ sorted) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int k = key[c];
+    if (k < 0) return;
+    sorted[start[k] + atomicSub(&cnt[k], 1) - 1] = c;
+}
+
 __global__ void k_dep_sort(const int* __restrict__ start, long nn, int* __restrict__ sorted) {
     const long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nn) return;
@@ -118,8 +129,11 @@ __global__ void k_dep_sort(const int* __restrict__ start, long nn, int* __restri
 
 // One vertex per thread: the sum over its 8 base-vertex buckets (planes k,
 // then k-1; rows j, then j-1; columns i, then i-1; each bucket's cells in
-// ascending order). (Measured: 64 us on 257 x 257 x 65; a column walk that
-// reuses each plane's bucket starts took 86 us.)
+// ascending order). The 8 bucket ranges are read first, all loads in flight
+// at once (three starts a row serve both columns); a plane of buckets with
+// no cell (its first and last start equal) is skipped as a whole, as a
+// colony fills only the lowest planes of its grid. (A column walk that
+// reuses each plane's bucket starts was slower.)
 __global__ void __launch_bounds__(256) k_dep_gather(const int* __restrict__ start, const int* __restrict__ sorted,
                                                     const double* __restrict__ frac, const double* __restrict__ rate,
                                                     int nx, int ny, int nz, double inv_vol, double* __restrict__ q) {
@@ -127,24 +141,39 @@ __global__ void __launch_bounds__(256) k_dep_gather(const int* __restrict__ star
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = blockIdx.z;
     if (i >= nx || j >= ny) return;
-    double acc = 0.0;
+    const long nxy = (long)nx * ny;
+    int sb[8], eb[8];  // bucket ranges, index dz * 4 + dy * 2 + dx
+#pragma unroll
     for (int dz = 0; dz <= 1; dz++) {
         const int bk = k - dz;
-        if (bk < 0 || bk >= nz - 1) continue;
+        bool zok = bk >= 0 && bk < nz - 1;
+        if (zok) zok = start[bk * nxy] != start[(bk + 1) * nxy];
+#pragma unroll
         for (int dy = 0; dy <= 1; dy++) {
             const int bj = j - dy;
-            if (bj < 0 || bj >= ny - 1) continue;
-            for (int dx = 0; dx <= 1; dx++) {
-                const int bi = i - dx;
-                if (bi < 0 || bi >= nx - 1) continue;
-                const long b = ((long)bk * ny + bj) * nx + bi;
-                for (int s = start[b]; s < start[b + 1]; s++) {
-                    const int c = sorted[s];
-                    const double fx = frac[3 * c], fy = frac[3 * c + 1], fz = frac[3 * c + 2];
-                    const double wx = dx ? fx : 1.0 - fx, wy = dy ? fy : 1.0 - fy, wz = dz ? fz : 1.0 - fz;
-                    acc += rate[c] * (wx * wy * wz);
-                }
-            }
+            const bool yok = zok && bj >= 0 && bj < ny - 1;
+            const bool x0 = yok && i < nx - 1;  // bucket column i
+            const bool x1 = yok && i >= 1;      // bucket column i - 1
+            const long b = ((long)bk * ny + bj) * nx + i;
+            const int sm = x1 ? start[b - 1] : 0;
+            const int s0 = (x0 || x1) ? start[b] : 0;
+            const int sp = x0 ? start[b + 1] : 0;
+            const int m = dz * 4 + dy * 2;
+            sb[m] = s0;
+            eb[m] = x0 ? sp : s0;
+            sb[m + 1] = sm;
+            eb[m + 1] = x1 ? s0 : sm;
+        }
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        const int dz = m >> 2, dy = (m >> 1) & 1, dx = m & 1;
+        for (int s = sb[m]; s < eb[m]; s++) {
+            const int c = sorted[s];
+            const double fx = frac[3 * c], fy = frac[3 * c + 1], fz = frac[3 * c + 2];
+            const double wx = dx ? fx : 1.0 - fx, wy = dy ? fy : 1.0 - fy, wz = dz ? fz : 1.0 - fz;
+            acc += rate[c] * (wx * wy * wz);
         }
     }
     q[((long)k * ny + j) * nx + i] = acc * inv_vol;
@@ -203,7 +232,8 @@ struct bcg_rd3d {
     double* dnorm = nullptr;
     double* hnorm = nullptr;  // pinned
     // deposition scratch
-    int *dkey = nullptr, *dcnt = nullptr, *dstart = nullptr, *dfill = nullptr, *dsorted = nullptr, *scan_tot = nullptr;
+    int *dkey = nullptr, *dcnt = nullptr, *dstart = nullptr, *dsorted = nullptr, *scan_tot = nullptr;
+    bool dcnt_zero = false;  // every bucket count is zero (set after a completed deposition)
     double *dfrac = nullptr, *dcm = nullptr, *drate = nullptr;
     int dcap = 0;
     hipEvent_t ev[5];  // [0..3] timing, [4] ordering against a mechanics context
@@ -284,6 +314,12 @@ constexpr long BOTTOM_MAX_POINTS = 4096;
 // below it the two-launch sweep has more parallelism per launch.
 constexpr long FUSED_MIN_POINTS = 1L << 21;
 constexpr long WIDE_MIN_POINTS = 1L << 24;  // FP32 levels that get the 62 x 30 sweep tile
+// Levels of at most this many points take the prolongation inside their
+// first red sweep (measured: 5.3 us against 4.7 + 4.6 us for the two
+// launches on 65 x 65 x 17 and 33 x 33 x 9; 18 us against 5 + 6 us on
+// 129 x 129 x 33, where the six interpolations a red point repeats cost more
+// than the launch).
+constexpr long PROLONG_FUSED_MAX_POINTS = 1L << 17;
 
 // Blocks of the ||b||^2 reduction (k_sumsq): one partial sum each in c->part,
 // which holds at least this many.
@@ -346,6 +382,37 @@ void smooth(bcg_rd3d* c, Level& v, int sweeps, bool zero = false) {
                                v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly, (T)v.lz, color);
 }
 
+// Split sweeps on a small level (one launch per colour), the first red sweep
+// as `first` says: a plain sweep, from u = 0 (k_rbgs_zero), or fused with
+// the prolongation from level w (k_rbgs_prolong). With no sweeps, u is
+// cleared or the correction added, as the fused forms would have done.
+enum class FirstRed { plain, zero, prolong };
+template <class T>
+void smooth_small(bcg_rd3d* c, Level& v, int sweeps, FirstRed first, Level* w = nullptr) {
+    T* u = Arr<T>::u(v);
+    if (sweeps == 0) {
+        if (first == FirstRed::zero)
+            hipLaunchKernelGGL((k_fill<T>), dim3(nblk(v.n(), 256)), dim3(256), 0, c->stream, u, v.n(), (T)0);
+        else if (first == FirstRed::prolong)
+            hipLaunchKernelGGL((k_prolong<T>), slab_grid(v), dim3(64, 4), 0, c->stream, u, Arr<T>::u(*w), v.nx, v.ny,
+                               v.nz, w->nx, w->ny, v.cx, v.cy, v.cz);
+        return;
+    }
+    for (int s = 0; s < sweeps; s++)
+        for (int color = 0; color < 2; color++) {
+            if (s == 0 && color == 0 && first == FirstRed::zero)
+                hipLaunchKernelGGL((k_rbgs_zero<T>), grid_rb(v), dim3(64, 4), 0, c->stream, u, Arr<T>::f(v),
+                                   Arr<T>::a0(v), v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly, (T)v.lz);
+            else if (s == 0 && color == 0 && first == FirstRed::prolong)
+                hipLaunchKernelGGL((k_rbgs_prolong<T>), grid_rb(v), dim3(64, 4), 0, c->stream, u, Arr<T>::f(v),
+                                   Arr<T>::a0(v), Arr<T>::u(*w), v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly, (T)v.lz, w->nx,
+                                   w->ny, v.cx, v.cy, v.cz);
+            else
+                hipLaunchKernelGGL((k_rbgs<T>), grid_rb(v), dim3(64, 4), 0, c->stream, u, Arr<T>::f(v), Arr<T>::a0(v),
+                                   v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly, (T)v.lz, color);
+        }
+}
+
 // The bottom levels by the single-workgroup FP64 solver, with level 0 on
 // (u0, f0), or, in an FP32 cycle, read from and written back to (u32, f32)
 // with (u0, f0) as working copies; `zero`: from u = 0.
@@ -387,6 +454,26 @@ void vcycle(bcg_rd3d* c, int l, bool zero = false) {
         return;
     }
     Level& w = c->L[l + 1];
+    if (c->prm.smoother == BCG_RD3D_SMOOTH_FUSED && v.n() < FUSED_MIN_POINTS) {
+        // small level: no clearing pass before the first sweep and, where the
+        // level is small enough for its redundant interpolations to cost less
+        // than a launch, the prolongation inside the first red sweep; the
+        // same values as below
+        smooth_small<T>(c, v, c->prm.nu1, zero ? FirstRed::zero : FirstRed::plain);
+        hipLaunchKernelGGL((k_residual<T>), slab_grid(v), dim3(64, 4), 0, c->stream, Arr<T>::u(v), Arr<T>::f(v),
+                           Arr<T>::a0(v), Arr<T>::r(v), v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly, (T)v.lz);
+        hipLaunchKernelGGL((k_restrict<T>), slab_grid(w), dim3(64, 4), 0, c->stream, Arr<T>::r(v), Arr<T>::f(w), v.nx,
+                           v.ny, v.nz, w.nx, w.ny, w.nz, v.cx, v.cy, v.cz);
+        vcycle<T>(c, l + 1, true);
+        if (v.n() <= PROLONG_FUSED_MAX_POINTS) {
+            smooth_small<T>(c, v, c->prm.nu2, FirstRed::prolong, &w);
+        } else {
+            hipLaunchKernelGGL((k_prolong<T>), slab_grid(v), dim3(64, 4), 0, c->stream, Arr<T>::u(v), Arr<T>::u(w),
+                               v.nx, v.ny, v.nz, w.nx, w.ny, v.cx, v.cy, v.cz);
+            smooth_small<T>(c, v, c->prm.nu2, FirstRed::plain);
+        }
+        return;
+    }
     smooth<T>(c, v, c->prm.nu1, zero);
     // z-marching residual on the large levels
     if (v.n() >= FUSED_MIN_POINTS) {
@@ -642,7 +729,7 @@ int bcg_rd3d_destroy(bcg_rd3d* c) {
     drop_graphs(c);
     std::vector<void*> ptrs = {c->x[0], c->x[1], c->cprev, c->b, c->q, c->save, c->grad, c->gpart, c->dout, c->dfield,
                                c->part, c->part2, c->bot_u0, c->bot_f0, c->dnorm, c->dkey, c->dcnt,
-                               c->dstart, c->dfill, c->dsorted, c->scan_tot, c->dfrac, c->dcm, c->drate};
+                               c->dstart, c->dsorted, c->scan_tot, c->dfrac, c->dcm, c->drate};
     for (size_t l = 0; l < c->L.size(); l++) {
         Level& v = c->L[l];
         if (l > 0) {
@@ -756,7 +843,6 @@ int dep_reserve(bcg_rd3d* c, int n, int want) {
         };
         A(&c->dcnt, nn + 1);
         A(&c->dstart, nn + 1);
-        A(&c->dfill, nn);
         A(&c->scan_tot, 1024 * 64);
         if (rc) return rc;
     }
@@ -769,8 +855,10 @@ template <class Src>
 int dep_run(bcg_rd3d* c, Src src, int n, int32_t* n_inside) {
     Level& v = c->L[0];
     const long nn = c->N;
-    HIPCHK(c, hipMemsetAsync(c->dcnt, 0, sizeof(int) * (nn + 1), c->stream));
-    HIPCHK(c, hipMemsetAsync(c->dfill, 0, sizeof(int) * nn, c->stream));
+    // the counts return to zero in the scatter below; cleared here only after
+    // a deposition that stopped before it
+    if (!c->dcnt_zero) HIPCHK(c, hipMemsetAsync(c->dcnt, 0, sizeof(int) * (nn + 1), c->stream));
+    c->dcnt_zero = n == 0;
     if (n > 0)
         hipLaunchKernelGGL(k_dep_key<Src>, dim3(nblk(n, 256)), dim3(256), 0, c->stream, src, n, c->prm.origin[0],
                            c->prm.origin[1], c->prm.origin[2], 1.0 / c->prm.h, v.nx, v.ny, v.nz, c->dkey, c->dfrac,
@@ -783,8 +871,9 @@ int dep_run(bcg_rd3d* c, Src src, int n, int32_t* n_inside) {
     hipLaunchKernelGGL(k_tscan_apply, dim3(nb), dim3(TSCAN_T), 0, c->stream, c->dcnt, (int)ns, c->scan_tot, c->dstart,
                        (int*)nullptr);
     if (n > 0) {
-        hipLaunchKernelGGL(k_dep_scatter, dim3(nblk(n, 256)), dim3(256), 0, c->stream, c->dkey, n, c->dstart, c->dfill,
-                           c->dsorted);
+        hipLaunchKernelGGL(k_dep_scatter_dec, dim3(nblk(n, 256)), dim3(256), 0, c->stream, c->dkey, n, c->dstart,
+                           c->dcnt, c->dsorted);
+        c->dcnt_zero = true;
         hipLaunchKernelGGL(k_dep_sort, dim3(nblk(nn, 256)), dim3(256), 0, c->stream, c->dstart, nn, c->dsorted);
     }
     hipLaunchKernelGGL(k_dep_gather, grid2(v), dim3(64, 4), 0, c->stream, c->dstart, c->dsorted, c->dfrac, c->drate,

@@ -47,6 +47,13 @@ __global__ void k_restrict(const T* r, T* fc, int nx, int ny, int nz, int ncx, i
                            int cz);
 template <class T>
 __global__ void k_prolong(T* u, const T* ec, int nx, int ny, int nz, int ncx, int ncy, int cx, int cy, int cz);
+// small levels: the red sweep from u = 0; the prolongation + the red sweep
+// after it
+template <class T>
+__global__ void k_rbgs_zero(T* u, const T* f, const T* a0, int nx, int ny, int nz, T lx, T ly, T lz);
+template <class T>
+__global__ void k_rbgs_prolong(T* u, const T* f, const T* a0, const T* ec, int nx, int ny, int nz, T lx, T ly, T lz,
+                               int ncx, int ncy, int cx, int cy, int cz);
 template <class T>
 __global__ void k_zm_restrict(const T* r, T* fc, int nx, int ny, int nz, int ncx, int ncy, int ncz, int cx, int cy, int cz,
                               int kchunk);

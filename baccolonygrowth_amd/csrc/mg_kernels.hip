@@ -90,6 +90,20 @@ __global__ void __launch_bounds__(256) k_rbgs(T* __restrict__ u, const T* __rest
     (void)sxy;
 }
 
+// r = f - A u at (i, j, k).
+template <class T>
+__device__ __forceinline__ T residual_point(const T* __restrict__ u, const T* __restrict__ f, const T* __restrict__ a0,
+                                            int i, int j, int k, int nx, int ny, int nz, T lx, T ly, T lz) {
+    if (k == 0) {
+        const long p = (long)j * nx + i;
+        return f[p] - u[p];
+    }
+    long p;
+    const T s = neigh_sum<T>(u, i, j, k, nx, ny, nz, lx, ly, lz, p);
+    const T d = a0[p] + T(2) * (lx + ly + lz);
+    return f[p] - (d * u[p] - s);
+}
+
 template <class T>
 __global__ void __launch_bounds__(256) k_residual(const T* __restrict__ u, const T* __restrict__ f,
                                                   const T* __restrict__ a0, T* __restrict__ r, int nx, int ny, int nz,
@@ -100,25 +114,17 @@ __global__ void __launch_bounds__(256) k_residual(const T* __restrict__ u, const
     const int j = by * blockDim.y + threadIdx.y;
     const int k = bz;
     if (i >= nx || j >= ny) return;
-    if (k == 0) {
-        const long p = (long)j * nx + i;
-        r[p] = f[p] - u[p];
-        return;
-    }
-    long p;
-    const T s = neigh_sum<T>(u, i, j, k, nx, ny, nz, lx, ly, lz, p);
-    const T d = a0[p] + T(2) * (lx + ly + lz);
-    r[p] = f[p] - (d * u[p] - s);
+    r[((long)k * ny + j) * nx + i] = residual_point<T>(u, f, a0, i, j, k, nx, ny, nz, lx, ly, lz);
 }
 
 // Full weighting at coarse point (I,J,K) (K >= 1); cx/cy/cz = 1 where that
-// direction is coarsened, Neumann faces mirrored.
-template <class T>
-__device__ __forceinline__ T restrict_point(const T* __restrict__ r, int nx, int ny, int nz, int I, int J, int K, int cx,
-                                            int cy, int cz) {
+// direction is coarsened, Neumann faces mirrored. rv(i, j, k): the fine
+// residual at a fine point.
+template <class T, class RV>
+__device__ __forceinline__ T restrict_point_of(RV rv, int nx, int ny, int nz, int I, int J, int K, int cx, int cy,
+                                               int cz) {
     if (K == 0) return T(0);  // Dirichlet rows: zero correction
     const int i0 = cx ? 2 * I : I, j0 = cy ? 2 * J : J, k0 = cz ? 2 * K : K;
-    const long sxy = (long)nx * ny;
     T acc = T(0);
     for (int c = -cz; c <= cz; c++) {
         const int kk = mirror(k0 + c, nz);
@@ -127,15 +133,24 @@ __device__ __forceinline__ T restrict_point(const T* __restrict__ r, int nx, int
         for (int b = -cy; b <= cy; b++) {
             const int jj = mirror(j0 + b, ny);
             const T wy = (b == 0) ? T(cy ? 0.5 : 1.0) : T(0.25);
-            const T* row = r + (long)kk * sxy + (long)jj * nx;
             T accx;
-            if (cx) accx = T(0.25) * row[mirror(i0 - 1, nx)] + T(0.5) * row[i0] + T(0.25) * row[mirror(i0 + 1, nx)];
-            else accx = row[i0];
+            if (cx)
+                accx = T(0.25) * rv(mirror(i0 - 1, nx), jj, kk) + T(0.5) * rv(i0, jj, kk) +
+                       T(0.25) * rv(mirror(i0 + 1, nx), jj, kk);
+            else accx = rv(i0, jj, kk);
             accy += wy * accx;
         }
         acc += wz * accy;
     }
     return acc;
+}
+
+template <class T>
+__device__ __forceinline__ T restrict_point(const T* __restrict__ r, int nx, int ny, int nz, int I, int J, int K, int cx,
+                                            int cy, int cz) {
+    const long sxy = (long)nx * ny;
+    auto rv = [&](int i, int j, int k) { return r[(long)k * sxy + (long)j * nx + i]; };
+    return restrict_point_of<T>(rv, nx, ny, nz, I, J, K, cx, cy, cz);
 }
 
 template <class T>
@@ -259,6 +274,64 @@ __global__ void __launch_bounds__(256) k_prolong(T* __restrict__ u, const T* __r
     const int k = bz;
     if (i >= nx || j >= ny) return;
     u[((long)k * ny + j) * nx + i] += prolong_point<T>(ec, ncx, ncy, i, j, k, cx, cy, cz);
+}
+
+// The red sweep that starts a smoothing pass from u = 0: the black
+// neighbours enter as zeros (the sweep that follows sets every black point
+// without reading it, so u need not be cleared first).
+template <class T>
+__global__ void __launch_bounds__(256) k_rbgs_zero(T* __restrict__ u, const T* __restrict__ f, const T* __restrict__ a0,
+                                                   int nx, int ny, int nz, T lx, T ly, T lz) {
+    int bx, by, bz;
+    slab_tile(((nx + 1) / 2 + 63) / 64, (ny + 3) / 4, bx, by, bz);
+    const int j = by * blockDim.y + threadIdx.y;
+    const int k = bz;
+    if (j >= ny) return;
+    const int i = 2 * (bx * blockDim.x + threadIdx.x) + ((j + k) & 1);
+    if (i >= nx) return;
+    const long p = ((long)k * ny + j) * nx + i;
+    if (k == 0) {
+        u[p] = f[p];
+        return;
+    }
+    const T s = T(0);
+    const T d = a0[p] + T(2) * (lx + ly + lz);
+    u[p] = (f[p] + s) / d;
+}
+
+// k_prolong followed by the red sweep, in one launch: a red point reads each
+// black neighbour as u + P e (what k_prolong would have stored there). Its
+// own interpolated value is overwritten and the black points are set by the
+// black sweep that follows without being read, so nothing else is stored.
+template <class T>
+__global__ void __launch_bounds__(256) k_rbgs_prolong(T* __restrict__ u, const T* __restrict__ f,
+                                                      const T* __restrict__ a0, const T* __restrict__ ec, int nx,
+                                                      int ny, int nz, T lx, T ly, T lz, int ncx, int ncy, int cx,
+                                                      int cy, int cz) {
+    int bx, by, bz;
+    slab_tile(((nx + 1) / 2 + 63) / 64, (ny + 3) / 4, bx, by, bz);
+    const int j = by * blockDim.y + threadIdx.y;
+    const int k = bz;
+    if (j >= ny) return;
+    const int i = 2 * (bx * blockDim.x + threadIdx.x) + ((j + k) & 1);
+    if (i >= nx) return;
+    const long p = ((long)k * ny + j) * nx + i;
+    if (k == 0) {
+        u[p] = f[p];
+        return;
+    }
+    auto v = [&](int ii, int jj, int kk) {
+        return u[((long)kk * ny + jj) * nx + ii] + prolong_point<T>(ec, ncx, ncy, ii, jj, kk, cx, cy, cz);
+    };
+    const T w = v(i > 0 ? i - 1 : i + 1, j, k);
+    const T e = v(i < nx - 1 ? i + 1 : i - 1, j, k);
+    const T sn = v(i, j > 0 ? j - 1 : j + 1, k);
+    const T nn = v(i, j < ny - 1 ? j + 1 : j - 1, k);
+    const T b = v(i, j, k - 1);
+    const T t = v(i, j, k < nz - 1 ? k + 1 : k - 1);
+    const T s = lx * (w + e) + ly * (sn + nn) + lz * (b + t);
+    const T d = a0[p] + T(2) * (lx + ly + lz);
+    u[p] = (f[p] + s) / d;
 }
 
 // u_fine += P e_coarse, a thread per fine (x, y) column marching up planes
@@ -688,6 +761,78 @@ __device__ __forceinline__ void bot_rbgs(const BotLevel& v, int color, int nthr,
     }
 }
 
+// A thread's point of each colour on a level whose colour has at most one
+// point per thread: index, neighbour offsets (bot_rbgs' mirrored stencil)
+// and the diagonal, formed once for all the level's sweeps.
+struct BotSlot {
+    int p[2], o[2][6];
+    double d[2];
+    bool on[2], dir[2];
+};
+
+__device__ __forceinline__ void bot_slot(const BotLevel& v, int tid, BotSlot& S) {
+    const int nx = v.nx, ny = v.ny, nz = v.nz, hx = (nx + 1) >> 1, sxy = nx * ny;
+    const int nslot = hx * ny * nz;
+    const double d2 = 2.0 * (v.lx + v.ly + v.lz);
+#pragma unroll
+    for (int color = 0; color < 2; color++) {
+        S.on[color] = false;
+        if (tid >= nslot) continue;
+        const int row = tid / hx;
+        const int k = row / ny, j = row - k * ny;
+        const int i = 2 * (tid - row * hx) + ((j + k + color) & 1);
+        if (i >= nx) continue;
+        const int p = k * sxy + j * nx + i;
+        S.on[color] = true;
+        S.dir[color] = k == 0;
+        S.p[color] = p;
+        int* o = S.o[color];
+        o[0] = i > 0 ? -1 : 1;
+        o[1] = i < nx - 1 ? 1 : -1;
+        o[2] = j > 0 ? -nx : nx;
+        o[3] = j < ny - 1 ? nx : -nx;
+        o[4] = -sxy;
+        o[5] = k < nz - 1 ? sxy : -sxy;
+        S.d[color] = k == 0 ? 1.0 : v.a0[p] + d2;
+    }
+}
+
+// bot_rbgs from a BotSlot (the same operations).
+__device__ __forceinline__ void bot_rbgs_slot(const BotLevel& v, const BotSlot& S, int color) {
+    if (!S.on[color]) return;
+    const int p = S.p[color];
+    if (S.dir[color]) {
+        v.u[p] = v.f[p];
+        return;
+    }
+    const int* o = S.o[color];
+    const double s = v.lx * (v.u[p + o[0]] + v.u[p + o[1]]) + v.ly * (v.u[p + o[2]] + v.u[p + o[3]]) +
+                     v.lz * (v.u[p + o[4]] + v.u[p + o[5]]);
+    v.u[p] = (v.f[p] + s) / S.d[color];
+}
+
+// `sweeps` red-black sweeps of a bottom level, a barrier after each colour.
+__device__ __forceinline__ void bot_sweeps(const BotLevel& v, int sweeps, int nthr, int tid) {
+    if (sweeps <= 0) return;
+    const int nslot = ((v.nx + 1) >> 1) * v.ny * v.nz;
+    if (nslot <= nthr) {
+        BotSlot S;
+        bot_slot(v, tid, S);
+        for (int s = 0; s < sweeps; s++)
+#pragma unroll
+            for (int color = 0; color < 2; color++) {
+                bot_rbgs_slot(v, S, color);
+                __syncthreads();
+            }
+    } else {
+        for (int s = 0; s < sweeps; s++)
+            for (int color = 0; color < 2; color++) {
+                bot_rbgs(v, color, nthr, tid);
+                __syncthreads();
+            }
+    }
+}
+
 // in_lds: the levels live in dynamic LDS for the whole launch (u, f, a0 per
 // level and one residual buffer of the largest level; only u of the first
 // level goes back to memory); otherwise they are worked on in place.
@@ -732,11 +877,7 @@ __global__ void __launch_bounds__(1024) k_bottom(BotLevels B, int nu1, int nu2, 
     for (int l = 0; l < last; l++) {
         const BotLevel v = S[l];  // in registers for the level's phases
         const BotLevel w = S[l + 1];
-        for (int s = 0; s < nu1; s++)
-            for (int color = 0; color < 2; color++) {
-                bot_rbgs(v, color, nthr, tid);
-                __syncthreads();
-            }
+        bot_sweeps(v, nu1, nthr, tid);
         // residual into r
         const int n = (int)v.nx * v.ny * v.nz, sxy = (int)v.nx * v.ny;
         const double d2 = 2.0 * (v.lx + v.ly + v.lz);
@@ -769,11 +910,7 @@ __global__ void __launch_bounds__(1024) k_bottom(BotLevels B, int nu1, int nu2, 
         const int n = (int)v.nx * v.ny * v.nz;
         for (int p = tid; p < n; p += nthr) v.u[p] = 0.0;
         __syncthreads();
-        for (int s = 0; s < coarse_sweeps; s++)
-            for (int color = 0; color < 2; color++) {
-                bot_rbgs(v, color, nthr, tid);
-                __syncthreads();
-            }
+        bot_sweeps(v, coarse_sweeps, nthr, tid);
     }
     for (int l = last - 1; l >= 0; l--) {
         const BotLevel v = S[l];  // in registers for the level's phases
@@ -786,11 +923,7 @@ __global__ void __launch_bounds__(1024) k_bottom(BotLevels B, int nu1, int nu2, 
             v.u[p] += prolong_point<double>(w.u, w.nx, w.ny, i, j, k, v.cx, v.cy, v.cz);
         }
         __syncthreads();
-        for (int s = 0; s < nu2; s++)
-            for (int color = 0; color < 2; color++) {
-                bot_rbgs(v, color, nthr, tid);
-                __syncthreads();
-            }
+        bot_sweeps(v, nu2, nthr, tid);
     }
     if (B.u32)
         for (int p = tid; p < n0; p += nthr) B.u32[p] = (float)S[0].u[p];
@@ -832,6 +965,13 @@ template __global__ void k_residual<double>(const double*, const double*, const 
 template __global__ void k_restrict<float>(const float*, float*, int, int, int, int, int, int, int, int, int);
 template __global__ void k_restrict<double>(const double*, double*, int, int, int, int, int, int, int, int, int);
 template __global__ void k_prolong<float>(float*, const float*, int, int, int, int, int, int, int, int);
+template __global__ void k_rbgs_zero<float>(float*, const float*, const float*, int, int, int, float, float, float);
+template __global__ void k_rbgs_zero<double>(double*, const double*, const double*, int, int, int, double, double,
+                                             double);
+template __global__ void k_rbgs_prolong<float>(float*, const float*, const float*, const float*, int, int, int, float,
+                                               float, float, int, int, int, int, int);
+template __global__ void k_rbgs_prolong<double>(double*, const double*, const double*, const double*, int, int, int,
+                                                double, double, double, int, int, int, int, int);
 template __global__ void k_prolong<double>(double*, const double*, int, int, int, int, int, int, int, int);
 template __global__ void k_zm_restrict<float>(const float*, float*, int, int, int, int, int, int, int, int, int, int);
 template __global__ void k_zm_restrict<double>(const double*, double*, int, int, int, int, int, int, int, int, int,
