@@ -23,7 +23,7 @@ struct BotLevels {
     float* u32;
     int zero;
 };
-__global__ void k_bottom(BotLevels B, int nu1, int nu2, int coarse_sweeps, int in_lds);
+__global__ void k_bottom(BotLevels B, int nu1, int nu2, int coarse_sweeps, int lds_from);
 constexpr long BOT_LDS_BYTES = 60 * 1024;  // levels kept in LDS when they fit
 __global__ void k_sum_chunks(const double* part, long n, double* out);
 

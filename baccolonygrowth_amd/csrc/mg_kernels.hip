@@ -740,24 +740,31 @@ __global__ void k_cic_gather(const double* __restrict__ cm, int n, double ox, do
 // row is i = 2m + ((j + k + color) & 1); the last slot of a row of odd
 // length may fall outside it).
 __device__ __forceinline__ void bot_rbgs(const BotLevel& v, int color, int nthr, int tid) {
-    const int nx = v.nx, ny = v.ny;
-    const int hx = (nx + 1) >> 1;
-    const int nslot = hx * ny * (int)v.nz;
+    // a thread keeps its x slot and steps over rows (j, k) R at a time
+    const int nx = v.nx, ny = v.ny, nz = v.nz;
+    const int hx = (nx + 1) >> 1, R = nthr / hx;
+    const int ix = tid % hx, r0 = tid / hx;
+    if (r0 >= R) return;
     const int sxy = nx * ny;
     const double d2 = 2.0 * (v.lx + v.ly + v.lz);
-    for (int t = tid; t < nslot; t += nthr) {
-        const int row = t / hx;
-        const int k = row / ny, j = row - k * ny;
-        const int i = 2 * (t - row * hx) + ((j + k + color) & 1);
-        if (i >= nx) continue;
-        const int p = k * sxy + j * nx + i;
-        if (k == 0) {
-            v.u[p] = v.f[p];
-            continue;
+    int k = r0 / ny, j = r0 - k * ny;
+    for (int r = r0; r < ny * nz; r += R) {
+        const int i = 2 * ix + ((j + k + color) & 1);
+        if (i < nx) {
+            const int p = k * sxy + j * nx + i;
+            if (k == 0) {
+                v.u[p] = v.f[p];
+            } else {
+                long q;
+                const double s = neigh_sum<double>(v.u, i, j, k, v.nx, v.ny, v.nz, v.lx, v.ly, v.lz, q);
+                v.u[p] = (v.f[p] + s) / (v.a0[p] + d2);
+            }
         }
-        long q;
-        const double s = neigh_sum<double>(v.u, i, j, k, v.nx, v.ny, v.nz, v.lx, v.ly, v.lz, q);
-        v.u[p] = (v.f[p] + s) / (v.a0[p] + d2);
+        j += R;
+        while (j >= ny) {
+            j -= ny;
+            k++;
+        }
     }
 }
 
@@ -833,36 +840,35 @@ __device__ __forceinline__ void bot_sweeps(const BotLevel& v, int sweeps, int nt
     }
 }
 
-// in_lds: the levels live in dynamic LDS for the whole launch (u, f, a0 per
-// level and one residual buffer of the largest level; only u of the first
-// level goes back to memory); otherwise they are worked on in place.
-__global__ void __launch_bounds__(1024) k_bottom(BotLevels B, int nu1, int nu2, int coarse_sweeps, int in_lds) {
+// Levels lds_from .. n-1 live in dynamic LDS for the whole launch (u, f, a0
+// per level and one residual buffer of the largest of them); the levels
+// above are worked on in place in memory. Only u of the first level goes
+// back to memory.
+__global__ void __launch_bounds__(1024) k_bottom(BotLevels B, int nu1, int nu2, int coarse_sweeps, int lds_from) {
     extern __shared__ double smem[];
     __shared__ BotLevel S[BOT_MAX_LEVELS];
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int last = B.n - 1;
+    const int in_lds = lds_from == 0;  // level 0 in LDS
     if (tid == 0) {
-        int off = 0, rmax = 0;
+        int off = 0;
         for (int l = 0; l < B.n; l++) {
             S[l] = B.L[l];
             const int n = (int)S[l].nx * S[l].ny * S[l].nz;
-            rmax = n > rmax ? n : rmax;
-            if (in_lds) {
+            if (l >= lds_from) {
                 S[l].u = smem + off;
                 S[l].f = smem + off + n;
                 S[l].a0 = smem + off + 2 * n;
                 off += 3 * n;
             }
         }
-        if (in_lds)
-            for (int l = 0; l < B.n; l++) S[l].r = smem + off;
+        for (int l = lds_from; l < B.n; l++) S[l].r = smem + off;
     }
     __syncthreads();
-    if (in_lds)
-        for (int l = 0; l < B.n; l++) {
-            const int n = (int)S[l].nx * S[l].ny * S[l].nz;
-            for (int p = tid; p < n; p += nthr) S[l].a0[p] = B.L[l].a0[p];
-        }
+    for (int l = lds_from; l < B.n; l++) {
+        const int n = (int)S[l].nx * S[l].ny * S[l].nz;
+        for (int p = tid; p < n; p += nthr) S[l].a0[p] = B.L[l].a0[p];
+    }
     // level 0 into its working arrays: from the FP32 arrays, from memory into
     // LDS, and/or from u = 0 (in place otherwise)
     const int n0 = (int)S[0].nx * S[0].ny * S[0].nz;
