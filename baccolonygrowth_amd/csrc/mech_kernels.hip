@@ -7,7 +7,7 @@
 //   move_grow both sum_forces passes, the two half kicks and the drift,
 //             then grow and the division flag; one thread per cell, no
 //             force atomics, neighbour records read as 128-byte lines
-//   compact   ascending list of dividing cells by wave ballot + scan
+//   compact   ascending list of dividing cells from the division bitmap
 //   divide    one thread per division
 //   patch     the reference's post-division list patch, replayed in
 //             division order by one workgroup

@@ -51,9 +51,10 @@ def phase_bytes(kbar: float) -> dict:
         # + order/slot/count writes (12) + list writes (4 kbar)
         "lists": 116.0 + 4.0 * kbar,
         # order+count (8) + list (4 kbar) + hot record in/out (128+128) +
-        # type, wall, growth rate (20) + DynFric/StaFric (48) + flag (4)
+        # type, wall, growth rate (20) + DynFric/StaFric (48) + their cell ID (4)
         "move_grow": 336.0 + 4.0 * kbar,
-        "compact": 8.0,
+        # the division bitmap, read by the count and the write pass
+        "compact": 0.25,
         "divide": 0.0,
     }
 
