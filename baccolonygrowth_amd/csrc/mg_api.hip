@@ -242,7 +242,7 @@ struct bcg_rd3d {
     // the z-marching launches: k_rbgs2 (FP64, FP32 narrow, FP32 wide tile),
     // the fine-level passes (residual, residual norm, refinement update),
     // k_zm_prolong and k_zm_restrict (FP64, FP32)
-    long res_rb[3] = {1, 1, 1}, res_zm = 1, res_zp[2] = {1, 1}, res_zr[2] = {1, 1};
+    long res_rb[4] = {1, 1, 1, 1}, res_zm = 1, res_zp[2] = {1, 1}, res_zr[2] = {1, 1};
     // bcg_rd3d_update_cells: its fixed-shape part (set-up, V-cycles, growth
     // rates) recorded as a HIP graph for one (mechanics context, V-cycle
     // count) and replayed
@@ -364,17 +364,12 @@ void smooth(bcg_rd3d* c, Level& v, int sweeps, bool zero = false) {
                 sweep(tmp, u);
             }
         };
-        // FP32 has the LDS for the wide tile; it pays where there are enough
-        // tiles to fill the device (measured on 513 x 513 x 129: 143 vs 166 us a
-        // sweep; on 257 x 257 x 65 the narrow tile is faster)
-        bool wide = false;
-        if constexpr (sizeof(T) == 4) {
-            if (v.n() >= WIDE_MIN_POINTS) {
-                sweeps2(RbWide{}, c->res_rb[2]);
-                wide = true;
-            }
-        }
-        if (!wide) sweeps2(RbNarrow{}, c->res_rb[sizeof(T) == 4]);
+        // the wide tile pays where there are enough tiles to fill the device
+        // (measured on 513 x 513 x 129: FP32 143 vs 166 us a sweep, FP64 264
+        // vs 280 us with 87 KB of LDS, one workgroup a CU; on 257 x 257 x 65
+        // the narrow tile is faster)
+        if (v.n() >= WIDE_MIN_POINTS) sweeps2(RbWide{}, c->res_rb[sizeof(T) == 4 ? 2 : 3]);
+        else sweeps2(RbNarrow{}, c->res_rb[sizeof(T) == 4]);
     }
     if (zero && s == 0)
         hipLaunchKernelGGL((k_fill<T>), dim3(nblk(v.n(), 256)), dim3(256), 0, c->stream, u, v.n(), (T)0);
@@ -498,9 +493,13 @@ void vcycle(bcg_rd3d* c, int l, bool zero = false) {
         const long cols = (long)nblk(w.nx, 64) * nblk(w.ny, 4);
         const int nc = z_ranges(cols, w.nz, c->res_zr[sizeof(T) == 4], 1, 4);
         const int kchunk = (w.nz + nc - 1) / nc;
-        hipLaunchKernelGGL((k_zm_restrict<T>), slab_grid(w.nx, w.ny, (w.nz + kchunk - 1) / kchunk), dim3(256), 0,
-                           c->stream, Arr<T>::r(v), Arr<T>::f(w), v.nx, v.ny, v.nz, w.nx, w.ny, w.nz, v.cx, v.cy, v.cz,
-                           kchunk);
+        if (v.cx && v.cy && v.cz)
+            hipLaunchKernelGGL((k_zm_restrict3<T>), slab_grid(w.nx, w.ny, (w.nz + kchunk - 1) / kchunk), dim3(256), 0,
+                               c->stream, Arr<T>::r(v), Arr<T>::f(w), v.nx, v.ny, v.nz, w.nx, w.ny, w.nz, kchunk);
+        else
+            hipLaunchKernelGGL((k_zm_restrict<T>), slab_grid(w.nx, w.ny, (w.nz + kchunk - 1) / kchunk), dim3(256), 0,
+                               c->stream, Arr<T>::r(v), Arr<T>::f(w), v.nx, v.ny, v.nz, w.nx, w.ny, w.nz, v.cx, v.cy,
+                               v.cz, kchunk);
     } else {
         hipLaunchKernelGGL((k_restrict<T>), slab_grid(w), dim3(64, 4), 0, c->stream, Arr<T>::r(v), Arr<T>::f(w), v.nx,
                            v.ny, v.nz, w.nx, w.ny, w.nz, v.cx, v.cy, v.cz);
@@ -662,6 +661,7 @@ int bcg_rd3d_create(const bcg_rd3d_params* p, bcg_rd3d** out) {
     c->res_rb[0] = resident(k_rbgs2<double, 2, RbNarrow>, RbNarrow::THREADS);
     c->res_rb[1] = resident(k_rbgs2<float, 2, RbNarrow>, RbNarrow::THREADS);
     c->res_rb[2] = resident(k_rbgs2<float, 2, RbWide>, RbWide::THREADS);
+    c->res_rb[3] = resident(k_rbgs2<double, 2, RbWide>, RbWide::THREADS);
     c->res_zm = std::min({resident(k_zm_residual<double>, ZM_THREADS), resident(k_zm_resnorm, ZM_THREADS),
                           resident(k_zm_ir, ZM_THREADS)});
     c->res_zp[0] = resident(k_zm_prolong<double>, 256);

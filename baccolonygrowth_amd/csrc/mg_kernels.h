@@ -61,6 +61,8 @@ template <class T>
 __global__ void k_zm_prolong(T* u, const T* ec, int nx, int ny, int nz, int ncx, int ncy, int cx, int cy, int cz,
                              int kchunk);
 template <class T>
+__global__ void k_zm_restrict3(const T* r, T* fc, int nx, int ny, int nz, int ncx, int ncy, int ncz, int kchunk);
+template <class T>
 __global__ void k_fill(T* a, long n, T v);
 template <class Ts, class Td>
 __global__ void k_cast(const Ts* a, Td* b, long n);

@@ -222,6 +222,67 @@ __global__ void __launch_bounds__(256) k_zm_restrict(const T* __restrict__ r, T*
     }
 }
 
+// k_zm_restrict with every axis coarsened: the fine planes 2K-1, 2K, 2K+1
+// of consecutive coarse planes are walked in order and the next two fine
+// planes' nine values are requested a coarse plane ahead (two register
+// sets), so that a column's loads overlap its sums. Same operations in the
+// same order as restrict_point.
+template <class T>
+__global__ void __launch_bounds__(256) k_zm_restrict3(const T* __restrict__ r, T* __restrict__ fc, int nx, int ny,
+                                                      int nz, int ncx, int ncy, int ncz, int kchunk) {
+    int bx, by, bz;
+    slab_tile((ncx + 63) / 64, (ncy + 3) / 4, bx, by, bz);
+    const int I = bx * 64 + (threadIdx.x & 63), J = by * 4 + (threadIdx.x >> 6);
+    if (I >= ncx || J >= ncy) return;
+    int K = bz * kchunk;
+    const int K1 = min(K + kchunk, ncz);
+    T* out = fc + (long)J * ncx + I;
+    const long scxy = (long)ncx * ncy;
+    if (K == 0) {
+        out[0] = T(0);  // Dirichlet rows: zero correction
+        K = 1;
+    }
+    if (K >= K1) return;
+    const int i0 = 2 * I, j0 = 2 * J;
+    const int im = mirror(i0 - 1, nx), ip = mirror(i0 + 1, nx);
+    const long r0 = (long)mirror(j0 - 1, ny) * nx, r1 = (long)j0 * nx, r2 = (long)mirror(j0 + 1, ny) * nx;
+    const long sxy = (long)nx * ny;
+    auto load9 = [&](int f, T (&v)[9]) {
+        const T* base = r + (long)mirror(f, nz) * sxy;
+        v[0] = base[r0 + im], v[1] = base[r0 + i0], v[2] = base[r0 + ip];
+        v[3] = base[r1 + im], v[4] = base[r1 + i0], v[5] = base[r1 + ip];
+        v[6] = base[r2 + im], v[7] = base[r2 + i0], v[8] = base[r2 + ip];
+    };
+    auto sum9 = [&](const T (&v)[9]) {
+        T accy = T(0);
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            const T wy = (b == 1) ? T(0.5) : T(0.25);
+            const T accx = T(0.25) * v[3 * b] + T(0.5) * v[3 * b + 1] + T(0.25) * v[3 * b + 2];
+            accy += wy * accx;
+        }
+        return accy;
+    };
+    T A[9], B[9];
+    load9(2 * K - 1, A);
+    const T p_first = sum9(A);
+    load9(2 * K, B);
+    load9(2 * K + 1, A);
+    T p_odd = p_first;
+    for (; K < K1; K++) {
+        const T p_even = sum9(B);
+        if (K + 1 < K1) load9(2 * K + 2, B);
+        const T p_next = sum9(A);
+        if (K + 1 < K1) load9(2 * K + 3, A);
+        T acc = T(0);
+        acc += T(0.25) * p_odd;
+        acc += T(0.5) * p_even;
+        acc += T(0.25) * p_next;
+        out[K * scxy] = acc;
+        p_odd = p_next;
+    }
+}
+
 // Trilinear interpolation of the coarse correction at fine point (i,j,k).
 template <class T>
 __device__ __forceinline__ T prolong_point(const T* __restrict__ ec, int ncx, int ncy, int i, int j, int k, int cx, int cy,
@@ -958,6 +1019,8 @@ template __global__ void k_rbgs2<float, 2, RbNarrow>(const float*, float*, const
                                                      float, float, float, int, int, int);
 template __global__ void k_rbgs2<float, 2, RbWide>(const float*, float*, const float*, const float*, int, int, int,
                                                    float, float, float, int, int, int);
+template __global__ void k_rbgs2<double, 2, RbWide>(const double*, double*, const double*, const double*, int, int, int,
+                                                    double, double, double, int, int, int);
 template __global__ void k_rbgs2<double, 2, RbNarrow>(const double*, double*, const double*, const double*, int, int,
                                                       int, double, double, double, int, int, int);
 template __global__ void k_zm_residual<float>(const float*, const float*, const float*, float*, int, int, int, float,
@@ -980,6 +1043,8 @@ template __global__ void k_rbgs_prolong<double>(double*, const double*, const do
                                                 double, double, double, int, int, int, int, int);
 template __global__ void k_prolong<double>(double*, const double*, int, int, int, int, int, int, int, int);
 template __global__ void k_zm_restrict<float>(const float*, float*, int, int, int, int, int, int, int, int, int, int);
+template __global__ void k_zm_restrict3<float>(const float*, float*, int, int, int, int, int, int, int);
+template __global__ void k_zm_restrict3<double>(const double*, double*, int, int, int, int, int, int, int);
 template __global__ void k_zm_restrict<double>(const double*, double*, int, int, int, int, int, int, int, int, int,
                                                int);
 template __global__ void k_zm_prolong<float>(float*, const float*, int, int, int, int, int, int, int, int, int);
