@@ -346,7 +346,7 @@ template <class T>
 void smooth(bcg_rd3d* c, Level& v, int sweeps, bool zero = false) {
     T* u = Arr<T>::u(v);
     int s = 0;
-    if (c->prm.smoother == BCG_RD3D_SMOOTH_FUSED && v.n() >= FUSED_MIN_POINTS) {
+    if (c->prm.smoother != BCG_RD3D_SMOOTH_SPLIT && v.n() >= FUSED_MIN_POINTS) {
         T* tmp = Arr<T>::r(v);
         // plane loads two steps ahead (measured ~4% faster than one step ahead on
         // the 513 x 513 x 129 level in both precisions)
@@ -365,10 +365,12 @@ void smooth(bcg_rd3d* c, Level& v, int sweeps, bool zero = false) {
             }
         };
         // the wide tile pays where there are enough tiles to fill the device
-        // (measured on 513 x 513 x 129: FP32 143 vs 166 us a sweep, FP64 264
-        // vs 280 us with 87 KB of LDS, one workgroup a CU; on 257 x 257 x 65
-        // the narrow tile is faster)
-        if (v.n() >= WIDE_MIN_POINTS) sweeps2(RbWide{}, c->res_rb[sizeof(T) == 4 ? 2 : 3]);
+        // (measured on 513 x 513 x 129: FP32 143 vs 166 us a sweep; FP64, with
+        // 87 KB of LDS and one workgroup a CU, 264 vs 280 us - taken only when
+        // asked for, BCG_RD3D_SMOOTH_FUSED_WIDE; on 257 x 257 x 65 the narrow
+        // tile is faster)
+        const bool wide = v.n() >= WIDE_MIN_POINTS && (sizeof(T) == 4 || c->prm.smoother == BCG_RD3D_SMOOTH_FUSED_WIDE);
+        if (wide) sweeps2(RbWide{}, c->res_rb[sizeof(T) == 4 ? 2 : 3]);
         else sweeps2(RbNarrow{}, c->res_rb[sizeof(T) == 4]);
     }
     if (zero && s == 0)
@@ -455,7 +457,7 @@ void vcycle(bcg_rd3d* c, int l, bool zero = false) {
         return;
     }
     Level& w = c->L[l + 1];
-    if (c->prm.smoother == BCG_RD3D_SMOOTH_FUSED && v.n() < FUSED_MIN_POINTS) {
+    if (c->prm.smoother != BCG_RD3D_SMOOTH_SPLIT && v.n() < FUSED_MIN_POINTS) {
         // small level: no clearing pass before the first sweep and, where the
         // level is small enough for its redundant interpolations to cost less
         // than a launch, the prolongation inside the first red sweep; the
@@ -479,9 +481,14 @@ void vcycle(bcg_rd3d* c, int l, bool zero = false) {
     // z-marching residual on the large levels
     if (v.n() >= FUSED_MIN_POINTS) {
         const ZmGrid g = zm_grid(v.nx, v.ny, v.nz, c->res_zm);
-        hipLaunchKernelGGL((k_zm_residual<T>), dim3(g.blocks()), dim3(ZM_THREADS), 0, c->stream, Arr<T>::u(v),
-                           Arr<T>::f(v), Arr<T>::a0(v), Arr<T>::r(v), v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly, (T)v.lz,
-                           g.tiles_x, g.tiles_y, g.kchunk);
+        if constexpr (sizeof(T) == 4)  // FP32: two planes a step (half the bytes a point of FP64)
+            hipLaunchKernelGGL((k_zm_residual2<T>), dim3(g.blocks()), dim3(ZM_THREADS), 0, c->stream, Arr<T>::u(v),
+                               Arr<T>::f(v), Arr<T>::a0(v), Arr<T>::r(v), v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly,
+                               (T)v.lz, g.tiles_x, g.tiles_y, g.kchunk);
+        else
+            hipLaunchKernelGGL((k_zm_residual<T>), dim3(g.blocks()), dim3(ZM_THREADS), 0, c->stream, Arr<T>::u(v),
+                               Arr<T>::f(v), Arr<T>::a0(v), Arr<T>::r(v), v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly,
+                               (T)v.lz, g.tiles_x, g.tiles_y, g.kchunk);
     } else {
         hipLaunchKernelGGL((k_residual<T>), slab_grid(v), dim3(64, 4), 0, c->stream, Arr<T>::u(v), Arr<T>::f(v),
                            Arr<T>::a0(v), Arr<T>::r(v), v.nx, v.ny, v.nz, (T)v.lx, (T)v.ly, (T)v.lz);
@@ -624,7 +631,8 @@ int bcg_rd3d_create(const bcg_rd3d_params* p, bcg_rd3d** out) {
     *out = nullptr;
     if (p->nx < 3 || p->ny < 3 || p->nz < 2 || !(p->h > 0) || !(p->D >= 0) || !(p->dt > 0) || p->nu1 < 0 ||
         p->nu2 < 0 || p->coarse_sweeps < 1 || p->max_levels < 1 || p->inner < 1 ||
-        (p->smoother != BCG_RD3D_SMOOTH_FUSED && p->smoother != BCG_RD3D_SMOOTH_SPLIT))
+        (p->smoother != BCG_RD3D_SMOOTH_FUSED && p->smoother != BCG_RD3D_SMOOTH_SPLIT &&
+         p->smoother != BCG_RD3D_SMOOTH_FUSED_WIDE))
         return BCG_ERR_ARG;
     bcg_rd3d* c = new bcg_rd3d();
     c->prm = *p;
@@ -1118,16 +1126,15 @@ int solve_core(bcg_rd3d* c, int mode, double tol, int max_iter, int* iters, doub
         }
         f.u64 = nullptr;
     } else {
-        // r = b - A x0 (x copied into the other buffer), r32 = r
+        // r = b - A x0, r32 = r (x0 stays where it is)
         double* xn = c->x[c->cur ^ 1];
         auto ir_update = [&](const double* xo_, double* xn_, const float* e, double inv_scale, double rscale) {
             hipLaunchKernelGGL(k_zm_ir, dim3(np), dim3(ZM_THREADS), 0, c->stream, xo_, xn_, e, inv_scale, c->b, f.a064,
                                f.f32, rscale, f.nx, f.ny, f.nz, f.lx, f.ly, f.lz, zg.tiles_x, zg.tiles_y, zg.kchunk,
                                c->part);
         };
-        ir_update(xo, xn, nullptr, 0.0, 1.0);
+        ir_update(xo, nullptr, nullptr, 0.0, 1.0);
         if ((rc = read_norm(c, np, &rn))) return rc;
-        c->cur ^= 1;
         r0 = bn;
         double scale = 1.0;  // r32 = r * scale
         while (it < max_iter && rn > tol * r0) {

@@ -41,6 +41,9 @@ __global__ void k_zm_ir(const double* xo, double* xn, const float* e, double inv
                         const double* a0, float* r32, double rscale, int nx, int ny, int nz, double lx, double ly,
                         double lz, int tiles_x, int tiles_y, int kchunk, double* part);
 template <class T>
+__global__ void k_zm_residual2(const T* u, const T* f, const T* a0, T* r, int nx, int ny, int nz, T lx, T ly, T lz,
+                               int tiles_x, int tiles_y, int kchunk);
+template <class T>
 __global__ void k_residual(const T* u, const T* f, const T* a0, T* r, int nx, int ny, int nz, T lx, T ly, T lz);
 template <class T>
 __global__ void k_restrict(const T* r, T* fc, int nx, int ny, int nz, int ncx, int ncy, int ncz, int cx, int cy,

@@ -666,6 +666,106 @@ __device__ __forceinline__ void zm_run(const Op& op, int nx, int ny, int nz, int
     }
 }
 
+// zm_run two planes a step: a 4-slot LDS ring holds planes k, k+1 (read) and
+// k+2, k+3 (being stored), so a workgroup passes one barrier per two planes
+// and a thread has two planes' loads in flight. The same point arithmetic
+// (Op::point) and, for `part`, the same per-workgroup sums in the same order
+// of planes.
+template <class T, class Op>
+__device__ __forceinline__ void zm_run2(const Op& op, int nx, int ny, int nz, int tiles_x, int tiles_y, int kchunk,
+                                        double* __restrict__ part) {
+    __shared__ T U[4][ZM_PLANE];
+    __shared__ double red[ZM_THREADS / 64];
+    constexpr int X = 8;
+    const int nb = gridDim.x, per = nb / X, rem = nb % X;
+    const int xcd = blockIdx.x % X;
+    const int b = xcd * per + (xcd < rem ? xcd : rem) + blockIdx.x / X;
+    const int bx = b % tiles_x, by = (b / tiles_x) % tiles_y, bz = b / (tiles_x * tiles_y);
+    const int tid = threadIdx.x;
+    ZmThread<T> t;
+    zm_init<T>(nx, ny, nz, bx, by, bz, kchunk, tid, t);
+    const int k0 = t.k0, k1 = t.k1;
+    // prologue: planes k0, k0+1 into their slots, the own column at k0-1 .. k0+1
+    T below = k0 >= 1 ? op.u(k0 - 1, t.pb) : T(0);
+    T center = op.u(k0, t.pb);
+    T c1 = k0 + 1 <= nz - 1 ? op.u(k0 + 1, t.pb) : T(0);
+    U[k0 & 3][t.li] = center;
+    if (t.lr >= 0) U[k0 & 3][t.lr] = op.u(k0, t.rb);
+    if (k0 + 1 <= nz - 1) {
+        U[(k0 + 1) & 3][t.li] = c1;
+        if (t.lr >= 0) U[(k0 + 1) & 3][t.lr] = op.u(k0 + 1, t.rb);
+    }
+    typename Op::Pre p0, p1;
+    op.fetch(k0, t.pb, p0);
+    if (k0 + 1 < k1) op.fetch(k0 + 1, t.pb, p1);
+    __syncthreads();
+    double acc = 0.0;
+    for (int k = k0; k < k1; k += 2) {
+        // loads of planes k+2, k+3 and of the next step's inputs
+        T na = T(0), nb2 = T(0), ra = T(0), rb = T(0);
+        if (k + 2 <= nz - 1) {
+            na = op.u(k + 2, t.pb);
+            if (t.lr >= 0) ra = op.u(k + 2, t.rb);
+        }
+        if (k + 3 <= nz - 1) {
+            nb2 = op.u(k + 3, t.pb);
+            if (t.lr >= 0) rb = op.u(k + 3, t.rb);
+        }
+        typename Op::Pre q0, q1;
+        if (k + 2 < k1) op.fetch(k + 2, t.pb, q0);
+        if (k + 3 < k1) op.fetch(k + 3, t.pb, q1);
+        if (t.act) {
+            const int i = t.li;
+            {
+                const T* P = U[k & 3];
+                const T above = k + 1 <= nz - 1 ? c1 : below;  // mirror across the top face
+                const T sm = op.lx * (P[i - 1] + P[i + 1]) + op.ly * (P[i - ZM_PX] + P[i + ZM_PX]) + op.lz * (below + above);
+                acc += op.point(k, t.pb, center, sm, p0);
+            }
+            if (k + 1 < k1) {
+                const T* P = U[(k + 1) & 3];
+                const T above = k + 2 <= nz - 1 ? na : center;
+                const T sm = op.lx * (P[i - 1] + P[i + 1]) + op.ly * (P[i - ZM_PX] + P[i + ZM_PX]) + op.lz * (center + above);
+                acc += op.point(k + 1, t.pb, c1, sm, p1);
+            }
+        }
+        if (k + 2 <= nz - 1) {
+            U[(k + 2) & 3][t.li] = na;
+            if (t.lr >= 0) U[(k + 2) & 3][t.lr] = ra;
+        }
+        if (k + 3 <= nz - 1) {
+            U[(k + 3) & 3][t.li] = nb2;
+            if (t.lr >= 0) U[(k + 3) & 3][t.lr] = rb;
+        }
+        below = c1;
+        center = na;
+        c1 = nb2;
+        p0 = q0;
+        p1 = q1;
+        __syncthreads();
+    }
+    if (part) {
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if ((tid & 63) == 0) red[tid >> 6] = acc;
+        __syncthreads();
+        if (tid == 0) {
+            double s = 0.0;
+            for (int w = 0; w < ZM_THREADS / 64; w++) s += red[w];
+            part[b] = s;
+        }
+    }
+}
+
+// k_zm_residual on zm_run2 (FP32, whose points carry half the bytes of FP64).
+template <class T>
+__global__ void __launch_bounds__(ZM_THREADS) k_zm_residual2(const T* __restrict__ u, const T* __restrict__ f,
+                                                             const T* __restrict__ a0, T* __restrict__ r, int nx,
+                                                             int ny, int nz, T lx, T ly, T lz, int tiles_x,
+                                                             int tiles_y, int kchunk) {
+    const ZmResidual<T> op{u, f, a0, r, (long)nx * ny, lx, ly, lz};
+    zm_run2<T>(op, nx, ny, nz, tiles_x, tiles_y, kchunk, nullptr);
+}
+
 template <class T>
 __global__ void __launch_bounds__(ZM_THREADS) k_zm_residual(const T* __restrict__ u, const T* __restrict__ f,
                                                             const T* __restrict__ a0, T* __restrict__ r, int nx, int ny,
@@ -1023,6 +1123,8 @@ template __global__ void k_rbgs2<double, 2, RbWide>(const double*, double*, cons
                                                     double, double, double, int, int, int);
 template __global__ void k_rbgs2<double, 2, RbNarrow>(const double*, double*, const double*, const double*, int, int,
                                                       int, double, double, double, int, int, int);
+template __global__ void k_zm_residual2<float>(const float*, const float*, const float*, float*, int, int, int, float,
+                                               float, float, int, int, int);
 template __global__ void k_zm_residual<float>(const float*, const float*, const float*, float*, int, int, int, float,
                                               float, float, int, int, int);
 template __global__ void k_zm_residual<double>(const double*, const double*, const double*, double*, int, int, int,

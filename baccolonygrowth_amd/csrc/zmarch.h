@@ -216,7 +216,8 @@ struct ZmResNorm {
 };
 
 // Mixed-precision outer step: operand x = x_old + e / scale (e == null: x_old),
-// written to x_new; r = b - A x in FP64, r32 = r * rscale, sum of r^2. Forming
+// written to x_new (x_new == null: not written); r = b - A x in FP64,
+// r32 = r * rscale, sum of r^2. Forming
 // x while loading the operand fuses the update into the residual pass; x_old
 // and x_new are different buffers, so no workgroup reads a value another
 // one writes.
@@ -243,7 +244,7 @@ struct ZmIrUpdate {
         q.a0 = k >= 1 ? zm_ld(a0, sxy, k, b) : 0.0;
     }
     RBGS_HD double point(int k, unsigned b, double c, double s, const Pre& q) const {
-        *(double*)zm_at(xn + k * sxy, b) = c;
+        if (xn) *(double*)zm_at(xn + k * sxy, b) = c;
         const double r = k == 0 ? q.b - c : q.b - ((q.a0 + 2.0 * (lx + ly + lz)) * c - s);
         *(float*)zm_at(r32 + k * sxy, b / 2) = (float)(r * rscale);
         return r * r;

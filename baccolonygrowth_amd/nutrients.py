@@ -23,6 +23,7 @@ MODES = {"fp64": 0, "mixed": 1}
 # bcg_rd3d_params.smoother (include/bcg.h)
 SMOOTH_FUSED = 0  # both colours of a red-black sweep in one z-marching launch
 SMOOTH_SPLIT = 1  # one launch per colour
+SMOOTH_FUSED_WIDE = 2  # FUSED, with the FP64 sweeps of levels >= 2^24 points on the wide tile too
 
 
 def default_rd3d_params(**kw) -> RD3DParams:

@@ -198,7 +198,7 @@ def vcycle_words(levels, nu1: int, nu2: int) -> float:
 
 def mg_bench(repeats: int = 3) -> dict:
     from baccolonygrowth_amd.colony import dome_window_centres
-    from baccolonygrowth_amd.nutrients import Nutrients3D, default_rd3d_params
+    from baccolonygrowth_amd.nutrients import SMOOTH_FUSED_WIDE, Nutrients3D, default_rd3d_params
 
     nz, ny, nx = MG_SHAPE
     half = (nx - 1) * MG_H / 2
@@ -223,6 +223,19 @@ def mg_bench(repeats: int = 3) -> dict:
             log(f"[bench] MG {mode}: {res[mode]['ms']:.2f} ms, {it} iterations, relres {rr:.2e}")
         s.set_field(c0)
         vms = {m: float(np.median([s.vcycle(m) for _ in range(5)])) for m in ("fp64", "mixed")}
+    # the FP64 solve with its finest-level sweeps on the wide tile too (an
+    # option, BCG_RD3D_SMOOTH_FUSED_WIDE; the same iterates, bitwise)
+    prm_w = default_rd3d_params(nx=nx, ny=ny, nz=nz, h=MG_H, D=600.0, dt=10.0, K=0.1, c_res=20.0,
+                                origin=(-half, -half, 0.0), nu1=2, nu2=2, inner=2, smoother=SMOOTH_FUSED_WIDE)
+    with Nutrients3D(prm_w) as s:
+        s.deposit(cm, np.full(len(cm), Q_CELL))
+        times = []
+        for _ in range(repeats):
+            s.set_field(c0)
+            s.step("fp64", tol=1e-10, max_iter=100)
+            times.append(s.last_times()["solve"])
+        fp64_wide = float(np.median(times))
+    log(f"[bench] MG fp64, wide FP64 tile: {fp64_wide:.2f} ms")
     words = vcycle_words(levels, 2, 2)
     gbs = {m: words * (8 if m == "fp64" else 4) / (vms[m] * 1e-3) / 1e9 for m in vms}
     err = float(np.abs(fields["mixed"] - fields["fp64"]).max())
@@ -235,6 +248,8 @@ def mg_bench(repeats: int = 3) -> dict:
         "fp64_ms": res["fp64"]["ms"],
         "mixed_ms": res["mixed"]["ms"],
         "speedup": res["fp64"]["ms"] / res["mixed"]["ms"],
+        "fp64_wide_tile_ms": fp64_wide,
+        "speedup_vs_fp64_wide_tile": fp64_wide / res["mixed"]["ms"],
         "fp64_iterations": res["fp64"]["iterations"],
         "mixed_refinements": res["mixed"]["iterations"],
         "relres": {m: res[m]["relres"] for m in res},

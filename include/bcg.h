@@ -246,6 +246,9 @@ typedef struct bcg_rd3d bcg_rd3d;
  * same iterates; SPLIT is kept as the plain form to check FUSED against.  */
 #define BCG_RD3D_SMOOTH_FUSED 0
 #define BCG_RD3D_SMOOTH_SPLIT 1
+/* FUSED, and the FP64 sweeps of levels of >= 2^24 points on the 62 x 30 tile
+   as well (87 KB of LDS: one workgroup a CU); FP32 always uses it there */
+#define BCG_RD3D_SMOOTH_FUSED_WIDE 2
 
 int bcg_rd3d_default_params(bcg_rd3d_params* p);
 int bcg_rd3d_create(const bcg_rd3d_params* p, bcg_rd3d** out);

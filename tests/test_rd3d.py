@@ -173,14 +173,14 @@ def test_fused_sweeps_equal_split_sweeps(nu, shape):
     in both solvers; odd sweep counts end with a split sweep. 225 x 161 x 65
     leaves partial 30 x 30 tiles on both x and y edges; 369 x 353 x 129
     (>= 2^24 points) runs its FP32 sweeps on 62 x 30 tiles, partial on both
-    edges too."""
-    from baccolonygrowth_amd.nutrients import SMOOTH_FUSED, SMOOTH_SPLIT
+    edges too, and with SMOOTH_FUSED_WIDE its FP64 sweeps as well."""
+    from baccolonygrowth_amd.nutrients import SMOOTH_FUSED, SMOOTH_FUSED_WIDE, SMOOTH_SPLIT
 
     print(f"[gpu] rd3d fused vs split sweeps {shape} nu={nu}", flush=True)
     c_old, q, _ = _case(shape, seed=12)
     for mode in ("fp64", "mixed"):
         out = {}
-        for sm in (SMOOTH_FUSED, SMOOTH_SPLIT):
+        for sm in (SMOOTH_FUSED, SMOOTH_FUSED_WIDE, SMOOTH_SPLIT):
             with _solver(shape, nu1=nu[0], nu2=nu[1], smoother=sm) as s:
                 s.set_field(c_old)
                 s.set_uptake(q)
@@ -189,6 +189,8 @@ def test_fused_sweeps_equal_split_sweeps(nu, shape):
         (a, ia, ra), (b, ib, rb) = out[SMOOTH_FUSED], out[SMOOTH_SPLIT]
         print(f"    {mode}: fused {ia} it {ra:.3e}, split {ib} it {rb:.3e}, max |diff| {np.abs(a - b).max():.3e}", flush=True)
         assert ia == ib and a.tobytes() == b.tobytes()
+        w, iw, _ = out[SMOOTH_FUSED_WIDE]
+        assert iw == ib and w.tobytes() == b.tobytes()
 
 
 @pytest.mark.gpu
