@@ -252,7 +252,7 @@ int bcg_create(const bcg_params* p, bcg_ctx** out) {
     A(&c->div_ids, M);
     A(&c->div_index_of, M);
     A(&c->ids_tmp, M);
-    A(&c->draws, 9 * M);
+    A(&c->draws, 9 * M);  // rand() values of the divisions' draws
     A(&c->mother_pre, M * HOT);
     const size_t nh = (size_t)c->P.nhx * c->P.nhy;
     A(&c->height, nh);
@@ -314,6 +314,7 @@ int bcg_destroy(bcg_ctx* c) {
         if (q) hipFree(q);
     if (c->hstats) hipHostFree(c->hstats);
     if (c->hids) hipHostFree(c->hids);
+    if (c->hdraws) hipHostFree(c->hdraws);
     for (int i = 0; i < 7; i++)
         if (c->ev[i]) hipEventDestroy(c->ev[i]);
     if (c->after) hipEventDestroy(c->after);
@@ -503,7 +504,11 @@ int bcg_rand_stream(uint32_t seed, uint64_t skip, int32_t n, int32_t* out) {
     GlibcRand g;
     g.seed(seed);
     g.discard(skip);
-    for (int i = 0; i < n; i++) out[i] = g.next();
+    // one draw at a time, in bulk, one at a time again (the same stream)
+    const int a = n / 3, b = 2 * n / 3;
+    for (int i = 0; i < a; i++) out[i] = g.next();
+    g.fill(out + a, (size_t)(b - a));
+    for (int i = b; i < n; i++) out[i] = g.next();
     return BCG_OK;
 }
 
@@ -668,9 +673,19 @@ int step_body(bcg_ctx* c, double dt, double t, int32_t flags, int32_t* n_div, in
     int D = std::min(c->last.ndiv, c->prm.maxCells - N0);
     if (D < 0) D = 0;
     if (D > 0) {
-        std::vector<double> u((size_t)9 * D);
-        for (auto& x : u) x = c->rng.centered();
-        HIPCHK(c, hipMemcpyAsync(c->draws, u.data(), u.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        // the 9 draws of each division as rand() values (mapped on the device),
+        // from pinned memory (the previous step's copy has completed)
+        const size_t nd = (size_t)9 * D;
+        if (nd > c->hdraws_cap) {
+            if (c->hdraws) HIPCHK(c, hipHostFree(c->hdraws));
+            c->hdraws = nullptr;
+            c->hdraws_cap = 0;
+            const size_t cap = std::max(nd, (size_t)9 * 1024);
+            HIPCHK(c, hipHostMalloc((void**)&c->hdraws, sizeof(int32_t) * cap, hipHostMallocDefault));
+            c->hdraws_cap = cap;
+        }
+        c->rng.fill(c->hdraws, nd);
+        HIPCHK(c, hipMemcpyAsync(c->draws, c->hdraws, nd * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
         hipLaunchKernelGGL(k_divide, dim3(nblk(D, 64)), dim3(64), 0, c->stream, c->hot[out], N0, D, c->div_ids,
                            c->draws, c->P, t, c->mother_pre, c->div_index_of, c->time_p, c->time_q, c->age_p,
                            c->age_q, c->type, c->ancestor, c->fric, c->fric_id, c->order, c->slot_of,

@@ -46,7 +46,10 @@ struct bcg_ctx {
     int *nl = nullptr, *nl_count = nullptr, *order = nullptr, *slot_of = nullptr;
     unsigned* divbits = nullptr;  // division flags, a bit per cell ID (zero between steps)
     int *div_ids = nullptr, *div_index_of = nullptr, *ids_tmp = nullptr;
-    double *draws = nullptr, *mother_pre = nullptr;
+    int32_t* draws = nullptr;   // rand() values of the divisions' draws, 9 a division
+    int32_t* hdraws = nullptr;  // pinned host staging of them
+    size_t hdraws_cap = 0;
+    double* mother_pre = nullptr;
     double *height = nullptr, *normal = nullptr, *wall = nullptr, *gr1 = nullptr, *gr2 = nullptr;
     double *Fbuf = nullptr, *Tbuf = nullptr;
     double* Sbuf = nullptr;  // mean_stress output, 5 per cell

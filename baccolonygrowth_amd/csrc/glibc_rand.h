@@ -36,6 +36,25 @@ class GlibcRand {
     // One draw, as rand().
     int32_t next() { return (int32_t)(next_raw() >> 1); }
 
+    // n draws as rand() in a row, into out: the recurrence over a straight
+    // buffer (no ring indexing), then the ring rewound to its last 34 values.
+    void fill(int32_t* out, size_t n) {
+        constexpr size_t B = 4096;
+        uint32_t W[34 + B];
+        while (n > 0) {
+            const size_t m = n < B ? n : B;
+            for (int j = 0; j < 34; j++) W[j] = r_[(k_ + j) % 34];  // oldest first
+            for (size_t j = 34; j < 34 + m; j++) {
+                W[j] = W[j - 31] + W[j - 3];
+                out[j - 34] = (int32_t)(W[j] >> 1);
+            }
+            for (int j = 0; j < 34; j++) r_[j] = W[m + j];
+            k_ = 0;
+            out += m;
+            n -= m;
+        }
+    }
+
     // ((float)rand()/RAND_MAX - 0.5) exactly as the reference evaluates it.
     double centered() {
         float f = (float)next() / (float)2147483647;

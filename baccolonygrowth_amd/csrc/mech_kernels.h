@@ -60,7 +60,7 @@ __global__ void k_mean_stress(const double* hot, int n, PhysParams P, const int*
 __global__ void k_flag_count(const unsigned* bits, int nw, int* tot);
 __global__ void k_flag_write(unsigned* bits, int nw, const int* off, int* ids);
 
-__global__ void k_divide(double* hot, int n0, int D, const int* div_ids, const double* u, PhysParams P, double t,
+__global__ void k_divide(double* hot, int n0, int D, const int* div_ids, const int32_t* u, PhysParams P, double t,
                          double* mother_pre, int* div_index_of, double* time_p, double* time_q, int* age_p,
                          int* age_q, int* type, int* ancestor, double* fric, int* fric_id, int* order,
                          int* slot_of, int* nl_count);

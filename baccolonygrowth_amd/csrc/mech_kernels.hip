@@ -1194,10 +1194,11 @@ __global__ void __launch_bounds__(256) k_flag_write(unsigned* __restrict__ bits,
 
 // ------------------------------------------------------------ divide ---
 // divide (ABM/grow.cpp:107-186) + DivideCell bookkeeping, one thread per
-// division; u holds the 9 draws of each division, already mapped to
-// ((float)rand()/RAND_MAX - 0.5) on the host.
+// division; u holds the 9 rand() values of each division's draws, mapped
+// here to ((float)rand()/RAND_MAX - 0.5) as the reference evaluates it
+// ((float)RAND_MAX is 2^31, so the division is an exact scaling).
 __global__ void k_divide(double* __restrict__ hot, int n0, int D, const int* __restrict__ div_ids,
-                         const double* __restrict__ u, PhysParams P, double t, double* __restrict__ mother_pre,
+                         const int32_t* __restrict__ u, PhysParams P, double t, double* __restrict__ mother_pre,
                          int* __restrict__ div_index_of, double* __restrict__ time_p, double* __restrict__ time_q,
                          int* __restrict__ age_p, int* __restrict__ age_q, int* __restrict__ type,
                          int* __restrict__ ancestor, double* __restrict__ fric, int* __restrict__ fric_id,
@@ -1210,7 +1211,9 @@ __global__ void k_divide(double* __restrict__ hot, int n0, int D, const int* __r
     double* hd = hot + (size_t)d * HOT;
     for (int i = 0; i < HOT; i++) mother_pre[(size_t)k * HOT + i] = hm[i];
     div_index_of[m] = k;
-    const double* r = u + 9 * (size_t)k;
+    double r[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) r[q] = (double)((float)u[9 * (size_t)k + q] / (float)2147483647) - 0.5;
     d3 mp = mk(hm[0], hm[1], hm[2]), mq = mk(hm[3], hm[4], hm[5]);
     const d3 mv = mk(hm[6], hm[7], hm[8]);
     double mL = hm[12], mR = hm[13];

@@ -60,6 +60,15 @@ def test_rand_stream_matches_host_libc():
         assert np.array_equal(_stream(seed, 0, 2000), host)
 
 
+def test_rand_stream_in_bulk_matches_host_libc():
+    # 20000 draws: the middle third comes from the bulk generator in several
+    # 4096-draw chunks, the last third one at a time after it
+    libc = ctypes.CDLL(None)
+    libc.srand(5)
+    host = np.array([libc.rand() for _ in range(20000)], np.int32)
+    assert np.array_equal(_stream(5, 0, 20000), host)
+
+
 @pytest.mark.parametrize("skip", [1023, 1024, 1025, 5000, 123457])
 def test_rand_jump_ahead_equals_stepping(skip):
     libc = ctypes.CDLL(None)
