@@ -574,11 +574,13 @@ def _env_case():
     return p, ep, cells
 
 
-def env_bench(repeats: int = 3, cpu: bool = True) -> dict:
+def env_bench(repeats: int = 3, cpu: bool = True, exec_flags: int = 0) -> dict:
     """One nutrient update (UPDATE density pass + UpdateEnvArrayMG_OxygenAcetate)
     on the GPU, median of `repeats` from the same starting state, and the
     reference's own code (compiled from source into oracle/_ref) on one host
-    thread when that library is present."""
+    thread when that library is present. exec_flags: the Environment's EXEC_*
+    flags (e.g. 2, EXEC_NO_GRAPH, every phase launched directly: what a
+    kernel trace can follow)."""
     from baccolonygrowth_amd.environment import ENV_INIT, ENV_UPDATE, Environment
     from baccolonygrowth_amd.simulation import Colony
 
@@ -588,6 +590,8 @@ def env_bench(repeats: int = 3, cpu: bool = True) -> dict:
     with Colony(p) as col:
         col.LoadCells(cells)
         with Environment(col, ep) as env:
+            if exec_flags:
+                env.set_execution(exec_flags)
             env.fields(ENV_INIT)
             env.nutrients()
             st0 = env.state()
@@ -731,6 +735,8 @@ def main():
     ap.add_argument("--no-env", action="store_true")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the founding-colony, adjoint and full coupled configurations")
+    ap.add_argument("--env-exec", type=int, default=0,
+                    help="EXEC_* flags of the planar nutrient update (--only env), e.g. 2: no graph replay")
     ap.add_argument("--only", choices=("mg", "env", "env_large", "coupled", "small", "adjoint"), default=None,
                     help="run one secondary section alone and print its JSON object (for profiling)")
     args = ap.parse_args()
@@ -739,7 +745,7 @@ def main():
         sys.exit(2)
 
     if args.only == "env":
-        print(json.dumps(env_bench(cpu=not args.no_cpu)), flush=True)
+        print(json.dumps(env_bench(cpu=not args.no_cpu, exec_flags=args.env_exec)), flush=True)
         return
     if args.only == "env_large":
         print(json.dumps(env_bench_large(cpu=not args.no_cpu)), flush=True)
