@@ -191,10 +191,10 @@ __device__ __forceinline__ double col_ox(const PhysParams& P, int x) { return (d
 __device__ __forceinline__ double col_oy(const PhysParams& P, int y) { return (double)(y - P.BoxY / 2) * P.BoxLength; }
 
 // FP32 prefilter record of an end point: x, y relative to its column origin
-// (ox, oy), z, and x^2 + y^2 of the rounded local x, y.
+// (ox, oy), z, and x^2 + y^2 + z^2 of the rounded local coordinates.
 __device__ __forceinline__ float4 local_point(d3 e, double ox, double oy) {
-    const float x = (float)(e.x - ox), y = (float)(e.y - oy);
-    return make_float4(x, y, (float)e.z, (float)((double)x * x + (double)y * y));
+    const float x = (float)(e.x - ox), y = (float)(e.y - oy), z = (float)e.z;
+    return make_float4(x, y, z, (float)((double)x * x + (double)y * y + (double)z * z));
 }
 
 __global__ void k_bin_count(const double* __restrict__ hot, int n, PhysParams P, int* __restrict__ key,
@@ -298,28 +298,31 @@ __device__ __forceinline__ int flush_queue(const PhysParams& P, const double* __
 // FP32 form of the first test of reduceNeighbours (ABM/Neighbours.cpp:69-73):
 // an end point e of the candidate within r = L_A/2 + Rc R_A of the centre c
 // of A. End points are stored by local_point (x, y relative to the origin o
-// of their bin column, w = x^2 + y^2), so that against one column
-//   |e - c|^2 = w - 2 x c'_x - 2 y c'_y + (e_z - c_z)^2 + |c'|^2,  c' = c - o,
-// one subtraction and three FMAs per end point (PreCol::near).
-// Conservative: for an end point within reach, the FP32 rounding of the
-// coordinates and of this sum is a few units of 2^-24 E^2 with
-// E = |c'_x| + |c'_y| + |c_z| + 2r; the threshold adds 1e-6 E^2 to the
-// squared reach, r carries 1e-3 for its own rounding, and the exact FP64
-// test decides every pair this keeps.
+// of their bin column, z, w = x^2 + y^2 + z^2), so that against one column
+//   |e - c'|^2 - r'^2 = w - 2 x c'_x - 2 y c'_y - 2 z c'_z + (|c'|^2 - r'^2),
+// c' = c - o: three FMAs and an add per end point, the column's constants in
+// PreCol, and the verdict is the sign bit of the smaller of the two sums.
+// Conservative: r carries 1e-3 for its own rounding, r'^2 = 1.0001 r^2 +
+// 4e-6 E^2 with E = |c'_x| + |c'_y| + |c'_z| + 2r. For an end point within
+// reach every partial sum is below 5 E^2 in magnitude, so the five roundings
+// of the sum and the roundings of c0 and of the stored coordinates stay below
+// 30 units of 2^-24 E^2 (1.8e-6 E^2); the exact FP64 test decides every pair
+// this keeps.
 struct Pre {
     double cx, cy;  // centre of A
     float cz, r;    // centre z; reach plus margin
 };
 struct PreCol {
-    float m2x, m2y, cz, thr;
-    // (each FMA takes at most one operand from the record, which the walk
-    // holds in scalar registers)
-    __device__ bool near(const float4 p, const float4 q) const {
-        const float dpz = p.z - cz, dqz = q.z - cz;
-        const float sp = fmaf(p.y, m2y, fmaf(p.x, m2x, fmaf(dpz, dpz, p.w)));
-        const float sq = fmaf(q.y, m2y, fmaf(q.x, m2x, fmaf(dqz, dqz, q.w)));
-        return fminf(sp, sq) <= thr;
+    float m2x, m2y, m2z, c0;
+    // (each FMA takes one operand from the record, which the walk holds in
+    // scalar registers; the sign bit of the result is the verdict: set when
+    // an end point is within reach)
+    __device__ float key(const float4 p, const float4 q) const {
+        const float sp = fmaf(p.z, m2z, fmaf(p.y, m2y, fmaf(p.x, m2x, c0))) + p.w;
+        const float sq = fmaf(q.z, m2z, fmaf(q.y, m2y, fmaf(q.x, m2x, c0))) + q.w;
+        return fminf(sp, sq);
     }
+    __device__ bool near(const float4 p, const float4 q) const { return (__float_as_uint(key(p, q)) >> 31) != 0; }
 };
 __device__ __forceinline__ Pre prefilter_of(const PhysParams& P, const Seg& A, d3 cmA) {
     Pre f;
@@ -336,8 +339,8 @@ __device__ __forceinline__ PreCol pre_col(const PhysParams& P, const Pre& f, int
     PreCol c;
     c.m2x = -2.0f * cx;
     c.m2y = -2.0f * cy;
-    c.cz = f.cz;
-    c.thr = f.r * f.r * 1.0001f - (cx * cx + cy * cy) + 1e-6f * (e * e);
+    c.m2z = -2.0f * f.cz;
+    c.c0 = (cx * cx + cy * cy + f.cz * f.cz) - f.r * f.r * 1.0001f - 4e-6f * (e * e);
     return c;
 }
 
@@ -420,7 +423,7 @@ __device__ __forceinline__ int wave_max(int v) {
 // WALK_GROUP candidates at a time. A group's
 // records are the same for every lane, so they come by scalar loads, and each
 // lane tests them all against its FP32 prefilter, shifting the verdicts into
-// a bit mask (one add with carry a candidate). The mask is then cut to the
+// a bit mask (one funnel shift of the sign bit a candidate). The mask is then cut to the
 // lane's own bins of the column (the position range [klo, khi): its bins
 // z0-1..z0+1, empty when the column is not next to its own (x0, y0)) and its set bits
 // are queued in ascending order. A lane meets its candidates in ascending bin
@@ -511,10 +514,8 @@ __device__ __forceinline__ int gather_wave(const PhysParams& P, const double* __
                 unsigned bits = 0;
                 for (int u0 = 0; u0 < n; u0 += WALK_STEP) {
 #pragma unroll
-                    for (int u = u0; u < u0 + WALK_STEP; u++) {
-                        unsigned carry;
-                        bits = __builtin_addc(bits, bits, col.near(cg[2 * u], cg[2 * u + 1]) ? 1u : 0u, &carry);
-                    }
+                    for (int u = u0; u < u0 + WALK_STEP; u++)  // bits << 1 | the verdict's sign bit
+                        bits = __builtin_amdgcn_alignbit(bits, __float_as_uint(col.key(cg[2 * u], cg[2 * u + 1])), 31);
                 }
                 // bit WALK_GROUP-1-u: candidate g+u passed the prefilter
                 if (n < WALK_GROUP) bits <<= WALK_GROUP - n;
@@ -640,11 +641,26 @@ __global__ void __launch_bounds__(256) k_gather_all(const double* __restrict__ h
         block_max[blockIdx.x] = max(max(wave_lmax[0], wave_lmax[1]), max(wave_lmax[2], wave_lmax[3]));
 }
 
-// Folds per-block values into one statistic (one workgroup, one atomic).
+// Folds per-block values into one statistic (one workgroup, one atomic). The
+// values are read as 16-byte vectors, four a thread at once (the loads of a
+// thread are independent: their latencies overlap); `part` is a whole device
+// allocation, so 16-byte aligned.
 __global__ void __launch_bounds__(1024) k_reduce_max(const int* __restrict__ part, int n, int* __restrict__ out) {
     __shared__ int red[16];
     int v = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) v = max(v, part[i]);
+    const int n4 = n >> 2;
+    const int4* __restrict__ p4 = reinterpret_cast<const int4*>(part);
+    const int T0 = (int)blockDim.x;
+    for (int i0 = threadIdx.x; i0 < n4; i0 += 4 * T0) {
+        int4 x[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) x[k] = p4[min(i0 + k * T0, n4 - 1)];  // (in bounds: i0 < n4)
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (i0 + k * T0 < n4) v = max(max(v, max(x[k].x, x[k].y)), max(x[k].z, x[k].w));
+    }
+    const int i = 4 * n4 + (int)threadIdx.x;
+    if (i < n) v = max(v, part[i]);
     v = wave_max(v);
     if (__lane_id() == 0) red[threadIdx.x / 64] = v;
     __syncthreads();
@@ -657,7 +673,20 @@ __global__ void __launch_bounds__(1024) k_reduce_sum(const unsigned* __restrict_
                                                      unsigned long long* __restrict__ out) {
     __shared__ unsigned long long red[16];
     unsigned long long v = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) v += part[i];
+    const int n4 = n >> 2;
+    const int4* __restrict__ p4 = reinterpret_cast<const int4*>(part);
+    const int T0 = (int)blockDim.x;
+    for (int i0 = threadIdx.x; i0 < n4; i0 += 4 * T0) {
+        int4 x[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) x[k] = p4[min(i0 + k * T0, n4 - 1)];  // (in bounds: i0 < n4)
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (i0 + k * T0 < n4)
+                v += (unsigned long long)(unsigned)x[k].x + (unsigned)x[k].y + (unsigned)x[k].z + (unsigned)x[k].w;
+    }
+    const int i = 4 * n4 + (int)threadIdx.x;
+    if (i < n) v += part[i];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     if (__lane_id() == 0) red[threadIdx.x / 64] = v;
