@@ -759,6 +759,7 @@ int bcg_env_nutrients(bcg_env* e, int32_t* count) {
     a.qO2Carnt = p.qO2Carnt;
     a.qO2Acent = p.qO2Acent;
     a.pAceAnant = p.pAceAnant;
+    a.cc = chem_const(a);
     a.crossFeeding = p.crossFeeding;
     // n = int(3600 UpdateTime 740 / dt), dt = 100 (ABM/Multigrid.cpp:993-994)
     const double dt = 100;

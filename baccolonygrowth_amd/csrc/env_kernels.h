@@ -22,6 +22,13 @@ enum ErrBits {
     ERR_COLONY_EDGE = 4, // a colony point with x <= 2 (the reference asserts "Need more boxes")
 };
 
+// Constants of the colony chemistry derived from the parameters
+// (UpdateEnvironment_OxygenAcetate, ABM/Nutrients.cpp:200-506): formed once on
+// the host into EnvArgs (IEEE divisions, the same values as on the device).
+struct ChemConst {
+    double mgrCarAers, mgrCarAnas, mgrAceAers, g1s, g2s, g3s;
+};
+
 // Everything the device passes need, by value.
 struct EnvArgs {
     // grid
@@ -36,6 +43,7 @@ struct EnvArgs {
     double maxGrowthRateCarAer, maxGrowthRateCarAna, maxGrowthRateAceAer;
     double Kw, Ka, Kb, Cb, c1;
     double qCarAernt, qCarAnant, qAceAernt, qO2Carnt, qO2Acent, pAceAnant;
+    ChemConst cc;  // chem_const of the above
     int crossFeeding;
     int n_outer;  // outer iterations of the nutrient update
     // state
@@ -73,6 +81,32 @@ struct EnvArgs {
     const int* bounds;                // minx, maxx, miny, maxy, maxz
     int* errflag;
 };
+
+inline ChemConst chem_const(const EnvArgs& a) {
+    ChemConst c;
+    if (a.maxGrowthRateCarAer != 0.0) {
+        c.mgrCarAers = a.qCarAernt / a.qCarAer;
+        c.g1s = a.KC_aer * (c.mgrCarAers / a.maxGrowthRateCarAer);
+    } else {
+        c.mgrCarAers = 0;
+        c.g1s = 1e6;
+    }
+    if (a.maxGrowthRateCarAna != 0.0) {
+        c.mgrCarAnas = a.qCarAnant / a.qCarAna;
+        c.g2s = a.KC_ana * (c.mgrCarAnas / a.maxGrowthRateCarAna);
+    } else {
+        c.mgrCarAnas = 0;
+        c.g2s = 1e6;
+    }
+    if (a.maxGrowthRateAceAer != 0.0) {
+        c.mgrAceAers = a.qAceAernt / a.qAceAer;
+        c.g3s = a.KA * (c.mgrAceAers / a.maxGrowthRateAceAer);
+    } else {
+        c.mgrAceAers = 0;
+        c.g3s = 1e6;
+    }
+    return c;
+}
 
 // density / height passes
 struct FieldArgs {

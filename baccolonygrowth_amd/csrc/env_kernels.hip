@@ -868,35 +868,6 @@ __device__ double acetate_buffer(const EnvArgs& a, double Cac, bcg_env_cell& p) 
 }
 
 // Per-update constants of the uptake kinetics (ABM/Nutrients.cpp:244-266).
-struct ChemConst {
-    double mgrCarAers, mgrCarAnas, mgrAceAers, g1s, g2s, g3s;
-};
-__device__ ChemConst chem_const(const EnvArgs& a) {
-    ChemConst c;
-    if (a.maxGrowthRateCarAer != 0.0) {
-        c.mgrCarAers = a.qCarAernt / a.qCarAer;
-        c.g1s = a.KC_aer * (c.mgrCarAers / a.maxGrowthRateCarAer);
-    } else {
-        c.mgrCarAers = 0;
-        c.g1s = 1e6;
-    }
-    if (a.maxGrowthRateCarAna != 0.0) {
-        c.mgrCarAnas = a.qCarAnant / a.qCarAna;
-        c.g2s = a.KC_ana * (c.mgrCarAnas / a.maxGrowthRateCarAna);
-    } else {
-        c.mgrCarAnas = 0;
-        c.g2s = 1e6;
-    }
-    if (a.maxGrowthRateAceAer != 0.0) {
-        c.mgrAceAers = a.qAceAernt / a.qAceAer;
-        c.g3s = a.KA * (c.mgrAceAers / a.maxGrowthRateAceAer);
-    } else {
-        c.mgrAceAers = 0;
-        c.g3s = 1e6;
-    }
-    return c;
-}
-
 // Working copies of the colony (fields C, W, O2, acetate in the order of
 // bcg_aga, each [z][x]) and of interface or agar rows ([field][x]) during an
 // update: the lanes of a wave take neighbouring x.
@@ -929,8 +900,31 @@ __device__ __forceinline__ void colony_point(const EnvArgs& a, const ChemConst& 
                              bcg_env_cell* Env, bcg_env_cell* prevEnv, int x, int z, bool full) {
     const int BX = a.BX;
     const gdouble* insT = G(a.insT);
-    auto I = [&](int xx, int zz) -> double { return insT[zz * BX + xx]; };
-    const double ins = I(x, z);
+    const gdouble* dshT = G(a.dshT);
+    // Every operand of the box is loaded before any decision, so that the
+    // loads overlap instead of waiting behind each branch. Below z = 0 the
+    // loads read row 0 (values not used there); 1 <= x <= BX - 2 and
+    // z <= BZ - 2 for every box the sweeps update.
+    const int zd = z > 0 ? z - 1 : 0;
+    const double Dp = dshT[z * BX + x];
+    const double ins = insT[z * BX + x];
+    const double Ixp = insT[z * BX + x + 1], Ixm = insT[z * BX + x - 1];
+    const double Izp = insT[(z + 1) * BX + x], Izm = insT[zd * BX + x];
+    // prev at (x, z), (x - 1, z), (x + 1, z), (x, z + 1), (x, z - 1),
+    // (x - 1, z - 1); the wall row below at x - 1 and x
+    double c[4], xm[4], xp[4], zp[4], zm[4], xmzm[4], wm[4], w0[4];
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        c[f] = prev(f, x, z);
+        xm[f] = prev(f, x - 1, z);
+        xp[f] = prev(f, x + 1, z);
+        zp[f] = prev(f, x, z + 1);
+        zm[f] = prev(f, x, zd);
+        xmzm[f] = prev(f, x - 1, zd);
+        wm[f] = pw(f, x - 1);
+        w0[f] = pw(f, x);
+    }
+    if (!(Dp > DBL_EPSILON)) return;  // (the sweeps update boxes with a density shift only)
     if (ins > 0.5) {
         // fC, fW and fI of the box itself enter nothing (fI = 0)
         double Cgr = 0, Wgr = 0, Agr = 0, Ogr = 0;
@@ -939,13 +933,13 @@ __device__ __forceinline__ void colony_point(const EnvArgs& a, const ChemConst& 
         if (x != 0) {
             // vertex averages of the four boxes (the wall row below z = 0)
             if (z == 0) {
-                Cgr = (pw(FC, x - 1) + prev(FC, x - 1, z) + pw(FC, x) + prev(FC, x, z)) / 4.0;
-                Ogr = (pw(FO, x - 1) + prev(FO, x - 1, z) + pw(FO, x) + prev(FO, x, z)) / 4.0;
-                Agr = (pw(FA, x - 1) + prev(FA, x - 1, z) + pw(FA, x) + prev(FA, x, z)) / 4.0;
+                Cgr = (wm[FC] + xm[FC] + w0[FC] + c[FC]) / 4.0;
+                Ogr = (wm[FO] + xm[FO] + w0[FO] + c[FO]) / 4.0;
+                Agr = (wm[FA] + xm[FA] + w0[FA] + c[FA]) / 4.0;
             } else {
-                Cgr = (prev(FC, x - 1, z - 1) + prev(FC, x - 1, z) + prev(FC, x, z - 1) + prev(FC, x, z)) / 4.0;
-                Ogr = (prev(FO, x - 1, z - 1) + prev(FO, x - 1, z) + prev(FO, x, z - 1) + prev(FO, x, z)) / 4.0;
-                Agr = (prev(FA, x - 1, z - 1) + prev(FA, x - 1, z) + prev(FA, x, z - 1) + prev(FA, x, z)) / 4.0;
+                Cgr = (xmzm[FC] + xm[FC] + zm[FC] + c[FC]) / 4.0;
+                Ogr = (xmzm[FO] + xm[FO] + zm[FO] + c[FO]) / 4.0;
+                Agr = (xmzm[FA] + xm[FA] + zm[FA] + c[FA]) / 4.0;
             }
         }
         bcg_env_cell& out = Env[x * a.BZ + z];
@@ -986,24 +980,23 @@ __device__ __forceinline__ void colony_point(const EnvArgs& a, const ChemConst& 
         const double qW = 0.0;
         if (!(x > 2)) flag(a.errflag, ERR_COLONY_EDGE);
         const double BL2 = a.BoxLength * a.BoxLength;
-        const double cC = prev(FC, x, z), cW = prev(FW, x, z), cO = prev(FO, x, z), cA = prev(FA, x, z);
+        const double cC = c[FC], cW = c[FW], cO = c[FO], cA = c[FA];
         double Czz, Wzz, Azz, Ozz;
         if (z == 0) {
-            Wzz = (prev(FW, x, z + 1) + pw(FW, x) - 2.0 * cW) / BL2;
-            Czz = (prev(FC, x, z + 1) + pw(FC, x) - 2.0 * cC) / BL2;
-            Azz = (prev(FA, x, z + 1) + pw(FA, x) - 2.0 * cA) / BL2;
-            Ozz = (prev(FO, x, z + 1) + pw(FO, x) - 2.0 * cO) / BL2;
+            Wzz = (zp[FW] + w0[FW] - 2.0 * cW) / BL2;
+            Czz = (zp[FC] + w0[FC] - 2.0 * cC) / BL2;
+            Azz = (zp[FA] + w0[FA] - 2.0 * cA) / BL2;
+            Ozz = (zp[FO] + w0[FO] - 2.0 * cO) / BL2;
         } else {
-            Wzz = (prev(FW, x, z + 1) + prev(FW, x, z - 1) - 2.0 * cW) / BL2;
-            Czz = (prev(FC, x, z + 1) + prev(FC, x, z - 1) - 2.0 * cC) / BL2;
-            Azz = (prev(FA, x, z + 1) + prev(FA, x, z - 1) - 2.0 * cA) / BL2;
-            Ozz = (prev(FO, x, z + 1) + prev(FO, x, z - 1) - 2.0 * cO) / BL2;
+            Wzz = (zp[FW] + zm[FW] - 2.0 * cW) / BL2;
+            Czz = (zp[FC] + zm[FC] - 2.0 * cC) / BL2;
+            Azz = (zp[FA] + zm[FA] - 2.0 * cA) / BL2;
+            Ozz = (zp[FO] + zm[FO] - 2.0 * cO) / BL2;
         }
-        const double Cxx = (prev(FC, x + 1, z) + prev(FC, x - 1, z) - 2.0 * cC) / BL2;
-        const double Wxx = (prev(FW, x + 1, z) + prev(FW, x - 1, z) - 2.0 * cW) / BL2;
-        const double Axx = (prev(FA, x + 1, z) + prev(FA, x - 1, z) - 2.0 * cA) / BL2;
-        const double Oxx = (prev(FO, x + 1, z) + prev(FO, x - 1, z) - 2.0 * cO) / BL2;
-        const double Dp = G(a.dshT)[z * BX + x];
+        const double Cxx = (xp[FC] + xm[FC] - 2.0 * cC) / BL2;
+        const double Wxx = (xp[FW] + xm[FW] - 2.0 * cW) / BL2;
+        const double Axx = (xp[FA] + xm[FA] - 2.0 * cA) / BL2;
+        const double Oxx = (xp[FO] + xm[FO] - 2.0 * cO) / BL2;
         const double Cnew = cC + (a.DiffColonyCarbon * (Cxx + Czz) - qC * 0.25 * Dp) * a.Cdt;
         const double CAcenew = cA + (a.DiffColonyAce * (Axx + Azz) + pA * 0.25 * Dp - qA * 0.25 * Dp) * a.Cdt;
         const double Onew = cO + (a.DiffColonyO2 * (Oxx + Ozz) - qO * 0.25 * Dp) * a.Cdt;
@@ -1016,11 +1009,10 @@ __device__ __forceinline__ void colony_point(const EnvArgs& a, const ChemConst& 
         // colony surface: mean of the neighbouring colony boxes (x, z; at
         // z = 0 without the box below)
         const bool top = z > 0;
-        const int ne = (I(x + 1, z) > 0.5) + (I(x - 1, z) > 0.5) + (I(x, z + 1) > 0.5) + (top ? (I(x, z - 1) > 0.5) : 0);
+        const int ne = (Ixp > 0.5) + (Ixm > 0.5) + (Izp > 0.5) + (top ? (Izm > 0.5) : 0);
         auto mean = [&](int f) -> double {
-            double s = prev(f, x + 1, z) * (I(x + 1, z) > 0.5) + prev(f, x - 1, z) * (I(x - 1, z) > 0.5) +
-                       prev(f, x, z + 1) * (I(x, z + 1) > 0.5);
-            if (top) s = s + prev(f, x, z - 1) * (I(x, z - 1) > 0.5);
+            double s = xp[f] * (Ixp > 0.5) + xm[f] * (Ixm > 0.5) + zp[f] * (Izp > 0.5);
+            if (top) s = s + zm[f] * (Izm > 0.5);
             return s / ne;
         };
         const double Cnew = mean(FC);
@@ -1351,7 +1343,6 @@ __device__ __forceinline__ void sweep_body(const EnvArgs& a, const ChemConst& cc
     const int BX = a.BX, AL = a.ML;
     const int minX = a.bounds[0], maxX = a.bounds[1], maxH = a.bounds[4];
     const int ncol = maxX - minX + 1, nrow = maxH + 1;
-    const gdouble* dshT = G(a.dshT);
     const ColS<T> cur{sw ? c1 : c0, x0, W, NF}, prev{sw ? c0 : c1, x0, W, NF};
     const RowS pw0{G(a.swall[sw ^ 1]), BX};
     const int nw = ncol * nrow;
@@ -1359,8 +1350,7 @@ __device__ __forceinline__ void sweep_body(const EnvArgs& a, const ChemConst& cc
         if (w < nw) {
             const int z = w / ncol, x = minX + (w - z * ncol);
             if (z == 0 && dir == 1) wall_point(a, cur, a.srow[sw ^ 1], a.swall[sw], a.swall[sw ^ 1], 0, x);
-            if (dshT[z * BX + x] > DBL_EPSILON)
-                colony_point(a, cc, cur, prev, pw0, a.colony[sw], a.colony[sw ^ 1], x, z, full);
+            colony_point(a, cc, cur, prev, pw0, a.colony[sw], a.colony[sw ^ 1], x, z, full);  // (boxes with a density shift)
             if (z == 0 && dir == -1) wall_point(a, cur, a.srow[sw ^ 1], a.swall[sw], a.swall[sw ^ 1], 0, x);
         } else {
             const int q = w - nw, l = q / BX, x = q - l * BX;
@@ -1396,7 +1386,7 @@ __global__ void __launch_bounds__(ENV_THREADS) k_env_nutrients(EnvArgs args) {
     const int tid = threadIdx.x;
     if (tid == 0) a = args;
     __syncthreads();
-    const ChemConst cc = chem_const(a);
+    const ChemConst cc = a.cc;
     const MG m(a, 100.0, G(a.phantom) + 3 * tid);  // dt = 100 (ABM/Multigrid.cpp:993)
     const int AL = a.ML;
     const int BX = a.BX, BZ = a.BZ, NB = BX * BZ, RW = 4 * BX;
@@ -1880,7 +1870,7 @@ __global__ void __launch_bounds__(PH_THREADS) k_nut_copy_from_mg(const EnvArgs* 
 // One sweep over the whole device, the colony in global memory.
 __global__ void __launch_bounds__(PH_THREADS) k_nut_sweep(const EnvArgs* __restrict__ args, int full, int dir, int sw) {
     const EnvArgs& a = *args;
-    const ChemConst cc = chem_const(a);
+    const ChemConst cc = a.cc;
     const int NB = a.BX * a.BZ;
     sweep_body<gdouble>(a, cc, full != 0, dir, sw, G(a.scol[0]), G(a.scol[1]), 0, a.BX, NB, gtid(), gstride());
 }
@@ -1894,7 +1884,7 @@ __global__ void __launch_bounds__(ENV_THREADS) k_nut_sweeps_wg(EnvArgs args, int
     const int tid = threadIdx.x;
     if (tid == 0) a = args;
     __syncthreads();
-    const ChemConst cc = chem_const(a);
+    const ChemConst cc = a.cc;
     const int BX = a.BX, NB = BX * a.BZ;
     const bool use_lds = a.lds_colony && a.lds_w == a.bounds[1] - a.bounds[0] + 3 &&
                          a.lds_nf == a.lds_w * (a.bounds[4] + 2);
