@@ -887,6 +887,40 @@ struct RowS {
     __device__ gdouble& operator()(int f, int x) const { return p[f * BX + x]; }
 };
 
+// UpdateWallMultigrid_OxygenAcetate (ABM/Nutrients.cpp:509-659) at level l,
+// box x, from its operands: inside the colony (level 0) from the colony box
+// above, e, and the agar row below, g; elsewhere from the x term (Cxx, Wxx,
+// Axx), the point's previous values Wx and g. out: the interface row of
+// level l of this buffer.
+__device__ __forceinline__ void wall_update(const EnvArgs& a, const RowS out, int l, int x, bool inside,
+                                            const double (&e)[4], const double (&g)[4], const double (&Wx)[4],
+                                            double Cxx, double Wxx, double Axx) {
+    if (inside) {
+        const double eC = e[FC], eW = e[FW], eO = e[FO], eA = e[FA];
+        const double Cnew = (a.DiffColonyCarbon * eC + a.DiffAgarCarbon * g[FC]) / (a.DiffColonyCarbon + a.DiffAgarCarbon);
+        const double Onew = (a.DiffColonyO2 * eO + a.DiffAgarO2 * g[FO]) / (a.DiffColonyO2 + a.DiffAgarO2);
+        out(FC, x) = smax(0.0, smin(Cnew, a.maxCarbon));
+        out(FO, x) = smax(0.0, smin(Onew, a.maxO2));
+        const double CAcenew = (a.DiffColonyAce * eA + a.DiffAgarAce * g[FA]) / (a.DiffColonyAce + a.DiffAgarAce);
+        out(FA, x) = smax(0.0, CAcenew);
+        const double Wnew = (a.DiffColony * eW + a.DiffAgar * g[FW]) / (a.DiffColony + a.DiffAgar);
+        out(FW, x) = smax(0.0, Wnew);
+        return;
+    }
+    const double hl = a.BoxLength * a.BoxLength * (double)(1 << (2 * l));
+    const double Wzz = 2.0 * (g[FW] - Wx[FW]) / hl;
+    const double Czz = 2.0 * (g[FC] - Wx[FC]) / hl;
+    const double Azz = 2.0 * (g[FA] - Wx[FA]) / hl;
+    const double Cnew = Wx[FC] + a.DiffAgarCarbon * (Cxx + Czz) * a.Cdt;
+    const double Onew = a.maxO2;
+    out(FC, x) = smax(0.0, smin(Cnew, a.maxCarbon));
+    out(FO, x) = smax(0.0, smin(Onew, a.maxO2));
+    const double CAcenew = Wx[FA] + a.DiffAgarAce * (Axx + Azz) * a.Cdt;
+    out(FA, x) = smax(0.0, CAcenew);
+    const double Wnew = Wx[FW] + a.DiffAgar * (Wxx + Wzz) * a.Cdt;
+    out(FW, x) = smax(0.0, Wnew);
+}
+
 // UpdateEnvironment_OxygenAcetate (ABM/Nutrients.cpp:200-506), non-cross-
 // feeding chemistry, one colony box. The reference also writes, in every
 // sweep, fields that no later sweep reads: the pathway growth rates and
@@ -895,9 +929,17 @@ struct RowS {
 // own acetate, is overwritten by the second, on the vertex average). Those
 // are computed when `full` is set: in the last two sweeps of an update, whose
 // writes are the ones that remain in the two buffers.
+//
+// A box of row z = 0 also updates the interface point below it
+// (UpdateWallMultigrid at level 0, x inside the swept window): in the
+// reference's order that point comes before the box when dir = 1 (it reads
+// the box's value of the sweep before) and after it when dir = -1 (the value
+// just written). g0: the first agar row of the previous buffer; out0: the
+// interface row of this buffer (both level 0).
 template <class CS>
 __device__ __forceinline__ void colony_point(const EnvArgs& a, const ChemConst& cc, const CS cur, const CS prev, const RowS pw,
-                             bcg_env_cell* Env, bcg_env_cell* prevEnv, int x, int z, bool full) {
+                             bcg_env_cell* Env, bcg_env_cell* prevEnv, int x, int z, bool full, int dir,
+                             const RowS g0, const RowS out0) {
     const int BX = a.BX;
     const gdouble* insT = G(a.insT);
     const gdouble* dshT = G(a.dshT);
@@ -911,8 +953,9 @@ __device__ __forceinline__ void colony_point(const EnvArgs& a, const ChemConst& 
     const double Ixp = insT[z * BX + x + 1], Ixm = insT[z * BX + x - 1];
     const double Izp = insT[(z + 1) * BX + x], Izm = insT[zd * BX + x];
     // prev at (x, z), (x - 1, z), (x + 1, z), (x, z + 1), (x, z - 1),
-    // (x - 1, z - 1); the wall row below at x - 1 and x
-    double c[4], xm[4], xp[4], zp[4], zm[4], xmzm[4], wm[4], w0[4];
+    // (x - 1, z - 1); the wall row below at x - 1, x and x + 1; for z = 0,
+    // the agar row below and this box's value of the sweep before
+    double c[4], xm[4], xp[4], zp[4], zm[4], xmzm[4], wm[4], w0[4], wp[4], gg[4], co[4], nv[4];
 #pragma unroll
     for (int f = 0; f < 4; f++) {
         c[f] = prev(f, x, z);
@@ -923,9 +966,14 @@ __device__ __forceinline__ void colony_point(const EnvArgs& a, const ChemConst& 
         xmzm[f] = prev(f, x - 1, zd);
         wm[f] = pw(f, x - 1);
         w0[f] = pw(f, x);
+        wp[f] = pw(f, x + 1);
+        gg[f] = g0(f, x);
+        co[f] = cur(f, x, 0);
     }
-    if (!(Dp > DBL_EPSILON)) return;  // (the sweeps update boxes with a density shift only)
-    if (ins > 0.5) {
+    // the sweeps update boxes with a density shift only
+    const bool upd = Dp > DBL_EPSILON;
+    bool wrote = false;
+    if (upd && ins > 0.5) {
         // fC, fW and fI of the box itself enter nothing (fI = 0)
         double Cgr = 0, Wgr = 0, Agr = 0, Ogr = 0;
         const double mgrCarAers = cc.mgrCarAers, mgrCarAnas = cc.mgrCarAnas, mgrAceAers = cc.mgrAceAers;
@@ -1000,12 +1048,13 @@ __device__ __forceinline__ void colony_point(const EnvArgs& a, const ChemConst& 
         const double Cnew = cC + (a.DiffColonyCarbon * (Cxx + Czz) - qC * 0.25 * Dp) * a.Cdt;
         const double CAcenew = cA + (a.DiffColonyAce * (Axx + Azz) + pA * 0.25 * Dp - qA * 0.25 * Dp) * a.Cdt;
         const double Onew = cO + (a.DiffColonyO2 * (Oxx + Ozz) - qO * 0.25 * Dp) * a.Cdt;
-        cur(FC, x, z) = smax(0.0, smin(Cnew, a.maxCarbon));
-        cur(FO, x, z) = smax(0.0, smin(Onew, a.maxO2));
-        cur(FA, x, z) = smax(0.0, CAcenew);
+        nv[FC] = smax(0.0, smin(Cnew, a.maxCarbon));
+        nv[FO] = smax(0.0, smin(Onew, a.maxO2));
+        nv[FA] = smax(0.0, CAcenew);
         const double Wnew = cW + (a.DiffColony * (Wxx + Wzz) + pW * Dp - qW * Dp) * a.Cdt;
-        cur(FW, x, z) = smax(0.0, Wnew);
-    } else if (ins > 0.3) {
+        nv[FW] = smax(0.0, Wnew);
+        wrote = true;
+    } else if (upd && ins > 0.3) {
         // colony surface: mean of the neighbouring colony boxes (x, z; at
         // z = 0 without the box below)
         const bool top = z > 0;
@@ -1018,33 +1067,55 @@ __device__ __forceinline__ void colony_point(const EnvArgs& a, const ChemConst& 
         const double Cnew = mean(FC);
         const double Wnew = mean(FW);
         const double CAcenew = mean(FA);
-        cur(FC, x, z) = smax(0.0, smin(Cnew, a.maxCarbon));
-        cur(FO, x, z) = smax(0.0, smin(a.maxO2, a.maxO2));
-        cur(FA, x, z) = smax(0.0, CAcenew);
-        cur(FW, x, z) = smax(0.0, Wnew);
+        nv[FC] = smax(0.0, smin(Cnew, a.maxCarbon));
+        nv[FO] = smax(0.0, smin(a.maxO2, a.maxO2));
+        nv[FA] = smax(0.0, CAcenew);
+        nv[FW] = smax(0.0, Wnew);
+        wrote = true;
+    }
+    if (wrote) {
+        cur(FC, x, z) = nv[FC];
+        cur(FO, x, z) = nv[FO];
+        cur(FA, x, z) = nv[FA];
+        cur(FW, x, z) = nv[FW];
+    }
+    if (z == 0) {
+        // the interface point below: inside the colony from this box (the
+        // value before this sweep when dir = 1, after it when dir = -1),
+        // elsewhere the interior form of its x term (1 <= x <= BX - 2)
+        const bool after = dir == -1 && wrote;
+        double e[4];
+#pragma unroll
+        for (int f = 0; f < 4; f++) e[f] = after ? nv[f] : co[f];
+        const double hl = a.BoxLength * a.BoxLength * (double)(1 << (2 * 0));
+        const double Wxx = (wm[FW] + wp[FW] - 2 * w0[FW]) / hl;
+        const double Cxx = (wm[FC] + wp[FC] - 2 * w0[FC]) / hl;
+        const double Axx = (wm[FA] + wp[FA] - 2 * w0[FA]) / hl;
+        wall_update(a, out0, 0, x, ins > 0.5, e, gg, w0, Cxx, Wxx, Axx);
     }
 }
 
-// UpdateWallMultigrid_OxygenAcetate (ABM/Nutrients.cpp:509-659) at level l,
-// box x: the colony-agar interface row. env: the colony being written this
-// sweep; row: the first agar row of the previous buffer; wal / pwal: the
-// interface rows of this and of the previous buffer (levels 4 BX apart).
+// The interface point (l, x) with its operands read here. env: the colony
+// being written this sweep; row: the first agar row of the previous buffer;
+// wal / pwal: the interface rows of this and of the previous buffer (levels
+// 4 BX apart).
 template <class CS>
 __device__ __forceinline__ void wall_point(const EnvArgs& a, const CS env, double* row, double* wal, double* pwal,
                                            int l, int x) {
     const int BX = a.BX;
     const RowS out{G(wal) + l * 4 * BX, BX}, W{G(pwal) + l * 4 * BX, BX};
-    const RowS g{G(row) + l * 4 * BX, BX};
-    if (l == 0 && G(a.insT)[x] > 0.5) {
-        const double eC = env(FC, x, 0), eW = env(FW, x, 0), eO = env(FO, x, 0), eA = env(FA, x, 0);
-        const double Cnew = (a.DiffColonyCarbon * eC + a.DiffAgarCarbon * g(FC, x)) / (a.DiffColonyCarbon + a.DiffAgarCarbon);
-        const double Onew = (a.DiffColonyO2 * eO + a.DiffAgarO2 * g(FO, x)) / (a.DiffColonyO2 + a.DiffAgarO2);
-        out(FC, x) = smax(0.0, smin(Cnew, a.maxCarbon));
-        out(FO, x) = smax(0.0, smin(Onew, a.maxO2));
-        const double CAcenew = (a.DiffColonyAce * eA + a.DiffAgarAce * g(FA, x)) / (a.DiffColonyAce + a.DiffAgarAce);
-        out(FA, x) = smax(0.0, CAcenew);
-        const double Wnew = (a.DiffColony * eW + a.DiffAgar * g(FW, x)) / (a.DiffColony + a.DiffAgar);
-        out(FW, x) = smax(0.0, Wnew);
+    const RowS gr{G(row) + l * 4 * BX, BX};
+    double e[4] = {0, 0, 0, 0}, g[4], Wx[4];
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        g[f] = gr(f, x);
+        Wx[f] = W(f, x);
+    }
+    const bool inside = l == 0 && G(a.insT)[x] > 0.5;
+    if (inside) {
+#pragma unroll
+        for (int f = 0; f < 4; f++) e[f] = env(f, x, 0);
+        wall_update(a, out, l, x, true, e, g, Wx, 0, 0, 0);
         return;
     }
     const double hl = a.BoxLength * a.BoxLength * (double)(1 << (2 * l));
@@ -1088,17 +1159,7 @@ __device__ __forceinline__ void wall_point(const EnvArgs& a, const CS env, doubl
         Cxx = (W(FC, x - 1) + W(FC, x + 1) - 2 * W(FC, x)) / hl;
         Axx = (W(FA, x - 1) + W(FA, x + 1) - 2 * W(FA, x)) / hl;
     }
-    const double Wzz = 2.0 * (g(FW, x) - W(FW, x)) / hl;
-    const double Czz = 2.0 * (g(FC, x) - W(FC, x)) / hl;
-    const double Azz = 2.0 * (g(FA, x) - W(FA, x)) / hl;
-    const double Cnew = W(FC, x) + a.DiffAgarCarbon * (Cxx + Czz) * a.Cdt;
-    const double Onew = a.maxO2;
-    out(FC, x) = smax(0.0, smin(Cnew, a.maxCarbon));
-    out(FO, x) = smax(0.0, smin(Onew, a.maxO2));
-    const double CAcenew = W(FA, x) + a.DiffAgarAce * (Axx + Azz) * a.Cdt;
-    out(FA, x) = smax(0.0, CAcenew);
-    const double Wnew = W(FW, x) + a.DiffAgar * (Wxx + Wzz) * a.Cdt;
-    out(FW, x) = smax(0.0, Wnew);
+    wall_update(a, out, l, x, false, e, g, Wx, Cxx, Wxx, Axx);
 }
 
 // Diffusivity of multigrid species sp (C, O2, acetate) and the
@@ -1344,14 +1405,13 @@ __device__ __forceinline__ void sweep_body(const EnvArgs& a, const ChemConst& cc
     const int minX = a.bounds[0], maxX = a.bounds[1], maxH = a.bounds[4];
     const int ncol = maxX - minX + 1, nrow = maxH + 1;
     const ColS<T> cur{sw ? c1 : c0, x0, W, NF}, prev{sw ? c0 : c1, x0, W, NF};
-    const RowS pw0{G(a.swall[sw ^ 1]), BX};
+    const RowS pw0{G(a.swall[sw ^ 1]), BX}, g0{G(a.srow[sw ^ 1]), BX}, out0{G(a.swall[sw]), BX};
     const int nw = ncol * nrow;
     for (int w = t0; w < nw + AL * BX; w += stride) {
         if (w < nw) {
             const int z = w / ncol, x = minX + (w - z * ncol);
-            if (z == 0 && dir == 1) wall_point(a, cur, a.srow[sw ^ 1], a.swall[sw], a.swall[sw ^ 1], 0, x);
-            colony_point(a, cc, cur, prev, pw0, a.colony[sw], a.colony[sw ^ 1], x, z, full);  // (boxes with a density shift)
-            if (z == 0 && dir == -1) wall_point(a, cur, a.srow[sw ^ 1], a.swall[sw], a.swall[sw ^ 1], 0, x);
+            // (with the interface point below a box of row 0)
+            colony_point(a, cc, cur, prev, pw0, a.colony[sw], a.colony[sw ^ 1], x, z, full, dir, g0, out0);
         } else {
             const int q = w - nw, l = q / BX, x = q - l * BX;
             if (l == 0 && x >= minX && x <= maxX) continue;
