@@ -190,11 +190,10 @@ __global__ void __launch_bounds__(TSCAN_T) k_tscan_apply(const int* in, int n, c
 __device__ __forceinline__ double col_ox(const PhysParams& P, int x) { return (double)(x - P.BoxX / 2) * P.BoxLength; }
 __device__ __forceinline__ double col_oy(const PhysParams& P, int y) { return (double)(y - P.BoxY / 2) * P.BoxLength; }
 
-// FP32 prefilter record of an end point: x, y relative to its column origin
-// (ox, oy), z, and x^2 + y^2 + z^2 of the rounded local coordinates.
+// FP32 prefilter record of an end point (prefilter_point): x, y relative to
+// its column origin (ox, oy).
 __device__ __forceinline__ float4 local_point(d3 e, double ox, double oy) {
-    const float x = (float)(e.x - ox), y = (float)(e.y - oy), z = (float)e.z;
-    return make_float4(x, y, z, (float)((double)x * x + (double)y * y + (double)z * z));
+    return prefilter_point(e.x - ox, e.y - oy, e.z);
 }
 
 __global__ void k_bin_count(const double* __restrict__ hot, int n, PhysParams P, int* __restrict__ key,
@@ -295,53 +294,28 @@ __device__ __forceinline__ int flush_queue(const PhysParams& P, const double* __
     return m;
 }
 
-// FP32 form of the first test of reduceNeighbours (ABM/Neighbours.cpp:69-73):
-// an end point e of the candidate within r = L_A/2 + Rc R_A of the centre c
-// of A. End points are stored by local_point (x, y relative to the origin o
-// of their bin column, z, w = x^2 + y^2 + z^2), so that against one column
-//   |e - c'|^2 - r'^2 = w - 2 x c'_x - 2 y c'_y - 2 z c'_z + (|c'|^2 - r'^2),
-// c' = c - o: three FMAs and an add per end point, the column's constants in
-// PreCol, and the verdict is the sign bit of the smaller of the two sums.
-// Conservative: r carries 1e-3 for its own rounding, r'^2 = 1.0001 r^2 +
-// 4e-6 E^2 with E = |c'_x| + |c'_y| + |c'_z| + 2r. For an end point within
-// reach every partial sum is below 5 E^2 in magnitude, so the five roundings
-// of the sum and the roundings of c0 and of the stored coordinates stay below
-// 30 units of 2^-24 E^2 (1.8e-6 E^2); the exact FP64 test decides every pair
-// this keeps.
+// The FP32 form of the first test of reduceNeighbours (prefilter_column and
+// PreCol in pair_class.h), against the cell A of a lane: A's centre in FP64,
+// its z and reach (prefilter_reach) in FP32.
 struct Pre {
     double cx, cy;  // centre of A
     float cz, r;    // centre z; reach plus margin
-};
-struct PreCol {
-    float m2x, m2y, m2z, c0;
-    // (each FMA takes one operand from the record, which the walk holds in
-    // scalar registers; the sign bit of the result is the verdict: set when
-    // an end point is within reach)
-    __device__ float key(const float4 p, const float4 q) const {
-        const float sp = fmaf(p.z, m2z, fmaf(p.y, m2y, fmaf(p.x, m2x, c0))) + p.w;
-        const float sq = fmaf(q.z, m2z, fmaf(q.y, m2y, fmaf(q.x, m2x, c0))) + q.w;
-        return fminf(sp, sq);
-    }
-    __device__ bool near(const float4 p, const float4 q) const { return (__float_as_uint(key(p, q)) >> 31) != 0; }
 };
 __device__ __forceinline__ Pre prefilter_of(const PhysParams& P, const Seg& A, d3 cmA) {
     Pre f;
     f.cx = cmA.x;
     f.cy = cmA.y;
     f.cz = (float)cmA.z;
-    f.r = (float)(A.L / 2.0 + P.Rc * A.R) + 1e-3f;
+    f.r = prefilter_reach(A.L / 2.0 + P.Rc * A.R);
     return f;
 }
 // The test constants of A against bin column (x, y).
 __device__ __forceinline__ PreCol pre_col(const PhysParams& P, const Pre& f, int x, int y) {
-    const float cx = (float)(f.cx - col_ox(P, x)), cy = (float)(f.cy - col_oy(P, y));
-    const float e = fabsf(cx) + fabsf(cy) + fabsf(f.cz) + 2.0f * f.r;
-    PreCol c;
-    c.m2x = -2.0f * cx;
-    c.m2y = -2.0f * cy;
-    c.m2z = -2.0f * f.cz;
-    c.c0 = (cx * cx + cy * cy + f.cz * f.cz) - f.r * f.r * 1.0001f - 4e-6f * (e * e);
-    return c;
+    return prefilter_column((float)(f.cx - col_ox(P, x)), (float)(f.cy - col_oy(P, y)), f.cz, f.r);
+}
+// The verdict: the sign bit of the key (set when an end point is within reach).
+__device__ __forceinline__ bool pre_near(const PreCol& c, const float4 p, const float4 q) {
+    return (__float_as_uint(c.key(p, q)) >> 31) != 0;
 }
 
 
@@ -375,7 +349,7 @@ __device__ __forceinline__ int gather_one(const PhysParams& P, const double* __r
     for (int xa = xlo; xa < xhi; xa++)
         for (int ya = ylo; ya < yhi; ya++) {
             const PreCol col = pre_col(P, pre, xa, ya);
-            auto near = [&](int k) { return col.near(cand[2 * k], cand[2 * k + 1]); };
+            auto near = [&](int k) { return pre_near(col, cand[2 * k], cand[2 * k + 1]); };
             const int a = xa * (P.BoxY * P.BoxZ) + ya * P.BoxZ;
             const int k1 = start[a + zhi];
             int k = start[a + zlo];
@@ -778,6 +752,8 @@ __device__ __forceinline__ void sum_forces(const PhysParams& P, const Seg& S, co
 // (structure of arrays, one column per cell of the block).
 // Pairs tested for contact at once (a super-chunk of PAIR_SUPER chunks).
 constexpr int PAIR_SUPER = 8;
+constexpr int PAIR_BATCH = 4;  // chunks whose loads are issued together (divides PAIR_SUPER)
+static_assert(PAIR_SUPER % PAIR_BATCH == 0, "whole batches");
 static_assert(MOVE_BLOCK == 64, "the pair dealing of pair_sums is one wave");
 
 struct MoveLds {
@@ -855,29 +831,50 @@ __device__ __forceinline__ void pair_sums(const PhysParams& P, MoveLds& L, const
     for (int sbase = 0; sbase < total; sbase += SC) {
         unsigned long long touch[PAIR_SUPER];
         int nc = 0;
+        // PAIR_BATCH chunks at a time: their owners, list entries and
+        // records are all requested before any is tested (one round trip to
+        // memory a batch; a pair past the end reads the last pair's and
+        // keeps nothing)
 #pragma unroll
-        for (int r = 0; r < PAIR_SUPER; r++) {
-            const int p = sbase + r * MOVE_BLOCK + tid;
-            bool t = false;
-            int c = 0, j = 0;
-            if (p < total) {
-                c = owner(p);
-                j = nl[(size_t)(p - L.off[c]) * stride + slot0 + c];
-                Seg B;
-                load_seg(hot, j, B);
-                const Seg A = lds_seg(L, c);
-                const d3 o = cm_of(A);
-                auto rel = [&](d3 e) { return make_float4((float)(e.x - o.x), (float)(e.y - o.y), (float)(e.z - o.z), 0.f); };
-                t = !surely_apart(rel(A.p), rel(A.q), rel(B.p), rel(B.q), (float)(A.R + B.R), DevRcp{});
+        for (int r0 = 0; r0 < PAIR_SUPER; r0 += PAIR_BATCH) {
+            if (sbase + r0 * MOVE_BLOCK >= total) {  // (uniform)
+#pragma unroll
+                for (int b = 0; b < PAIR_BATCH; b++) touch[r0 + b] = 0;
+                continue;
             }
-            touch[r] = __ballot(t);
-            if (t) {
-                const int k = nc + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(touch[r] >> 32),
-                                                                 __builtin_amdgcn_mbcnt_lo((unsigned)touch[r], 0u));
-                L.cp_j[k] = j;
-                L.cp_c[k] = (unsigned char)c;
+            int cb[PAIR_BATCH], jb[PAIR_BATCH];
+#pragma unroll
+            for (int b = 0; b < PAIR_BATCH; b++) {
+                const int p = min(sbase + (r0 + b) * MOVE_BLOCK + tid, total - 1);
+                cb[b] = owner(p);
+                jb[b] = nl[(size_t)(p - L.off[cb[b]]) * stride + slot0 + cb[b]];
             }
-            nc += __popcll(touch[r]);
+            Seg Bb[PAIR_BATCH];
+#pragma unroll
+            for (int b = 0; b < PAIR_BATCH; b++) load_seg(hot, jb[b], Bb[b]);
+#pragma unroll
+            for (int b = 0; b < PAIR_BATCH; b++) {
+                const int r = r0 + b;
+                const int c = cb[b], j = jb[b];
+                bool t = false;
+                if (sbase + r * MOVE_BLOCK + tid < total) {
+                    const Seg& B = Bb[b];
+                    const Seg A = lds_seg(L, c);
+                    const d3 o = cm_of(A);
+                    auto rel = [&](d3 e) {
+                        return make_float4((float)(e.x - o.x), (float)(e.y - o.y), (float)(e.z - o.z), 0.f);
+                    };
+                    t = !surely_apart(rel(A.p), rel(A.q), rel(B.p), rel(B.q), (float)(A.R + B.R), DevRcp{});
+                }
+                touch[r] = __ballot(t);
+                if (t) {
+                    const int k = nc + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(touch[r] >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((unsigned)touch[r], 0u));
+                    L.cp_j[k] = j;
+                    L.cp_c[k] = (unsigned char)c;
+                }
+                nc += __popcll(touch[r]);
+            }
         }
         // this cell's pairs of the super-chunk that may touch: [cs, ce) of the compact list
         auto before = [&](int p) {  // pairs that may touch in [sbase, p), sbase <= p <= sbase + SC

@@ -1,4 +1,5 @@
-// pair_class.h - FP32 decisions about a pair of cells: the neighbour-list
+// pair_class.h - FP32 decisions about a pair of cells: the list build's
+// end-point prefilter (PreCol), the neighbour-list
 // predicate (classify_pair) and "surely not in contact" (surely_apart).
 //
 // reduceNeighbours (ABM/Neighbours.cpp:72-88) keeps neighbour B of cell A when
@@ -13,6 +14,48 @@
 #include <math.h>
 
 namespace bcg {
+
+// The list build's FP32 prefilter: the first test of reduceNeighbours
+// (ABM/Neighbours.cpp:69-73), an end point e of the candidate within
+// r = L_A/2 + Rc R_A of the centre c of A, made conservative. End points are
+// stored relative to the origin o of their bin column (prefilter_point: x, y,
+// z and w = x^2 + y^2 + z^2 of the rounded coordinates), so that against one
+// column
+//   |e - c'|^2 - r'^2 = w - 2 x c'_x - 2 y c'_y - 2 z c'_z + (|c'|^2 - r'^2),
+// c' = c - o: three FMAs and an add per end point, the column's constants in
+// PreCol, and the verdict is the sign bit of the smaller of the two sums.
+// Conservative: r carries 1e-3 for its own rounding (prefilter_reach),
+// r'^2 = 1.0001 r^2 + 4e-6 E^2 with E = |c'_x| + |c'_y| + |c'_z| + 2r. For an
+// end point within reach every partial sum is below 5 E^2 in magnitude, so
+// the five roundings of the sum and the roundings of c0 and of the stored
+// coordinates stay below 30 units of 2^-24 E^2 (1.8e-6 E^2); the exact FP64
+// test decides every pair this keeps.
+__host__ __device__ inline float4 prefilter_point(double x, double y, double z) {
+    const float fx = (float)x, fy = (float)y, fz = (float)z;
+    return make_float4(fx, fy, fz, (float)((double)fx * fx + (double)fy * fy + (double)fz * fz));
+}
+__host__ __device__ inline float prefilter_reach(double re) { return (float)re + 1e-3f; }
+struct PreCol {
+    float m2x, m2y, m2z, c0;
+    // (in the list build each FMA takes one operand from the record, which
+    // its walk holds in scalar registers)
+    __host__ __device__ float key(const float4 p, const float4 q) const {
+        const float sp = fmaf(p.z, m2z, fmaf(p.y, m2y, fmaf(p.x, m2x, c0))) + p.w;
+        const float sq = fmaf(q.z, m2z, fmaf(q.y, m2y, fmaf(q.x, m2x, c0))) + q.w;
+        return fminf(sp, sq);
+    }
+};
+// The constants of A's test against one column: cx, cy its centre relative to
+// the column origin, cz its z, r its reach (prefilter_reach).
+__host__ __device__ inline PreCol prefilter_column(float cx, float cy, float cz, float r) {
+    const float e = fabsf(cx) + fabsf(cy) + fabsf(cz) + 2.0f * r;
+    PreCol c;
+    c.m2x = -2.0f * cx;
+    c.m2y = -2.0f * cy;
+    c.m2z = -2.0f * cz;
+    c.c0 = (cx * cx + cy * cy + cz * cz) - r * r * 1.0001f - 4e-6f * (e * e);
+    return c;
+}
 
 // 1 keep, 0 reject, -1 undecided (the FP64 predicate decides).
 // a_p, a_q: A's end points in the frame of A's bin column (x, y relative to the

@@ -34,6 +34,22 @@ int pc_classify_batch(int n, const float* rec, const float* thr, const float* sh
     return 0;
 }
 
+// The list build's end-point prefilter over a batch: ends n x 6 (the
+// candidate's end points p, q with x, y relative to its column origin, FP64),
+// centre n x 3 (A's centre, x and y relative to the same origin, FP64), reach n
+// (A's L/2 + Rc R, FP64); out: 1 when the key's sign bit is set (kept).
+int pc_prefilter_batch(int n, const double* ends, const double* centre, const double* reach, int* out) {
+    for (int i = 0; i < n; i++) {
+        const double* e = ends + 6 * (long)i;
+        const double* c = centre + 3 * (long)i;
+        const float4 p = bcg::prefilter_point(e[0], e[1], e[2]), q = bcg::prefilter_point(e[3], e[4], e[5]);
+        const bcg::PreCol col =
+            bcg::prefilter_column((float)c[0], (float)c[1], (float)c[2], bcg::prefilter_reach(reach[i]));
+        out[i] = signbit(col.key(p, q)) ? 1 : 0;
+    }
+    return 0;
+}
+
 // surely_apart over a batch: rec as above (end points relative to a common
 // origin), sigma: n values.
 int pc_apart_batch(int n, const float* rec, const float* sigma, int mode, int* out) {

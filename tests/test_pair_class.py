@@ -35,6 +35,8 @@ def harness():
     fp = ctypes.POINTER(ctypes.c_float)
     lib.pc_classify_batch.argtypes = [ctypes.c_int, fp, fp, fp, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]
     lib.pc_apart_batch.argtypes = [ctypes.c_int, fp, fp, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.pc_prefilter_batch.argtypes = [ctypes.c_int, dp, dp, dp, ctypes.POINTER(ctypes.c_int32)]
     return lib
 
 
@@ -325,3 +327,43 @@ def test_contact_near_parallel_and_crossing(harness):
     ca = 0.5 * (a["p"] + a["q"])
     b = cells_of(ca + dist[:, None] * side + rng.uniform(-3, 3, n)[:, None] * ua, ub, rng.uniform(2, 4, n))
     check_apart(harness, a, b)
+
+
+def test_list_prefilter_keeps_every_end_point_within_reach(harness):
+    """The list build's FP32 end-point prefilter (PreCol, pair_class.h) is
+    conservative: every candidate with an end point within L_A/2 + Rc R_A of
+    A's centre (the FP64 first test of reduceNeighbours, ABM/Neighbours.cpp:
+    69-73) passes it, here for end points placed just inside the reach, with
+    centres anywhere near the candidate's bin column and z up to 60 um. It is
+    also a filter within its stated band: a kept candidate has an end point
+    with |e - c|^2 <= 1.0001 r'^2 + 4e-6 E^2 (r' = reach + 1e-3, E =
+    |c'_x| + |c'_y| + |c'_z| + 2 r') up to the FP32 rounding (2e-6 E^2 here)."""
+    rng = np.random.default_rng(11)
+    n = 1_000_000
+    c = np.column_stack([rng.uniform(-10.0, 15.0, n), rng.uniform(-10.0, 15.0, n), rng.uniform(0.0, 60.0, n)])
+    re = rng.uniform(0.5, 4.0, n)
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    # near end point: just inside the reach for the first half, anywhere within
+    # 1.5 reach for the second; the far one a cell length further
+    t = np.where(np.arange(n) < n // 2, re * (1.0 - rng.uniform(0.0, 1e-6, n)), re * rng.uniform(0.0, 1.5, n))
+    p = c + u * t[:, None]
+    v = rng.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1)[:, None]
+    q = p + v * rng.uniform(0.5, 6.0, n)[:, None]
+    ends = np.ascontiguousarray(np.hstack([p, q]))
+    centre = np.ascontiguousarray(c)
+    reach = np.ascontiguousarray(re)
+    out = np.zeros(n, np.int32)
+    dp = ctypes.POINTER(ctypes.c_double)
+    harness.pc_prefilter_batch(n, ends.ctypes.data_as(dp), centre.ctypes.data_as(dp), reach.ctypes.data_as(dp),
+                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    d = np.minimum(np.linalg.norm(p - c, axis=1), np.linalg.norm(q - c, axis=1))
+    inside = d <= re
+    assert inside.sum() > n // 2
+    assert out[inside].all(), np.flatnonzero(inside & (out == 0))[:5]
+    rr = re + 1e-3
+    E = np.abs(c).sum(axis=1) + 2 * rr
+    far = d**2 > 1.0001 * rr**2 + 6e-6 * E**2
+    assert far.sum() > n // 10
+    assert not out[far].any(), np.flatnonzero(far & (out == 1))[:5]
