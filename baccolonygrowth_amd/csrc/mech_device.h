@@ -273,24 +273,39 @@ __device__ __forceinline__ double surf_height(const PhysParams& P, const double*
     return height_interp(H, P.nhx, P.nhy, i, j, dx, dy);
 }
 
-// F_surf_tension: ABM/Forces.cpp:209-317.
-__device__ __forceinline__ void F_surf_tension(const PhysParams& P, const Seg& S, d3 cm, const double* __restrict__ H,
-                                               const double* __restrict__ N, d3& F, d3& T) {
+// The operands of F_surf_tension for a cell: the surface height at its end
+// points and centre, and the surface normal of its height column (read
+// whether or not the force applies: the column is clamped to the grid).
+struct SurfOps {
+    double hp, hq, hcm;
+    d3 Nv;
+};
+__device__ __forceinline__ SurfOps surf_ops(const PhysParams& P, const Seg& S, d3 cm, const double* __restrict__ H,
+                                            const double* __restrict__ N) {
     int xa = (int)((cm.x + P.BoxX / 2 * P.BoxLength) / (P.BoxLength / P.ref));
     int ya = 0;
     if (!P.planar) ya = (int)((cm.y + P.BoxY / 2 * P.BoxLength) / (P.BoxLength / P.ref));
     xa = xa < 0 ? 0 : (xa > P.nhx - 1 ? P.nhx - 1 : xa);
     ya = ya < 0 ? 0 : (ya > P.nhy - 1 ? P.nhy - 1 : ya);
-    double height_p = surf_height(P, H, S.p);
-    double height_q = surf_height(P, H, S.q);
-    double height_cm = surf_height(P, H, cm);
+    SurfOps o;
+    o.hp = surf_height(P, H, S.p);
+    o.hq = surf_height(P, H, S.q);
+    o.hcm = surf_height(P, H, cm);
+    const double* n = N + 3 * ((long)xa * P.nhy + ya);
+    o.Nv = mk(n[0], n[1], n[2]);
+    return o;
+}
+
+// F_surf_tension: ABM/Forces.cpp:209-317, from its operands.
+__device__ __forceinline__ void F_surf_tension_ops(const PhysParams& P, const Seg& S, d3 cm, const SurfOps& o, d3& F,
+                                                   d3& T) {
+    const double height_p = o.hp, height_q = o.hq, height_cm = o.hcm;
     if (height_cm > (cm.z + P.BoxLength)) {
         F = mk(0, 0, 0);
         T = mk(0, 0, 0);
         return;
     }
-    const double* n = N + 3 * ((long)xa * P.nhy + ya);
-    d3 Nv = mk(n[0], n[1], n[2]);
+    const d3 Nv = o.Nv;
     d3 F1 = mk(0, 0, 0), F2 = mk(0, 0, 0);
     double dh = S.p.z - (height_p - P.DH / Nv.z);
     if (!(dh < 0.0)) {
@@ -307,6 +322,10 @@ __device__ __forceinline__ void F_surf_tension(const PhysParams& P, const Seg& S
     F = add(F1, F2);
     T = cross(sub(S.p, cm), F1);
     T = add(T, cross(sub(S.q, cm), F2));
+}
+__device__ __forceinline__ void F_surf_tension(const PhysParams& P, const Seg& S, d3 cm, const double* __restrict__ H,
+                                               const double* __restrict__ N, d3& F, d3& T) {
+    F_surf_tension_ops(P, S, cm, surf_ops(P, S, cm, H, N), F, T);
 }
 
 // Half kick shared by UpdatePositions / UpdateVelocities:

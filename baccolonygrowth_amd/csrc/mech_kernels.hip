@@ -692,8 +692,7 @@ __global__ void __launch_bounds__(256) k_gather_ids(const double* __restrict__ h
 // wall contact, viscous drag, surface tension, planar constraint. F and T
 // hold the cell-cell sums on entry.
 __device__ __forceinline__ void finish_forces(const PhysParams& P, const Seg& S, const Kin& K, d3 cm, double wall_z,
-                                              const double* __restrict__ H, const double* __restrict__ N, d3& F, d3& T,
-                                              d3& sf, d3& df, int& draws) {
+                                              const SurfOps& so, d3& F, d3& T, d3& sf, d3& df, int& draws) {
     sf = mk(0, 0, 0);
     df = mk(0, 0, 0);
     if (dmin(S.q.z, S.p.z) < 1.2 * S.R) {
@@ -713,7 +712,7 @@ __device__ __forceinline__ void finish_forces(const PhysParams& P, const Seg& S,
     F = add(F, scl(K.v, -P.viscosity * 6.0 * kPI * S.R));
     T = add(T, scl(K.w, -P.viscosity * 6.0 * kPI * S.R));
     d3 Fs, Ts;
-    F_surf_tension(P, S, cm, H, N, Fs, Ts);
+    F_surf_tension_ops(P, S, cm, so, Fs, Ts);
     F = add(F, Fs);
     T = add(T, Ts);
     if (P.planar) {
@@ -745,7 +744,7 @@ __device__ __forceinline__ void sum_forces(const PhysParams& P, const Seg& S, co
         j = jn;
         B = Bn;
     }
-    finish_forces(P, S, K, cm, wall_z, H, N, F, T, sf, df, draws);
+    finish_forces(P, S, K, cm, wall_z, surf_ops(P, S, cm, H, N), F, T, sf, df, draws);
 }
 
 // Cell state of one block of k_move_grow staged in LDS for its pair phases
@@ -895,16 +894,12 @@ __device__ __forceinline__ void pair_sums(const PhysParams& P, MoveLds& L, const
                 const int c = L.cp_c[k], j = L.cp_j[k];
                 Seg B;
                 load_seg(hot, j, B);
+                Kin KB;  // requested with the segment (F_cc reads it when the cells touch)
+                load_kin(hot, j, KB);
                 const Seg A = lds_seg(L, c);
                 d3 f, tc;
                 F_cc(
-                    P, A, cm_of(A), B, [&]() { return lds_kin(L, c); },
-                    [&]() {
-                        Kin KB;
-                        load_kin(hot, j, KB);
-                        return KB;
-                    },
-                    f, tc);
+                    P, A, cm_of(A), B, [&]() { return lds_kin(L, c); }, [&]() { return KB; }, f, tc);
                 L.con[0][tid] = f.x;
                 L.con[1][tid] = f.y;
                 L.con[2][tid] = f.z;
@@ -940,6 +935,9 @@ __global__ void __launch_bounds__(MOVE_BLOCK) k_move_grow(
     const int t = slot0 + tid;
     int draws = 0;
     bool live = false;  // a cell inside the bin grid
+    SurfOps so{};       // F_surf_tension's operands at the state of the next pass
+    int a2 = -1;        // bin of the half-step position (where the cell grows)
+    double GRa = 0;     // the growth rate there (read before pass 2)
     int id = 0, cnt = 0;
     double Ldiv = 0, wall_z = 0;
     if (t < n) {
@@ -965,6 +963,7 @@ __global__ void __launch_bounds__(MOVE_BLOCK) k_move_grow(
             cnt = nl_count[t];
             lds_put_seg(L, tid, S);
             lds_put_kin(L, tid, K);
+            so = surf_ops(P, S, cm_of(S), H, Nrm);  // (its loads overlap the pair sums)
         }
     }
     int total;
@@ -979,7 +978,7 @@ __global__ void __launch_bounds__(MOVE_BLOCK) k_move_grow(
         const Seg S = lds_seg(L, tid);
         const Kin K = lds_kin(L, tid);
         const d3 cm = cm_of(S);
-        finish_forces(P, S, K, cm, wall_z, H, Nrm, F, T, sf, df, draws);
+        finish_forces(P, S, K, cm, wall_z, so, F, T, sf, df, draws);
         // UpdatePositions: ABM/Integrate.cpp:34-90
         Kin K1;
         half_kick(dt, F, T, S, K, K1);
@@ -994,26 +993,29 @@ __global__ void __launch_bounds__(MOVE_BLOCK) k_move_grow(
         }
         lds_put_seg(L, tid, S1);
         lds_put_kin(L, tid, K1);
+        so = surf_ops(P, S1, cm_of(S1), H, Nrm);
+        // pass 2 changes only the velocities: the position the cell grows
+        // at is known here
+        int x2, y2, z2;
+        a2 = bin_of(P, cm_of(S1), x2, y2, z2);
+        if (a2 >= 0) GRa = (P.crossFeeding && type[id] != 1) ? gr2[a2] : gr1[a2];
     }
     // pass 2 at the half step, against the neighbours' old records
     pair_sums(P, L, hot_old, nl, stride, slot0, F, T);
     if (live) {
         Seg S1 = lds_seg(L, tid);
         const Kin K1 = lds_kin(L, tid);
-        finish_forces(P, S1, K1, cm_of(S1), wall_z, H, Nrm, F, T, sf, df, draws);
+        finish_forces(P, S1, K1, cm_of(S1), wall_z, so, F, T, sf, df, draws);
         // UpdateVelocities: ABM/Integrate.cpp:93-127
         Kin K2;
         half_kick(dt, F, T, S1, K1, K2);
         if (P.planar) K2.v.y = 0;
         // grow: ABM/grow.cpp:17-102 at the MoveCell newAddress (ABM/Cells.cpp:32)
-        int x, y, z;
-        int a2 = bin_of(P, cm_of(S1), x, y, z);
         double GR = 0;
         if (a2 < 0) {
             atomicOr(err, ERRBIT_DOMAIN);
         } else {
-            GR = (P.crossFeeding && type[id] != 1) ? gr2[a2] : gr1[a2];
-            GR = GR * 1.5850;
+            GR = GRa * 1.5850;
             d3 v = sub(S1.q, S1.p);
             double dL = S1.L * dt * GR;
             d3 dv = scl(v, dL * 0.5 / S1.L);
