@@ -438,21 +438,42 @@ __device__ __forceinline__ int gather_wave(const PhysParams& P, const double* __
         offs[lane] = off;
         for (int e = 0; e < nq; e++) own[off + e] = (unsigned char)lane;
         wave_lds_sync();
-        for (int p = lane; p < tot; p += 64) {
-            const int o = own[p];
-            int* const q = qw + (p - offs[o]) * qstride + o;
-            const unsigned e = (unsigned)*q;
-            const int k = (int)(e & QK_MASK);
-            const int code = (int)(e >> QK_BITS);
-            const int dx = code / 3 - 1, dy = code % 3 - 1;
-            const float4 bp = cand[2 * k], bq = cand[2 * k + 1];
-            const int j = sorted[k];
-            int cls = 0;
-            if (k != base + o) {  // the cell itself is skipped
-                const float2 r = orr[o];
-                cls = classify_pair(op[o], oq[o], r.x, r.y, bp, bq, (float)dx * BL, (float)dy * BL, DevRcp{});
+        // pair p's owner lane, queue slot, candidate position, column code
+        // and records
+        struct PairOps {
+            int o, k, code, j;
+            int* q;
+            float4 bp, bq;
+        };
+        auto pair_ops = [&](int p) {
+            PairOps r;
+            r.o = own[p];
+            r.q = qw + (p - offs[r.o]) * qstride + r.o;
+            const unsigned e = (unsigned)*r.q;
+            r.k = (int)(e & QK_MASK);
+            r.code = (int)(e >> QK_BITS);
+            r.bp = cand[2 * r.k];
+            r.bq = cand[2 * r.k + 1];
+            r.j = sorted[r.k];
+            return r;
+        };
+        if (lane < tot) {
+            PairOps c = pair_ops(lane);
+            for (int p = lane; p < tot; p += 64) {
+                // the lane's next pair is requested before this one is decided
+                // (past the last pair, the last pair again: read, not used;
+                // the slots read are never those written in this round)
+                const PairOps nx = pair_ops(min(p + 64, tot - 1));
+                const int dx = c.code / 3 - 1, dy = c.code % 3 - 1;
+                int cls = 0;
+                if (c.k != base + c.o) {  // the cell itself is skipped
+                    const float2 r = orr[c.o];
+                    cls = classify_pair(op[c.o], oq[c.o], r.x, r.y, c.bp, c.bq, (float)dx * BL, (float)dy * BL,
+                                        DevRcp{});
+                }
+                *c.q = cls > 0 ? c.j : (cls < 0 ? -2 - c.k : -1);
+                c = nx;
             }
-            *q = cls > 0 ? j : (cls < 0 ? -2 - k : -1);
         }
         wave_lds_sync();
         for (int e = 0; e < nq; e++) {
